@@ -1,0 +1,174 @@
+/*
+ * ipl.h - C ABI of the MI355X-native interpolation-based lattice Boltzmann step.
+ *
+ * The reference (Kasahara1341/IP_CU_LBM) has no plugin or FFI layer: its boundary is the
+ * CUDA C++ template kernel library declared in all.hpp:39-156, linked into one driver main()
+ * per case.  This header replaces that library for the driver run loop
+ * (LockExchange/LockExchange.cu:199-279 and its siblings): init, N x step, readout.
+ *
+ * Conventions
+ *   - every entry point returns int status (IPL_OK == 0); ipl_error_string() names the code,
+ *     ipl_last_error_message() gives the detail of the last failure on a context;
+ *   - the library owns device memory; host pointers are borrowed for the duration of a call;
+ *   - one context is not thread-safe; distinct contexts are independent;
+ *   - host arrays of cell fields are in canonical cell order: x slowest, then y, then z
+ *     fastest, solid cells skipped (the reference's lnum order, LockExchange.cu:57-74).
+ */
+#ifndef IPL_H
+#define IPL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------------- */
+enum {
+    IPL_OK = 0,
+    IPL_ERR_INVALID_CONFIG = 1,   /* rejected at init (e.g. cell width below the lattice spacing) */
+    IPL_ERR_HIP = 2,              /* a HIP runtime call failed */
+    IPL_ERR_OUT_OF_MEMORY = 3,
+    IPL_ERR_NONFINITE = 4,        /* NaN/Inf detected in the macroscopic fields */
+    IPL_ERR_UNSUPPORTED = 5,      /* the requested path cannot run this configuration */
+    IPL_ERR_ARGUMENT = 6,         /* bad argument to a call (size, field id, ...) */
+    IPL_ERR_BUDGET = 7            /* batch admission: the job does not fit the memory budget */
+};
+
+/* ---- configuration enums ------------------------------------------------------------------- */
+enum { IPL_BC_SLIP = 0, IPL_BC_BOUNCE_BACK = 1 };            /* IP_process 'slip' argument, adv.cu:246 */
+enum { IPL_COLL_MRT = 0, IPL_COLL_SRT = 1 };                 /* col_f_MRT / col_f_SRT, collision.cu:115,151 */
+enum { IPL_SCALAR_NONE = 0, IPL_SCALAR_SALINITY = 1, IPL_SCALAR_PHASE_FIELD = 2 };
+enum { IPL_F64 = 0, IPL_F32 = 1, IPL_F16 = 2 };              /* population storage type */
+enum { IPL_PATH_AUTO = 0, IPL_PATH_PLAIN = 1, IPL_PATH_FUSED = 2 };
+/* flux_axes: bit 0 = x, bit 1 = y, bit 2 = z; IPL_FLUX_REFERENCE selects the reference's
+ * choice (x in 2D, x and y with the two-axis corner scheme in 3D, adv.cu:50-147). */
+enum { IPL_FLUX_NONE = 0, IPL_FLUX_X = 1, IPL_FLUX_Y = 2, IPL_FLUX_Z = 4, IPL_FLUX_REFERENCE = -1 };
+
+/* fields readable / writable with ipl_read / ipl_write */
+enum {
+    IPL_FIELD_P = 0,      /* pressure-like zeroth moment p = sum f            (update_rho, collision.cu:238) */
+    IPL_FIELD_U = 1, IPL_FIELD_V = 2, IPL_FIELD_W = 3,
+    IPL_FIELD_RHO = 4,    /* mass density rho_L + (rho_H - rho_L) phi + beta_s S */
+    IPL_FIELD_S = 5,      /* salinity */
+    IPL_FIELD_PHI = 6,    /* phase field */
+    IPL_FIELD_FX = 7, IPL_FIELD_FY = 8, IPL_FIELD_FZ = 9,
+    IPL_FIELD_TAU = 10, IPL_FIELD_TAUS = 11,
+    IPL_FIELD_F = 12,     /* flow populations, [k][cell] (k-major), cells canonical */
+    IPL_FIELD_G = 13,     /* scalar populations, [k][cell] */
+    IPL_FIELD_COUNT = 14
+};
+
+/* Run configuration.  Mirrors the reference's Items class plus items[] parameter block
+ * (all.hpp:31-37, itemIndex.hpp:4-33) and the constants each driver hard-codes
+ * (e.g. LockExchange.cu:13-15,77,223). */
+typedef struct ipl_config {
+    int32_t lattice;          /* 9 (D2Q9, x-z plane), 19 (D3Q19) or 27 (D3Q27) */
+    int32_t nx, ny, nz;       /* cells per axis; ny == 1 for D2Q9 */
+    double dz;                /* lattice spacing = width of a uniform cell (items[IDX_dz]) */
+    double dt;                /* time step; c = dz / dt (basic.cu:71) */
+    const double* pos[3];     /* cell-centre coordinate per axis index (length nx, ny, nz) */
+    const double* width[3];   /* cell width per axis index; must be >= dz on flux axes */
+    const uint8_t* solid;     /* optional nx*ny*nz mask in (x,y,z) order, 1 = excluded; NULL = none */
+    int32_t periodic[3];      /* 1 = periodic axis (no walls), 0 = walls at both ends */
+    int32_t wall_bc;          /* IPL_BC_SLIP or IPL_BC_BOUNCE_BACK */
+    int32_t flux_axes;        /* see IPL_FLUX_* */
+    int32_t stencil;          /* 5 (reference, centred) or 3 (cm, c, cp) Lagrange points */
+    int32_t collision;        /* IPL_COLL_MRT or IPL_COLL_SRT */
+    double s_rates[27];       /* MRT rates in moment order; stress rows use 1/tau per cell */
+    double tau;               /* uniform tau when init->tau is NULL */
+    double taus;              /* uniform scalar tau when init->taus is NULL */
+    double rho_L, rho_H;      /* mass density of phi = 0 / phi = 1 (update_rho, collision.cu:251) */
+    double beta_s;            /* density per unit salinity (0.824493 in the reference) */
+    double gravity;           /* acceleration along -z (9.81 in the reference, collision.cu:78-79) */
+    int32_t boussinesq;       /* 1: Fz -= S beta_s g ; 0: Fz -= rho g */
+    double sigma;             /* surface tension (items[IDX_sigma]) */
+    double pf_thickness;      /* phase-field interface width W (items[IDX_PFthick]) */
+    int32_t scalar;           /* IPL_SCALAR_* */
+    int32_t scalar_lattice;   /* 0 = same lattice as the flow; 7 = D3Q7 (3D only) */
+    int32_t storage;          /* IPL_F64 / IPL_F32 / IPL_F16 (FP16 only on the fused path) */
+    int32_t path;             /* IPL_PATH_* */
+    double f16_scale;         /* FP16 storage: populations stored as (f - w p0) * f16_scale; 0 = auto */
+    int32_t check_finite;     /* 1: ipl_step checks p and u for NaN/Inf after the last step */
+    double pf_tau;            /* phase-field relaxation time; the reference hard-codes 0.58
+                                 (collision.cu:218), i.e. mobility (pf_tau - 0.5) c^2 dt / 3 */
+    int32_t reserved[6];
+} ipl_config;
+
+/* Initial state, canonical cell order, length = number of calc cells.  NULL entries default to
+ * zero (p, u, v, w, S), to 0 (phi) or to the config scalars (tau, taus).  Populations start at
+ * equilibrium: f = feq(p, u) (LockExchange.cu:133-139); g = geq(S or phi, u). */
+typedef struct ipl_init {
+    const double* p;
+    const double* u;
+    const double* v;
+    const double* w;
+    const double* S;
+    const double* phi;
+    const double* tau;
+    const double* taus;
+} ipl_init;
+
+typedef struct ipl_ctx ipl_ctx;
+
+/* Geometry, neighbour / ghost tables, boundary remap tables, MRT matrices, initial populations;
+ * replaces the driver prologue: input_items + Items::setc + set_M + grid/neib build +
+ * set_neibghor_wall + set_bound2D/3D + cuMallocCopy (basic.cu:57-98,171-237;
+ * set_bound.cu:25-147; cuda_mem.cu:3-7; LockExchange.cu:12-189). */
+int ipl_create(const ipl_config* cfg, const ipl_init* init, ipl_ctx** out);
+
+/* Advances nsteps steps: equ_f, Force, collision, IP_process(f), scalar collision,
+ * IP_process(g), update_scalar, update_rho, set_wall_rho, resetF (LockExchange.cu:220-278,
+ * Zhou_soliton4B1.cu:220-277 with its stale call sites repaired). */
+int ipl_step(ipl_ctx* ctx, int32_t nsteps);
+
+/* Same as ipl_step but returns the device time of the steps (HIP events on the context's
+ * stream), in milliseconds. */
+int ipl_step_timed(ipl_ctx* ctx, int32_t nsteps, double* ms);
+
+/* Device -> host copy of a field in canonical cell order (the reference's save-interval
+ * cudaMemcpy, LockExchange.cu:245-256).  bytes must equal the field size in doubles. */
+int ipl_read(ipl_ctx* ctx, int32_t field, double* host, size_t count);
+
+/* Host -> device overwrite of a field (canonical order, doubles). */
+int ipl_write(ipl_ctx* ctx, int32_t field, const double* host, size_t count);
+
+/* Volume-weighted global sums in FP64 of p, S and phi over calc cells: the conservation
+ * monitor printed by output() (basic.cu:9-15). */
+int ipl_global_sums(ipl_ctx* ctx, double out[3]);
+
+/* Number of calc cells, ghost cells and population storage bytes of a context. */
+int ipl_info(ipl_ctx* ctx, int64_t* n_calc, int64_t* n_ghost, int64_t* device_bytes, int32_t* path);
+
+int ipl_destroy(ipl_ctx* ctx);
+
+const char* ipl_error_string(int code);
+const char* ipl_last_error_message(ipl_ctx* ctx);
+
+/* ---- stage-wise entry points (parity harness; plain path only) ----------------------------- */
+enum {
+    IPL_STAGE_EQU_F = 0,        /* collision.cu:3-14 */
+    IPL_STAGE_FORCE = 1,        /* collision.cu:57-112 */
+    IPL_STAGE_COLLIDE = 2,      /* col_f_MRT / col_f_SRT, collision.cu:115-165 */
+    IPL_STAGE_IP_F = 3,         /* IP_process on f, adv.cu:245-257 */
+    IPL_STAGE_SCALAR_COLLIDE = 4, /* col_g_reg / col_PF, collision.cu:167-223 */
+    IPL_STAGE_IP_G = 5,         /* IP_process on g */
+    IPL_STAGE_UPDATE_SCALAR = 6,/* collision.cu:225-236 */
+    IPL_STAGE_UPDATE_RHO = 7,   /* collision.cu:238-257 */
+    IPL_STAGE_SET_WALL_RHO = 8, /* basic.cu:254-261 (rho, and phi with a phase field) */
+    IPL_STAGE_RESET_F = 9       /* resetF, collision.cu:258-264 */
+};
+int ipl_stage(ipl_ctx* ctx, int32_t stage);
+
+/* ---- host-only diagnostics ------------------------------------------------------------------
+ * Builds the cell graph of cfg on the host (no device work) and copies one table out:
+ * which = 0 neighbour table [k][n_calc + n_ghost] (ghosts after calc cells, basic.cu:203-237),
+ * 1 / 2 = slip remap nextB / nextK [k][n_ghost] (set_bound.cu:25-147), 3 = stencil weights
+ * [(axis*2 + sign)*npts + j][n_calc].  With out == NULL returns the table length in *count. */
+int ipl_build_tables(const ipl_config* cfg, int32_t which, void* out, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IPL_H */

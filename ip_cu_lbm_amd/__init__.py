@@ -1,0 +1,109 @@
+"""MI355X-native interpolation-based lattice Boltzmann step (HIP, gfx950).
+
+Host-side mirror of the reference's run loop (init -> N x step -> readout, e.g.
+LockExchange/LockExchange.cu:12-291) over the C ABI of include/ipl.h:
+
+    from ip_cu_lbm_amd import Simulation, cases
+    sim = Simulation(cases.lock_exchange(storage=F32))
+    sim.step(100)
+    u = sim.read("u")
+
+Every compute call goes to the HIP library ip_cu_lbm_amd/_lib/libipl.so; there is no CPU
+fallback, and the constructor raises if the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _abi, cases
+from ._abi import (BC_BOUNCE_BACK, BC_SLIP, COLL_MRT, COLL_SRT, F16, F32, F64, FIELDS, FLUX_NONE,
+                   FLUX_REFERENCE, FLUX_X, FLUX_Y, FLUX_Z, PATH_AUTO, PATH_FUSED, PATH_PLAIN,
+                   SCALAR_NONE, SCALAR_PHASE_FIELD, SCALAR_SALINITY, STAGES, IplError)
+
+__all__ = ["Simulation", "cases", "IplError", "F64", "F32", "F16", "PATH_AUTO", "PATH_PLAIN",
+           "PATH_FUSED"]
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Simulation:
+    """One simulation context on the current GPU (ipl_create ... ipl_destroy)."""
+
+    def __init__(self, case: "cases.Case"):
+        self._lib = _abi.load_library()
+        self.case = case
+        cfg, ini, keep = case.to_c()
+        h = ctypes.c_void_p()
+        rc = self._lib.ipl_create(ctypes.byref(cfg), ctypes.byref(ini), ctypes.byref(h))
+        del keep
+        if rc != 0:
+            msg = self._lib.ipl_last_error_message(None).decode()
+            raise IplError(rc, f"{self._lib.ipl_error_string(rc).decode()}: {msg}")
+        self._h = h
+        n, g, b, p = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+        self._lib.ipl_info(h, ctypes.byref(n), ctypes.byref(g), ctypes.byref(b), ctypes.byref(p))
+        self.n_calc, self.n_ghost, self.device_bytes, self.path = n.value, g.value, b.value, p.value
+
+    # ---- lifecycle --------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ipl_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self._lib.ipl_last_error_message(self._h).decode()
+            raise IplError(rc, f"{self._lib.ipl_error_string(rc).decode()}: {msg}")
+
+    # ---- run loop ---------------------------------------------------------------------------
+    def step(self, n: int = 1):
+        self._check(self._lib.ipl_step(self._h, int(n)))
+
+    def step_timed(self, n: int) -> float:
+        """Runs n steps and returns their device time in milliseconds (HIP events)."""
+        ms = ctypes.c_double()
+        self._check(self._lib.ipl_step_timed(self._h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def stage(self, name):
+        sid = STAGES[name] if isinstance(name, str) else int(name)
+        self._check(self._lib.ipl_stage(self._h, sid))
+
+    # ---- readout ----------------------------------------------------------------------------
+    def _qf(self, field):
+        if field == "f":
+            return self.case.lattice
+        return self.case.scalar_lattice or self.case.lattice
+
+    def read(self, field: str) -> np.ndarray:
+        if field in ("f", "g"):
+            out = np.empty(self._qf(field) * self.n_calc)
+        else:
+            out = np.empty(self.n_calc)
+        self._check(self._lib.ipl_read(self._h, FIELDS[field], _dp(out), out.size))
+        return out.reshape(-1, self.n_calc) if field in ("f", "g") else out
+
+    def write(self, field: str, data):
+        a = np.ascontiguousarray(data, dtype=np.float64).ravel()
+        self._check(self._lib.ipl_write(self._h, FIELDS[field], _dp(a), a.size))
+
+    def global_sums(self) -> np.ndarray:
+        out = np.zeros(3)
+        self._check(self._lib.ipl_global_sums(self._h, _dp(out)))
+        return out

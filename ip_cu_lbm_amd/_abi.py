@@ -1,0 +1,112 @@
+"""ctypes mirror of include/ipl.h and the loader of the HIP library.
+
+The layout of ``IplConfig`` / ``IplInit`` must match ``ipl_config`` / ``ipl_init`` field by field;
+tests/test_abi.py checks the sizes against the C compiler's.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int32, c_int64, c_size_t, c_uint8, c_void_p, c_char_p
+
+IPL_OK = 0
+IPL_ERR_INVALID_CONFIG = 1
+IPL_ERR_HIP = 2
+IPL_ERR_OUT_OF_MEMORY = 3
+IPL_ERR_NONFINITE = 4
+IPL_ERR_UNSUPPORTED = 5
+IPL_ERR_ARGUMENT = 6
+IPL_ERR_BUDGET = 7
+
+BC_SLIP, BC_BOUNCE_BACK = 0, 1
+COLL_MRT, COLL_SRT = 0, 1
+SCALAR_NONE, SCALAR_SALINITY, SCALAR_PHASE_FIELD = 0, 1, 2
+F64, F32, F16 = 0, 1, 2
+PATH_AUTO, PATH_PLAIN, PATH_FUSED = 0, 1, 2
+FLUX_NONE, FLUX_X, FLUX_Y, FLUX_Z, FLUX_REFERENCE = 0, 1, 2, 4, -1
+
+FIELDS = {
+    "p": 0, "u": 1, "v": 2, "w": 3, "rho": 4, "S": 5, "phi": 6,
+    "Fx": 7, "Fy": 8, "Fz": 9, "tau": 10, "taus": 11, "f": 12, "g": 13,
+}
+
+STAGES = {
+    "equ_f": 0, "Force": 1, "collide": 2, "IP_f": 3, "scalar_collide": 4, "IP_g": 5,
+    "update_scalar": 6, "update_rho": 7, "set_wall_rho": 8, "resetF": 9,
+}
+
+_DP = POINTER(c_double)
+
+
+class IplConfig(ctypes.Structure):
+    _fields_ = [
+        ("lattice", c_int32), ("nx", c_int32), ("ny", c_int32), ("nz", c_int32),
+        ("dz", c_double), ("dt", c_double),
+        ("pos", _DP * 3), ("width", _DP * 3),
+        ("solid", POINTER(c_uint8)),
+        ("periodic", c_int32 * 3), ("wall_bc", c_int32), ("flux_axes", c_int32),
+        ("stencil", c_int32), ("collision", c_int32),
+        ("s_rates", c_double * 27), ("tau", c_double), ("taus", c_double),
+        ("rho_L", c_double), ("rho_H", c_double), ("beta_s", c_double), ("gravity", c_double),
+        ("boussinesq", c_int32), ("sigma", c_double), ("pf_thickness", c_double),
+        ("scalar", c_int32), ("scalar_lattice", c_int32), ("storage", c_int32), ("path", c_int32),
+        ("f16_scale", c_double), ("check_finite", c_int32), ("pf_tau", c_double),
+        ("reserved", c_int32 * 6),
+    ]
+
+
+class IplInit(ctypes.Structure):
+    _fields_ = [(n, _DP) for n in ("p", "u", "v", "w", "S", "phi", "tau", "taus")]
+
+
+def lib_path() -> str:
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libipl.so")
+
+
+_LIB = None
+
+
+def load_library():
+    """Load the HIP library.  Raises if it has not been built: there is no CPU fallback."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = lib_path()
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"HIP library {path} is not built; run `python -m ip_cu_lbm_amd.build` "
+            "(hipcc --offload-arch=gfx950)")
+    lib = ctypes.CDLL(path)
+    vp = c_void_p
+    lib.ipl_create.argtypes = [POINTER(IplConfig), POINTER(IplInit), POINTER(vp)]
+    lib.ipl_create.restype = c_int32
+    lib.ipl_step.argtypes = [vp, c_int32]
+    lib.ipl_step.restype = c_int32
+    lib.ipl_step_timed.argtypes = [vp, c_int32, POINTER(c_double)]
+    lib.ipl_step_timed.restype = c_int32
+    lib.ipl_read.argtypes = [vp, c_int32, _DP, c_size_t]
+    lib.ipl_read.restype = c_int32
+    lib.ipl_write.argtypes = [vp, c_int32, _DP, c_size_t]
+    lib.ipl_write.restype = c_int32
+    lib.ipl_global_sums.argtypes = [vp, _DP]
+    lib.ipl_global_sums.restype = c_int32
+    lib.ipl_info.argtypes = [vp, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64), POINTER(c_int32)]
+    lib.ipl_info.restype = c_int32
+    lib.ipl_destroy.argtypes = [vp]
+    lib.ipl_destroy.restype = c_int32
+    lib.ipl_error_string.argtypes = [c_int32]
+    lib.ipl_error_string.restype = c_char_p
+    lib.ipl_last_error_message.argtypes = [vp]
+    lib.ipl_last_error_message.restype = c_char_p
+    lib.ipl_stage.argtypes = [vp, c_int32]
+    lib.ipl_stage.restype = c_int32
+    lib.ipl_build_tables.argtypes = [POINTER(IplConfig), c_int32, c_void_p, POINTER(c_int64)]
+    lib.ipl_build_tables.restype = c_int32
+    _LIB = lib
+    return lib
+
+
+class IplError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ipl error {code}: {msg}")
+        self.code = code

@@ -1,0 +1,260 @@
+"""Case definitions: geometry, parameters and seeded initial fields.
+
+A ``Case`` is everything ``ipl_create`` needs: the run configuration (the reference's Items /
+items[] block plus the constants its drivers hard-code) and the initial macroscopic fields in
+canonical cell order (x slowest, z fastest, as the reference's lnum loop,
+LockExchange.cu:57-74).  Random fields come from SplitMix64 on the host in FP64, so every
+consumer of a case sees identical bits.
+
+The five workloads named by the project's baseline are built by ``lattice_taylor_green``,
+``lock_exchange``, ``falling_sphere_flow``, ``solitary_wave`` and the serving mix in
+``ip_cu_lbm_amd.serve``.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import _abi
+
+SEED_BASE = 20261203
+
+
+def splitmix64(seed: int, n: int, stream: int = 0) -> np.ndarray:
+    """n outputs of SplitMix64 (Steele, Lea, Flood 2014) started at seed + stream * 2^32."""
+    gamma = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        s = np.uint64((seed + (stream << 32)) & 0xFFFFFFFFFFFFFFFF) + gamma * np.arange(1, n + 1, dtype=np.uint64)
+        z = s
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed: int, n: int, lo: float, hi: float, stream: int = 0) -> np.ndarray:
+    """n doubles uniform in [lo, hi): the top 53 bits of SplitMix64."""
+    u = (splitmix64(seed, n, stream) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+    return lo + (hi - lo) * u
+
+
+def axis_from_widths(widths: np.ndarray):
+    widths = np.asarray(widths, dtype=np.float64)
+    pos = np.cumsum(widths) - 0.5 * widths
+    return pos, widths
+
+
+def stretched_widths(n: int, ratio: float, dz: float = 1.0) -> np.ndarray:
+    """Widths growing geometrically by `ratio` per cell away from the axis centre (finest at
+    the centre): width_i = dz * ratio**j, j = cells between i and the centre pair."""
+    i = np.arange(n, dtype=np.float64)
+    j = np.floor(np.abs(i - (n - 1) / 2.0))
+    return dz * ratio ** j
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    lattice: int
+    n: tuple  # (nx, ny, nz)
+    pos: list  # 3 arrays
+    width: list  # 3 arrays
+    dz: float = 1.0
+    dt: float = 1.0
+    periodic: tuple = (0, 0, 0)
+    wall_bc: int = _abi.BC_BOUNCE_BACK
+    flux_axes: int = _abi.FLUX_REFERENCE
+    stencil: int = 5
+    collision: int = _abi.COLL_MRT
+    s_rates: Optional[np.ndarray] = None
+    tau: float = 0.6
+    taus: float = 0.6
+    rho_L: float = 1.0
+    rho_H: float = 1.0
+    beta_s: float = 0.0
+    gravity: float = 0.0
+    boussinesq: int = 1
+    sigma: float = 0.0
+    pf_thickness: float = 1.0
+    scalar: int = _abi.SCALAR_NONE
+    scalar_lattice: int = 0
+    storage: int = _abi.F64
+    path: int = _abi.PATH_AUTO
+    f16_scale: float = 0.0
+    check_finite: int = 0
+    pf_tau: float = 0.58
+    solid: Optional[np.ndarray] = None
+    init: Dict[str, np.ndarray] = dataclasses.field(default_factory=dict)
+    steps: int = 100
+
+    @property
+    def n_cells(self) -> int:
+        if self.solid is not None:
+            return int(np.count_nonzero(self.solid == 0))
+        return int(self.n[0] * self.n[1] * self.n[2])
+
+    def coords(self):
+        """Cell-centre coordinates of the calc cells in canonical order (X, Y, Z arrays)."""
+        X, Y, Z = np.meshgrid(self.pos[0], self.pos[1], self.pos[2], indexing="ij")
+        X, Y, Z = X.ravel(), Y.ravel(), Z.ravel()
+        if self.solid is not None:
+            keep = self.solid.ravel() == 0
+            X, Y, Z = X[keep], Y[keep], Z[keep]
+        return X, Y, Z
+
+    def replace(self, **kw) -> "Case":
+        return dataclasses.replace(self, **kw)
+
+    # ---- ctypes marshalling (keeps the numpy buffers alive on the returned holder) ----------
+    def to_c(self):
+        keep = []
+
+        def dp(a):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+        cfg = _abi.IplConfig()
+        cfg.lattice = self.lattice
+        cfg.nx, cfg.ny, cfg.nz = self.n
+        cfg.dz, cfg.dt = self.dz, self.dt
+        for a in range(3):
+            cfg.pos[a] = dp(self.pos[a])
+            cfg.width[a] = dp(self.width[a])
+        if self.solid is not None:
+            s = np.ascontiguousarray(self.solid, dtype=np.uint8).ravel()
+            keep.append(s)
+            cfg.solid = s.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        for a in range(3):
+            cfg.periodic[a] = int(self.periodic[a])
+        cfg.wall_bc = self.wall_bc
+        cfg.flux_axes = self.flux_axes
+        cfg.stencil = self.stencil
+        cfg.collision = self.collision
+        rates = np.ones(27) if self.s_rates is None else np.asarray(self.s_rates, dtype=np.float64)
+        for i in range(min(27, len(rates))):
+            cfg.s_rates[i] = float(rates[i])
+        cfg.tau, cfg.taus = self.tau, self.taus
+        cfg.rho_L, cfg.rho_H = self.rho_L, self.rho_H
+        cfg.beta_s, cfg.gravity = self.beta_s, self.gravity
+        cfg.boussinesq = self.boussinesq
+        cfg.sigma, cfg.pf_thickness = self.sigma, self.pf_thickness
+        cfg.scalar, cfg.scalar_lattice = self.scalar, self.scalar_lattice
+        cfg.storage, cfg.path = self.storage, self.path
+        cfg.f16_scale = self.f16_scale
+        cfg.check_finite = self.check_finite
+        cfg.pf_tau = self.pf_tau
+        ini = _abi.IplInit()
+        for key in ("p", "u", "v", "w", "S", "phi", "tau", "taus"):
+            if key in self.init and self.init[key] is not None:
+                arr = np.asarray(self.init[key], dtype=np.float64)
+                if arr.shape != (self.n_cells,):
+                    raise ValueError(f"init field {key} has shape {arr.shape}, expected ({self.n_cells},)")
+                setattr(ini, key, dp(arr))
+        return cfg, ini, keep
+
+
+def _uniform_axis(n: int, dz: float = 1.0):
+    return axis_from_widths(np.full(n, dz))
+
+
+def lattice_taylor_green(nx: int = 256, nz: int = 128, ratio: float = 1.01, tau: float = 0.6,
+                         amp: float = 0.01, seed: int = SEED_BASE + 1, steps: int = 1000,
+                         stencil: int = 5, storage: int = _abi.F64, path: int = _abi.PATH_AUTO) -> Case:
+    """Plain-method case: D2Q9 MRT, x geometrically stretched from the centre, uniform z,
+    periodic x, half-way bounce-back walls in z, rho = 1 + U(-1e-3, 1e-3) and a Taylor-Green
+    velocity of amplitude `amp` that vanishes normal to the walls."""
+    px, wx = axis_from_widths(stretched_widths(nx, ratio))
+    py, wy = _uniform_axis(1)
+    pz, wz = _uniform_axis(nz)
+    Lx, Lz = wx.sum(), wz.sum()
+    case = Case(name=f"taylor_green_{nx}x{nz}", lattice=9, n=(nx, 1, nz), pos=[px, py, pz],
+                width=[wx, wy, wz], periodic=(1, 0, 0), wall_bc=_abi.BC_BOUNCE_BACK,
+                flux_axes=_abi.FLUX_X, stencil=stencil,
+                s_rates=np.array([0, 1.4, 1.4, 0, 1.2, 0, 1.2, 1 / tau, 1 / tau]), tau=tau,
+                storage=storage, path=path, steps=steps)
+    X, _, Z = case.coords()
+    kx, kz = 2 * np.pi / Lx, np.pi / Lz
+    rho = 1.0 + uniform(seed, case.n_cells, -1e-3, 1e-3)
+    case.init = {
+        "p": rho - 1.0,
+        "u": amp * np.sin(kx * X) * np.cos(kz * Z),
+        "w": -amp * (kx / kz) * np.cos(kx * X) * np.sin(kz * Z),
+    }
+    return case
+
+
+def lock_exchange(nx: int = 512, ny: int = 32, nz: int = 64, ratio: float = 1.02, tau: float = 0.55,
+                  g_reduced: float = 1e-5, diffusivity: float = 1e-3, width: float = 3.0,
+                  noise: float = 1e-4, seed: int = SEED_BASE + 2, steps: int = 5000,
+                  storage: int = _abi.F64, path: int = _abi.PATH_AUTO, scalar_lattice: int = 7,
+                  lattice: int = 19) -> Case:
+    """Stratified lock exchange: D3Q19 MRT flow + D3Q7 salinity, Boussinesq buoyancy with
+    reduced gravity g' = beta_s * g, z stretched toward mid-depth, slip walls at the x ends and
+    in z, periodic span (y).  S = (1 - tanh((x - x0)/width))/2 across the lock gate x0 = Lx/2
+    plus seeded noise of amplitude `noise` where 0.01 < S < 0.99."""
+    px, wx = _uniform_axis(nx)
+    py, wy = _uniform_axis(ny)
+    pz, wz = axis_from_widths(stretched_widths(nz, ratio))
+    cs2 = 0.25 if scalar_lattice == 7 else 1.0 / 3.0
+    taus = diffusivity / cs2 + 0.5
+    beta_s = 1e-3
+    rates = np.ones(27)
+    case = Case(name=f"lock_exchange_{nx}x{ny}x{nz}", lattice=lattice, n=(nx, ny, nz),
+                pos=[px, py, pz], width=[wx, wy, wz], periodic=(0, 1, 0), wall_bc=_abi.BC_SLIP,
+                flux_axes=_abi.FLUX_Z, stencil=5, s_rates=rates, tau=tau, taus=taus,
+                rho_L=1.0, rho_H=1.0, beta_s=beta_s, gravity=g_reduced / beta_s, boussinesq=1,
+                scalar=_abi.SCALAR_SALINITY, scalar_lattice=scalar_lattice, storage=storage,
+                path=path, steps=steps)
+    X, _, _ = case.coords()
+    x0 = wx.sum() / 2.0
+    S = 0.5 * (1.0 - np.tanh((X - x0) / width))
+    band = (S > 0.01) & (S < 0.99)
+    S = S + band * uniform(seed, case.n_cells, -noise, noise)
+    case.init = {"S": S}
+    return case
+
+
+def solitary_wave(nx: int = 1024, nz: int = 256, ratio: float = 1.01, width: float = 4.0,
+                  mobility: float = 0.02, rho_ratio: float = 1.03, depth_lower: float = 0.6,
+                  amp: float = 0.3, x0: float = 200.0, noise: float = 1e-5, tau: float = 0.6,
+                  gravity: float = 1e-5, sigma: float = 0.0, seed: int = SEED_BASE + 4,
+                  steps: int = 10000, storage: int = _abi.F64, path: int = _abi.PATH_AUTO) -> Case:
+    """Two-layer internal solitary wave: D2Q9 MRT flow + D2Q9 conservative Allen-Cahn phase
+    field (collision.cu:190-223; step order of J_Zhou_solitary_wave4B1/Zhou_soliton4B1.cu:220-277).
+    phi = 1 marks the denser lower layer (rho_H = rho_ratio * rho_L), z is stretched away from
+    the undisturbed interface (finest there), the interface is a depression sech^2 hump of
+    amplitude `amp` times the upper-layer depth centred at x0, plus seeded noise on phi."""
+    px, wx = _uniform_axis(nx)
+    py, wy = _uniform_axis(1)
+    # finest cells at the interface height, geometric growth toward bottom and top
+    i_int = int(round(depth_lower * nz))
+    j = np.abs(np.arange(nz) - (i_int - 0.5))
+    wz = ratio ** np.floor(j)
+    pz, wz = axis_from_widths(wz)
+    Lz = wz.sum()
+    z_int = np.cumsum(wz)[i_int - 1]
+    h_upper = Lz - z_int
+    case = Case(name=f"solitary_wave_{nx}x{nz}", lattice=9, n=(nx, 1, nz), pos=[px, py, pz],
+                width=[wx, wy, wz], periodic=(1, 0, 0), wall_bc=_abi.BC_SLIP, flux_axes=_abi.FLUX_Z,
+                stencil=5, s_rates=np.array([0, 1.4, 1.4, 0, 1.2, 0, 1.2, 1 / tau, 1 / tau]), tau=tau,
+                taus=0.5 + 3 * mobility, pf_tau=0.5 + 3 * mobility, rho_L=1.0, rho_H=rho_ratio,
+                gravity=gravity, boussinesq=0, sigma=sigma, pf_thickness=width,
+                scalar=_abi.SCALAR_PHASE_FIELD, storage=storage, path=path, steps=steps)
+    X, _, Z = case.coords()
+    Delta = 20.0 * width
+    eta = -amp * h_upper / np.cosh((X - x0) / Delta) ** 2
+    phi = 0.5 - 0.5 * np.tanh(2.0 * (Z - (z_int + eta)) / width)
+    phi = phi + uniform(seed, case.n_cells, -noise, noise)
+    case.init = {"phi": phi}
+    return case
+
+
+CASES = {
+    "taylor_green": lattice_taylor_green,
+    "lock_exchange": lock_exchange,
+    "solitary_wave": solitary_wave,
+}

@@ -1,0 +1,200 @@
+// api.cpp - the extern "C" boundary (include/ipl.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "engine.hpp"
+
+struct ipl_ctx {
+    ipl_config cfg{};
+    ipl::Graph graph;
+    std::unique_ptr<ipl::Engine> engine;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string last_error;
+    int device = 0;
+};
+
+namespace {
+
+int fail(ipl_ctx* ctx, const ipl::Status& s) {
+    if (ctx) ctx->last_error = s.msg;
+    return s.code;
+}
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+extern "C" {
+
+const char* ipl_error_string(int code) {
+    switch (code) {
+    case IPL_OK: return "ok";
+    case IPL_ERR_INVALID_CONFIG: return "invalid configuration";
+    case IPL_ERR_HIP: return "HIP runtime error";
+    case IPL_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case IPL_ERR_NONFINITE: return "non-finite value in the macroscopic fields";
+    case IPL_ERR_UNSUPPORTED: return "configuration not supported by the requested path";
+    case IPL_ERR_ARGUMENT: return "bad argument";
+    case IPL_ERR_BUDGET: return "job does not fit the memory budget";
+    default: return "unknown error";
+    }
+}
+
+const char* ipl_last_error_message(ipl_ctx* ctx) {
+    return ctx ? ctx->last_error.c_str() : g_create_error.c_str();
+}
+
+int ipl_create(const ipl_config* cfg, const ipl_init* init, ipl_ctx** out) {
+    if (!cfg || !out) return IPL_ERR_ARGUMENT;
+    *out = nullptr;
+    g_create_error.clear();
+    ipl_ctx* ctx = new (std::nothrow) ipl_ctx();
+    if (!ctx) return IPL_ERR_OUT_OF_MEMORY;
+    ctx->cfg = *cfg;
+    ctx->cfg.solid = nullptr;
+    for (int a = 0; a < 3; ++a) { ctx->cfg.pos[a] = nullptr; ctx->cfg.width[a] = nullptr; }
+    std::string err;
+    if (!ipl::build_graph(*cfg, ctx->graph, err)) {
+        g_create_error = err;
+        delete ctx;
+        return IPL_ERR_INVALID_CONFIG;
+    }
+    auto bail = [&](const ipl::Status& s) {
+        g_create_error = s.msg;
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+        if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+        ctx->engine.reset();
+        delete ctx;
+        return s.code;
+    };
+    {
+        hipError_t e = hipGetDevice(&ctx->device);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+        if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+        if (e != hipSuccess) return bail(ipl::Status::err(IPL_ERR_HIP, std::string("HIP setup: ") + hipGetErrorString(e)));
+    }
+    ipl::HostState hs;
+    ipl::Status st = ipl::make_host_state(ctx->graph, ctx->cfg, init, hs);
+    if (!st) return bail(st);
+    std::string why;
+    const bool fused_ok = ipl::fused_supported(ctx->graph, ctx->cfg, why);
+    int path = ctx->cfg.path;
+    if (path == IPL_PATH_AUTO) path = fused_ok ? IPL_PATH_FUSED : IPL_PATH_PLAIN;
+    if (path == IPL_PATH_FUSED && !fused_ok) return bail(ipl::Status::err(IPL_ERR_UNSUPPORTED, "fused path: " + why));
+    if (path == IPL_PATH_FUSED) ctx->engine = ipl::make_fused_engine(ctx->graph, ctx->cfg, hs, ctx->stream, st);
+    else ctx->engine = ipl::make_plain_engine(ctx->graph, ctx->cfg, hs, ctx->stream, st);
+    if (!st || !ctx->engine) return bail(st ? ipl::Status::err(IPL_ERR_UNSUPPORTED, "no engine") : st);
+    *out = ctx;
+    return IPL_OK;
+}
+
+int ipl_step(ipl_ctx* ctx, int32_t nsteps) {
+    if (!ctx || nsteps < 0) return IPL_ERR_ARGUMENT;
+    ipl::Status s = ctx->engine->step(nsteps);
+    if (!s) return fail(ctx, s);
+    if (ctx->cfg.check_finite) {
+        bool finite = true;
+        s = ctx->engine->check_finite(&finite);
+        if (!s) return fail(ctx, s);
+        if (!finite) return fail(ctx, ipl::Status::err(IPL_ERR_NONFINITE, "NaN or Inf in p/u/v/w"));
+    }
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(ctx, ipl::Status::err(IPL_ERR_HIP, hipGetErrorString(e)));
+    return IPL_OK;
+}
+
+int ipl_step_timed(ipl_ctx* ctx, int32_t nsteps, double* ms) {
+    if (!ctx || nsteps < 0 || !ms) return IPL_ERR_ARGUMENT;
+    hipError_t e = hipEventRecord(ctx->ev0, ctx->stream);
+    if (e != hipSuccess) return fail(ctx, ipl::Status::err(IPL_ERR_HIP, hipGetErrorString(e)));
+    ipl::Status s = ctx->engine->step(nsteps);
+    if (!s) return fail(ctx, s);
+    e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    float t = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+    if (e != hipSuccess) return fail(ctx, ipl::Status::err(IPL_ERR_HIP, hipGetErrorString(e)));
+    *ms = t;
+    return IPL_OK;
+}
+
+int ipl_stage(ipl_ctx* ctx, int32_t stage) {
+    if (!ctx) return IPL_ERR_ARGUMENT;
+    ipl::Status s = ctx->engine->stage(stage);
+    if (!s) return fail(ctx, s);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(ctx, ipl::Status::err(IPL_ERR_HIP, hipGetErrorString(e)));
+    return IPL_OK;
+}
+
+int ipl_read(ipl_ctx* ctx, int32_t field, double* host, size_t count) {
+    if (!ctx || !host) return IPL_ERR_ARGUMENT;
+    ipl::Status s = ctx->engine->read(field, host, count);
+    return s ? IPL_OK : fail(ctx, s);
+}
+
+int ipl_write(ipl_ctx* ctx, int32_t field, const double* host, size_t count) {
+    if (!ctx || !host) return IPL_ERR_ARGUMENT;
+    ipl::Status s = ctx->engine->write(field, host, count);
+    return s ? IPL_OK : fail(ctx, s);
+}
+
+int ipl_global_sums(ipl_ctx* ctx, double out[3]) {
+    if (!ctx || !out) return IPL_ERR_ARGUMENT;
+    ipl::Status s = ctx->engine->global_sums(out);
+    return s ? IPL_OK : fail(ctx, s);
+}
+
+int ipl_info(ipl_ctx* ctx, int64_t* n_calc, int64_t* n_ghost, int64_t* device_bytes, int32_t* path) {
+    if (!ctx) return IPL_ERR_ARGUMENT;
+    if (n_calc) *n_calc = ctx->graph.N;
+    if (n_ghost) *n_ghost = ctx->graph.G;
+    if (device_bytes) *device_bytes = ctx->engine->device_bytes();
+    if (path) *path = ctx->engine->path();
+    return IPL_OK;
+}
+
+int ipl_destroy(ipl_ctx* ctx) {
+    if (!ctx) return IPL_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->engine.reset();
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return IPL_OK;
+}
+
+int ipl_build_tables(const ipl_config* cfg, int32_t which, void* out, int64_t* count) {
+    if (!cfg || !count) return IPL_ERR_ARGUMENT;
+    ipl::Graph g;
+    std::string err;
+    if (!ipl::build_graph(*cfg, g, err)) {
+        g_create_error = err;
+        return IPL_ERR_INVALID_CONFIG;
+    }
+    const std::vector<int32_t>* t = nullptr;
+    if (which == 0) t = &g.neib;
+    else if (which == 1) t = &g.nextB;
+    else if (which == 2) t = &g.nextK;
+    if (which == 3) {
+        if (!out) { *count = (int64_t)g.weights.size(); return IPL_OK; }
+        if (*count != (int64_t)g.weights.size()) return IPL_ERR_ARGUMENT;
+        std::memcpy(out, g.weights.data(), g.weights.size() * sizeof(double));
+        return IPL_OK;
+    }
+    if (!t) return IPL_ERR_ARGUMENT;
+    if (!out) { *count = (int64_t)t->size(); return IPL_OK; }
+    if (*count != (int64_t)t->size()) return IPL_ERR_ARGUMENT;
+    std::memcpy(out, t->data(), t->size() * sizeof(int32_t));
+    return IPL_OK;
+}
+
+}  // extern "C"

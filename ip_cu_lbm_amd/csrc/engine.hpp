@@ -1,0 +1,96 @@
+// engine.hpp - the per-context engine interface behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ipl.h"
+#include "graph.hpp"
+#include "plain.hpp"
+
+namespace ipl {
+
+// Status with a message; every HIP error is turned into one of these.
+struct Status {
+    int code = IPL_OK;
+    std::string msg;
+    static Status ok() { return {}; }
+    static Status err(int c, std::string m) { return {c, std::move(m)}; }
+    explicit operator bool() const { return code == IPL_OK; }
+};
+
+#define IPL_HIP_TRY(expr)                                                                      \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return ::ipl::Status::err(e_ == hipErrorOutOfMemory ? IPL_ERR_OUT_OF_MEMORY : IPL_ERR_HIP, \
+                                      std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+// Device allocation that frees itself.
+class DeviceArena {
+public:
+    ~DeviceArena() { release(); }
+    Status alloc(void** p, size_t bytes) {
+        *p = nullptr;
+        if (bytes == 0) bytes = 256;
+        IPL_HIP_TRY(hipMalloc(p, bytes));
+        ptrs_.push_back(*p);
+        total_ += bytes;
+        return Status::ok();
+    }
+    template <typename T> Status alloc_n(T** p, size_t n) { return alloc((void**)p, n * sizeof(T)); }
+    void release() {
+        for (void* p : ptrs_) (void)hipFree(p);
+        ptrs_.clear();
+        total_ = 0;
+    }
+    size_t total() const { return total_; }
+
+private:
+    std::vector<void*> ptrs_;
+    size_t total_ = 0;
+};
+
+// Host-side initial state expanded to all cells (calc + ghosts) in FP64.
+struct HostState {
+    std::vector<double> p, u, v, w, rho, phi, S, tau, taus;
+    std::vector<double> f, g;  // [k][Ntot]
+};
+Status make_host_state(const Graph& g, const ipl_config& cfg, const ipl_init* init, HostState& hs);
+
+class Engine {
+public:
+    virtual ~Engine() = default;
+    virtual Status step(int nsteps) = 0;
+    virtual Status stage(int s) = 0;
+    virtual Status read(int field, double* host, size_t count) = 0;
+    virtual Status write(int field, const double* host, size_t count) = 0;
+    virtual Status global_sums(double out[3]) = 0;
+    virtual Status check_finite(bool* finite) = 0;
+    virtual int64_t device_bytes() const = 0;
+    virtual int path() const = 0;
+    hipStream_t stream = nullptr;
+};
+
+Params make_params(const ipl_config& cfg, const Graph& g);
+
+std::unique_ptr<Engine> make_plain_engine(const Graph& g, const ipl_config& cfg, const HostState& hs,
+                                          hipStream_t st, Status& status);
+std::unique_ptr<Engine> make_fused_engine(const Graph& g, const ipl_config& cfg, const HostState& hs,
+                                          hipStream_t st, Status& status);
+// Whether the fused path can run this configuration; fills why when it cannot.
+bool fused_supported(const Graph& g, const ipl_config& cfg, std::string& why);
+
+// Deterministic FP64 reductions on the device (fixed block partition, fixed order).
+template <typename T>
+Status device_weighted_sums(const T* const* fields, int nfields, const T* wx, const T* wy, const T* wz,
+                            int64_t n, double inv_vol, double* out, hipStream_t st);
+template <typename T>
+Status device_all_finite(const T* const* fields, int nfields, int64_t n, bool* finite, hipStream_t st);
+
+}  // namespace ipl

@@ -1,0 +1,580 @@
+// plain.hip - the plain GPU path: one kernel per stage of the reference step, structure-of-
+// arrays populations [k][cell] (coalesced across the 64 lanes of a wavefront), neighbour and
+// ghost tables from graph.cpp, FP64 Lagrange weights precomputed at init.  It follows the
+// reference's stage semantics exactly (including its ghost copies and scratch aliasing inside
+// IP_process, adv.cu:245-257) so it is bitwise-comparable with the FP64 oracle up to
+// reassociation, and it is the "plain FP64" baseline of the fused path.
+#include <hip/hip_runtime.h>
+
+#include "lattice.hpp"
+#include "plain.hpp"
+
+namespace ipl {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// Reference direction tables of the 3D two-axis corner scheme (set_cpm / set_cxypm,
+// basic.cu:122-153); unset entries are 0 (= the cell itself), as in the reference.
+__constant__ int c_cp[27] = {0, 1, 2, 3, 4, 0, 0, 7, 8, 9, 10, 1, 1, 3, 3, 2, 2, 4, 4, 7, 7, 8, 8, 9, 9, 10, 10};
+__constant__ int c_cm[27] = {0, 3, 4, 1, 2, 0, 0, 10, 9, 8, 7, 3, 3, 1, 1, 4, 4, 2, 2, 10, 10, 9, 9, 8, 8, 7, 7};
+__constant__ int c_cip[27] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 8, 8, 7, 7, 10, 10, 9, 9};
+__constant__ int c_cim[27] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 9, 9, 10, 10, 7, 7, 8, 8};
+__constant__ int c_cxp[27] = {0, 1, 0, 3, 0, 0, 0, 1, 1, 3, 3, 1, 1, 3, 3, 0, 0, 0, 0, 1, 1, 1, 1, 3, 3, 3, 3};
+__constant__ int c_cxm[27] = {0, 3, 0, 1, 0, 0, 0, 3, 3, 1, 1, 3, 3, 1, 1, 0, 0, 0, 0, 3, 3, 3, 3, 1, 1, 1, 1};
+__constant__ int c_cyp[27] = {0, 0, 2, 0, 4, 0, 0, 2, 4, 2, 4, 0, 0, 0, 0, 2, 2, 4, 4, 2, 2, 4, 4, 2, 2, 4, 4};
+__constant__ int c_cym[27] = {0, 0, 4, 0, 2, 0, 0, 4, 2, 4, 2, 0, 0, 0, 0, 4, 4, 2, 2, 4, 4, 2, 2, 4, 4, 2, 2};
+
+template <typename T> __device__ inline void atomic_add(T* a, T v) { atomicAdd(a, v); }
+
+// ---- equ_f (collision.cu:3-14) -----------------------------------------------------------
+template <typename T, int Q>
+__global__ void k_equ_f(PlainView<T> s, Params P) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    const T c = P.c, c2 = P.c * P.c;
+    const T u = s.u[i], v = s.v[i], w = s.w[i], p = s.p[i];
+    const T u2 = u * u + v * v + w * w;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const Dir d = Lattice<Q>::c(k);
+        const T t = (T(d.x) * c * u + T(d.y) * c * v + T(d.z) * c * w) / c2;
+        s.feq[k * s.Ntot + i] = T(Lattice<Q>::w(k)) * (p + T(3.0) * t + T(4.5) * t * t - T(1.5) * u2 / c2);
+    }
+}
+
+// ---- Force (collision.cu:57-112) ---------------------------------------------------------
+template <typename T, int Q>
+__global__ void k_force(PlainView<T> s, Params P) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    constexpr bool d3 = Lattice<Q>::dim == 3;
+    constexpr int up = d3 ? 5 : 2, down = d3 ? 6 : 4;
+    const int64_t Nt = s.Ntot;
+    const int n1 = s.neib[1 * Nt + i], n3 = s.neib[3 * Nt + i], n2 = s.neib[2 * Nt + i], n4 = s.neib[4 * Nt + i];
+    const int nu = s.neib[up * Nt + i], nd = s.neib[down * Nt + i];
+    const T eps = T(P.dz) / T(1000);
+    const T* ph = s.phi; const T* rh = s.rho;
+    const T* px = s.pos[0]; const T* py = s.pos[1]; const T* pz = s.pos[2];
+    T gx = (ph[n1] - ph[n3]) / (px[n1] - px[n3] + eps);
+    T gy = 0;
+    if (d3) gy = (ph[n2] - ph[n4]) / (py[n2] - py[n4] + eps);
+    T gz = (ph[nu] - ph[nd]) / (pz[nu] - pz[nd] + eps);
+    const T wdx = s.del[0][i], wd2 = s.del[d3 ? 1 : 2][i];
+    const T lap = (ph[n1] - T(2) * ph[i] + ph[n3]) / (wdx * wdx) + (ph[n2] - T(2) * ph[i] + ph[n4]) / (wd2 * wd2);
+    const T beta = T(12.0) * T(P.sigma) / T(P.pf_thickness), kap = T(3.0) * T(P.sigma) * T(P.pf_thickness) / T(2.0);
+    const T st = T(4) * beta * ph[i] * (ph[i] - T(1.0)) * (ph[i] - T(0.5)) - kap * lap;
+    T Fx = s.Fx[i] + st * gx, Fy = s.Fy[i] + st * gy, Fz = s.Fz[i] + st * gz;
+    if (P.boussinesq) Fz += -s.S[i] * T(P.beta_s) * T(P.gravity);
+    else Fz += -rh[i] * T(P.gravity);
+    gx = (rh[n1] - rh[n3]) / (px[n1] - px[n3] + eps);
+    gy = 0;
+    if (d3) gy = (rh[n2] - rh[n4]) / (py[n2] - py[n4] + eps);
+    gz = (rh[nu] - rh[nd]) / (pz[nu] - pz[nd] + eps);
+    const T c = P.c, c2 = P.c * P.c;
+    Fx += -s.p[i] * c2 / T(3.0) * gx;
+    Fy += -s.p[i] * c2 / T(3.0) * gy;
+    Fz += -s.p[i] * c2 / T(3.0) * gz;
+    s.Fx[i] = Fx; s.Fy[i] = Fy; s.Fz[i] = Fz;
+    const T den = rh[i] * c2 / T(3.0);
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const Dir d = Lattice<Q>::c(k);
+        s.Fk[k * Nt + i] = T(Lattice<Q>::w(k)) * T(P.dt) * (T(d.x) * c * Fx + T(d.y) * c * Fy + T(d.z) * c * Fz) / den;
+    }
+}
+
+// ---- col_f_MRT / col_f_SRT (collision.cu:115-165) ----------------------------------------
+// MRT in moment space: m = M d, m_j *= s_j (stress rows 1/tau of the cell), f -= M^-1 m.
+// M and M^-1 are compile-time constants, so the transforms reduce to adds.
+template <typename T, int Q>
+__global__ void k_collide(PlainView<T> s, Params P) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    const int64_t Nt = s.Ntot;
+    T d[Q], fk[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        fk[k] = s.Fk[k * Nt + i];
+        d[k] = s.f[k * Nt + i] - s.feq[k * Nt + i] + fk[k] / T(2.0);
+    }
+    const T tau = s.tau[i];
+    if (P.collision == IPL_COLL_SRT) {
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            const T fn = s.f[k * Nt + i] - d[k] / tau + fk[k];
+            s.ftmp[k * Nt + i] = fn;
+        }
+    } else {
+        const T itau = T(1) / tau;
+        T m[Q];
+#pragma unroll
+        for (int r = 0; r < Q; ++r) {
+            T acc = 0;
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                const double mk = moment<Q>(r, k);
+                if (mk != 0.0) acc += T(mk) * d[k];
+            }
+            m[r] = acc * (is_stress<Q>(r) ? itau : T(P.rates[r]));
+        }
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            T acc = 0;
+#pragma unroll
+            for (int r = 0; r < Q; ++r) {
+                const double mi = moment_inv<Q>(k, r);
+                if (mi != 0.0) acc += T(mi) * m[r];
+            }
+            s.ftmp[k * Nt + i] = s.f[k * Nt + i] - acc + fk[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < Q; ++k) { s.f[k * Nt + i] = 0; s.feq[k * Nt + i] = 0; }
+}
+
+// ---- col_g_reg (collision.cu:167-188) ----------------------------------------------------
+template <typename T, int QS>
+__global__ void k_col_g_reg(PlainView<T> s, Params P) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    const int64_t Nt = s.Ntot;
+    const T c = P.c, c2 = P.c * P.c;
+    constexpr bool linear = QS == 7;
+    const T cs2inv = linear ? T(4.0) : T(3.0);
+    const T u = s.u[i], v = s.v[i], w = s.w[i], S = s.S[i];
+    const T u2 = u * u + v * v + w * w;
+    T geq[QS];
+    T jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int k = 0; k < QS; ++k) {
+        const Dir d = Lattice<QS>::c(k);
+        const T t = (T(d.x) * c * u + T(d.y) * c * v + T(d.z) * c * w) / c2;
+        T e = T(1.0) + cs2inv * t;
+        if (!linear) e += T(4.5) * t * t - T(1.5) * u2 / c2;
+        geq[k] = T(Lattice<QS>::w(k)) * S * e;
+        const T gn = s.g[k * Nt + i] - geq[k];
+        jx += gn * T(d.x) * c; jy += gn * T(d.y) * c; jz += gn * T(d.z) * c;
+    }
+    const T om = T(1.0) - T(1.0) / s.taus[i];
+#pragma unroll
+    for (int k = 0; k < QS; ++k) {
+        const Dir d = Lattice<QS>::c(k);
+        const T gneq = T(Lattice<QS>::w(k)) * cs2inv * (jx * T(d.x) * c + jy * T(d.y) * c + jz * T(d.z) * c) / c2;
+        s.gtmp[k * Nt + i] = geq[k] + om * gneq;
+        s.geq[k * Nt + i] = 0;
+        s.g[k * Nt + i] = 0;
+    }
+}
+
+// ---- col_PF (collision.cu:190-223) -------------------------------------------------------
+template <typename T, int Q>
+__global__ void k_col_pf(PlainView<T> s, Params P) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    constexpr bool d3 = Lattice<Q>::dim == 3;
+    constexpr int up = d3 ? 5 : 2, down = d3 ? 6 : 4;
+    const int64_t Nt = s.Ntot;
+    const T eps = T(P.dz) / T(1000);
+    const T* ph = s.phi;
+    const int n1 = s.neib[1 * Nt + i], n3 = s.neib[3 * Nt + i], n2 = s.neib[2 * Nt + i], n4 = s.neib[4 * Nt + i];
+    const int nu = s.neib[up * Nt + i], nd = s.neib[down * Nt + i];
+    const T gx = (ph[n1] - ph[n3]) / (s.pos[0][n1] - s.pos[0][n3] + eps);
+    T gy = 0;
+    if (d3) gy = (ph[n2] - ph[n4]) / (s.pos[1][n2] - s.pos[1][n4] + eps);
+    const T gz = (ph[nu] - ph[nd]) / (s.pos[2][nu] - s.pos[2][nd] + eps);
+    const T u = s.u[i], v = s.v[i], w = s.w[i], phi = ph[i];
+    const T nabu = (phi * u - s.phiold[i] * s.uold[i]) / T(P.dt);
+    const T nabv = (phi * v - s.phiold[i] * s.vold[i]) / T(P.dt);
+    const T nabw = (phi * w - s.phiold[i] * s.wold[i]) / T(P.dt);
+    s.phiold[i] = phi; s.uold[i] = u; s.vold[i] = v; s.wold[i] = w;
+    const T c = P.c, c2 = P.c * P.c;
+    const T u2 = u * u + v * v + w * w;
+    const T nrm = sqrt(gx * gx + gy * gy + gz * gz + T(1e-8));
+    const T src = (T(1.0) - T(4.0) * (phi - T(0.5)) * (phi - T(0.5))) / T(P.pf_thickness);
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const Dir d = Lattice<Q>::c(k);
+        const T cx = T(d.x) * c, cy = T(d.y) * c, cz = T(d.z) * c;
+        const T t = (cx * u + cy * v + cz * w) / c2;
+        const T geq = T(Lattice<Q>::w(k)) * phi * (T(1.0) + T(3.0) * t + T(4.5) * t * t - T(1.5) * u2 / c2);
+        const T Fkp = T(P.dt) * T(Lattice<Q>::w(k)) *
+                      (src * (cx * gx + cy * gy + cz * gz) / nrm + T(3) * (cx * nabu + cy * nabv + cz * nabw) / c2);
+        const T g = s.g[k * Nt + i];
+        s.gtmp[k * Nt + i] = g - (g - geq - Fkp * (T(P.pf_tau) - T(0.5))) / T(P.pf_tau);
+        s.geq[k * Nt + i] = 0;
+        s.g[k * Nt + i] = 0;
+    }
+}
+
+// ---- update_scalar (collision.cu:225-236) ------------------------------------------------
+template <typename T, int QS>
+__global__ void k_update_scalar(PlainView<T> s, T* out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    T a = 0;
+#pragma unroll
+    for (int k = 0; k < QS; ++k) a += s.g[k * s.Ntot + i];
+    out[i] = a;
+}
+
+// ---- update_rho (collision.cu:238-257) ---------------------------------------------------
+template <typename T, int Q>
+__global__ void k_update_rho(PlainView<T> s, Params P) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    const T c = P.c;
+    T p = 0, ux = 0, uy = 0, uz = 0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const Dir d = Lattice<Q>::c(k);
+        const T f = s.f[k * s.Ntot + i];
+        p += f; ux += T(d.x) * c * f; uy += T(d.y) * c * f; uz += T(d.z) * c * f;
+    }
+    const T rho = T(P.rho_L) + (T(P.rho_H) - T(P.rho_L)) * s.phi[i] + s.S[i] * T(P.beta_s);
+    s.p[i] = p;
+    s.rho[i] = rho;
+    s.u[i] = ux + s.Fx[i] * T(P.dt) / (T(2.0) * rho);
+    s.v[i] = uy + s.Fy[i] * T(P.dt) / (T(2.0) * rho);
+    s.w[i] = uz + s.Fz[i] * T(P.dt) / (T(2.0) * rho);
+}
+
+// ---- set_wall_rho (basic.cu:254-261) / resetF (collision.cu:258-264) ---------------------
+template <typename T>
+__global__ void k_set_wall_field(PlainView<T> s, T* a) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + s.N;
+    if (g >= s.Ntot) return;
+    a[g] = a[s.neib[g]];
+}
+
+template <typename T>
+__global__ void k_reset_force(PlainView<T> s) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    s.Fx[i] = 0; s.Fy[i] = 0; s.Fz[i] = 0;
+}
+
+// ======================= IP_process (adv.cu:245-257) =====================================
+// Pops<T> names the four buffers of one transported field: f doubles as finy, feq as fin.
+template <typename T> struct Pops { T *f, *ftmp, *fout, *fin; const int32_t *nextB, *nextK; };
+
+// set_wall_f (adv.cu:3-13)
+template <typename T, int QF>
+__global__ void k_set_wall_f(PlainView<T> s, Pops<T> b) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + s.N;
+    if (g >= s.Ntot) return;
+    const int o = s.neib[g];
+#pragma unroll
+    for (int k = 0; k < QF; ++k) b.ftmp[k * s.Ntot + g] = b.ftmp[k * s.Ntot + o];
+}
+
+// slab reconstruction along one axis for direction k (adv.cu:76-99)
+template <typename T, int QN, int QF>
+__device__ inline void flux_1d(const PlainView<T>& s, const Pops<T>& b, const Params& P, int64_t i, int k, int axis) {
+    const int64_t Nt = s.Ntot;
+    const int sg = comp<QF>(k, axis);
+    const int dp = axis_dir<QN>(axis, sg), dm = axis_dir<QN>(axis, -sg);
+    const int cp = s.neib[dp * Nt + i], cm = s.neib[dm * Nt + i];
+    const int si = sg > 0 ? 0 : 1;
+    const int np = P.npts;
+    const T* W = s.W + (size_t)((axis * 2 + si) * np) * s.N;
+    T F;
+    if (np == 5) {
+        const int cpp = s.neib[dp * Nt + cp], cmm = s.neib[dm * Nt + cm];
+        F = W[0 * s.N + i] * b.ftmp[k * Nt + cmm] + W[1 * s.N + i] * b.ftmp[k * Nt + cm] +
+            W[2 * s.N + i] * b.ftmp[k * Nt + i] + W[3 * s.N + i] * b.ftmp[k * Nt + cp] +
+            W[4 * s.N + i] * b.ftmp[k * Nt + cpp];
+    } else {
+        F = W[0 * s.N + i] * b.ftmp[k * Nt + cm] + W[1 * s.N + i] * b.ftmp[k * Nt + i] + W[2 * s.N + i] * b.ftmp[k * Nt + cp];
+    }
+    b.fin[k * Nt + i] = F * s.rdel[axis][cp];
+    b.fout[k * Nt + i] = F * s.rdel[axis][i];
+}
+
+// corner directions of the 3D x-y scheme (adv.cu:100-145); the 5x5 tensor stencil and
+// three scattered contributions, with FP atomics as in the reference
+template <typename T, int QF>
+__device__ inline void flux_corner(const PlainView<T>& s, const Pops<T>& b, const Params& P, int64_t i, int kk) {
+    const int64_t Nt = s.Ntot;
+    auto nb = [&](int cell, int d) { return s.neib[d * Nt + cell]; };
+    int idf[25];
+    idf[12] = (int)i;
+    idf[13] = nb(i, c_cxp[kk]); idf[14] = nb(idf[13], c_cxp[kk]);
+    idf[11] = nb(i, c_cxm[kk]); idf[10] = nb(idf[11], c_cxm[kk]);
+    idf[17] = nb(i, c_cyp[kk]); idf[22] = nb(idf[17], c_cyp[kk]);
+    idf[7] = nb(i, c_cym[kk]); idf[2] = nb(idf[7], c_cym[kk]);
+    idf[18] = nb(i, c_cp[kk]); idf[24] = nb(idf[18], c_cp[kk]);
+    idf[6] = nb(i, c_cm[kk]); idf[0] = nb(idf[6], c_cm[kk]);
+    idf[8] = nb(i, c_cip[kk]); idf[4] = nb(idf[8], c_cip[kk]);
+    idf[16] = nb(i, c_cim[kk]); idf[20] = nb(idf[16], c_cim[kk]);
+    idf[19] = nb(idf[14], c_cyp[kk]); idf[9] = nb(idf[14], c_cym[kk]);
+    idf[15] = nb(idf[10], c_cyp[kk]); idf[5] = nb(idf[10], c_cym[kk]);
+    idf[23] = nb(idf[22], c_cxp[kk]); idf[21] = nb(idf[22], c_cxm[kk]);
+    idf[3] = nb(idf[2], c_cxp[kk]); idf[1] = nb(idf[2], c_cxm[kk]);
+    const int sx = comp<QF>(kk, 0), sy = comp<QF>(kk, 1);
+    const T* Wx = s.W + (size_t)((0 * 2 + (sx > 0 ? 0 : 1)) * 5) * s.N;
+    const T* Wy = s.W + (size_t)((1 * 2 + (sy > 0 ? 0 : 1)) * 5) * s.N;
+    T cxw[5], cyw[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) { cxw[j] = Wx[j * s.N + i]; cyw[j] = Wy[j * s.N + i]; }
+    const T* ft = b.ftmp + (size_t)kk * Nt;
+    T fx = 0, fy = 0, fxy = 0;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) fx += cxw[j] * ft[idf[j + 10]];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) fy += cyw[j] * ft[idf[2 + j * 5]];
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) fxy += cxw[a] * cyw[c] * ft[idf[a + c * 5]];
+    const T dz = P.dz;
+    const T* dX = s.del[0]; const T* dY = s.del[1];
+    b.fout[kk * Nt + i] = fx * dz / dX[i] + fy * dz / dY[i] - fxy * (dz * dz) / (dX[i] * dY[i]);
+    atomic_add(&b.fin[kk * Nt + i], fxy * (dz * dz) / (dX[idf[13]] * dY[idf[17]]));
+    atomic_add(&b.fin[kk * Nt + nb(i, c_cxm[kk])], (fy - fxy * dz / dX[idf[17]]) * dz / dY[idf[17]]);
+    atomic_add(&b.f[kk * Nt + nb(i, c_cym[kk])], (fx - fxy * dz / dY[idf[13]]) * dz / dX[idf[13]]);
+}
+
+// set_out (adv.cu:50-147)
+template <typename T, int QN, int QF>
+__global__ void k_set_out(PlainView<T> s, Pops<T> b, Params P) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    const int64_t Nt = s.Ntot;
+#pragma unroll
+    for (int k = 0; k < QF; ++k) {
+        int nflux = 0, ax = 0;
+        if (P.flux_mode == FLUX_MODE_ONE_AXIS) {
+            if (comp<QF>(k, P.flux_axis) != 0) { nflux = 1; ax = P.flux_axis; }
+        } else if (P.flux_mode == FLUX_MODE_CORNER) {
+            const bool hx = comp<QF>(k, 0) != 0, hy = comp<QF>(k, 1) != 0;
+            nflux = (int)hx + (int)hy; ax = hx ? 0 : 1;
+        }
+        if (nflux == 0) {
+            const T v = b.ftmp[k * Nt + i];
+            b.fout[k * Nt + i] = v;
+            b.fin[k * Nt + i] = v;
+        } else if (nflux == 1) {
+            flux_1d<T, QN, QF>(s, b, P, i, k, ax);
+        } else {
+            flux_corner<T, QF>(s, b, P, i, k);
+        }
+    }
+}
+
+// set_fin (adv.cu:148-175), corner scheme only; atomics where the reference races
+template <typename T, int QF>
+__global__ void k_set_fin(PlainView<T> s, Pops<T> b) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + s.N;
+    if (g >= s.Ntot) return;
+    const int64_t Nt = s.Ntot;
+    const int nk[8] = {9, 8, 10, 7, 7, 10, 8, 9};
+    const int nkz[8] = {19, 20, 21, 22, 23, 24, 25, 26};
+    const int nneib[8] = {1, 2, 1, 4, 3, 2, 3, 4};
+    const int k3[12] = {7, 8, 9, 10, 19, 21, 23, 25, 20, 22, 24, 26};
+    for (int k = 0; k < 4; ++k) {
+        const int t0 = s.neib[nneib[k * 2] * Nt + g], t1 = s.neib[nneib[k * 2 + 1] * Nt + g];
+        if (t0 != g) {  // the reference's (1 - neib/id) factor is 0 on self links
+            const int a = nk[2 * (k3[k] - 7)];
+            atomic_add(&b.fin[a * Nt + t0], b.fin[k3[k] * Nt + g]);
+            if (QF == 27) {
+                atomic_add(&b.fin[nkz[2 * (a - 7)] * Nt + t0], b.fin[k3[k + 4] * Nt + g]);
+                atomic_add(&b.fin[nkz[2 * (a - 7) + 1] * Nt + t0], b.fin[k3[k + 8] * Nt + g]);
+            }
+        }
+        if (t1 != g) {
+            const int bb = nk[2 * (k3[k] - 7) + 1];
+            atomic_add(&b.f[bb * Nt + t1], b.f[k3[k] * Nt + g]);
+            if (QF == 27) {
+                atomic_add(&b.f[nkz[2 * (bb - 7)] * Nt + t1], b.f[k3[k + 4] * Nt + g]);
+                atomic_add(&b.f[nkz[2 * (bb - 7) + 1] * Nt + t1], b.f[k3[k + 8] * Nt + g]);
+            }
+        }
+    }
+}
+
+// reset_f (adv.cu:176-187)
+template <typename T, int QF>
+__global__ void k_reset_f(PlainView<T> s, Pops<T> b) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+#pragma unroll
+    for (int k = 0; k < QF; ++k) {
+        const int64_t j = k * s.Ntot + i;
+        b.fin[j] += b.f[j];
+        b.f[j] = b.ftmp[j];
+    }
+}
+
+// set_wall_f_inv (adv.cu:188-232): ghost <- owner, mirrored along the flux axis; the 3D
+// corner scheme mirrors x and y and takes the xy / corner directions through nextB/nextK
+template <typename T, int QN, int QF>
+__global__ void k_set_wall_f_inv(PlainView<T> s, Pops<T> b, T* a, Params P) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + s.N;
+    if (g >= s.Ntot) return;
+    const int64_t Nt = s.Ntot;
+    const int o = s.neib[g];
+    if (P.flux_mode == FLUX_MODE_CORNER) {
+        const int kxy[19] = {0, 3, 4, 1, 2, 0, 0, 0, 0, 0, 0, 13, 14, 11, 12, 17, 18, 15, 16};
+        const int kn1[15] = {0, 1, 2, 3, 4, 5, 6, 11, 12, 13, 14, 15, 16, 17, 18};
+        for (int q = 0; q < 15; ++q) a[kn1[q] * Nt + g] = a[kxy[kn1[q]] * Nt + o];
+        const int k3[12] = {7, 8, 9, 10, 19, 21, 23, 25, 20, 22, 24, 26};
+        const int nkz[8] = {20, 19, 22, 21, 24, 23, 26, 25};
+        const int64_t G = s.Ntot - s.N, wid = g - s.N;
+        for (int q = 0; q < 4; ++q) {
+            const int kq = opposite<QF>(k3[q]);
+            const int nk = opposite<QF>(b.nextK[kq * G + wid]);
+            const int neB = b.nextB[kq * G + wid];
+            const int src = s.neib[neB * Nt + o];
+            a[k3[q] * Nt + g] = a[nk * Nt + src];
+            if (QF == 27) {
+                a[k3[q + 4] * Nt + g] = a[nkz[2 * (nk - 7) + 1] * Nt + src];
+                a[k3[q + 8] * Nt + g] = a[nkz[2 * (nk - 7)] * Nt + src];
+            }
+        }
+    } else if (P.flux_mode == FLUX_MODE_ONE_AXIS) {
+#pragma unroll
+        for (int k = 1; k < QF; ++k) {
+            const int ax = P.flux_axis;
+            if (comp<QF>(k, ax) == 0) continue;  // never read by set_tmp
+            a[k * Nt + g] = a[mirror<QF>(k, ax) * Nt + o];
+        }
+    }
+}
+
+// set_tmp (adv.cu:234-244)
+template <typename T, int QN, int QF>
+__global__ void k_set_tmp(PlainView<T> s, Pops<T> b, Params P) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    const int64_t Nt = s.Ntot;
+#pragma unroll
+    for (int k = 0; k < QF; ++k) {
+        int d = 0;
+        if (P.flux_mode == FLUX_MODE_ONE_AXIS) {
+            const int sg = comp<QF>(k, P.flux_axis);
+            if (sg) d = axis_dir<QN>(P.flux_axis, sg);
+        } else if (P.flux_mode == FLUX_MODE_CORNER) {
+            d = c_cp[k];
+        }
+        const int n = d ? s.neib[d * Nt + i] : (int)i;
+        b.ftmp[k * Nt + i] = b.f[k * Nt + n] + (b.fin[k * Nt + i] - b.fout[k * Nt + n]);
+    }
+}
+
+// set_wall_propagation (set_bound.cu:149-168)
+template <typename T, int QF>
+__global__ void k_set_wall_propagation(PlainView<T> s, Pops<T> b, Params P) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + s.N;
+    if (g >= s.Ntot) return;
+    const int64_t Nt = s.Ntot, G = s.Ntot - s.N, wid = g - s.N;
+    const int o = s.neib[g];
+#pragma unroll
+    for (int k = 1; k < QF; ++k) {
+        if (P.wall_bc == IPL_BC_BOUNCE_BACK) {
+            b.ftmp[k * Nt + g] = b.ftmp[opposite<QF>(k) * Nt + o];
+        } else {
+            const int src = s.neib[b.nextB[k * G + wid] * Nt + o];
+            b.ftmp[k * Nt + g] = b.ftmp[b.nextK[k * G + wid] * Nt + src];
+        }
+    }
+}
+
+// propagation (propagation.cu:2-12)
+template <typename T, int QF>
+__global__ void k_propagation(PlainView<T> s, Pops<T> b) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    const int64_t Nt = s.Ntot;
+#pragma unroll
+    for (int k = 0; k < QF; ++k) b.f[k * Nt + i] = b.ftmp[k * Nt + s.neib[opposite<QF>(k) * Nt + i]];
+}
+
+// reset_wall_fin (basic.cu:274-284)
+template <typename T, int QF>
+__global__ void k_reset_wall_fin(PlainView<T> s, Pops<T> b) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + s.N;
+    if (g >= s.Ntot) return;
+#pragma unroll
+    for (int k = 0; k < QF; ++k) { b.fin[k * s.Ntot + g] = 0; b.f[k * s.Ntot + g] = 0; }
+}
+
+template <typename T, int QN, int QF>
+void ip_process(const PlainView<T>& s, const Pops<T>& b, const Params& P, hipStream_t st) {
+    const int64_t G = s.Ntot - s.N;
+    const unsigned bn = blocks_for(s.N), bg = blocks_for(G > 0 ? G : 1);
+    if (G > 0) k_set_wall_f<T, QF><<<bg, kBlock, 0, st>>>(s, b);
+    k_set_out<T, QN, QF><<<bn, kBlock, 0, st>>>(s, b, P);
+    if (G > 0 && P.flux_mode == FLUX_MODE_CORNER) k_set_fin<T, QF><<<bg, kBlock, 0, st>>>(s, b);
+    k_reset_f<T, QF><<<bn, kBlock, 0, st>>>(s, b);
+    if (G > 0) {
+        k_set_wall_f_inv<T, QN, QF><<<bg, kBlock, 0, st>>>(s, b, b.f, P);
+        k_set_wall_f_inv<T, QN, QF><<<bg, kBlock, 0, st>>>(s, b, b.fout, P);
+    }
+    k_set_tmp<T, QN, QF><<<bn, kBlock, 0, st>>>(s, b, P);
+    if (G > 0) k_set_wall_propagation<T, QF><<<bg, kBlock, 0, st>>>(s, b, P);
+    k_propagation<T, QF><<<bn, kBlock, 0, st>>>(s, b);
+    if (G > 0) k_reset_wall_fin<T, QF><<<bg, kBlock, 0, st>>>(s, b);
+}
+
+template <typename T, int Q, int QS>
+int run_stage(const PlainView<T>& s, const Params& P, int stage, hipStream_t st) {
+    const unsigned bn = blocks_for(s.N);
+    const int64_t G = s.Ntot - s.N;
+    const unsigned bg = blocks_for(G > 0 ? G : 1);
+    switch (stage) {
+    case IPL_STAGE_EQU_F: k_equ_f<T, Q><<<bn, kBlock, 0, st>>>(s, P); break;
+    case IPL_STAGE_FORCE: k_force<T, Q><<<bn, kBlock, 0, st>>>(s, P); break;
+    case IPL_STAGE_COLLIDE: k_collide<T, Q><<<bn, kBlock, 0, st>>>(s, P); break;
+    case IPL_STAGE_IP_F: {
+        Pops<T> b{s.f, s.ftmp, s.fout, s.feq, s.nextB, s.nextK};
+        ip_process<T, Q, Q>(s, b, P, st);
+        break;
+    }
+    case IPL_STAGE_SCALAR_COLLIDE:
+        if constexpr (QS > 0) {
+            if (P.scalar == IPL_SCALAR_SALINITY) k_col_g_reg<T, QS><<<bn, kBlock, 0, st>>>(s, P);
+            else if constexpr (QS == Q) k_col_pf<T, Q><<<bn, kBlock, 0, st>>>(s, P);
+        }
+        break;
+    case IPL_STAGE_IP_G:
+        if constexpr (QS > 0) {
+            Pops<T> b{s.g, s.gtmp, s.gout, s.geq, s.snextB, s.snextK};
+            ip_process<T, Q, QS>(s, b, P, st);
+        }
+        break;
+    case IPL_STAGE_UPDATE_SCALAR:
+        if constexpr (QS > 0)
+            k_update_scalar<T, QS><<<bn, kBlock, 0, st>>>(s, P.scalar == IPL_SCALAR_SALINITY ? s.S : s.phi);
+        break;
+    case IPL_STAGE_UPDATE_RHO: k_update_rho<T, Q><<<bn, kBlock, 0, st>>>(s, P); break;
+    case IPL_STAGE_SET_WALL_RHO:
+        if (G > 0) {
+            k_set_wall_field<T><<<bg, kBlock, 0, st>>>(s, s.rho);
+            if (P.scalar == IPL_SCALAR_PHASE_FIELD) k_set_wall_field<T><<<bg, kBlock, 0, st>>>(s, s.phi);
+        }
+        break;
+    case IPL_STAGE_RESET_F: k_reset_force<T><<<bn, kBlock, 0, st>>>(s); break;
+    default: return IPL_ERR_ARGUMENT;
+    }
+    return IPL_OK;
+}
+
+}  // namespace
+
+template <typename T>
+int plain_stage(const PlainView<T>& s, const Params& P, int Q, int Qs, int stage, hipStream_t st) {
+#define IPL_CASE(QQ, QSS) if (Q == QQ && Qs == QSS) return run_stage<T, QQ, QSS>(s, P, stage, st);
+    IPL_CASE(9, 0) IPL_CASE(9, 9) IPL_CASE(19, 0) IPL_CASE(19, 19) IPL_CASE(19, 7)
+    IPL_CASE(27, 0) IPL_CASE(27, 27) IPL_CASE(27, 7)
+#undef IPL_CASE
+    return IPL_ERR_UNSUPPORTED;
+}
+
+template int plain_stage<double>(const PlainView<double>&, const Params&, int, int, int, hipStream_t);
+template int plain_stage<float>(const PlainView<float>&, const Params&, int, int, int, hipStream_t);
+
+}  // namespace ipl

@@ -1,0 +1,1100 @@
+// oracle.cpp - FP64 CPU restatement of the reference's lattice Boltzmann step.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle for the HIP library in
+// ip_cu_lbm_amd/: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+// it, and only as the checker (or as the timed CPU baseline).  The product never calls it.
+//
+// It restates, in the reference's own data model (array-of-structures populations f[i*Q+k],
+// an int32 neighbour table neib[i*Q+k], one ghost cell appended per boundary cell), the
+// kernels of function/{collision,adv,propagation,set_bound,basic}.cu and the step order of
+// the case drivers.  Every device kernel becomes one OpenMP loop with the same per-thread
+// body; citations are path:line into the reference.
+//
+// Deviations from the reference, all deliberate and shared with the GPU library:
+//  - true FP64 everywhere: the reference's <double> instantiation rounds through float in
+//    fminf/fmaxf/powf/sqrtf and a 'float tmp' (adv.cu:17-38, collision.cu:175,213-215);
+//  - builder extensions the reference lacks: periodic axes (neighbour wrap + position unwrap
+//    in the stencil), a single flux axis along y or z (the reference streams z exactly and
+//    advects only x/y), D3Q19 and D3Q7 lattices, a 3-point stencil, gravity / beta_s as
+//    parameters instead of the literals 9.81 / 0.824493.
+//
+// Parity status per function is listed at the end of this file (PARITY block).
+
+#include "../include/ipl.h"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+// ------------------------------------------------------------------------------------------
+// Lattice constants: Items::setc (basic.cu:70-98).  D2Q9 lives in the x-z plane.  D3Q19 and
+// D3Q7 are prefixes of the reference's D3Q27 ordering (rest, +-x/+-y/+-z, xy, xz, yz edges).
+// ------------------------------------------------------------------------------------------
+struct Lattice {
+    int Q = 0;
+    int dim = 0;
+    int cx[27] = {0}, cy[27] = {0}, cz[27] = {0};
+    double w[27] = {0};
+    int kinv[27] = {0};
+};
+
+Lattice make_lattice(int Q) {
+    Lattice L;
+    L.Q = Q;
+    if (Q == 9) {
+        L.dim = 2;
+        const int x[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+        const int z[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+        for (int k = 0; k < 9; ++k) { L.cx[k] = x[k]; L.cz[k] = z[k]; }
+        L.w[0] = 4.0 / 9.0;
+        for (int k = 1; k < 5; ++k) { L.w[k] = 1.0 / 9.0; L.w[k + 4] = 1.0 / 36.0; }
+    } else {
+        L.dim = 3;
+        // basic.cu:89-94
+        const int x[27] = {0, 1, 0, -1, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1, 0, 0, 0, 0, 1, 1, 1, 1, -1, -1, -1, -1};
+        const int y[27] = {0, 0, 1, 0, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1};
+        const int z[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1};
+        for (int k = 0; k < Q; ++k) { L.cx[k] = x[k]; L.cy[k] = y[k]; L.cz[k] = z[k]; }
+        if (Q == 27) {
+            L.w[0] = 8.0 / 27.0;
+            for (int k = 1; k < 7; ++k) L.w[k] = 2.0 / 27.0;
+            for (int k = 7; k < 19; ++k) L.w[k] = 1.0 / 54.0;
+            for (int k = 19; k < 27; ++k) L.w[k] = 1.0 / 216.0;
+        } else if (Q == 19) {
+            L.w[0] = 1.0 / 3.0;
+            for (int k = 1; k < 7; ++k) L.w[k] = 1.0 / 18.0;
+            for (int k = 7; k < 19; ++k) L.w[k] = 1.0 / 36.0;
+        } else if (Q == 7) {
+            L.w[0] = 1.0 / 4.0;
+            for (int k = 1; k < 7; ++k) L.w[k] = 1.0 / 8.0;
+        }
+    }
+    // set_k_inv (basic.cu:112-121): opposite direction
+    for (int k = 0; k < Q; ++k)
+        for (int j = 0; j < Q; ++j)
+            if (L.cx[j] == -L.cx[k] && L.cy[j] == -L.cy[k] && L.cz[j] == -L.cz[k]) L.kinv[k] = j;
+    return L;
+}
+
+// ------------------------------------------------------------------------------------------
+// MRT moment matrices.  Rows are the moment polynomials of the reference's MRT_D3Q9.csv and
+// MRT_D3Q27.csv (checked against those files by tests/test_oracle.py); D3Q19 uses the
+// d'Humieres et al. (2002) basis in the same direction ordering.  M^-1 = M^T diag(1/|row|^2)
+// because the rows are orthogonal.  set_M (basic.cu:171-201) then forms
+// MM = M^-1 diag(S) M with the stress rates zeroed; the stress rows use 1/tau per cell.
+// ------------------------------------------------------------------------------------------
+void mrt_matrix(const Lattice& L, std::vector<double>& M, std::vector<int>& stress) {
+    const int Q = L.Q;
+    M.assign(Q * Q, 0.0);
+    stress.clear();
+    for (int k = 0; k < Q; ++k) {
+        const double x = L.cx[k], y = L.cy[k], z = L.cz[k];
+        const double c2 = x * x + y * y + z * z;
+        double r[27];
+        if (Q == 9) {
+            r[0] = 1; r[1] = -4 + 3 * c2; r[2] = 4 - 10.5 * c2 + 4.5 * c2 * c2;
+            r[3] = x; r[4] = (-5 + 3 * c2) * x; r[5] = z; r[6] = (-5 + 3 * c2) * z;
+            r[7] = x * x - z * z; r[8] = x * z;
+        } else if (Q == 27) {
+            r[0] = 1; r[1] = x; r[2] = y; r[3] = z; r[4] = c2 - 2;
+            r[5] = 3 * x * x - c2; r[6] = y * y - z * z; r[7] = x * y; r[8] = y * z; r[9] = x * z;
+            r[10] = x * (3 * c2 - 7); r[11] = y * (3 * c2 - 7); r[12] = z * (3 * c2 - 7);
+            const double p4 = (9 * c2 * c2 - 39 * c2 + 38) / 2;
+            r[13] = x * p4; r[14] = y * p4; r[15] = z * p4;
+            r[16] = (3 * c2 * c2 - 11 * c2 + 8) / 2;
+            r[17] = (9 * c2 * c2 * c2 - 45 * c2 * c2 + 60 * c2 - 16) / 2;
+            r[18] = (3 * x * x - c2) * (3 * c2 - 5); r[19] = (y * y - z * z) * (3 * c2 - 5);
+            r[20] = x * y * (3 * c2 - 8); r[21] = y * z * (3 * c2 - 8); r[22] = x * z * (3 * c2 - 8);
+            r[23] = x * (y * y - z * z); r[24] = y * (z * z - x * x); r[25] = z * (x * x - y * y);
+            r[26] = x * y * z;
+        } else {  // 19
+            r[0] = 1; r[1] = 19 * c2 - 30; r[2] = (21 * c2 * c2 - 53 * c2 + 24) / 2;
+            r[3] = x; r[4] = (5 * c2 - 9) * x; r[5] = y; r[6] = (5 * c2 - 9) * y;
+            r[7] = z; r[8] = (5 * c2 - 9) * z;
+            r[9] = 3 * x * x - c2; r[10] = (3 * c2 - 5) * (3 * x * x - c2);
+            r[11] = y * y - z * z; r[12] = (3 * c2 - 5) * (y * y - z * z);
+            r[13] = x * y; r[14] = y * z; r[15] = x * z;
+            r[16] = x * (y * y - z * z); r[17] = y * (z * z - x * x); r[18] = z * (x * x - y * y);
+        }
+        for (int i = 0; i < Q; ++i) M[i * Q + k] = r[i];
+    }
+    if (Q == 9) stress = {7, 8};
+    else if (Q == 27) stress = {5, 6, 7, 8, 9};
+    else stress = {9, 11, 13, 14, 15};
+}
+
+void mrt_inverse(int Q, const std::vector<double>& M, std::vector<double>& Minv) {
+    Minv.assign(Q * Q, 0.0);
+    for (int i = 0; i < Q; ++i) {
+        double n2 = 0;
+        for (int k = 0; k < Q; ++k) n2 += M[i * Q + k] * M[i * Q + k];
+        for (int k = 0; k < Q; ++k) Minv[k * Q + i] = M[i * Q + k] / n2;
+    }
+}
+
+// Lagrange_interpolation (adv.cu:40-48): same product order.
+void lagrange(double* coeff, const double* x, double xa, int n) {
+    for (int i = 0; i < n; ++i) {
+        coeff[i] = 1.0;
+        for (int j = 0; j < i; ++j) coeff[i] *= (xa - x[j]) / (x[i] - x[j]);
+        for (int j = i + 1; j < n; ++j) coeff[i] *= (xa - x[j]) / (x[i] - x[j]);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Reference direction tables for the 3D two-axis (x,y) corner scheme: set_cpm, set_cxypm
+// (basic.cu:122-153).  Entries the reference leaves unset are 0 (= the cell itself).
+// ------------------------------------------------------------------------------------------
+struct RefTables {
+    int cp[27] = {0}, cm[27] = {0}, cip[27] = {0}, cim[27] = {0};
+    int cxp[27] = {0}, cxm[27] = {0}, cyp[27] = {0}, cym[27] = {0};
+};
+
+RefTables ref_tables_3d() {
+    RefTables t;
+    const int cp[27] = {0, 1, 2, 3, 4, 0, 0, 7, 8, 9, 10, 1, 1, 3, 3, 2, 2, 4, 4, 7, 7, 8, 8, 9, 9, 10, 10};
+    const int cm[27] = {0, 3, 4, 1, 2, 0, 0, 10, 9, 8, 7, 3, 3, 1, 1, 4, 4, 2, 2, 10, 10, 9, 9, 8, 8, 7, 7};
+    for (int k = 0; k < 27; ++k) { t.cp[k] = cp[k]; t.cm[k] = cm[k]; }
+    const int cip[8] = {8, 8, 7, 7, 10, 10, 9, 9}, cim[8] = {9, 9, 10, 10, 7, 7, 8, 8};
+    for (int k = 0; k < 8; ++k) { t.cip[19 + k] = cip[k]; t.cim[19 + k] = cim[k]; }
+    const int cxp[27] = {0, 1, 0, 3, 0, 0, 0, 1, 1, 3, 3, 1, 1, 3, 3, 0, 0, 0, 0, 1, 1, 1, 1, 3, 3, 3, 3};
+    const int cxm[27] = {0, 3, 0, 1, 0, 0, 0, 3, 3, 1, 1, 3, 3, 1, 1, 0, 0, 0, 0, 3, 3, 3, 3, 1, 1, 1, 1};
+    const int cyp[27] = {0, 0, 2, 0, 4, 0, 0, 2, 4, 2, 4, 0, 0, 0, 0, 2, 2, 4, 4, 2, 2, 4, 4, 2, 2, 4, 4};
+    const int cym[27] = {0, 0, 4, 0, 2, 0, 0, 4, 2, 4, 2, 0, 0, 0, 0, 4, 4, 2, 2, 4, 4, 2, 2, 4, 4, 2, 2};
+    for (int k = 0; k < 27; ++k) { t.cxp[k] = cxp[k]; t.cxm[k] = cxm[k]; t.cyp[k] = cyp[k]; t.cym[k] = cym[k]; }
+    return t;
+}
+
+// Population set of one transported field with the reference's buffer aliasing inside
+// IP_process (adv.cu:246-256): fin lives in 'feq', finy in 'f', post-collision in 'ftmp'.
+struct PopSet {
+    int Q = 0;
+    Lattice L;
+    std::vector<double> f, ftmp, fout, feq, Fk;
+    std::vector<int> nextB, nextK;  // G*Q, set_bound2D/3D
+};
+
+enum FluxMode { FLUX_NONE = 0, FLUX_ONE_AXIS = 1, FLUX_CORNER = 2 };
+
+struct Oracle {
+    ipl_config cfg{};
+    Lattice L;        // flow lattice
+    int Q = 0, dim = 0;
+    int N = 0, G = 0, Ntot = 0;
+    double dz = 1, dt = 1, c = 1;
+    FluxMode flux_mode = FLUX_NONE;
+    int flux_axis = 0;      // 0,1,2 for FLUX_ONE_AXIS
+    int npts = 5;
+    double period[3] = {0, 0, 0};
+    RefTables rt;
+
+    std::vector<int> neib;                       // Ntot*Q
+    std::vector<double> pos[3], del[3];          // Ntot
+    std::vector<double> p, u, v, w, rho, phi, S, tau, taus, Fx, Fy, Fz;
+    std::vector<double> phiold, uold, vold, wold;
+    std::vector<double> M, Minv, MM, Sr;
+    std::vector<int> stress;
+
+    PopSet fl, sc;   // flow and scalar populations
+    bool has_scalar = false;
+    std::string err;
+};
+
+inline int axis_dir(const Lattice& L, int axis, int sgn) {
+    for (int k = 1; k < L.Q; ++k) {
+        const int c[3] = {L.cx[k], L.cy[k], L.cz[k]};
+        if (c[axis] == sgn && c[(axis + 1) % 3] == 0 && c[(axis + 2) % 3] == 0) return k;
+    }
+    return 0;
+}
+inline int comp(const Lattice& L, int k, int axis) {
+    return axis == 0 ? L.cx[k] : (axis == 1 ? L.cy[k] : L.cz[k]);
+}
+
+// ------------------------------------------------------------------------------------------
+// set_bound2D (set_bound.cu:25-58) and set_bound3D (set_bound.cu:60-147): slip remap tables
+// per ghost and direction.  Restated with the same tables and the same tests.
+// ------------------------------------------------------------------------------------------
+void set_bound2D(Oracle& o, PopSet& ps) {
+    const int Q = ps.Q, nb = o.N, QN = o.Q;
+    int nei[2][2][9] = {}, kinv[2][2][9] = {};
+    const Lattice& L = ps.L;
+    for (int k = 0; k < 9; ++k) {
+        nei[std::abs(L.cx[k])][std::abs(L.cz[k])][k] = 0;
+        kinv[std::abs(L.cx[k])][std::abs(L.cz[k])][k] = L.kinv[k];
+        kinv[0][0][k] = L.kinv[k];
+    }
+    nei[1][0][5] = 3; nei[0][1][5] = 4; nei[1][0][6] = 1; nei[0][1][6] = 4;
+    nei[1][0][7] = 1; nei[0][1][7] = 2; nei[1][0][8] = 3; nei[0][1][8] = 2;
+    kinv[1][0][5] = 8; kinv[0][1][5] = 6; kinv[1][0][6] = 7; kinv[0][1][6] = 5;
+    kinv[1][0][7] = 6; kinv[0][1][7] = 8; kinv[1][0][8] = 5; kinv[0][1][8] = 7;
+    ps.nextB.assign((size_t)o.G * Q, 0);
+    ps.nextK.assign((size_t)o.G * Q, 0);
+    for (int i = 0; i < o.G; ++i) {
+        const size_t cid = (size_t)o.neib[(size_t)(nb + i) * QN] * QN;
+        for (int k = 1; k < 9; ++k) {
+            int cxx = L.cx[k], czz = L.cz[k];
+            if (o.neib[cid + L.kinv[k]] >= nb) {
+                if ((o.neib[cid + 1] >= nb && cxx == -1) || (o.neib[cid + 3] >= nb && cxx == 1)) cxx = 0;
+                if ((o.neib[cid + 2] >= nb && czz == -1) || (o.neib[cid + 4] >= nb && czz == 1)) czz = 0;
+            }
+            ps.nextB[(size_t)i * Q + k] = nei[std::abs(cxx)][std::abs(czz)][k];
+            ps.nextK[(size_t)i * Q + k] = kinv[std::abs(cxx)][std::abs(czz)][k];
+        }
+    }
+}
+
+void set_bound3D(Oracle& o, PopSet& ps) {
+    const int Q = ps.Q, nb = o.N, QN = o.Q;
+    const Lattice& L = ps.L;
+    static int nei[2][2][2][27], kinv[2][2][2][27];
+    std::memset(nei, 0, sizeof(nei));
+    std::memset(kinv, 0, sizeof(kinv));
+    const Lattice L27 = make_lattice(27);
+    for (int k = 0; k < 27; ++k) {
+        nei[std::abs(L27.cx[k])][std::abs(L27.cy[k])][std::abs(L27.cz[k])][k] = 0;
+        kinv[std::abs(L27.cx[k])][std::abs(L27.cy[k])][std::abs(L27.cz[k])][k] = L27.kinv[k];
+        kinv[0][0][0][k] = L27.kinv[k];
+    }
+    // set_bound.cu:130-143
+    nei[1][0][0][7] = 3; nei[0][1][0][7] = 4; nei[1][0][0][8] = 3; nei[0][1][0][8] = 2;
+    nei[1][0][0][9] = 1; nei[0][1][0][9] = 4; nei[1][0][0][10] = 1; nei[0][1][0][10] = 2;
+    nei[1][0][0][11] = 3; nei[0][0][1][11] = 6; nei[1][0][0][12] = 3; nei[0][0][1][12] = 5;
+    nei[1][0][0][13] = 1; nei[0][0][1][13] = 6; nei[1][0][0][14] = 1; nei[0][0][1][14] = 5;
+    nei[0][1][0][15] = 4; nei[0][0][1][15] = 6; nei[0][1][0][16] = 4; nei[0][0][1][16] = 5;
+    nei[0][1][0][17] = 2; nei[0][0][1][17] = 6; nei[0][1][0][18] = 2; nei[0][0][1][18] = 5;
+    {
+        const int t[8][6] = {{3, 4, 6, 10, 14, 18}, {3, 4, 5, 10, 13, 17}, {3, 2, 6, 9, 14, 16}, {3, 2, 5, 9, 13, 15},
+                             {1, 4, 6, 8, 12, 18},  {1, 4, 5, 8, 11, 17},  {1, 2, 6, 7, 12, 16}, {1, 2, 5, 7, 11, 15}};
+        for (int q = 0; q < 8; ++q) {
+            const int k = 19 + q;
+            nei[1][0][0][k] = t[q][0]; nei[0][1][0][k] = t[q][1]; nei[0][0][1][k] = t[q][2];
+            nei[1][1][0][k] = t[q][3]; nei[1][0][1][k] = t[q][4]; nei[0][1][1][k] = t[q][5];
+        }
+    }
+    // set_bound.cu:145-158
+    kinv[1][0][0][7] = 8; kinv[0][1][0][7] = 9; kinv[1][0][0][8] = 7; kinv[0][1][0][8] = 10;
+    kinv[1][0][0][9] = 10; kinv[0][1][0][9] = 7; kinv[1][0][0][10] = 9; kinv[0][1][0][10] = 8;
+    kinv[1][0][0][11] = 12; kinv[0][0][1][11] = 13; kinv[1][0][0][12] = 11; kinv[0][0][1][12] = 14;
+    kinv[1][0][0][13] = 14; kinv[0][0][1][13] = 11; kinv[1][0][0][14] = 13; kinv[0][0][1][14] = 12;
+    kinv[0][1][0][15] = 16; kinv[0][0][1][15] = 17; kinv[0][1][0][16] = 15; kinv[0][0][1][16] = 18;
+    kinv[0][1][0][17] = 18; kinv[0][0][1][17] = 15; kinv[0][1][0][18] = 17; kinv[0][0][1][18] = 16;
+    {
+        const int t[8][6] = {{22, 24, 25, 20, 21, 23}, {21, 23, 26, 19, 22, 24}, {20, 26, 23, 22, 19, 25},
+                             {19, 25, 24, 21, 20, 26}, {26, 20, 21, 24, 25, 19}, {25, 19, 22, 23, 26, 20},
+                             {24, 22, 19, 26, 23, 21}, {23, 21, 20, 25, 24, 22}};
+        for (int q = 0; q < 8; ++q) {
+            const int k = 19 + q;
+            kinv[1][0][0][k] = t[q][0]; kinv[0][1][0][k] = t[q][1]; kinv[0][0][1][k] = t[q][2];
+            kinv[1][1][0][k] = t[q][3]; kinv[1][0][1][k] = t[q][4]; kinv[0][1][1][k] = t[q][5];
+        }
+    }
+    ps.nextB.assign((size_t)o.G * Q, 0);
+    ps.nextK.assign((size_t)o.G * Q, 0);
+    for (int i = 0; i < o.G; ++i) {
+        const size_t cid = (size_t)o.neib[(size_t)(nb + i) * QN] * QN;
+        auto nbv = [&](int d) { return d < QN ? o.neib[cid + d] : -1; };
+        for (int k = 1; k < Q; ++k) {
+            const int cxk = L.cx[k], cyk = L.cy[k], czk = L.cz[k];
+            int cxx = cxk, cyy = cyk, czz = czk;
+            if (o.neib[cid + L.kinv[k]] >= nb) {
+                if ((nbv(1) >= nb && cxx == -1) || (nbv(3) >= nb && cxx == 1)) cxx = 0;
+                if ((nbv(2) >= nb && cyy == -1) || (nbv(4) >= nb && cyy == 1)) cyy = 0;
+                if ((nbv(5) >= nb && czz == -1) || (nbv(6) >= nb && czz == 1)) czz = 0;
+                if (QN > 18) {  // edge tests read the xz / yz neighbours (set_bound.cu:477-501)
+                    if ((nbv(11) < nb && cxk == -1 && czk == -1) || (nbv(14) < nb && cxk == 1 && czk == 1)) { cxx = cxk; czz = czk; }
+                    if ((nbv(12) < nb && cxk == -1 && czk == 1) || (nbv(13) < nb && cxk == 1 && czk == -1)) { cxx = cxk; czz = czk; }
+                    if ((nbv(15) < nb && cyk == -1 && czk == -1) || (nbv(18) < nb && cyk == 1 && czk == 1)) { cyy = cyk; czz = czk; }
+                    if ((nbv(16) < nb && cyk == -1 && czk == 1) || (nbv(17) < nb && cyk == 1 && czk == -1)) { cyy = cyk; czz = czk; }
+                    if ((nbv(11) >= nb && cxx == -1 && czz == -1) || (nbv(14) >= nb && cxx == 1 && czz == 1)) { cxx = 0; czz = 0; }
+                    if ((nbv(12) >= nb && cxx == -1 && czz == 1) || (nbv(13) >= nb && cxx == 1 && czz == -1)) { cxx = 0; czz = 0; }
+                    if ((nbv(15) >= nb && cyy == -1 && czz == -1) || (nbv(18) >= nb && cyy == 1 && czz == 1)) { cyy = 0; czz = 0; }
+                    if ((nbv(16) >= nb && cyy == -1 && czz == 1) || (nbv(17) >= nb && cyy == 1 && czz == -1)) { cyy = 0; czz = 0; }
+                }
+            }
+            ps.nextB[(size_t)i * Q + k] = nei[std::abs(cxx)][std::abs(cyy)][std::abs(czz)][k];
+            ps.nextK[(size_t)i * Q + k] = kinv[std::abs(cxx)][std::abs(cyy)][std::abs(czz)][k];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Grid and neighbour build (LockExchange.cu:57-127) with set_neibghor_wall (basic.cu:203-237).
+// ------------------------------------------------------------------------------------------
+bool build(Oracle& o, const ipl_config& cfg, const ipl_init* init) {
+    o.cfg = cfg;
+    o.Q = cfg.lattice;
+    if (o.Q != 9 && o.Q != 19 && o.Q != 27) { o.err = "lattice must be 9, 19 or 27"; return false; }
+    o.L = make_lattice(o.Q);
+    o.dim = o.L.dim;
+    const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
+    if (nx < 1 || ny < 1 || nz < 1 || (o.dim == 2 && ny != 1)) { o.err = "bad grid size"; return false; }
+    o.dz = cfg.dz; o.dt = cfg.dt; o.c = cfg.dz / cfg.dt;
+    const int n[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a) {
+        if (!cfg.pos[a] || !cfg.width[a]) { o.err = "missing axis geometry"; return false; }
+        if (cfg.periodic[a]) {
+            double L = 0;
+            for (int i = 0; i < n[a]; ++i) L += cfg.width[a][i];
+            o.period[a] = L;
+        }
+    }
+    // flux mode
+    int fa = cfg.flux_axes;
+    if (fa == IPL_FLUX_REFERENCE) fa = (o.dim == 2) ? IPL_FLUX_X : (IPL_FLUX_X | IPL_FLUX_Y);
+    if (fa == 0) o.flux_mode = FLUX_NONE;
+    else if (fa == IPL_FLUX_X || fa == IPL_FLUX_Y || fa == IPL_FLUX_Z) {
+        o.flux_mode = FLUX_ONE_AXIS;
+        o.flux_axis = fa == IPL_FLUX_X ? 0 : (fa == IPL_FLUX_Y ? 1 : 2);
+        if (o.dim == 2 && o.flux_axis == 1) { o.err = "no y flux axis in 2D"; return false; }
+    } else if (fa == (IPL_FLUX_X | IPL_FLUX_Y) && o.dim == 3) {
+        o.flux_mode = FLUX_CORNER;
+        o.rt = ref_tables_3d();
+    } else { o.err = "unsupported flux axes"; return false; }
+    o.npts = cfg.stencil == 3 ? 3 : 5;
+    // width >= dz on flux axes (the slab point must lie inside the cell, adv.cu:77,85-86)
+    for (int a = 0; a < 3; ++a) {
+        const bool isflux = (o.flux_mode == FLUX_ONE_AXIS && o.flux_axis == a) ||
+                            (o.flux_mode == FLUX_CORNER && a < 2);
+        if (!isflux) continue;
+        for (int i = 0; i < n[a]; ++i)
+            if (cfg.width[a][i] < o.dz * (1 - 1e-12)) { o.err = "cell width below lattice spacing on a flux axis"; return false; }
+    }
+
+    // lnum
+    const size_t ncell = (size_t)nx * ny * nz;
+    std::vector<int> lnum(ncell, -1);
+    int N = 0;
+    for (size_t q = 0; q < ncell; ++q)
+        if (!(cfg.solid && cfg.solid[q])) lnum[q] = N++;
+    o.N = N;
+    const int Q = o.Q;
+    std::vector<int> ci(N), cj(N), cl(N);
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int l = 0; l < nz; ++l) {
+                const int id = lnum[((size_t)i * ny + j) * nz + l];
+                if (id >= 0) { ci[id] = i; cj[id] = j; cl[id] = l; }
+            }
+    // neib (LockExchange.cu:117-127), with periodic wrap as the builder extension
+    o.neib.assign((size_t)N * Q, -1);
+    for (int id = 0; id < N; ++id)
+        for (int k = 0; k < Q; ++k) {
+            int t[3] = {ci[id] + o.L.cx[k], cj[id] + o.L.cy[k], cl[id] + o.L.cz[k]};
+            bool ok = true;
+            for (int a = 0; a < 3; ++a) {
+                if (t[a] < 0 || t[a] >= n[a]) {
+                    if (cfg.periodic[a]) t[a] = (t[a] + n[a]) % n[a];
+                    else ok = false;
+                }
+            }
+            o.neib[(size_t)id * Q + k] = ok ? lnum[((size_t)t[0] * ny + t[1]) * nz + t[2]] : -1;
+        }
+    // geometry and fields of calc cells
+    for (int a = 0; a < 3; ++a) { o.pos[a].resize(N); o.del[a].resize(N); }
+    for (int id = 0; id < N; ++id) {
+        const int ix[3] = {ci[id], cj[id], cl[id]};
+        for (int a = 0; a < 3; ++a) { o.pos[a][id] = cfg.pos[a][ix[a]]; o.del[a][id] = cfg.width[a][ix[a]]; }
+    }
+    auto take = [&](const double* src, double dflt) {
+        std::vector<double> v(N);
+        for (int id = 0; id < N; ++id) v[id] = src ? src[id] : dflt;
+        return v;
+    };
+    o.p = take(init ? init->p : nullptr, 0.0);
+    o.u = take(init ? init->u : nullptr, 0.0);
+    o.v = take(init ? init->v : nullptr, 0.0);
+    o.w = take(init ? init->w : nullptr, 0.0);
+    o.S = take(init ? init->S : nullptr, 0.0);
+    o.phi = take(init ? init->phi : nullptr, 0.0);
+    o.tau = take(init ? init->tau : nullptr, cfg.tau);
+    o.taus = take(init ? init->taus : nullptr, cfg.taus);
+    o.rho.resize(N);
+    for (int id = 0; id < N; ++id)  // LockExchange.cu:93
+        o.rho[id] = cfg.rho_L + o.phi[id] * (cfg.rho_H - cfg.rho_L) + o.S[id] * cfg.beta_s;
+    o.Fx.assign(N, 0); o.Fy.assign(N, 0); o.Fz.assign(N, 0);
+    o.phiold = o.phi; o.uold = o.u; o.vold = o.v; o.wold = o.w;
+
+    // set_neibghor_wall (basic.cu:203-237): one ghost per calc cell with a missing link
+    int G = 0;
+    for (int id = 0; id < N; ++id) {
+        bool missing = false;
+        for (int k = 1; k < Q; ++k) if (o.neib[(size_t)id * Q + k] == -1) { missing = true; break; }
+        if (!missing) continue;
+        const int g = N + G;
+        o.neib.push_back(id);
+        for (int k = 1; k < Q; ++k) o.neib.push_back(g);
+        o.p.push_back(o.p[id]); o.rho.push_back(o.rho[id]); o.phi.push_back(o.phi[id]);
+        for (int a = 0; a < 3; ++a) { o.pos[a].push_back(o.pos[a][id]); o.del[a].push_back(o.del[a][id]); }
+        o.u.push_back(0); o.v.push_back(0); o.w.push_back(0);
+        for (int k = 1; k < Q; ++k)
+            if (o.neib[(size_t)id * Q + k] == -1) {
+                o.neib[(size_t)id * Q + k] = g;
+                o.neib[(size_t)g * Q + o.L.kinv[k]] = id;
+            }
+        ++G;
+    }
+    o.G = G;
+    o.Ntot = N + G;
+
+    // MRT matrices, set_M (basic.cu:171-201)
+    mrt_matrix(o.L, o.M, o.stress);
+    mrt_inverse(Q, o.M, o.Minv);
+    o.Sr.assign(Q, 0);
+    for (int i = 0; i < Q; ++i) o.Sr[i] = cfg.s_rates[i];
+    for (int s : o.stress) o.Sr[s] = 0;
+    o.MM.assign(Q * Q, 0);
+    for (int i = 0; i < Q; ++i)
+        for (int j = 0; j < Q; ++j) {
+            double acc = 0;
+            for (int k = 0; k < Q; ++k) acc += o.Minv[i * Q + k] * (o.Sr[k] * o.M[k * Q + j]);
+            o.MM[i * Q + j] = acc;
+        }
+
+    // populations: f = feq(p,u) (LockExchange.cu:133-139)
+    auto setup_pops = [&](PopSet& ps, int Qp) {
+        ps.Q = Qp;
+        ps.L = make_lattice(Qp);
+        const size_t sz = (size_t)o.Ntot * Qp;
+        ps.f.assign(sz, 0); ps.ftmp.assign(sz, 0); ps.fout.assign(sz, 0); ps.feq.assign(sz, 0); ps.Fk.assign(sz, 0);
+        if (o.dim == 2) set_bound2D(o, ps); else set_bound3D(o, ps);
+    };
+    setup_pops(o.fl, Q);
+    const double c2 = o.c * o.c;
+    for (int id = 0; id < N; ++id)
+        for (int k = 0; k < Q; ++k) {
+            const double t = (o.L.cx[k] * o.c * o.u[id] + o.L.cy[k] * o.c * o.v[id] + o.L.cz[k] * o.c * o.w[id]) / c2;
+            o.fl.f[(size_t)id * Q + k] = o.L.w[k] * (o.p[id] + 3.0 * t + 4.5 * t * t -
+                1.5 * (o.u[id] * o.u[id] + o.v[id] * o.v[id] + o.w[id] * o.w[id]) / c2);
+        }
+    o.has_scalar = cfg.scalar != IPL_SCALAR_NONE;
+    if (o.has_scalar) {
+        int Qs = cfg.scalar_lattice ? cfg.scalar_lattice : Q;
+        if (!(Qs == Q || (Qs == 7 && o.dim == 3))) { o.err = "scalar lattice must equal the flow lattice or be 7 in 3D"; return false; }
+        if (cfg.scalar == IPL_SCALAR_PHASE_FIELD && Qs != Q) { o.err = "phase field uses the flow lattice"; return false; }
+        if (o.flux_mode == FLUX_CORNER && Qs == 7) { o.err = "D3Q7 scalar needs a single flux axis"; return false; }
+        setup_pops(o.sc, Qs);
+        const Lattice& Ls = o.sc.L;
+        const bool linear = Qs == 7;
+        const double cs2inv = linear ? 4.0 : 3.0;
+        for (int id = 0; id < N; ++id) {
+            const double s = cfg.scalar == IPL_SCALAR_SALINITY ? o.S[id] : o.phi[id];
+            for (int k = 0; k < Qs; ++k) {
+                const double t = (Ls.cx[k] * o.c * o.u[id] + Ls.cy[k] * o.c * o.v[id] + Ls.cz[k] * o.c * o.w[id]) / c2;
+                double e = 1.0 + cs2inv * t;
+                if (!linear) e += 4.5 * t * t - 1.5 * (o.u[id] * o.u[id] + o.v[id] * o.v[id] + o.w[id] * o.w[id]) / c2;
+                o.sc.f[(size_t)id * Qs + k] = Ls.w[k] * s * e;
+            }
+        }
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------------------------------
+// Kernels of collision.cu, restated per cell.
+// ------------------------------------------------------------------------------------------
+
+// equ_f (collision.cu:3-14): feq of the flow
+void equ_f(Oracle& o) {
+    const int Q = o.Q; const double c2 = o.c * o.c;
+    #pragma omp parallel for schedule(static)
+    for (int id = 0; id < o.N; ++id)
+        for (int k = 0; k < Q; ++k) {
+            const double t = (o.L.cx[k] * o.c * o.u[id] + o.L.cy[k] * o.c * o.v[id] + o.L.cz[k] * o.c * o.w[id]) / c2;
+            o.fl.feq[(size_t)id * Q + k] = o.L.w[k] * (o.p[id] + 3.0 * t + 4.5 * t * t -
+                1.5 * (o.u[id] * o.u[id] + o.v[id] * o.v[id] + o.w[id] * o.w[id]) / c2);
+        }
+}
+
+// Force (collision.cu:57-112): surface tension + buoyancy + pressure-gradient term into
+// Fx,Fy,Fz (accumulating), then Fk = w dt (c.F) / (rho c^2 / 3).
+void force(Oracle& o) {
+    const int Q = o.Q; const double c2 = o.c * o.c;
+    const int up = o.dim == 3 ? 5 : 2, down = o.dim == 3 ? 6 : 4;
+    const double eps = o.dz / 1000;
+    const double beta = 12.0 * o.cfg.sigma / o.cfg.pf_thickness, kap = 3.0 * o.cfg.sigma * o.cfg.pf_thickness / 2.0;
+    const int lap2_axis = o.dim == 3 ? 1 : 2;   // axis of directions 2/4
+    #pragma omp parallel for schedule(static)
+    for (int id = 0; id < o.N; ++id) {
+        const int* nb = &o.neib[(size_t)id * Q];
+        const double* px = o.pos[0].data(); const double* py = o.pos[1].data(); const double* pz = o.pos[2].data();
+        const double* ph = o.phi.data(); const double* rh = o.rho.data();
+        double nx_ = (ph[nb[1]] - ph[nb[3]]) / (px[nb[1]] - px[nb[3]] + eps);
+        double ny_ = 0;
+        if (o.dim == 3) ny_ = (ph[nb[2]] - ph[nb[4]]) / (py[nb[2]] - py[nb[4]] + eps);
+        double nz_ = (ph[nb[up]] - ph[nb[down]]) / (pz[nb[up]] - pz[nb[down]] + eps);
+        // the reference divides the 2/4 term by dz^2 (collision.cu:70-71); the width of that
+        // axis equals dz on every reference grid
+        const double wd2 = o.del[lap2_axis][id];
+        const double lap = (ph[nb[1]] - 2 * ph[id] + ph[nb[3]]) / (o.del[0][id] * o.del[0][id]) +
+                           (ph[nb[2]] - 2 * ph[id] + ph[nb[4]]) / (wd2 * wd2);
+        const double st = 4 * beta * ph[id] * (ph[id] - 1.0) * (ph[id] - 0.5) - kap * lap;
+        o.Fx[id] += st * nx_; o.Fy[id] += st * ny_; o.Fz[id] += st * nz_;
+        if (o.cfg.boussinesq) o.Fz[id] += -o.S[id] * o.cfg.beta_s * o.cfg.gravity;
+        else o.Fz[id] += -o.rho[id] * o.cfg.gravity;
+        nx_ = (rh[nb[1]] - rh[nb[3]]) / (px[nb[1]] - px[nb[3]] + eps);
+        ny_ = 0;
+        if (o.dim == 3) ny_ = (rh[nb[2]] - rh[nb[4]]) / (py[nb[2]] - py[nb[4]] + eps);
+        nz_ = (rh[nb[up]] - rh[nb[down]]) / (pz[nb[up]] - pz[nb[down]] + eps);
+        o.Fx[id] += -o.p[id] * c2 / 3.0 * nx_;
+        o.Fy[id] += -o.p[id] * c2 / 3.0 * ny_;
+        o.Fz[id] += -o.p[id] * c2 / 3.0 * nz_;
+        for (int k = 0; k < Q; ++k)
+            o.fl.Fk[(size_t)id * Q + k] = o.L.w[k] * o.dt *
+                (o.L.cx[k] * o.c * o.Fx[id] + o.L.cy[k] * o.c * o.Fy[id] + o.L.cz[k] * o.c * o.Fz[id]) /
+                (o.rho[id] * c2 / 3.0);
+    }
+}
+
+// col_f_MRT (collision.cu:115-150) / col_f_SRT (collision.cu:151-165)
+void collide(Oracle& o) {
+    const int Q = o.Q;
+    PopSet& P = o.fl;
+    #pragma omp parallel for schedule(static)
+    for (int id = 0; id < o.N; ++id) {
+        double* f = &P.f[(size_t)id * Q]; double* feq = &P.feq[(size_t)id * Q];
+        double* ft = &P.ftmp[(size_t)id * Q]; const double* Fk = &P.Fk[(size_t)id * Q];
+        if (o.cfg.collision == IPL_COLL_SRT) {
+            for (int k = 0; k < Q; ++k) {
+                f[k] = f[k] - (f[k] - feq[k] + Fk[k] / 2.0) / o.tau[id] + Fk[k];
+                ft[k] = f[k];
+            }
+        } else {
+            for (int k = 0; k < Q; ++k) feq[k] = (f[k] - feq[k] + Fk[k] / 2.0);
+            double mles[27];
+            for (int i = 0; i < Q; ++i) {
+                double tmp = 0;
+                for (int k = 0; k < Q; ++k) {
+                    mles[k] = 0;
+                    for (int s : o.stress) mles[k] += o.Minv[i * Q + s] * o.M[s * Q + k] / o.tau[id];
+                }
+                for (int k = 0; k < Q; ++k) tmp += (o.MM[i * Q + k] + mles[k]) * feq[k];
+                f[i] = f[i] - tmp + Fk[i];
+                ft[i] = f[i];
+            }
+        }
+        for (int k = 0; k < Q; ++k) { feq[k] = 0; f[k] = 0; }
+    }
+}
+
+// col_g_reg (collision.cu:167-188): regularized scalar collision.  On D3Q7 the equilibrium
+// is linear and 1/cs^2 = 4 (builder extension); otherwise the reference's form.
+void col_g_reg(Oracle& o) {
+    PopSet& P = o.sc; const int Q = P.Q; const Lattice& L = P.L;
+    const double c2 = o.c * o.c;
+    const bool linear = Q == 7;
+    const double cs2inv = linear ? 4.0 : 3.0;
+    #pragma omp parallel for schedule(static)
+    for (int id = 0; id < o.N; ++id) {
+        double* g = &P.f[(size_t)id * Q]; double* geq = &P.feq[(size_t)id * Q]; double* gt = &P.ftmp[(size_t)id * Q];
+        double gkx = 0, gky = 0, gkz = 0;
+        const double u2 = o.u[id] * o.u[id] + o.v[id] * o.v[id] + o.w[id] * o.w[id];
+        for (int k = 0; k < Q; ++k) {
+            const double t = (L.cx[k] * o.c * o.u[id] + L.cy[k] * o.c * o.v[id] + L.cz[k] * o.c * o.w[id]) / c2;
+            double e = 1.0 + cs2inv * t;
+            if (!linear) e += 4.5 * t * t - 1.5 * u2 / c2;
+            geq[k] = L.w[k] * o.S[id] * e;
+            gt[k] = g[k] - geq[k];
+            gkx += gt[k] * L.cx[k] * o.c; gky += gt[k] * L.cy[k] * o.c; gkz += gt[k] * L.cz[k] * o.c;
+        }
+        for (int k = 0; k < Q; ++k) {
+            gt[k] = L.w[k] * cs2inv * (gkx * L.cx[k] * o.c + gky * L.cy[k] * o.c + gkz * L.cz[k] * o.c) / c2;
+            g[k] = geq[k] + (1.0 - 1.0 / o.taus[id]) * gt[k];
+            gt[k] = g[k]; geq[k] = 0; g[k] = 0;
+        }
+    }
+}
+
+// col_PF (collision.cu:190-223): conservative Allen-Cahn collision; the reference hard-codes
+// tau_phi = 0.58 and the source weight tau_phi - 0.5 = 0.08, here cfg.pf_tau.
+void col_PF(Oracle& o) {
+    PopSet& P = o.sc; const int Q = P.Q; const Lattice& L = P.L;
+    const double c2 = o.c * o.c;
+    const int up = o.dim == 3 ? 5 : 2, down = o.dim == 3 ? 6 : 4;
+    const double eps = o.dz / 1000;
+    #pragma omp parallel for schedule(static)
+    for (int id = 0; id < o.N; ++id) {
+        const int* nb = &o.neib[(size_t)id * o.Q];
+        const double* ph = o.phi.data();
+        const double* px = o.pos[0].data(); const double* py = o.pos[1].data(); const double* pz = o.pos[2].data();
+        const double nx_ = (ph[nb[1]] - ph[nb[3]]) / (px[nb[1]] - px[nb[3]] + eps);
+        double ny_ = 0;
+        if (o.dim == 3) ny_ = (ph[nb[2]] - ph[nb[4]]) / (py[nb[2]] - py[nb[4]] + eps);
+        const double nz_ = (ph[nb[up]] - ph[nb[down]]) / (pz[nb[up]] - pz[nb[down]] + eps);
+        const double nabu = (ph[id] * o.u[id] - o.phiold[id] * o.uold[id]) / o.dt;
+        const double nabv = (ph[id] * o.v[id] - o.phiold[id] * o.vold[id]) / o.dt;
+        const double nabw = (ph[id] * o.w[id] - o.phiold[id] * o.wold[id]) / o.dt;
+        o.phiold[id] = ph[id]; o.uold[id] = o.u[id]; o.vold[id] = o.v[id]; o.wold[id] = o.w[id];
+        double* g = &P.f[(size_t)id * Q]; double* geq = &P.feq[(size_t)id * Q]; double* gt = &P.ftmp[(size_t)id * Q];
+        const double u2 = o.u[id] * o.u[id] + o.v[id] * o.v[id] + o.w[id] * o.w[id];
+        const double nrm = std::sqrt(nx_ * nx_ + ny_ * ny_ + nz_ * nz_ + 1e-8);
+        for (int k = 0; k < Q; ++k) {
+            const double cxk = L.cx[k] * o.c, cyk = L.cy[k] * o.c, czk = L.cz[k] * o.c;
+            const double t = (cxk * o.u[id] + cyk * o.v[id] + czk * o.w[id]) / c2;
+            geq[k] = L.w[k] * ph[id] * (1.0 + 3.0 * t + 4.5 * t * t - 1.5 * u2 / c2);
+            const double Fkp = o.dt * L.w[k] * (
+                (1.0 - 4.0 * (ph[id] - 0.5) * (ph[id] - 0.5)) / o.cfg.pf_thickness * (cxk * nx_ + cyk * ny_ + czk * nz_) / nrm
+                + 3 * (cxk * nabu + cyk * nabv + czk * nabw) / c2);
+            const double tp = o.cfg.pf_tau;  // 0.58 in the reference
+            g[k] = g[k] - (g[k] - geq[k] - Fkp / 1.0 * (tp - 0.5)) / tp + Fkp * 0;
+            gt[k] = g[k]; geq[k] = 0; g[k] = 0;
+        }
+    }
+}
+
+// update_scalar (collision.cu:225-236)
+void update_scalar(Oracle& o, std::vector<double>& scalar) {
+    PopSet& P = o.sc; const int Q = P.Q;
+    #pragma omp parallel for schedule(static)
+    for (int id = 0; id < o.N; ++id) {
+        double s = 0;
+        for (int k = 0; k < Q; ++k) s += P.f[(size_t)id * Q + k];
+        scalar[id] = s;
+    }
+}
+
+// update_rho (collision.cu:238-257): macroscopic readout
+void update_rho(Oracle& o) {
+    const int Q = o.Q;
+    #pragma omp parallel for schedule(static)
+    for (int id = 0; id < o.N; ++id) {
+        double p = 0, ux = 0, uy = 0, uz = 0;
+        for (int k = 0; k < Q; ++k) {
+            const double f = o.fl.f[(size_t)id * Q + k];
+            p += f; ux += o.L.cx[k] * o.c * f; uy += o.L.cy[k] * o.c * f; uz += o.L.cz[k] * o.c * f;
+        }
+        o.p[id] = p;
+        o.rho[id] = o.cfg.rho_L + (o.cfg.rho_H - o.cfg.rho_L) * o.phi[id] + o.S[id] * o.cfg.beta_s;
+        o.u[id] = ux + o.Fx[id] * o.dt / (2.0 * o.rho[id]);
+        o.v[id] = uy + o.Fy[id] * o.dt / (2.0 * o.rho[id]);
+        o.w[id] = uz + o.Fz[id] * o.dt / (2.0 * o.rho[id]);
+    }
+}
+
+// set_wall_rho (basic.cu:254-261)
+void set_wall_rho(Oracle& o, std::vector<double>& a) {
+    for (int g = o.N; g < o.Ntot; ++g) a[g] = a[o.neib[(size_t)g * o.Q]];
+}
+
+// ------------------------------------------------------------------------------------------
+// IP_process (adv.cu:245-257): the interpolated propagation, ten launches in the same order.
+// ------------------------------------------------------------------------------------------
+
+// set_posl_x / set_posl_y (adv.cu:15-28) for a stencil ordered along sign s, plus the
+// periodic unwrap of the builder extension.
+void fix_positions(const Oracle& o, int axis, double* x, int n, int s) {
+    if (o.cfg.periodic[axis]) {
+        const double Lp = o.period[axis];
+        const int c = n / 2;
+        for (int j = c + 1; j < n; ++j) while (s * (x[j] - x[j - 1]) <= 0) x[j] += s * Lp;
+        for (int j = c - 1; j >= 0; --j) while (s * (x[j + 1] - x[j]) <= 0) x[j] -= s * Lp;
+    }
+    const double d = o.dz;
+    if (n == 5) {
+        if (s == 1) {
+            x[1] = std::min(x[1], x[2] - d); x[0] = std::min(x[0], x[1] - d);
+            x[3] = std::max(x[3], x[2] + d); x[4] = std::max(x[4], x[3] + d);
+        } else {
+            x[1] = std::max(x[1], x[2] + d); x[0] = std::max(x[0], x[1] + d);
+            x[3] = std::min(x[3], x[2] - d); x[4] = std::min(x[4], x[3] - d);
+        }
+    } else {
+        if (s == 1) { x[0] = std::min(x[0], x[1] - d); x[2] = std::max(x[2], x[1] + d); }
+        else { x[0] = std::max(x[0], x[1] + d); x[2] = std::min(x[2], x[1] - d); }
+    }
+}
+
+// 1D slab flux of cell id along axis for direction k: the x/y loops of set_out (adv.cu:76-99)
+void flux_1d(Oracle& o, PopSet& P, int id, int k, int axis) {
+    const int Q = P.Q, QN = o.Q;
+    const int s = comp(P.L, k, axis);
+    const int dp = axis_dir(o.L, axis, s), dm = axis_dir(o.L, axis, -s);
+    const int cp = o.neib[(size_t)id * QN + dp], cm = o.neib[(size_t)id * QN + dm];
+    const int cpp = o.neib[(size_t)cp * QN + dp], cmm = o.neib[(size_t)cm * QN + dm];
+    const std::vector<double>& pa = o.pos[axis];
+    const std::vector<double>& da = o.del[axis];
+    const double xa = pa[id] + s * (da[id] - o.dz) / 2.0;
+    double coeff[5], x[5];
+    double F;
+    if (o.npts == 5) {
+        x[0] = pa[cmm]; x[1] = pa[cm]; x[2] = pa[id]; x[3] = pa[cp]; x[4] = pa[cpp];
+        fix_positions(o, axis, x, 5, s);
+        lagrange(coeff, x, xa, 5);
+        F = coeff[0] * P.ftmp[(size_t)cmm * Q + k] + coeff[1] * P.ftmp[(size_t)cm * Q + k] +
+            coeff[2] * P.ftmp[(size_t)id * Q + k] + coeff[3] * P.ftmp[(size_t)cp * Q + k] +
+            coeff[4] * P.ftmp[(size_t)cpp * Q + k];
+    } else {
+        x[0] = pa[cm]; x[1] = pa[id]; x[2] = pa[cp];
+        fix_positions(o, axis, x, 3, s);
+        lagrange(coeff, x, xa, 3);
+        F = coeff[0] * P.ftmp[(size_t)cm * Q + k] + coeff[1] * P.ftmp[(size_t)id * Q + k] +
+            coeff[2] * P.ftmp[(size_t)cp * Q + k];
+    }
+    P.feq[(size_t)id * Q + k] = F * (o.dz / da[cp]);
+    P.fout[(size_t)id * Q + k] = F * (o.dz / da[id]);
+}
+
+// Corner directions of the 3D x-y scheme: the num3 loop of set_out (adv.cu:100-145)
+void flux_corner(Oracle& o, PopSet& P, int id, int kk) {
+    const int Q = P.Q, QN = o.Q;
+    const RefTables& t = o.rt;
+    auto nb = [&](int cell, int d) { return o.neib[(size_t)cell * QN + d]; };
+    int idf[25];
+    idf[12] = id;
+    idf[13] = nb(id, t.cxp[kk]); idf[14] = nb(idf[13], t.cxp[kk]);
+    idf[11] = nb(id, t.cxm[kk]); idf[10] = nb(idf[11], t.cxm[kk]);
+    idf[17] = nb(id, t.cyp[kk]); idf[22] = nb(idf[17], t.cyp[kk]);
+    idf[7] = nb(id, t.cym[kk]); idf[2] = nb(idf[7], t.cym[kk]);
+    idf[18] = nb(id, t.cp[kk]); idf[24] = nb(idf[18], t.cp[kk]);
+    idf[6] = nb(id, t.cm[kk]); idf[0] = nb(idf[6], t.cm[kk]);
+    idf[8] = nb(id, t.cip[kk]); idf[4] = nb(idf[8], t.cip[kk]);
+    idf[16] = nb(id, t.cim[kk]); idf[20] = nb(idf[16], t.cim[kk]);
+    idf[19] = nb(idf[14], t.cyp[kk]); idf[9] = nb(idf[14], t.cym[kk]);
+    idf[15] = nb(idf[10], t.cyp[kk]); idf[5] = nb(idf[10], t.cym[kk]);
+    idf[23] = nb(idf[22], t.cxp[kk]); idf[21] = nb(idf[22], t.cxm[kk]);
+    idf[3] = nb(idf[2], t.cxp[kk]); idf[1] = nb(idf[2], t.cxm[kk]);
+    const int sx = P.L.cx[kk], sy = P.L.cy[kk];
+    double cxw[5], cyw[5], x[5];
+    double xa = o.pos[0][id] + sx * (o.del[0][id] - o.dz) / 2.0;
+    x[0] = o.pos[0][idf[10]]; x[1] = o.pos[0][idf[11]]; x[2] = o.pos[0][id]; x[3] = o.pos[0][idf[13]]; x[4] = o.pos[0][idf[14]];
+    fix_positions(o, 0, x, 5, sx);
+    lagrange(cxw, x, xa, 5);
+    double fx = 0;
+    for (int i = 0; i < 5; ++i) fx += cxw[i] * P.ftmp[(size_t)idf[i + 10] * Q + kk];
+    xa = o.pos[1][id] + sy * (o.del[1][id] - o.dz) / 2.0;
+    x[0] = o.pos[1][idf[2]]; x[1] = o.pos[1][idf[7]]; x[2] = o.pos[1][id]; x[3] = o.pos[1][idf[17]]; x[4] = o.pos[1][idf[22]];
+    fix_positions(o, 1, x, 5, sy);
+    lagrange(cyw, x, xa, 5);
+    double fy = 0;
+    for (int i = 0; i < 5; ++i) fy += cyw[i] * P.ftmp[(size_t)idf[2 + i * 5] * Q + kk];
+    double fxy = 0;
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) fxy += cxw[i] * cyw[j] * P.ftmp[(size_t)idf[i + j * 5] * Q + kk];
+    const double dz = o.dz;
+    const std::vector<double>& dX = o.del[0]; const std::vector<double>& dY = o.del[1];
+    P.fout[(size_t)id * Q + kk] = fx * dz / dX[id] + fy * dz / dY[id] - fxy * (dz * dz) / (dX[id] * dY[id]);
+    const double a0 = fxy * (dz * dz) / (dX[idf[13]] * dY[idf[17]]);
+    const double a1 = (fy - fxy * dz / dX[idf[17]]) * dz / dY[idf[17]];
+    const double a2 = (fx - fxy * dz / dY[idf[13]]) * dz / dX[idf[13]];
+    double* fin = P.feq.data(); double* finy = P.f.data();
+    #pragma omp atomic
+    fin[(size_t)id * Q + kk] += a0;
+    #pragma omp atomic
+    fin[(size_t)nb(id, t.cxm[kk]) * Q + kk] += a1;
+    #pragma omp atomic
+    finy[(size_t)nb(id, t.cym[kk]) * Q + kk] += a2;
+}
+
+void ip_process(Oracle& o, PopSet& P) {
+    const int Q = P.Q, QN = o.Q, N = o.N, Ntot = o.Ntot;
+    const Lattice& L = P.L;
+    // set_wall_f (adv.cu:3-13): ghost <- owner
+    for (int g = N; g < Ntot; ++g) {
+        const int ow = o.neib[(size_t)g * QN];
+        for (int k = 0; k < Q; ++k) P.ftmp[(size_t)g * Q + k] = P.ftmp[(size_t)ow * Q + k];
+    }
+    // set_out (adv.cu:50-147)
+    #pragma omp parallel for schedule(static)
+    for (int id = 0; id < N; ++id) {
+        for (int k = 0; k < Q; ++k) {
+            int nflux = 0, ax = -1;
+            if (o.flux_mode == FLUX_ONE_AXIS) { if (comp(L, k, o.flux_axis) != 0) { nflux = 1; ax = o.flux_axis; } }
+            else if (o.flux_mode == FLUX_CORNER) {
+                const bool hx = L.cx[k] != 0, hy = L.cy[k] != 0;
+                nflux = (int)hx + (int)hy; ax = hx ? 0 : 1;
+            }
+            if (nflux == 0) {
+                P.fout[(size_t)id * Q + k] = P.ftmp[(size_t)id * Q + k];
+                P.feq[(size_t)id * Q + k] = P.ftmp[(size_t)id * Q + k];
+            } else if (nflux == 1) {
+                flux_1d(o, P, id, k, ax);
+            }
+        }
+        if (o.flux_mode == FLUX_CORNER)
+            for (int k = 7; k < Q; ++k)
+                if (L.cx[k] != 0 && L.cy[k] != 0) flux_corner(o, P, id, k);
+    }
+    // set_fin (adv.cu:148-175): 3D corner scheme only; ghosts pass their corner fluxes on
+    if (o.flux_mode == FLUX_CORNER) {
+        const int nk[8] = {9, 8, 10, 7, 7, 10, 8, 9};
+        const int nkz[8] = {19, 20, 21, 22, 23, 24, 25, 26};
+        const int nneib[8] = {1, 2, 1, 4, 3, 2, 3, 4};
+        const int k3[12] = {7, 8, 9, 10, 19, 21, 23, 25, 20, 22, 24, 26};
+        double* fin = P.feq.data(); double* finy = P.f.data();
+        for (int g = N; g < Ntot; ++g)
+            for (int k = 0; k < 4; ++k) {
+                const int t0 = o.neib[(size_t)g * QN + nneib[k * 2 + 0]];
+                const int t1 = o.neib[(size_t)g * QN + nneib[k * 2 + 1]];
+                const double f0 = (double)(1 - t0 / g), f1 = (double)(1 - t1 / g);
+                const int a = nk[2 * (k3[k] - 7) + 0], b = nk[2 * (k3[k] - 7) + 1];
+                fin[(size_t)t0 * Q + a] += fin[(size_t)g * Q + k3[k]] * f0;
+                finy[(size_t)t1 * Q + b] += finy[(size_t)g * Q + k3[k]] * f1;
+                if (Q == 27) {
+                    fin[(size_t)t0 * Q + nkz[2 * (a - 7) + 0]] += fin[(size_t)g * Q + k3[k + 4]] * f0;
+                    fin[(size_t)t0 * Q + nkz[2 * (a - 7) + 1]] += fin[(size_t)g * Q + k3[k + 8]] * f0;
+                    finy[(size_t)t1 * Q + nkz[2 * (b - 7) + 0]] += finy[(size_t)g * Q + k3[k + 4]] * f1;
+                    finy[(size_t)t1 * Q + nkz[2 * (b - 7) + 1]] += finy[(size_t)g * Q + k3[k + 8]] * f1;
+                }
+            }
+    }
+    // reset_f (adv.cu:176-187)
+    #pragma omp parallel for schedule(static)
+    for (int id = 0; id < N; ++id)
+        for (int k = 0; k < Q; ++k) {
+            P.feq[(size_t)id * Q + k] += P.f[(size_t)id * Q + k];
+            P.f[(size_t)id * Q + k] = P.ftmp[(size_t)id * Q + k];
+        }
+    // set_wall_f_inv on f and on fout (adv.cu:188-232): ghost <- mirrored owner
+    auto wall_inv = [&](std::vector<double>& a) {
+        for (int g = N; g < Ntot; ++g) {
+            const int ow = o.neib[(size_t)g * QN];
+            if (o.flux_mode == FLUX_CORNER) {
+                // xy mirror of axis / xz / yz directions, then corners through nextB/nextK
+                int kxy[19] = {0, 3, 4, 1, 2, 0, 0, 0, 0, 0, 0, 13, 14, 11, 12, 17, 18, 15, 16};
+                const int kn1[15] = {0, 1, 2, 3, 4, 5, 6, 11, 12, 13, 14, 15, 16, 17, 18};
+                for (int q = 0; q < 15; ++q) a[(size_t)g * Q + kn1[q]] = a[(size_t)ow * Q + kxy[kn1[q]]];
+                const int k3[12] = {7, 8, 9, 10, 19, 21, 23, 25, 20, 22, 24, 26};
+                const int nkz[8] = {20, 19, 22, 21, 24, 23, 26, 25};
+                const size_t wid = (size_t)(g - N) * Q;
+                for (int q = 0; q < 4; ++q) {
+                    const int nk = L.kinv[P.nextK[wid + L.kinv[k3[q]]]], neB = P.nextB[wid + L.kinv[k3[q]]];
+                    const int src = o.neib[(size_t)ow * QN + neB];
+                    a[(size_t)g * Q + k3[q]] = a[(size_t)src * Q + nk];
+                    if (Q == 27) {
+                        a[(size_t)g * Q + k3[q + 4]] = a[(size_t)src * Q + nkz[2 * (nk - 7) + 1]];
+                        a[(size_t)g * Q + k3[q + 8]] = a[(size_t)src * Q + nkz[2 * (nk - 7) + 0]];
+                    }
+                }
+            } else {
+                // mirror along the flux axis: the 2D x table of adv.cu:196-199 for x
+                for (int k = 1; k < Q; ++k) {
+                    int mk = 0;
+                    if (o.flux_mode == FLUX_ONE_AXIS) {
+                        int c[3] = {L.cx[k], L.cy[k], L.cz[k]};
+                        if (c[o.flux_axis] != 0) {
+                            c[o.flux_axis] = -c[o.flux_axis];
+                            for (int j = 0; j < Q; ++j)
+                                if (L.cx[j] == c[0] && L.cy[j] == c[1] && L.cz[j] == c[2]) mk = j;
+                        }
+                    }
+                    a[(size_t)g * Q + k] = a[(size_t)ow * Q + mk];
+                }
+            }
+        }
+    };
+    wall_inv(P.f);
+    wall_inv(P.fout);
+    // set_tmp (adv.cu:234-244): ftmp[i,k] = f[n,k] + (fin[i,k] - fout[n,k]), n = neib[i, cp[k]]
+    #pragma omp parallel for schedule(static)
+    for (int id = 0; id < N; ++id)
+        for (int k = 0; k < Q; ++k) {
+            int d = 0;
+            if (o.flux_mode == FLUX_ONE_AXIS) {
+                const int s = comp(L, k, o.flux_axis);
+                if (s) d = axis_dir(o.L, o.flux_axis, s);
+            } else if (o.flux_mode == FLUX_CORNER) d = o.rt.cp[k];
+            const int n = o.neib[(size_t)id * QN + d];
+            P.ftmp[(size_t)id * Q + k] = P.f[(size_t)n * Q + k] + (P.feq[(size_t)id * Q + k] - P.fout[(size_t)n * Q + k]);
+        }
+    // set_wall_propagation (set_bound.cu:149-168)
+    for (int g = N; g < Ntot; ++g) {
+        const int ow = o.neib[(size_t)g * QN];
+        const size_t wid = (size_t)(g - N) * Q;
+        for (int k = 1; k < Q; ++k) {
+            if (o.cfg.wall_bc == IPL_BC_BOUNCE_BACK)
+                P.ftmp[(size_t)g * Q + k] = P.ftmp[(size_t)ow * Q + L.kinv[k]];
+            else {
+                const int src = o.neib[(size_t)ow * QN + P.nextB[wid + k]];
+                P.ftmp[(size_t)g * Q + k] = P.ftmp[(size_t)src * Q + P.nextK[wid + k]];
+            }
+        }
+    }
+    // propagation (propagation.cu:2-12)
+    #pragma omp parallel for schedule(static)
+    for (int id = 0; id < N; ++id)
+        for (int k = 0; k < Q; ++k)
+            P.f[(size_t)id * Q + k] = P.ftmp[(size_t)o.neib[(size_t)id * QN + L.kinv[k]] * Q + k];
+    // reset_wall_fin (basic.cu:274-284)
+    for (int g = N; g < Ntot; ++g)
+        for (int k = 0; k < Q; ++k) { P.feq[(size_t)g * Q + k] = 0; P.f[(size_t)g * Q + k] = 0; }
+}
+
+void reset_force(Oracle& o) {
+    std::fill(o.Fx.begin(), o.Fx.end(), 0.0);
+    std::fill(o.Fy.begin(), o.Fy.end(), 0.0);
+    std::fill(o.Fz.begin(), o.Fz.end(), 0.0);
+}
+
+void stage(Oracle& o, int s) {
+    switch (s) {
+    case IPL_STAGE_EQU_F: equ_f(o); break;
+    case IPL_STAGE_FORCE: force(o); break;
+    case IPL_STAGE_COLLIDE: collide(o); break;
+    case IPL_STAGE_IP_F: ip_process(o, o.fl); break;
+    case IPL_STAGE_SCALAR_COLLIDE:
+        if (o.cfg.scalar == IPL_SCALAR_SALINITY) col_g_reg(o);
+        else if (o.cfg.scalar == IPL_SCALAR_PHASE_FIELD) col_PF(o);
+        break;
+    case IPL_STAGE_IP_G: if (o.has_scalar) ip_process(o, o.sc); break;
+    case IPL_STAGE_UPDATE_SCALAR:
+        if (o.cfg.scalar == IPL_SCALAR_SALINITY) update_scalar(o, o.S);
+        else if (o.cfg.scalar == IPL_SCALAR_PHASE_FIELD) update_scalar(o, o.phi);
+        break;
+    case IPL_STAGE_UPDATE_RHO: update_rho(o); break;
+    case IPL_STAGE_SET_WALL_RHO:
+        set_wall_rho(o, o.rho);
+        if (o.cfg.scalar == IPL_SCALAR_PHASE_FIELD) set_wall_rho(o, o.phi);
+        break;
+    case IPL_STAGE_RESET_F: reset_force(o); break;
+    }
+}
+
+// One step in driver order (LockExchange.cu:220-278; Zhou_soliton4B1.cu:220-277)
+void step(Oracle& o) {
+    for (int s = IPL_STAGE_EQU_F; s <= IPL_STAGE_RESET_F; ++s) stage(o, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+void* orc_create(const ipl_config* cfg, const ipl_init* init, char* err, int errlen) {
+    Oracle* o = new Oracle();
+    if (!build(*o, *cfg, init)) {
+        if (err && errlen > 0) std::snprintf(err, errlen, "%s", o->err.c_str());
+        delete o;
+        return nullptr;
+    }
+    return o;
+}
+
+void orc_destroy(void* h) { delete static_cast<Oracle*>(h); }
+
+int orc_step(void* h, int n) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    for (int i = 0; i < n; ++i) step(o);
+    return 0;
+}
+
+double orc_time_steps(void* h, int n) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < n; ++i) step(o);
+    const auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+int orc_stage(void* h, int s) {
+    stage(*static_cast<Oracle*>(h), s);
+    return 0;
+}
+
+int orc_info(void* h, int64_t* n_calc, int64_t* n_ghost) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    *n_calc = o.N; *n_ghost = o.G;
+    return 0;
+}
+
+static std::vector<double>* field_ptr(Oracle& o, int field) {
+    switch (field) {
+    case IPL_FIELD_P: return &o.p; case IPL_FIELD_U: return &o.u; case IPL_FIELD_V: return &o.v;
+    case IPL_FIELD_W: return &o.w; case IPL_FIELD_RHO: return &o.rho; case IPL_FIELD_S: return &o.S;
+    case IPL_FIELD_PHI: return &o.phi; case IPL_FIELD_FX: return &o.Fx; case IPL_FIELD_FY: return &o.Fy;
+    case IPL_FIELD_FZ: return &o.Fz; case IPL_FIELD_TAU: return &o.tau; case IPL_FIELD_TAUS: return &o.taus;
+    default: return nullptr;
+    }
+}
+
+int orc_read(void* h, int field, double* out, size_t count) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    if (field == IPL_FIELD_F || field == IPL_FIELD_G) {
+        PopSet& P = field == IPL_FIELD_F ? o.fl : o.sc;
+        if (field == IPL_FIELD_G && !o.has_scalar) return IPL_ERR_ARGUMENT;
+        if (count != (size_t)P.Q * o.N) return IPL_ERR_ARGUMENT;
+        for (int k = 0; k < P.Q; ++k)
+            for (int id = 0; id < o.N; ++id) out[(size_t)k * o.N + id] = P.f[(size_t)id * P.Q + k];
+        return 0;
+    }
+    std::vector<double>* v = field_ptr(o, field);
+    if (!v || count != (size_t)o.N) return IPL_ERR_ARGUMENT;
+    std::copy(v->begin(), v->begin() + o.N, out);
+    return 0;
+}
+
+int orc_write(void* h, int field, const double* in, size_t count) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    if (field == IPL_FIELD_F || field == IPL_FIELD_G) {
+        PopSet& P = field == IPL_FIELD_F ? o.fl : o.sc;
+        if (count != (size_t)P.Q * o.N) return IPL_ERR_ARGUMENT;
+        for (int k = 0; k < P.Q; ++k)
+            for (int id = 0; id < o.N; ++id) P.f[(size_t)id * P.Q + k] = in[(size_t)k * o.N + id];
+        return 0;
+    }
+    std::vector<double>* v = field_ptr(o, field);
+    if (!v || count != (size_t)o.N) return IPL_ERR_ARGUMENT;
+    std::copy(in, in + count, v->begin());
+    return 0;
+}
+
+// output() global sums (basic.cu:9-13): volume = delX delY / dz^2
+int orc_global_sums(void* h, double out[3]) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    double tp = 0, ts = 0, tf = 0;
+    for (int i = 0; i < o.N; ++i) {
+        const double vol = o.del[0][i] * o.del[1][i] * o.del[2][i] / (o.dz * o.dz * o.dz);
+        tp += o.p[i] * vol; ts += o.S[i] * vol; tf += o.phi[i] * vol;
+    }
+    out[0] = tp; out[1] = ts; out[2] = tf;
+    return 0;
+}
+
+// Builder tables for cross-checks: 0 = neib (Ntot*Q), 1 = nextB, 2 = nextK (G*Q, flow lattice)
+int64_t orc_table(void* h, int which, int32_t* out, int64_t count) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    const std::vector<int>& t = which == 0 ? o.neib : (which == 1 ? o.fl.nextB : o.fl.nextK);
+    if (!out) return (int64_t)t.size();
+    if (count != (int64_t)t.size()) return -1;
+    std::copy(t.begin(), t.end(), out);
+    return count;
+}
+
+void orc_lagrange(const double* x, double xa, int n, double* coeff) { lagrange(coeff, x, xa, n); }
+
+int orc_mrt(int Q, double* M, double* Minv, int32_t* stress, int32_t* nstress) {
+    if (Q != 9 && Q != 19 && Q != 27) return -1;
+    Lattice L = make_lattice(Q);
+    std::vector<double> m, mi; std::vector<int> st;
+    mrt_matrix(L, m, st);
+    mrt_inverse(Q, m, mi);
+    std::copy(m.begin(), m.end(), M);
+    std::copy(mi.begin(), mi.end(), Minv);
+    for (size_t i = 0; i < st.size(); ++i) stress[i] = st[i];
+    *nstress = (int32_t)st.size();
+    return 0;
+}
+
+int orc_lattice(int Q, int32_t* c, double* w) {
+    if (Q != 7 && Q != 9 && Q != 19 && Q != 27) return -1;
+    Lattice L = make_lattice(Q);
+    for (int k = 0; k < Q; ++k) { c[3 * k] = L.cx[k]; c[3 * k + 1] = L.cy[k]; c[3 * k + 2] = L.cz[k]; w[k] = L.w[k]; }
+    return 0;
+}
+
+}  // extern "C"
+
+/* PARITY
+ * Pinned (tests/test_oracle.py): MRT rows vs the reference's MRT_D3Q9/MRT_D3Q27.csv and M^-1 vs
+ * Minv_*.csv; lattice moments (sum w = 1, sum w c c = I/3); Lagrange weights (partition of unity,
+ * exact on polynomials of degree < n, unit vector at a node); equilibrium moments; a rest state
+ * is a fixed point; MRT with all rates 1/tau equals SRT; uniform grid = exact shift; periodic
+ * translation invariance; volume-weighted mass conservation with walls; Poiseuille profile with
+ * bounce-back; slip keeps a plug flow; set_bound2D tables against the mirror rule.
+ * Parity unpinned: the reference's own FP32 trajectories on NVIDIA hardware (CUDA device math
+ * and atomicAdd ordering cannot run here); the 3D two-axis corner scheme (set_out num3 /
+ * set_fin) is pinned only by mass conservation and the uniform-grid reduction.
+ */

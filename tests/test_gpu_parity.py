@@ -1,0 +1,95 @@
+"""HIP library vs the FP64 oracle on the same seeded inputs (plain path).
+
+Tolerance: FP64 storage <= 1e-12 relative L2 on p, u, v, w, S and phi after N steps; FP32
+storage <= 5e-6 on the velocities of short runs.  Calls go through the C ABI."""
+import numpy as np
+import pytest
+
+from ip_cu_lbm_amd import F32, F64, PATH_PLAIN, Simulation, _abi, cases
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FIELDS_FLOW = ("p", "u", "w")
+
+
+def _compare(case, steps, fields, tol, oracle_lib, report=None):
+    o = oracle_lib.Oracle(case)
+    with Simulation(case) as sim:
+        assert sim.n_calc == o.n_calc and sim.n_ghost == o.n_ghost
+        sim.step(steps)
+        o.step(steps)
+        errs = {f: rel_l2(sim.read(f), o.read(f)) for f in fields}
+        sums = (sim.global_sums(), o.global_sums())
+    print(case.name, steps, errs, flush=True)
+    for f, e in errs.items():
+        assert e <= tol, (f, e)
+    return errs, sums
+
+
+def test_taylor_green_small_fp64(oracle_lib):
+    case = cases.lattice_taylor_green(nx=64, nz=32, path=PATH_PLAIN)
+    _compare(case, 300, FIELDS_FLOW, 1e-12, oracle_lib)
+
+
+def test_taylor_green_full_fp64_1000_steps(oracle_lib):
+    case = cases.lattice_taylor_green(path=PATH_PLAIN)
+    errs, (sg, so) = _compare(case, 1000, FIELDS_FLOW, 1e-12, oracle_lib)
+    assert abs(sg[0] - so[0]) <= 1e-12 * max(1.0, abs(so[0]))
+
+
+def test_taylor_green_three_point_stencil_fp64(oracle_lib):
+    case = cases.lattice_taylor_green(nx=48, nz=16, stencil=3, path=PATH_PLAIN)
+    _compare(case, 200, FIELDS_FLOW, 1e-12, oracle_lib)
+
+
+def test_walls_slip_all_sides_fp64(oracle_lib):
+    case = cases.lattice_taylor_green(nx=40, nz=24, path=PATH_PLAIN).replace(periodic=(0, 0, 0), wall_bc=_abi.BC_SLIP)
+    _compare(case, 200, FIELDS_FLOW, 1e-12, oracle_lib)
+
+
+def test_lock_exchange_d3q19_d3q7_fp64(oracle_lib):
+    case = cases.lock_exchange(nx=32, ny=6, nz=16, path=PATH_PLAIN)
+    _compare(case, 150, ("p", "u", "v", "w", "S"), 1e-12, oracle_lib)
+
+
+def test_lock_exchange_same_q_salinity_bounce_back_fp64(oracle_lib):
+    case = cases.lock_exchange(nx=24, ny=6, nz=12, scalar_lattice=0, path=PATH_PLAIN).replace(wall_bc=_abi.BC_BOUNCE_BACK)
+    _compare(case, 100, ("p", "u", "v", "w", "S"), 1e-12, oracle_lib)
+
+
+@pytest.mark.parametrize("q,bc", [(27, _abi.BC_SLIP), (27, _abi.BC_BOUNCE_BACK), (19, _abi.BC_SLIP)])
+def test_reference_corner_scheme_3d_fp64(oracle_lib, q, bc):
+    nx, ny, nz = 10, 8, 6
+    px, wx = cases.axis_from_widths(cases.stretched_widths(nx, 1.1))
+    py, wy = cases.axis_from_widths(cases.stretched_widths(ny, 1.15))
+    pz, wz = cases.axis_from_widths(np.ones(nz))
+    case = cases.Case(name=f"corner_q{q}", lattice=q, n=(nx, ny, nz), pos=[px, py, pz], width=[wx, wy, wz],
+                      wall_bc=bc, flux_axes=_abi.FLUX_REFERENCE, tau=0.7, path=PATH_PLAIN,
+                      s_rates=np.full(27, 1.2))
+    X, Y, Z = case.coords()
+    case.init = {"u": 0.01 * np.sin(X / 3.0) * np.cos(Z), "v": 0.01 * np.cos(Y / 2.0),
+                 "p": cases.uniform(7, case.n_cells, -1e-3, 1e-3)}
+    _compare(case, 60, ("p", "u", "v", "w"), 1e-12, oracle_lib)
+
+
+def test_solitary_wave_phase_field_fp64(oracle_lib):
+    case = cases.solitary_wave(nx=96, nz=48, x0=30, sigma=1e-4, path=PATH_PLAIN)
+    _compare(case, 150, ("p", "u", "w", "phi"), 1e-12, oracle_lib)
+
+
+def test_taylor_green_fp32_storage(oracle_lib):
+    case = cases.lattice_taylor_green(nx=64, nz=32, storage=F32, path=PATH_PLAIN)
+    _compare(case, 200, ("u", "w"), 5e-6, oracle_lib)
+
+
+def test_nonfinite_detected():
+    case = cases.lattice_taylor_green(nx=32, nz=16, path=PATH_PLAIN).replace(check_finite=1)
+    with Simulation(case) as sim:
+        sim.step(2)
+        f = sim.read("f")
+        f[3, 17] = np.nan
+        sim.write("f", f)
+        with pytest.raises(_abi.IplError) as ei:
+            sim.step(1)
+        assert ei.value.code == _abi.IPL_ERR_NONFINITE

@@ -1,0 +1,193 @@
+"""The FP64 oracle pinned against the reference's fixtures, closed forms and invariants."""
+import os
+
+import numpy as np
+import pytest
+
+from ip_cu_lbm_amd import cases
+from ip_cu_lbm_amd import _abi
+from conftest import rel_l2
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _csv(name, q):
+    return np.loadtxt(os.path.join(GOLDEN, name), delimiter=",", ndmin=2)[:q, :q]
+
+
+@pytest.mark.parametrize("q", [9, 27])
+def test_mrt_matrix_matches_reference_csv(oracle_lib, q):
+    M, Mi, stress = oracle_lib.mrt(q)
+    assert np.array_equal(M, _csv(f"MRT_D3Q{q}.csv", q))
+    assert np.abs(Mi - _csv(f"Minv_D3Q{q}.csv", q)).max() < 1e-15
+    # the stress rows zeroed by set_M (basic.cu:185-186)
+    assert stress == ([7, 8] if q == 9 else [5, 6, 7, 8, 9])
+
+
+@pytest.mark.parametrize("q", [9, 19, 27])
+def test_mrt_basis_invariants(oracle_lib, q):
+    M, Mi, stress = oracle_lib.mrt(q)
+    assert np.abs(M @ Mi - np.eye(q)).max() < 1e-14
+    G = M @ M.T
+    assert np.abs(G - np.diag(np.diag(G))).max() == 0.0  # orthogonal rows
+    c, w = oracle_lib.lattice(q)
+    # conserved moments: density and momentum rows are 1 and c
+    assert np.array_equal(M[0], np.ones(q))
+    rows = {tuple(r) for r in M.astype(int).tolist()}
+    for a in range(3 if q != 9 else 0):
+        assert tuple(c[:, a]) in rows
+    # stress rows are quadratic in c and traceless or off-diagonal
+    for s in stress:
+        r = M[s]
+        assert abs(np.dot(r, w)) < 1e-15  # zero at rest equilibrium
+
+
+@pytest.mark.parametrize("q", [7, 9, 19, 27])
+def test_lattice_isotropy(oracle_lib, q):
+    c, w = oracle_lib.lattice(q)
+    assert abs(w.sum() - 1.0) < 1e-15
+    cs2 = 0.25 if q == 7 else 1.0 / 3.0
+    dims = [0, 2] if q == 9 else [0, 1, 2]
+    T2 = np.einsum("k,ka,kb->ab", w, c, c)
+    for a in dims:
+        for b in dims:
+            assert abs(T2[a, b] - (cs2 if a == b else 0.0)) < 1e-15
+    assert np.abs(np.einsum("k,ka->a", w, c)).max() < 1e-15
+    if q != 7:  # fourth-order isotropy: sum w c_a^2 c_b^2 = cs4 (1 + 2 delta_ab)
+        T4 = np.einsum("k,ka,ka,kb,kb->ab", w, c, c, c, c)
+        for a in dims:
+            for b in dims:
+                assert abs(T4[a, b] - cs2 * cs2 * (3.0 if a == b else 1.0)) < 1e-15
+
+
+def test_lagrange_weights(oracle_lib):
+    rng = np.random.default_rng(1)
+    for n in (3, 5):
+        x = np.sort(rng.uniform(-3, 3, n)) + np.arange(n)
+        xa = rng.uniform(x[0], x[-1])
+        cw = oracle_lib.lagrange(x, xa)
+        assert abs(cw.sum() - 1.0) < 1e-13
+        for deg in range(n):  # exact on polynomials of degree < n
+            assert abs(np.dot(cw, x ** deg) - xa ** deg) < 1e-11 * max(1, abs(xa) ** deg)
+        cw0 = oracle_lib.lagrange(x, x[n // 2])
+        assert np.array_equal(cw0, np.eye(n)[n // 2])
+
+
+def _small_tg(**kw):
+    return cases.lattice_taylor_green(nx=32, nz=16, **kw)
+
+
+def test_rest_state_is_fixed_point(oracle_lib):
+    case = _small_tg()
+    case.init = {}
+    o = oracle_lib.Oracle(case)
+    o.step(20)
+    assert np.abs(o.read("f")).max() == 0.0
+    assert np.abs(o.read("u")).max() == 0.0
+
+
+def test_mrt_with_uniform_rates_equals_srt(oracle_lib):
+    a = _small_tg()
+    a.s_rates = np.full(9, 1.0 / a.tau)
+    b = a.replace(collision=_abi.COLL_SRT)
+    oa, ob = oracle_lib.Oracle(a), oracle_lib.Oracle(b)
+    oa.step(30)
+    ob.step(30)
+    for fld in ("p", "u", "w"):
+        assert rel_l2(oa.read(fld), ob.read(fld)) < 1e-13
+
+
+def test_uniform_grid_reduces_to_exact_streaming(oracle_lib):
+    a = _small_tg(ratio=1.0)
+    b = a.replace(flux_axes=_abi.FLUX_NONE)
+    oa, ob = oracle_lib.Oracle(a), oracle_lib.Oracle(b)
+    oa.step(40)
+    ob.step(40)
+    for fld in ("p", "u", "w"):
+        assert rel_l2(oa.read(fld), ob.read(fld)) < 1e-13
+
+
+def test_periodic_translation_invariance(oracle_lib):
+    nx, nz = 24, 12
+    a = cases.lattice_taylor_green(nx=nx, nz=nz, ratio=1.0)
+    b = a.replace(init={k: np.roll(v.reshape(nx, nz), 3, axis=0).ravel() for k, v in a.init.items()})
+    oa, ob = oracle_lib.Oracle(a), oracle_lib.Oracle(b)
+    oa.step(25)
+    ob.step(25)
+    ua = np.roll(oa.read("u").reshape(nx, nz), 3, axis=0).ravel()
+    assert rel_l2(ob.read("u"), ua) < 1e-12
+
+
+def test_mass_conserved_periodic_stretched(oracle_lib):
+    # periodic in x and z: the flux form telescopes, sum p * dx is conserved to round-off
+    a = cases.lattice_taylor_green(nx=32, nz=16)
+    a = a.replace(periodic=(1, 0, 1))
+    o = oracle_lib.Oracle(a)
+    s0 = o.global_sums()[0]
+    o.step(100)
+    assert abs(o.global_sums()[0] - s0) < 1e-14
+
+
+def _poiseuille(ratio, nx=24, stencil=5):
+    # flow along z driven by gravity between bounce-back walls at the x ends, z periodic
+    nz = 4
+    px, wx = cases.axis_from_widths(cases.stretched_widths(nx, ratio))
+    pz, wz = cases.axis_from_widths(np.ones(nz))
+    py, wy = cases.axis_from_widths(np.ones(1))
+    tau, g = 0.8, 1e-6
+    case = cases.Case(name="poiseuille", lattice=9, n=(nx, 1, nz), pos=[px, py, pz], width=[wx, wy, wz],
+                      periodic=(0, 0, 1), wall_bc=_abi.BC_BOUNCE_BACK, flux_axes=_abi.FLUX_X,
+                      stencil=stencil, s_rates=np.array([0, 1.4, 1.4, 0, 1.2, 0, 1.2, 1 / tau, 1 / tau]),
+                      tau=tau, rho_L=1.0, rho_H=1.0, gravity=g, boussinesq=0)
+    nu = (tau - 0.5) / 3.0
+    L = wx.sum()
+    exact = -g / (2 * nu) * px * (L - px)
+    return case, exact
+
+
+def _poiseuille_error(oracle_lib, ratio, nx):
+    case, exact = _poiseuille(ratio, nx)
+    o = oracle_lib.Oracle(case)
+    o.step(int(6000 * (nx / 24) ** 2))
+    w = o.read("w").reshape(case.n[0], case.n[2])[:, 0]
+    return rel_l2(w, exact)
+
+
+def test_poiseuille_profile_bounce_back(oracle_lib):
+    # second order on a uniform grid (half-way bounce-back)
+    assert _poiseuille_error(oracle_lib, 1.0, 24) < 2e-3
+    # stretched walls: the ghost-mirror wall cell is first order; halving the cells with the
+    # same total stretch halves the error
+    e1 = _poiseuille_error(oracle_lib, 1.02, 24)
+    e2 = _poiseuille_error(oracle_lib, 1.01, 48)
+    assert e1 < 0.07 and 0.4 < e2 / e1 < 0.6
+
+
+def test_slip_walls_keep_plug_flow(oracle_lib):
+    a = cases.lattice_taylor_green(nx=16, nz=8, ratio=1.0)
+    a = a.replace(wall_bc=_abi.BC_SLIP, init={"u": np.full(a.n_cells, 0.02)})
+    o = oracle_lib.Oracle(a)
+    o.step(50)
+    assert np.abs(o.read("u") - 0.02).max() < 1e-14
+    assert np.abs(o.read("w")).max() < 1e-14
+
+
+def test_salinity_diffusion_closed_form(oracle_lib):
+    # pure diffusion of a sine mode on a periodic uniform grid decays as exp(-D k^2 t)
+    nx, ny, nz = 32, 4, 4
+    px, wx = cases.axis_from_widths(np.ones(nx))
+    py, wy = cases.axis_from_widths(np.ones(ny))
+    pz, wz = cases.axis_from_widths(np.ones(nz))
+    D = 0.05
+    for q_s, cs2 in ((7, 0.25), (19, 1.0 / 3.0)):
+        case = cases.Case(name="diff", lattice=19, n=(nx, ny, nz), pos=[px, py, pz], width=[wx, wy, wz],
+                          periodic=(1, 1, 1), flux_axes=_abi.FLUX_NONE, tau=0.8, taus=D / cs2 + 0.5,
+                          scalar=_abi.SCALAR_SALINITY, scalar_lattice=q_s)
+        X, _, _ = case.coords()
+        k = 2 * np.pi / nx
+        case.init = {"S": 1.0 + 0.1 * np.sin(k * X)}
+        o = oracle_lib.Oracle(case)
+        nstep = 200
+        o.step(nstep)
+        amp = np.abs(o.read("S") - 1.0).max()
+        assert abs(amp / (0.1 * np.exp(-D * k * k * nstep)) - 1) < 0.01
