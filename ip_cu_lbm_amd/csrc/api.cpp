@@ -81,16 +81,19 @@ int ipl_create(const ipl_config* cfg, const ipl_init* init, ipl_ctx** out) {
         if (e != hipSuccess) return bail(ipl::Status::err(IPL_ERR_HIP, std::string("HIP setup: ") + hipGetErrorString(e)));
     }
     ipl::HostState hs;
-    ipl::Status st = ipl::make_host_state(ctx->graph, ctx->cfg, init, hs);
+    ipl::Status st = ipl::make_host_state(ctx->graph, *cfg, init, hs);
     if (!st) return bail(st);
+    // engines are built from the caller's config: its geometry pointers are valid for this call
     std::string why;
-    const bool fused_ok = ipl::fused_supported(ctx->graph, ctx->cfg, why);
-    int path = ctx->cfg.path;
-    if (path == IPL_PATH_AUTO) path = fused_ok ? IPL_PATH_FUSED : IPL_PATH_PLAIN;
-    if (path == IPL_PATH_FUSED && !fused_ok) return bail(ipl::Status::err(IPL_ERR_UNSUPPORTED, "fused path: " + why));
-    if (path == IPL_PATH_FUSED) ctx->engine = ipl::make_fused_engine(ctx->graph, ctx->cfg, hs, ctx->stream, st);
-    else ctx->engine = ipl::make_plain_engine(ctx->graph, ctx->cfg, hs, ctx->stream, st);
-    if (!st || !ctx->engine) return bail(st ? ipl::Status::err(IPL_ERR_UNSUPPORTED, "no engine") : st);
+    const bool fused_ok = ipl::fused_supported(ctx->graph, *cfg, why);
+    const int req = cfg->path;
+    if (req == IPL_PATH_FUSED && !fused_ok) return bail(ipl::Status::err(IPL_ERR_UNSUPPORTED, "fused path: " + why));
+    if (req == IPL_PATH_FUSED || (req == IPL_PATH_AUTO && fused_ok)) {
+        ctx->engine = ipl::make_fused_engine(ctx->graph, *cfg, hs, ctx->stream, st);
+        if (!ctx->engine && req == IPL_PATH_FUSED) return bail(st);
+    }
+    if (!ctx->engine) ctx->engine = ipl::make_plain_engine(ctx->graph, *cfg, hs, ctx->stream, st);
+    if (!ctx->engine) return bail(st ? ipl::Status::err(IPL_ERR_UNSUPPORTED, "no engine") : st);
     *out = ctx;
     return IPL_OK;
 }

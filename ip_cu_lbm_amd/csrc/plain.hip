@@ -30,6 +30,14 @@ __constant__ int c_cym[27] = {0, 0, 4, 0, 2, 0, 0, 4, 2, 4, 2, 0, 0, 0, 0, 4, 4,
 
 template <typename T> __device__ inline void atomic_add(T* a, T v) { atomicAdd(a, v); }
 
+// position difference a - b along an axis, adding the period across a periodic seam
+template <typename T>
+__device__ inline T dpos(const PlainView<T>& s, const Params& P, int axis, int a, int b) {
+    T d = s.pos[axis][a] - s.pos[axis][b];
+    if (P.periodic[axis] && d <= T(0)) d += T(P.period[axis]);
+    return d;
+}
+
 // ---- equ_f (collision.cu:3-14) -----------------------------------------------------------
 template <typename T, int Q>
 __global__ void k_equ_f(PlainView<T> s, Params P) {
@@ -58,22 +66,21 @@ __global__ void k_force(PlainView<T> s, Params P) {
     const int nu = s.neib[up * Nt + i], nd = s.neib[down * Nt + i];
     const T eps = T(P.dz) / T(1000);
     const T* ph = s.phi; const T* rh = s.rho;
-    const T* px = s.pos[0]; const T* py = s.pos[1]; const T* pz = s.pos[2];
-    T gx = (ph[n1] - ph[n3]) / (px[n1] - px[n3] + eps);
+    T gx = (ph[n1] - ph[n3]) / (dpos(s, P, 0, n1, n3) + eps);
     T gy = 0;
-    if (d3) gy = (ph[n2] - ph[n4]) / (py[n2] - py[n4] + eps);
-    T gz = (ph[nu] - ph[nd]) / (pz[nu] - pz[nd] + eps);
-    const T wdx = s.del[0][i], wd2 = s.del[d3 ? 1 : 2][i];
-    const T lap = (ph[n1] - T(2) * ph[i] + ph[n3]) / (wdx * wdx) + (ph[n2] - T(2) * ph[i] + ph[n4]) / (wd2 * wd2);
+    if (d3) gy = (ph[n2] - ph[n4]) / (dpos(s, P, 1, n2, n4) + eps);
+    T gz = (ph[nu] - ph[nd]) / (dpos(s, P, 2, nu, nd) + eps);
+    const T wdx = s.del[0][i], dz = P.dz;
+    const T lap = (ph[n1] - T(2) * ph[i] + ph[n3]) / (wdx * wdx) + (ph[n2] - T(2) * ph[i] + ph[n4]) / (dz * dz);
     const T beta = T(12.0) * T(P.sigma) / T(P.pf_thickness), kap = T(3.0) * T(P.sigma) * T(P.pf_thickness) / T(2.0);
     const T st = T(4) * beta * ph[i] * (ph[i] - T(1.0)) * (ph[i] - T(0.5)) - kap * lap;
     T Fx = s.Fx[i] + st * gx, Fy = s.Fy[i] + st * gy, Fz = s.Fz[i] + st * gz;
     if (P.boussinesq) Fz += -s.S[i] * T(P.beta_s) * T(P.gravity);
     else Fz += -rh[i] * T(P.gravity);
-    gx = (rh[n1] - rh[n3]) / (px[n1] - px[n3] + eps);
+    gx = (rh[n1] - rh[n3]) / (dpos(s, P, 0, n1, n3) + eps);
     gy = 0;
-    if (d3) gy = (rh[n2] - rh[n4]) / (py[n2] - py[n4] + eps);
-    gz = (rh[nu] - rh[nd]) / (pz[nu] - pz[nd] + eps);
+    if (d3) gy = (rh[n2] - rh[n4]) / (dpos(s, P, 1, n2, n4) + eps);
+    gz = (rh[nu] - rh[nd]) / (dpos(s, P, 2, nu, nd) + eps);
     const T c = P.c, c2 = P.c * P.c;
     Fx += -s.p[i] * c2 / T(3.0) * gx;
     Fy += -s.p[i] * c2 / T(3.0) * gy;
@@ -182,10 +189,10 @@ __global__ void k_col_pf(PlainView<T> s, Params P) {
     const T* ph = s.phi;
     const int n1 = s.neib[1 * Nt + i], n3 = s.neib[3 * Nt + i], n2 = s.neib[2 * Nt + i], n4 = s.neib[4 * Nt + i];
     const int nu = s.neib[up * Nt + i], nd = s.neib[down * Nt + i];
-    const T gx = (ph[n1] - ph[n3]) / (s.pos[0][n1] - s.pos[0][n3] + eps);
+    const T gx = (ph[n1] - ph[n3]) / (dpos(s, P, 0, n1, n3) + eps);
     T gy = 0;
-    if (d3) gy = (ph[n2] - ph[n4]) / (s.pos[1][n2] - s.pos[1][n4] + eps);
-    const T gz = (ph[nu] - ph[nd]) / (s.pos[2][nu] - s.pos[2][nd] + eps);
+    if (d3) gy = (ph[n2] - ph[n4]) / (dpos(s, P, 1, n2, n4) + eps);
+    const T gz = (ph[nu] - ph[nd]) / (dpos(s, P, 2, nu, nd) + eps);
     const T u = s.u[i], v = s.v[i], w = s.w[i], phi = ph[i];
     const T nabu = (phi * u - s.phiold[i] * s.uold[i]) / T(P.dt);
     const T nabv = (phi * v - s.phiold[i] * s.vold[i]) / T(P.dt);

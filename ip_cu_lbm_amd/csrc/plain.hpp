@@ -17,6 +17,8 @@ struct Params {
     double rho_L, rho_H, beta_s, gravity, sigma, pf_thickness, pf_tau;
     int boussinesq, collision, scalar, wall_bc;
     int flux_mode, flux_axis, npts;
+    int periodic[3];
+    double period[3];
 };
 
 // Device pointers of the plain path.  Populations are [k][Ntot]; cell fields are [Ntot].

@@ -17,6 +17,7 @@ Params make_params(const ipl_config& cfg, const Graph& g) {
     P.boussinesq = cfg.boussinesq; P.collision = cfg.collision; P.scalar = cfg.scalar;
     P.wall_bc = cfg.wall_bc;
     P.flux_mode = g.flux_mode; P.flux_axis = g.flux_axis < 0 ? 0 : g.flux_axis; P.npts = g.npts;
+    for (int a = 0; a < 3; ++a) { P.periodic[a] = cfg.periodic[a]; P.period[a] = g.period[a]; }
     return P;
 }
 

@@ -511,6 +511,14 @@ void equ_f(Oracle& o) {
         }
 }
 
+// position difference a - b along axis; across a periodic seam the period is added
+// (builder extension: the reference has no periodic axes)
+inline double dpos(const Oracle& o, int axis, int a, int b) {
+    double d = o.pos[axis][a] - o.pos[axis][b];
+    if (o.cfg.periodic[axis] && d <= 0) d += o.period[axis];
+    return d;
+}
+
 // Force (collision.cu:57-112): surface tension + buoyancy + pressure-gradient term into
 // Fx,Fy,Fz (accumulating), then Fk = w dt (c.F) / (rho c^2 / 3).
 void force(Oracle& o) {
@@ -518,29 +526,26 @@ void force(Oracle& o) {
     const int up = o.dim == 3 ? 5 : 2, down = o.dim == 3 ? 6 : 4;
     const double eps = o.dz / 1000;
     const double beta = 12.0 * o.cfg.sigma / o.cfg.pf_thickness, kap = 3.0 * o.cfg.sigma * o.cfg.pf_thickness / 2.0;
-    const int lap2_axis = o.dim == 3 ? 1 : 2;   // axis of directions 2/4
     #pragma omp parallel for schedule(static)
     for (int id = 0; id < o.N; ++id) {
         const int* nb = &o.neib[(size_t)id * Q];
-        const double* px = o.pos[0].data(); const double* py = o.pos[1].data(); const double* pz = o.pos[2].data();
         const double* ph = o.phi.data(); const double* rh = o.rho.data();
-        double nx_ = (ph[nb[1]] - ph[nb[3]]) / (px[nb[1]] - px[nb[3]] + eps);
+        double nx_ = (ph[nb[1]] - ph[nb[3]]) / (dpos(o, 0, nb[1], nb[3]) + eps);
         double ny_ = 0;
-        if (o.dim == 3) ny_ = (ph[nb[2]] - ph[nb[4]]) / (py[nb[2]] - py[nb[4]] + eps);
-        double nz_ = (ph[nb[up]] - ph[nb[down]]) / (pz[nb[up]] - pz[nb[down]] + eps);
-        // the reference divides the 2/4 term by dz^2 (collision.cu:70-71); the width of that
-        // axis equals dz on every reference grid
-        const double wd2 = o.del[lap2_axis][id];
+        if (o.dim == 3) ny_ = (ph[nb[2]] - ph[nb[4]]) / (dpos(o, 1, nb[2], nb[4]) + eps);
+        double nz_ = (ph[nb[up]] - ph[nb[down]]) / (dpos(o, 2, nb[up], nb[down]) + eps);
+        // x term over delX^2, the 2/4 term over dz^2 as written in the reference
+        // (collision.cu:70-71), also where 2/4 are the y directions of a 3D lattice
         const double lap = (ph[nb[1]] - 2 * ph[id] + ph[nb[3]]) / (o.del[0][id] * o.del[0][id]) +
-                           (ph[nb[2]] - 2 * ph[id] + ph[nb[4]]) / (wd2 * wd2);
+                           (ph[nb[2]] - 2 * ph[id] + ph[nb[4]]) / (o.dz * o.dz);
         const double st = 4 * beta * ph[id] * (ph[id] - 1.0) * (ph[id] - 0.5) - kap * lap;
         o.Fx[id] += st * nx_; o.Fy[id] += st * ny_; o.Fz[id] += st * nz_;
         if (o.cfg.boussinesq) o.Fz[id] += -o.S[id] * o.cfg.beta_s * o.cfg.gravity;
         else o.Fz[id] += -o.rho[id] * o.cfg.gravity;
-        nx_ = (rh[nb[1]] - rh[nb[3]]) / (px[nb[1]] - px[nb[3]] + eps);
+        nx_ = (rh[nb[1]] - rh[nb[3]]) / (dpos(o, 0, nb[1], nb[3]) + eps);
         ny_ = 0;
-        if (o.dim == 3) ny_ = (rh[nb[2]] - rh[nb[4]]) / (py[nb[2]] - py[nb[4]] + eps);
-        nz_ = (rh[nb[up]] - rh[nb[down]]) / (pz[nb[up]] - pz[nb[down]] + eps);
+        if (o.dim == 3) ny_ = (rh[nb[2]] - rh[nb[4]]) / (dpos(o, 1, nb[2], nb[4]) + eps);
+        nz_ = (rh[nb[up]] - rh[nb[down]]) / (dpos(o, 2, nb[up], nb[down]) + eps);
         o.Fx[id] += -o.p[id] * c2 / 3.0 * nx_;
         o.Fy[id] += -o.p[id] * c2 / 3.0 * ny_;
         o.Fz[id] += -o.p[id] * c2 / 3.0 * nz_;
@@ -621,11 +626,10 @@ void col_PF(Oracle& o) {
     for (int id = 0; id < o.N; ++id) {
         const int* nb = &o.neib[(size_t)id * o.Q];
         const double* ph = o.phi.data();
-        const double* px = o.pos[0].data(); const double* py = o.pos[1].data(); const double* pz = o.pos[2].data();
-        const double nx_ = (ph[nb[1]] - ph[nb[3]]) / (px[nb[1]] - px[nb[3]] + eps);
+        const double nx_ = (ph[nb[1]] - ph[nb[3]]) / (dpos(o, 0, nb[1], nb[3]) + eps);
         double ny_ = 0;
-        if (o.dim == 3) ny_ = (ph[nb[2]] - ph[nb[4]]) / (py[nb[2]] - py[nb[4]] + eps);
-        const double nz_ = (ph[nb[up]] - ph[nb[down]]) / (pz[nb[up]] - pz[nb[down]] + eps);
+        if (o.dim == 3) ny_ = (ph[nb[2]] - ph[nb[4]]) / (dpos(o, 1, nb[2], nb[4]) + eps);
+        const double nz_ = (ph[nb[up]] - ph[nb[down]]) / (dpos(o, 2, nb[up], nb[down]) + eps);
         const double nabu = (ph[id] * o.u[id] - o.phiold[id] * o.uold[id]) / o.dt;
         const double nabv = (ph[id] * o.v[id] - o.phiold[id] * o.vold[id]) / o.dt;
         const double nabw = (ph[id] * o.w[id] - o.phiold[id] * o.wold[id]) / o.dt;

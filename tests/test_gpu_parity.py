@@ -93,3 +93,62 @@ def test_nonfinite_detected():
         with pytest.raises(_abi.IplError) as ei:
             sim.step(1)
         assert ei.value.code == _abi.IPL_ERR_NONFINITE
+
+
+# ---- fused path ---------------------------------------------------------------------------
+from ip_cu_lbm_amd import F16, PATH_FUSED  # noqa: E402
+
+
+def test_fused_taylor_green_fp64(oracle_lib):
+    case = cases.lattice_taylor_green(nx=64, nz=32, path=PATH_FUSED)
+    _compare(case, 300, FIELDS_FLOW, 1e-12, oracle_lib)
+
+
+def test_fused_taylor_green_full_fp64(oracle_lib):
+    case = cases.lattice_taylor_green(path=PATH_FUSED)
+    _compare(case, 1000, FIELDS_FLOW, 1e-12, oracle_lib)
+
+
+@pytest.mark.parametrize("bc", [_abi.BC_SLIP, _abi.BC_BOUNCE_BACK])
+def test_fused_walls_on_stream_axis_fp64(oracle_lib, bc):
+    case = cases.lattice_taylor_green(nx=40, nz=24, path=PATH_FUSED).replace(periodic=(0, 0, 0), wall_bc=bc)
+    _compare(case, 200, FIELDS_FLOW, 1e-12, oracle_lib)
+
+
+def test_fused_three_point_fp64(oracle_lib):
+    case = cases.lattice_taylor_green(nx=48, nz=16, stencil=3, path=PATH_FUSED)
+    _compare(case, 200, FIELDS_FLOW, 1e-12, oracle_lib)
+
+
+@pytest.mark.parametrize("scalar_lattice,bc", [(7, _abi.BC_SLIP), (0, _abi.BC_BOUNCE_BACK)])
+def test_fused_lock_exchange_fp64(oracle_lib, scalar_lattice, bc):
+    case = cases.lock_exchange(nx=32, ny=6, nz=16, scalar_lattice=scalar_lattice, path=PATH_FUSED).replace(wall_bc=bc)
+    _compare(case, 150, ("p", "u", "v", "w", "S"), 1e-12, oracle_lib)
+
+
+def test_fused_uniform_3d_no_flux_fp64(oracle_lib):
+    nx, ny, nz = 12, 10, 16
+    px, wx = cases.axis_from_widths(np.ones(nx))
+    py, wy = cases.axis_from_widths(np.ones(ny))
+    pz, wz = cases.axis_from_widths(np.ones(nz))
+    case = cases.Case(name="uniform_q19", lattice=19, n=(nx, ny, nz), pos=[px, py, pz], width=[wx, wy, wz],
+                      periodic=(1, 0, 0), wall_bc=_abi.BC_BOUNCE_BACK, flux_axes=_abi.FLUX_NONE, tau=0.7,
+                      gravity=1e-5, boussinesq=0, path=PATH_FUSED, s_rates=np.full(27, 1.1))
+    X, Y, Z = case.coords()
+    case.init = {"u": 0.01 * np.sin(2 * np.pi * X / nx), "p": cases.uniform(3, case.n_cells, -1e-3, 1e-3)}
+    _compare(case, 100, ("p", "u", "v", "w"), 1e-12, oracle_lib)
+
+
+def test_fused_solitary_wave_fp64(oracle_lib):
+    case = cases.solitary_wave(nx=96, nz=48, x0=30, sigma=1e-4, path=PATH_FUSED)
+    _compare(case, 150, ("p", "u", "w", "phi"), 1e-12, oracle_lib)
+
+
+def test_fused_lock_exchange_fp32(oracle_lib):
+    case = cases.lock_exchange(nx=32, ny=6, nz=16, storage=F32, path=PATH_FUSED)
+    _compare(case, 150, ("u", "w", "S"), 1e-4, oracle_lib)
+
+
+def test_fused_lock_exchange_fp16(oracle_lib):
+    case = cases.lock_exchange(nx=32, ny=6, nz=16, storage=F16, path=PATH_FUSED)
+    _compare(case, 150, ("u", "w", "S"), 2e-2, oracle_lib)
