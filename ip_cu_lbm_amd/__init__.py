@@ -33,8 +33,8 @@ def _dp(a):
 class Simulation:
     """One simulation context on the current GPU (ipl_create ... ipl_destroy)."""
 
-    def __init__(self, case: "cases.Case"):
-        self._lib = _abi.load_library()
+    def __init__(self, case: "cases.Case", library: str = None):
+        self._lib = _abi.load_library(library)
         self.case = case
         cfg, ini, keep = case.to_c()
         h = ctypes.c_void_p()

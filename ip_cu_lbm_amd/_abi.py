@@ -63,15 +63,15 @@ def lib_path() -> str:
     return os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libipl.so")
 
 
-_LIB = None
+_LIBS = {}
 
 
-def load_library():
-    """Load the HIP library.  Raises if it has not been built: there is no CPU fallback."""
-    global _LIB
-    if _LIB is not None:
-        return _LIB
-    path = lib_path()
+def load_library(path: str = None):
+    """Load the HIP library.  Raises if it has not been built: there is no CPU fallback.
+    `path` lets the test suite load the CPU emulation build of the same sources."""
+    path = path or lib_path()
+    if path in _LIBS:
+        return _LIBS[path]
     if not os.path.exists(path):
         raise RuntimeError(
             f"HIP library {path} is not built; run `python -m ip_cu_lbm_amd.build` "
@@ -102,7 +102,7 @@ def load_library():
     lib.ipl_stage.restype = c_int32
     lib.ipl_build_tables.argtypes = [POINTER(IplConfig), c_int32, c_void_p, POINTER(c_int64)]
     lib.ipl_build_tables.restype = c_int32
-    _LIB = lib
+    _LIBS[path] = lib
     return lib
 
 

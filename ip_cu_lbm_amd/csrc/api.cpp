@@ -1,5 +1,5 @@
 // api.cpp - the extern "C" boundary (include/ipl.h).
-#include <hip/hip_runtime.h>
+#include "hip_compat.hpp"
 
 #include <cmath>
 #include <cstring>

@@ -1,7 +1,7 @@
 // engine.hpp - the per-context engine interface behind the C ABI.
 #pragma once
 
-#include <hip/hip_runtime.h>
+#include "hip_compat.hpp"
 
 #include <cstdint>
 #include <memory>

@@ -28,8 +28,7 @@
 // The macroscopic fields of step n are recomputed from f^n, g^n and the force of step n-1,
 // which is all update_rho needs (collision.cu:238-257); a small pass before the step writes
 // rho (and phi) for the gradients of Force and col_PF when they are not uniform.
-#include <hip/hip_runtime.h>
-#include <hip/hip_fp16.h>
+#include "hip_compat.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -288,8 +287,7 @@ __device__ inline MT grad_axis(const FGeom& G, const FTables<MT>& tb, const MT* 
 
 template <typename ST, typename MT, int Q, int QS, int NPTS>
 __global__ void __launch_bounds__(1024) k_fused_step(FGeom G, FTables<MT> tb, FBufs<ST, MT> b, FParams P) {
-    extern __shared__ unsigned char smem_raw[];
-    MT* lds_f = reinterpret_cast<MT*>(smem_raw);
+    IPL_DYNAMIC_LDS(MT, lds_f);
     MT* lds_g = lds_f + Q * G.ns;
     const int i = threadIdx.x;
     const bool active = i < G.ns;
@@ -668,7 +666,7 @@ public:
         for (int k = 0; k < q; ++k) {
             for (int64_t id = 0; id < N; ++id) h[perm_[id]] = src[(size_t)k * Nt + id];
             IPL_HIP_TRY(hipMemcpyAsync(tmp, h.data(), NC * sizeof(double), hipMemcpyHostToDevice, stream));
-            k_store_pops<ST, MT><<<(NC + 255) / 256, 256, 0, stream>>>(geo_, tmp, dst, k, q);
+            IPL_LAUNCH((k_store_pops<ST, MT>), (NC + 255) / 256, 256, 0, stream, geo_, tmp, dst, k, q);
             IPL_HIP_TRY(hipStreamSynchronize(stream));
         }
         IPL_HIP_TRY(hipFree(tmp));
@@ -683,9 +681,9 @@ public:
         b.rho_n = rho_n_; b.phi_n = phi_n_;
         b.phiold = phiold_; b.uold = uold_; b.vold = vold_; b.wold = wold_;
         if (geo_.need_grad)
-            k_scalar_fields<ST, MT, QS><<<(geo_.ncell + 255) / 256, 256, 0, stream>>>(geo_, ga_, rho_n_, phi_n_, P_);
-        hipLaunchKernelGGL((k_fused_step<ST, MT, Q, QS, NPTS>), dim3((unsigned)geo_.lines), dim3(block_),
-                           lds_bytes_, stream, geo_, tb_, b, P_);
+            IPL_LAUNCH((k_scalar_fields<ST, MT, QS>), (geo_.ncell + 255) / 256, 256, 0, stream, geo_, ga_, rho_n_, phi_n_, P_);
+        IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NPTS>), (unsigned)geo_.lines, block_, lds_bytes_, stream,
+                        geo_, tb_, b, P_);
         std::swap(fa_, fb_);
         std::swap(ga_, gb_);
         std::swap(Fa_, Fb_);
@@ -715,7 +713,7 @@ public:
 
     template <int Q, int QS>
     void launch_readout() {
-        k_readout<ST, MT, Q, QS><<<(geo_.ncell + 255) / 256, 256, 0, stream>>>(geo_, fa_, ga_, Fa_, out_, P_);
+        IPL_LAUNCH((k_readout<ST, MT, Q, QS>), (geo_.ncell + 255) / 256, 256, 0, stream, geo_, fa_, ga_, Fa_, out_, P_);
     }
 
     Status readout() {
@@ -734,7 +732,7 @@ public:
             const int q = field == IPL_FIELD_F ? gr_.Q : gr_.Qs;
             if (!q || count != (size_t)q * N) return Status::err(IPL_ERR_ARGUMENT, "population read size mismatch");
             for (int k = 0; k < q; ++k) {
-                k_convert_pops<ST, MT><<<(NC + 255) / 256, 256, 0, stream>>>(geo_, field == IPL_FIELD_F ? fa_ : ga_, out_, k, q);
+                IPL_LAUNCH((k_convert_pops<ST, MT>), (NC + 255) / 256, 256, 0, stream, geo_, field == IPL_FIELD_F ? fa_ : ga_, out_, k, q);
                 IPL_HIP_TRY(hipMemcpyAsync(h.data(), out_, NC * sizeof(double), hipMemcpyDeviceToHost, stream));
                 IPL_HIP_TRY(hipStreamSynchronize(stream));
                 for (int64_t id = 0; id < N; ++id) host[(size_t)k * N + id] = h[perm_[id]];
@@ -780,7 +778,7 @@ public:
             for (int k = 0; k < q; ++k) {
                 for (int64_t id = 0; id < N; ++id) h[perm_[id]] = host[(size_t)k * N + id];
                 IPL_HIP_TRY(hipMemcpyAsync(out_, h.data(), NC * sizeof(double), hipMemcpyHostToDevice, stream));
-                k_store_pops<ST, MT><<<(NC + 255) / 256, 256, 0, stream>>>(geo_, out_, field == IPL_FIELD_F ? fa_ : ga_, k, q);
+                IPL_LAUNCH((k_store_pops<ST, MT>), (NC + 255) / 256, 256, 0, stream, geo_, out_, field == IPL_FIELD_F ? fa_ : ga_, k, q);
                 IPL_HIP_TRY(hipStreamSynchronize(stream));
             }
             return Status::ok();
@@ -795,8 +793,8 @@ public:
         double res[3];
         const int slots[3] = {0, 5, 6};
         for (int q = 0; q < 3; ++q) {
-            k_wsum<MT><<<1024, 256, 0, stream>>>(out_ + (size_t)slots[q] * geo_.ncell, 1, geo_.ncell, vol_, partial_);
-            k_wsum_final<<<1, 256, 0, stream>>>(partial_, partial_ + 3 * 1024);
+            IPL_LAUNCH_SYNC((k_wsum<MT>), 1024, 256, 0, stream, out_ + (size_t)slots[q] * geo_.ncell, 1, geo_.ncell, vol_, partial_);
+            IPL_LAUNCH_SYNC(k_wsum_final, 1, 256, 0, stream, partial_, partial_ + 3 * 1024);
             IPL_HIP_TRY(hipMemcpyAsync(&res[q], partial_ + 3 * 1024, sizeof(double), hipMemcpyDeviceToHost, stream));
             IPL_HIP_TRY(hipStreamSynchronize(stream));
         }
@@ -867,7 +865,12 @@ std::unique_ptr<Engine> make_fused_engine(const Graph& g, const ipl_config& cfg,
     };
     if (cfg.storage == IPL_F64) return make((FusedEngine<double, double>*)nullptr);
     if (cfg.storage == IPL_F32) return make((FusedEngine<float, float>*)nullptr);
-    if (cfg.storage == IPL_F16) return make((FusedEngine<__half, float>*)nullptr);
+    if (cfg.storage == IPL_F16) {
+        // FP16 storage is switched off: its first GPU run (D3Q19 + D3Q7 lock exchange) ended in
+        // a device memory fault whose cause is not yet shown; the CPU build of this code is clean.
+        status = Status::err(IPL_ERR_UNSUPPORTED, "FP16 storage is disabled pending a GPU fault analysis");
+        return nullptr;
+    }
     status = Status::err(IPL_ERR_ARGUMENT, "storage type");
     return nullptr;
 }

@@ -4,7 +4,7 @@
 // reference's stage semantics exactly (including its ghost copies and scratch aliasing inside
 // IP_process, adv.cu:245-257) so it is bitwise-comparable with the FP64 oracle up to
 // reassociation, and it is the "plain FP64" baseline of the fused path.
-#include <hip/hip_runtime.h>
+#include "hip_compat.hpp"
 
 #include "lattice.hpp"
 #include "plain.hpp"
@@ -513,18 +513,18 @@ template <typename T, int QN, int QF>
 void ip_process(const PlainView<T>& s, const Pops<T>& b, const Params& P, hipStream_t st) {
     const int64_t G = s.Ntot - s.N;
     const unsigned bn = blocks_for(s.N), bg = blocks_for(G > 0 ? G : 1);
-    if (G > 0) k_set_wall_f<T, QF><<<bg, kBlock, 0, st>>>(s, b);
-    k_set_out<T, QN, QF><<<bn, kBlock, 0, st>>>(s, b, P);
-    if (G > 0 && P.flux_mode == FLUX_MODE_CORNER) k_set_fin<T, QF><<<bg, kBlock, 0, st>>>(s, b);
-    k_reset_f<T, QF><<<bn, kBlock, 0, st>>>(s, b);
+    if (G > 0) IPL_LAUNCH((k_set_wall_f<T, QF>), bg, kBlock, 0, st, s, b);
+    IPL_LAUNCH((k_set_out<T, QN, QF>), bn, kBlock, 0, st, s, b, P);
+    if (G > 0 && P.flux_mode == FLUX_MODE_CORNER) IPL_LAUNCH((k_set_fin<T, QF>), bg, kBlock, 0, st, s, b);
+    IPL_LAUNCH((k_reset_f<T, QF>), bn, kBlock, 0, st, s, b);
     if (G > 0) {
-        k_set_wall_f_inv<T, QN, QF><<<bg, kBlock, 0, st>>>(s, b, b.f, P);
-        k_set_wall_f_inv<T, QN, QF><<<bg, kBlock, 0, st>>>(s, b, b.fout, P);
+        IPL_LAUNCH((k_set_wall_f_inv<T, QN, QF>), bg, kBlock, 0, st, s, b, b.f, P);
+        IPL_LAUNCH((k_set_wall_f_inv<T, QN, QF>), bg, kBlock, 0, st, s, b, b.fout, P);
     }
-    k_set_tmp<T, QN, QF><<<bn, kBlock, 0, st>>>(s, b, P);
-    if (G > 0) k_set_wall_propagation<T, QF><<<bg, kBlock, 0, st>>>(s, b, P);
-    k_propagation<T, QF><<<bn, kBlock, 0, st>>>(s, b);
-    if (G > 0) k_reset_wall_fin<T, QF><<<bg, kBlock, 0, st>>>(s, b);
+    IPL_LAUNCH((k_set_tmp<T, QN, QF>), bn, kBlock, 0, st, s, b, P);
+    if (G > 0) IPL_LAUNCH((k_set_wall_propagation<T, QF>), bg, kBlock, 0, st, s, b, P);
+    IPL_LAUNCH((k_propagation<T, QF>), bn, kBlock, 0, st, s, b);
+    if (G > 0) IPL_LAUNCH((k_reset_wall_fin<T, QF>), bg, kBlock, 0, st, s, b);
 }
 
 template <typename T, int Q, int QS>
@@ -533,9 +533,9 @@ int run_stage(const PlainView<T>& s, const Params& P, int stage, hipStream_t st)
     const int64_t G = s.Ntot - s.N;
     const unsigned bg = blocks_for(G > 0 ? G : 1);
     switch (stage) {
-    case IPL_STAGE_EQU_F: k_equ_f<T, Q><<<bn, kBlock, 0, st>>>(s, P); break;
-    case IPL_STAGE_FORCE: k_force<T, Q><<<bn, kBlock, 0, st>>>(s, P); break;
-    case IPL_STAGE_COLLIDE: k_collide<T, Q><<<bn, kBlock, 0, st>>>(s, P); break;
+    case IPL_STAGE_EQU_F: IPL_LAUNCH((k_equ_f<T, Q>), bn, kBlock, 0, st, s, P); break;
+    case IPL_STAGE_FORCE: IPL_LAUNCH((k_force<T, Q>), bn, kBlock, 0, st, s, P); break;
+    case IPL_STAGE_COLLIDE: IPL_LAUNCH((k_collide<T, Q>), bn, kBlock, 0, st, s, P); break;
     case IPL_STAGE_IP_F: {
         Pops<T> b{s.f, s.ftmp, s.fout, s.feq, s.nextB, s.nextK};
         ip_process<T, Q, Q>(s, b, P, st);
@@ -543,8 +543,8 @@ int run_stage(const PlainView<T>& s, const Params& P, int stage, hipStream_t st)
     }
     case IPL_STAGE_SCALAR_COLLIDE:
         if constexpr (QS > 0) {
-            if (P.scalar == IPL_SCALAR_SALINITY) k_col_g_reg<T, QS><<<bn, kBlock, 0, st>>>(s, P);
-            else if constexpr (QS == Q) k_col_pf<T, Q><<<bn, kBlock, 0, st>>>(s, P);
+            if (P.scalar == IPL_SCALAR_SALINITY) IPL_LAUNCH((k_col_g_reg<T, QS>), bn, kBlock, 0, st, s, P);
+            else if constexpr (QS == Q) IPL_LAUNCH((k_col_pf<T, Q>), bn, kBlock, 0, st, s, P);
         }
         break;
     case IPL_STAGE_IP_G:
@@ -555,16 +555,16 @@ int run_stage(const PlainView<T>& s, const Params& P, int stage, hipStream_t st)
         break;
     case IPL_STAGE_UPDATE_SCALAR:
         if constexpr (QS > 0)
-            k_update_scalar<T, QS><<<bn, kBlock, 0, st>>>(s, P.scalar == IPL_SCALAR_SALINITY ? s.S : s.phi);
+            IPL_LAUNCH((k_update_scalar<T, QS>), bn, kBlock, 0, st, s, P.scalar == IPL_SCALAR_SALINITY ? s.S : s.phi);
         break;
-    case IPL_STAGE_UPDATE_RHO: k_update_rho<T, Q><<<bn, kBlock, 0, st>>>(s, P); break;
+    case IPL_STAGE_UPDATE_RHO: IPL_LAUNCH((k_update_rho<T, Q>), bn, kBlock, 0, st, s, P); break;
     case IPL_STAGE_SET_WALL_RHO:
         if (G > 0) {
-            k_set_wall_field<T><<<bg, kBlock, 0, st>>>(s, s.rho);
-            if (P.scalar == IPL_SCALAR_PHASE_FIELD) k_set_wall_field<T><<<bg, kBlock, 0, st>>>(s, s.phi);
+            IPL_LAUNCH((k_set_wall_field<T>), bg, kBlock, 0, st, s, s.rho);
+            if (P.scalar == IPL_SCALAR_PHASE_FIELD) IPL_LAUNCH((k_set_wall_field<T>), bg, kBlock, 0, st, s, s.phi);
         }
         break;
-    case IPL_STAGE_RESET_F: k_reset_force<T><<<bn, kBlock, 0, st>>>(s); break;
+    case IPL_STAGE_RESET_F: IPL_LAUNCH((k_reset_force<T>), bn, kBlock, 0, st, s); break;
     default: return IPL_ERR_ARGUMENT;
     }
     return IPL_OK;

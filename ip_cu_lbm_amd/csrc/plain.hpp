@@ -1,7 +1,7 @@
 // plain.hpp - device views and run parameters shared by the HIP kernels and the host API.
 #pragma once
 
-#include <hip/hip_runtime.h>
+#include "hip_compat.hpp"
 
 #include <cstdint>
 

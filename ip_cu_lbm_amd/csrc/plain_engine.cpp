@@ -1,5 +1,5 @@
 // plain_engine.cpp - host side of the plain path: buffers, uploads, the stage loop, readout.
-#include <hip/hip_runtime.h>
+#include "hip_compat.hpp"
 
 #include <cmath>
 #include <cstring>

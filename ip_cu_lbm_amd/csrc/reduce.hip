@@ -2,7 +2,7 @@
 // each a fixed-order shared-memory tree, then one block over the partials.  Used for the
 // volume-weighted conservation sums of output() (basic.cu:9-13) and the NaN check the
 // drivers do at save intervals (LockExchange.cu:259-261).
-#include <hip/hip_runtime.h>
+#include "hip_compat.hpp"
 
 #include "engine.hpp"
 
@@ -72,9 +72,9 @@ Status device_weighted_sums(const T* const* fields, int nfields, const T* wx, co
                             double inv_vol, double* out, hipStream_t st) {
     double* buf = nullptr;
     IPL_HIP_TRY(hipMallocAsync((void**)&buf, (3 * kRedGrid + 3) * sizeof(double), st));
-    k_partial_wsum<T><<<kRedGrid, kRedBlock, 0, st>>>(nfields > 0 ? fields[0] : nullptr, nfields > 1 ? fields[1] : nullptr,
+    IPL_LAUNCH_SYNC((k_partial_wsum<T>), kRedGrid, kRedBlock, 0, st, nfields > 0 ? fields[0] : nullptr, nfields > 1 ? fields[1] : nullptr,
                                                         nfields > 2 ? fields[2] : nullptr, wx, wy, wz, n, buf);
-    k_final_sum<<<1, kRedBlock, 0, st>>>(buf, 3, inv_vol, buf + 3 * kRedGrid);
+    IPL_LAUNCH_SYNC(k_final_sum, 1, kRedBlock, 0, st, buf, 3, inv_vol, buf + 3 * kRedGrid);
     double h[3];
     IPL_HIP_TRY(hipMemcpyAsync(h, buf + 3 * kRedGrid, sizeof(h), hipMemcpyDeviceToHost, st));
     IPL_HIP_TRY(hipFreeAsync(buf, st));
@@ -87,8 +87,8 @@ template <typename T>
 Status device_all_finite(const T* const* fields, int nfields, int64_t n, bool* finite, hipStream_t st) {
     int* flag = nullptr;
     IPL_HIP_TRY(hipMallocAsync((void**)&flag, sizeof(int), st));
-    k_zero_int<<<1, 1, 0, st>>>(flag);
-    k_nonfinite<T><<<kRedGrid, kRedBlock, 0, st>>>(nfields > 0 ? fields[0] : nullptr, nfields > 1 ? fields[1] : nullptr,
+    IPL_LAUNCH(k_zero_int, 1, 1, 0, st, flag);
+    IPL_LAUNCH((k_nonfinite<T>), kRedGrid, kRedBlock, 0, st, nfields > 0 ? fields[0] : nullptr, nfields > 1 ? fields[1] : nullptr,
                                                      nfields > 2 ? fields[2] : nullptr, nfields > 3 ? fields[3] : nullptr,
                                                      n, flag);
     int h = 0;

