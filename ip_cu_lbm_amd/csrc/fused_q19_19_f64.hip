@@ -1,0 +1,10 @@
+// fused_q19_19_f64.hip - fused-path instantiations: D3Q19 flow, D3Q19 scalar, FP64 storage.
+#include "fused_kernels.hpp"
+
+using ipl::fusedk::FBufs;
+using ipl::fusedk::FGeom;
+using ipl::fusedk::FParams;
+using ipl::fusedk::FTables;
+
+IPL_FUSED_INSTANTIATE(double, double, 19, 19)
+IPL_FUSED_INSTANTIATE_SCALAR(double, double, 19)

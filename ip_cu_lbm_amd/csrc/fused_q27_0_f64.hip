@@ -1,0 +1,10 @@
+// fused_q27_0_f64.hip - fused-path instantiations: D3Q27 flow, no scalar, FP64 storage.
+#include "fused_kernels.hpp"
+
+using ipl::fusedk::FBufs;
+using ipl::fusedk::FGeom;
+using ipl::fusedk::FParams;
+using ipl::fusedk::FTables;
+
+IPL_FUSED_INSTANTIATE(double, double, 27, 0)
+IPL_FUSED_INSTANTIATE_POPS(double, double, 27)

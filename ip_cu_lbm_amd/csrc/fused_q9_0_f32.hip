@@ -1,0 +1,12 @@
+// fused_q9_0_f32.hip - fused-path instantiations: D2Q9 flow, no scalar, FP32 storage.
+#include "fused_kernels.hpp"
+
+using ipl::fusedk::FBufs;
+using ipl::fusedk::FGeom;
+using ipl::fusedk::FParams;
+using ipl::fusedk::FTables;
+
+IPL_FUSED_INSTANTIATE(float, float, 9, 0)
+IPL_FUSED_INSTANTIATE_POPS(float, float, 9)
+IPL_FUSED_INSTANTIATE_SCALAR(float, float, 0)
+IPL_FUSED_INSTANTIATE_POPS(float, float, 7)

@@ -1,0 +1,12 @@
+// fused_q9_0_f64.hip - fused-path instantiations: D2Q9 flow, no scalar, FP64 storage.
+#include "fused_kernels.hpp"
+
+using ipl::fusedk::FBufs;
+using ipl::fusedk::FGeom;
+using ipl::fusedk::FParams;
+using ipl::fusedk::FTables;
+
+IPL_FUSED_INSTANTIATE(double, double, 9, 0)
+IPL_FUSED_INSTANTIATE_POPS(double, double, 9)
+IPL_FUSED_INSTANTIATE_SCALAR(double, double, 0)
+IPL_FUSED_INSTANTIATE_POPS(double, double, 7)

@@ -16,6 +16,13 @@ void launch(unsigned grid, unsigned block, const std::function<void()>& body) {
         }
 }
 
+// IPL_EMU_REVERSE=1 runs workgroups in reverse order: results that change expose two
+// workgroups writing one location (an order the GPU does not guarantee)
+static bool reverse_order() {
+    const char* e = std::getenv("IPL_EMU_REVERSE");
+    return e && e[0] == '1';
+}
+
 void launch_sync(unsigned grid, unsigned block, size_t lds, const std::function<void()>& body) {
     std::barrier<> bar((std::ptrdiff_t)block);
     // exact size: an LDS overrun is a heap overflow under AddressSanitizer
@@ -27,8 +34,9 @@ void launch_sync(unsigned grid, unsigned block, size_t lds, const std::function<
             blockDim.x = block; gridDim.x = grid; threadIdx.x = t;
             ipl_emu_barrier = &bar;
             ipl_emu_lds = lds ? mem.data() : nullptr;
-            for (unsigned b = 0; b < grid; ++b) {
-                blockIdx.x = b;
+            const bool rev = reverse_order();
+            for (unsigned bb = 0; bb < grid; ++bb) {
+                blockIdx.x = rev ? grid - 1 - bb : bb;
                 body();
                 bar.arrive_and_wait();  // the next workgroup reuses the LDS buffer
             }

@@ -1,7 +1,10 @@
-"""HIP library vs the FP64 oracle on the same seeded inputs (plain path).
+"""HIP library vs the FP64 oracle on the same seeded inputs.
 
-Tolerance: FP64 storage <= 1e-12 relative L2 on p, u, v, w, S and phi after N steps; FP32
-storage <= 5e-6 on the velocities of short runs.  Calls go through the C ABI."""
+Tolerance: FP64 storage <= 1e-12 relative L2 after N steps on p, S, phi and on the velocity
+vector field (u, v, w) - a single component of a nearly two-dimensional flow (the span-wise v of
+the lock exchange) is round-off noise, so velocity is compared as one vector field.  FP32
+storage: <= 1e-5 on velocity over these short runs (measured FP32-vs-FP64 drift ~1e-6).
+Calls go through the C ABI."""
 import numpy as np
 import pytest
 
@@ -10,7 +13,13 @@ from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-FIELDS_FLOW = ("p", "u", "w")
+FIELDS_FLOW = ("p", "vel")
+
+
+def _read(x, f):
+    if f == "vel":
+        return np.stack([x.read(c) for c in "uvw"])
+    return x.read(f)
 
 
 def _compare(case, steps, fields, tol, oracle_lib, report=None):
@@ -19,7 +28,7 @@ def _compare(case, steps, fields, tol, oracle_lib, report=None):
         assert sim.n_calc == o.n_calc and sim.n_ghost == o.n_ghost
         sim.step(steps)
         o.step(steps)
-        errs = {f: rel_l2(sim.read(f), o.read(f)) for f in fields}
+        errs = {f: rel_l2(_read(sim, f), _read(o, f)) for f in fields}
         sums = (sim.global_sums(), o.global_sums())
     print(case.name, steps, errs, flush=True)
     for f, e in errs.items():
@@ -50,12 +59,12 @@ def test_walls_slip_all_sides_fp64(oracle_lib):
 
 def test_lock_exchange_d3q19_d3q7_fp64(oracle_lib):
     case = cases.lock_exchange(nx=32, ny=6, nz=16, path=PATH_PLAIN)
-    _compare(case, 150, ("p", "u", "v", "w", "S"), 1e-12, oracle_lib)
+    _compare(case, 150, ("p", "vel", "S"), 1e-12, oracle_lib)
 
 
 def test_lock_exchange_same_q_salinity_bounce_back_fp64(oracle_lib):
     case = cases.lock_exchange(nx=24, ny=6, nz=12, scalar_lattice=0, path=PATH_PLAIN).replace(wall_bc=_abi.BC_BOUNCE_BACK)
-    _compare(case, 100, ("p", "u", "v", "w", "S"), 1e-12, oracle_lib)
+    _compare(case, 100, ("p", "vel", "S"), 1e-12, oracle_lib)
 
 
 @pytest.mark.parametrize("q,bc", [(27, _abi.BC_SLIP), (27, _abi.BC_BOUNCE_BACK), (19, _abi.BC_SLIP)])
@@ -70,17 +79,17 @@ def test_reference_corner_scheme_3d_fp64(oracle_lib, q, bc):
     X, Y, Z = case.coords()
     case.init = {"u": 0.01 * np.sin(X / 3.0) * np.cos(Z), "v": 0.01 * np.cos(Y / 2.0),
                  "p": cases.uniform(7, case.n_cells, -1e-3, 1e-3)}
-    _compare(case, 60, ("p", "u", "v", "w"), 1e-12, oracle_lib)
+    _compare(case, 60, ("p", "vel"), 1e-12, oracle_lib)
 
 
 def test_solitary_wave_phase_field_fp64(oracle_lib):
     case = cases.solitary_wave(nx=96, nz=48, x0=30, sigma=1e-4, path=PATH_PLAIN)
-    _compare(case, 150, ("p", "u", "w", "phi"), 1e-12, oracle_lib)
+    _compare(case, 150, ("p", "vel", "phi"), 1e-12, oracle_lib)
 
 
 def test_taylor_green_fp32_storage(oracle_lib):
     case = cases.lattice_taylor_green(nx=64, nz=32, storage=F32, path=PATH_PLAIN)
-    _compare(case, 200, ("u", "w"), 5e-6, oracle_lib)
+    _compare(case, 200, ("vel",), 1e-5, oracle_lib)
 
 
 def test_nonfinite_detected():
@@ -123,7 +132,7 @@ def test_fused_three_point_fp64(oracle_lib):
 @pytest.mark.parametrize("scalar_lattice,bc", [(7, _abi.BC_SLIP), (0, _abi.BC_BOUNCE_BACK)])
 def test_fused_lock_exchange_fp64(oracle_lib, scalar_lattice, bc):
     case = cases.lock_exchange(nx=32, ny=6, nz=16, scalar_lattice=scalar_lattice, path=PATH_FUSED).replace(wall_bc=bc)
-    _compare(case, 150, ("p", "u", "v", "w", "S"), 1e-12, oracle_lib)
+    _compare(case, 150, ("p", "vel", "S"), 1e-12, oracle_lib)
 
 
 def test_fused_uniform_3d_no_flux_fp64(oracle_lib):
@@ -136,20 +145,34 @@ def test_fused_uniform_3d_no_flux_fp64(oracle_lib):
                       gravity=1e-5, boussinesq=0, path=PATH_FUSED, s_rates=np.full(27, 1.1))
     X, Y, Z = case.coords()
     case.init = {"u": 0.01 * np.sin(2 * np.pi * X / nx), "p": cases.uniform(3, case.n_cells, -1e-3, 1e-3)}
-    _compare(case, 100, ("p", "u", "v", "w"), 1e-12, oracle_lib)
+    _compare(case, 100, ("p", "vel"), 1e-12, oracle_lib)
 
 
 def test_fused_solitary_wave_fp64(oracle_lib):
     case = cases.solitary_wave(nx=96, nz=48, x0=30, sigma=1e-4, path=PATH_FUSED)
-    _compare(case, 150, ("p", "u", "w", "phi"), 1e-12, oracle_lib)
+    _compare(case, 150, ("p", "vel", "phi"), 1e-12, oracle_lib)
 
 
 def test_fused_lock_exchange_fp32(oracle_lib):
     case = cases.lock_exchange(nx=32, ny=6, nz=16, storage=F32, path=PATH_FUSED)
-    _compare(case, 150, ("u", "w", "S"), 1e-4, oracle_lib)
+    _compare(case, 150, ("vel", "S"), 1e-5, oracle_lib)
+
+
+def test_fused_taylor_green_fp32(oracle_lib):
+    case = cases.lattice_taylor_green(nx=64, nz=32, storage=F32, path=PATH_FUSED)
+    _compare(case, 200, ("vel",), 1e-5, oracle_lib)
+
+
+def test_fused_matches_plain_fp64_lock_exchange():
+    base = cases.lock_exchange(nx=32, ny=6, nz=16)
+    with Simulation(base.replace(path=PATH_PLAIN)) as a, Simulation(base.replace(path=PATH_FUSED)) as b:
+        a.step(100)
+        b.step(100)
+        assert rel_l2(_read(b, "vel"), _read(a, "vel")) < 1e-12
+        assert rel_l2(b.read("S"), a.read("S")) < 1e-12
 
 
 @pytest.mark.skip(reason="FP16 storage is switched off: its first GPU run faulted and the cause is not yet shown")
 def test_fused_lock_exchange_fp16(oracle_lib):
     case = cases.lock_exchange(nx=32, ny=6, nz=16, storage=F16, path=PATH_FUSED)
-    _compare(case, 150, ("u", "w", "S"), 2e-2, oracle_lib)
+    _compare(case, 150, ("vel", "S"), 2e-2, oracle_lib)
