@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the interpolation-based lattice Boltzmann step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload: the stratified lock exchange, D3Q19 MRT flow + D3Q7 salinity with Boussinesq
+buoyancy, 512 x 32 x 64 cells, z stretched by 1.02 toward mid-depth (ip_cu_lbm_amd.cases.
+lock_exchange), synthetic seeded initial fields.  The headline value is the MLUPS of the fused
+step with FP32 populations (FP32 arithmetic); the plain FP64 path (one kernel per reference
+stage) and the fused FP64 path are measured beside it.  Inputs are resident in HBM before the
+timed region; timing uses HIP events on the library's stream, K steps between two
+synchronisations.
+
+The roofline object is for the dominant kernel (the fused step): algorithmic bytes per launch =
+2 x (19 + 7) populations x 4 B per cell (each population read once and written once) x cells,
+divided by the kernel's mean launch duration from per-launch HIP events; peak = 8 TB/s HBM.
+"traffic" is the measured HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE when a
+counter summary for this workload exists under profiles/ (see --pmc-summary), else null.
+
+The CPU baseline is the FP64 oracle (a C++/OpenMP restatement of the reference kernels) on a
+bounded number of steps of the same workload, on the host's cores.
+
+Only one GPU is used; --gpus above 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBPS = 8000.0
+WORKLOAD = "lock_exchange_512x32x64_d3q19_d3q7"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", f"traffic_{WORKLOAD}_fp32.json")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def measure(case, steps, warmup, profile_steps=0):
+    """MLUPS of `steps` steps after `warmup` steps, plus per-launch profile."""
+    from ip_cu_lbm_amd import Simulation
+    with Simulation(case) as sim:
+        sim.step(warmup)
+        t0 = time.perf_counter()
+        ms = sim.step_timed(steps)
+        wall = time.perf_counter() - t0
+        prof = sim.profile_steps(profile_steps) if profile_steps else None
+        n = sim.n_calc
+        path = sim.path
+        sums = sim.global_sums()
+    return {"mlups": n * steps / (ms * 1e3), "ms_per_step": ms / steps, "wall_s": wall, "n_cells": n,
+            "path": path, "profile": prof, "sums": sums.tolist()}
+
+
+def cpu_baseline(case, seconds):
+    import oracle
+    oracle.build()
+    o = oracle.Oracle(case)
+    t1 = o.time_steps(1)
+    n = max(1, min(200, int(seconds / max(t1, 1e-3))))
+    t = o.time_steps(n)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(o.n_calc * n / t / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{n} steps of the {WORKLOAD} case in FP64 (the oracle), {t:.1f} s, "
+                      f"OpenMP threads={threads}"}
+
+
+def read_traffic():
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as fh:
+        d = json.load(fh)
+    return d.get("hbm_bytes_per_launch")
+
+
+def pmc_summary(fetch_dir, write_dir, cells):
+    """Turns two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) into the traffic
+    file, scaling each counter by its reading on the library's copy of a known-size buffer."""
+    import csv
+    import glob
+
+    def rows(d):
+        out = []
+        for fn in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(fn) as fh:
+                out += list(csv.DictReader(fh))
+        return out
+
+    def mean_for(rs, name_part, counter):
+        vals = [float(r["Counter_Value"]) for r in rs if name_part in r["Kernel_Name"] and r["Counter_Name"] == counter]
+        return sum(vals) / len(vals) if vals else None
+
+    fr, wr = rows(fetch_dir), rows(write_dir)
+    f_main = mean_for(fr, "k_fused_step", "FETCH_SIZE")
+    w_main = mean_for(wr, "k_fused_step", "WRITE_SIZE")
+    f_copy = mean_for(fr, "k_store_pops", "FETCH_SIZE")   # reads an FP64 row of `cells` values
+    w_copy = mean_for(wr, "k_store_pops", "WRITE_SIZE")   # writes an FP32 row of `cells` values
+    fs = (cells * 8 / 1024.0) / f_copy if f_copy else 1.0
+    ws = (cells * 4 / 1024.0) / w_copy if w_copy else 1.0
+    total = (f_main * fs + w_main * ws) * 1024.0
+    d = {"workload": WORKLOAD, "fetch_kib": f_main, "write_kib": w_main, "fetch_scale": fs, "write_scale": ws,
+         "hbm_bytes_per_launch": total, "algorithmic_bytes_per_launch": 2 * 26 * 4 * cells}
+    os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
+    with open(TRAFFIC_FILE, "w") as fh:
+        json.dump(d, fh, indent=1)
+    print(json.dumps(d))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 plain / fused comparison runs")
+    ap.add_argument("--only-headline", action="store_true", help="run just the headline workload (for profilers)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write the traffic file from two rocprofv3 --pmc output directories and exit")
+    a = ap.parse_args(argv)
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}, WORLD_SIZE {world}); "
+              "independent simulations shard by job, one process per GPU, with no collective in the step.",
+              file=sys.stderr)
+        return 2
+
+    from ip_cu_lbm_amd import F32, F64, PATH_FUSED, PATH_PLAIN, cases
+
+    if a.pmc_summary:
+        pmc_summary(a.pmc_summary[0], a.pmc_summary[1], 512 * 32 * 64)
+        return 0
+
+    head_case = cases.lock_exchange(storage=F32, path=PATH_FUSED)
+    log(f"[bench] headline: {WORKLOAD} fused FP32, warmup {a.warmup}, steps {a.steps}")
+    head = measure(head_case, a.steps, a.warmup, profile_steps=min(a.steps, 200))
+    cells = head["n_cells"]
+    b_alg = 2 * (19 + 7) * 4 * cells
+    main_ms = head["profile"][0]
+    achieved = b_alg / (main_ms * 1e-3) / 1e9
+    log(f"[bench] fused FP32: {head['mlups']:.0f} MLUPS, kernel {main_ms * 1e3:.1f} us, {achieved:.0f} GB/s")
+
+    extra = {}
+    if not a.no_compare and not a.only_headline:
+        k = max(20, a.steps // 20)
+        plain64 = measure(cases.lock_exchange(storage=F64, path=PATH_PLAIN), k, 5)
+        log(f"[bench] plain FP64: {plain64['mlups']:.0f} MLUPS")
+        fused64 = measure(cases.lock_exchange(storage=F64, path=PATH_FUSED), max(100, a.steps // 4), 20,
+                          profile_steps=100)
+        log(f"[bench] fused FP64: {fused64['mlups']:.0f} MLUPS")
+        plain32 = measure(cases.lock_exchange(storage=F32, path=PATH_PLAIN), k, 5)
+        log(f"[bench] plain FP32: {plain32['mlups']:.0f} MLUPS")
+        extra = {
+            "fused_fp32_mlups": round(head["mlups"], 1),
+            "fused_fp64_mlups": round(fused64["mlups"], 1),
+            "plain_fp64_mlups": round(plain64["mlups"], 1),
+            "plain_fp32_mlups": round(plain32["mlups"], 1),
+            "speedup_fused_fp32_vs_plain_fp64": round(head["mlups"] / plain64["mlups"], 2),
+            "fused_fp64_kernel_gbps": round(2 * 26 * 8 * cells / (fused64["profile"][0] * 1e-3) / 1e9, 1),
+            "fp16_storage": "disabled (see DESIGN.md)",
+        }
+
+    cpu = None
+    if not a.no_cpu_baseline and not a.only_headline:
+        log("[bench] CPU baseline (FP64 oracle) ...")
+        cpu = cpu_baseline(cases.lock_exchange(storage=F64), a.cpu_seconds)
+        log(f"[bench] CPU: {cpu['value']} MLUPS on {cpu['cores']} threads")
+
+    line = {
+        "metric": "MLUPS & % HBM roofline, FP32/FP16 vs FP64 plain; batched p50/p99 job latency",
+        "value": round(head["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(head["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 lock-exchange initial state)",
+        "config": {"workload": WORKLOAD, "cells": cells, "lattice": "D3Q19 MRT + D3Q7 salinity",
+                   "stretch": "z, ratio 1.02 toward mid-depth", "storage": "fp32", "path": "fused",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": read_traffic(),
+                     "kernel": "k_fused_step", "kernel_ms": round(main_ms, 5),
+                     "algorithmic_bytes_per_launch": b_alg,
+                     "aux_kernel_ms_per_step": round(head["profile"][1], 5)},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -127,6 +127,13 @@ int ipl_step(ipl_ctx* ctx, int32_t nsteps);
  * stream), in milliseconds. */
 int ipl_step_timed(ipl_ctx* ctx, int32_t nsteps, double* ms);
 
+/* Runs nsteps steps with every kernel launch bracketed by HIP events on the context's stream.
+ * out[0] = mean duration (ms) of one launch of the dominant kernel (the fused step kernel; on the
+ * plain path the whole step), out[1] = mean per-step time (ms) of the remaining kernels,
+ * out[2] = total (ms).  The reference times its whole loop with a host clock only
+ * (LockExchange.cu:197,284-291). */
+int ipl_profile_steps(ipl_ctx* ctx, int32_t nsteps, double out[3]);
+
 /* Device -> host copy of a field in canonical cell order (the reference's save-interval
  * cudaMemcpy, LockExchange.cu:245-256).  bytes must equal the field size in doubles. */
 int ipl_read(ipl_ctx* ctx, int32_t field, double* host, size_t count);

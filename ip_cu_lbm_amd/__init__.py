@@ -81,6 +81,13 @@ class Simulation:
         self._check(self._lib.ipl_step_timed(self._h, int(n), ctypes.byref(ms)))
         return ms.value
 
+    def profile_steps(self, n: int):
+        """Runs n steps with per-launch HIP-event timing; returns (dominant kernel ms per launch,
+        other kernels ms per step, total ms)."""
+        out = np.zeros(3)
+        self._check(self._lib.ipl_profile_steps(self._h, int(n), _dp(out)))
+        return float(out[0]), float(out[1]), float(out[2])
+
     def stage(self, name):
         sid = STAGES[name] if isinstance(name, str) else int(name)
         self._check(self._lib.ipl_stage(self._h, sid))

@@ -98,6 +98,8 @@ def load_library(path: str = None):
     lib.ipl_error_string.restype = c_char_p
     lib.ipl_last_error_message.argtypes = [vp]
     lib.ipl_last_error_message.restype = c_char_p
+    lib.ipl_profile_steps.argtypes = [vp, c_int32, _DP]
+    lib.ipl_profile_steps.restype = c_int32
     lib.ipl_stage.argtypes = [vp, c_int32]
     lib.ipl_stage.restype = c_int32
     lib.ipl_build_tables.argtypes = [POINTER(IplConfig), c_int32, c_void_p, POINTER(c_int64)]

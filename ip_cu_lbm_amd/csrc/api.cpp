@@ -128,6 +128,12 @@ int ipl_step_timed(ipl_ctx* ctx, int32_t nsteps, double* ms) {
     return IPL_OK;
 }
 
+int ipl_profile_steps(ipl_ctx* ctx, int32_t nsteps, double out[3]) {
+    if (!ctx || nsteps <= 0 || !out) return IPL_ERR_ARGUMENT;
+    ipl::Status s = ctx->engine->profile(nsteps, out);
+    return s ? IPL_OK : fail(ctx, s);
+}
+
 int ipl_stage(ipl_ctx* ctx, int32_t stage) {
     if (!ctx) return IPL_ERR_ARGUMENT;
     ipl::Status s = ctx->engine->stage(stage);

@@ -72,6 +72,10 @@ public:
     virtual Status write(int field, const double* host, size_t count) = 0;
     virtual Status global_sums(double out[3]) = 0;
     virtual Status check_finite(bool* finite) = 0;
+    // Runs nsteps steps timing every launch with HIP events on the engine's stream:
+    // out[0] = mean duration of the dominant kernel's launch (ms), out[1] = mean per-step time of
+    // the other kernels (ms), out[2] = summed time of all timed launches (ms).
+    virtual Status profile(int nsteps, double out[3]) = 0;
     virtual int64_t device_bytes() const = 0;
     virtual int path() const = 0;
     hipStream_t stream = nullptr;

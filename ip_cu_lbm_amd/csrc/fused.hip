@@ -201,15 +201,18 @@ public:
     }
 
     template <int Q, int QS>
-    Status step_q(int nsteps) {
+    Status step_q(int nsteps, std::vector<hipEvent_t>* ev = nullptr) {
         for (int s = 0; s < nsteps; ++s) {
             FBufs<ST, MT> b{};
             b.f_src = fa_; b.f_dst = fb_; b.g_src = ga_; b.g_dst = gb_;
             b.F_old = Fa_; b.F_new = Fb_;
             b.rho_n = rho_n_; b.phi_n = phi_n_;
             b.phiold = phiold_; b.uold = uold_; b.vold = vold_; b.wold = wold_;
+            if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 0], stream));
             if (geo_.need_grad) fusedk::fused_launch_scalar_fields<ST, MT, QS>(geo_, ga_, rho_n_, phi_n_, P_, stream);
+            if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 1], stream));
             fusedk::fused_launch_step<ST, MT, Q, QS>(geo_, tb_, b, P_, gr_.npts, block_, lds_bytes_, stream);
+            if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 2], stream));
             std::swap(fa_, fb_);
             std::swap(ga_, gb_);
             std::swap(Fa_, Fb_);
@@ -249,6 +252,30 @@ public:
     }
     Status readout() {
         return dispatch([&](auto q, auto qs) { return this->template readout_q<decltype(q)::value, decltype(qs)::value>(); });
+    }
+
+    Status profile(int nsteps, double out[3]) override {
+        std::vector<hipEvent_t> ev((size_t)4 * nsteps, nullptr);
+        for (auto& e : ev) IPL_HIP_TRY(hipEventCreate(&e));
+        Status r = dispatch([&](auto q, auto qs) {
+            return this->template step_q<decltype(q)::value, decltype(qs)::value>(nsteps, &ev);
+        });
+        double main_ms = 0, aux_ms = 0;
+        if (r) {
+            IPL_HIP_TRY(hipEventSynchronize(ev[4 * (nsteps - 1) + 2]));
+            for (int s = 0; s < nsteps; ++s) {
+                float a = 0, m = 0;
+                IPL_HIP_TRY(hipEventElapsedTime(&a, ev[4 * s + 0], ev[4 * s + 1]));
+                IPL_HIP_TRY(hipEventElapsedTime(&m, ev[4 * s + 1], ev[4 * s + 2]));
+                aux_ms += a; main_ms += m;
+            }
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        if (!r) return r;
+        out[0] = main_ms / nsteps;
+        out[1] = aux_ms / nsteps;
+        out[2] = main_ms + aux_ms;
+        return Status::ok();
     }
 
     Status stage(int) override { return Status::err(IPL_ERR_UNSUPPORTED, "the fused path has no separate stages"); }

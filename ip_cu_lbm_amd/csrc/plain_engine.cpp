@@ -180,6 +180,33 @@ public:
         return Status::ok();
     }
 
+    // the plain path has no single dominant kernel: out[0] is the whole step
+    Status profile(int nsteps, double out[3]) override {
+        std::vector<hipEvent_t> ev((size_t)nsteps + 1, nullptr);
+        for (auto& e : ev) IPL_HIP_TRY(hipEventCreate(&e));
+        Status r = Status::ok();
+        IPL_HIP_TRY(hipEventRecord(ev[0], stream));
+        for (int n = 0; n < nsteps && r; ++n) {
+            r = step(1);
+            if (r) IPL_HIP_TRY(hipEventRecord(ev[n + 1], stream));
+        }
+        double tot = 0;
+        if (r) {
+            IPL_HIP_TRY(hipEventSynchronize(ev[nsteps]));
+            for (int n = 0; n < nsteps; ++n) {
+                float m = 0;
+                IPL_HIP_TRY(hipEventElapsedTime(&m, ev[n], ev[n + 1]));
+                tot += m;
+            }
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        if (!r) return r;
+        out[0] = tot / nsteps;
+        out[1] = 0;
+        out[2] = tot;
+        return Status::ok();
+    }
+
     T* field_ptr(int field) const {
         switch (field) {
         case IPL_FIELD_P: return v_.p; case IPL_FIELD_U: return v_.u; case IPL_FIELD_V: return v_.v;
