@@ -4,12 +4,14 @@
 #include "hip_compat.hpp"
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <type_traits>
 #include <vector>
 
 #include "engine.hpp"
 #include "fused_common.hpp"
+#include "lattice.hpp"
 
 namespace ipl {
 
@@ -77,9 +79,13 @@ public:
         G.ncell = G.lines * G.ns;
         G.wall_bc = cfg_.wall_bc;
         G.scalar = cfg_.scalar;
-        const bool rho_varies = (cfg_.scalar == IPL_SCALAR_SALINITY && cfg_.beta_s != 0.0) ||
-                                (cfg_.scalar == IPL_SCALAR_PHASE_FIELD && cfg_.rho_H != cfg_.rho_L);
-        G.need_grad = rho_varies || cfg_.sigma != 0.0 || cfg_.scalar == IPL_SCALAR_PHASE_FIELD;
+        // rho and phi of the neighbours enter Force / col_PF only through differences, which
+        // vanish where the field is uniform: the salinity case has phi = 0, a scalar-free case
+        // has rho = rho_L everywhere
+        G.grad_rho = (cfg_.scalar == IPL_SCALAR_SALINITY && cfg_.beta_s != 0.0) ||
+                     (cfg_.scalar == IPL_SCALAR_PHASE_FIELD && cfg_.rho_H != cfg_.rho_L);
+        G.grad_phi = cfg_.scalar == IPL_SCALAR_PHASE_FIELD;
+        G.need_grad = G.grad_rho || G.grad_phi;
         double scale = 1.0;
         if (std::is_same<ST, __half>::value) scale = cfg_.f16_scale > 0 ? cfg_.f16_scale : 8192.0;
         G.scale = (float)scale;
@@ -91,6 +97,20 @@ public:
         P_.sigma = cfg_.sigma; P_.pf_thickness = cfg_.pf_thickness; P_.pf_tau = cfg_.pf_tau;
         P_.tau = hs.tau[0]; P_.taus = hs.taus[0];
         P_.boussinesq = cfg_.boussinesq; P_.collision = cfg_.collision;
+        P_.inv_c = 1.0 / P_.c; P_.inv_c2 = 1.0 / (P_.c * P_.c); P_.half_dt = P_.dt / 2.0;
+        P_.fk_coef = 3.0 * P_.dt / P_.c;
+        P_.itau = 1.0 / P_.tau; P_.om_s = 1.0 - 1.0 / P_.taus;
+        for (int r = 0; r < gr_.Q; ++r)
+            P_.rnorm[r] = (moment_row_is_stress(gr_.Q, r) ? P_.itau : P_.rates[r]) / moment_row_norm2(gr_.Q, r);
+        P_.press = P_.c * P_.c / 3.0;
+        P_.st_beta = cfg_.pf_thickness > 0 ? 4.0 * 12.0 * cfg_.sigma / cfg_.pf_thickness : 0.0;
+        P_.kap = 3.0 * cfg_.sigma * cfg_.pf_thickness / 2.0;
+        P_.inv_dz2 = 1.0 / (P_.dz * P_.dz);
+        P_.inv_dt = 1.0 / P_.dt;
+        P_.inv_pfW = cfg_.pf_thickness > 0 ? 1.0 / cfg_.pf_thickness : 0.0;
+        P_.pf_itau = cfg_.pf_tau > 0 ? 1.0 / cfg_.pf_tau : 0.0;
+        P_.pf_fcoef = cfg_.pf_tau - 0.5;
+        K_ = fusedk::make_coef<MT>(P_);
         const int Q = gr_.Q, Qs = gr_.Qs;
         const int64_t NC = G.ncell, N = gr_.N;
         // internal <-> canonical permutation
@@ -149,18 +169,31 @@ public:
         IPL_HIP_TRY(hipMemcpyAsync(dW, W.data(), W.size() * sizeof(MT), hipMemcpyHostToDevice, st));
         IPL_HIP_TRY(hipMemcpyAsync(drd, rd.data(), rd.size() * sizeof(MT), hipMemcpyHostToDevice, st));
         tb_.W = dW; tb_.rdel = drd;
-        std::vector<std::vector<MT>> axis_h(6);
+        // gradient reciprocals 1 / (pos[x+1] - pos[x-1] + dz/1000) with the wall cell repeated
+        // beyond a wall and the period added across a periodic seam, and 1 / width_x^2
         for (int a = 0; a < 3; ++a) {
-            axis_h[a].resize(G.n[a]);
-            axis_h[3 + a].resize(G.n[a]);
-            for (int x = 0; x < G.n[a]; ++x) { axis_h[a][x] = (MT)cfg_pos_[a][x]; axis_h[3 + a][x] = (MT)cfg_widths_[a][x]; }
-            MT *dp, *dw;
-            if (!(r = mem_.alloc_n(&dp, G.n[a]))) return r;
-            if (!(r = mem_.alloc_n(&dw, G.n[a]))) return r;
-            IPL_HIP_TRY(hipMemcpyAsync(dp, axis_h[a].data(), G.n[a] * sizeof(MT), hipMemcpyHostToDevice, st));
-            IPL_HIP_TRY(hipMemcpyAsync(dw, axis_h[3 + a].data(), G.n[a] * sizeof(MT), hipMemcpyHostToDevice, st));
-            tb_.pos[a] = dp; tb_.wid[a] = dw;
-            tb_.period[a] = (MT)gr_.period[a];
+            const int na = G.n[a];
+            std::vector<MT> inv(na);
+            for (int x = 0; x < na; ++x) {
+                auto wrap = [&](int y) { return y < 0 ? (cfg_.periodic[a] ? y + na : 0) : (y >= na ? (cfg_.periodic[a] ? y - na : na - 1) : y); };
+                double d = cfg_pos_[a][wrap(x + 1)] - cfg_pos_[a][wrap(x - 1)];
+                if (cfg_.periodic[a] && d <= 0.0) d += gr_.period[a];
+                inv[x] = (MT)(1.0 / (d + gr_.dz / 1000.0));
+            }
+            MT* dinv;
+            if (!(r = mem_.alloc_n(&dinv, na))) return r;
+            IPL_HIP_TRY(hipMemcpyAsync(dinv, inv.data(), na * sizeof(MT), hipMemcpyHostToDevice, st));
+            IPL_HIP_TRY(hipStreamSynchronize(st));
+            tb_.invd[a] = dinv;
+        }
+        {
+            std::vector<MT> iw(G.n[0]);
+            for (int x = 0; x < G.n[0]; ++x) iw[x] = (MT)(1.0 / (cfg_widths_[0][x] * cfg_widths_[0][x]));
+            MT* diw;
+            if (!(r = mem_.alloc_n(&diw, G.n[0]))) return r;
+            IPL_HIP_TRY(hipMemcpyAsync(diw, iw.data(), G.n[0] * sizeof(MT), hipMemcpyHostToDevice, st));
+            IPL_HIP_TRY(hipStreamSynchronize(st));
+            tb_.inv_wx2 = diw;
         }
         // per-cell volume (FP64) for the conservation sums, internal order
         std::vector<double> vol(NC);
@@ -209,9 +242,9 @@ public:
             b.rho_n = rho_n_; b.phi_n = phi_n_;
             b.phiold = phiold_; b.uold = uold_; b.vold = vold_; b.wold = wold_;
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 0], stream));
-            if (geo_.need_grad) fusedk::fused_launch_scalar_fields<ST, MT, QS>(geo_, ga_, rho_n_, phi_n_, P_, stream);
+            if (geo_.need_grad) fusedk::fused_launch_scalar_fields<ST, MT, QS>(geo_, ga_, rho_n_, phi_n_, K_, stream);
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 1], stream));
-            fusedk::fused_launch_step<ST, MT, Q, QS>(geo_, tb_, b, P_, gr_.npts, block_, lds_bytes_, stream);
+            fusedk::fused_launch_step<ST, MT, Q, QS>(geo_, tb_, b, K_, gr_.npts, block_, lds_bytes_, stream);
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 2], stream));
             std::swap(fa_, fb_);
             std::swap(ga_, gb_);
@@ -375,6 +408,7 @@ private:
     std::vector<double> cfg_pos_[3], cfg_widths_[3];
     FGeom geo_;
     FParams P_;
+    fusedk::FCoef<MT> K_{};
     FTables<MT> tb_{};
     DeviceArena mem_;
     ST *fa_ = nullptr, *fb_ = nullptr, *ga_ = nullptr, *gb_ = nullptr;
@@ -393,12 +427,18 @@ bool fused_supported(const Graph& g, const ipl_config& cfg, std::string& why) {
     if (g.flux_mode == FLUX_MODE_CORNER) { why = "the two-axis corner scheme runs on the plain path"; return false; }
     const int s = g.flux_mode == FLUX_MODE_ONE_AXIS ? g.flux_axis : 2;
     if (g.n[s] > 512) { why = "stream-axis lines longer than 512 cells"; return false; }
+    // every population and field array is addressed with 32-bit byte offsets from its base
+    const size_t row_bytes = (size_t)g.N * (cfg.storage == IPL_F64 ? 8 : 4);
+    if ((size_t)std::max(std::max(g.Q, g.Qs), 3) * row_bytes >= ((size_t)1 << 32)) {
+        why = "a population array of 4 GiB or more (32-bit offsets)";
+        return false;
+    }
     const bool built = (g.Q == 9 && (g.Qs == 0 || g.Qs == 9)) || (g.Q == 19 && (g.Qs == 0 || g.Qs == 7 || g.Qs == 19)) ||
                        (g.Q == 27 && g.Qs == 0);
     if (!built) { why = "lattice combination not built for the fused path"; return false; }
     if (g.flux_mode == FLUX_MODE_ONE_AXIS && !cfg.periodic[s] && g.n[s] < 3) { why = "stream axis too short"; return false; }
     const size_t mt = cfg.storage == IPL_F64 ? 8 : 4;
-    const size_t lds = (size_t)(g.Q + g.Qs) * g.n[s] * mt;
+    const size_t lds = (size_t)(g.Q + g.Qs + 2) * g.n[s] * mt;
     if (lds > 64 * 1024) { why = "line does not fit the LDS budget"; return false; }
     for (int a = 0; a < 3; ++a)
         if (g.n[a] > 1 && !cfg.periodic[a] && g.n[a] < 2) { why = "axis too short"; return false; }

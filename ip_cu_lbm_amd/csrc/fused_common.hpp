@@ -19,7 +19,9 @@ struct FGeom {
     int64_t lines, ncell;
     int wall_bc;
     int flux;        // 1 = interpolated flux along s
-    int need_grad;   // gradients of rho / phi are needed (Force or col_PF)
+    int need_grad;   // the scalar-field pass runs (rho and/or phi gradients are needed)
+    int grad_rho;    // rho varies: the pressure-gradient term of Force is nonzero
+    int grad_phi;    // phase field: grad phi and its Laplacian (Force, col_PF)
     int scalar;
     float inv_scale; // FP16 storage: stored value = f / w_k * scale
     float scale;
@@ -29,9 +31,8 @@ template <typename MT>
 struct FTables {
     const MT* W;       // [sign(0:+,1:-)][j][ns]
     const MT* rdel;    // [ns] dz / width along s
-    const MT* pos[3];  // per axis index
-    const MT* wid[3];
-    MT period[3];      // sum of widths along periodic axes
+    const MT* invd[3]; // per axis index: 1 / (pos[+1] - pos[-1] + dz/1000), seam-corrected
+    const MT* inv_wx2; // per x index: 1 / width_x^2
 };
 
 template <typename ST, typename MT>
@@ -50,15 +51,48 @@ struct FParams {
     double rho_L, rho_H, beta_s, gravity, sigma, pf_thickness, pf_tau;
     double tau, taus;
     int boussinesq, collision;
+    // FP64 host-precomputed factors, so that the step divides once per cell (1 / rho)
+    double inv_c, inv_c2, half_dt, fk_coef;  // 1/c, 1/c^2, dt/2, 3 dt / c
+    double itau, om_s;                       // 1/tau, 1 - 1/tau_s
+    double rnorm[27];                        // MRT: rate of row r (1/tau on stress rows) / |row r|^2
+    double press;                            // c^2 / 3
+    double st_beta, kap, inv_dz2;            // 4 * 12 sigma / W, 3 sigma W / 2, 1/dz^2
+    double inv_dt, inv_pfW, pf_itau, pf_fcoef; // 1/dt, 1/W, 1/tau_phi, tau_phi - 1/2
 };
 
+// The coefficients of the step in the arithmetic type, cast once on the host (the kernels then
+// hold no FP64 constants and convert nothing).
+template <typename MT>
+struct FCoef {
+    MT c, inv_c, inv_c2, half_dt, fk_coef, dt;
+    MT itau, om_s, rnorm[27];
+    MT rho_L, drho, beta_s, gravity;   // drho = rho_H - rho_L (in MT)
+    MT press, st_beta, kap, inv_dz2;
+    MT inv_dt, inv_pfW, pf_itau, pf_fcoef;
+    int boussinesq, collision;
+};
+
+template <typename MT>
+inline FCoef<MT> make_coef(const FParams& P) {
+    FCoef<MT> k{};
+    k.c = (MT)P.c; k.inv_c = (MT)P.inv_c; k.inv_c2 = (MT)P.inv_c2; k.half_dt = (MT)P.half_dt;
+    k.fk_coef = (MT)P.fk_coef; k.dt = (MT)P.dt;
+    k.itau = (MT)P.itau; k.om_s = (MT)P.om_s;
+    for (int r = 0; r < 27; ++r) k.rnorm[r] = (MT)P.rnorm[r];
+    k.rho_L = (MT)P.rho_L; k.drho = (MT)P.rho_H - (MT)P.rho_L; k.beta_s = (MT)P.beta_s; k.gravity = (MT)P.gravity;
+    k.press = (MT)P.press; k.st_beta = (MT)P.st_beta; k.kap = (MT)P.kap; k.inv_dz2 = (MT)P.inv_dz2;
+    k.inv_dt = (MT)P.inv_dt; k.inv_pfW = (MT)P.inv_pfW; k.pf_itau = (MT)P.pf_itau; k.pf_fcoef = (MT)P.pf_fcoef;
+    k.boussinesq = P.boussinesq; k.collision = P.collision;
+    return k;
+}
+
 template <typename ST, typename MT, int Q, int QS>
-void fused_launch_step(const FGeom& G, const FTables<MT>& tb, const FBufs<ST, MT>& b, const FParams& P, int npts,
+void fused_launch_step(const FGeom& G, const FTables<MT>& tb, const FBufs<ST, MT>& b, const FCoef<MT>& P, int npts,
                        unsigned block, size_t lds, hipStream_t st);
 template <typename ST, typename MT, int Q, int QS>
 size_t fused_lds_bytes(int s, int ns);
 template <typename ST, typename MT, int QS>
-void fused_launch_scalar_fields(const FGeom& G, const ST* g_src, MT* rho_n, MT* phi_n, const FParams& P, hipStream_t st);
+void fused_launch_scalar_fields(const FGeom& G, const ST* g_src, MT* rho_n, MT* phi_n, const FCoef<MT>& P, hipStream_t st);
 template <typename ST, typename MT, int Q, int QS>
 void fused_launch_readout(const FGeom& G, const ST* f_src, const ST* g_src, const MT* F_old, double* out,
                           const FParams& P, hipStream_t st);

@@ -49,6 +49,16 @@ namespace fusedk {
 
 constexpr int kMaxLine = 512;  // work-items per workgroup = cells per line, at most
 
+// minimum waves per SIMD the register allocator is asked to leave room for (0: no hint)
+#ifndef IPL_FUSED_WAVES_PER_EU
+#define IPL_FUSED_WAVES_PER_EU 0
+#endif
+#if IPL_FUSED_WAVES_PER_EU > 0 && !defined(IPL_CPU_EMULATION)
+#define IPL_FUSED_OCCUPANCY __attribute__((amdgpu_waves_per_eu(IPL_FUSED_WAVES_PER_EU)))
+#else
+#define IPL_FUSED_OCCUPANCY
+#endif
+
 // ---- storage conversion ------------------------------------------------------------------
 template <typename ST, typename MT> struct Store;
 template <typename MT> struct Store<double, MT> {
@@ -102,59 +112,130 @@ __device__ inline int wrap_or_clamp(int x, int n, int periodic) {
     return x;
 }
 
-template <int S> __device__ inline int64_t cell_index(const FGeom& G, const int c[3]) {
-    constexpr int o1 = S == 0 ? 1 : 0, o2 = S == 2 ? 1 : 2;
-    return ((int64_t)c[o1] * G.n[o2] + c[o2]) * G.ns + c[S];
+// pointer to element idx of an array: a 32-bit byte offset from a base held in scalar registers,
+// the form of the global instructions' scalar-base addressing (the host keeps every population and
+// field array below 4 GiB, so idx counts rows as well: idx = row * ncell + cell)
+template <typename T> __device__ inline T* elem(T* row, uint32_t idx) {
+    return reinterpret_cast<T*>(reinterpret_cast<typename std::conditional<std::is_const<T>::value, const char, char>::type*>(row) +
+                                (uint64_t)(idx * (uint32_t)sizeof(T)));
 }
 
-// Writes the transported value of direction K whose source is cell m (destination m + c_K).
-template <typename ST, typename MT, int Q, int K, int S>
-__device__ inline void push(const FGeom& G, ST* dst, const int m[3], MT value) {
-    constexpr Dir ck = Lattice<Q>::c(K);
-    const int cv[3] = {ck.x, ck.y, ck.z};
-    int d[3];
-    int out = 0;  // mask of the axes the destination leaves
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        int x = m[a] + cv[a];
-        if (cv[a] != 0 && (x < 0 || x >= G.n[a])) {
-            if (G.periodic[a]) x = x < 0 ? x + G.n[a] : x - G.n[a];
-            else out |= 1 << a;
+template <int S> struct LineAxes {
+    static constexpr int o1 = S == 0 ? 1 : 0, o2 = S == 2 ? 1 : 2;
+};
+
+// The workgroup's line (c1, c2) and its eight neighbour lines, all wave-uniform: base<da, db>() is
+// the first cell of line (c1+da, c2+db), wrapped on a periodic axis and the workgroup's own line
+// where that line would leave the box (the ghost copies its owner, basic.cu:221-225); out1<da>() /
+// out2<db>() flag the displacements that leave the box through a wall.  Named members only, no
+// arrays: a selection between array elements becomes a load from private memory.
+struct LineNb {
+    uint32_t bmm, bm0, bmp, b0m, b00, b0p, bpm, bp0, bpp;
+    bool o1m, o1p, o2m, o2p;
+    int c1, c2;
+    template <int DA, int DB> __device__ uint32_t base() const {
+        if constexpr (DA < 0) return DB < 0 ? bmm : (DB == 0 ? bm0 : bmp);
+        else if constexpr (DA == 0) return DB < 0 ? b0m : (DB == 0 ? b00 : b0p);
+        else return DB < 0 ? bpm : (DB == 0 ? bp0 : bpp);
+    }
+    template <int DA> __device__ bool out1() const { return DA < 0 ? o1m : (DA > 0 ? o1p : false); }
+    template <int DB> __device__ bool out2() const { return DB < 0 ? o2m : (DB > 0 ? o2p : false); }
+};
+
+template <int S>
+__device__ inline LineNb line_neighbours(const FGeom& G, int c1, int c2) {
+    constexpr int o1 = LineAxes<S>::o1, o2 = LineAxes<S>::o2;
+    const int n1 = G.n[o1], n2 = G.n[o2];
+    const uint32_t ns = (uint32_t)G.ns;
+    LineNb nb;
+    nb.c1 = c1; nb.c2 = c2;
+    auto step = [&](int c, int d, int n, int per, bool& out) {
+        int a = c + d;
+        out = false;
+        if (a < 0 || a >= n) {
+            if (per) a = a < 0 ? a + n : a - n;
+            else { a = c; out = true; }
         }
-        d[a] = x;
+        return a;
+    };
+    const int a_m = step(c1, -1, n1, G.periodic[o1], nb.o1m), a_p = step(c1, 1, n1, G.periodic[o1], nb.o1p);
+    const int b_m = step(c2, -1, n2, G.periodic[o2], nb.o2m), b_p = step(c2, 1, n2, G.periodic[o2], nb.o2p);
+    auto lb = [&](int a, int b) { return (uint32_t)(a * n2 + b) * ns; };
+    nb.bmm = lb(a_m, b_m); nb.bm0 = lb(a_m, c2); nb.bmp = lb(a_m, b_p);
+    nb.b0m = lb(c1, b_m);  nb.b00 = lb(c1, c2);  nb.b0p = lb(c1, b_p);
+    nb.bpm = lb(a_p, b_m); nb.bp0 = lb(a_p, c2); nb.bpp = lb(a_p, b_p);
+    return nb;
+}
+
+// Writes the transported value of direction K whose source is the cell at S-index si of the
+// workgroup's line; the destination is that cell + c_K.
+template <typename ST, typename MT, int Q, int K, int S>
+__device__ inline void push(const FGeom& G, const LineNb& nb, ST* __restrict__ dst, int si, MT value) {
+    constexpr int o1 = LineAxes<S>::o1, o2 = LineAxes<S>::o2;
+    constexpr int ka = comp<Q>(K, o1), kb = comp<Q>(K, o2), ks = comp<Q>(K, S);
+    const int ns = G.ns;
+    const uint32_t NC = (uint32_t)G.ncell;
+    // the destination line leaves the box: wave-uniform
+    const bool out1 = nb.template out1<ka>(), out2 = nb.template out2<kb>();
+    // destination S-index: per work-item
+    int xs = si + ks;
+    bool sout = false;
+    if (ks != 0) {
+        if (G.periodic[S]) xs = xs < 0 ? xs + ns : (xs >= ns ? xs - ns : xs);
+        else sout = xs < 0 || xs >= ns;
     }
-    if (!out) {
-        Store<ST, MT>::store(dst + (int64_t)K * G.ncell + cell_index<S>(G, d), value, Lattice<Q>::w(K), G.scale);
+    constexpr double wk = Lattice<Q>::w(K);  // bounce-back and reflection keep |c|, hence the weight
+    constexpr int kopp = opposite<Q>(K);
+    const uint32_t own = nb.b00 + (uint32_t)si;
+    if (!out1 && !out2) {
+        const uint32_t lbase = nb.template base<ka, kb>();
+        if (!sout) {
+            Store<ST, MT>::store(elem(dst, K * NC + lbase + (uint32_t)xs), value, wk, G.scale);
+        } else if (G.wall_bc == IPL_BC_BOUNCE_BACK) {
+            Store<ST, MT>::store(elem(dst, kopp * NC + own), value, wk, G.scale);
+        } else {
+            constexpr int kr = reflect_dir<Q>(K, 1 << S);
+            Store<ST, MT>::store(elem(dst, kr * NC + lbase + (uint32_t)si), value, wk, G.scale);
+        }
         return;
     }
+    // boundary lines only
     if (G.wall_bc == IPL_BC_BOUNCE_BACK) {
-        constexpr int kb = opposite<Q>(K);
-        Store<ST, MT>::store(dst + (int64_t)kb * G.ncell + cell_index<S>(G, m), value, Lattice<Q>::w(kb), G.scale);
+        Store<ST, MT>::store(elem(dst, kopp * NC + own), value, wk, G.scale);
         return;
     }
-    // slip: reflect the destination and the direction across every wall crossed
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-        if (out & (1 << a)) d[a] = d[a] < 0 ? 0 : G.n[a] - 1;
+    // slip: reflect the destination and the direction across every wall crossed; the crossed
+    // coordinate stays the source's, which lies on that wall
+    // (recomputed from the coordinates: a selection between table members would move the table
+    // to private memory)
+    const int n1 = G.n[o1], n2 = G.n[o2];
+    int y1 = nb.c1, y2 = nb.c2;
+    if (!out1) { y1 += ka; y1 = y1 < 0 ? y1 + n1 : (y1 >= n1 ? y1 - n1 : y1); }
+    if (!out2) { y2 += kb; y2 = y2 < 0 ? y2 + n2 : (y2 >= n2 ? y2 - n2 : y2); }
+    const uint32_t lbase = (uint32_t)(y1 * n2 + y2) * (uint32_t)ns;
+    const int ys = sout ? si : xs;
+    const int out = (out1 ? 1 << o1 : 0) | (out2 ? 1 << o2 : 0) | (sout ? 1 << S : 0);
     constexpr int r1 = reflect_dir<Q>(K, 1), r2 = reflect_dir<Q>(K, 2), r3 = reflect_dir<Q>(K, 3);
     constexpr int r4 = reflect_dir<Q>(K, 4), r5 = reflect_dir<Q>(K, 5), r6 = reflect_dir<Q>(K, 6);
     constexpr int r7 = reflect_dir<Q>(K, 7);
     const int kr = out == 1 ? r1 : out == 2 ? r2 : out == 3 ? r3 : out == 4 ? r4 : out == 5 ? r5 : out == 6 ? r6 : r7;
-    const double wr = Lattice<Q>::w(K);  // reflection keeps |c|, hence the weight
-    Store<ST, MT>::store(dst + (int64_t)kr * G.ncell + cell_index<S>(G, d), value, wr, G.scale);
+    Store<ST, MT>::store(elem(dst, (uint32_t)kr * NC + lbase + (uint32_t)ys), value, wk, G.scale);
 }
 
 // Slab flux, flux update and push of one population set (flow or scalar) of a line.
-// P: collided populations of this work-item's cell.  lds: [sdir slot][ns] of the workgroup.
+// P: collided populations of this work-item's cell.  lds: [ns][SL] of the workgroup, one row of
+// the S-directions' values per cell (SL = their count + 1, an odd stride for the LDS banks), so a
+// direction's slot is an immediate offset from the cell's row.
 template <typename ST, typename MT, int Q, int NPTS, int S>
-__device__ inline void transport(const FGeom& G, const FTables<MT>& tb, ST* dst, const MT (&P)[Q], MT* lds,
-                                 int i, bool active, const int cc[3]) {
+__device__ inline void transport(const FGeom& G, const LineNb& nb, const FTables<MT>& tb, ST* __restrict__ dst,
+                                 const MT (&P)[Q], MT* lds, int i, bool active) {
     const int ns = G.ns;
+    constexpr int SL = sdir_count<Q, S>() + 1;
     if (!G.flux) {
         if (active)
             static_for<Q>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                push<ST, MT, Q, k, S>(G, dst, cc, P[k]);
+                push<ST, MT, Q, k, S>(G, nb, dst, i, P[k]);
             });
         return;
     }
@@ -162,25 +243,32 @@ __device__ inline void transport(const FGeom& G, const FTables<MT>& tb, ST* dst,
     if (active)
         static_for<Q>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            if constexpr (comp<Q>(k, S) != 0) lds[sdir_slot<Q, S>(k) * ns + i] = P[k];
+            if constexpr (comp<Q>(k, S) != 0) lds[i * SL + sdir_slot<Q, S>(k)] = P[k];
         });
     __syncthreads();
     // 2. slab values F (adv.cu:76-87); stencil points beyond a wall repeat the wall cell
     MT Fv[Q];
     const int per = G.periodic[S];
+    MT Wp[NPTS], Wm[NPTS];
+    if (active) {
+#pragma unroll
+        for (int j = 0; j < NPTS; ++j) {
+            Wp[j] = *elem(tb.W, (uint32_t)(j * ns + i));
+            Wm[j] = *elem(tb.W, (uint32_t)((NPTS + j) * ns + i));
+        }
+    }
     static_for<Q>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         constexpr int sg = comp<Q>(k, S);
         Fv[k] = MT(0);
         if constexpr (sg != 0) {
             if (active) {
-                const MT* Wk = tb.W + (sg > 0 ? 0 : NPTS) * ns;
                 constexpr int slot = sdir_slot<Q, S>(k);
                 MT acc = 0;
 #pragma unroll
                 for (int j = 0; j < NPTS; ++j) {
                     const int ii = wrap_or_clamp(i + (j - NPTS / 2) * sg, ns, per);
-                    const MT term = Wk[j * ns + i] * lds[slot * ns + ii];
+                    const MT term = (sg > 0 ? Wp[j] : Wm[j]) * lds[ii * SL + slot];
                     acc = j == 0 ? term : acc + term;
                 }
                 Fv[k] = acc;
@@ -191,17 +279,17 @@ __device__ inline void transport(const FGeom& G, const FTables<MT>& tb, ST* dst,
     if (active)
         static_for<Q>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            if constexpr (comp<Q>(k, S) != 0) lds[sdir_slot<Q, S>(k) * ns + i] = Fv[k];
+            if constexpr (comp<Q>(k, S) != 0) lds[i * SL + sdir_slot<Q, S>(k)] = Fv[k];
         });
     __syncthreads();
     if (!active) return;
-    const MT rn = tb.rdel[i];
+    const MT rn = *elem(tb.rdel, (uint32_t)i);
     static_for<Q>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         constexpr int sg = comp<Q>(k, S);
         if constexpr (sg == 0) {
             // no component along S: set_tmp returns P exactly (adv.cu:64-65,241)
-            push<ST, MT, Q, k, S>(G, dst, cc, P[k]);
+            push<ST, MT, Q, k, S>(G, nb, dst, i, P[k]);
         } else {
             constexpr int slot = sdir_slot<Q, S>(k);
             // U of this cell is the transported value of m = n - sg S
@@ -209,235 +297,331 @@ __device__ inline void transport(const FGeom& G, const FTables<MT>& tb, ST* dst,
             bool have = im >= 0 && im < ns;
             if (!have && per) { im = im < 0 ? im + ns : im - ns; have = true; }
             if (have) {
-                const MT Um = P[k] + (lds[slot * ns + im] * rn - Fv[k] * rn);
-                int m[3] = {cc[0], cc[1], cc[2]};
-                m[S] = im;
-                push<ST, MT, Q, k, S>(G, dst, m, Um);
+                const MT Um = P[k] + (lds[im * SL + slot] * rn - Fv[k] * rn);
+                push<ST, MT, Q, k, S>(G, nb, dst, im, Um);
             }
             // downstream wall: the ghost's mirrored copy (adv.cu:196-199 along S)
             const int ip = i + sg;
             if (!per && (ip < 0 || ip >= ns)) {
                 constexpr int mk = mirror<Q>(k, S);
                 const MT Tm = P[mk] + (Fv[k] * rn - Fv[mk] * rn);
-                push<ST, MT, Q, k, S>(G, dst, cc, Tm);
+                push<ST, MT, Q, k, S>(G, nb, dst, i, Tm);
             }
         }
     });
 }
 
-// MRT / SRT collision in moment space (collision.cu:115-165)
+// A direction that represents its opposite pair (the rest direction stands alone).
+template <int Q> constexpr bool is_rep(int k) { return k != 0 && k < opposite<Q>(k); }
+
+// +1 when row r of the moment basis is even under c -> -c, -1 when it is odd, 0 otherwise
+template <int Q> constexpr int row_parity(int r) {
+    bool even = true, odd = true;
+    for (int k = 0; k < Q; ++k) {
+        const double a = moment<Q>(r, k), b = moment<Q>(r, opposite<Q>(k));
+        even = even && a == b;
+        odd = odd && a == -b;
+    }
+    return even ? 1 : (odd ? -1 : 0);
+}
+
+// first column among {0} and the pair representatives that contributes to row r
+template <int Q> constexpr int first_term_col(int r) {
+    for (int k = 0; k < Q; ++k)
+        if ((k == 0 || is_rep<Q>(k)) && moment<Q>(r, k) != 0.0 && (k != 0 || row_parity<Q>(r) > 0)) return k;
+    return Q;
+}
+// first row that contributes to column k with the given parity
+template <int Q> constexpr int first_term_row(int k, int parity) {
+    for (int r = 0; r < Q; ++r)
+        if (row_parity<Q>(r) == parity && moment<Q>(r, k) != 0.0) return r;
+    return Q;
+}
+
+template <typename MT> __device__ inline MT scaled(double c, MT x) {
+    return c == 1.0 ? x : (c == -1.0 ? -x : MT(c) * x);
+}
+
+// DX x + DY y + DZ z over the nonzero components only
+template <int DX, int DY, int DZ, typename MT> __device__ inline MT dot3(MT x, MT y, MT z) {
+    if constexpr (DX != 0) {
+        MT acc = scaled<MT>(DX, x);
+        if constexpr (DY != 0) acc += scaled<MT>(DY, y);
+        if constexpr (DZ != 0) acc += scaled<MT>(DZ, z);
+        return acc;
+    } else if constexpr (DY != 0) {
+        MT acc = scaled<MT>(DY, y);
+        if constexpr (DZ != 0) acc += scaled<MT>(DZ, z);
+        return acc;
+    } else if constexpr (DZ != 0) {
+        return scaled<MT>(DZ, z);
+    } else {
+        return MT(0);
+    }
+}
+
+// Equilibrium (equ_f, collision.cu:3-14), forcing term Fk (collision.cu:100-110) and the MRT /
+// SRT collision (collision.cu:115-165) of one cell, written for opposite pairs (k, kb):
+// feq_k = B + A and feq_kb = B - A with B = w (p + 4.5 t^2 - 1.5 u^2/c^2), A = 3 w t, and
+// Fk_kb = -Fk_k, so the pair sums s = d_k + d_kb = f_k + f_kb - 2B and differences
+// a = d_k - d_kb = f_k - f_kb - 2A + Fk_k of d = f - feq + Fk/2 are formed directly.  The basis
+// rows are even or odd under c -> -c, so even moments take the sums and odd moments the
+// differences, and M^-1 = M^T diag(1/|row|^2) (orthogonal rows, norms folded into the rates on
+// the host) gives f*_k = f_k - (E + O) + Fk_k, f*_kb = f_kb - (E - O) - Fk_k with E, O the even
+// and odd parts of column k.  f is replaced by f*.
 template <typename MT, int Q>
-__device__ inline void collide_cell(const MT (&f)[Q], const MT (&feq)[Q], const MT (&fk)[Q], MT tau,
-                                    const double* rates, int collision, MT (&P)[Q]) {
-    MT d[Q];
-#pragma unroll
-    for (int k = 0; k < Q; ++k) d[k] = f[k] - feq[k] + fk[k] / MT(2.0);
-    if (collision == IPL_COLL_SRT) {
-#pragma unroll
-        for (int k = 0; k < Q; ++k) P[k] = f[k] - d[k] / tau + fk[k];
+__device__ inline void collide_pairs(MT (&f)[Q], MT p, MT u, MT v, MT w, MT u2c, MT Fx, MT Fy, MT Fz, MT fkc, MT ic,
+                                     const FCoef<MT>& P) {
+    MT sv[Q], av[Q], fkv[Q];
+    sv[0] = f[0] - MT(Lattice<Q>::w(0)) * (p - MT(1.5) * u2c);
+    static_for<Q>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if constexpr (is_rep<Q>(k)) {
+            constexpr int kb = opposite<Q>(k);
+            constexpr Dir d = Lattice<Q>::c(k);
+            constexpr double wk = Lattice<Q>::w(k);
+            const MT t = dot3<d.x, d.y, d.z>(u, v, w) * ic;
+            const MT B = MT(wk) * (p + MT(4.5) * t * t - MT(1.5) * u2c);
+            const MT A = MT(3.0 * wk) * t;
+            const MT eF = dot3<d.x, d.y, d.z>(Fx, Fy, Fz);
+            fkv[k] = MT(wk) * fkc * eF;
+            sv[k] = (f[k] + f[kb]) - MT(2) * B;
+            av[k] = (f[k] - f[kb]) - MT(2) * A + fkv[k];
+        }
+    });
+    if (P.collision == IPL_COLL_SRT) {
+        const MT itau = P.itau;
+        f[0] = f[0] - sv[0] * itau;
+        static_for<Q>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            if constexpr (is_rep<Q>(k)) {
+                constexpr int kb = opposite<Q>(k);
+                const MT dk = MT(0.5) * (sv[k] + av[k]), dkb = MT(0.5) * (sv[k] - av[k]);
+                f[k] = f[k] - dk * itau + fkv[k];
+                f[kb] = f[kb] - dkb * itau - fkv[k];
+            }
+        });
         return;
     }
-    const MT itau = MT(1) / tau;
     MT m[Q];
-#pragma unroll
-    for (int r = 0; r < Q; ++r) {
+    static_for<Q>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        constexpr int par = row_parity<Q>(r);
+        static_assert(par != 0, "moment rows must be even or odd");
         MT acc = 0;
-#pragma unroll
-        for (int k = 0; k < Q; ++k) {
-            const double mk = moment<Q>(r, k);
-            if (mk != 0.0) acc += MT(mk) * d[k];
+        static_for<Q>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            if constexpr (k == 0 || is_rep<Q>(k)) {
+                constexpr double mk = moment<Q>(r, k);
+                if constexpr (mk != 0.0 && (k != 0 || par > 0)) {
+                    const MT term = scaled<MT>(mk, par > 0 ? sv[k] : av[k]);
+                    if constexpr (k == first_term_col<Q>(r)) acc = term;
+                    else acc += term;
+                }
+            }
+        });
+        m[r] = acc * P.rnorm[r];
+    });
+    static_for<Q>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if constexpr (k == 0 || is_rep<Q>(k)) {
+            MT E = 0, O = 0;
+            static_for<Q>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                constexpr double mk = moment<Q>(r, k);
+                constexpr int par = row_parity<Q>(r);
+                if constexpr (mk != 0.0) {
+                    const MT term = scaled<MT>(mk, m[r]);
+                    if constexpr (par > 0) {
+                        if constexpr (r == first_term_row<Q>(k, 1)) E = term; else E += term;
+                    } else {
+                        if constexpr (r == first_term_row<Q>(k, -1)) O = term; else O += term;
+                    }
+                }
+            });
+            if constexpr (k == 0) {
+                f[0] = f[0] - E;
+            } else {
+                constexpr int kb = opposite<Q>(k);
+                f[k] = f[k] - (E + O) + fkv[k];
+                f[kb] = f[kb] - (E - O) - fkv[k];
+            }
         }
-        m[r] = acc * (is_stress<Q>(r) ? itau : MT(rates[r]));
-    }
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        MT acc = 0;
-#pragma unroll
-        for (int r = 0; r < Q; ++r) {
-            const double mi = moment_inv<Q>(k, r);
-            if (mi != 0.0) acc += MT(mi) * m[r];
-        }
-        P[k] = f[k] - acc + fk[k];
-    }
+    });
 }
 
-// neighbour of cell cc along axis a at offset +-1: the wall cell itself beyond a wall (the
-// ghost copies its owner, basic.cu:221-225), wrapped on a periodic axis
-template <int S>
-__device__ inline int64_t neighbour(const FGeom& G, const int cc[3], int a, int off) {
-    int c[3] = {cc[0], cc[1], cc[2]};
-    c[a] = wrap_or_clamp(cc[a] + off, G.n[a], G.periodic[a]);
-    return cell_index<S>(G, c);
+// index of the neighbour of the work-item's cell along axis A at offset +-1: the wall cell
+// itself beyond a wall (the ghost copies its owner, basic.cu:221-225), wrapped on a periodic axis
+template <int S, int A, int OFF>
+__device__ inline uint32_t nb_index(const FGeom& G, const LineNb& nb, int i) {
+    constexpr int o1 = LineAxes<S>::o1;
+    if constexpr (A == S) return nb.b00 + (uint32_t)wrap_or_clamp(i + OFF, G.ns, G.periodic[S]);
+    else if constexpr (A == o1) return nb.template base<OFF, 0>() + (uint32_t)i;
+    else return nb.template base<0, OFF>() + (uint32_t)i;
 }
 
-// (q[+] - q[-]) / (pos[+] - pos[-] + dz/1000) along axis a (Force, collision.cu:66-69), the
-// period added across a periodic seam
-template <typename MT, int S>
-__device__ inline MT grad_axis(const FGeom& G, const FTables<MT>& tb, const MT* q, const int cc[3], int a, MT eps) {
-    const int ip = wrap_or_clamp(cc[a] + 1, G.n[a], G.periodic[a]);
-    const int im = wrap_or_clamp(cc[a] - 1, G.n[a], G.periodic[a]);
-    MT d = tb.pos[a][ip] - tb.pos[a][im];
-    if (G.periodic[a] && d <= MT(0)) d += tb.period[a];
-    return (q[neighbour<S>(G, cc, a, 1)] - q[neighbour<S>(G, cc, a, -1)]) / (d + eps);
+// (q[+] - q[-]) / (pos[+] - pos[-] + dz/1000) along axis A (Force, collision.cu:66-69); the
+// reciprocal, with the period added across a periodic seam, is a host table
+template <typename MT, int S, int A>
+__device__ inline MT grad_axis(const FGeom& G, const LineNb& nb, const FTables<MT>& tb, const MT* __restrict__ q,
+                               int c1, int c2, int i) {
+    constexpr int o1 = LineAxes<S>::o1;
+    const int x = A == S ? i : (A == o1 ? c1 : c2);
+    return (*elem(q, nb_index<S, A, 1>(G, nb, i)) - *elem(q, nb_index<S, A, -1>(G, nb, i))) * *elem(tb.invd[A], (uint32_t)x);
 }
 
 template <typename ST, typename MT, int Q, int QS, int NPTS, int S>
-__global__ void __launch_bounds__(kMaxLine) k_fused_step(FGeom G, FTables<MT> tb, FBufs<ST, MT> b, FParams P) {
+__global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGeom G, FTables<MT> tb, FBufs<ST, MT> b, FCoef<MT> P) {
     IPL_DYNAMIC_LDS(MT, lds_f);
-    MT* lds_g = lds_f + sdir_count<Q, S>() * G.ns;
-    constexpr int o1 = S == 0 ? 1 : 0, o2 = S == 2 ? 1 : 2;
+    MT* lds_g = lds_f + (sdir_count<Q, S>() + 1) * G.ns;
+    constexpr int o1 = LineAxes<S>::o1, o2 = LineAxes<S>::o2;
     const int i = threadIdx.x;
     const bool active = i < G.ns;
-    const int64_t L = blockIdx.x;
-    int cc[3];
-    cc[S] = active ? i : 0;
-    cc[o1] = (int)(L / G.n[o2]);
-    cc[o2] = (int)(L % G.n[o2]);
-    const int64_t n = L * G.ns + (active ? i : 0);
-    const int64_t NC = G.ncell;
-    const MT c = (MT)P.c, c2 = (MT)(P.c * P.c), dt = (MT)P.dt;
+    const int L = blockIdx.x;
+    const int n2 = G.n[o2];
+    const int c1 = L / n2, c2 = L - (L / n2) * n2;
+    const int ii = active ? i : 0;
+    const uint32_t n = (uint32_t)L * (uint32_t)G.ns + (uint32_t)ii;
+    const uint32_t NC = (uint32_t)G.ncell;
+    const ST* __restrict__ fsrc = b.f_src;
+    const ST* __restrict__ gsrc = b.g_src;
+    const MT c = P.c, ic = P.inv_c, ic2 = P.inv_c2;
+    const LineNb nb = line_neighbours<S>(G, c1, c2);
 
     MT Pf[Q], Pg[QS > 0 ? QS : 1];
     if (active) {
-        MT f[Q], g[QS > 0 ? QS : 1];
+        MT g[QS > 0 ? QS : 1];
 #pragma unroll
-        for (int k = 0; k < Q; ++k) f[k] = Store<ST, MT>::load(b.f_src + k * NC + n, Lattice<Q>::w(k), G.inv_scale);
+        for (int k = 0; k < Q; ++k) Pf[k] = Store<ST, MT>::load(elem(fsrc, k * NC + n), Lattice<Q>::w(k), G.inv_scale);
         if constexpr (QS > 0) {
 #pragma unroll
-            for (int k = 0; k < QS; ++k) g[k] = Store<ST, MT>::load(b.g_src + k * NC + n, Lattice<QS>::w(k), G.inv_scale);
+            for (int k = 0; k < QS; ++k) g[k] = Store<ST, MT>::load(elem(gsrc, k * NC + n), Lattice<QS>::w(k), G.inv_scale);
         }
         // ---- macroscopic fields of step n (update_rho, collision.cu:244-254) ----
-        MT p = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-        for (int k = 0; k < Q; ++k) {
-            const Dir d = Lattice<Q>::c(k);
-            p += f[k];
-            jx += MT(d.x) * c * f[k]; jy += MT(d.y) * c * f[k]; jz += MT(d.z) * c * f[k];
-        }
+        MT p = Pf[0], ex = 0, ey = 0, ez = 0;
+        static_for<Q - 1>([&](auto kc) {
+            constexpr int k = decltype(kc)::value + 1;
+            constexpr Dir d = Lattice<Q>::c(k);
+            p += Pf[k];
+            if constexpr (d.x > 0) ex += Pf[k]; else if constexpr (d.x < 0) ex -= Pf[k];
+            if constexpr (d.y > 0) ey += Pf[k]; else if constexpr (d.y < 0) ey -= Pf[k];
+            if constexpr (d.z > 0) ez += Pf[k]; else if constexpr (d.z < 0) ez -= Pf[k];
+        });
         MT sc = 0;
         if constexpr (QS > 0) {
+            sc = g[0];
 #pragma unroll
-            for (int k = 0; k < QS; ++k) sc += g[k];
+            for (int k = 1; k < QS; ++k) sc += g[k];
         }
         const MT S_ = G.scalar == IPL_SCALAR_SALINITY ? sc : MT(0);
         const MT phi = G.scalar == IPL_SCALAR_PHASE_FIELD ? sc : MT(0);
-        const MT rho = MT(P.rho_L) + (MT(P.rho_H) - MT(P.rho_L)) * phi + S_ * MT(P.beta_s);
-        const MT u = jx + b.F_old[n] * dt / (MT(2.0) * rho);
-        const MT v = jy + b.F_old[NC + n] * dt / (MT(2.0) * rho);
-        const MT w = jz + b.F_old[2 * NC + n] * dt / (MT(2.0) * rho);
-        // ---- equ_f (collision.cu:3-14) ----
-        const MT u2 = u * u + v * v + w * w;
-        MT feq[Q];
-#pragma unroll
-        for (int k = 0; k < Q; ++k) {
-            const Dir d = Lattice<Q>::c(k);
-            const MT t = (MT(d.x) * c * u + MT(d.y) * c * v + MT(d.z) * c * w) / c2;
-            feq[k] = MT(Lattice<Q>::w(k)) * (p + MT(3.0) * t + MT(4.5) * t * t - MT(1.5) * u2 / c2);
-        }
+        const MT rho = P.rho_L + P.drho * phi + S_ * P.beta_s;
+        const MT irho = MT(1) / rho;
+        const MT hr = P.half_dt * irho;
+        const MT u = ex * c + *elem(b.F_old, n) * hr;
+        const MT v = ey * c + *elem(b.F_old, NC + n) * hr;
+        const MT w = ez * c + *elem(b.F_old, 2 * NC + n) * hr;
         // ---- Force (collision.cu:57-112) ----
         constexpr bool d3 = Lattice<Q>::dim == 3;
         MT Fx = 0, Fy = 0, Fz = 0;
         MT gpx = 0, gpy = 0, gpz = 0;  // grad phi (col_PF reuses it)
-        const MT eps = MT(P.dz) / MT(1000);
-        if (G.need_grad) {
-            gpx = grad_axis<MT, S>(G, tb, b.phi_n, cc, 0, eps);
-            if (d3) gpy = grad_axis<MT, S>(G, tb, b.phi_n, cc, 1, eps);
-            gpz = grad_axis<MT, S>(G, tb, b.phi_n, cc, 2, eps);
+        if (G.grad_phi) {
+            gpx = grad_axis<MT, S, 0>(G, nb, tb, b.phi_n, c1, c2, ii);
+            if constexpr (d3) gpy = grad_axis<MT, S, 1>(G, nb, tb, b.phi_n, c1, c2, ii);
+            gpz = grad_axis<MT, S, 2>(G, nb, tb, b.phi_n, c1, c2, ii);
             // x term over delX^2, the term of directions 2/4 (z in 2D, y in 3D) over dz^2
             // (collision.cu:70-71)
             constexpr int a2 = d3 ? 1 : 2;
-            const MT wx = tb.wid[0][cc[0]], dzz = MT(P.dz);
-            const MT lap = (b.phi_n[neighbour<S>(G, cc, 0, 1)] - MT(2) * phi + b.phi_n[neighbour<S>(G, cc, 0, -1)]) / (wx * wx) +
-                           (b.phi_n[neighbour<S>(G, cc, a2, 1)] - MT(2) * phi + b.phi_n[neighbour<S>(G, cc, a2, -1)]) / (dzz * dzz);
-            const MT beta = MT(12.0) * MT(P.sigma) / MT(P.pf_thickness);
-            const MT kap = MT(3.0) * MT(P.sigma) * MT(P.pf_thickness) / MT(2.0);
-            const MT st = MT(4) * beta * phi * (phi - MT(1.0)) * (phi - MT(0.5)) - kap * lap;
+            const MT lap = (*elem(b.phi_n, nb_index<S, 0, 1>(G, nb, ii)) - MT(2) * phi +
+                            *elem(b.phi_n, nb_index<S, 0, -1>(G, nb, ii))) * *elem(tb.inv_wx2, (uint32_t)(S == 0 ? ii : (o1 == 0 ? c1 : c2))) +
+                           (*elem(b.phi_n, nb_index<S, a2, 1>(G, nb, ii)) - MT(2) * phi +
+                            *elem(b.phi_n, nb_index<S, a2, -1>(G, nb, ii))) * P.inv_dz2;
+            const MT st = P.st_beta * phi * (phi - MT(1.0)) * (phi - MT(0.5)) - P.kap * lap;
             Fx += st * gpx; Fy += st * gpy; Fz += st * gpz;
         }
-        if (P.boussinesq) Fz += -S_ * MT(P.beta_s) * MT(P.gravity);
-        else Fz += -rho * MT(P.gravity);
-        if (G.need_grad) {
-            const MT grx = grad_axis<MT, S>(G, tb, b.rho_n, cc, 0, eps);
-            const MT gry = d3 ? grad_axis<MT, S>(G, tb, b.rho_n, cc, 1, eps) : MT(0);
-            const MT grz = grad_axis<MT, S>(G, tb, b.rho_n, cc, 2, eps);
-            Fx += -p * c2 / MT(3.0) * grx;
-            Fy += -p * c2 / MT(3.0) * gry;
-            Fz += -p * c2 / MT(3.0) * grz;
+        if (P.boussinesq) Fz += -S_ * P.beta_s * P.gravity;
+        else Fz += -rho * P.gravity;
+        if (G.grad_rho) {
+            const MT pp = -p * P.press;
+            Fx += pp * grad_axis<MT, S, 0>(G, nb, tb, b.rho_n, c1, c2, ii);
+            if constexpr (d3) Fy += pp * grad_axis<MT, S, 1>(G, nb, tb, b.rho_n, c1, c2, ii);
+            Fz += pp * grad_axis<MT, S, 2>(G, nb, tb, b.rho_n, c1, c2, ii);
         }
-        b.F_new[n] = Fx; b.F_new[NC + n] = Fy; b.F_new[2 * NC + n] = Fz;
-        MT fk[Q];
-        const MT den = rho * c2 / MT(3.0);
-#pragma unroll
-        for (int k = 0; k < Q; ++k) {
-            const Dir d = Lattice<Q>::c(k);
-            fk[k] = MT(Lattice<Q>::w(k)) * dt * (MT(d.x) * c * Fx + MT(d.y) * c * Fy + MT(d.z) * c * Fz) / den;
-        }
-        // ---- collision ----
-        collide_cell<MT, Q>(f, feq, fk, MT(P.tau), P.rates, P.collision, Pf);
+        *elem(b.F_new, n) = Fx; *elem(b.F_new, NC + n) = Fy; *elem(b.F_new, 2 * NC + n) = Fz;
+        // ---- equ_f, forcing term and collision ----
+        const MT u2c = (u * u + v * v + w * w) * ic2;
+        const MT fkc = P.fk_coef * irho;
+        collide_pairs<MT, Q>(Pf, p, u, v, w, u2c, Fx, Fy, Fz, fkc, ic, P);
         // ---- scalar collision ----
         if constexpr (QS > 0) {
             if (G.scalar == IPL_SCALAR_SALINITY) {  // col_g_reg (collision.cu:167-188)
                 constexpr bool linear = QS == 7;
                 const MT cs2inv = linear ? MT(4.0) : MT(3.0);
                 MT geq[QS];
-                MT gx = 0, gy = 0, gz = 0;
-#pragma unroll
-                for (int k = 0; k < QS; ++k) {
-                    const Dir d = Lattice<QS>::c(k);
-                    const MT t = (MT(d.x) * c * u + MT(d.y) * c * v + MT(d.z) * c * w) / c2;
+                MT jx = 0, jy = 0, jz = 0;
+                static_for<QS>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    constexpr Dir d = Lattice<QS>::c(k);
+                    const MT t = dot3<d.x, d.y, d.z>(u, v, w) * ic;
                     MT e = MT(1.0) + cs2inv * t;
-                    if (!linear) e += MT(4.5) * t * t - MT(1.5) * u2 / c2;
+                    if constexpr (!linear) e += MT(4.5) * t * t - MT(1.5) * u2c;
                     geq[k] = MT(Lattice<QS>::w(k)) * S_ * e;
                     const MT gn = g[k] - geq[k];
-                    gx += gn * MT(d.x) * c; gy += gn * MT(d.y) * c; gz += gn * MT(d.z) * c;
-                }
-                const MT om = MT(1.0) - MT(1.0) / MT(P.taus);
-#pragma unroll
-                for (int k = 0; k < QS; ++k) {
-                    const Dir d = Lattice<QS>::c(k);
-                    const MT gneq = MT(Lattice<QS>::w(k)) * cs2inv * (gx * MT(d.x) * c + gy * MT(d.y) * c + gz * MT(d.z) * c) / c2;
+                    if constexpr (d.x > 0) jx += gn; else if constexpr (d.x < 0) jx -= gn;
+                    if constexpr (d.y > 0) jy += gn; else if constexpr (d.y < 0) jy -= gn;
+                    if constexpr (d.z > 0) jz += gn; else if constexpr (d.z < 0) jz -= gn;
+                });
+                const MT om = P.om_s;
+                static_for<QS>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    constexpr Dir d = Lattice<QS>::c(k);
+                    const MT gneq = MT(Lattice<QS>::w(k)) * cs2inv * dot3<d.x, d.y, d.z>(jx, jy, jz);
                     Pg[k] = geq[k] + om * gneq;
-                }
+                });
             } else {  // col_PF (collision.cu:190-223)
-                const MT nabu = (phi * u - b.phiold[n] * b.uold[n]) / dt;
-                const MT nabv = (phi * v - b.phiold[n] * b.vold[n]) / dt;
-                const MT nabw = (phi * w - b.phiold[n] * b.wold[n]) / dt;
-                b.phiold[n] = phi; b.uold[n] = u; b.vold[n] = v; b.wold[n] = w;
-                const MT nrm = sqrt(gpx * gpx + gpy * gpy + gpz * gpz + MT(1e-8));
-                const MT src = (MT(1.0) - MT(4.0) * (phi - MT(0.5)) * (phi - MT(0.5))) / MT(P.pf_thickness);
-                const MT tp = MT(P.pf_tau);
-#pragma unroll
-                for (int k = 0; k < QS; ++k) {
-                    const Dir d = Lattice<QS>::c(k);
-                    const MT cx = MT(d.x) * c, cy = MT(d.y) * c, cz = MT(d.z) * c;
-                    const MT t = (cx * u + cy * v + cz * w) / c2;
-                    const MT geq = MT(Lattice<QS>::w(k)) * phi * (MT(1.0) + MT(3.0) * t + MT(4.5) * t * t - MT(1.5) * u2 / c2);
-                    const MT Fkp = dt * MT(Lattice<QS>::w(k)) *
-                                   (src * (cx * gpx + cy * gpy + cz * gpz) / nrm + MT(3) * (cx * nabu + cy * nabv + cz * nabw) / c2);
-                    Pg[k] = g[k] - (g[k] - geq - Fkp * (tp - MT(0.5))) / tp;
-                }
+                const MT idt = P.inv_dt;
+                const MT nabu = (phi * u - *elem(b.phiold, n) * *elem(b.uold, n)) * idt;
+                const MT nabv = (phi * v - *elem(b.phiold, n) * *elem(b.vold, n)) * idt;
+                const MT nabw = (phi * w - *elem(b.phiold, n) * *elem(b.wold, n)) * idt;
+                *elem(b.phiold, n) = phi; *elem(b.uold, n) = u; *elem(b.vold, n) = v; *elem(b.wold, n) = w;
+                const MT inrm = MT(1) / sqrt(gpx * gpx + gpy * gpy + gpz * gpz + MT(1e-8));
+                const MT src = (MT(1.0) - MT(4.0) * (phi - MT(0.5)) * (phi - MT(0.5))) * P.inv_pfW * c * inrm;
+                const MT nc = MT(3) * c * ic2;
+                const MT dtc = P.dt;
+                const MT itp = P.pf_itau, fco = P.pf_fcoef;
+                static_for<QS>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    constexpr Dir d = Lattice<QS>::c(k);
+                    constexpr double wk = Lattice<QS>::w(k);
+                    const MT t = dot3<d.x, d.y, d.z>(u, v, w) * ic;
+                    const MT geq = MT(wk) * phi * (MT(1.0) + MT(3.0) * t + MT(4.5) * t * t - MT(1.5) * u2c);
+                    const MT Fkp = dtc * MT(wk) *
+                                   (src * dot3<d.x, d.y, d.z>(gpx, gpy, gpz) + nc * dot3<d.x, d.y, d.z>(nabu, nabv, nabw));
+                    Pg[k] = g[k] - (g[k] - geq - Fkp * fco) * itp;
+                });
             }
         }
     }
     // ---- interpolated propagation of f and g ----
-    transport<ST, MT, Q, NPTS, S>(G, tb, b.f_dst, Pf, lds_f, i, active, cc);
-    if constexpr (QS > 0) transport<ST, MT, QS, NPTS, S>(G, tb, b.g_dst, Pg, lds_g, i, active, cc);
+    transport<ST, MT, Q, NPTS, S>(G, nb, tb, b.f_dst, Pf, lds_f, ii, active);
+    if constexpr (QS > 0) transport<ST, MT, QS, NPTS, S>(G, nb, tb, b.g_dst, Pg, lds_g, ii, active);
 }
 
 // rho (and phi) of step n for the gradients of the next fused step
 template <typename ST, typename MT, int QS>
-__global__ void k_scalar_fields(FGeom G, const ST* g_src, MT* rho_n, MT* phi_n, FParams P) {
+__global__ void k_scalar_fields(FGeom G, const ST* g_src, MT* rho_n, MT* phi_n, FCoef<MT> P) {
     const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (n >= G.ncell) return;
     MT sc = 0;
     if constexpr (QS > 0) {
+        sc = Store<ST, MT>::load(g_src + n, Lattice<QS>::w(0), G.inv_scale);
 #pragma unroll
-        for (int k = 0; k < QS; ++k) sc += Store<ST, MT>::load(g_src + k * G.ncell + n, Lattice<QS>::w(k), G.inv_scale);
+        for (int k = 1; k < QS; ++k) sc += Store<ST, MT>::load(g_src + k * G.ncell + n, Lattice<QS>::w(k), G.inv_scale);
     }
     const MT S = G.scalar == IPL_SCALAR_SALINITY ? sc : MT(0);
     const MT phi = G.scalar == IPL_SCALAR_PHASE_FIELD ? sc : MT(0);
-    rho_n[n] = MT(P.rho_L) + (MT(P.rho_H) - MT(P.rho_L)) * phi + S * MT(P.beta_s);
-    phi_n[n] = phi;
+    if (G.grad_rho) rho_n[n] = P.rho_L + P.drho * phi + S * P.beta_s;
+    if (G.grad_phi) phi_n[n] = phi;
 }
 
 // readout: p, u, v, w, rho, S, phi of the current state into [7][ncell] doubles
@@ -488,7 +672,7 @@ __global__ void k_store_pops(FGeom G, const double* src, ST* dst, int q) {
 
 // ---- launchers (declared in fused_common.hpp, instantiated in fused_q*.hip) ---------------
 template <typename ST, typename MT, int Q, int QS>
-void fused_launch_step(const FGeom& G, const FTables<MT>& tb, const FBufs<ST, MT>& b, const FParams& P, int npts,
+void fused_launch_step(const FGeom& G, const FTables<MT>& tb, const FBufs<ST, MT>& b, const FCoef<MT>& P, int npts,
                        unsigned block, size_t lds, hipStream_t st) {
     const unsigned grid = (unsigned)G.lines;
 #define IPL_FSTEP(NP, SS) IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS>), grid, block, lds, st, G, tb, b, P)
@@ -504,15 +688,18 @@ void fused_launch_step(const FGeom& G, const FTables<MT>& tb, const FBufs<ST, MT
 
 template <typename ST, typename MT, int Q, int QS>
 size_t fused_lds_bytes(int s, int ns) {
-    int nf = 0, ng = 0;
+    // one padded row of the S-directions' values per cell for each population set
+    int nf = 1, ng = 0;
     for (int k = 0; k < Q; ++k) nf += comp<Q>(k, s) != 0;
-    if constexpr (QS > 0)
+    if constexpr (QS > 0) {
+        ng = 1;
         for (int k = 0; k < QS; ++k) ng += comp<QS>(k, s) != 0;
+    }
     return (size_t)(nf + ng) * ns * sizeof(MT);
 }
 
 template <typename ST, typename MT, int QS>
-void fused_launch_scalar_fields(const FGeom& G, const ST* g_src, MT* rho_n, MT* phi_n, const FParams& P, hipStream_t st) {
+void fused_launch_scalar_fields(const FGeom& G, const ST* g_src, MT* rho_n, MT* phi_n, const FCoef<MT>& P, hipStream_t st) {
     IPL_LAUNCH((k_scalar_fields<ST, MT, QS>), (G.ncell + 255) / 256, 256, 0, st, G, g_src, rho_n, phi_n, P);
 }
 
@@ -535,13 +722,13 @@ void fused_launch_convert(const FGeom& G, const ST* src, double* dst, int q, boo
 // explicit instantiation of every launcher of one (storage, math, Q, QS) combination
 #define IPL_FUSED_INSTANTIATE(ST, MT, Q, QS)                                                                \
     template void ipl::fusedk::fused_launch_step<ST, MT, Q, QS>(                                             \
-        const FGeom&, const FTables<MT>&, const FBufs<ST, MT>&, const FParams&, int, unsigned, size_t, hipStream_t); \
+        const FGeom&, const FTables<MT>&, const FBufs<ST, MT>&, const FCoef<MT>&, int, unsigned, size_t, hipStream_t); \
     template size_t ipl::fusedk::fused_lds_bytes<ST, MT, Q, QS>(int, int);                                    \
     template void ipl::fusedk::fused_launch_readout<ST, MT, Q, QS>(const FGeom&, const ST*, const ST*, const MT*, \
                                                                     double*, const FParams&, hipStream_t);
 #define IPL_FUSED_INSTANTIATE_SCALAR(ST, MT, QS)                                                           \
     template void ipl::fusedk::fused_launch_scalar_fields<ST, MT, QS>(const FGeom&, const ST*, MT*, MT*,    \
-                                                                       const FParams&, hipStream_t);
+                                                                       const FCoef<MT>&, hipStream_t);
 #define IPL_FUSED_INSTANTIATE_POPS(ST, MT, Q)                                                               \
     template void ipl::fusedk::fused_launch_convert<ST, MT, Q>(const FGeom&, const ST*, double*, int, bool, ST*, \
                                                                 hipStream_t);

@@ -2,6 +2,7 @@
 #include "fused_kernels.hpp"
 
 using ipl::fusedk::FBufs;
+using ipl::fusedk::FCoef;
 using ipl::fusedk::FGeom;
 using ipl::fusedk::FParams;
 using ipl::fusedk::FTables;

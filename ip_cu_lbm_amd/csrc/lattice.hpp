@@ -145,4 +145,12 @@ template <int Q> constexpr IPL_HD bool is_stress(int i) {
     return false;
 }
 
+// run-time Q forms for host code
+inline double moment_row_norm2(int Q, int i) {
+    return Q == 9 ? moment_norm2<9>(i) : (Q == 19 ? moment_norm2<19>(i) : moment_norm2<27>(i));
+}
+inline bool moment_row_is_stress(int Q, int i) {
+    return Q == 9 ? is_stress<9>(i) : (Q == 19 ? is_stress<19>(i) : is_stress<27>(i));
+}
+
 }  // namespace ipl
