@@ -227,8 +227,8 @@ __device__ inline void push(const FGeom& G, const LineNb& nb, ST* __restrict__ d
 // the S-directions' values per cell (SL = their count + 1, an odd stride for the LDS banks), so a
 // direction's slot is an immediate offset from the cell's row.
 template <typename ST, typename MT, int Q, int NPTS, int S>
-__device__ inline void transport(const FGeom& G, const LineNb& nb, const FTables<MT>& tb, ST* __restrict__ dst,
-                                 const MT (&P)[Q], MT* lds, int i, bool active) {
+__device__ inline void transport(const FGeom& G, const LineNb& nb, const MT (&Wp)[NPTS], const MT (&Wm)[NPTS], MT rn,
+                                 ST* __restrict__ dst, const MT (&P)[Q], MT* lds, int i, bool active) {
     const int ns = G.ns;
     constexpr int SL = sdir_count<Q, S>() + 1;
     if (!G.flux) {
@@ -249,14 +249,6 @@ __device__ inline void transport(const FGeom& G, const LineNb& nb, const FTables
     // 2. slab values F (adv.cu:76-87); stencil points beyond a wall repeat the wall cell
     MT Fv[Q];
     const int per = G.periodic[S];
-    MT Wp[NPTS], Wm[NPTS];
-    if (active) {
-#pragma unroll
-        for (int j = 0; j < NPTS; ++j) {
-            Wp[j] = *elem(tb.W, (uint32_t)(j * ns + i));
-            Wm[j] = *elem(tb.W, (uint32_t)((NPTS + j) * ns + i));
-        }
-    }
     static_for<Q>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         constexpr int sg = comp<Q>(k, S);
@@ -283,7 +275,6 @@ __device__ inline void transport(const FGeom& G, const LineNb& nb, const FTables
         });
     __syncthreads();
     if (!active) return;
-    const MT rn = *elem(tb.rdel, (uint32_t)i);
     static_for<Q>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         constexpr int sg = comp<Q>(k, S);
@@ -602,26 +593,69 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
             }
         }
     }
-    // ---- interpolated propagation of f and g ----
-    transport<ST, MT, Q, NPTS, S>(G, nb, tb, b.f_dst, Pf, lds_f, ii, active);
-    if constexpr (QS > 0) transport<ST, MT, QS, NPTS, S>(G, nb, tb, b.g_dst, Pg, lds_g, ii, active);
+    // ---- interpolated propagation of f and g (one set of slab weights for both) ----
+    MT Wp[NPTS], Wm[NPTS], rn = MT(0);
+    if (G.flux) {
+#pragma unroll
+        for (int j = 0; j < NPTS; ++j) {
+            Wp[j] = *elem(tb.W, (uint32_t)(j * G.ns + ii));
+            Wm[j] = *elem(tb.W, (uint32_t)((NPTS + j) * G.ns + ii));
+        }
+        rn = *elem(tb.rdel, (uint32_t)ii);
+    }
+    transport<ST, MT, Q, NPTS, S>(G, nb, Wp, Wm, rn, b.f_dst, Pf, lds_f, ii, active);
+    if constexpr (QS > 0) transport<ST, MT, QS, NPTS, S>(G, nb, Wp, Wm, rn, b.g_dst, Pg, lds_g, ii, active);
 }
 
-// rho (and phi) of step n for the gradients of the next fused step
+// rho (and phi) of step n for the gradients of the next fused step; four consecutive cells per
+// work-item with 4-wide loads when the rows allow it (ncell a multiple of 4)
+template <typename T> struct alignas(4 * sizeof(T)) Vec4 { T v[4]; };
+
 template <typename ST, typename MT, int QS>
 __global__ void k_scalar_fields(FGeom G, const ST* g_src, MT* rho_n, MT* phi_n, FCoef<MT> P) {
-    const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (n >= G.ncell) return;
-    MT sc = 0;
+    const int64_t n0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * 4;
+    if (n0 >= G.ncell) return;
+    const int64_t NC = G.ncell;
+    MT sc[4] = {MT(0), MT(0), MT(0), MT(0)};
+    const bool vec = (NC & 3) == 0;
     if constexpr (QS > 0) {
-        sc = Store<ST, MT>::load(g_src + n, Lattice<QS>::w(0), G.inv_scale);
 #pragma unroll
-        for (int k = 1; k < QS; ++k) sc += Store<ST, MT>::load(g_src + k * G.ncell + n, Lattice<QS>::w(k), G.inv_scale);
+        for (int k = 0; k < QS; ++k) {
+            ST x[4];
+            if (vec) {
+                const Vec4<ST> v = *reinterpret_cast<const Vec4<ST>*>(g_src + k * NC + n0);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) x[j] = v.v[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) x[j] = n0 + j < NC ? g_src[k * NC + n0 + j] : ST{};
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const MT y = Store<ST, MT>::load(&x[j], Lattice<QS>::w(k), G.inv_scale);
+                sc[j] = k == 0 ? y : sc[j] + y;
+            }
+        }
     }
-    const MT S = G.scalar == IPL_SCALAR_SALINITY ? sc : MT(0);
-    const MT phi = G.scalar == IPL_SCALAR_PHASE_FIELD ? sc : MT(0);
-    if (G.grad_rho) rho_n[n] = P.rho_L + P.drho * phi + S * P.beta_s;
-    if (G.grad_phi) phi_n[n] = phi;
+    Vec4<MT> r, ph;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const MT S = G.scalar == IPL_SCALAR_SALINITY ? sc[j] : MT(0);
+        const MT phi = G.scalar == IPL_SCALAR_PHASE_FIELD ? sc[j] : MT(0);
+        r.v[j] = P.rho_L + P.drho * phi + S * P.beta_s;
+        ph.v[j] = phi;
+    }
+    if (vec) {
+        if (G.grad_rho) *reinterpret_cast<Vec4<MT>*>(rho_n + n0) = r;
+        if (G.grad_phi) *reinterpret_cast<Vec4<MT>*>(phi_n + n0) = ph;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (n0 + j < NC) {
+                if (G.grad_rho) rho_n[n0 + j] = r.v[j];
+                if (G.grad_phi) phi_n[n0 + j] = ph.v[j];
+            }
+    }
 }
 
 // readout: p, u, v, w, rho, S, phi of the current state into [7][ncell] doubles
@@ -700,7 +734,8 @@ size_t fused_lds_bytes(int s, int ns) {
 
 template <typename ST, typename MT, int QS>
 void fused_launch_scalar_fields(const FGeom& G, const ST* g_src, MT* rho_n, MT* phi_n, const FCoef<MT>& P, hipStream_t st) {
-    IPL_LAUNCH((k_scalar_fields<ST, MT, QS>), (G.ncell + 255) / 256, 256, 0, st, G, g_src, rho_n, phi_n, P);
+    const int64_t items = (G.ncell + 3) / 4;
+    IPL_LAUNCH((k_scalar_fields<ST, MT, QS>), (items + 255) / 256, 256, 0, st, G, g_src, rho_n, phi_n, P);
 }
 
 template <typename ST, typename MT, int Q, int QS>

@@ -153,6 +153,18 @@ def test_fused_solitary_wave_fp64(oracle_lib):
     _compare(case, 150, ("p", "vel", "phi"), 1e-12, oracle_lib)
 
 
+@pytest.mark.parametrize("walls", [False, True])
+def test_fused_ragged_sizes_fp64(oracle_lib, walls):
+    """Cell counts that are not multiples of 4 or 64 (scalar-field pass tail, partial wavefront
+    lines), with and without walls on every face."""
+    case = cases.lock_exchange(nx=11, ny=3, nz=9, path=PATH_FUSED)
+    if walls:
+        case = case.replace(periodic=(0, 0, 0))
+    _compare(case, 120, ("p", "vel", "S"), 1e-12, oracle_lib)
+    pf = cases.solitary_wave(nx=33, nz=21, x0=10, sigma=1e-4, path=PATH_FUSED)
+    _compare(pf, 120, ("p", "vel", "phi"), 1e-12, oracle_lib)
+
+
 def test_fused_lock_exchange_fp32(oracle_lib):
     case = cases.lock_exchange(nx=32, ny=6, nz=16, storage=F32, path=PATH_FUSED)
     _compare(case, 150, ("vel", "S"), 1e-5, oracle_lib)
