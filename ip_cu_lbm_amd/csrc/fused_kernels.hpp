@@ -275,28 +275,60 @@ __device__ inline void transport(const FGeom& G, const LineNb& nb, const MT (&Wp
         });
     __syncthreads();
     if (!active) return;
+    constexpr int o1 = LineAxes<S>::o1, o2 = LineAxes<S>::o2;
+    const uint32_t NC = (uint32_t)G.ncell;
     static_for<Q>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         constexpr int sg = comp<Q>(k, S);
+        constexpr int ka = comp<Q>(k, o1), kb = comp<Q>(k, o2);
+        constexpr double wk = Lattice<Q>::w(k);
         if constexpr (sg == 0) {
-            // no component along S: set_tmp returns P exactly (adv.cu:64-65,241)
-            push<ST, MT, Q, k, S>(G, nb, dst, i, P[k]);
-        } else {
-            constexpr int slot = sdir_slot<Q, S>(k);
-            // U of this cell is the transported value of m = n - sg S
-            int im = i - sg;
-            bool have = im >= 0 && im < ns;
-            if (!have && per) { im = im < 0 ? im + ns : im - ns; have = true; }
-            if (have) {
-                const MT Um = P[k] + (lds[im * SL + slot] * rn - Fv[k] * rn);
-                push<ST, MT, Q, k, S>(G, nb, dst, im, Um);
-            }
-            // downstream wall: the ghost's mirrored copy (adv.cu:196-199 along S)
-            const int ip = i + sg;
-            if (!per && (ip < 0 || ip >= ns)) {
-                constexpr int mk = mirror<Q>(k, S);
-                const MT Tm = P[mk] + (Fv[k] * rn - Fv[mk] * rn);
-                push<ST, MT, Q, k, S>(G, nb, dst, i, Tm);
+            // no component along S: set_tmp returns P exactly (adv.cu:64-65,241); the destination
+            // keeps the S-index
+            if (!nb.template out1<ka>() && !nb.template out2<kb>())
+                Store<ST, MT>::store(elem(dst, k * NC + nb.template base<ka, kb>() + (uint32_t)i), P[k], wk, G.scale);
+            else
+                push<ST, MT, Q, k, S>(G, nb, dst, i, P[k]);
+        } else if constexpr (sg > 0) {
+            // the pair (k, kb_) of opposite S-directions.  U of this cell is the transported value
+            // of m = n - sg S, which lands back on this cell's S-index: Uk comes from m = i - 1,
+            // Ukb from m = i + 1.  At the downstream S-wall of a direction the wall cell also
+            // pushes the ghost's mirrored copy T (adv.cu:196-199 along S): the last cell for k, the
+            // first for kb_.  The first cell has no Uk and the last no Ukb, so each work-item
+            // stores exactly two values: (Uk or T of kb_) and (Ukb or T of k).
+            constexpr int kb_ = opposite<Q>(k);
+            constexpr int mk = mirror<Q>(k, S), mkb = mirror<Q>(kb_, S);
+            constexpr int sk = sdir_slot<Q, S>(k), skb = sdir_slot<Q, S>(kb_);
+            const int im = wrap_or_clamp(i - 1, ns, per), ip = wrap_or_clamp(i + 1, ns, per);
+            const MT Uk = P[k] + (lds[im * SL + sk] * rn - Fv[k] * rn);
+            const MT Ukb = P[kb_] + (lds[ip * SL + skb] * rn - Fv[kb_] * rn);
+            const bool first = !per && i == 0, last = !per && i == ns - 1;
+            const bool lines_in = !nb.template out1<ka>() && !nb.template out2<kb>() &&
+                                  !nb.template out1<-ka>() && !nb.template out2<-kb>();
+            if (lines_in) {
+                const MT Tkb = P[mkb] + (Fv[kb_] * rn - Fv[mkb] * rn);
+                const MT Tk = P[mk] + (Fv[k] * rn - Fv[mk] * rn);
+                const uint32_t fwd = nb.template base<ka, kb>() + (uint32_t)i;
+                const uint32_t bwd = nb.template base<-ka, -kb>() + (uint32_t)i;
+                uint32_t a1, a2;
+                if (G.wall_bc == IPL_BC_BOUNCE_BACK) {
+                    // T of kb_ at the first cell returns as k into that cell; T of k at the last as kb_
+                    a1 = k * NC + (first ? nb.b00 + (uint32_t)i : fwd);
+                    a2 = kb_ * NC + (last ? nb.b00 + (uint32_t)i : bwd);
+                } else {
+                    // slip: reflected across the S-wall, the line offset kept
+                    constexpr int rkb = reflect_dir<Q>(kb_, 1 << S), rk = reflect_dir<Q>(k, 1 << S);
+                    a1 = first ? rkb * NC + bwd : k * NC + fwd;
+                    a2 = last ? rk * NC + fwd : kb_ * NC + bwd;
+                }
+                Store<ST, MT>::store(elem(dst, a1), first ? Tkb : Uk, wk, G.scale);
+                Store<ST, MT>::store(elem(dst, a2), last ? Tk : Ukb, wk, G.scale);
+            } else {
+                // boundary lines: the general redirect of push()
+                if (!first) push<ST, MT, Q, k, S>(G, nb, dst, im, Uk);
+                if (!last) push<ST, MT, Q, kb_, S>(G, nb, dst, ip, Ukb);
+                if (last) push<ST, MT, Q, k, S>(G, nb, dst, i, P[mk] + (Fv[k] * rn - Fv[mk] * rn));
+                if (first) push<ST, MT, Q, kb_, S>(G, nb, dst, i, P[mkb] + (Fv[kb_] * rn - Fv[mkb] * rn));
             }
         }
     });
