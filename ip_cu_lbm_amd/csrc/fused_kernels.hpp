@@ -120,6 +120,16 @@ template <typename T> __device__ inline T* elem(T* row, uint32_t idx) {
                                 (uint64_t)(idx * (uint32_t)sizeof(T)));
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  Line L of the
+// grid goes to the XCD that owns a contiguous block of lines, so the neighbour lines a line
+// gathers from (and pushes to) are mostly processed by the same XCD at the same time.
+constexpr int kXcds = 8;
+__device__ inline int xcd_line(int b, int lines) {
+    const int q = lines / kXcds, r = lines % kXcds;
+    const int x = b % kXcds, slot = b / kXcds;
+    return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
 template <int S> struct LineAxes {
     static constexpr int o1 = S == 0 ? 1 : 0, o2 = S == 2 ? 1 : 2;
 };
@@ -500,7 +510,7 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
     constexpr int o1 = LineAxes<S>::o1, o2 = LineAxes<S>::o2;
     const int i = threadIdx.x;
     const bool active = i < G.ns;
-    const int L = blockIdx.x;
+    const int L = xcd_line((int)blockIdx.x, (int)G.lines);
     const int n2 = G.n[o2];
     const int c1 = L / n2, c2 = L - (L / n2) * n2;
     const int ii = active ? i : 0;
