@@ -168,6 +168,55 @@ enum {
 };
 int ipl_stage(ipl_ctx* ctx, int32_t stage);
 
+/* ---- immersed boundary: one rigid body carried by Lagrangian markers (function/ibm.cu) -------
+ * Replaces the drivers' marker set-up (SphereFall3D/SphereFall.cu:197-261) and adds the IB
+ * sub-step to every ipl_step, in the drivers' order (SphereFall.cu:372-383, after update_rho and
+ * resetF): marker forces and spreading, update_velIBM, update_IBbody, update_IBpoint,
+ * search_IBlattice. */
+enum ipl_ib_method {
+    IPL_IB_STRESS = 0,  /* stress-based wall force of Suzuki & Yoshino, get_IBMGw2 (ibm.cu:124-274) */
+    IPL_IB_DIRECT = 1   /* direct forcing, IB_directForcing (ibm.cu:276-344); iterations > 1 repeats
+                           interpolate-force-spread with corrected node velocities (multi-direct forcing) */
+};
+enum ipl_ib_kernel {
+    IPL_IB_KERNEL_REFERENCE = 0, /* the reference's three nodes per axis: STRESS weights x (and z in 2D)
+                                    by the truncated 4-point Peskin function weightFunction (ibm.cu:61-66),
+                                    y and z (3D) by 3-point Lagrange; DIRECT uses 3-point Lagrange */
+    IPL_IB_KERNEL_PESKIN4 = 1    /* DIRECT only: the full 4-point Peskin support on every axis */
+};
+typedef struct ipl_ib_config {
+    int32_t method;               /* IPL_IB_STRESS or IPL_IB_DIRECT */
+    int32_t kernel;               /* IPL_IB_KERNEL_* */
+    int32_t iterations;           /* DIRECT: forcing passes per step (1 = IB_directForcing) */
+    int32_t moving;               /* 1: update_IBbody integrates the body; 0: fixed body (forces still summed) */
+    int32_t n_markers;
+    const double* marker_pos;     /* 3*n offsets from the body centre in the body frame (oposw) */
+    const double* marker_normal;  /* 3*n outward unit normals in the body frame (onB_vec) */
+    double spacing;               /* marker spacing dIBM (items[IDX_dIBM], SphereFall.cu:277) */
+    double rho_f;                 /* fluid density: rhof of get_IBMGw2 / update_IBbody; the weight of the
+                                     body force of direct forcing (the literal 1000 of ibm.cu:342) */
+    double density;               /* body density densB */
+    double mass;                  /* body mass massB */
+    double inertia[3];            /* principal moments of inertia (IB) */
+    double gravity[3];            /* acceleration in the buoyancy term of update_IBbody: the reference
+                                     adds (1 - rhof/densB) m 9.81 along +z (ibm.cu:391) */
+    double pos[3], vel[3];        /* initial body centre and velocity */
+    double omega[3];              /* initial angular velocity, body frame (angleVB) */
+    double quat[4];               /* initial orientation quaternion (q0, q1, q2, q3) */
+    int32_t reserved[8];
+} ipl_ib_config;
+
+/* Attaches the body to a context (once, before the first step); nodes of a marker must lie inside
+ * the box: the reference's ghost positions make the 3-point Lagrange weights singular there. */
+int ipl_ib_attach(ipl_ctx* ctx, const ipl_ib_config* ib);
+/* FP64 body state after the last step: pos[3] vel[3] omega[3] quat[4] force[3] torque[3] (19 values;
+ * force = sum of marker forces + buoyancy, torque about the centre, as in update_IBbody). */
+int ipl_ib_body_state(ipl_ctx* ctx, double out[19]);
+/* Marker arrays (IB_csv, ibm.cu:3-14): positions, velocities, forces Gw, normals (3 per marker) or
+ * the nearest calc cell of each marker (1 per marker, canonical index). */
+enum ipl_ib_array { IPL_IB_POS = 0, IPL_IB_VEL = 1, IPL_IB_FORCE = 2, IPL_IB_NORMAL = 3, IPL_IB_NODE = 4 };
+int ipl_ib_read_markers(ipl_ctx* ctx, int32_t which, double* out, size_t count);
+
 /* ---- host-only diagnostics ------------------------------------------------------------------
  * Builds the cell graph of cfg on the host (no device work) and copies one table out:
  * which = 0 neighbour table [k][n_calc + n_ghost] (ghosts after calc cells, basic.cu:203-237),

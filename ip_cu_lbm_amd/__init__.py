@@ -18,7 +18,8 @@ import ctypes
 import numpy as np
 
 from . import _abi, cases
-from ._abi import (BC_BOUNCE_BACK, BC_SLIP, COLL_MRT, COLL_SRT, F16, F32, F64, FIELDS, FLUX_NONE,
+from ._abi import (BC_BOUNCE_BACK, BC_SLIP, COLL_MRT, COLL_SRT, F16, F32, F64, FIELDS, FLUX_NONE, IB_DIRECT,
+                   IB_KERNEL_PESKIN4, IB_KERNEL_REFERENCE, IB_STRESS,
                    FLUX_REFERENCE, FLUX_X, FLUX_Y, FLUX_Z, PATH_AUTO, PATH_FUSED, PATH_PLAIN,
                    SCALAR_NONE, SCALAR_PHASE_FIELD, SCALAR_SALINITY, STAGES, IplError)
 
@@ -47,6 +48,8 @@ class Simulation:
         n, g, b, p = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
         self._lib.ipl_info(h, ctypes.byref(n), ctypes.byref(g), ctypes.byref(b), ctypes.byref(p))
         self.n_calc, self.n_ghost, self.device_bytes, self.path = n.value, g.value, b.value, p.value
+        if case.body is not None:
+            self.ib_attach(case.body)
 
     # ---- lifecycle --------------------------------------------------------------------------
     def close(self):
@@ -114,3 +117,24 @@ class Simulation:
         out = np.zeros(3)
         self._check(self._lib.ipl_global_sums(self._h, _dp(out)))
         return out
+
+    # ---- immersed boundary ------------------------------------------------------------------
+    def ib_attach(self, body: "cases.Body"):
+        cfg, keep = body.to_c()
+        self._check(self._lib.ipl_ib_attach(self._h, ctypes.byref(cfg)))
+        self._n_markers = len(body.markers)
+        del keep
+
+    def body_state(self) -> dict:
+        """FP64 body state: pos, vel, omega (body frame), quat, force, torque."""
+        out = np.zeros(19)
+        self._check(self._lib.ipl_ib_body_state(self._h, _dp(out)))
+        return {"pos": out[0:3], "vel": out[3:6], "omega": out[6:9], "quat": out[9:13], "force": out[13:16],
+                "torque": out[16:19]}
+
+    def markers(self, which: str) -> np.ndarray:
+        """Marker arrays (IB_csv): 'pos', 'vel', 'force', 'normal' (M x 3) or 'node' (M)."""
+        m = self._n_markers
+        out = np.empty(m if which == "node" else 3 * m)
+        self._check(self._lib.ipl_ib_read_markers(self._h, _abi.IB_ARRAYS[which], _dp(out), out.size))
+        return out if which == "node" else out.reshape(m, 3)

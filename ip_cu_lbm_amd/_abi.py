@@ -59,6 +59,22 @@ class IplInit(ctypes.Structure):
     _fields_ = [(n, _DP) for n in ("p", "u", "v", "w", "S", "phi", "tau", "taus")]
 
 
+IB_STRESS, IB_DIRECT = 0, 1
+IB_KERNEL_REFERENCE, IB_KERNEL_PESKIN4 = 0, 1
+IB_ARRAYS = {"pos": 0, "vel": 1, "force": 2, "normal": 3, "node": 4}
+
+
+class IplIbConfig(ctypes.Structure):
+    _fields_ = [
+        ("method", c_int32), ("kernel", c_int32), ("iterations", c_int32), ("moving", c_int32),
+        ("n_markers", c_int32), ("marker_pos", _DP), ("marker_normal", _DP),
+        ("spacing", c_double), ("rho_f", c_double), ("density", c_double), ("mass", c_double),
+        ("inertia", c_double * 3), ("gravity", c_double * 3),
+        ("pos", c_double * 3), ("vel", c_double * 3), ("omega", c_double * 3), ("quat", c_double * 4),
+        ("reserved", c_int32 * 8),
+    ]
+
+
 def lib_path() -> str:
     return os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libipl.so")
 
@@ -104,6 +120,12 @@ def load_library(path: str = None):
     lib.ipl_stage.restype = c_int32
     lib.ipl_build_tables.argtypes = [POINTER(IplConfig), c_int32, c_void_p, POINTER(c_int64)]
     lib.ipl_build_tables.restype = c_int32
+    lib.ipl_ib_attach.argtypes = [vp, POINTER(IplIbConfig)]
+    lib.ipl_ib_attach.restype = c_int32
+    lib.ipl_ib_body_state.argtypes = [vp, _DP]
+    lib.ipl_ib_body_state.restype = c_int32
+    lib.ipl_ib_read_markers.argtypes = [vp, c_int32, _DP, c_size_t]
+    lib.ipl_ib_read_markers.restype = c_int32
     _LIBS[path] = lib
     return lib
 

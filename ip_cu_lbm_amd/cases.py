@@ -89,6 +89,7 @@ class Case:
     solid: Optional[np.ndarray] = None
     init: Dict[str, np.ndarray] = dataclasses.field(default_factory=dict)
     steps: int = 100
+    body: Optional["Body"] = None
 
     @property
     def n_cells(self) -> int:
@@ -155,6 +156,122 @@ class Case:
                     raise ValueError(f"init field {key} has shape {arr.shape}, expected ({self.n_cells},)")
                 setattr(ini, key, dp(arr))
         return cfg, ini, keep
+
+
+@dataclasses.dataclass
+class Body:
+    """A rigid body carried by Lagrangian markers (the drivers' marker set-up,
+    SphereFall3D/SphereFall.cu:197-232): body-frame marker offsets and unit normals, spacing,
+    mass properties and the initial state."""
+    markers: np.ndarray           # (M, 3) offsets from the centre, body frame
+    normals: np.ndarray           # (M, 3) outward unit normals, body frame
+    spacing: float                # dIBM
+    mass: float
+    density: float
+    inertia: tuple
+    pos: tuple
+    vel: tuple = (0.0, 0.0, 0.0)
+    omega: tuple = (0.0, 0.0, 0.0)
+    quat: tuple = (1.0, 0.0, 0.0, 0.0)
+    gravity: tuple = (0.0, 0.0, 0.0)
+    rho_f: float = 1.0
+    method: int = _abi.IB_STRESS
+    kernel: int = _abi.IB_KERNEL_REFERENCE
+    iterations: int = 1
+    moving: int = 1
+
+    def to_c(self):
+        keep = []
+
+        def dp(a):
+            a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+            keep.append(a)
+            return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+        c = _abi.IplIbConfig()
+        c.method, c.kernel, c.iterations, c.moving = self.method, self.kernel, self.iterations, self.moving
+        c.n_markers = len(self.markers)
+        c.marker_pos = dp(self.markers)
+        c.marker_normal = dp(self.normals)
+        c.spacing, c.rho_f, c.density, c.mass = self.spacing, self.rho_f, self.density, self.mass
+        for a in range(3):
+            c.inertia[a] = float(self.inertia[a])
+            c.gravity[a] = float(self.gravity[a])
+            c.pos[a] = float(self.pos[a])
+            c.vel[a] = float(self.vel[a])
+            c.omega[a] = float(self.omega[a])
+        for i in range(4):
+            c.quat[i] = float(self.quat[i])
+        return c, keep
+
+
+def icosphere(n: int) -> np.ndarray:
+    """Unit-sphere points of an icosahedron whose faces are subdivided n times per edge and
+    projected to the sphere (10 n^2 + 2 points): the marker sets of the reference's sphere drivers
+    (make_3DSphere.py, sphere_points_n*.csv).  Marker spacing: the icosahedron edge / n,
+    4 R / sqrt(10 + 2 sqrt 5) / n (SphereFall.cu:277)."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    V = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    V /= np.linalg.norm(V[0])
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    pts = {}
+    for a, b, c in F:
+        A, B, C = V[a], V[b], V[c]
+        for i in range(n + 1):
+            for j in range(n + 1 - i):
+                p = A + (B - A) * (i / n) + (C - A) * (j / n)
+                p = p / np.linalg.norm(p)
+                pts.setdefault(tuple(np.round(p, 12)), p)
+    out = np.array(list(pts.values()))
+    return out[np.lexsort((out[:, 2], out[:, 1], out[:, 0]))]
+
+
+def fibonacci_sphere(m: int) -> np.ndarray:
+    """m unit-sphere points on a Fibonacci lattice: z_i = 1 - (2 i + 1) / m, azimuth i times the
+    golden angle.  Mean spacing sqrt(4 pi / m) per unit radius."""
+    i = np.arange(m, dtype=np.float64)
+    z = 1.0 - (2.0 * i + 1.0) / m
+    r = np.sqrt(1.0 - z * z)
+    phi = i * np.pi * (3.0 - 5 ** 0.5)
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+
+
+def sphere_body(radius: float, centre, markers: np.ndarray, spacing: float, density_ratio: float, rho_f: float = 1.0,
+                gravity=(0.0, 0.0, 0.0), **kw) -> Body:
+    """A solid sphere: markers on the surface (unit points times the radius), normals the unit
+    points, mass and moment of inertia 2/5 m R^2 (SphereFall.cu:219-224)."""
+    m = density_ratio * rho_f * 4.0 / 3.0 * np.pi * radius ** 3
+    inertia = 2.0 / 5.0 * m * radius ** 2
+    return Body(markers=markers * radius, normals=markers.copy(), spacing=spacing, mass=m, density=density_ratio * rho_f,
+                inertia=(inertia, inertia, inertia), pos=tuple(float(c) for c in centre), gravity=tuple(gravity),
+                rho_f=rho_f, **kw)
+
+
+def falling_sphere(n=(192, 192, 384), diameter: float = 24.0, markers: int = 1800, density_ratio: float = 1.5,
+                   tau: float = 0.644, gravity: float = 1.0e-4, iterations: int = 3, kernel: int = _abi.IB_KERNEL_PESKIN4,
+                   method: int = _abi.IB_DIRECT, storage: int = _abi.F32, path: int = _abi.PATH_AUTO, steps: int = 20000,
+                   wall_bc: int = _abi.BC_BOUNCE_BACK, marker_set: str = "fibonacci") -> Case:
+    """Falling sphere with the immersed boundary: D3Q19 MRT on a uniform box, a sphere of the given
+    diameter (cells) released at rest near the top and falling along -z under
+    (1 - rho_f/rho_B) m g; multi-direct forcing with the 4-point Peskin kernel by default.  With
+    nu = (tau - 1/2)/3 = 0.048 and g = 1e-4 the settling Reynolds number is about 10 (Schiller-
+    Naumann drag).  The fluid itself carries no gravity (the buoyancy is in the body equation)."""
+    nx, ny, nz = n
+    pos, wid = zip(*[_uniform_axis(k) for k in n])
+    R = diameter / 2.0
+    pts = fibonacci_sphere(markers) if marker_set == "fibonacci" else icosphere(int(marker_set))
+    spacing = (np.sqrt(4.0 * np.pi / len(pts)) * R if marker_set == "fibonacci"
+               else 4.0 * R / np.sqrt(10.0 + 2.0 * np.sqrt(5.0)) / int(marker_set))
+    centre = (nx / 2.0, ny / 2.0, nz - 3.0 * diameter)
+    body = sphere_body(R, centre, pts, spacing, density_ratio, gravity=(0.0, 0.0, -gravity), method=method,
+                       kernel=kernel, iterations=iterations)
+    rates = np.full(27, 1.0)
+    return Case(name=f"falling_sphere_{nx}x{ny}x{nz}", lattice=19, n=n, pos=list(pos), width=list(wid),
+                periodic=(0, 0, 0), wall_bc=wall_bc, flux_axes=_abi.FLUX_NONE, stencil=5, s_rates=rates, tau=tau,
+                storage=storage, path=path, steps=steps, body=body)
 
 
 def _uniform_axis(n: int, dz: float = 1.0):

@@ -181,6 +181,24 @@ int ipl_destroy(ipl_ctx* ctx) {
     return IPL_OK;
 }
 
+int ipl_ib_attach(ipl_ctx* ctx, const ipl_ib_config* ib) {
+    if (!ctx || !ib) return IPL_ERR_ARGUMENT;
+    ipl::Status s = ctx->engine->ib_attach(*ib);
+    return s ? IPL_OK : fail(ctx, s);
+}
+
+int ipl_ib_body_state(ipl_ctx* ctx, double out[19]) {
+    if (!ctx || !out) return IPL_ERR_ARGUMENT;
+    ipl::Status s = ctx->engine->ib_body_state(out);
+    return s ? IPL_OK : fail(ctx, s);
+}
+
+int ipl_ib_read_markers(ipl_ctx* ctx, int32_t which, double* out, size_t count) {
+    if (!ctx || !out) return IPL_ERR_ARGUMENT;
+    ipl::Status s = ctx->engine->ib_read_markers(which, out, count);
+    return s ? IPL_OK : fail(ctx, s);
+}
+
 int ipl_build_tables(const ipl_config* cfg, int32_t which, void* out, int64_t* count) {
     if (!cfg || !count) return IPL_ERR_ARGUMENT;
     ipl::Graph g;

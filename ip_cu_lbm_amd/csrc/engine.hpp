@@ -78,6 +78,12 @@ public:
     virtual Status profile(int nsteps, double out[3]) = 0;
     virtual int64_t device_bytes() const = 0;
     virtual int path() const = 0;
+    // immersed boundary (fused path only)
+    virtual Status ib_attach(const ipl_ib_config&) {
+        return Status::err(IPL_ERR_UNSUPPORTED, "the immersed boundary runs on the fused path");
+    }
+    virtual Status ib_body_state(double*) { return Status::err(IPL_ERR_ARGUMENT, "no body attached"); }
+    virtual Status ib_read_markers(int, double*, size_t) { return Status::err(IPL_ERR_ARGUMENT, "no body attached"); }
     hipStream_t stream = nullptr;
 };
 

@@ -11,6 +11,7 @@
 
 #include "engine.hpp"
 #include "fused_common.hpp"
+#include "ibm.hpp"
 #include "lattice.hpp"
 
 namespace ipl {
@@ -86,6 +87,7 @@ public:
                      (cfg_.scalar == IPL_SCALAR_PHASE_FIELD && cfg_.rho_H != cfg_.rho_L);
         G.grad_phi = cfg_.scalar == IPL_SCALAR_PHASE_FIELD;
         G.need_grad = G.grad_rho || G.grad_phi;
+        G.half_force = 1;
         double scale = 1.0;
         if (std::is_same<ST, __half>::value) scale = cfg_.f16_scale > 0 ? cfg_.f16_scale : 8192.0;
         G.scale = (float)scale;
@@ -246,6 +248,11 @@ public:
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 1], stream));
             fusedk::fused_launch_step<ST, MT, Q, QS>(geo_, tb_, b, K_, gr_.npts, block_, lds_bytes_, stream);
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 2], stream));
+            if (ib_.active()) {
+                // IB sub-step on f^{n+1}, with the Force field of this step (SphereFall.cu:372-383)
+                Status r = ib_.substep(fb_, gb_, Fb_, std::is_same<ST, double>::value ? IPL_F64 : IPL_F32);
+                if (!r) return r;
+            }
             std::swap(fa_, fb_);
             std::swap(ga_, gb_);
             std::swap(Fa_, Fb_);
@@ -281,8 +288,25 @@ public:
     }
 
     Status step(int nsteps) override {
-        return dispatch([&](auto q, auto qs) { return this->template step_q<decltype(q)::value, decltype(qs)::value>(nsteps); });
+        Status r = dispatch([&](auto q, auto qs) { return this->template step_q<decltype(q)::value, decltype(qs)::value>(nsteps); });
+        if (r && ib_.active()) r = ib_.check();
+        return r;
     }
+
+    Status ib_attach(const ipl_ib_config& cfg) override {
+        if (std::is_same<ST, __half>::value) return Status::err(IPL_ERR_UNSUPPORTED, "the immersed boundary needs FP64 or FP32 storage");
+        IbGrid g{};
+        for (int a = 0; a < 3; ++a) { g.n[a] = geo_.n[a]; g.periodic[a] = geo_.periodic[a]; g.period[a] = gr_.period[a]; }
+        g.s = geo_.s; g.o1 = geo_.o1; g.o2 = geo_.o2; g.ns = geo_.ns; g.ncell = geo_.ncell;
+        g.dim = gr_.dim; g.Q = gr_.Q; g.Qs = gr_.Qs; g.scalar = cfg_.scalar;
+        g.dz = gr_.dz; g.dt = gr_.dt; g.c = gr_.c; g.tau = P_.tau;
+        g.rho_L = cfg_.rho_L; g.rho_H = cfg_.rho_H; g.beta_s = cfg_.beta_s;
+        Status r = ib_.attach(cfg, g, cfg_pos_, stream);
+        if (r) geo_.half_force = 0;
+        return r;
+    }
+    Status ib_body_state(double* out) override { return ib_.body_state(out); }
+    Status ib_read_markers(int which, double* out, size_t count) override { return ib_.read_markers(which, out, count); }
     Status readout() {
         return dispatch([&](auto q, auto qs) { return this->template readout_q<decltype(q)::value, decltype(qs)::value>(); });
     }
@@ -416,6 +440,7 @@ private:
     MT *phiold_ = nullptr, *uold_ = nullptr, *vold_ = nullptr, *wold_ = nullptr;
     double *vol_ = nullptr, *out_ = nullptr, *partial_ = nullptr;
     std::vector<int64_t> perm_;
+    IbSolver ib_;
     unsigned block_ = 64;
     size_t lds_bytes_ = 0;
 };

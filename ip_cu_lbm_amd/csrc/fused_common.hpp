@@ -22,6 +22,8 @@ struct FGeom {
     int need_grad;   // the scalar-field pass runs (rho and/or phi gradients are needed)
     int grad_rho;    // rho varies: the pressure-gradient term of Force is nonzero
     int grad_phi;    // phase field: grad phi and its Laplacian (Force, col_PF)
+    int half_force;  // u = sum e c f + F_old dt / (2 rho) (update_rho); 0 after an immersed-boundary
+                     // sub-step, whose update_velIBM leaves u = sum e f (ibm.cu:356-360)
     int scalar;
     float inv_scale; // FP16 storage: stored value = f / w_k * scale
     float scale;

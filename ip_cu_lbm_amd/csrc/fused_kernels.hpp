@@ -550,10 +550,13 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
         const MT phi = G.scalar == IPL_SCALAR_PHASE_FIELD ? sc : MT(0);
         const MT rho = P.rho_L + P.drho * phi + S_ * P.beta_s;
         const MT irho = MT(1) / rho;
-        const MT hr = P.half_dt * irho;
-        const MT u = ex * c + *elem(b.F_old, n) * hr;
-        const MT v = ey * c + *elem(b.F_old, NC + n) * hr;
-        const MT w = ez * c + *elem(b.F_old, 2 * NC + n) * hr;
+        MT u = ex, v = ey, w = ez;
+        if (G.half_force) {
+            const MT hr = P.half_dt * irho;
+            u = ex * c + *elem(b.F_old, n) * hr;
+            v = ey * c + *elem(b.F_old, NC + n) * hr;
+            w = ez * c + *elem(b.F_old, 2 * NC + n) * hr;
+        }
         // ---- Force (collision.cu:57-112) ----
         constexpr bool d3 = Lattice<Q>::dim == 3;
         MT Fx = 0, Fy = 0, Fz = 0;
@@ -723,9 +726,15 @@ __global__ void k_readout(FGeom G, const ST* f_src, const ST* g_src, const MT* F
     const MT phi = G.scalar == IPL_SCALAR_PHASE_FIELD ? sc : MT(0);
     const MT rho = MT(P.rho_L) + (MT(P.rho_H) - MT(P.rho_L)) * phi + S * MT(P.beta_s);
     out[0 * NC + n] = p;
-    out[1 * NC + n] = jx + F_old[n] * MT(P.dt) / (MT(2.0) * rho);
-    out[2 * NC + n] = jy + F_old[NC + n] * MT(P.dt) / (MT(2.0) * rho);
-    out[3 * NC + n] = jz + F_old[2 * NC + n] * MT(P.dt) / (MT(2.0) * rho);
+    if (G.half_force) {
+        out[1 * NC + n] = jx + F_old[n] * MT(P.dt) / (MT(2.0) * rho);
+        out[2 * NC + n] = jy + F_old[NC + n] * MT(P.dt) / (MT(2.0) * rho);
+        out[3 * NC + n] = jz + F_old[2 * NC + n] * MT(P.dt) / (MT(2.0) * rho);
+    } else {  // update_velIBM: u = sum e f (ibm.cu:356-360)
+        out[1 * NC + n] = jx / c;
+        out[2 * NC + n] = jy / c;
+        out[3 * NC + n] = jz / c;
+    }
     out[4 * NC + n] = rho;
     out[5 * NC + n] = S;
     out[6 * NC + n] = phi;

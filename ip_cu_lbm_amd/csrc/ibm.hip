@@ -1,0 +1,750 @@
+// ibm.hip - immersed-boundary sub-step of the fused path (function/ibm.cu); see ibm.hpp.
+#include "ibm.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "lattice.hpp"
+
+namespace ipl {
+namespace ibk {
+
+// body state, FP64, one array on the device
+enum BodySlot { B_POS = 0, B_VEL = 3, B_OMEGA = 6, B_QUAT = 9, B_QUATOLD = 13, B_R = 17, B_FB = 26, B_T = 29, B_SIZE = 32 };
+
+struct IbArgs {
+    int n[3], periodic[3];
+    int s, o1, o2, ns;
+    int dim, Qs, scalar;
+    uint32_t ncell;
+    double dz, dt, c, tau;
+    double rho_L, drho, beta_s;
+    int M, S, lo, hi, block3;   // slots per marker; node offsets lo..hi per axis; reference block table
+    int method, kernel, moving;
+    double spread, wscale, gwscale;
+    double rho_f, density, mass, inertia[3], gravity[3];
+    int64_t vmax;
+    const double* pos[3];
+};
+
+__device__ inline uint32_t cell_of(const IbArgs& A, const int c[3]) {
+    return (uint32_t)(c[A.o1] * A.n[A.o2] + c[A.o2]) * (uint32_t)A.ns + (uint32_t)c[A.s];
+}
+
+// node displaced by d from base: false where the reference's neighbour is a ghost (outside a
+// wall); pos receives its coordinates (a ghost sits on its owner; unwrapped across a seam)
+__device__ inline bool node_at(const IbArgs& A, const int base[3], const int d[3], int c[3], double pos[3]) {
+    bool ghost = false;
+    int wrap[3] = {0, 0, 0};
+    for (int a = 0; a < 3; ++a) {
+        c[a] = base[a] + d[a];
+        if (c[a] < 0 || c[a] >= A.n[a]) {
+            if (A.periodic[a]) {
+                wrap[a] = c[a] < 0 ? -1 : 1;
+                c[a] -= wrap[a] * A.n[a];
+            } else {
+                ghost = true;
+            }
+        }
+    }
+    // a periodic axis unwraps by its period (sum of widths), held after the coordinates
+    for (int a = 0; a < 3; ++a)
+        pos[a] = ghost ? A.pos[a][base[a]] : A.pos[a][c[a]] + wrap[a] * A.pos[a][A.n[a]];
+    return !ghost;
+}
+
+// displacement of slot s of the node block: i (x) outer, j, l (z) inner over lo..hi; the
+// reference's table (idfIndex, ibm.cu:132-141) maps (x-1, z-1) of the 2D block to (x+1, z+1)
+__device__ inline void slot_offset(const IbArgs& A, int s, int d[3]) {
+    const int np = A.hi - A.lo + 1, npy = A.dim == 3 ? np : 1;
+    d[0] = s / (npy * np) + A.lo;
+    d[1] = A.dim == 3 ? (s / np) % npy + A.lo : 0;
+    d[2] = s % np + A.lo;
+    if (A.block3 && A.dim == 2 && d[0] == -1 && d[2] == -1) { d[0] = 1; d[2] = 1; }
+}
+
+__device__ inline double peskin4(double r) {
+    if (r < 1.0) return (3.0 - 2.0 * r + sqrt(1.0 + 4.0 * r - 4.0 * r * r)) / 8.0;
+    if (r <= 2.0) return (5.0 - 2.0 * r - sqrt(-7.0 + 12.0 * r - 4.0 * r * r)) / 8.0;
+    return 0.0;
+}
+
+// Lagrange_interpolation (adv.cu:40-48) with n = 3
+__device__ inline void lagrange3(double* coeff, const double* x, double xa) {
+    for (int i = 0; i < 3; ++i) {
+        coeff[i] = 1.0;
+        for (int j = 0; j < 3; ++j)
+            if (j != i) coeff[i] *= (xa - x[j]) / (x[i] - x[j]);
+    }
+}
+
+// Node weights per axis (get_IBMGw2 ibm.cu:146-189 for the stress method, IB_directForcing
+// ibm.cu:292-313 for direct forcing, the full Peskin support for the PESKIN4 kernel)
+__device__ inline void axis_weights(const IbArgs& A, const int base[3], const double xw[3], double W[3][5]) {
+    const int np = A.hi - A.lo + 1;
+    const bool stress = A.method == IPL_IB_STRESS;
+    const bool pk4 = !stress && A.kernel == IPL_IB_KERNEL_PESKIN4;
+    for (int a = 0; a < 3; ++a) {
+        if (A.dim == 2 && a == 1) {
+            for (int t = 0; t < np; ++t) W[a][t] = t == -A.lo ? 1.0 : 0.0;
+            continue;
+        }
+        double xs[5];
+        bool ok[5];
+        for (int t = 0; t < np; ++t) {
+            int d[3] = {0, 0, 0}, c[3];
+            d[a] = A.lo + t;
+            double pp[3];
+            ok[t] = node_at(A, base, d, c, pp);
+            xs[t] = pp[a];
+        }
+        const bool peskin = pk4 || (stress && (a == 0 || (A.dim == 2 && a == 2)));
+        if (peskin) {
+            for (int t = 0; t < np; ++t) W[a][t] = peskin4(fabs(xw[a] - xs[t]) / A.dz) * (ok[t] ? 1.0 : 0.0);
+            if (stress && A.dim == 2) {
+                double sum = 0;
+                for (int t = 0; t < np; ++t) sum += W[a][t];
+                for (int t = 0; t < np; ++t) W[a][t] /= sum;
+            }
+        } else {
+            lagrange3(W[a], xs, xw[a]);
+        }
+    }
+}
+
+__device__ inline double slot_weight(const IbArgs& A, const double W[3][5], int s) {
+    const int np = A.hi - A.lo + 1, npy = A.dim == 3 ? np : 1;
+    const int i = s / (npy * np), j = A.dim == 3 ? (s / np) % npy : -A.lo, l = s % np;
+    return W[0][i] * W[1][j] * W[2][l];
+}
+
+template <typename ST> __device__ inline double ldpop(const ST* f, uint32_t idx) { return (double)f[idx]; }
+
+// index of cell c (wrapped) in the box, -1 outside
+__device__ inline int64_t box_index(const IbArgs& A, const int* box, const int c[3]) {
+    int b[3];
+    for (int a = 0; a < 3; ++a) {
+        int x = c[a] - box[a];
+        if (x < 0 && A.periodic[a]) x += A.n[a];
+        if (x < 0 || x >= box[3 + a]) return -1;
+        b[a] = x;
+    }
+    return ((int64_t)b[0] * box[4] + b[1]) * box[5] + b[2];
+}
+
+// ---- k_ib_sort: markers sorted by (nearest cell, index); bounding box of the node blocks -----
+__global__ void k_ib_sort(IbArgs A, const int* node, unsigned long long* sorted, int* box, int mpad) {
+    IPL_DYNAMIC_LDS(unsigned long long, keys);
+    __shared__ int mn[3][256], mx[3][256];
+    const int t = threadIdx.x, nt = blockDim.x;
+    for (int i = t; i < mpad; i += nt) {
+        if (i < A.M) {
+            const int c[3] = {node[3 * i], node[3 * i + 1], node[3 * i + 2]};
+            keys[i] = ((unsigned long long)cell_of(A, c) << 32) | (unsigned long long)i;
+        } else {
+            keys[i] = ~0ull;
+        }
+    }
+    __syncthreads();
+    // bitonic sort of mpad (a power of two) keys
+    for (int size = 2; size <= mpad; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = t; i < mpad; i += nt) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const unsigned long long a = keys[i], b = keys[j];
+                    if ((a > b) == up) { keys[i] = b; keys[j] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = t; i < A.M; i += nt) sorted[i] = keys[i];
+    // bounding box of the marker nodes (first 256 work-items)
+    if (t < 256) {
+        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+        for (int i = t; i < A.M; i += 256)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = min(lo[a], node[3 * i + a]);
+                hi[a] = max(hi[a], node[3 * i + a]);
+            }
+        for (int a = 0; a < 3; ++a) { mn[a][t] = lo[a]; mx[a][t] = hi[a]; }
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int a = 0; a < 3; ++a) {
+                mn[a][t] = min(mn[a][t], mn[a][t + w]);
+                mx[a][t] = max(mx[a][t], mx[a][t + w]);
+            }
+        __syncthreads();
+    }
+    if (t == 0) {
+        int64_t vol = 1;
+        for (int a = 0; a < 3; ++a) {
+            int lo = mn[a][0] + A.lo, hi = mx[a][0] + A.hi;
+            if (A.periodic[a]) {
+                if (hi - lo + 1 >= A.n[a]) { lo = 0; hi = A.n[a] - 1; }
+                else if (lo < 0) lo += A.n[a];
+            } else {
+                lo = max(lo, 0);
+                hi = min(hi, A.n[a] - 1);
+            }
+            box[a] = lo;
+            box[3 + a] = hi - lo + 1;
+            vol *= box[3 + a];
+        }
+        box[6] = vol <= A.vmax ? 0 : 1;  // overflow flag (the host sizes vmax so that it stays 0)
+    }
+}
+
+// ---- k_ib_force: one work-item per marker ------------------------------------------------------
+// Stress method (get_IBMGw2, ibm.cu:124-274): interpolate f at the marker, split into outer (fw1)
+// and inner (fw2) populations by the normal, spread minus the traction of (fw1 - fw2), marker force
+// Gw from the traction of fw1.  Direct forcing (IB_directForcing, ibm.cu:276-344): interpolate the
+// node velocity (update_rho's u, plus dt F_ib / 3 of earlier passes), spread (U_w - u)/dt.
+template <typename ST, typename MT, int Q>
+__global__ void k_ib_force(IbArgs A, int it, const ST* __restrict__ f, const ST* __restrict__ g, const MT* __restrict__ F,
+                           const int* __restrict__ node, const double* __restrict__ posw, const double* __restrict__ nB,
+                           const double* __restrict__ velw, double* __restrict__ Gw, double* __restrict__ contrib,
+                           const int* __restrict__ box, const double* __restrict__ Fbox) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= A.M) return;
+    const int base[3] = {node[3 * m], node[3 * m + 1], node[3 * m + 2]};
+    const double xw[3] = {posw[3 * m], posw[3 * m + 1], posw[3 * m + 2]};
+    const double U[3] = {velw[3 * m], velw[3 * m + 1], velw[3 * m + 2]};
+    double W[3][5];
+    axis_weights(A, base, xw, W);
+    const double c = A.c, c2 = c * c;
+    double* out = contrib + (size_t)m * A.S * 3;
+    if (A.method == IPL_IB_STRESS) {
+        double fw[Q];
+#pragma unroll
+        for (int k = 0; k < Q; ++k) fw[k] = 0.0;
+        for (int s = 0; s < A.S; ++s) {
+            int d[3], cc[3];
+            double pp[3];
+            slot_offset(A, s, d);
+            if (!node_at(A, base, d, cc, pp)) continue;
+            const double wgt = slot_weight(A, W, s);
+            const uint32_t idx = cell_of(A, cc);
+#pragma unroll
+            for (int k = 0; k < Q; ++k) fw[k] += wgt * ldpop(f, (uint32_t)k * A.ncell + idx);
+        }
+        const double nv[3] = {nB[3 * m], nB[3 * m + 1], nB[3 * m + 2]};
+        const double sina = sin(22.5 / 180.0 * 3.14159);  // the reference's literal angle
+        double fw1[Q], fw2[Q];
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            const Dir e = Lattice<Q>::c(k);
+            const double cn = (nv[0] * e.x + nv[1] * e.y + nv[2] * e.z) / c;
+            const double lim = sqrt((double)(e.x * e.x + e.y * e.y + e.z * e.z)) / c * sina;
+            const double bounce = fw[opposite<Q>(k)] + 6.0 * Lattice<Q>::w(k) * (e.x * U[0] + e.y * U[1] + e.z * U[2]) / c2;
+            const double beta = fabs(cn) / (lim + 1e-10);
+            const double mixed = (1.0 - beta) * fw[k] + beta * bounce;
+            fw1[k] = cn <= 0 ? fw[k] : (cn <= lim ? mixed : bounce);
+            fw2[k] = cn >= 0 ? fw[k] : (cn >= -lim ? mixed : bounce);
+        }
+        const double tau = A.tau;
+        // traction sigma . n: sigma_ab = -sum_k [g_k delta_ab / (2 tau)
+        //   + (1 - 1/(2 tau)) 3 (g_k (e_a - U_a)(e_b - U_b) - (g_k - w_k) U_a U_b) / c^2]
+        auto traction = [&](bool diff, double t[3]) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                double sg[3] = {0, 0, 0};
+#pragma unroll
+                for (int k = 0; k < Q; ++k) {
+                    const Dir e = Lattice<Q>::c(k);
+                    const double ev[3] = {(double)e.x, (double)e.y, (double)e.z};
+                    const double gk = diff ? fw1[k] - fw2[k] : fw1[k];
+#pragma unroll
+                    for (int b = 0; b < 3; ++b)
+                        sg[b] -= 0.5 / tau * gk * (a == b ? 1.0 : 0.0) +
+                                 (1.0 - 0.5 / tau) * 3 * (gk * (ev[a] - U[a]) * (ev[b] - U[b]) - (gk - Lattice<Q>::w(k)) * U[a] * U[b]) / c2;
+                }
+                t[a] = sg[0] * nv[0] + sg[1] * nv[1] + sg[2] * nv[2];
+            }
+        };
+        double gs[3], gw[3];
+        traction(true, gs);
+        for (int s = 0; s < A.S; ++s) {
+            int d[3], cc[3];
+            double pp[3];
+            slot_offset(A, s, d);
+            const bool ok = node_at(A, base, d, cc, pp);
+            const double wgt = slot_weight(A, W, s);
+            for (int a = 0; a < 3; ++a) out[3 * s + a] = ok ? -gs[a] * A.spread * wgt : 0.0;
+        }
+        traction(false, gw);
+        for (int a = 0; a < 3; ++a) Gw[3 * m + a] = gw[a] * A.gwscale;
+        return;
+    }
+    // direct forcing
+    double um[3] = {0, 0, 0};
+    for (int s = 0; s < A.S; ++s) {
+        int d[3], cc[3];
+        double pp[3];
+        slot_offset(A, s, d);
+        if (!node_at(A, base, d, cc, pp)) continue;
+        const double wgt = slot_weight(A, W, s);
+        const uint32_t idx = cell_of(A, cc);
+        // update_rho's velocity (collision.cu:244-254) at the node
+        double p = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            const Dir e = Lattice<Q>::c(k);
+            const double fv = ldpop(f, (uint32_t)k * A.ncell + idx);
+            p += fv; jx += e.x * c * fv; jy += e.y * c * fv; jz += e.z * c * fv;
+        }
+        double sc = 0;
+        for (int k = 0; k < A.Qs; ++k) sc += ldpop(g, (uint32_t)k * A.ncell + idx);
+        const double phi = A.scalar == IPL_SCALAR_PHASE_FIELD ? sc : 0.0, S = A.scalar == IPL_SCALAR_SALINITY ? sc : 0.0;
+        const double rho = A.rho_L + A.drho * phi + S * A.beta_s;
+        double u[3] = {jx + (double)F[idx] * A.dt / (2.0 * rho), jy + (double)F[A.ncell + idx] * A.dt / (2.0 * rho),
+                       jz + (double)F[2 * (size_t)A.ncell + idx] * A.dt / (2.0 * rho)};
+        if (it > 0) {
+            const int64_t b = box_index(A, box, cc);
+            if (b >= 0)
+                for (int a = 0; a < 3; ++a) u[a] += A.dt * Fbox[3 * b + a] / 3.0;
+        }
+        for (int a = 0; a < 3; ++a) um[a] += wgt * u[a];
+    }
+    double gwv[3];
+    for (int a = 0; a < 3; ++a) gwv[a] = (U[a] - um[a]) / A.dt;
+    for (int s = 0; s < A.S; ++s) {
+        int d[3], cc[3];
+        double pp[3];
+        slot_offset(A, s, d);
+        const bool ok = node_at(A, base, d, cc, pp);
+        const double wgt = slot_weight(A, W, s);
+        for (int a = 0; a < 3; ++a) out[3 * s + a] = ok ? gwv[a] * A.spread * wgt : 0.0;
+    }
+    for (int a = 0; a < 3; ++a) Gw[3 * m + a] = (it == 0 ? 0.0 : Gw[3 * m + a]) + -gwv[a] * A.wscale;
+}
+
+// ---- k_ib_gather: spread force of every cell of the box ----------------------------------------
+__global__ void k_ib_gather(IbArgs A, int it, const unsigned long long* __restrict__ sorted, const double* __restrict__ contrib,
+                            const int* __restrict__ box, double* __restrict__ Fbox) {
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t vol = (int64_t)box[3] * box[4] * box[5];
+    if (b >= vol || b >= A.vmax) return;
+    int c[3];
+    c[2] = (int)(b % box[5]);
+    c[1] = (int)((b / box[5]) % box[4]);
+    c[0] = (int)(b / ((int64_t)box[4] * box[5]));
+    for (int a = 0; a < 3; ++a) {
+        c[a] += box[a];
+        if (c[a] >= A.n[a]) c[a] -= A.n[a];
+    }
+    double F[3] = {0, 0, 0};
+    for (int s = 0; s < A.S; ++s) {
+        int d[3], src[3];
+        slot_offset(A, s, d);
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            src[a] = c[a] - d[a];
+            if (src[a] < 0 || src[a] >= A.n[a]) {
+                if (A.periodic[a]) src[a] += src[a] < 0 ? A.n[a] : -A.n[a];
+                else ok = false;
+            }
+        }
+        if (!ok) continue;
+        const unsigned long long key = (unsigned long long)cell_of(A, src) << 32;
+        // first sorted entry >= key
+        int lo = 0, hi = A.M;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (sorted[mid] < key) lo = mid + 1; else hi = mid;
+        }
+        for (int r = lo; r < A.M && (sorted[r] >> 32) == (key >> 32); ++r) {
+            const int m = (int)(sorted[r] & 0xffffffffull);
+            const double* cm = contrib + ((size_t)m * A.S + s) * 3;
+            F[0] += cm[0]; F[1] += cm[1]; F[2] += cm[2];
+        }
+    }
+    for (int a = 0; a < 3; ++a) Fbox[3 * b + a] = (it == 0 ? 0.0 : Fbox[3 * b + a]) + F[a];
+}
+
+// ---- k_ib_apply: update_velIBM (ibm.cu:346-365) on the box ------------------------------------
+template <typename ST, int Q>
+__global__ void k_ib_apply(IbArgs A, ST* __restrict__ f, const int* __restrict__ box, const double* __restrict__ Fbox) {
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t vol = (int64_t)box[3] * box[4] * box[5];
+    if (b >= vol || b >= A.vmax) return;
+    const double F[3] = {Fbox[3 * b], Fbox[3 * b + 1], Fbox[3 * b + 2]};
+    if (F[0] == 0.0 && F[1] == 0.0 && F[2] == 0.0) return;
+    int c[3];
+    c[2] = (int)(b % box[5]);
+    c[1] = (int)((b / box[5]) % box[4]);
+    c[0] = (int)(b / ((int64_t)box[4] * box[5]));
+    for (int a = 0; a < 3; ++a) {
+        c[a] += box[a];
+        if (c[a] >= A.n[a]) c[a] -= A.n[a];
+    }
+    const uint32_t idx = cell_of(A, c);
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const Dir e = Lattice<Q>::c(k);
+        ST& fv = f[(uint32_t)k * A.ncell + idx];
+        fv = (ST)((double)fv + Lattice<Q>::w(k) * A.dt * (e.x * F[0] + e.y * F[1] + e.z * F[2]));
+    }
+}
+
+// rotation matrix of a quaternion (set_quaternionS, ibm.cu:16-28)
+__device__ inline void rotation(const double* q, double* R) {
+    const double a = q[0], b = q[1], c = q[2], d = q[3];
+    R[0] = a * a - c * c - d * d + b * b; R[1] = 2 * (b * c - a * d);         R[2] = 2 * (b * d + a * c);
+    R[3] = 2 * (b * c + a * d);         R[4] = a * a - d * d - b * b + c * c; R[5] = 2 * (c * d - a * b);
+    R[6] = 2 * (b * d - a * c);         R[7] = 2 * (c * d + a * b);         R[8] = a * a - b * b - c * c + d * d;
+}
+
+// ---- k_ib_body: update_IBbody (ibm.cu:367-424), one workgroup of 256 ---------------------------
+__global__ void k_ib_body(IbArgs A, const double* __restrict__ Gw, const double* __restrict__ posw, double* body) {
+    __shared__ double red[6][256];
+    const int t = threadIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const double* P = body + B_POS;
+    for (int m = t; m < A.M; m += 256) {
+        const double* G = Gw + 3 * m;
+        const double r[3] = {posw[3 * m] - P[0], posw[3 * m + 1] - P[1], posw[3 * m + 2] - P[2]};
+        acc[0] += G[0]; acc[1] += G[1]; acc[2] += G[2];
+        acc[3] += r[1] * G[2] - r[2] * G[1];
+        acc[4] += r[2] * G[0] - r[0] * G[2];
+        acc[5] += r[0] * G[1] - r[1] * G[0];
+    }
+    for (int q = 0; q < 6; ++q) red[q][t] = acc[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int q = 0; q < 6; ++q) red[q][t] += red[q][t + w];
+        __syncthreads();
+    }
+    if (t != 0) return;
+    double* FB = body + B_FB;
+    double* T = body + B_T;
+    double FBold[3], Told[3], velBold[3], angold[3], Rold[9];
+    rotation(body + B_QUATOLD, Rold);
+    for (int a = 0; a < 3; ++a) {
+        FBold[a] = FB[a]; Told[a] = T[a];
+        velBold[a] = body[B_VEL + a]; angold[a] = body[B_OMEGA + a];
+        FB[a] = red[a][0];
+        T[a] = red[3 + a][0];
+    }
+    for (int a = 0; a < 3; ++a) FB[a] += (1.0 - A.rho_f / A.density) * A.mass * A.gravity[a];
+    if (!A.moving) return;
+    const double dt = A.dt;
+    for (int a = 0; a < 3; ++a) {
+        body[B_VEL + a] += dt * (3.0 * FB[a] - FBold[a]) / A.mass / 2.0;
+        body[B_POS + a] += dt * (3.0 * body[B_VEL + a] - velBold[a]) / 2.0;
+    }
+    const double* R = body + B_R;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            body[B_OMEGA + i] += dt * (3.0 * R[j * 3 + i] * T[j] - Rold[j * 3 + i] * Told[j]) / A.inertia[i] / 2.0;
+    double q[4], w[3], qo[4];
+    for (int i = 0; i < 4; ++i) { q[i] = body[B_QUAT + i]; qo[i] = body[B_QUATOLD + i]; }
+    for (int i = 0; i < 3; ++i) w[i] = body[B_OMEGA + i];
+    double* Qn = body + B_QUAT;
+    Qn[0] += dt * 3.0 * (-w[0] * q[1] - w[1] * q[2] - w[2] * q[3]) / 4.0;
+    Qn[1] += dt * 3.0 * (w[0] * q[0] + w[2] * q[2] - w[1] * q[3]) / 4.0;
+    Qn[2] += dt * 3.0 * (w[1] * q[0] - w[2] * q[1] + w[0] * q[3]) / 4.0;
+    Qn[3] += dt * 3.0 * (w[2] * q[0] + w[1] * q[1] - w[0] * q[2]) / 4.0;
+    Qn[0] -= dt * (-angold[0] * qo[1] - angold[1] * qo[2] - angold[2] * qo[3]) / 4.0;
+    Qn[1] -= dt * (angold[0] * qo[0] + angold[2] * qo[2] - angold[1] * qo[3]) / 4.0;
+    Qn[2] -= dt * (angold[1] * qo[0] - angold[2] * qo[1] + angold[0] * qo[3]) / 4.0;
+    Qn[3] -= dt * (angold[2] * qo[0] + angold[1] * qo[1] - angold[0] * qo[2]) / 4.0;
+    for (int i = 0; i < 4; ++i) body[B_QUATOLD + i] = q[i];
+    rotation(Qn, body + B_R);
+}
+
+// ---- k_ib_points: update_IBpoint (ibm.cu:426-447) and search_IBlattice (ibm.cu:449-463) --------
+template <int Q>
+__global__ void k_ib_points(IbArgs A, const double* __restrict__ body, const double* __restrict__ oposw,
+                            const double* __restrict__ onB, double* __restrict__ posw, double* __restrict__ nB,
+                            double* __restrict__ velw, int* __restrict__ node) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= A.M) return;
+    const double* R = body + B_R;
+    const double* w = body + B_OMEGA;
+    const double* op = oposw + 3 * m;
+    const double* on = onB + 3 * m;
+    const double wxo[3] = {w[1] * op[2] - w[2] * op[1], w[2] * op[0] - w[0] * op[2], w[0] * op[1] - w[1] * op[0]};
+    double x[3];
+    for (int i = 0; i < 3; ++i) {
+        double p = body[B_POS + i], nv = 0.0;
+        for (int j = 0; j < 3; ++j) {
+            p += R[i * 3 + j] * op[j];
+            nv += R[i * 3 + j] * on[j];
+        }
+        posw[3 * m + i] = p;
+        nB[3 * m + i] = nv;
+        velw[3 * m + i] = body[B_VEL + i] + R[i * 3 + 0] * wxo[0] + R[i * 3 + 1] * wxo[1] + R[i * 3 + 2] * wxo[2];
+        x[i] = p;
+    }
+    // nearest cell among the current one and its lattice neighbours; a ghost sits on its owner
+    // and never wins, a wrapped neighbour keeps its own (far) coordinates
+    int best[3] = {node[3 * m], node[3 * m + 1], node[3 * m + 2]};
+    const int cur[3] = {best[0], best[1], best[2]};
+    auto d2 = [&](const int c[3]) {
+        double s = 0;
+        for (int a = 0; a < 3; ++a) s += (A.pos[a][c[a]] - x[a]) * (A.pos[a][c[a]] - x[a]);
+        return s;
+    };
+    double bd = d2(cur);
+#pragma unroll
+    for (int k = 1; k < Q; ++k) {
+        const Dir e = Lattice<Q>::c(k);
+        int c[3] = {cur[0] + e.x, cur[1] + e.y, cur[2] + e.z};
+        bool ok = true;
+        for (int a = 0; a < 3; ++a)
+            if (c[a] < 0 || c[a] >= A.n[a]) {
+                if (A.periodic[a]) c[a] += c[a] < 0 ? A.n[a] : -A.n[a];
+                else ok = false;
+            }
+        if (!ok) continue;
+        const double dd = d2(c);
+        if (dd < bd) { bd = dd; best[0] = c[0]; best[1] = c[1]; best[2] = c[2]; }
+    }
+    node[3 * m] = best[0]; node[3 * m + 1] = best[1]; node[3 * m + 2] = best[2];
+}
+
+}  // namespace ibk
+
+namespace {
+
+void rotation_host(const double* q, double* R) {
+    const double a = q[0], b = q[1], c = q[2], d = q[3];
+    R[0] = a * a - c * c - d * d + b * b; R[1] = 2 * (b * c - a * d);         R[2] = 2 * (b * d + a * c);
+    R[3] = 2 * (b * c + a * d);         R[4] = a * a - d * d - b * b + c * c; R[5] = 2 * (c * d - a * b);
+    R[6] = 2 * (b * d - a * c);         R[7] = 2 * (c * d + a * b);         R[8] = a * a - b * b - c * c + d * d;
+}
+
+}  // namespace
+
+IbSolver::~IbSolver() = default;
+
+Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::vector<double>* pos, hipStream_t st) {
+    if (M_ > 0) return Status::err(IPL_ERR_ARGUMENT, "a body is already attached");
+    if (cfg.n_markers <= 0 || !cfg.marker_pos || !cfg.marker_normal)
+        return Status::err(IPL_ERR_ARGUMENT, "markers missing");
+    if (cfg.n_markers > kIbMaxMarkers) return Status::err(IPL_ERR_UNSUPPORTED, "more than 4096 markers");
+    if (cfg.method != IPL_IB_STRESS && cfg.method != IPL_IB_DIRECT) return Status::err(IPL_ERR_ARGUMENT, "IB method");
+    if (cfg.method == IPL_IB_STRESS && cfg.kernel != IPL_IB_KERNEL_REFERENCE)
+        return Status::err(IPL_ERR_ARGUMENT, "the stress method uses the reference weights");
+    if (cfg.kernel != IPL_IB_KERNEL_REFERENCE && cfg.kernel != IPL_IB_KERNEL_PESKIN4)
+        return Status::err(IPL_ERR_ARGUMENT, "IB kernel");
+    if (!(cfg.mass > 0) || !(cfg.density > 0) || !(cfg.spacing > 0))
+        return Status::err(IPL_ERR_ARGUMENT, "body mass, density and marker spacing must be positive");
+    if (cfg.moving && !(cfg.inertia[0] > 0 && cfg.inertia[1] > 0 && cfg.inertia[2] > 0))
+        return Status::err(IPL_ERR_ARGUMENT, "a moving body needs positive moments of inertia");
+    g_ = g;
+    cfg_ = cfg;
+    st_ = st;
+    M_ = cfg.n_markers;
+    const bool pk4 = cfg.method == IPL_IB_DIRECT && cfg.kernel == IPL_IB_KERNEL_PESKIN4;
+    reach_lo_ = pk4 ? -2 : -1;
+    reach_hi_ = pk4 ? 2 : 1;
+    const int np = reach_hi_ - reach_lo_ + 1;
+    S_ = np * np * (g.dim == 3 ? np : 1);
+    Status r;
+    // axis coordinates; entry n holds the period (sum of widths) for the seam unwrap
+    for (int a = 0; a < 3; ++a) {
+        std::vector<double> t(pos[a].begin(), pos[a].end());
+        t.push_back(g.period[a]);
+        if (!(r = mem_.alloc_n(&pos_[a], t.size()))) return r;
+        IPL_HIP_TRY(hipMemcpyAsync(pos_[a], t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        IPL_HIP_TRY(hipStreamSynchronize(st));
+    }
+    // body frame -> world: markers, unit normals, nearest cells (per-axis nearest centre, lower
+    // index on ties: the first minimum of the drivers' exhaustive search, SphereFall.cu:250-257)
+    double R[9];
+    rotation_host(cfg.quat, R);
+    std::vector<double> posw(3 * M_), nB(3 * M_), velw(3 * M_, 0.0);
+    std::vector<int> node(3 * M_);
+    double rmax = 0;
+    for (int m = 0; m < M_; ++m) {
+        const double* op = cfg.marker_pos + 3 * m;
+        const double* on = cfg.marker_normal + 3 * m;
+        const double nn = std::sqrt(on[0] * on[0] + on[1] * on[1] + on[2] * on[2]);
+        rmax = std::max(rmax, std::sqrt(op[0] * op[0] + op[1] * op[1] + op[2] * op[2]));
+        for (int i = 0; i < 3; ++i) {
+            double p = 0, nv = 0;
+            for (int j = 0; j < 3; ++j) { p += R[i * 3 + j] * op[j]; nv += R[i * 3 + j] * on[j]; }
+            nB[3 * m + i] = nv / nn;
+            posw[3 * m + i] = cfg.pos[i] + p;
+        }
+        for (int a = 0; a < 3; ++a) {
+            int best = 0;
+            double bd = 1e300;
+            for (int x = 0; x < g.n[a]; ++x) {
+                const double d = (pos[a][x] - posw[3 * m + a]) * (pos[a][x] - posw[3 * m + a]);
+                if (d < bd) { bd = d; best = x; }
+            }
+            node[3 * m + a] = best;
+        }
+    }
+    // box bound: the marker nodes span at most 2 rmax / dz + 3 cells per axis (widths >= dz);
+    // a periodic axis may be covered whole when the body straddles its seam
+    vmax_ = 1;
+    for (int a = 0; a < 3; ++a) {
+        int ext = (int)std::ceil(2.0 * rmax / g.dz) + 3 + (reach_hi_ - reach_lo_);
+        if (g.dim == 2 && a == 1) ext = 1;
+        if (g.periodic[a] || ext > g.n[a]) ext = std::min(g.periodic[a] ? g.n[a] : ext, g.n[a]);
+        box_max_[a] = ext;
+        vmax_ *= ext;
+    }
+    std::vector<double> body(ibk::B_SIZE, 0.0);
+    for (int a = 0; a < 3; ++a) {
+        body[ibk::B_POS + a] = cfg.pos[a];
+        body[ibk::B_VEL + a] = cfg.vel[a];
+        body[ibk::B_OMEGA + a] = cfg.omega[a];
+    }
+    for (int i = 0; i < 4; ++i) { body[ibk::B_QUAT + i] = cfg.quat[i]; body[ibk::B_QUATOLD + i] = cfg.quat[i]; }
+    for (int i = 0; i < 9; ++i) body[ibk::B_R + i] = R[i];
+    if (!(r = mem_.alloc_n(&oposw_, 3 * M_))) return r;
+    if (!(r = mem_.alloc_n(&onB_, 3 * M_))) return r;
+    if (!(r = mem_.alloc_n(&posw_, 3 * M_))) return r;
+    if (!(r = mem_.alloc_n(&nB_, 3 * M_))) return r;
+    if (!(r = mem_.alloc_n(&velw_, 3 * M_))) return r;
+    if (!(r = mem_.alloc_n(&Gw_, 3 * M_))) return r;
+    if (!(r = mem_.alloc_n(&node_, 3 * M_))) return r;
+    if (!(r = mem_.alloc_n(&contrib_, (size_t)3 * M_ * S_))) return r;
+    if (!(r = mem_.alloc_n(&Fbox_, (size_t)3 * vmax_))) return r;
+    if (!(r = mem_.alloc_n(&body_, (size_t)ibk::B_SIZE))) return r;
+    if (!(r = mem_.alloc_n(&box_, 8))) return r;
+    if (!(r = mem_.alloc_n(&sorted_, (size_t)M_))) return r;
+    IPL_HIP_TRY(hipMemcpyAsync(oposw_, cfg.marker_pos, 3 * M_ * sizeof(double), hipMemcpyHostToDevice, st));
+    IPL_HIP_TRY(hipMemcpyAsync(onB_, cfg.marker_normal, 3 * M_ * sizeof(double), hipMemcpyHostToDevice, st));
+    IPL_HIP_TRY(hipMemcpyAsync(posw_, posw.data(), 3 * M_ * sizeof(double), hipMemcpyHostToDevice, st));
+    IPL_HIP_TRY(hipMemcpyAsync(nB_, nB.data(), 3 * M_ * sizeof(double), hipMemcpyHostToDevice, st));
+    IPL_HIP_TRY(hipMemcpyAsync(velw_, velw.data(), 3 * M_ * sizeof(double), hipMemcpyHostToDevice, st));
+    IPL_HIP_TRY(hipMemsetAsync(Gw_, 0, 3 * M_ * sizeof(double), st));
+    IPL_HIP_TRY(hipMemcpyAsync(node_, node.data(), 3 * M_ * sizeof(int), hipMemcpyHostToDevice, st));
+    IPL_HIP_TRY(hipMemcpyAsync(body_, body.data(), ibk::B_SIZE * sizeof(double), hipMemcpyHostToDevice, st));
+    IPL_HIP_TRY(hipMemsetAsync(box_, 0, 8 * sizeof(int), st));
+    IPL_HIP_TRY(hipStreamSynchronize(st));
+    return Status::ok();
+}
+
+namespace {
+
+ibk::IbArgs make_args(const IbGrid& g, const ipl_ib_config& cfg, int M, int S, int lo, int hi, int64_t vmax,
+                      double* const pos[3]) {
+    ibk::IbArgs A{};
+    for (int a = 0; a < 3; ++a) { A.n[a] = g.n[a]; A.periodic[a] = g.periodic[a]; A.pos[a] = pos[a]; }
+    A.s = g.s; A.o1 = g.o1; A.o2 = g.o2; A.ns = g.ns;
+    A.dim = g.dim; A.Qs = g.Qs; A.scalar = g.scalar;
+    A.ncell = (uint32_t)g.ncell;
+    A.dz = g.dz; A.dt = g.dt; A.c = g.c; A.tau = g.tau;
+    A.rho_L = g.rho_L; A.drho = g.rho_H - g.rho_L; A.beta_s = g.beta_s;
+    A.M = M; A.S = S; A.lo = lo; A.hi = hi;
+    A.block3 = cfg.kernel == IPL_IB_KERNEL_REFERENCE;
+    A.method = cfg.method; A.kernel = cfg.kernel; A.moving = cfg.moving;
+    const double c2 = g.c * g.c;
+    const int dim = g.dim;
+    if (cfg.method == IPL_IB_STRESS) {
+        A.spread = std::pow(cfg.spacing / g.dz, dim - 1) / g.dz;                 // ibm.cu:239-241
+        A.gwscale = cfg.rho_f * c2 / 3.0 * std::pow(cfg.spacing, dim - 1);      // ibm.cu:270-272
+    } else {
+        A.spread = std::pow(cfg.spacing / g.dz, dim - 1) * 3.0 / c2;             // ibm.cu:336-338
+        A.wscale = cfg.rho_f * std::pow(cfg.spacing, dim - 1) * g.dz;           // ibm.cu:342
+    }
+    A.rho_f = cfg.rho_f; A.density = cfg.density; A.mass = cfg.mass;
+    for (int a = 0; a < 3; ++a) { A.inertia[a] = cfg.inertia[a]; A.gravity[a] = cfg.gravity[a]; }
+    A.vmax = vmax;
+    return A;
+}
+
+template <typename ST, typename MT>
+void force_q(int Q, const ibk::IbArgs& A, int it, const ST* f, const ST* g, const MT* F, const int* node, const double* posw,
+             const double* nB, const double* velw, double* Gw, double* contrib, const int* box, const double* Fbox,
+             hipStream_t st) {
+    const unsigned blocks = (unsigned)((A.M + 63) / 64);
+#define IPL_IBF(QQ) IPL_LAUNCH((ibk::k_ib_force<ST, MT, QQ>), blocks, 64, 0, st, A, it, f, g, F, node, posw, nB, velw, Gw, contrib, box, Fbox)
+    if (Q == 9) IPL_IBF(9); else if (Q == 19) IPL_IBF(19); else IPL_IBF(27);
+#undef IPL_IBF
+}
+
+template <typename ST>
+void apply_q(int Q, const ibk::IbArgs& A, ST* f, const int* box, const double* Fbox, hipStream_t st) {
+    const unsigned blocks = (unsigned)((A.vmax + 255) / 256);
+    if (Q == 9) IPL_LAUNCH((ibk::k_ib_apply<ST, 9>), blocks, 256, 0, st, A, f, box, Fbox);
+    else if (Q == 19) IPL_LAUNCH((ibk::k_ib_apply<ST, 19>), blocks, 256, 0, st, A, f, box, Fbox);
+    else IPL_LAUNCH((ibk::k_ib_apply<ST, 27>), blocks, 256, 0, st, A, f, box, Fbox);
+}
+
+}  // namespace
+
+Status IbSolver::substep(void* f, const void* g, const void* F, int storage) {
+    if (M_ == 0) return Status::ok();
+    const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_);
+    int mpad = 1;
+    while (mpad < M_) mpad <<= 1;
+    IPL_LAUNCH_SYNC(ibk::k_ib_sort, 1, 1024, (size_t)mpad * sizeof(unsigned long long), st_, A, node_, sorted_, box_, mpad);
+    const int iters = cfg_.method == IPL_IB_DIRECT ? std::max(1, cfg_.iterations) : 1;
+    const unsigned gblocks = (unsigned)((vmax_ + 255) / 256);
+    for (int it = 0; it < iters; ++it) {
+        if (storage == IPL_F64)
+            force_q<double, double>(g_.Q, A, it, (const double*)f, (const double*)g, (const double*)F, node_, posw_, nB_, velw_, Gw_,
+                                    contrib_, box_, Fbox_, st_);
+        else
+            force_q<float, float>(g_.Q, A, it, (const float*)f, (const float*)g, (const float*)F, node_, posw_, nB_, velw_, Gw_,
+                                  contrib_, box_, Fbox_, st_);
+        IPL_LAUNCH(ibk::k_ib_gather, gblocks, 256, 0, st_, A, it, sorted_, contrib_, box_, Fbox_);
+    }
+    if (storage == IPL_F64) apply_q<double>(g_.Q, A, (double*)f, box_, Fbox_, st_);
+    else apply_q<float>(g_.Q, A, (float*)f, box_, Fbox_, st_);
+    IPL_LAUNCH_SYNC(ibk::k_ib_body, 1, 256, 0, st_, A, Gw_, posw_, body_);
+    if (cfg_.moving) {
+        const unsigned blocks = (unsigned)((M_ + 63) / 64);
+        if (g_.Q == 9) IPL_LAUNCH((ibk::k_ib_points<9>), blocks, 64, 0, st_, A, body_, oposw_, onB_, posw_, nB_, velw_, node_);
+        else if (g_.Q == 19) IPL_LAUNCH((ibk::k_ib_points<19>), blocks, 64, 0, st_, A, body_, oposw_, onB_, posw_, nB_, velw_, node_);
+        else IPL_LAUNCH((ibk::k_ib_points<27>), blocks, 64, 0, st_, A, body_, oposw_, onB_, posw_, nB_, velw_, node_);
+    }
+    IPL_HIP_TRY(hipGetLastError());
+    return Status::ok();
+}
+
+Status IbSolver::check() {
+    if (M_ == 0) return Status::ok();
+    int flag = 0;
+    IPL_HIP_TRY(hipMemcpyAsync(&flag, box_ + 6, sizeof(int), hipMemcpyDeviceToHost, st_));
+    IPL_HIP_TRY(hipStreamSynchronize(st_));
+    if (flag) return Status::err(IPL_ERR_UNSUPPORTED, "the marker node blocks outgrew the box sized at attach");
+    return Status::ok();
+}
+
+Status IbSolver::body_state(double out[19]) {
+    if (M_ == 0) return Status::err(IPL_ERR_ARGUMENT, "no body attached");
+    double b[ibk::B_SIZE];
+    IPL_HIP_TRY(hipMemcpyAsync(b, body_, sizeof(b), hipMemcpyDeviceToHost, st_));
+    IPL_HIP_TRY(hipStreamSynchronize(st_));
+    for (int a = 0; a < 3; ++a) { out[a] = b[ibk::B_POS + a]; out[3 + a] = b[ibk::B_VEL + a]; out[6 + a] = b[ibk::B_OMEGA + a]; }
+    for (int i = 0; i < 4; ++i) out[9 + i] = b[ibk::B_QUAT + i];
+    for (int a = 0; a < 3; ++a) { out[13 + a] = b[ibk::B_FB + a]; out[16 + a] = b[ibk::B_T + a]; }
+    return Status::ok();
+}
+
+Status IbSolver::read_markers(int which, double* out, size_t count) {
+    if (M_ == 0) return Status::err(IPL_ERR_ARGUMENT, "no body attached");
+    if (which == IPL_IB_NODE) {
+        if (count != (size_t)M_) return Status::err(IPL_ERR_ARGUMENT, "marker array size mismatch");
+        std::vector<int> node(3 * M_);
+        IPL_HIP_TRY(hipMemcpyAsync(node.data(), node_, node.size() * sizeof(int), hipMemcpyDeviceToHost, st_));
+        IPL_HIP_TRY(hipStreamSynchronize(st_));
+        for (int m = 0; m < M_; ++m)
+            out[m] = (double)(((int64_t)node[3 * m] * g_.n[1] + node[3 * m + 1]) * g_.n[2] + node[3 * m + 2]);
+        return Status::ok();
+    }
+    const double* src = which == IPL_IB_POS ? posw_ : which == IPL_IB_VEL ? velw_ : which == IPL_IB_FORCE ? Gw_ :
+                        which == IPL_IB_NORMAL ? nB_ : nullptr;
+    if (!src) return Status::err(IPL_ERR_ARGUMENT, "marker array id");
+    if (count != (size_t)3 * M_) return Status::err(IPL_ERR_ARGUMENT, "marker array size mismatch");
+    IPL_HIP_TRY(hipMemcpyAsync(out, src, count * sizeof(double), hipMemcpyDeviceToHost, st_));
+    IPL_HIP_TRY(hipStreamSynchronize(st_));
+    return Status::ok();
+}
+
+}  // namespace ipl

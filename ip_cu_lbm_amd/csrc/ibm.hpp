@@ -1,0 +1,77 @@
+// ibm.hpp - immersed boundary on the fused path's line layout (function/ibm.cu).
+//
+// One rigid body carried by Lagrangian markers.  Every step, after the fused step has written
+// f^{n+1} (and the Force field F^n), the sub-step of the drivers (SphereFall.cu:372-383) runs:
+//   k_ib_sort    one workgroup: markers sorted by their nearest cell (bitonic, in LDS) and the
+//                bounding box of their node blocks
+//   k_ib_force   one work-item per marker: node weights, interpolation of f (stress method,
+//                get_IBMGw2) or of the node velocity (direct forcing, IB_directForcing), marker
+//                force, and the contribution of every block slot to the spread force
+//   k_ib_gather  one work-item per cell of the box: the spread force of the cell, summed over the
+//                block slots in slot order and, per slot, over markers in index order (no atomics:
+//                the sum has one order whatever the scheduling)
+//   (force + gather repeated for multi-direct forcing)
+//   k_ib_apply   update_velIBM on the cells of the box: f += w dt (e . F)
+//   k_ib_body    one workgroup: FP64 force / torque sums in a fixed tree, buoyancy, Adams-Bashforth
+//                update of the body (update_IBbody)
+//   k_ib_points  one work-item per marker: update_IBpoint and search_IBlattice
+// All marker and body arithmetic is FP64 whatever the population storage.
+#pragma once
+
+#include "hip_compat.hpp"
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/ipl.h"
+#include "engine.hpp"
+
+namespace ipl {
+
+// What the IB kernels need to know of the grid and the fused layout.
+struct IbGrid {
+    int n[3];
+    int periodic[3];
+    int s, o1, o2, ns;   // cell = ((c[o1] * n[o2]) + c[o2]) * ns + c[s]
+    int64_t ncell;
+    int dim, Q, Qs, scalar;
+    double dz, dt, c, tau;
+    double rho_L, rho_H, beta_s;
+    double period[3];    // sum of widths of periodic axes
+};
+
+class IbSolver {
+public:
+    ~IbSolver();
+    // Uploads markers and body, finds the nearest cells; pos[a] = cell-centre coordinates per axis.
+    Status attach(const ipl_ib_config& cfg, const IbGrid& g, const std::vector<double>* pos, hipStream_t st);
+    // The IB sub-step on the populations f of the fused path (storage F64 or F32), with F the Force
+    // field of the step (same type as the arithmetic) and g the scalar populations (or null).
+    Status substep(void* f, const void* g, const void* F, int storage);
+    Status body_state(double out[19]);
+    // IPL_ERR_UNSUPPORTED when a marker block left the bound the box was sized for
+    Status check();
+    Status read_markers(int which, double* out, size_t count);
+    bool active() const { return M_ > 0; }
+
+    // slots of a marker's node block
+    int slots() const { return S_; }
+
+private:
+    IbGrid g_{};
+    ipl_ib_config cfg_{};
+    hipStream_t st_ = nullptr;
+    int M_ = 0, S_ = 0, reach_lo_ = -1, reach_hi_ = 1;
+    int64_t vmax_ = 0;
+    int box_max_[3] = {0, 0, 0};
+    DeviceArena mem_;
+    double *pos_[3] = {nullptr, nullptr, nullptr};
+    double *oposw_ = nullptr, *onB_ = nullptr, *posw_ = nullptr, *nB_ = nullptr, *velw_ = nullptr, *Gw_ = nullptr;
+    double *contrib_ = nullptr, *w_ = nullptr, *Fbox_ = nullptr, *body_ = nullptr;
+    int *node_ = nullptr, *box_ = nullptr;
+    unsigned long long* sorted_ = nullptr;
+};
+
+constexpr int kIbMaxMarkers = 4096;  // one workgroup sorts them in 32 KiB of LDS
+
+}  // namespace ipl

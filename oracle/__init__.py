@@ -47,6 +47,12 @@ def load():
         lib.orc_lagrange.argtypes = [DP, c_double, c_int, DP]
         lib.orc_mrt.argtypes = [c_int, DP, DP, POINTER(c_int32), POINTER(c_int32)]
         lib.orc_lattice.argtypes = [c_int, POINTER(c_int32), DP]
+        lib.orc_ib_attach.argtypes = [c_void_p, c_void_p, c_char_p, c_int]
+        lib.orc_ib_body_state.argtypes = [c_void_p, DP]
+        lib.orc_ib_read_markers.argtypes = [c_void_p, c_int, DP, c_size_t]
+        lib.orc_ib_peskin4.argtypes = [c_double]
+        lib.orc_ib_peskin4.restype = c_double
+        lib.orc_ib_rotation.argtypes = [DP, DP]
         _LIB = lib
     return _LIB
 
@@ -72,6 +78,32 @@ class Oracle:
         lib.orc_info(h, ctypes.byref(n), ctypes.byref(g))
         self.n_calc, self.n_ghost = n.value, g.value
         del keep
+        if getattr(case, "body", None) is not None:
+            self.ib_attach(case.body)
+
+    def ib_attach(self, body):
+        cfg, keep = body.to_c()
+        err = ctypes.create_string_buffer(256)
+        rc = self._lib.orc_ib_attach(self._h, ctypes.byref(cfg), err, 256)
+        if rc != 0:
+            raise ValueError(f"oracle rejected the body: {err.value.decode()}")
+        self._n_markers = len(body.markers)
+        del keep
+
+    def body_state(self) -> dict:
+        out = np.zeros(19)
+        self._lib.orc_ib_body_state(self._h, _dp(out))
+        return {"pos": out[0:3], "vel": out[3:6], "omega": out[6:9], "quat": out[9:13], "force": out[13:16],
+                "torque": out[16:19]}
+
+    def markers(self, which: str) -> np.ndarray:
+        from ip_cu_lbm_amd._abi import IB_ARRAYS
+        m = self._n_markers
+        out = np.empty(m if which == "node" else 3 * m)
+        rc = self._lib.orc_ib_read_markers(self._h, IB_ARRAYS[which], _dp(out), out.size)
+        if rc != 0:
+            raise ValueError(f"oracle marker read failed ({rc})")
+        return out if which == "node" else out.reshape(m, 3)
 
     def close(self):
         if self._h:
@@ -125,6 +157,17 @@ class Oracle:
         out = np.empty(n, dtype=np.int32)
         self._lib.orc_table(self._h, which, out.ctypes.data_as(POINTER(c_int32)), n)
         return out
+
+
+def peskin4(r: float) -> float:
+    return load().orc_ib_peskin4(float(r))
+
+
+def rotation(q) -> np.ndarray:
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    R = np.empty(9)
+    load().orc_ib_rotation(_dp(q), _dp(R))
+    return R.reshape(3, 3)
 
 
 def lagrange(x, xa):

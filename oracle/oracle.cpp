@@ -203,6 +203,21 @@ struct Oracle {
     PopSet fl, sc;   // flow and scalar populations
     bool has_scalar = false;
     std::string err;
+
+    // grid index of calc cells (for the immersed boundary's node blocks)
+    int n[3] = {0, 0, 0};
+    std::vector<int> lnum;        // (x*ny + y)*nz + z -> calc id or -1
+    std::vector<int> cidx[3];     // calc id -> coordinate index per axis
+
+    // immersed boundary (ibm.cu)
+    struct IB {
+        bool active = false;
+        ipl_ib_config cfg{};
+        int M = 0;
+        std::vector<double> oposw, onB, posw, nB, velw, Gw;  // 3M
+        std::vector<int> lattice_id;                          // M
+        double posB[3], velB[3], angleVB[3], quat[4], quatold[4], quaS[9], FB[3], Torque[3];
+    } ib;
 };
 
 inline int axis_dir(const Lattice& L, int axis, int sgn) {
@@ -381,6 +396,9 @@ bool build(Oracle& o, const ipl_config& cfg, const ipl_init* init) {
                 const int id = lnum[((size_t)i * ny + j) * nz + l];
                 if (id >= 0) { ci[id] = i; cj[id] = j; cl[id] = l; }
             }
+    o.n[0] = nx; o.n[1] = ny; o.n[2] = nz;
+    o.lnum = lnum;
+    o.cidx[0] = ci; o.cidx[1] = cj; o.cidx[2] = cl;
     // neib (LockExchange.cu:117-127), with periodic wrap as the builder extension
     o.neib.assign((size_t)N * Q, -1);
     for (int id = 0; id < N; ++id)
@@ -934,6 +952,376 @@ void reset_force(Oracle& o) {
     std::fill(o.Fz.begin(), o.Fz.end(), 0.0);
 }
 
+// ------------------------------------------------------------------------------------------
+// Immersed boundary (function/ibm.cu) and the drivers' marker set-up (SphereFall.cu:197-261).
+// One rigid body; markers are processed in index order, so every sum below has a fixed order
+// (the reference spreads with atomicAdd, whose order is unspecified).
+// ------------------------------------------------------------------------------------------
+
+// rotation matrix (row-major) of the orientation quaternion, set_quaternionS (ibm.cu:16-28)
+void ib_rotation(const double q[4], double R[9]) {
+    const double a = q[0], b = q[1], c = q[2], d = q[3];
+    R[0] = a * a - c * c - d * d + b * b; R[1] = 2 * (b * c - a * d);         R[2] = 2 * (b * d + a * c);
+    R[3] = 2 * (b * c + a * d);         R[4] = a * a - d * d - b * b + c * c; R[5] = 2 * (c * d - a * b);
+    R[6] = 2 * (b * d - a * c);         R[7] = 2 * (c * d + a * b);         R[8] = a * a - b * b - c * c + d * d;
+}
+
+// 4-point Peskin function of r = |x - x_node| / dz, weightFunction (ibm.cu:61-66)
+double ib_peskin4(double r) {
+    if (r < 1.0) return (3.0 - 2.0 * r + std::sqrt(1.0 + 4.0 * r - 4.0 * r * r)) / 8.0;
+    if (r <= 2.0) return (5.0 - 2.0 * r - std::sqrt(-7.0 + 12.0 * r - 4.0 * r * r)) / 8.0;
+    return 0.0;
+}
+
+// Calc id of the cell displaced by d from calc cell id, or -1 where the reference's neighbour is
+// a ghost (outside a wall, or a masked cell).  pos receives the node coordinates: a ghost sits on
+// its owner (basic.cu:221-225); across a periodic seam the coordinate is unwrapped.
+int ib_node(const Oracle& o, int id, const int d[3], double pos[3]) {
+    int c[3];
+    double shift[3] = {0, 0, 0};
+    bool ghost = false;
+    for (int a = 0; a < 3; ++a) {
+        c[a] = o.cidx[a][id] + d[a];
+        if (c[a] < 0 || c[a] >= o.n[a]) {
+            if (o.cfg.periodic[a]) {
+                shift[a] = c[a] < 0 ? -o.period[a] : o.period[a];
+                c[a] = c[a] < 0 ? c[a] + o.n[a] : c[a] - o.n[a];
+            } else {
+                ghost = true;
+            }
+        }
+    }
+    const int nid = ghost ? -1 : o.lnum[((size_t)c[0] * o.n[1] + c[1]) * o.n[2] + c[2]];
+    for (int a = 0; a < 3; ++a) pos[a] = nid < 0 ? o.pos[a][id] : o.pos[a][nid] + shift[a];
+    return nid;
+}
+
+// Displacement of node (i, j, l) of the 3x3(x3) block, the neighbour directions of idfIndex
+// (ibm.cu:132-141).  In 2D the entry of (x-1, z-1) is direction 5 of D2Q9, i.e. (x+1, z+1);
+// that node is kept as the reference has it.
+void ib_block_offset(int dim, int i, int j, int l, int d[3]) {
+    if (dim == 2) {
+        d[0] = i - 1; d[1] = 0; d[2] = l - 1;
+        if (i == 0 && l == 0) { d[0] = 1; d[2] = 1; }
+    } else {
+        d[0] = i - 1; d[1] = j - 1; d[2] = l - 1;
+    }
+}
+
+struct IBWeights {
+    int lo, hi;                 // node offsets lo..hi on each axis (y: 0..0 in 2D)
+    double w[3][5];             // weights per axis, indexed by offset - lo
+};
+
+// Node weights of marker m on the block around its nearest cell.
+//  stress  (get_IBMGw2, ibm.cu:146-189): x (and z in 2D) by the Peskin function on the three nodes,
+//          masked at ghosts and, in 2D, renormalised; y, z (3D) by 3-point Lagrange.
+//  direct  (IB_directForcing, ibm.cu:292-313): 3-point Lagrange on every axis.
+//  peskin4 (direct, builder extension): the Peskin function on offsets -2..2 (its full support).
+IBWeights ib_weights(const Oracle& o, int m, bool stress) {
+    const auto& B = o.ib;
+    const int id = B.lattice_id[m];
+    const double* xw = &B.posw[3 * m];
+    IBWeights W{};
+    const bool pk4 = !stress && B.cfg.kernel == IPL_IB_KERNEL_PESKIN4;
+    W.lo = pk4 ? -2 : -1; W.hi = pk4 ? 2 : 1;
+    const int npt = W.hi - W.lo + 1;
+    for (int a = 0; a < 3; ++a) {
+        if (o.dim == 2 && a == 1) { for (int t = 0; t < npt; ++t) W.w[a][t] = (t == -W.lo) ? 1.0 : 0.0; continue; }
+        double xs[5];
+        int ids[5];
+        for (int t = 0; t < npt; ++t) {
+            int d[3] = {0, 0, 0};
+            d[a] = W.lo + t;
+            double pp[3];
+            ids[t] = ib_node(o, id, d, pp);
+            xs[t] = pp[a];
+        }
+        const bool peskin = pk4 || (stress && (a == 0 || (o.dim == 2 && a == 2)));
+        if (peskin) {
+            for (int t = 0; t < npt; ++t) W.w[a][t] = ib_peskin4(std::fabs(xw[a] - xs[t]) / o.dz) * (ids[t] >= 0 ? 1.0 : 0.0);
+            if (stress && o.dim == 2) {
+                double s = 0;
+                for (int t = 0; t < npt; ++t) s += W.w[a][t];
+                for (int t = 0; t < npt; ++t) W.w[a][t] /= s;
+            }
+        } else {
+            lagrange(W.w[a], xs, xw[a], 3);
+        }
+    }
+    return W;
+}
+
+// Calls fn(node id, weight) for every node of marker m's block in the reference's loop order
+// (x outer, y, z inner); ghost nodes are skipped (the factor (id < num_calc) of ibm.cu:194).
+template <typename Fn>
+void ib_for_nodes(const Oracle& o, int m, const IBWeights& W, bool block3, Fn&& fn) {
+    const int id = o.ib.lattice_id[m];
+    const int jlo = o.dim == 2 ? 0 : W.lo, jhi = o.dim == 2 ? 0 : W.hi;
+    for (int i = W.lo; i <= W.hi; ++i)
+        for (int j = jlo; j <= jhi; ++j)
+            for (int l = W.lo; l <= W.hi; ++l) {
+                int d[3];
+                if (block3) ib_block_offset(o.dim, i + 1, j + 1, l + 1, d);
+                else { d[0] = i; d[1] = j; d[2] = l; }
+                double pp[3];
+                const int nid = ib_node(o, id, d, pp);
+                const double wgt = W.w[0][i - W.lo] * W.w[1][j - W.lo] * W.w[2][l - W.lo];
+                if (nid >= 0) fn(nid, wgt);
+            }
+}
+
+// get_IBMGw2 (ibm.cu:124-274): f interpolated at the marker, split into the outer (fw1) and inner
+// (fw2) populations by the marker normal, the wall stress difference spread to F, and the marker
+// force Gw from the outer populations.
+void ib_stress(Oracle& o) {
+    auto& B = o.ib;
+    const int Q = o.Q;
+    const double c = o.c, c2 = c * c;
+    const double alpha = 22.5 / 180.0 * 3.14159;  // the reference's literal
+    const int dimension = o.dim;
+    const double spread = std::pow(B.cfg.spacing / o.dz, dimension - 1) / o.dz;
+    for (int m = 0; m < B.M; ++m) {
+        const IBWeights W = ib_weights(o, m, true);
+        std::vector<double> fw(Q, 0.0), fw1(Q), fw2(Q);
+        ib_for_nodes(o, m, W, true, [&](int nid, double wgt) {
+            for (int k = 0; k < Q; ++k) fw[k] += wgt * o.fl.f[(size_t)nid * Q + k];
+        });
+        const double U[3] = {B.velw[3 * m], B.velw[3 * m + 1], B.velw[3 * m + 2]};
+        const double nv[3] = {B.nB[3 * m], B.nB[3 * m + 1], B.nB[3 * m + 2]};
+        for (int k = 0; k < Q; ++k) {
+            const double e[3] = {(double)o.L.cx[k], (double)o.L.cy[k], (double)o.L.cz[k]};
+            const double cn = (nv[0] * e[0] + nv[1] * e[1] + nv[2] * e[2]) / c;
+            const double lim = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) / c * std::sin(alpha);
+            const double bounce = fw[o.L.kinv[k]] + 6.0 * o.L.w[k] * (e[0] * U[0] + e[1] * U[1] + e[2] * U[2]) / c2;
+            const double beta = std::fabs(cn) / (lim + 1e-10);
+            const double mixed = (1.0 - beta) * fw[k] + beta * bounce;
+            fw1[k] = cn <= 0 ? fw[k] : (cn <= lim ? mixed : bounce);
+            fw2[k] = cn >= 0 ? fw[k] : (cn >= -lim ? mixed : bounce);
+        }
+        const double tau = o.tau[B.lattice_id[m]];
+        // traction sigma . n of the populations g (ibm.cu:207-227): sigma_ab = -sum_k [ (1/(2 tau)) g_k delta_ab
+        //   + (1 - 1/(2 tau)) 3 (g_k (e_a - U_a)(e_b - U_b) - (g_k - w_k) U_a U_b) / c^2 ]
+        auto traction = [&](const std::vector<double>& g1, const std::vector<double>* g2, double out[3]) {
+            for (int a = 0; a < 3; ++a) {
+                double sg[3] = {0, 0, 0};
+                for (int k = 0; k < Q; ++k) {
+                    const double gk = g2 ? g1[k] - (*g2)[k] : g1[k];
+                    const double e[3] = {(double)o.L.cx[k], (double)o.L.cy[k], (double)o.L.cz[k]};
+                    for (int b = 0; b < 3; ++b)
+                        sg[b] -= 0.5 / tau * gk * (a == b ? 1.0 : 0.0) +
+                                 (1.0 - 0.5 / tau) * 3 * (gk * (e[a] - U[a]) * (e[b] - U[b]) - (gk - o.L.w[k]) * U[a] * U[b]) / c2;
+                }
+                out[a] = sg[0] * nv[0] + sg[1] * nv[1] + sg[2] * nv[2];
+            }
+        };
+        double g[3];
+        traction(fw1, &fw2, g);
+        for (int a = 0; a < 3; ++a) g[a] = -g[a];
+        ib_for_nodes(o, m, W, true, [&](int nid, double wgt) {
+            o.Fx[nid] += g[0] * spread * wgt;
+            o.Fy[nid] += g[1] * spread * wgt;
+            o.Fz[nid] += g[2] * spread * wgt;
+        });
+        double gw[3];
+        traction(fw1, nullptr, gw);
+        const double scale = B.cfg.rho_f * c2 / 3.0 * std::pow(B.cfg.spacing, dimension - 1);
+        for (int a = 0; a < 3; ++a) B.Gw[3 * m + a] = gw[a] * scale;
+    }
+}
+
+// IB_directForcing (ibm.cu:276-344): node velocity interpolated at the marker, force
+// (U_w - u)/dt spread to F.  With iterations > 1 (multi-direct forcing, builder extension) the
+// node velocities are corrected by the spread force, du = dt dF / 3 (what update_velIBM does to
+// them, sum_k w_k e_k e_k = 1/3), and the pass is repeated.
+void ib_direct(Oracle& o) {
+    auto& B = o.ib;
+    const double c2 = o.c * o.c;
+    const int dimension = o.dim;
+    const double spread = std::pow(B.cfg.spacing / o.dz, dimension - 1) * 3.0 / c2;
+    const double wscale = B.cfg.rho_f * std::pow(B.cfg.spacing, dimension - 1) * o.dz;
+    const bool block3 = B.cfg.kernel != IPL_IB_KERNEL_PESKIN4;
+    std::vector<double> vel[3] = {o.u, o.v, o.w};
+    std::vector<IBWeights> W(B.M);
+    for (int m = 0; m < B.M; ++m) W[m] = ib_weights(o, m, false);
+    const int iters = std::max(1, B.cfg.iterations);
+    std::vector<double> dF[3];
+    for (int it = 0; it < iters; ++it) {
+        for (int a = 0; a < 3; ++a) dF[a].assign(o.N, 0.0);
+        std::vector<double> gw(3 * (size_t)B.M);
+        for (int m = 0; m < B.M; ++m) {
+            double um[3] = {0, 0, 0};
+            ib_for_nodes(o, m, W[m], block3, [&](int nid, double wgt) {
+                for (int a = 0; a < 3; ++a) um[a] += wgt * vel[a][nid];
+            });
+            for (int a = 0; a < 3; ++a) gw[3 * m + a] = (B.velw[3 * m + a] - um[a]) / o.dt;
+        }
+        for (int m = 0; m < B.M; ++m) {
+            ib_for_nodes(o, m, W[m], block3, [&](int nid, double wgt) {
+                for (int a = 0; a < 3; ++a) dF[a][nid] += gw[3 * m + a] * spread * wgt;
+            });
+            for (int a = 0; a < 3; ++a) B.Gw[3 * m + a] += -gw[3 * m + a] * wscale;
+        }
+        for (int id = 0; id < o.N; ++id) {
+            o.Fx[id] += dF[0][id]; o.Fy[id] += dF[1][id]; o.Fz[id] += dF[2][id];
+            for (int a = 0; a < 3; ++a) vel[a][id] += o.dt * dF[a][id] / 3.0;
+        }
+    }
+}
+
+// update_velIBM (ibm.cu:346-365): f += w dt (e . F); p and u recomputed from f (no half-force term)
+void ib_update_vel(Oracle& o) {
+    const int Q = o.Q;
+    #pragma omp parallel for schedule(static)
+    for (int id = 0; id < o.N; ++id) {
+        double p = 0, ux = 0, uy = 0, uz = 0;
+        for (int k = 0; k < Q; ++k) {
+            double& f = o.fl.f[(size_t)id * Q + k];
+            f = f + o.L.w[k] * o.dt * (o.L.cx[k] * o.Fx[id] + o.L.cy[k] * o.Fy[id] + o.L.cz[k] * o.Fz[id]);
+            p += f; ux += o.L.cx[k] * f; uy += o.L.cy[k] * f; uz += o.L.cz[k] * f;
+        }
+        o.p[id] = p; o.u[id] = ux; o.v[id] = uy; o.w[id] = uz;
+    }
+}
+
+// update_IBbody (ibm.cu:367-424): force and torque summed over the markers, buoyancy, and (for a
+// moving body) second-order Adams-Bashforth steps of velocity, position, body-frame angular
+// velocity and orientation quaternion.
+void ib_body(Oracle& o) {
+    auto& B = o.ib;
+    const double dt = o.dt;
+    double Rold[9], FBold[3], Told[3], velBold[3], angold[3];
+    ib_rotation(B.quatold, Rold);
+    for (int a = 0; a < 3; ++a) {
+        FBold[a] = B.FB[a]; B.FB[a] = 0; Told[a] = B.Torque[a]; B.Torque[a] = 0;
+        velBold[a] = B.velB[a]; angold[a] = B.angleVB[a];
+    }
+    for (int m = 0; m < B.M; ++m) {
+        const double* G = &B.Gw[3 * m];
+        const double r[3] = {B.posw[3 * m] - B.posB[0], B.posw[3 * m + 1] - B.posB[1], B.posw[3 * m + 2] - B.posB[2]};
+        for (int a = 0; a < 3; ++a) B.FB[a] += G[a];
+        B.Torque[0] += r[1] * G[2] - r[2] * G[1];
+        B.Torque[1] += r[2] * G[0] - r[0] * G[2];
+        B.Torque[2] += r[0] * G[1] - r[1] * G[0];
+    }
+    for (int a = 0; a < 3; ++a) B.FB[a] += (1.0 - B.cfg.rho_f / B.cfg.density) * B.cfg.mass * B.cfg.gravity[a];
+    if (!B.cfg.moving) return;
+    for (int a = 0; a < 3; ++a) {
+        B.velB[a] += dt * (3.0 * B.FB[a] - FBold[a]) / B.cfg.mass / 2.0;
+        B.posB[a] += dt * (3.0 * B.velB[a] - velBold[a]) / 2.0;
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            B.angleVB[i] += dt * (3.0 * B.quaS[j * 3 + i] * B.Torque[j] - Rold[j * 3 + i] * Told[j]) / B.cfg.inertia[i] / 2.0;
+    double q[4], w[3];
+    for (int i = 0; i < 4; ++i) q[i] = B.quat[i];
+    for (int i = 0; i < 3; ++i) w[i] = B.angleVB[i];
+    // dq/dt = (1/2) Omega(w) q, Adams-Bashforth: q += dt (3 Omega(w) q - Omega(w_old) q_old) / 4
+    B.quat[0] += dt * 3.0 * (-w[0] * q[1] - w[1] * q[2] - w[2] * q[3]) / 4.0;
+    B.quat[1] += dt * 3.0 * (w[0] * q[0] + w[2] * q[2] - w[1] * q[3]) / 4.0;
+    B.quat[2] += dt * 3.0 * (w[1] * q[0] - w[2] * q[1] + w[0] * q[3]) / 4.0;
+    B.quat[3] += dt * 3.0 * (w[2] * q[0] + w[1] * q[1] - w[0] * q[2]) / 4.0;
+    const double* qo = B.quatold;
+    B.quat[0] -= dt * (-angold[0] * qo[1] - angold[1] * qo[2] - angold[2] * qo[3]) / 4.0;
+    B.quat[1] -= dt * (angold[0] * qo[0] + angold[2] * qo[2] - angold[1] * qo[3]) / 4.0;
+    B.quat[2] -= dt * (angold[1] * qo[0] - angold[2] * qo[1] + angold[0] * qo[3]) / 4.0;
+    B.quat[3] -= dt * (angold[2] * qo[0] + angold[1] * qo[1] - angold[0] * qo[2]) / 4.0;
+    for (int i = 0; i < 4; ++i) B.quatold[i] = q[i];
+    ib_rotation(B.quat, B.quaS);
+}
+
+// update_IBpoint (ibm.cu:426-447) and search_IBlattice (ibm.cu:449-463)
+void ib_points(Oracle& o) {
+    auto& B = o.ib;
+    const double* R = B.quaS;
+    const double* w = B.angleVB;
+    for (int m = 0; m < B.M; ++m) {
+        const double* op = &B.oposw[3 * m];
+        const double* on = &B.onB[3 * m];
+        const double wxo[3] = {w[1] * op[2] - w[2] * op[1], w[2] * op[0] - w[0] * op[2], w[0] * op[1] - w[1] * op[0]};
+        for (int i = 0; i < 3; ++i) {
+            B.posw[3 * m + i] = B.posB[i];
+            B.velw[3 * m + i] = B.velB[i];
+            B.nB[3 * m + i] = 0;
+            for (int j = 0; j < 3; ++j) {
+                B.posw[3 * m + i] += R[i * 3 + j] * op[j];
+                B.nB[3 * m + i] += R[i * 3 + j] * on[j];
+            }
+            B.velw[3 * m + i] += R[i * 3 + 0] * wxo[0] + R[i * 3 + 1] * wxo[1] + R[i * 3 + 2] * wxo[2];
+        }
+        // nearest node among the current one and its lattice neighbours (ghosts sit on their owner)
+        const int id = B.lattice_id[m];
+        auto d2 = [&](int c) {
+            double s = 0;
+            for (int a = 0; a < 3; ++a) s += (o.pos[a][c] - B.posw[3 * m + a]) * (o.pos[a][c] - B.posw[3 * m + a]);
+            return s;
+        };
+        int near = id;
+        double best = d2(id);
+        for (int k = 1; k < o.Q; ++k) {
+            const int nid = o.neib[(size_t)id * o.Q + k];
+            const double dd = d2(nid);
+            if (dd < best) { best = dd; near = nid; }
+        }
+        B.lattice_id[m] = near;
+    }
+}
+
+// The IB sub-step after update_rho and resetF (SphereFall.cu:372-383), then resetF
+// (SphereFall.cu:433).
+void ib_substep(Oracle& o) {
+    auto& B = o.ib;
+    reset_force(o);
+    std::fill(B.Gw.begin(), B.Gw.end(), 0.0);
+    if (B.cfg.method == IPL_IB_STRESS) ib_stress(o);
+    else ib_direct(o);
+    ib_update_vel(o);
+    ib_body(o);
+    if (B.cfg.moving) ib_points(o);
+    reset_force(o);
+}
+
+// Marker set-up (SphereFall.cu:203-258): marker positions and unit normals from the body state,
+// nearest calc cell by exhaustive search (first minimum in calc order).
+bool ib_attach(Oracle& o, const ipl_ib_config& cfg) {
+    auto& B = o.ib;
+    if (cfg.n_markers <= 0 || !cfg.marker_pos || !cfg.marker_normal) { o.err = "markers missing"; return false; }
+    if (cfg.method == IPL_IB_STRESS && cfg.kernel != IPL_IB_KERNEL_REFERENCE) { o.err = "the stress method uses the reference weights"; return false; }
+    B.cfg = cfg;
+    B.M = cfg.n_markers;
+    B.oposw.assign(cfg.marker_pos, cfg.marker_pos + 3 * B.M);
+    B.onB.assign(cfg.marker_normal, cfg.marker_normal + 3 * B.M);
+    B.posw.assign(3 * B.M, 0); B.nB.assign(3 * B.M, 0); B.velw.assign(3 * B.M, 0); B.Gw.assign(3 * B.M, 0);
+    B.lattice_id.assign(B.M, 0);
+    for (int a = 0; a < 3; ++a) {
+        B.posB[a] = cfg.pos[a]; B.velB[a] = cfg.vel[a]; B.angleVB[a] = cfg.omega[a]; B.FB[a] = 0; B.Torque[a] = 0;
+    }
+    for (int i = 0; i < 4; ++i) { B.quat[i] = cfg.quat[i]; B.quatold[i] = cfg.quat[i]; }
+    ib_rotation(B.quat, B.quaS);
+    for (int m = 0; m < B.M; ++m) {
+        const double* op = &B.oposw[3 * m];
+        const double* on = &B.onB[3 * m];
+        const double nn = std::sqrt(on[0] * on[0] + on[1] * on[1] + on[2] * on[2]);
+        for (int i = 0; i < 3; ++i) {
+            double p = 0, nv = 0;
+            for (int j = 0; j < 3; ++j) { p += B.quaS[i * 3 + j] * op[j]; nv += B.quaS[i * 3 + j] * on[j]; }
+            B.nB[3 * m + i] = nv / nn;
+            B.posw[3 * m + i] = B.posB[i] + p;
+        }
+        // the drivers leave velw at zero until the first update_IBpoint
+        double best = 1e300;
+        int near = 0;
+        for (int id = 0; id < o.N; ++id) {
+            double s = 0;
+            for (int a = 0; a < 3; ++a) s += (o.pos[a][id] - B.posw[3 * m + a]) * (o.pos[a][id] - B.posw[3 * m + a]);
+            if (s < best) { best = s; near = id; }
+        }
+        B.lattice_id[m] = near;
+    }
+    B.active = true;
+    return true;
+}
+
 void stage(Oracle& o, int s) {
     switch (s) {
     case IPL_STAGE_EQU_F: equ_f(o); break;
@@ -961,6 +1349,7 @@ void stage(Oracle& o, int s) {
 // One step in driver order (LockExchange.cu:220-278; Zhou_soliton4B1.cu:220-277)
 void step(Oracle& o) {
     for (int s = IPL_STAGE_EQU_F; s <= IPL_STAGE_RESET_F; ++s) stage(o, s);
+    if (o.ib.active) ib_substep(o);
 }
 
 }  // namespace
@@ -1068,6 +1457,44 @@ int64_t orc_table(void* h, int which, int32_t* out, int64_t count) {
 }
 
 void orc_lagrange(const double* x, double xa, int n, double* coeff) { lagrange(coeff, x, xa, n); }
+
+int orc_ib_attach(void* h, const ipl_ib_config* cfg, char* err, int errlen) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    if (!ib_attach(o, *cfg)) {
+        if (err && errlen > 0) std::snprintf(err, errlen, "%s", o.err.c_str());
+        return IPL_ERR_ARGUMENT;
+    }
+    return 0;
+}
+
+int orc_ib_body_state(void* h, double* out) {
+    const auto& B = static_cast<Oracle*>(h)->ib;
+    for (int a = 0; a < 3; ++a) { out[a] = B.posB[a]; out[3 + a] = B.velB[a]; out[6 + a] = B.angleVB[a]; }
+    for (int i = 0; i < 4; ++i) out[9 + i] = B.quat[i];
+    for (int a = 0; a < 3; ++a) { out[13 + a] = B.FB[a]; out[16 + a] = B.Torque[a]; }
+    return 0;
+}
+
+int orc_ib_read_markers(void* h, int which, double* out, size_t count) {
+    const auto& B = static_cast<Oracle*>(h)->ib;
+    if (which == IPL_IB_NODE) {
+        if (count != (size_t)B.M) return IPL_ERR_ARGUMENT;
+        const Oracle& o = *static_cast<Oracle*>(h);
+        for (int m = 0; m < B.M; ++m) {
+            const int id = B.lattice_id[m];
+            out[m] = (double)(((size_t)o.cidx[0][id] * o.n[1] + o.cidx[1][id]) * o.n[2] + o.cidx[2][id]);
+        }
+        return 0;
+    }
+    const std::vector<double>* v = which == IPL_IB_POS ? &B.posw : which == IPL_IB_VEL ? &B.velw :
+                                   which == IPL_IB_FORCE ? &B.Gw : which == IPL_IB_NORMAL ? &B.nB : nullptr;
+    if (!v || count != v->size()) return IPL_ERR_ARGUMENT;
+    std::copy(v->begin(), v->end(), out);
+    return 0;
+}
+
+double orc_ib_peskin4(double r) { return ib_peskin4(r); }
+void orc_ib_rotation(const double* q, double* R) { ib_rotation(q, R); }
 
 int orc_mrt(int Q, double* M, double* Minv, int32_t* stress, int32_t* nstress) {
     if (Q != 9 && Q != 19 && Q != 27) return -1;

@@ -27,6 +27,8 @@
 
 using std::isfinite;
 using std::sqrt;
+using std::min;
+using std::max;
 
 struct ipl_emu_idx3 { unsigned x = 0, y = 0, z = 0; };
 extern thread_local ipl_emu_idx3 threadIdx, blockIdx, blockDim, gridDim;
