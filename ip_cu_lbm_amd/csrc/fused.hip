@@ -302,7 +302,10 @@ public:
         g.dz = gr_.dz; g.dt = gr_.dt; g.c = gr_.c; g.tau = P_.tau;
         g.rho_L = cfg_.rho_L; g.rho_H = cfg_.rho_H; g.beta_s = cfg_.beta_s;
         Status r = ib_.attach(cfg, g, cfg_pos_, stream);
-        if (r) geo_.half_force = 0;
+        if (r) {
+            geo_.half_force = 0;
+            K_.uc = MT(1);
+        }
         return r;
     }
     Status ib_body_state(double* out) override { return ib_.body_state(out); }

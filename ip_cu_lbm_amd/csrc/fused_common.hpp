@@ -67,6 +67,7 @@ struct FParams {
 template <typename MT>
 struct FCoef {
     MT c, inv_c, inv_c2, half_dt, fk_coef, dt;
+    MT uc;                             // velocity scale of sum e f: c, or 1 after an IB sub-step
     MT itau, om_s, rnorm[27];
     MT rho_L, drho, beta_s, gravity;   // drho = rho_H - rho_L (in MT)
     MT press, st_beta, kap, inv_dz2;
@@ -79,6 +80,7 @@ inline FCoef<MT> make_coef(const FParams& P) {
     FCoef<MT> k{};
     k.c = (MT)P.c; k.inv_c = (MT)P.inv_c; k.inv_c2 = (MT)P.inv_c2; k.half_dt = (MT)P.half_dt;
     k.fk_coef = (MT)P.fk_coef; k.dt = (MT)P.dt;
+    k.uc = (MT)P.c;
     k.itau = (MT)P.itau; k.om_s = (MT)P.om_s;
     for (int r = 0; r < 27; ++r) k.rnorm[r] = (MT)P.rnorm[r];
     k.rho_L = (MT)P.rho_L; k.drho = (MT)P.rho_H - (MT)P.rho_L; k.beta_s = (MT)P.beta_s; k.gravity = (MT)P.gravity;

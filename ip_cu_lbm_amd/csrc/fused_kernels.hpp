@@ -236,19 +236,70 @@ __device__ inline void push(const FGeom& G, const LineNb& nb, ST* __restrict__ d
 // P: collided populations of this work-item's cell.  lds: [ns][SL] of the workgroup, one row of
 // the S-directions' values per cell (SL = their count + 1, an odd stride for the LDS banks), so a
 // direction's slot is an immediate offset from the cell's row.
-template <typename ST, typename MT, int Q, int NPTS, int S>
+template <typename ST, typename MT, int Q, int NPTS, int S, bool FLUX>
 __device__ inline void transport(const FGeom& G, const LineNb& nb, const MT (&Wp)[NPTS], const MT (&Wm)[NPTS], MT rn,
                                  ST* __restrict__ dst, const MT (&P)[Q], MT* lds, int i, bool active) {
     const int ns = G.ns;
     constexpr int SL = sdir_count<Q, S>() + 1;
-    if (!G.flux) {
+    if constexpr (!FLUX) {
+        // uniform grid: exact streaming (propagation.cu:2-12).  Opposite S-directions (k, kb_) are
+        // stored as a pair at this cell's S-index, so a wavefront writes whole lines: row k of the
+        // line displaced by (ka, kb) receives P_k of the cell before (i - 1, through LDS), row kb_
+        // of the opposite line P_kb of the cell after.  The first (last) cell of a walled line has
+        // no cell before (after); its slot carries instead the wall redirect of its own P_kb (P_k):
+        // bounce-back into its opposite row, slip into the row reflected across the S-wall
+        // (set_wall_propagation, set_bound.cu:149-168).
         if (active)
             static_for<Q>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                push<ST, MT, Q, k, S>(G, nb, dst, i, P[k]);
+                if constexpr (comp<Q>(k, S) != 0) lds[i * SL + sdir_slot<Q, S>(k)] = P[k];
             });
+        __syncthreads();
+        if (!active) return;
+        constexpr int o1 = LineAxes<S>::o1, o2 = LineAxes<S>::o2;
+        const uint32_t NC = (uint32_t)G.ncell;
+        const int per = G.periodic[S];
+        static_for<Q>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int sg = comp<Q>(k, S);
+            constexpr int ka = comp<Q>(k, o1), kb = comp<Q>(k, o2);
+            constexpr double wk = Lattice<Q>::w(k);
+            if constexpr (sg == 0) {
+                if (!nb.template out1<ka>() && !nb.template out2<kb>())
+                    Store<ST, MT>::store(elem(dst, k * NC + nb.template base<ka, kb>() + (uint32_t)i), P[k], wk, G.scale);
+                else
+                    push<ST, MT, Q, k, S>(G, nb, dst, i, P[k]);
+            } else if constexpr (sg > 0) {
+                constexpr int kb_ = opposite<Q>(k);
+                constexpr int sk = sdir_slot<Q, S>(k), skb = sdir_slot<Q, S>(kb_);
+                const bool lines_in = !nb.template out1<ka>() && !nb.template out2<kb>() &&
+                                      !nb.template out1<-ka>() && !nb.template out2<-kb>();
+                const bool last = !per && i == ns - 1, first = !per && i == 0;
+                const int im = wrap_or_clamp(i - 1, ns, per), ip = wrap_or_clamp(i + 1, ns, per);
+                if (lines_in) {
+                    const MT Uk = lds[im * SL + sk], Ukb = lds[ip * SL + skb];
+                    const uint32_t fwd = nb.template base<ka, kb>() + (uint32_t)i;
+                    const uint32_t bwd = nb.template base<-ka, -kb>() + (uint32_t)i;
+                    const uint32_t own = nb.b00 + (uint32_t)i;
+                    uint32_t a1, a2;
+                    if (G.wall_bc == IPL_BC_BOUNCE_BACK) {
+                        a1 = k * NC + (first ? own : fwd);
+                        a2 = kb_ * NC + (last ? own : bwd);
+                    } else {
+                        constexpr int rk = reflect_dir<Q>(k, 1 << S), rkb = reflect_dir<Q>(kb_, 1 << S);
+                        a1 = first ? rkb * NC + bwd : k * NC + fwd;
+                        a2 = last ? rk * NC + fwd : kb_ * NC + bwd;
+                    }
+                    Store<ST, MT>::store(elem(dst, a1), first ? P[kb_] : Uk, wk, G.scale);
+                    Store<ST, MT>::store(elem(dst, a2), last ? P[k] : Ukb, wk, G.scale);
+                } else {
+                    push<ST, MT, Q, k, S>(G, nb, dst, i, P[k]);
+                    push<ST, MT, Q, kb_, S>(G, nb, dst, i, P[kb_]);
+                }
+            }
+        });
         return;
-    }
+    } else {
     // 1. collided values of the S-directions into LDS
     if (active)
         static_for<Q>([&](auto kc) {
@@ -342,6 +393,7 @@ __device__ inline void transport(const FGeom& G, const LineNb& nb, const MT (&Wp
             }
         }
     });
+    }
 }
 
 // A direction that represents its opposite pair (the rest direction stands alone).
@@ -503,7 +555,7 @@ __device__ inline MT grad_axis(const FGeom& G, const LineNb& nb, const FTables<M
     return (*elem(q, nb_index<S, A, 1>(G, nb, i)) - *elem(q, nb_index<S, A, -1>(G, nb, i))) * *elem(tb.invd[A], (uint32_t)x);
 }
 
-template <typename ST, typename MT, int Q, int QS, int NPTS, int S>
+template <typename ST, typename MT, int Q, int QS, int NPTS, int S, bool FLUX>
 __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGeom G, FTables<MT> tb, FBufs<ST, MT> b, FCoef<MT> P) {
     IPL_DYNAMIC_LDS(MT, lds_f);
     MT* lds_g = lds_f + (sdir_count<Q, S>() + 1) * G.ns;
@@ -550,13 +602,18 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
         const MT phi = G.scalar == IPL_SCALAR_PHASE_FIELD ? sc : MT(0);
         const MT rho = P.rho_L + P.drho * phi + S_ * P.beta_s;
         const MT irho = MT(1) / rho;
-        MT u = ex, v = ey, w = ez;
+        // u = sum e c f + F_old dt / (2 rho); after an immersed-boundary sub-step u = sum e f
+        // (uc = 1, no force term).  Only the loads are conditional, so the code stays one path.
+        MT F0 = MT(0), F1 = MT(0), F2 = MT(0);
         if (G.half_force) {
-            const MT hr = P.half_dt * irho;
-            u = ex * c + *elem(b.F_old, n) * hr;
-            v = ey * c + *elem(b.F_old, NC + n) * hr;
-            w = ez * c + *elem(b.F_old, 2 * NC + n) * hr;
+            F0 = *elem(b.F_old, n);
+            F1 = *elem(b.F_old, NC + n);
+            F2 = *elem(b.F_old, 2 * NC + n);
         }
+        const MT hr = P.half_dt * irho;
+        const MT u = ex * P.uc + F0 * hr;
+        const MT v = ey * P.uc + F1 * hr;
+        const MT w = ez * P.uc + F2 * hr;
         // ---- Force (collision.cu:57-112) ----
         constexpr bool d3 = Lattice<Q>::dim == 3;
         MT Fx = 0, Fy = 0, Fz = 0;
@@ -640,7 +697,7 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
     }
     // ---- interpolated propagation of f and g (one set of slab weights for both) ----
     MT Wp[NPTS], Wm[NPTS], rn = MT(0);
-    if (G.flux) {
+    if constexpr (FLUX) {
 #pragma unroll
         for (int j = 0; j < NPTS; ++j) {
             Wp[j] = *elem(tb.W, (uint32_t)(j * G.ns + ii));
@@ -648,8 +705,8 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
         }
         rn = *elem(tb.rdel, (uint32_t)ii);
     }
-    transport<ST, MT, Q, NPTS, S>(G, nb, Wp, Wm, rn, b.f_dst, Pf, lds_f, ii, active);
-    if constexpr (QS > 0) transport<ST, MT, QS, NPTS, S>(G, nb, Wp, Wm, rn, b.g_dst, Pg, lds_g, ii, active);
+    transport<ST, MT, Q, NPTS, S, FLUX>(G, nb, Wp, Wm, rn, b.f_dst, Pf, lds_f, ii, active);
+    if constexpr (QS > 0) transport<ST, MT, QS, NPTS, S, FLUX>(G, nb, Wp, Wm, rn, b.g_dst, Pg, lds_g, ii, active);
 }
 
 // rho (and phi) of step n for the gradients of the next fused step; four consecutive cells per
@@ -760,13 +817,20 @@ template <typename ST, typename MT, int Q, int QS>
 void fused_launch_step(const FGeom& G, const FTables<MT>& tb, const FBufs<ST, MT>& b, const FCoef<MT>& P, int npts,
                        unsigned block, size_t lds, hipStream_t st) {
     const unsigned grid = (unsigned)G.lines;
-#define IPL_FSTEP(NP, SS) IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS>), grid, block, lds, st, G, tb, b, P)
-    if constexpr (Lattice<Q>::dim == 2) {
-        if (npts == 5) { if (G.s == 0) IPL_FSTEP(5, 0); else IPL_FSTEP(5, 2); }
-        else { if (G.s == 0) IPL_FSTEP(3, 0); else IPL_FSTEP(3, 2); }
+#define IPL_FSTEP(NP, SS, FL) IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS, FL>), grid, block, lds, st, G, tb, b, P)
+    // the stream axis and the stencil are template arguments; a uniform grid has no stencil
+    if (!G.flux) {
+        if constexpr (Lattice<Q>::dim == 2) {
+            if (G.s == 0) IPL_FSTEP(5, 0, false); else IPL_FSTEP(5, 2, false);
+        } else {
+            if (G.s == 0) IPL_FSTEP(5, 0, false); else if (G.s == 1) IPL_FSTEP(5, 1, false); else IPL_FSTEP(5, 2, false);
+        }
+    } else if constexpr (Lattice<Q>::dim == 2) {
+        if (npts == 5) { if (G.s == 0) IPL_FSTEP(5, 0, true); else IPL_FSTEP(5, 2, true); }
+        else { if (G.s == 0) IPL_FSTEP(3, 0, true); else IPL_FSTEP(3, 2, true); }
     } else {
-        if (npts == 5) { if (G.s == 0) IPL_FSTEP(5, 0); else if (G.s == 1) IPL_FSTEP(5, 1); else IPL_FSTEP(5, 2); }
-        else { if (G.s == 0) IPL_FSTEP(3, 0); else if (G.s == 1) IPL_FSTEP(3, 1); else IPL_FSTEP(3, 2); }
+        if (npts == 5) { if (G.s == 0) IPL_FSTEP(5, 0, true); else if (G.s == 1) IPL_FSTEP(5, 1, true); else IPL_FSTEP(5, 2, true); }
+        else { if (G.s == 0) IPL_FSTEP(3, 0, true); else if (G.s == 1) IPL_FSTEP(3, 1, true); else IPL_FSTEP(3, 2, true); }
     }
 #undef IPL_FSTEP
 }

@@ -200,133 +200,11 @@ __global__ void k_ib_sort(IbArgs A, const int* node, unsigned long long* sorted,
     }
 }
 
-// ---- k_ib_force: one work-item per marker ------------------------------------------------------
-// Stress method (get_IBMGw2, ibm.cu:124-274): interpolate f at the marker, split into outer (fw1)
-// and inner (fw2) populations by the normal, spread minus the traction of (fw1 - fw2), marker force
-// Gw from the traction of fw1.  Direct forcing (IB_directForcing, ibm.cu:276-344): interpolate the
-// node velocity (update_rho's u, plus dt F_ib / 3 of earlier passes), spread (U_w - u)/dt.
+// ---- k_ib_ubase: update_rho's velocity (collision.cu:244-254) at every cell of the box -------
+// (direct forcing interpolates it; computed once per cell instead of once per marker and slot)
 template <typename ST, typename MT, int Q>
-__global__ void k_ib_force(IbArgs A, int it, const ST* __restrict__ f, const ST* __restrict__ g, const MT* __restrict__ F,
-                           const int* __restrict__ node, const double* __restrict__ posw, const double* __restrict__ nB,
-                           const double* __restrict__ velw, double* __restrict__ Gw, double* __restrict__ contrib,
-                           const int* __restrict__ box, const double* __restrict__ Fbox) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= A.M) return;
-    const int base[3] = {node[3 * m], node[3 * m + 1], node[3 * m + 2]};
-    const double xw[3] = {posw[3 * m], posw[3 * m + 1], posw[3 * m + 2]};
-    const double U[3] = {velw[3 * m], velw[3 * m + 1], velw[3 * m + 2]};
-    double W[3][5];
-    axis_weights(A, base, xw, W);
-    const double c = A.c, c2 = c * c;
-    double* out = contrib + (size_t)m * A.S * 3;
-    if (A.method == IPL_IB_STRESS) {
-        double fw[Q];
-#pragma unroll
-        for (int k = 0; k < Q; ++k) fw[k] = 0.0;
-        for (int s = 0; s < A.S; ++s) {
-            int d[3], cc[3];
-            double pp[3];
-            slot_offset(A, s, d);
-            if (!node_at(A, base, d, cc, pp)) continue;
-            const double wgt = slot_weight(A, W, s);
-            const uint32_t idx = cell_of(A, cc);
-#pragma unroll
-            for (int k = 0; k < Q; ++k) fw[k] += wgt * ldpop(f, (uint32_t)k * A.ncell + idx);
-        }
-        const double nv[3] = {nB[3 * m], nB[3 * m + 1], nB[3 * m + 2]};
-        const double sina = sin(22.5 / 180.0 * 3.14159);  // the reference's literal angle
-        double fw1[Q], fw2[Q];
-#pragma unroll
-        for (int k = 0; k < Q; ++k) {
-            const Dir e = Lattice<Q>::c(k);
-            const double cn = (nv[0] * e.x + nv[1] * e.y + nv[2] * e.z) / c;
-            const double lim = sqrt((double)(e.x * e.x + e.y * e.y + e.z * e.z)) / c * sina;
-            const double bounce = fw[opposite<Q>(k)] + 6.0 * Lattice<Q>::w(k) * (e.x * U[0] + e.y * U[1] + e.z * U[2]) / c2;
-            const double beta = fabs(cn) / (lim + 1e-10);
-            const double mixed = (1.0 - beta) * fw[k] + beta * bounce;
-            fw1[k] = cn <= 0 ? fw[k] : (cn <= lim ? mixed : bounce);
-            fw2[k] = cn >= 0 ? fw[k] : (cn >= -lim ? mixed : bounce);
-        }
-        const double tau = A.tau;
-        // traction sigma . n: sigma_ab = -sum_k [g_k delta_ab / (2 tau)
-        //   + (1 - 1/(2 tau)) 3 (g_k (e_a - U_a)(e_b - U_b) - (g_k - w_k) U_a U_b) / c^2]
-        auto traction = [&](bool diff, double t[3]) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                double sg[3] = {0, 0, 0};
-#pragma unroll
-                for (int k = 0; k < Q; ++k) {
-                    const Dir e = Lattice<Q>::c(k);
-                    const double ev[3] = {(double)e.x, (double)e.y, (double)e.z};
-                    const double gk = diff ? fw1[k] - fw2[k] : fw1[k];
-#pragma unroll
-                    for (int b = 0; b < 3; ++b)
-                        sg[b] -= 0.5 / tau * gk * (a == b ? 1.0 : 0.0) +
-                                 (1.0 - 0.5 / tau) * 3 * (gk * (ev[a] - U[a]) * (ev[b] - U[b]) - (gk - Lattice<Q>::w(k)) * U[a] * U[b]) / c2;
-                }
-                t[a] = sg[0] * nv[0] + sg[1] * nv[1] + sg[2] * nv[2];
-            }
-        };
-        double gs[3], gw[3];
-        traction(true, gs);
-        for (int s = 0; s < A.S; ++s) {
-            int d[3], cc[3];
-            double pp[3];
-            slot_offset(A, s, d);
-            const bool ok = node_at(A, base, d, cc, pp);
-            const double wgt = slot_weight(A, W, s);
-            for (int a = 0; a < 3; ++a) out[3 * s + a] = ok ? -gs[a] * A.spread * wgt : 0.0;
-        }
-        traction(false, gw);
-        for (int a = 0; a < 3; ++a) Gw[3 * m + a] = gw[a] * A.gwscale;
-        return;
-    }
-    // direct forcing
-    double um[3] = {0, 0, 0};
-    for (int s = 0; s < A.S; ++s) {
-        int d[3], cc[3];
-        double pp[3];
-        slot_offset(A, s, d);
-        if (!node_at(A, base, d, cc, pp)) continue;
-        const double wgt = slot_weight(A, W, s);
-        const uint32_t idx = cell_of(A, cc);
-        // update_rho's velocity (collision.cu:244-254) at the node
-        double p = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-        for (int k = 0; k < Q; ++k) {
-            const Dir e = Lattice<Q>::c(k);
-            const double fv = ldpop(f, (uint32_t)k * A.ncell + idx);
-            p += fv; jx += e.x * c * fv; jy += e.y * c * fv; jz += e.z * c * fv;
-        }
-        double sc = 0;
-        for (int k = 0; k < A.Qs; ++k) sc += ldpop(g, (uint32_t)k * A.ncell + idx);
-        const double phi = A.scalar == IPL_SCALAR_PHASE_FIELD ? sc : 0.0, S = A.scalar == IPL_SCALAR_SALINITY ? sc : 0.0;
-        const double rho = A.rho_L + A.drho * phi + S * A.beta_s;
-        double u[3] = {jx + (double)F[idx] * A.dt / (2.0 * rho), jy + (double)F[A.ncell + idx] * A.dt / (2.0 * rho),
-                       jz + (double)F[2 * (size_t)A.ncell + idx] * A.dt / (2.0 * rho)};
-        if (it > 0) {
-            const int64_t b = box_index(A, box, cc);
-            if (b >= 0)
-                for (int a = 0; a < 3; ++a) u[a] += A.dt * Fbox[3 * b + a] / 3.0;
-        }
-        for (int a = 0; a < 3; ++a) um[a] += wgt * u[a];
-    }
-    double gwv[3];
-    for (int a = 0; a < 3; ++a) gwv[a] = (U[a] - um[a]) / A.dt;
-    for (int s = 0; s < A.S; ++s) {
-        int d[3], cc[3];
-        double pp[3];
-        slot_offset(A, s, d);
-        const bool ok = node_at(A, base, d, cc, pp);
-        const double wgt = slot_weight(A, W, s);
-        for (int a = 0; a < 3; ++a) out[3 * s + a] = ok ? gwv[a] * A.spread * wgt : 0.0;
-    }
-    for (int a = 0; a < 3; ++a) Gw[3 * m + a] = (it == 0 ? 0.0 : Gw[3 * m + a]) + -gwv[a] * A.wscale;
-}
-
-// ---- k_ib_gather: spread force of every cell of the box ----------------------------------------
-__global__ void k_ib_gather(IbArgs A, int it, const unsigned long long* __restrict__ sorted, const double* __restrict__ contrib,
-                            const int* __restrict__ box, double* __restrict__ Fbox) {
+__global__ void k_ib_ubase(IbArgs A, const ST* __restrict__ f, const ST* __restrict__ g, const MT* __restrict__ F,
+                           const int* __restrict__ box, double* __restrict__ Ubox) {
     const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t vol = (int64_t)box[3] * box[4] * box[5];
     if (b >= vol || b >= A.vmax) return;
@@ -338,8 +216,197 @@ __global__ void k_ib_gather(IbArgs A, int it, const unsigned long long* __restri
         c[a] += box[a];
         if (c[a] >= A.n[a]) c[a] -= A.n[a];
     }
+    const uint32_t idx = cell_of(A, c);
+    const double cl = A.c;
+    double jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const Dir e = Lattice<Q>::c(k);
+        const double fv = ldpop(f, (uint32_t)k * A.ncell + idx);
+        jx += e.x * cl * fv; jy += e.y * cl * fv; jz += e.z * cl * fv;
+    }
+    double sc = 0;
+    for (int k = 0; k < A.Qs; ++k) sc += ldpop(g, (uint32_t)k * A.ncell + idx);
+    const double phi = A.scalar == IPL_SCALAR_PHASE_FIELD ? sc : 0.0, S = A.scalar == IPL_SCALAR_SALINITY ? sc : 0.0;
+    const double rho = A.rho_L + A.drho * phi + S * A.beta_s;
+    Ubox[3 * b + 0] = jx + (double)F[idx] * A.dt / (2.0 * rho);
+    Ubox[3 * b + 1] = jy + (double)F[A.ncell + idx] * A.dt / (2.0 * rho);
+    Ubox[3 * b + 2] = jz + (double)F[2 * (size_t)A.ncell + idx] * A.dt / (2.0 * rho);
+}
+
+// ---- k_ib_force: one 64-lane workgroup per marker, lanes over the slots of its node block ----
+// Stress method (get_IBMGw2, ibm.cu:124-274): f interpolated at the marker, split into outer (fw1)
+// and inner (fw2) populations by the normal; minus the traction of (fw1 - fw2) spread, marker force
+// Gw from the traction of fw1.  Direct forcing (IB_directForcing, ibm.cu:276-344): node velocity
+// (plus dt F_ib / 3 of earlier passes) interpolated, (U_w - u)/dt spread.  Slot sums are reduced in
+// a fixed tree, so every run adds in the same order.
+constexpr int kIbLanes = 64;
+
+template <typename ST, typename MT, int Q>
+__global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const ST* __restrict__ f, const int* __restrict__ node,
+                                                      const double* __restrict__ posw, const double* __restrict__ nB,
+                                                      const double* __restrict__ velw, double* __restrict__ Gw,
+                                                      double* __restrict__ contrib, const int* __restrict__ box,
+                                                      const double* __restrict__ Fbox, const double* __restrict__ Ubox) {
+    __shared__ double red[kIbLanes * (Q > 3 ? Q : 3)];
+    __shared__ double share[3];
+    const int m = blockIdx.x, t = threadIdx.x;
+    const int base[3] = {node[3 * m], node[3 * m + 1], node[3 * m + 2]};
+    const double xw[3] = {posw[3 * m], posw[3 * m + 1], posw[3 * m + 2]};
+    const double U[3] = {velw[3 * m], velw[3 * m + 1], velw[3 * m + 2]};
+    double W[3][5];
+    axis_weights(A, base, xw, W);
+    const double c = A.c, c2 = c * c;
+    double* out = contrib + (size_t)m * A.S * 3;
+    const int nr = A.method == IPL_IB_STRESS ? Q : 3;
+    // 1. per-lane partial sums over the lane's slots
+    double part[Q > 3 ? Q : 3];
+#pragma unroll
+    for (int k = 0; k < (Q > 3 ? Q : 3); ++k) part[k] = 0.0;
+    for (int s = t; s < A.S; s += kIbLanes) {
+        int d[3], cc[3];
+        double pp[3];
+        slot_offset(A, s, d);
+        if (!node_at(A, base, d, cc, pp)) continue;
+        const double wgt = slot_weight(A, W, s);
+        if (A.method == IPL_IB_STRESS) {
+            const uint32_t idx = cell_of(A, cc);
+#pragma unroll
+            for (int k = 0; k < Q; ++k) part[k] += wgt * ldpop(f, (uint32_t)k * A.ncell + idx);
+        } else {
+            const int64_t b = box_index(A, box, cc);
+            if (b < 0) continue;  // cannot happen: the box holds every node block
+            for (int a = 0; a < 3; ++a) {
+                double u = Ubox[3 * b + a];
+                if (it > 0) u += A.dt * Fbox[3 * b + a] / 3.0;
+                part[a] += wgt * u;
+            }
+        }
+    }
+    for (int k = 0; k < nr; ++k) red[k * kIbLanes + t] = part[k];
+    __syncthreads();
+    for (int w = kIbLanes / 2; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < nr; ++k) red[k * kIbLanes + t] += red[k * kIbLanes + t + w];
+        __syncthreads();
+    }
+    // 2. the marker force (lane 0)
+    if (t == 0) {
+        if (A.method == IPL_IB_STRESS) {
+            double fw[Q], fw1[Q], fw2[Q];
+#pragma unroll
+            for (int k = 0; k < Q; ++k) fw[k] = red[k * kIbLanes];
+            const double nv[3] = {nB[3 * m], nB[3 * m + 1], nB[3 * m + 2]};
+            const double sina = sin(22.5 / 180.0 * 3.14159);  // the reference's literal angle
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                const Dir e = Lattice<Q>::c(k);
+                const double cn = (nv[0] * e.x + nv[1] * e.y + nv[2] * e.z) / c;
+                const double lim = sqrt((double)(e.x * e.x + e.y * e.y + e.z * e.z)) / c * sina;
+                const double bounce = fw[opposite<Q>(k)] + 6.0 * Lattice<Q>::w(k) * (e.x * U[0] + e.y * U[1] + e.z * U[2]) / c2;
+                const double beta = fabs(cn) / (lim + 1e-10);
+                const double mixed = (1.0 - beta) * fw[k] + beta * bounce;
+                fw1[k] = cn <= 0 ? fw[k] : (cn <= lim ? mixed : bounce);
+                fw2[k] = cn >= 0 ? fw[k] : (cn >= -lim ? mixed : bounce);
+            }
+            const double tau = A.tau;
+            // traction sigma . n: sigma_ab = -sum_k [g_k delta_ab / (2 tau)
+            //   + (1 - 1/(2 tau)) 3 (g_k (e_a - U_a)(e_b - U_b) - (g_k - w_k) U_a U_b) / c^2]
+            auto traction = [&](bool diff, double tr[3]) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    double sg[3] = {0, 0, 0};
+#pragma unroll
+                    for (int k = 0; k < Q; ++k) {
+                        const Dir e = Lattice<Q>::c(k);
+                        const double ev[3] = {(double)e.x, (double)e.y, (double)e.z};
+                        const double gk = diff ? fw1[k] - fw2[k] : fw1[k];
+#pragma unroll
+                        for (int b2 = 0; b2 < 3; ++b2)
+                            sg[b2] -= 0.5 / tau * gk * (a == b2 ? 1.0 : 0.0) +
+                                      (1.0 - 0.5 / tau) * 3 * (gk * (ev[a] - U[a]) * (ev[b2] - U[b2]) - (gk - Lattice<Q>::w(k)) * U[a] * U[b2]) / c2;
+                    }
+                    tr[a] = sg[0] * nv[0] + sg[1] * nv[1] + sg[2] * nv[2];
+                }
+            };
+            double gs[3], gw[3];
+            traction(true, gs);
+            traction(false, gw);
+            for (int a = 0; a < 3; ++a) {
+                share[a] = -gs[a];
+                Gw[3 * m + a] = gw[a] * A.gwscale;
+            }
+        } else {
+            for (int a = 0; a < 3; ++a) {
+                const double gwv = (U[a] - red[a * kIbLanes]) / A.dt;
+                share[a] = gwv;
+                Gw[3 * m + a] = (it == 0 ? 0.0 : Gw[3 * m + a]) + -gwv * A.wscale;
+            }
+        }
+    }
+    __syncthreads();
+    // 3. contribution of every slot to the spread force
+    const double g0 = share[0], g1 = share[1], g2 = share[2];
+    for (int s = t; s < A.S; s += kIbLanes) {
+        int d[3], cc[3];
+        double pp[3];
+        slot_offset(A, s, d);
+        const bool ok = node_at(A, base, d, cc, pp);
+        const double wgt = slot_weight(A, W, s);
+        out[3 * s + 0] = ok ? g0 * A.spread * wgt : 0.0;
+        out[3 * s + 1] = ok ? g1 * A.spread * wgt : 0.0;
+        out[3 * s + 2] = ok ? g2 * A.spread * wgt : 0.0;
+    }
+}
+
+// ---- k_ib_ranges: for every cell of the box, the run of sorted markers whose nearest cell it is
+__global__ void k_ib_ranges(IbArgs A, const unsigned long long* __restrict__ sorted, const int* __restrict__ box,
+                            int2* __restrict__ range) {
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t vol = (int64_t)box[3] * box[4] * box[5];
+    if (b >= vol || b >= A.vmax) return;
+    int c[3];
+    c[2] = (int)(b % box[5]);
+    c[1] = (int)((b / box[5]) % box[4]);
+    c[0] = (int)(b / ((int64_t)box[4] * box[5]));
+    for (int a = 0; a < 3; ++a) {
+        c[a] += box[a];
+        if (c[a] >= A.n[a]) c[a] -= A.n[a];
+    }
+    const unsigned long long key = (unsigned long long)cell_of(A, c) << 32;
+    int lo = 0, hi = A.M;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (sorted[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    int e = lo;
+    while (e < A.M && (sorted[e] >> 32) == (key >> 32)) ++e;
+    range[b] = make_int2(lo, e - lo);
+}
+
+// ---- k_ib_gather: spread force of every cell of the box ----------------------------------------
+// One workgroup per cell, one lane per block slot: cell c receives slot s of every marker whose
+// nearest cell is c - d(s) (markers by index), and the slot sums are added in a fixed tree.
+constexpr int kIbGatherLanes = 128;  // >= the slots of a block (125 for the 5-node support)
+
+__global__ void __launch_bounds__(kIbGatherLanes) k_ib_gather(IbArgs A, int it, const unsigned long long* __restrict__ sorted,
+                                                               const int2* __restrict__ range, const double* __restrict__ contrib,
+                                                               const int* __restrict__ box, double* __restrict__ Fbox) {
+    __shared__ double red[3][kIbGatherLanes];
+    const int64_t b = blockIdx.x;
+    const int s = threadIdx.x;
+    const int64_t vol = (int64_t)box[3] * box[4] * box[5];
+    if (b >= vol || b >= A.vmax) return;  // whole workgroup: b is uniform
+    int c[3];
+    c[2] = (int)(b % box[5]);
+    c[1] = (int)((b / box[5]) % box[4]);
+    c[0] = (int)(b / ((int64_t)box[4] * box[5]));
+    for (int a = 0; a < 3; ++a) {
+        c[a] += box[a];
+        if (c[a] >= A.n[a]) c[a] -= A.n[a];
+    }
     double F[3] = {0, 0, 0};
-    for (int s = 0; s < A.S; ++s) {
+    if (s < A.S) {
         int d[3], src[3];
         slot_offset(A, s, d);
         bool ok = true;
@@ -350,21 +417,25 @@ __global__ void k_ib_gather(IbArgs A, int it, const unsigned long long* __restri
                 else ok = false;
             }
         }
-        if (!ok) continue;
-        const unsigned long long key = (unsigned long long)cell_of(A, src) << 32;
-        // first sorted entry >= key
-        int lo = 0, hi = A.M;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (sorted[mid] < key) lo = mid + 1; else hi = mid;
-        }
-        for (int r = lo; r < A.M && (sorted[r] >> 32) == (key >> 32); ++r) {
-            const int m = (int)(sorted[r] & 0xffffffffull);
-            const double* cm = contrib + ((size_t)m * A.S + s) * 3;
-            F[0] += cm[0]; F[1] += cm[1]; F[2] += cm[2];
+        const int64_t bs = ok ? box_index(A, box, src) : -1;  // no marker is nearest to a cell outside the box
+        if (bs >= 0) {
+            const int2 r = range[bs];
+            for (int q = r.x; q < r.x + r.y; ++q) {
+                const int m = (int)(sorted[q] & 0xffffffffull);
+                const double* cm = contrib + ((size_t)m * A.S + s) * 3;
+                F[0] += cm[0]; F[1] += cm[1]; F[2] += cm[2];
+            }
         }
     }
-    for (int a = 0; a < 3; ++a) Fbox[3 * b + a] = (it == 0 ? 0.0 : Fbox[3 * b + a]) + F[a];
+    for (int a = 0; a < 3; ++a) red[a][s] = F[a];
+    __syncthreads();
+    for (int w = kIbGatherLanes / 2; w > 0; w >>= 1) {
+        if (s < w)
+            for (int a = 0; a < 3; ++a) red[a][s] += red[a][s + w];
+        __syncthreads();
+    }
+    if (s == 0)
+        for (int a = 0; a < 3; ++a) Fbox[3 * b + a] = (it == 0 ? 0.0 : Fbox[3 * b + a]) + red[a][0];
 }
 
 // ---- k_ib_apply: update_velIBM (ibm.cu:346-365) on the box ------------------------------------
@@ -547,7 +618,7 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
     reach_lo_ = pk4 ? -2 : -1;
     reach_hi_ = pk4 ? 2 : 1;
     const int np = reach_hi_ - reach_lo_ + 1;
-    S_ = np * np * (g.dim == 3 ? np : 1);
+    S_ = np * np * (g.dim == 3 ? np : 1);  // at most 125 <= the gather's lanes
     Status r;
     // axis coordinates; entry n holds the period (sum of widths) for the seam unwrap
     for (int a = 0; a < 3; ++a) {
@@ -612,6 +683,8 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
     if (!(r = mem_.alloc_n(&node_, 3 * M_))) return r;
     if (!(r = mem_.alloc_n(&contrib_, (size_t)3 * M_ * S_))) return r;
     if (!(r = mem_.alloc_n(&Fbox_, (size_t)3 * vmax_))) return r;
+    if (!(r = mem_.alloc_n(&Ubox_, (size_t)3 * vmax_))) return r;
+    if (!(r = mem_.alloc_n(&range_, (size_t)vmax_))) return r;
     if (!(r = mem_.alloc_n(&body_, (size_t)ibk::B_SIZE))) return r;
     if (!(r = mem_.alloc_n(&box_, 8))) return r;
     if (!(r = mem_.alloc_n(&sorted_, (size_t)M_))) return r;
@@ -658,13 +731,20 @@ ibk::IbArgs make_args(const IbGrid& g, const ipl_ib_config& cfg, int M, int S, i
 }
 
 template <typename ST, typename MT>
-void force_q(int Q, const ibk::IbArgs& A, int it, const ST* f, const ST* g, const MT* F, const int* node, const double* posw,
-             const double* nB, const double* velw, double* Gw, double* contrib, const int* box, const double* Fbox,
+void force_q(int Q, const ibk::IbArgs& A, int it, const ST* f, const int* node, const double* posw, const double* nB,
+             const double* velw, double* Gw, double* contrib, const int* box, const double* Fbox, const double* Ubox,
              hipStream_t st) {
-    const unsigned blocks = (unsigned)((A.M + 63) / 64);
-#define IPL_IBF(QQ) IPL_LAUNCH((ibk::k_ib_force<ST, MT, QQ>), blocks, 64, 0, st, A, it, f, g, F, node, posw, nB, velw, Gw, contrib, box, Fbox)
+#define IPL_IBF(QQ) IPL_LAUNCH_SYNC((ibk::k_ib_force<ST, MT, QQ>), A.M, ibk::kIbLanes, 0, st, A, it, f, node, posw, nB, velw, Gw, contrib, box, Fbox, Ubox)
     if (Q == 9) IPL_IBF(9); else if (Q == 19) IPL_IBF(19); else IPL_IBF(27);
 #undef IPL_IBF
+}
+
+template <typename ST, typename MT>
+void ubase_q(int Q, const ibk::IbArgs& A, const ST* f, const ST* g, const MT* F, const int* box, double* Ubox, hipStream_t st) {
+    const unsigned blocks = (unsigned)((A.vmax + 255) / 256);
+    if (Q == 9) IPL_LAUNCH((ibk::k_ib_ubase<ST, MT, 9>), blocks, 256, 0, st, A, f, g, F, box, Ubox);
+    else if (Q == 19) IPL_LAUNCH((ibk::k_ib_ubase<ST, MT, 19>), blocks, 256, 0, st, A, f, g, F, box, Ubox);
+    else IPL_LAUNCH((ibk::k_ib_ubase<ST, MT, 27>), blocks, 256, 0, st, A, f, g, F, box, Ubox);
 }
 
 template <typename ST>
@@ -685,14 +765,17 @@ Status IbSolver::substep(void* f, const void* g, const void* F, int storage) {
     IPL_LAUNCH_SYNC(ibk::k_ib_sort, 1, 1024, (size_t)mpad * sizeof(unsigned long long), st_, A, node_, sorted_, box_, mpad);
     const int iters = cfg_.method == IPL_IB_DIRECT ? std::max(1, cfg_.iterations) : 1;
     const unsigned gblocks = (unsigned)((vmax_ + 255) / 256);
+    if (cfg_.method == IPL_IB_DIRECT) {
+        if (storage == IPL_F64) ubase_q<double, double>(g_.Q, A, (const double*)f, (const double*)g, (const double*)F, box_, Ubox_, st_);
+        else ubase_q<float, float>(g_.Q, A, (const float*)f, (const float*)g, (const float*)F, box_, Ubox_, st_);
+    }
+    IPL_LAUNCH(ibk::k_ib_ranges, gblocks, 256, 0, st_, A, sorted_, box_, range_);
     for (int it = 0; it < iters; ++it) {
         if (storage == IPL_F64)
-            force_q<double, double>(g_.Q, A, it, (const double*)f, (const double*)g, (const double*)F, node_, posw_, nB_, velw_, Gw_,
-                                    contrib_, box_, Fbox_, st_);
+            force_q<double, double>(g_.Q, A, it, (const double*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box_, Fbox_, Ubox_, st_);
         else
-            force_q<float, float>(g_.Q, A, it, (const float*)f, (const float*)g, (const float*)F, node_, posw_, nB_, velw_, Gw_,
-                                  contrib_, box_, Fbox_, st_);
-        IPL_LAUNCH(ibk::k_ib_gather, gblocks, 256, 0, st_, A, it, sorted_, contrib_, box_, Fbox_);
+            force_q<float, float>(g_.Q, A, it, (const float*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box_, Fbox_, Ubox_, st_);
+        IPL_LAUNCH_SYNC(ibk::k_ib_gather, (unsigned)vmax_, ibk::kIbGatherLanes, 0, st_, A, it, sorted_, range_, contrib_, box_, Fbox_);
     }
     if (storage == IPL_F64) apply_q<double>(g_.Q, A, (double*)f, box_, Fbox_, st_);
     else apply_q<float>(g_.Q, A, (float*)f, box_, Fbox_, st_);

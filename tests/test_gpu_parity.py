@@ -148,6 +148,15 @@ def test_fused_uniform_3d_no_flux_fp64(oracle_lib):
     _compare(case, 100, ("p", "vel"), 1e-12, oracle_lib)
 
 
+@pytest.mark.parametrize("bc", [_abi.BC_SLIP, _abi.BC_BOUNCE_BACK])
+def test_fused_uniform_3d_walls_everywhere_fp64(oracle_lib, bc):
+    """Uniform grid (exact streaming), walls on all six faces: the wall redirects of the paired
+    pushes at both ends of every line and on the boundary lines."""
+    case = cases.lock_exchange(nx=10, ny=6, nz=9, path=PATH_FUSED).replace(periodic=(0, 0, 0), flux_axes=_abi.FLUX_NONE,
+                                                                           wall_bc=bc)
+    _compare(case, 120, ("p", "vel", "S"), 1e-12, oracle_lib)
+
+
 def test_fused_solitary_wave_fp64(oracle_lib):
     case = cases.solitary_wave(nx=96, nz=48, x0=30, sigma=1e-4, path=PATH_FUSED)
     _compare(case, 150, ("p", "vel", "phi"), 1e-12, oracle_lib)
