@@ -58,6 +58,42 @@ def measure(case, steps, warmup, profile_steps=0):
             "path": path, "profile": prof, "sums": sums.tolist()}
 
 
+def measure_sphere(steps, warmup):
+    """The falling-sphere workload (192 x 192 x 384 D3Q19 FP32, 1800 Fibonacci markers, multi-
+    direct forcing with 3 passes and the 4-point Peskin kernel, FP64 body): MLUPS of the whole
+    step, the fused kernel's time and bandwidth (2 x 19 x 4 B per cell), the IB sub-step's share."""
+    from ip_cu_lbm_amd import Simulation, cases
+    case = cases.falling_sphere()
+    with Simulation(case) as sim:
+        sim.step(warmup)
+        ms = sim.step_timed(steps)
+        prof = sim.profile_steps(min(steps, 50))
+        body = sim.body_state()
+        n = sim.n_calc
+    step_us = ms / steps * 1e3
+    kern_us = prof[0] * 1e3
+    return {"workload": "falling_sphere_192x192x384_d3q19_ibm_mdf3_peskin4", "cells": n,
+            "markers": len(case.body.markers), "mlups": round(n * steps / (ms * 1e3), 1),
+            "step_us": round(step_us, 1), "kernel_us": round(kern_us, 1),
+            "kernel_gbps": round(2 * 19 * 4 * n / (kern_us * 1e-6) / 1e9, 1),
+            "ib_us": round(step_us - kern_us, 1), "steps": warmup + steps,
+            "body_vel_z": float(body["vel"][2]), "body_force_z": float(body["force"][2])}
+
+
+def measure_serving(n_jobs=256):
+    """The serving mix: 256 seeded Poisson arrivals (mean gap 20 ms) of D2Q9/D3Q19 FP32 jobs of
+    mixed sizes (2D sides 64-512, 3D sides 32-128, 500-5000 steps, ~30 % with salinity, ~10 % with
+    an immersed body), admitted under a 200 GB device budget and advanced by continuous batching
+    in chunks of 200 steps.  Latency is completion - arrival; aggregate MLUPS is all jobs' cell
+    updates over the makespan (first arrival to last completion), busy MLUPS over the time spent
+    in batch steps."""
+    from ip_cu_lbm_amd import serve
+    jobs = serve.serving_mix(n_jobs=n_jobs)
+    res = serve.Server(budget_bytes=200e9, chunk=200).run(jobs)
+    res["workload"] = f"serving_mix_{n_jobs}_jobs_fp32"
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res.items()}
+
+
 def cpu_baseline(case, seconds):
     import oracle
     oracle.build()
@@ -157,6 +193,13 @@ def main(argv=None):
         log(f"[bench] fused FP64: {fused64['mlups']:.0f} MLUPS")
         plain32 = measure(cases.lock_exchange(storage=F32, path=PATH_PLAIN), k, 5)
         log(f"[bench] plain FP32: {plain32['mlups']:.0f} MLUPS")
+        sph = measure_sphere(max(100, a.steps // 10), 20)
+        log(f"[bench] falling sphere: {sph['mlups']:.0f} MLUPS, fused kernel {sph['kernel_us']:.1f} us, "
+            f"IB sub-step {sph['ib_us']:.1f} us per step")
+        srv = measure_serving()
+        srv["aggregate_vs_single_large_case"] = round(srv["aggregate_mlups"] / head["mlups"], 3)
+        log(f"[bench] serving mix: {srv['jobs']} jobs, p50 {srv['p50_s']:.2f} s, p99 {srv['p99_s']:.2f} s, "
+            f"aggregate {srv['aggregate_mlups']:.0f} MLUPS (busy {srv['busy_mlups']:.0f})")
         extra = {
             "fused_fp32_mlups": round(head["mlups"], 1),
             "fused_fp64_mlups": round(fused64["mlups"], 1),
@@ -165,6 +208,8 @@ def main(argv=None):
             "speedup_fused_fp32_vs_plain_fp64": round(head["mlups"] / plain64["mlups"], 2),
             "fused_fp64_kernel_gbps": round(2 * 26 * 8 * cells / (fused64["profile"][0] * 1e-3) / 1e9, 1),
             "fp16_storage": "disabled (see DESIGN.md)",
+            "falling_sphere": sph,
+            "serving": srv,
         }
 
     cpu = None

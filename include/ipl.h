@@ -217,6 +217,28 @@ int ipl_ib_body_state(ipl_ctx* ctx, double out[19]);
 enum ipl_ib_array { IPL_IB_POS = 0, IPL_IB_VEL = 1, IPL_IB_FORCE = 2, IPL_IB_NORMAL = 3, IPL_IB_NODE = 4 };
 int ipl_ib_read_markers(ipl_ctx* ctx, int32_t which, double* out, size_t count);
 
+/* ---- batches: many independent simulations advanced together -----------------------------
+ * A batch holds contexts (jobs) of any sizes and configurations.  ipl_batch_step launches the
+ * steps of every member on its own stream (the fused path replays a captured HIP graph of its
+ * step), so the jobs' work overlaps on the device, then waits for all of them.  Members keep
+ * their own buffers: a job's results are the same alone and in any mix.  The reference runs one
+ * case per process and has no counterpart; this is the north star's batching / serving layer. */
+typedef struct ipl_batch ipl_batch;
+int ipl_batch_create(ipl_batch** out);
+int ipl_batch_add(ipl_batch* batch, ipl_ctx* ctx);
+int ipl_batch_remove(ipl_batch* batch, ipl_ctx* ctx);
+/* Advances member i by nsteps[i] steps (members in the order they were added), or every member by
+ * nsteps[0] when all == 1. */
+int ipl_batch_step(ipl_batch* batch, const int32_t* nsteps, int32_t all);
+int ipl_batch_destroy(ipl_batch* batch);       /* the members stay valid */
+const char* ipl_batch_last_error_message(ipl_batch* batch);
+
+/* Device bytes ipl_create would allocate for cfg (population, force and field buffers, tables and
+ * staging), for admission against a memory budget; no device work. */
+int ipl_estimate_device_bytes(const ipl_config* cfg, int64_t* bytes);
+/* 0: launch every step directly; 1 (default): replay captured graphs where the path supports it. */
+int ipl_set_graph(ipl_ctx* ctx, int32_t on);
+
 /* ---- host-only diagnostics ------------------------------------------------------------------
  * Builds the cell graph of cfg on the host (no device work) and copies one table out:
  * which = 0 neighbour table [k][n_calc + n_ghost] (ghosts after calc cells, basic.cu:203-237),

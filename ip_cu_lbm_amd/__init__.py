@@ -23,8 +23,8 @@ from ._abi import (BC_BOUNCE_BACK, BC_SLIP, COLL_MRT, COLL_SRT, F16, F32, F64, F
                    FLUX_REFERENCE, FLUX_X, FLUX_Y, FLUX_Z, PATH_AUTO, PATH_FUSED, PATH_PLAIN,
                    SCALAR_NONE, SCALAR_PHASE_FIELD, SCALAR_SALINITY, STAGES, IplError)
 
-__all__ = ["Simulation", "cases", "IplError", "F64", "F32", "F16", "PATH_AUTO", "PATH_PLAIN",
-           "PATH_FUSED"]
+__all__ = ["Simulation", "Batch", "estimate_device_bytes", "cases", "IplError", "F64", "F32", "F16", "PATH_AUTO",
+           "PATH_PLAIN", "PATH_FUSED"]
 
 
 def _dp(a):
@@ -77,6 +77,10 @@ class Simulation:
     # ---- run loop ---------------------------------------------------------------------------
     def step(self, n: int = 1):
         self._check(self._lib.ipl_step(self._h, int(n)))
+
+    def set_graph(self, on: bool):
+        """Replay captured HIP graphs of the step (default on) or launch every step directly."""
+        self._check(self._lib.ipl_set_graph(self._h, 1 if on else 0))
 
     def step_timed(self, n: int) -> float:
         """Runs n steps and returns their device time in milliseconds (HIP events)."""
@@ -138,3 +142,70 @@ class Simulation:
         out = np.empty(m if which == "node" else 3 * m)
         self._check(self._lib.ipl_ib_read_markers(self._h, _abi.IB_ARRAYS[which], _dp(out), out.size))
         return out if which == "node" else out.reshape(m, 3)
+
+
+class Batch:
+    """Independent simulations advanced together (ipl_batch_*): every member's steps are launched
+    on its own stream before the batch waits for all of them."""
+
+    def __init__(self, library: str = None):
+        self._lib = _abi.load_library(library)
+        h = ctypes.c_void_p()
+        rc = self._lib.ipl_batch_create(ctypes.byref(h))
+        if rc != 0:
+            raise IplError(rc, "ipl_batch_create")
+        self._h = h
+        self.members = []
+
+    def add(self, sim: Simulation):
+        rc = self._lib.ipl_batch_add(self._h, sim._h)
+        if rc != 0:
+            raise IplError(rc, "ipl_batch_add")
+        self.members.append(sim)
+
+    def remove(self, sim: Simulation):
+        rc = self._lib.ipl_batch_remove(self._h, sim._h)
+        if rc != 0:
+            raise IplError(rc, "ipl_batch_remove")
+        self.members.remove(sim)
+
+    def step(self, counts):
+        """Advances member i by counts[i] steps (an int: every member by that many)."""
+        if isinstance(counts, int):
+            arr = (ctypes.c_int32 * 1)(counts)
+            rc = self._lib.ipl_batch_step(self._h, arr, 1)
+        else:
+            arr = (ctypes.c_int32 * len(counts))(*[int(c) for c in counts])
+            rc = self._lib.ipl_batch_step(self._h, arr, 0)
+        if rc != 0:
+            msg = self._lib.ipl_batch_last_error_message(self._h).decode()
+            raise IplError(rc, f"{self._lib.ipl_error_string(rc).decode()}: {msg}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ipl_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def estimate_device_bytes(case: "cases.Case", library: str = None) -> int:
+    """Device bytes ipl_create would allocate for the case (admission against a budget)."""
+    lib = _abi.load_library(library)
+    cfg, _ini, keep = case.to_c()
+    out = ctypes.c_int64()
+    rc = lib.ipl_estimate_device_bytes(ctypes.byref(cfg), ctypes.byref(out))
+    del keep
+    if rc != 0:
+        raise IplError(rc, "ipl_estimate_device_bytes")
+    return int(out.value)

@@ -126,6 +126,22 @@ def load_library(path: str = None):
     lib.ipl_ib_body_state.restype = c_int32
     lib.ipl_ib_read_markers.argtypes = [vp, c_int32, _DP, c_size_t]
     lib.ipl_ib_read_markers.restype = c_int32
+    lib.ipl_batch_create.argtypes = [POINTER(vp)]
+    lib.ipl_batch_create.restype = c_int32
+    lib.ipl_batch_add.argtypes = [vp, vp]
+    lib.ipl_batch_add.restype = c_int32
+    lib.ipl_batch_remove.argtypes = [vp, vp]
+    lib.ipl_batch_remove.restype = c_int32
+    lib.ipl_batch_step.argtypes = [vp, POINTER(c_int32), c_int32]
+    lib.ipl_batch_step.restype = c_int32
+    lib.ipl_batch_destroy.argtypes = [vp]
+    lib.ipl_batch_destroy.restype = c_int32
+    lib.ipl_batch_last_error_message.argtypes = [vp]
+    lib.ipl_batch_last_error_message.restype = c_char_p
+    lib.ipl_estimate_device_bytes.argtypes = [POINTER(IplConfig), POINTER(c_int64)]
+    lib.ipl_estimate_device_bytes.restype = c_int32
+    lib.ipl_set_graph.argtypes = [vp, c_int32]
+    lib.ipl_set_graph.restype = c_int32
     _LIBS[path] = lib
     return lib
 

@@ -274,6 +274,28 @@ def falling_sphere(n=(192, 192, 384), diameter: float = 24.0, markers: int = 180
                 storage=storage, path=path, steps=steps, body=body)
 
 
+def cylinder_2d(n=(256, 256), diameter: float = None, markers: int = 400, density_ratio: float = 3.0,
+                tau: float = 0.7, gravity: float = 5e-5, method: int = _abi.IB_STRESS,
+                storage: int = _abi.F32, path: int = _abi.PATH_AUTO, steps: int = 2000) -> Case:
+    """A falling cylinder in the x-z plane (D2Q9 MRT, uniform box, bounce-back walls): markers on a
+    circle, body mass and moment of inertia per unit depth (rho_B pi R^2, m R^2 / 2), the stress
+    method of the reference's 2D cylinder drivers by default."""
+    nx, nz = n
+    pos, wid = zip(*[_uniform_axis(k) for k in (nx, 1, nz)])
+    d = diameter if diameter is not None else max(6.0, min(nx, nz) / 5.0)
+    R = d / 2.0
+    th = 2.0 * np.pi * (np.arange(markers) + 0.5) / markers
+    unit = np.stack([np.cos(th), np.zeros_like(th), np.sin(th)], axis=1)
+    m = density_ratio * np.pi * R * R
+    body = Body(markers=unit * R, normals=unit.copy(), spacing=2.0 * np.pi * R / markers, mass=m,
+                density=density_ratio, inertia=(0.5 * m * R * R,) * 3, pos=(nx / 2.0, 0.5, nz / 2.0),
+                gravity=(0.0, 0.0, -gravity), method=method)
+    return Case(name=f"cylinder_{nx}x{nz}", lattice=9, n=(nx, 1, nz), pos=list(pos), width=list(wid),
+                periodic=(0, 0, 0), wall_bc=_abi.BC_BOUNCE_BACK, flux_axes=_abi.FLUX_NONE, stencil=5,
+                s_rates=np.array([1.0, 1.4, 1.4, 1.0, 1.2, 1.0, 1.2, 1 / tau, 1 / tau]), tau=tau,
+                storage=storage, path=path, steps=steps, body=body)
+
+
 def _uniform_axis(n: int, dz: float = 1.0):
     return axis_from_widths(np.full(n, dz))
 

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "engine.hpp"
 
@@ -197,6 +198,101 @@ int ipl_ib_read_markers(ipl_ctx* ctx, int32_t which, double* out, size_t count) 
     if (!ctx || !out) return IPL_ERR_ARGUMENT;
     ipl::Status s = ctx->engine->ib_read_markers(which, out, count);
     return s ? IPL_OK : fail(ctx, s);
+}
+
+}  // extern "C"
+
+struct ipl_batch {
+    std::vector<ipl_ctx*> members;
+    std::string last_error;
+};
+
+extern "C" {
+
+int ipl_batch_create(ipl_batch** out) {
+    if (!out) return IPL_ERR_ARGUMENT;
+    *out = new (std::nothrow) ipl_batch();
+    return *out ? IPL_OK : IPL_ERR_OUT_OF_MEMORY;
+}
+
+int ipl_batch_add(ipl_batch* b, ipl_ctx* ctx) {
+    if (!b || !ctx) return IPL_ERR_ARGUMENT;
+    for (ipl_ctx* m : b->members)
+        if (m == ctx) return IPL_ERR_ARGUMENT;
+    b->members.push_back(ctx);
+    return IPL_OK;
+}
+
+int ipl_batch_remove(ipl_batch* b, ipl_ctx* ctx) {
+    if (!b || !ctx) return IPL_ERR_ARGUMENT;
+    for (size_t i = 0; i < b->members.size(); ++i)
+        if (b->members[i] == ctx) {
+            b->members.erase(b->members.begin() + (long)i);
+            return IPL_OK;
+        }
+    return IPL_ERR_ARGUMENT;
+}
+
+int ipl_batch_step(ipl_batch* b, const int32_t* nsteps, int32_t all) {
+    if (!b || !nsteps) return IPL_ERR_ARGUMENT;
+    const size_t n = b->members.size();
+    // launch every member first (each on its own stream), then wait for all of them
+    for (size_t i = 0; i < n; ++i) {
+        const int k = all ? nsteps[0] : nsteps[i];
+        if (k < 0) return IPL_ERR_ARGUMENT;
+        if (k == 0) continue;
+        ipl::Status s = b->members[i]->engine->launch_steps(k);
+        if (!s) { b->last_error = s.msg; return s.code; }
+    }
+    for (size_t i = 0; i < n; ++i) {
+        ipl_ctx* c = b->members[i];
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) { b->last_error = hipGetErrorString(e); return IPL_ERR_HIP; }
+        ipl::Status s = c->engine->post_step();
+        if (!s) { b->last_error = s.msg; return s.code; }
+    }
+    return IPL_OK;
+}
+
+int ipl_batch_destroy(ipl_batch* b) {
+    delete b;
+    return IPL_OK;
+}
+
+const char* ipl_batch_last_error_message(ipl_batch* b) { return b ? b->last_error.c_str() : ""; }
+
+int ipl_set_graph(ipl_ctx* ctx, int32_t on) {
+    if (!ctx) return IPL_ERR_ARGUMENT;
+    ctx->engine->set_graph(on != 0);
+    return IPL_OK;
+}
+
+int ipl_estimate_device_bytes(const ipl_config* cfg, int64_t* bytes) {
+    if (!cfg || !bytes) return IPL_ERR_ARGUMENT;
+    int64_t n = (int64_t)cfg->nx * cfg->ny * cfg->nz;
+    if (cfg->solid) {
+        int64_t solid = 0;
+        for (int64_t i = 0; i < n; ++i) solid += cfg->solid[i] != 0;
+        n -= solid;
+    }
+    const int Q = cfg->lattice;
+    const int Qs = cfg->scalar == IPL_SCALAR_NONE ? 0 : (cfg->scalar_lattice ? cfg->scalar_lattice : Q);
+    const int64_t s = cfg->storage == IPL_F64 ? 8 : (cfg->storage == IPL_F32 ? 4 : 2);
+    const int64_t m = cfg->storage == IPL_F64 ? 8 : 4;
+    const int64_t per_axis = 4 * (int64_t)(cfg->nx + cfg->ny + cfg->nz) * 8;
+    if (cfg->path != IPL_PATH_PLAIN) {
+        // fused: ping-pong populations, force rows, scalar-field rows, phase-field history,
+        // FP64 cell volumes and the FP64 readout staging rows
+        int64_t b = 2 * (int64_t)(Q + Qs) * s * n + 6 * m * n + 8 * n + 7 * 8 * n;
+        if (cfg->scalar != IPL_SCALAR_NONE) b += 2 * m * n;
+        if (cfg->scalar == IPL_SCALAR_PHASE_FIELD) b += 4 * m * n;
+        *bytes = b + per_axis + (1 << 20);
+    } else {
+        // plain: five population sets per field, neighbour table, ghosts (~n^(2/3) of n), fields
+        const int64_t nt = n + 6 * (int64_t)std::cbrt((double)n) * (int64_t)std::cbrt((double)n);
+        *bytes = 5 * (int64_t)(Q + Qs) * s * nt + 4 * (int64_t)Q * nt + 20 * m * nt + per_axis + (1 << 20);
+    }
+    return IPL_OK;
 }
 
 int ipl_build_tables(const ipl_config* cfg, int32_t which, void* out, int64_t* count) {

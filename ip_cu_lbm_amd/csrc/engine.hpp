@@ -67,6 +67,12 @@ class Engine {
 public:
     virtual ~Engine() = default;
     virtual Status step(int nsteps) = 0;
+    // step() without the final host check: launches only (a batch launches every member first)
+    virtual Status launch_steps(int nsteps) { return step(nsteps); }
+    // host-side check after the launched steps completed (the IB box bound)
+    virtual Status post_step() { return Status::ok(); }
+    // replay steps from a captured HIP graph where the engine supports it
+    virtual void set_graph(bool) {}
     virtual Status stage(int s) = 0;
     virtual Status read(int field, double* host, size_t count) = 0;
     virtual Status write(int field, const double* host, size_t count) = 0;

@@ -148,6 +148,8 @@ public:
                 IPL_HIP_TRY(hipStreamSynchronize(st));
             }
         }
+        // the readout buffer (7 FP64 rows) doubles as the staging row of the population upload
+        if (!(r = mem_.alloc_n(&out_, (size_t)7 * NC))) return r;
         if (!(r = upload_pops(hs.f, Q, gr_.Ntot, fa_))) return r;
         if (Qs && !(r = upload_pops(hs.g, Qs, gr_.Ntot, ga_))) return r;
         // tables
@@ -204,7 +206,6 @@ public:
             vol[perm_[id]] = cfg_widths_[0][x] * cfg_widths_[1][y] * cfg_widths_[2][z] / (gr_.dz * gr_.dz * gr_.dz);
         }
         if (!(r = mem_.alloc_n(&vol_, (size_t)NC))) return r;
-        if (!(r = mem_.alloc_n(&out_, (size_t)7 * NC))) return r;
         if (!(r = mem_.alloc_n(&partial_, (size_t)3 * 1024 + 3))) return r;
         IPL_HIP_TRY(hipMemcpyAsync(vol_, vol.data(), NC * sizeof(double), hipMemcpyHostToDevice, st));
         IPL_HIP_TRY(hipStreamSynchronize(st));
@@ -222,8 +223,7 @@ public:
 
     Status upload_pops(const std::vector<double>& src, int q, int64_t Nt, ST* dst) {
         const int64_t NC = geo_.ncell, N = gr_.N;
-        double* tmp;
-        IPL_HIP_TRY(hipMalloc(&tmp, NC * sizeof(double)));
+        double* tmp = out_;
         std::vector<double> h(NC);
         for (int k = 0; k < q; ++k) {
             for (int64_t id = 0; id < N; ++id) h[perm_[id]] = src[(size_t)k * Nt + id];
@@ -231,7 +231,6 @@ public:
             convert_row<ST, MT>(q, geo_, nullptr, tmp, k, false, dst, stream);
             IPL_HIP_TRY(hipStreamSynchronize(stream));
         }
-        IPL_HIP_TRY(hipFree(tmp));
         return Status::ok();
     }
 
@@ -287,10 +286,58 @@ public:
         return Status::err(IPL_ERR_UNSUPPORTED, "lattice combination not built for the fused path");
     }
 
+    // Steps are replayed from a HIP graph of kGraphSteps steps, captured on first use: an even
+    // count, so the ping-pong buffers are back in their captured roles after every replay.  The
+    // step records kernel launches only (no memset or copy nodes).  Remaining steps are launched
+    // directly.
+    static constexpr int kGraphSteps = 20;
+
     Status step(int nsteps) override {
-        Status r = dispatch([&](auto q, auto qs) { return this->template step_q<decltype(q)::value, decltype(qs)::value>(nsteps); });
-        if (r && ib_.active()) r = ib_.check();
+        Status r = launch_steps(nsteps);
+        if (r) r = post_step();
         return r;
+    }
+
+    Status launch_steps(int nsteps) override {
+        int left = nsteps;
+        if (use_graph_ && left >= kGraphSteps) {
+            // the graph fixes which buffer is read first: re-capture after an odd direct count
+            if (graph_exec_ && graph_src_ != fa_) drop_graph();
+            if (!graph_exec_) {
+                Status r = capture_graph();
+                if (!r) return r;
+            }
+            while (left >= kGraphSteps) {
+                IPL_HIP_TRY(hipGraphLaunch(graph_exec_, stream));
+                left -= kGraphSteps;
+            }
+        }
+        if (left == 0) return Status::ok();
+        return dispatch([&](auto q, auto qs) { return this->template step_q<decltype(q)::value, decltype(qs)::value>(left); });
+    }
+
+    Status post_step() override { return ib_.active() ? ib_.check() : Status::ok(); }
+
+    Status capture_graph() {
+        graph_src_ = fa_;
+        IPL_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        Status r = dispatch([&](auto q, auto qs) { return this->template step_q<decltype(q)::value, decltype(qs)::value>(kGraphSteps); });
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(stream, &g);
+        if (!r) { if (g) (void)hipGraphDestroy(g); return r; }
+        if (e != hipSuccess) return Status::err(IPL_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        const hipError_t e2 = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e2 != hipSuccess) { graph_exec_ = nullptr; return Status::err(IPL_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e2)); }
+        return Status::ok();
+    }
+
+    void set_graph(bool on) override {
+        use_graph_ = on;
+    }
+
+    ~FusedEngine() override {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     }
 
     Status ib_attach(const ipl_ib_config& cfg) override {
@@ -305,6 +352,7 @@ public:
         if (r) {
             geo_.half_force = 0;
             K_.uc = MT(1);
+            drop_graph();  // the captured step has no IB sub-step
         }
         return r;
     }
@@ -444,7 +492,19 @@ private:
     double *vol_ = nullptr, *out_ = nullptr, *partial_ = nullptr;
     std::vector<int64_t> perm_;
     IbSolver ib_;
+    hipGraphExec_t graph_exec_ = nullptr;
+    const ST* graph_src_ = nullptr;
+#ifdef IPL_CPU_EMULATION
+    bool use_graph_ = false;
+#else
+    bool use_graph_ = true;
+#endif
     unsigned block_ = 64;
+
+    void drop_graph() {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
     size_t lds_bytes_ = 0;
 };
 

@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
             for (int k = 0; k < Q; ++k) part[k] += wgt * ldpop(f, (uint32_t)k * A.ncell + idx);
         } else {
             const int64_t b = box_index(A, box, cc);
-            if (b < 0) continue;  // cannot happen: the box holds every node block
+            if (b < 0 || b >= A.vmax) continue;  // cannot happen: the box holds every node block
             for (int a = 0; a < 3; ++a) {
                 double u = Ubox[3 * b + a];
                 if (it > 0) u += A.dt * Fbox[3 * b + a] / 3.0;
@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(kIbGatherLanes) k_ib_gather(IbArgs A, int it, 
             }
         }
         const int64_t bs = ok ? box_index(A, box, src) : -1;  // no marker is nearest to a cell outside the box
-        if (bs >= 0) {
+        if (bs >= 0 && bs < A.vmax) {
             const int2 r = range[bs];
             for (int q = r.x; q < r.x + r.y; ++q) {
                 const int m = (int)(sorted[q] & 0xffffffffull);

@@ -1,0 +1,151 @@
+"""Graph replay, batched stepping and the serving loop on the GPU.
+
+A step replayed from a captured HIP graph runs the same kernels with the same arguments as a
+directly launched one, and members of a batch share nothing but the device, so every comparison
+here is bit-exact: graph vs direct, each job alone vs the same job inside a batch or a served mix.
+Each graph test replays several times with reads (device-to-host copies and synchronizations)
+between the replays and checks the state after every one of them."""
+import numpy as np
+import pytest
+
+from ip_cu_lbm_amd import F32, F64, PATH_FUSED, Batch, Simulation, _abi, cases, estimate_device_bytes, serve
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(sim, fields=("p", "u", "v", "w")):
+    return {f: sim.read(f) for f in fields}
+
+
+def _same(a, b, what):
+    for k in a:
+        assert np.array_equal(a[k], b[k]), (what, k, np.abs(a[k] - b[k]).max())
+
+
+def _small_cases():
+    le = cases.lock_exchange(nx=40, ny=6, nz=18, storage=F32, path=PATH_FUSED)
+    tg = cases.lattice_taylor_green(nx=96, nz=48, storage=F32, path=PATH_FUSED)
+    pf = cases.solitary_wave(nx=64, nz=32, x0=20, sigma=1e-4, path=PATH_FUSED)
+    cyl = cases.cylinder_2d(n=(72, 80), markers=160, storage=F32, path=PATH_FUSED)
+    sph = cases.falling_sphere(n=(24, 22, 30), diameter=8.0, marker_set="3", storage=F32, path=PATH_FUSED,
+                               method=_abi.IB_STRESS, kernel=_abi.IB_KERNEL_REFERENCE, tau=0.7, gravity=2e-4)
+    sph.body.pos = (12.3, 10.8, 15.4)
+    return {"le": (le, ("p", "u", "v", "w", "S")), "tg": (tg, ("p", "u", "w")), "pf": (pf, ("p", "u", "w", "phi")),
+            "cyl": (cyl, ("p", "u", "w")), "sph": (sph, ("p", "u", "v", "w"))}
+
+
+@pytest.mark.parametrize("name", ["le", "tg", "pf", "cyl", "sph"])
+def test_graph_replay_matches_direct_launches(name):
+    """Graph replay (20-step graphs) vs direct launches, bit for bit, after each of several
+    replays; an odd direct count in between swaps the buffer roles and forces a re-capture."""
+    case, fields = _small_cases()[name]
+    with Simulation(case) as g, Simulation(case) as d:
+        d.set_graph(False)
+        for n in (20, 20, 40, 7, 20, 60, 3, 21):
+            g.step(n)
+            d.step(n)
+            _same(_state(g, fields), _state(d, fields), (name, n))
+        if case.body is not None:
+            bg, bd = g.body_state(), d.body_state()
+            for k in bg:
+                assert np.array_equal(bg[k], bd[k]), k
+            assert np.array_equal(g.markers("force"), d.markers("force"))
+
+
+def test_batch_members_match_alone():
+    """Every member of a batch, advanced by its own step counts, ends bit-identical to the same
+    case stepped alone; members join and leave between batch steps."""
+    cs = _small_cases()
+    names = list(cs)
+    counts = {"le": [30, 25, 40], "tg": [50, 0, 13], "pf": [20, 20, 20], "cyl": [7, 40, 33], "sph": [25, 25, 5]}
+    alone = {}
+    for n in names:
+        case, fields = cs[n]
+        with Simulation(case) as s:
+            s.step(sum(counts[n]))
+            alone[n] = _state(s, fields)
+    sims = {n: Simulation(cs[n][0]) for n in names}
+    try:
+        with Batch() as b:
+            for n in names[:3]:
+                b.add(sims[n])
+            b.step([counts[n][0] for n in names[:3]])
+            for n in names[3:]:
+                b.add(sims[n])
+            # the late joiners catch up with a first chunk of their own
+            b.step([counts[n][1] for n in names[:3]] + [counts[n][0] for n in names[3:]])
+            b.remove(sims["tg"])
+            sims["tg"].step(counts["tg"][2])
+            rest = [n for n in names if n != "tg"]
+            b.step([counts[n][2] if n in names[:3] else counts[n][1] for n in rest])
+            b.step([0 if n in names[:3] else counts[n][2] for n in rest])
+        for n in names:
+            _same(_state(sims[n], cs[n][1]), alone[n], n)
+    finally:
+        for s in sims.values():
+            s.close()
+
+
+def test_batch_uniform_count():
+    cs = _small_cases()
+    with Simulation(cs["le"][0]) as a, Simulation(cs["tg"][0]) as b, Simulation(cs["le"][0]) as ref:
+        with Batch() as bt:
+            bt.add(a)
+            bt.add(b)
+            bt.step(45)
+        ref.step(45)
+        _same(_state(a, cs["le"][1]), _state(ref, cs["le"][1]), "uniform count")
+
+
+def test_estimate_covers_allocation():
+    """The admission estimate is within 25 % above what the context actually allocates."""
+    for name, (case, _f) in _small_cases().items():
+        est = estimate_device_bytes(case)
+        with Simulation(case) as s:
+            real = s.device_bytes
+        print(name, est, real, flush=True)
+        assert real <= est <= 1.25 * real + (2 << 20), (name, est, real)
+
+
+def test_serving_mix_results_match_alone():
+    """A small served mix (continuous batching, memory budget forcing a queue): every job's final
+    fields equal the same job run alone."""
+    jobs = serve.serving_mix(n_jobs=14, mean_gap_s=0.002, steps=(40, 160), seed=cases.SEED_BASE + 11)
+    finals = {}
+
+    def keep(job):
+        finals[job.jid] = _state(job.sim, ("p", "u", "w"))
+
+    budget = 3 * max(estimate_device_bytes(j.case) for j in jobs)
+    res = serve.Server(budget_bytes=budget, chunk=30).run(jobs, log=print, on_done=keep)
+    print(res, flush=True)
+    assert res["jobs"] == len(jobs) and len(finals) == len(jobs)
+    assert res["p99_s"] >= res["p50_s"] > 0 and res["aggregate_mlups"] > 0
+    for j in jobs:
+        with Simulation(j.case) as s:
+            s.step(j.steps)
+            _same(_state(s, ("p", "u", "w")), finals[j.jid], j.case.name)
+
+
+def test_cylinder_2d_stress_fp64(oracle_lib):
+    """2D immersed boundary (D2Q9, markers on a circle in the x-z plane) vs the oracle."""
+    from conftest import rel_l2
+    c = cases.cylinder_2d(n=(48, 56), markers=140, storage=F64, path=PATH_FUSED, density_ratio=3.0, gravity=2e-4)
+    c.body.pos = (24.3, 0.5, 27.6)
+    X, _, Z = c.coords()
+    c.init = {"u": 0.01 * np.sin(2 * np.pi * Z / 56), "p": cases.uniform(9, c.n_cells, -1e-4, 1e-4)}
+    for method, kernel, iters in ((_abi.IB_STRESS, _abi.IB_KERNEL_REFERENCE, 1),
+                                  (_abi.IB_DIRECT, _abi.IB_KERNEL_REFERENCE, 1),
+                                  (_abi.IB_DIRECT, _abi.IB_KERNEL_PESKIN4, 3)):
+        c.body.method, c.body.kernel, c.body.iterations = method, kernel, iters
+        with Simulation(c) as sim:
+            sim.step(40)
+            gb, gf, gu = sim.body_state(), sim.markers("force"), np.stack([sim.read("u"), sim.read("w")])
+        o = oracle_lib.Oracle(c)
+        o.step(40)
+        ob, of, ou = o.body_state(), o.markers("force"), np.stack([o.read("u"), o.read("w")])
+        o.close()
+        errs = (rel_l2(gu, ou), rel_l2(gf, of), rel_l2(gb["force"], ob["force"]), rel_l2(gb["vel"], ob["vel"]))
+        print("cylinder", method, kernel, iters, errs, flush=True)
+        assert errs[0] <= 1e-12 and errs[2] <= 1e-12 and errs[3] <= 1e-12
+        assert errs[1] <= (1e-10 if iters > 1 else 1e-12)
