@@ -23,6 +23,7 @@ struct IbArgs {
     int M, S, lo, hi, block3;   // slots per marker; node offsets lo..hi per axis; reference block table
     int method, kernel, moving;
     double spread, wscale, gwscale;
+    double sina, inv_c, inv_c2, half_itau;  // stress method: sin of the split angle, 1/c, 1/c^2, 1/(2 tau)
     double rho_f, density, mass, inertia[3], gravity[3];
     int64_t vmax;
     const double* pos[3];
@@ -133,70 +134,89 @@ __device__ inline int64_t box_index(const IbArgs& A, const int* box, const int c
     return ((int64_t)b[0] * box[4] + b[1]) * box[5] + b[2];
 }
 
-// ---- k_ib_sort: markers sorted by (nearest cell, index); bounding box of the node blocks -----
-__global__ void k_ib_sort(IbArgs A, const int* node, unsigned long long* sorted, int* box, int mpad) {
+// ---- k_ib_rank: markers sorted by (nearest cell, index) and the marker run of every cell -------
+// Keys are (nearest cell << 32 | marker index), all distinct.  A workgroup of 16 wavefronts takes
+// 64 markers: it loads every key into LDS and finds the bounding box of the node blocks; lane l of
+// wavefront p counts, over the p-th sixteenth of the keys, those below marker l's key (its rank)
+// and those below the first and past the last key of its cell (the cell's run).  The sixteen
+// partial counts are added in order; the first marker of a cell writes the cell's run (start,
+// count), the apply kernel of the previous step having cleared the others.
+constexpr int kIbRankThreads = 1024, kIbRankMarkers = 64, kIbRankParts = kIbRankThreads / kIbRankMarkers;
+
+__global__ void __launch_bounds__(kIbRankThreads) k_ib_rank(IbArgs A, const int* __restrict__ node,
+                                                             unsigned long long* __restrict__ sorted, int* __restrict__ box,
+                                                             int2* __restrict__ range, int mpad) {
     IPL_DYNAMIC_LDS(unsigned long long, keys);
-    __shared__ int mn[3][256], mx[3][256];
-    const int t = threadIdx.x, nt = blockDim.x;
-    for (int i = t; i < mpad; i += nt) {
+    __shared__ int work[6][kIbRankThreads];  // min / max of the node coordinates, then partial counts
+    __shared__ int bx[6];
+    const int t = threadIdx.x;
+    int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+    for (int i = t; i < mpad; i += kIbRankThreads) {
         if (i < A.M) {
             const int c[3] = {node[3 * i], node[3 * i + 1], node[3 * i + 2]};
             keys[i] = ((unsigned long long)cell_of(A, c) << 32) | (unsigned long long)i;
+            for (int a = 0; a < 3; ++a) { lo[a] = min(lo[a], c[a]); hi[a] = max(hi[a], c[a]); }
         } else {
-            keys[i] = ~0ull;
+            keys[i] = ~0ull;  // above every key
         }
     }
+    for (int a = 0; a < 3; ++a) { work[a][t] = lo[a]; work[3 + a][t] = hi[a]; }
     __syncthreads();
-    // bitonic sort of mpad (a power of two) keys
-    for (int size = 2; size <= mpad; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = t; i < mpad; i += nt) {
-                const int j = i ^ stride;
-                if (j > i) {
-                    const bool up = (i & size) == 0;
-                    const unsigned long long a = keys[i], b = keys[j];
-                    if ((a > b) == up) { keys[i] = b; keys[j] = a; }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    for (int i = t; i < A.M; i += nt) sorted[i] = keys[i];
-    // bounding box of the marker nodes (first 256 work-items)
-    if (t < 256) {
-        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
-        for (int i = t; i < A.M; i += 256)
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = min(lo[a], node[3 * i + a]);
-                hi[a] = max(hi[a], node[3 * i + a]);
-            }
-        for (int a = 0; a < 3; ++a) { mn[a][t] = lo[a]; mx[a][t] = hi[a]; }
-    }
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = kIbRankThreads / 2; w > 0; w >>= 1) {
         if (t < w)
             for (int a = 0; a < 3; ++a) {
-                mn[a][t] = min(mn[a][t], mn[a][t + w]);
-                mx[a][t] = max(mx[a][t], mx[a][t + w]);
+                work[a][t] = min(work[a][t], work[a][t + w]);
+                work[3 + a][t] = max(work[3 + a][t], work[3 + a][t + w]);
             }
         __syncthreads();
     }
     if (t == 0) {
         int64_t vol = 1;
         for (int a = 0; a < 3; ++a) {
-            int lo = mn[a][0] + A.lo, hi = mx[a][0] + A.hi;
+            int l = work[a][0] + A.lo, h = work[3 + a][0] + A.hi;
             if (A.periodic[a]) {
-                if (hi - lo + 1 >= A.n[a]) { lo = 0; hi = A.n[a] - 1; }
-                else if (lo < 0) lo += A.n[a];
+                if (h - l + 1 >= A.n[a]) { l = 0; h = A.n[a] - 1; }
+                else if (l < 0) l += A.n[a];
             } else {
-                lo = max(lo, 0);
-                hi = min(hi, A.n[a] - 1);
+                l = max(l, 0);
+                h = min(h, A.n[a] - 1);
             }
-            box[a] = lo;
-            box[3 + a] = hi - lo + 1;
-            vol *= box[3 + a];
+            bx[a] = l;
+            bx[3 + a] = h - l + 1;
+            vol *= bx[3 + a];
         }
-        box[6] = vol <= A.vmax ? 0 : 1;  // overflow flag (the host sizes vmax so that it stays 0)
+        if (blockIdx.x == 0) {
+            for (int a = 0; a < 6; ++a) box[a] = bx[a];
+            box[6] = vol <= A.vmax ? 0 : 1;  // overflow flag (the host sizes vmax so that it stays 0)
+        }
+    }
+    __syncthreads();
+    const int l = t % kIbRankMarkers, part = t / kIbRankMarkers;
+    const int m = blockIdx.x * kIbRankMarkers + l;
+    const unsigned long long mine = m < A.M ? keys[m] : ~0ull;
+    const unsigned long long first = mine & 0xffffffff00000000ull, past = first + (1ull << 32);
+    int below = 0, lower = 0, upto = 0;
+    const int chunk = mpad / kIbRankParts;  // mpad is a multiple of 2 * kIbRankParts
+    for (int j = part * chunk; j < (part + 1) * chunk; j += 2) {
+        const unsigned long long k0 = keys[j], k1 = keys[j + 1];
+        below += (k0 < mine) + (k1 < mine);
+        lower += (k0 < first) + (k1 < first);
+        upto += (k0 < past) + (k1 < past);
+    }
+    __syncthreads();  // done with the box work area
+    work[0][t] = below; work[1][t] = lower; work[2][t] = upto;
+    __syncthreads();
+    if (part != 0 || m >= A.M) return;
+    for (int p = 1; p < kIbRankParts; ++p) {
+        below += work[0][p * kIbRankMarkers + l];
+        lower += work[1][p * kIbRankMarkers + l];
+        upto += work[2][p * kIbRankMarkers + l];
+    }
+    sorted[below] = mine;
+    if (below == lower) {  // first marker of its cell
+        const int c[3] = {node[3 * m], node[3 * m + 1], node[3 * m + 2]};
+        const int64_t b = box_index(A, bx, c);
+        if (b >= 0 && b < A.vmax) range[b] = make_int2(lower, upto - lower);
     }
 }
 
@@ -248,7 +268,8 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
                                                       const double* __restrict__ velw, double* __restrict__ Gw,
                                                       double* __restrict__ contrib, const int* __restrict__ box,
                                                       const double* __restrict__ Fbox, const double* __restrict__ Ubox) {
-    __shared__ double red[kIbLanes * (Q > 3 ? Q : 3)];
+    constexpr int kRows = Q > 20 ? Q : 20;  // the slot sums, then the 20 moment sums
+    __shared__ double red[kIbLanes * kRows];
     __shared__ double share[3];
     const int m = blockIdx.x, t = threadIdx.x;
     const int base[3] = {node[3 * m], node[3 * m + 1], node[3 * m + 2]};
@@ -283,65 +304,109 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
             }
         }
     }
-    for (int k = 0; k < nr; ++k) red[k * kIbLanes + t] = part[k];
+#pragma unroll
+    for (int k = 0; k < (Q > 3 ? Q : 3); ++k)
+        if (k < nr) red[k * kIbLanes + t] = part[k];
     __syncthreads();
     for (int w = kIbLanes / 2; w > 0; w >>= 1) {
         if (t < w)
             for (int k = 0; k < nr; ++k) red[k * kIbLanes + t] += red[k * kIbLanes + t + w];
         __syncthreads();
     }
-    // 2. the marker force (lane 0)
-    if (t == 0) {
-        if (A.method == IPL_IB_STRESS) {
-            double fw[Q], fw1[Q], fw2[Q];
+    // 2. the marker force
+    if (A.method == IPL_IB_STRESS) {
+        // Lane k takes direction k: f at the marker split by the normal into the outer (fw1) and
+        // inner (fw2) populations (ibm.cu:191-236), then its share of the moments sum_k g,
+        // sum_k g e_a and sum_k g e_a e_b for g = fw1 - fw2 and g = fw1.
+        double mom[20];
 #pragma unroll
-            for (int k = 0; k < Q; ++k) fw[k] = red[k * kIbLanes];
-            const double nv[3] = {nB[3 * m], nB[3 * m + 1], nB[3 * m + 2]};
-            const double sina = sin(22.5 / 180.0 * 3.14159);  // the reference's literal angle
+        for (int q = 0; q < 20; ++q) mom[q] = 0.0;
+        if (t < Q) {
+            int ex = 0, ey = 0, ez = 0, op = 0;
+            double wk = 0.0;
 #pragma unroll
-            for (int k = 0; k < Q; ++k) {
-                const Dir e = Lattice<Q>::c(k);
-                const double cn = (nv[0] * e.x + nv[1] * e.y + nv[2] * e.z) / c;
-                const double lim = sqrt((double)(e.x * e.x + e.y * e.y + e.z * e.z)) / c * sina;
-                const double bounce = fw[opposite<Q>(k)] + 6.0 * Lattice<Q>::w(k) * (e.x * U[0] + e.y * U[1] + e.z * U[2]) / c2;
-                const double beta = fabs(cn) / (lim + 1e-10);
-                const double mixed = (1.0 - beta) * fw[k] + beta * bounce;
-                fw1[k] = cn <= 0 ? fw[k] : (cn <= lim ? mixed : bounce);
-                fw2[k] = cn >= 0 ? fw[k] : (cn >= -lim ? mixed : bounce);
-            }
-            const double tau = A.tau;
-            // traction sigma . n: sigma_ab = -sum_k [g_k delta_ab / (2 tau)
-            //   + (1 - 1/(2 tau)) 3 (g_k (e_a - U_a)(e_b - U_b) - (g_k - w_k) U_a U_b) / c^2]
-            auto traction = [&](bool diff, double tr[3]) {
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    double sg[3] = {0, 0, 0};
-#pragma unroll
-                    for (int k = 0; k < Q; ++k) {
-                        const Dir e = Lattice<Q>::c(k);
-                        const double ev[3] = {(double)e.x, (double)e.y, (double)e.z};
-                        const double gk = diff ? fw1[k] - fw2[k] : fw1[k];
-#pragma unroll
-                        for (int b2 = 0; b2 < 3; ++b2)
-                            sg[b2] -= 0.5 / tau * gk * (a == b2 ? 1.0 : 0.0) +
-                                      (1.0 - 0.5 / tau) * 3 * (gk * (ev[a] - U[a]) * (ev[b2] - U[b2]) - (gk - Lattice<Q>::w(k)) * U[a] * U[b2]) / c2;
-                    }
-                    tr[a] = sg[0] * nv[0] + sg[1] * nv[1] + sg[2] * nv[2];
+            for (int k = 0; k < Q; ++k)
+                if (t == k) {
+                    const Dir e = Lattice<Q>::c(k);
+                    ex = e.x; ey = e.y; ez = e.z;
+                    op = opposite<Q>(k);
+                    wk = Lattice<Q>::w(k);
                 }
-            };
-            double gs[3], gw[3];
-            traction(true, gs);
-            traction(false, gw);
-            for (int a = 0; a < 3; ++a) {
-                share[a] = -gs[a];
-                Gw[3 * m + a] = gw[a] * A.gwscale;
+            const double fk = red[t * kIbLanes], fo = red[op * kIbLanes];
+            const double nv[3] = {nB[3 * m], nB[3 * m + 1], nB[3 * m + 2]};
+            const double cn = (nv[0] * ex + nv[1] * ey + nv[2] * ez) * A.inv_c;
+            const double lim = sqrt((double)(ex * ex + ey * ey + ez * ez)) * A.inv_c * A.sina;
+            const double bounce = fo + 6.0 * wk * (ex * U[0] + ey * U[1] + ez * U[2]) * A.inv_c2;
+            const double beta = fabs(cn) / (lim + 1e-10);
+            const double mixed = (1.0 - beta) * fk + beta * bounce;
+            const double fw1 = cn <= 0 ? fk : (cn <= lim ? mixed : bounce);
+            const double fw2 = cn >= 0 ? fk : (cn >= -lim ? mixed : bounce);
+            const double ev[3] = {(double)ex, (double)ey, (double)ez};
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const double g = h == 0 ? fw1 - fw2 : fw1;
+                double* o = mom + 10 * h;
+                o[0] = g;
+                o[1] = g * ev[0]; o[2] = g * ev[1]; o[3] = g * ev[2];
+                o[4] = g * ev[0] * ev[0]; o[5] = g * ev[1] * ev[1]; o[6] = g * ev[2] * ev[2];
+                o[7] = g * ev[0] * ev[1]; o[8] = g * ev[0] * ev[2]; o[9] = g * ev[1] * ev[2];
             }
-        } else {
+        }
+        __syncthreads();  // every lane has read its slot sums
+#pragma unroll
+        for (int q = 0; q < 20; ++q) red[q * kIbLanes + t] = mom[q];
+        __syncthreads();
+        for (int w = kIbLanes / 2; w > 0; w >>= 1) {
+            if (t < w)
+#pragma unroll
+                for (int q = 0; q < 20; ++q) red[q * kIbLanes + t] += red[q * kIbLanes + t + w];
+            __syncthreads();
+        }
+        // traction sigma . n with sigma_ab = -sum_k [g_k delta_ab / (2 tau)
+        //   + (1 - 1/(2 tau)) 3 (g_k (e_a - U_a)(e_b - U_b) - (g_k - w_k) U_a U_b) / c^2]
+        // = -[G delta_ab / (2 tau) + (1 - 1/(2 tau)) 3 / c^2 (G_ab - G_a U_b - U_a G_b + U_a U_b)]
+        // (sum_k w_k = 1), the same on every lane
+        const double nv[3] = {nB[3 * m], nB[3 * m + 1], nB[3 * m + 2]};
+        const double cf = (1.0 - A.half_itau) * 3.0 * A.inv_c2;
+        double tr[2][3];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            double M[10];
+#pragma unroll
+            for (int q = 0; q < 10; ++q) M[q] = red[(10 * h + q) * kIbLanes];
+            const double G2[3][3] = {{M[4], M[7], M[8]}, {M[7], M[5], M[9]}, {M[8], M[9], M[6]}};
+#pragma unroll
             for (int a = 0; a < 3; ++a) {
-                const double gwv = (U[a] - red[a * kIbLanes]) / A.dt;
-                share[a] = gwv;
-                Gw[3 * m + a] = (it == 0 ? 0.0 : Gw[3 * m + a]) + -gwv * A.wscale;
+                double acc = 0.0;
+#pragma unroll
+                for (int b2 = 0; b2 < 3; ++b2) {
+                    const double sg = -((a == b2 ? M[0] * A.half_itau : 0.0) +
+                                        cf * (G2[a][b2] - M[1 + a] * U[b2] - U[a] * M[1 + b2] + U[a] * U[b2]));
+                    acc += sg * nv[b2];
+                }
+                tr[h][a] = acc;
             }
+        }
+        if (t == 0)
+            for (int a = 0; a < 3; ++a) Gw[3 * m + a] = tr[1][a] * A.gwscale;
+        const double gs0 = -tr[0][0], gs1 = -tr[0][1], gs2 = -tr[0][2];
+        for (int s = t; s < A.S; s += kIbLanes) {
+            int d[3], cc[3];
+            double pp[3];
+            slot_offset(A, s, d);
+            const bool ok = node_at(A, base, d, cc, pp);
+            const double wgt = slot_weight(A, W, s);
+            out[3 * s + 0] = ok ? gs0 * A.spread * wgt : 0.0;
+            out[3 * s + 1] = ok ? gs1 * A.spread * wgt : 0.0;
+            out[3 * s + 2] = ok ? gs2 * A.spread * wgt : 0.0;
+        }
+        return;
+    }
+    if (t == 0) {
+        for (int a = 0; a < 3; ++a) {
+            const double gwv = (U[a] - red[a * kIbLanes]) / A.dt;
+            share[a] = gwv;
+            Gw[3 * m + a] = (it == 0 ? 0.0 : Gw[3 * m + a]) + -gwv * A.wscale;
         }
     }
     __syncthreads();
@@ -359,42 +424,19 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
     }
 }
 
-// ---- k_ib_ranges: for every cell of the box, the run of sorted markers whose nearest cell it is
-__global__ void k_ib_ranges(IbArgs A, const unsigned long long* __restrict__ sorted, const int* __restrict__ box,
-                            int2* __restrict__ range) {
-    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int64_t vol = (int64_t)box[3] * box[4] * box[5];
-    if (b >= vol || b >= A.vmax) return;
-    int c[3];
-    c[2] = (int)(b % box[5]);
-    c[1] = (int)((b / box[5]) % box[4]);
-    c[0] = (int)(b / ((int64_t)box[4] * box[5]));
-    for (int a = 0; a < 3; ++a) {
-        c[a] += box[a];
-        if (c[a] >= A.n[a]) c[a] -= A.n[a];
-    }
-    const unsigned long long key = (unsigned long long)cell_of(A, c) << 32;
-    int lo = 0, hi = A.M;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (sorted[mid] < key) lo = mid + 1; else hi = mid;
-    }
-    int e = lo;
-    while (e < A.M && (sorted[e] >> 32) == (key >> 32)) ++e;
-    range[b] = make_int2(lo, e - lo);
-}
-
 // ---- k_ib_gather: spread force of every cell of the box ----------------------------------------
-// One workgroup per cell, one lane per block slot: cell c receives slot s of every marker whose
-// nearest cell is c - d(s) (markers by index), and the slot sums are added in a fixed tree.
-constexpr int kIbGatherLanes = 128;  // >= the slots of a block (125 for the 5-node support)
+// One wavefront per cell, lanes over the block slots (two each for the 125-slot support): cell c
+// receives slot s of every marker whose nearest cell is c - d(s) (markers by index), and the
+// lane sums are added in a fixed tree.  A cell that no marker block reaches leaves at once.
+constexpr int kIbGatherLanes = 64;
 
 __global__ void __launch_bounds__(kIbGatherLanes) k_ib_gather(IbArgs A, int it, const unsigned long long* __restrict__ sorted,
                                                                const int2* __restrict__ range, const double* __restrict__ contrib,
                                                                const int* __restrict__ box, double* __restrict__ Fbox) {
     __shared__ double red[3][kIbGatherLanes];
+    __shared__ int hit;
     const int64_t b = blockIdx.x;
-    const int s = threadIdx.x;
+    const int t = threadIdx.x;
     const int64_t vol = (int64_t)box[3] * box[4] * box[5];
     if (b >= vol || b >= A.vmax) return;  // whole workgroup: b is uniform
     int c[3];
@@ -405,8 +447,11 @@ __global__ void __launch_bounds__(kIbGatherLanes) k_ib_gather(IbArgs A, int it, 
         c[a] += box[a];
         if (c[a] >= A.n[a]) c[a] -= A.n[a];
     }
+    if (t == 0) hit = 0;
+    __syncthreads();
     double F[3] = {0, 0, 0};
-    if (s < A.S) {
+    bool any = false;
+    for (int s = t; s < A.S; s += kIbGatherLanes) {
         int d[3], src[3];
         slot_offset(A, s, d);
         bool ok = true;
@@ -420,6 +465,7 @@ __global__ void __launch_bounds__(kIbGatherLanes) k_ib_gather(IbArgs A, int it, 
         const int64_t bs = ok ? box_index(A, box, src) : -1;  // no marker is nearest to a cell outside the box
         if (bs >= 0 && bs < A.vmax) {
             const int2 r = range[bs];
+            any = any || r.y > 0;
             for (int q = r.x; q < r.x + r.y; ++q) {
                 const int m = (int)(sorted[q] & 0xffffffffull);
                 const double* cm = contrib + ((size_t)m * A.S + s) * 3;
@@ -427,21 +473,30 @@ __global__ void __launch_bounds__(kIbGatherLanes) k_ib_gather(IbArgs A, int it, 
             }
         }
     }
-    for (int a = 0; a < 3; ++a) red[a][s] = F[a];
+    if (any) hit = 1;
+    __syncthreads();
+    if (!hit) {  // uniform: no contribution reaches this cell
+        if (t == 0 && it == 0) { Fbox[3 * b] = 0.0; Fbox[3 * b + 1] = 0.0; Fbox[3 * b + 2] = 0.0; }
+        return;
+    }
+    for (int a = 0; a < 3; ++a) red[a][t] = F[a];
     __syncthreads();
     for (int w = kIbGatherLanes / 2; w > 0; w >>= 1) {
-        if (s < w)
-            for (int a = 0; a < 3; ++a) red[a][s] += red[a][s + w];
+        if (t < w)
+            for (int a = 0; a < 3; ++a) red[a][t] += red[a][t + w];
         __syncthreads();
     }
-    if (s == 0)
+    if (t == 0)
         for (int a = 0; a < 3; ++a) Fbox[3 * b + a] = (it == 0 ? 0.0 : Fbox[3 * b + a]) + red[a][0];
 }
 
 // ---- k_ib_apply: update_velIBM (ibm.cu:346-365) on the box ------------------------------------
+// It also clears the marker runs of every box cell for the next step's rank kernel.
 template <typename ST, int Q>
-__global__ void k_ib_apply(IbArgs A, ST* __restrict__ f, const int* __restrict__ box, const double* __restrict__ Fbox) {
+__global__ void k_ib_apply(IbArgs A, ST* __restrict__ f, const int* __restrict__ box, const double* __restrict__ Fbox,
+                           int2* __restrict__ range) {
     const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (b < A.vmax) range[b] = make_int2(0, 0);
     const int64_t vol = (int64_t)box[3] * box[4] * box[5];
     if (b >= vol || b >= A.vmax) return;
     const double F[3] = {Fbox[3 * b], Fbox[3 * b + 1], Fbox[3 * b + 2]};
@@ -471,13 +526,20 @@ __device__ inline void rotation(const double* q, double* R) {
     R[6] = 2 * (b * d - a * c);         R[7] = 2 * (c * d + a * b);         R[8] = a * a - b * b - c * c + d * d;
 }
 
-// ---- k_ib_body: update_IBbody (ibm.cu:367-424), one workgroup of 256 ---------------------------
-__global__ void k_ib_body(IbArgs A, const double* __restrict__ Gw, const double* __restrict__ posw, double* body) {
-    __shared__ double red[6][256];
+// ---- k_ib_body: update_IBbody (ibm.cu:367-424) in one workgroup: the force and torque sums in a
+// fixed tree, the body update on work-item 0
+constexpr int kIbBodyThreads = 1024;
+
+__device__ inline void ib_body_update(const IbArgs& A, double fx, double fy, double fz, double tx, double ty, double tz,
+                                      double* body);
+
+__global__ void __launch_bounds__(kIbBodyThreads) k_ib_body(IbArgs A, const double* __restrict__ Gw,
+                                                            const double* __restrict__ posw, double* body) {
+    __shared__ double red[6][kIbBodyThreads];
     const int t = threadIdx.x;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     const double* P = body + B_POS;
-    for (int m = t; m < A.M; m += 256) {
+    for (int m = t; m < A.M; m += kIbBodyThreads) {
         const double* G = Gw + 3 * m;
         const double r[3] = {posw[3 * m] - P[0], posw[3 * m + 1] - P[1], posw[3 * m + 2] - P[2]};
         acc[0] += G[0]; acc[1] += G[1]; acc[2] += G[2];
@@ -487,12 +549,18 @@ __global__ void k_ib_body(IbArgs A, const double* __restrict__ Gw, const double*
     }
     for (int q = 0; q < 6; ++q) red[q][t] = acc[q];
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = kIbBodyThreads / 2; w > 0; w >>= 1) {
         if (t < w)
             for (int q = 0; q < 6; ++q) red[q][t] += red[q][t + w];
         __syncthreads();
     }
-    if (t != 0) return;
+    if (t == 0) ib_body_update(A, red[0][0], red[1][0], red[2][0], red[3][0], red[4][0], red[5][0], body);
+}
+
+// the Newton-Euler update of the body from its force and torque sums
+__device__ inline void ib_body_update(const IbArgs& A, double fx, double fy, double fz, double tx, double ty, double tz,
+                                      double* body) {
+    const double sums[6] = {fx, fy, fz, tx, ty, tz};
     double* FB = body + B_FB;
     double* T = body + B_T;
     double FBold[3], Told[3], velBold[3], angold[3], Rold[9];
@@ -500,8 +568,8 @@ __global__ void k_ib_body(IbArgs A, const double* __restrict__ Gw, const double*
     for (int a = 0; a < 3; ++a) {
         FBold[a] = FB[a]; Told[a] = T[a];
         velBold[a] = body[B_VEL + a]; angold[a] = body[B_OMEGA + a];
-        FB[a] = red[a][0];
-        T[a] = red[3 + a][0];
+        FB[a] = sums[a];
+        T[a] = sums[3 + a];
     }
     for (int a = 0; a < 3; ++a) FB[a] += (1.0 - A.rho_f / A.density) * A.mass * A.gravity[a];
     if (!A.moving) return;
@@ -697,6 +765,7 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
     IPL_HIP_TRY(hipMemcpyAsync(node_, node.data(), 3 * M_ * sizeof(int), hipMemcpyHostToDevice, st));
     IPL_HIP_TRY(hipMemcpyAsync(body_, body.data(), ibk::B_SIZE * sizeof(double), hipMemcpyHostToDevice, st));
     IPL_HIP_TRY(hipMemsetAsync(box_, 0, 8 * sizeof(int), st));
+    IPL_HIP_TRY(hipMemsetAsync(range_, 0, (size_t)vmax_ * sizeof(int2), st));  // later cleared by k_ib_apply
     IPL_HIP_TRY(hipStreamSynchronize(st));
     return Status::ok();
 }
@@ -724,6 +793,10 @@ ibk::IbArgs make_args(const IbGrid& g, const ipl_ib_config& cfg, int M, int S, i
         A.spread = std::pow(cfg.spacing / g.dz, dim - 1) * 3.0 / c2;             // ibm.cu:336-338
         A.wscale = cfg.rho_f * std::pow(cfg.spacing, dim - 1) * g.dz;           // ibm.cu:342
     }
+    A.sina = std::sin(22.5 / 180.0 * 3.14159);  // the reference's literal angle (ibm.cu:201)
+    A.inv_c = 1.0 / g.c;
+    A.inv_c2 = 1.0 / c2;
+    A.half_itau = 0.5 / g.tau;
     A.rho_f = cfg.rho_f; A.density = cfg.density; A.mass = cfg.mass;
     for (int a = 0; a < 3; ++a) { A.inertia[a] = cfg.inertia[a]; A.gravity[a] = cfg.gravity[a]; }
     A.vmax = vmax;
@@ -748,11 +821,11 @@ void ubase_q(int Q, const ibk::IbArgs& A, const ST* f, const ST* g, const MT* F,
 }
 
 template <typename ST>
-void apply_q(int Q, const ibk::IbArgs& A, ST* f, const int* box, const double* Fbox, hipStream_t st) {
+void apply_q(int Q, const ibk::IbArgs& A, ST* f, const int* box, const double* Fbox, int2* range, hipStream_t st) {
     const unsigned blocks = (unsigned)((A.vmax + 255) / 256);
-    if (Q == 9) IPL_LAUNCH((ibk::k_ib_apply<ST, 9>), blocks, 256, 0, st, A, f, box, Fbox);
-    else if (Q == 19) IPL_LAUNCH((ibk::k_ib_apply<ST, 19>), blocks, 256, 0, st, A, f, box, Fbox);
-    else IPL_LAUNCH((ibk::k_ib_apply<ST, 27>), blocks, 256, 0, st, A, f, box, Fbox);
+    if (Q == 9) IPL_LAUNCH((ibk::k_ib_apply<ST, 9>), blocks, 256, 0, st, A, f, box, Fbox, range);
+    else if (Q == 19) IPL_LAUNCH((ibk::k_ib_apply<ST, 19>), blocks, 256, 0, st, A, f, box, Fbox, range);
+    else IPL_LAUNCH((ibk::k_ib_apply<ST, 27>), blocks, 256, 0, st, A, f, box, Fbox, range);
 }
 
 }  // namespace
@@ -760,16 +833,15 @@ void apply_q(int Q, const ibk::IbArgs& A, ST* f, const int* box, const double* F
 Status IbSolver::substep(void* f, const void* g, const void* F, int storage) {
     if (M_ == 0) return Status::ok();
     const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_);
-    int mpad = 1;
-    while (mpad < M_) mpad <<= 1;
-    IPL_LAUNCH_SYNC(ibk::k_ib_sort, 1, 1024, (size_t)mpad * sizeof(unsigned long long), st_, A, node_, sorted_, box_, mpad);
+    constexpr int kPad = 2 * ibk::kIbRankParts;
+    const int mpad = (M_ + kPad - 1) / kPad * kPad;
+    IPL_LAUNCH_SYNC(ibk::k_ib_rank, (unsigned)((M_ + ibk::kIbRankMarkers - 1) / ibk::kIbRankMarkers), ibk::kIbRankThreads,
+                    (size_t)mpad * sizeof(unsigned long long), st_, A, node_, sorted_, box_, range_, mpad);
     const int iters = cfg_.method == IPL_IB_DIRECT ? std::max(1, cfg_.iterations) : 1;
-    const unsigned gblocks = (unsigned)((vmax_ + 255) / 256);
     if (cfg_.method == IPL_IB_DIRECT) {
         if (storage == IPL_F64) ubase_q<double, double>(g_.Q, A, (const double*)f, (const double*)g, (const double*)F, box_, Ubox_, st_);
         else ubase_q<float, float>(g_.Q, A, (const float*)f, (const float*)g, (const float*)F, box_, Ubox_, st_);
     }
-    IPL_LAUNCH(ibk::k_ib_ranges, gblocks, 256, 0, st_, A, sorted_, box_, range_);
     for (int it = 0; it < iters; ++it) {
         if (storage == IPL_F64)
             force_q<double, double>(g_.Q, A, it, (const double*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box_, Fbox_, Ubox_, st_);
@@ -777,9 +849,9 @@ Status IbSolver::substep(void* f, const void* g, const void* F, int storage) {
             force_q<float, float>(g_.Q, A, it, (const float*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box_, Fbox_, Ubox_, st_);
         IPL_LAUNCH_SYNC(ibk::k_ib_gather, (unsigned)vmax_, ibk::kIbGatherLanes, 0, st_, A, it, sorted_, range_, contrib_, box_, Fbox_);
     }
-    if (storage == IPL_F64) apply_q<double>(g_.Q, A, (double*)f, box_, Fbox_, st_);
-    else apply_q<float>(g_.Q, A, (float*)f, box_, Fbox_, st_);
-    IPL_LAUNCH_SYNC(ibk::k_ib_body, 1, 256, 0, st_, A, Gw_, posw_, body_);
+    if (storage == IPL_F64) apply_q<double>(g_.Q, A, (double*)f, box_, Fbox_, range_, st_);
+    else apply_q<float>(g_.Q, A, (float*)f, box_, Fbox_, range_, st_);
+    IPL_LAUNCH_SYNC(ibk::k_ib_body, 1, ibk::kIbBodyThreads, 0, st_, A, Gw_, posw_, body_);
     if (cfg_.moving) {
         const unsigned blocks = (unsigned)((M_ + 63) / 64);
         if (g_.Q == 9) IPL_LAUNCH((ibk::k_ib_points<9>), blocks, 64, 0, st_, A, body_, oposw_, onB_, posw_, nB_, velw_, node_);

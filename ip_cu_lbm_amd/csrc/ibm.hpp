@@ -2,19 +2,20 @@
 //
 // One rigid body carried by Lagrangian markers.  Every step, after the fused step has written
 // f^{n+1} (and the Force field F^n), the sub-step of the drivers (SphereFall.cu:372-383) runs:
-//   k_ib_sort    one workgroup: markers sorted by their nearest cell (bitonic, in LDS) and the
-//                bounding box of their node blocks
+//   k_ib_rank    markers sorted by (nearest cell, index) by counting ranks, every workgroup
+//                holding all markers' cells in LDS; the bounding box of their node blocks and the
+//                run of sorted markers nearest to each box cell
 //   k_ib_ubase   (direct forcing) update_rho's velocity at every cell of the box
 //   k_ib_force   one 64-lane workgroup per marker, lanes over the block slots: node weights,
 //                interpolation of f (stress method, get_IBMGw2) or of the node velocity (direct
-//                forcing, IB_directForcing) reduced in a fixed tree, marker force, and the
-//                contribution of every block slot to the spread force
-//   k_ib_ranges  one work-item per cell of the box: the run of sorted markers nearest to it
-//   k_ib_gather  one work-item per cell of the box: the spread force of the cell, summed over the
-//                block slots in slot order and, per slot, over markers in index order (no atomics:
-//                the sum has one order whatever the scheduling)
+//                forcing, IB_directForcing) reduced in a fixed tree, marker force (stress method:
+//                one lane per direction, moment sums in a fixed tree), and the contribution of
+//                every block slot to the spread force
+//   k_ib_gather  one wavefront per cell of the box: the spread force of the cell, summed over
+//                markers in index order per slot and over slots in a fixed tree (no atomics: the
+//                sum has one order whatever the scheduling)
 //   (force + gather repeated for multi-direct forcing)
-//   k_ib_apply   update_velIBM on the cells of the box: f += w dt (e . F)
+//   k_ib_apply   update_velIBM on the cells of the box: f += w dt (e . F); clears the marker runs
 //   k_ib_body    one workgroup: FP64 force / torque sums in a fixed tree, buoyancy, Adams-Bashforth
 //                update of the body (update_IBbody)
 //   k_ib_points  one work-item per marker: update_IBpoint and search_IBlattice
@@ -76,6 +77,6 @@ private:
     unsigned long long* sorted_ = nullptr;
 };
 
-constexpr int kIbMaxMarkers = 4096;  // one workgroup sorts them in 32 KiB of LDS
+constexpr int kIbMaxMarkers = 4096;  // the rank kernel holds every marker's key in 32 KiB of LDS
 
 }  // namespace ipl

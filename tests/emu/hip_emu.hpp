@@ -32,6 +32,7 @@ using std::max;
 
 struct ipl_emu_idx3 { unsigned x = 0, y = 0, z = 0; };
 struct int2 { int x, y; };
+struct alignas(16) uint4 { unsigned x, y, z, w; };
 inline int2 make_int2(int a, int b) { return int2{a, b}; }
 extern thread_local ipl_emu_idx3 threadIdx, blockIdx, blockDim, gridDim;
 extern thread_local std::barrier<>* ipl_emu_barrier;
