@@ -157,6 +157,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 plain / fused comparison runs")
     ap.add_argument("--only-headline", action="store_true", help="run just the headline workload (for profilers)")
+    ap.add_argument("--only-sphere", action="store_true", help="run just the falling-sphere workload (for profilers)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the traffic file from two rocprofv3 --pmc output directories and exit")
     a = ap.parse_args(argv)
@@ -172,6 +173,10 @@ def main(argv=None):
 
     if a.pmc_summary:
         pmc_summary(a.pmc_summary[0], a.pmc_summary[1], 512 * 32 * 64)
+        return 0
+
+    if a.only_sphere:
+        print(json.dumps(measure_sphere(a.steps, a.warmup)), flush=True)
         return 0
 
     head_case = cases.lock_exchange(storage=F32, path=PATH_FUSED)
