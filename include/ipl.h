@@ -239,6 +239,28 @@ int ipl_estimate_device_bytes(const ipl_config* cfg, int64_t* bytes);
 /* 0: launch every step directly; 1 (default): replay captured graphs where the path supports it. */
 int ipl_set_graph(ipl_ctx* ctx, int32_t on);
 
+/* Open (inflow / outflow) boundary, replacing set_velocity1 + set_wall_boundary1
+ * (Karman_vortex_2D/Karman_vortexxz.cu:38-57, called at :391-392 and :425-430): at the end of
+ * every step the listed cells (canonical indices) are reset to the equilibrium of the given
+ * p, u, v, w (per cell), and the velocity of later steps starts from those values.  The driver
+ * fixes the inlet velocity with set_velocity1 and re-imposes each open cell's own p, u, which
+ * stay at their initial values while no force acts there; this call takes those values directly.
+ * n = 0 removes the boundary.  Replaces an earlier list. */
+int ipl_set_open_boundary(ipl_ctx* ctx, const int32_t* cells, int64_t n, const double* p, const double* u,
+                          const double* v, const double* w);
+
+/* ---- host-only set-up helpers (no device work) --------------------------------------------- */
+/* hydrostatic_pressure (basic.cu:239-252): the initial p of every calc cell (canonical order),
+ * p = 3 g (nz dz / 2 - z) / c^2 / 1000 with boussinesq != 0, else 3 g (center_z - z) / c^2,
+ * g = 9.81.  The reference's own f from that function is overwritten by its only caller with
+ * feq(p, u) (Zhou_soliton4B2.cu:137-145); pass p as the initial field to get the same start. */
+int ipl_hydrostatic_pressure(const ipl_config* cfg, int32_t boussinesq, double center_z, double* p_out, int64_t count);
+/* set_walln (set_bound.cu:8-23): the calc cells (canonical indices, ascending) whose neighbour in
+ * the face's direction is a ghost: face 1 = +x, 2 = +y, 3 = -x, 4 = -y, 5 = +z, 6 = -z (D2Q9:
+ * faces 2 and 4 are empty).  *count holds the capacity of out on entry and the number of cells
+ * on return; with out == NULL only the number is returned. */
+int ipl_wall_cells(const ipl_config* cfg, int32_t face, int32_t* out, int64_t* count);
+
 /* ---- host-only diagnostics ------------------------------------------------------------------
  * Builds the cell graph of cfg on the host (no device work) and copies one table out:
  * which = 0 neighbour table [k][n_calc + n_ghost] (ghosts after calc cells, basic.cu:203-237),

@@ -23,7 +23,7 @@ from ._abi import (BC_BOUNCE_BACK, BC_SLIP, COLL_MRT, COLL_SRT, F16, F32, F64, F
                    FLUX_REFERENCE, FLUX_X, FLUX_Y, FLUX_Z, PATH_AUTO, PATH_FUSED, PATH_PLAIN,
                    SCALAR_NONE, SCALAR_PHASE_FIELD, SCALAR_SALINITY, STAGES, IplError)
 
-__all__ = ["Simulation", "Batch", "estimate_device_bytes", "cases", "IplError", "F64", "F32", "F16", "PATH_AUTO",
+__all__ = ["Simulation", "Batch", "estimate_device_bytes", "hydrostatic_pressure", "wall_cells", "cases", "IplError", "F64", "F32", "F16", "PATH_AUTO",
            "PATH_PLAIN", "PATH_FUSED"]
 
 
@@ -81,6 +81,15 @@ class Simulation:
     def set_graph(self, on: bool):
         """Replay captured HIP graphs of the step (default on) or launch every step directly."""
         self._check(self._lib.ipl_set_graph(self._h, 1 if on else 0))
+
+    def set_open_boundary(self, cells, p, u, v, w):
+        """Inflow / outflow cells (canonical indices) reset to feq(p, u, v, w) after every step
+        (set_velocity1 + set_wall_boundary1); scalars or per-cell arrays; empty cells remove it."""
+        cells = np.ascontiguousarray(cells, dtype=np.int32).ravel()
+        n = cells.size
+        vals = [np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=np.float64), (n,))) for x in (p, u, v, w)]
+        self._check(self._lib.ipl_set_open_boundary(self._h, cells.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n,
+                                                    *[_dp(x) for x in vals]))
 
     def step_timed(self, n: int) -> float:
         """Runs n steps and returns their device time in milliseconds (HIP events)."""
@@ -197,6 +206,34 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def hydrostatic_pressure(case: "cases.Case", boussinesq: bool, center_z: float = 0.0, library: str = None) -> np.ndarray:
+    """The initial hydrostatic p of every calc cell (hydrostatic_pressure, basic.cu:239-252)."""
+    lib = _abi.load_library(library)
+    cfg, _ini, keep = case.to_c()
+    out = np.empty(case.n_cells)
+    rc = lib.ipl_hydrostatic_pressure(ctypes.byref(cfg), 1 if boussinesq else 0, float(center_z), _dp(out), out.size)
+    del keep
+    if rc != 0:
+        raise IplError(rc, "ipl_hydrostatic_pressure")
+    return out
+
+
+def wall_cells(case: "cases.Case", face: int, library: str = None) -> np.ndarray:
+    """Calc cells next to a wall face (set_walln, set_bound.cu:8-23): 1 +x, 2 +y, 3 -x, 4 -y, 5 +z, 6 -z."""
+    lib = _abi.load_library(library)
+    cfg, _ini, keep = case.to_c()
+    n = ctypes.c_int64()
+    rc = lib.ipl_wall_cells(ctypes.byref(cfg), int(face), None, ctypes.byref(n))
+    out = np.empty(max(n.value, 1), dtype=np.int32)
+    if rc == 0:
+        rc = lib.ipl_wall_cells(ctypes.byref(cfg), int(face), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                ctypes.byref(n))
+    del keep
+    if rc != 0:
+        raise IplError(rc, "ipl_wall_cells")
+    return out[:n.value]
 
 
 def estimate_device_bytes(case: "cases.Case", library: str = None) -> int:

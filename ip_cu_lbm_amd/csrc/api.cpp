@@ -295,6 +295,58 @@ int ipl_estimate_device_bytes(const ipl_config* cfg, int64_t* bytes) {
     return IPL_OK;
 }
 
+int ipl_set_open_boundary(ipl_ctx* ctx, const int32_t* cells, int64_t n, const double* p, const double* u,
+                          const double* v, const double* w) {
+    if (!ctx || n < 0 || (n > 0 && (!cells || !p || !u || !v || !w))) return IPL_ERR_ARGUMENT;
+    std::vector<double> vals((size_t)4 * n);
+    for (int64_t i = 0; i < n; ++i) {
+        vals[4 * i] = p[i]; vals[4 * i + 1] = u[i]; vals[4 * i + 2] = v[i]; vals[4 * i + 3] = w[i];
+    }
+    ipl::Status s = ctx->engine->set_open_boundary(cells, n, vals.data());
+    return s ? IPL_OK : fail(ctx, s);
+}
+
+int ipl_hydrostatic_pressure(const ipl_config* cfg, int32_t boussinesq, double center_z, double* p_out, int64_t count) {
+    if (!cfg || !p_out) return IPL_ERR_ARGUMENT;
+    ipl::Graph g;
+    std::string err;
+    if (!ipl::build_graph(*cfg, g, err)) {
+        g_create_error = err;
+        return IPL_ERR_INVALID_CONFIG;
+    }
+    if (count != g.N) return IPL_ERR_ARGUMENT;
+    const double grav = 9.81, c2 = g.c * g.c;
+    for (int64_t i = 0; i < g.N; ++i) {
+        const double z = g.pos[2][i];
+        p_out[i] = boussinesq ? 3.0 * grav * (cfg->nz / 2.0 * cfg->dz - z) / c2 / 1000.0 : 3.0 * grav * (center_z - z) / c2;
+    }
+    return IPL_OK;
+}
+
+int ipl_wall_cells(const ipl_config* cfg, int32_t face, int32_t* out, int64_t* count) {
+    if (!cfg || !count || face < 1 || face > 6) return IPL_ERR_ARGUMENT;
+    ipl::Graph g;
+    std::string err;
+    if (!ipl::build_graph(*cfg, g, err)) {
+        g_create_error = err;
+        return IPL_ERR_INVALID_CONFIG;
+    }
+    // directions of set_walln (set_bound.cu:11-13); the rest direction never has a ghost
+    static const int d9[7] = {0, 1, 0, 3, 0, 2, 4}, d3[7] = {0, 1, 2, 3, 4, 5, 6};
+    const int dir = g.Q == 9 ? d9[face] : d3[face];
+    int64_t m = 0;
+    for (int64_t i = 0; i < g.N; ++i)
+        if (g.neib[(size_t)dir * g.Ntot + i] >= g.N) {
+            if (out) {
+                if (m >= *count) return IPL_ERR_ARGUMENT;
+                out[m] = (int32_t)i;
+            }
+            ++m;
+        }
+    *count = m;
+    return IPL_OK;
+}
+
 int ipl_build_tables(const ipl_config* cfg, int32_t which, void* out, int64_t* count) {
     if (!cfg || !count) return IPL_ERR_ARGUMENT;
     ipl::Graph g;

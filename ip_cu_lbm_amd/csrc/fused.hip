@@ -23,6 +23,24 @@ using fusedk::FGeom;
 using fusedk::FParams;
 using fusedk::FTables;
 
+// open boundary (set_wall_boundary1, Karman_vortexxz.cu:38-49): f of each listed cell (internal
+// index) <- the equilibrium of its (p, u, v, w)
+template <typename ST, int Q>
+__global__ void k_open_boundary(ST* __restrict__ f, uint32_t ncell, const int32_t* __restrict__ idx,
+                                const double* __restrict__ val, int64_t n, double c) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t cell = (uint32_t)idx[i];
+    const double p = val[4 * i], u = val[4 * i + 1], v = val[4 * i + 2], w = val[4 * i + 3];
+    const double c2 = c * c, u2 = (u * u + v * v + w * w) / c2;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const Dir e = Lattice<Q>::c(k);
+        const double t = (e.x * c * u + e.y * c * v + e.z * c * w) / c2;
+        f[(uint32_t)k * ncell + cell] = (ST)(Lattice<Q>::w(k) * (p + 3.0 * t + 4.5 * t * t - 1.5 * u2));
+    }
+}
+
 // deterministic FP64 volume-weighted sum: fixed grid of partials, fixed-order trees
 __global__ void k_wsum(const double* field, int64_t n, const double* vol, double* partial) {
     __shared__ double sh[256];
@@ -252,6 +270,9 @@ public:
                 Status r = ib_.substep(fb_, gb_, Fb_, std::is_same<ST, double>::value ? IPL_F64 : IPL_F32);
                 if (!r) return r;
             }
+            if (ob_n_ > 0)
+                IPL_LAUNCH((k_open_boundary<ST, Q>), (unsigned)((ob_n_ + 255) / 256), 256, 0, stream, fb_,
+                           (uint32_t)geo_.ncell, ob_idx_, ob_val_, ob_n_, gr_.c);
             std::swap(fa_, fb_);
             std::swap(ga_, gb_);
             std::swap(Fa_, Fb_);
@@ -358,6 +379,30 @@ public:
     }
     Status ib_body_state(double* out) override { return ib_.body_state(out); }
     Status ib_read_markers(int which, double* out, size_t count) override { return ib_.read_markers(which, out, count); }
+
+    Status set_open_boundary(const int32_t* cells, int64_t n, const double* vals) override {
+        if (std::is_same<ST, __half>::value) return Status::err(IPL_ERR_UNSUPPORTED, "open boundary needs FP64 or FP32 storage");
+        std::vector<int32_t> idx((size_t)n);
+        for (int64_t i = 0; i < n; ++i) {
+            if (cells[i] < 0 || cells[i] >= gr_.N) return Status::err(IPL_ERR_ARGUMENT, "open-boundary cell out of range");
+            idx[i] = (int32_t)perm_[cells[i]];
+        }
+        IPL_HIP_TRY(hipStreamSynchronize(stream));
+        if (n > ob_cap_) {
+            Status r;
+            if (!(r = mem_.alloc_n(&ob_idx_, (size_t)n))) return r;
+            if (!(r = mem_.alloc_n(&ob_val_, (size_t)4 * n))) return r;
+            ob_cap_ = n;
+        }
+        if (n > 0) {
+            IPL_HIP_TRY(hipMemcpyAsync(ob_idx_, idx.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+            IPL_HIP_TRY(hipMemcpyAsync(ob_val_, vals, (size_t)4 * n * sizeof(double), hipMemcpyHostToDevice, stream));
+            IPL_HIP_TRY(hipStreamSynchronize(stream));
+        }
+        ob_n_ = n;
+        drop_graph();  // the captured steps hold the previous list
+        return Status::ok();
+    }
     Status readout() {
         return dispatch([&](auto q, auto qs) { return this->template readout_q<decltype(q)::value, decltype(qs)::value>(); });
     }
@@ -500,6 +545,10 @@ private:
     bool use_graph_ = true;
 #endif
     unsigned block_ = 64;
+
+    int32_t* ob_idx_ = nullptr;   // open-boundary cells (internal index) and their (p, u, v, w)
+    double* ob_val_ = nullptr;
+    int64_t ob_n_ = 0, ob_cap_ = 0;
 
     void drop_graph() {
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);

@@ -53,6 +53,10 @@ def load():
         lib.orc_ib_peskin4.argtypes = [c_double]
         lib.orc_ib_peskin4.restype = c_double
         lib.orc_ib_rotation.argtypes = [DP, DP]
+        lib.orc_set_open_boundary.argtypes = [c_void_p, POINTER(c_int32), c_int, DP, DP, DP, DP]
+        lib.orc_hydrostatic_pressure.argtypes = [c_void_p, c_int, c_double, DP]
+        lib.orc_wall_cells.argtypes = [c_void_p, c_int, POINTER(c_int32)]
+        lib.orc_wall_cells.restype = c_int
         _LIB = lib
     return _LIB
 
@@ -151,6 +155,24 @@ class Oracle:
         out = np.zeros(3)
         self._lib.orc_global_sums(self._h, _dp(out))
         return out
+
+    def set_open_boundary(self, cells, p, u, v, w):
+        cells = np.ascontiguousarray(cells, dtype=np.int32).ravel()
+        n = cells.size
+        vals = [np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=np.float64), (n,))) for x in (p, u, v, w)]
+        rc = self._lib.orc_set_open_boundary(self._h, cells.ctypes.data_as(POINTER(c_int32)), n, *[_dp(x) for x in vals])
+        if rc != 0:
+            raise ValueError("open-boundary cell out of range")
+
+    def hydrostatic_pressure(self, boussinesq: bool, center_z: float = 0.0) -> np.ndarray:
+        out = np.empty(self.n_calc)
+        self._lib.orc_hydrostatic_pressure(self._h, 1 if boussinesq else 0, float(center_z), _dp(out))
+        return out
+
+    def wall_cells(self, face: int) -> np.ndarray:
+        out = np.empty(max(self.n_calc, 1), dtype=np.int32)
+        m = self._lib.orc_wall_cells(self._h, int(face), out.ctypes.data_as(POINTER(c_int32)))
+        return out[:m]
 
     def table(self, which: int) -> np.ndarray:
         n = self._lib.orc_table(self._h, which, None, 0)

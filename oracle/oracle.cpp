@@ -218,6 +218,10 @@ struct Oracle {
         std::vector<int> lattice_id;                          // M
         double posB[3], velB[3], angleVB[3], quat[4], quatold[4], quaS[9], FB[3], Torque[3];
     } ib;
+
+    // open boundary: cells reset to feq(p, u, v, w) at the end of every step
+    std::vector<int> ob_cells;
+    std::vector<double> ob_val;   // (p, u, v, w) per cell
 };
 
 inline int axis_dir(const Lattice& L, int axis, int sgn) {
@@ -1347,9 +1351,28 @@ void stage(Oracle& o, int s) {
 }
 
 // One step in driver order (LockExchange.cu:220-278; Zhou_soliton4B1.cu:220-277)
+// set_velocity1 + set_wall_boundary1 (Karman_vortexxz.cu:38-57): the macroscopic fields of the
+// listed cells take the prescribed values and their populations the equilibrium of them (the
+// reference's f form, collision.cu:3-14)
+void open_boundary(Oracle& o) {
+    const int Q = o.Q;
+    const double c2 = o.c * o.c;
+    for (size_t n = 0; n < o.ob_cells.size(); ++n) {
+        const int id = o.ob_cells[n];
+        const double* q = &o.ob_val[4 * n];
+        o.p[id] = q[0]; o.u[id] = q[1]; o.v[id] = q[2]; o.w[id] = q[3];
+        for (int k = 0; k < Q; ++k) {
+            const double t = (o.L.cx[k] * o.c * o.u[id] + o.L.cy[k] * o.c * o.v[id] + o.L.cz[k] * o.c * o.w[id]) / c2;
+            o.fl.f[(size_t)id * Q + k] = o.L.w[k] * (o.p[id] + 3.0 * t + 4.5 * t * t -
+                1.5 * (o.u[id] * o.u[id] + o.v[id] * o.v[id] + o.w[id] * o.w[id]) / c2);
+        }
+    }
+}
+
 void step(Oracle& o) {
     for (int s = IPL_STAGE_EQU_F; s <= IPL_STAGE_RESET_F; ++s) stage(o, s);
     if (o.ib.active) ib_substep(o);
+    if (!o.ob_cells.empty()) open_boundary(o);
 }
 
 }  // namespace
@@ -1457,6 +1480,41 @@ int64_t orc_table(void* h, int which, int32_t* out, int64_t count) {
 }
 
 void orc_lagrange(const double* x, double xa, int n, double* coeff) { lagrange(coeff, x, xa, n); }
+
+int orc_set_open_boundary(void* h, const int* cells, int n, const double* p, const double* u, const double* v,
+                          const double* w) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    o.ob_cells.assign(cells, cells + n);
+    o.ob_val.resize((size_t)4 * n);
+    for (int i = 0; i < n; ++i) {
+        if (cells[i] < 0 || cells[i] >= o.N) return IPL_ERR_ARGUMENT;
+        o.ob_val[4 * i] = p[i]; o.ob_val[4 * i + 1] = u[i]; o.ob_val[4 * i + 2] = v[i]; o.ob_val[4 * i + 3] = w[i];
+    }
+    return 0;
+}
+
+// hydrostatic_pressure (basic.cu:239-252): p of every calc cell from its z position
+int orc_hydrostatic_pressure(void* h, int boussinesq, double center_z, double* p_out) {
+    const Oracle& o = *static_cast<Oracle*>(h);
+    for (int i = o.N - 1; i >= 0; --i) {
+        if (boussinesq) p_out[i] = 3 * 9.81 * (o.cfg.nz / 2.0 * o.dz - o.pos[2][i]) / std::pow(o.c, 2) / 1000;
+        else p_out[i] = 3 * 9.81 * (center_z - o.pos[2][i]) / std::pow(o.c, 2);
+    }
+    return 0;
+}
+
+// set_walln (set_bound.cu:8-23) on the oracle's neighbour table; returns the number of cells
+int orc_wall_cells(void* h, int face, int* out) {
+    const Oracle& o = *static_cast<Oracle*>(h);
+    int dir1 = 1, dir2, dir3 = 3, dir4, dir5, dir6;
+    if (o.Q == 9) { dir2 = 0; dir4 = 0; dir5 = 2; dir6 = 4; }
+    else { dir2 = 2; dir4 = 4; dir5 = 5; dir6 = 6; }
+    const int dirs[7] = {0, dir1, dir2, dir3, dir4, dir5, dir6};
+    int m = 0;
+    for (int i = 0; i < o.N; ++i)
+        if (o.neib[(size_t)i * o.Q + dirs[face]] >= o.N) out[m++] = i;
+    return m;
+}
 
 int orc_ib_attach(void* h, const ipl_ib_config* cfg, char* err, int errlen) {
     Oracle& o = *static_cast<Oracle*>(h);
