@@ -93,7 +93,12 @@ typedef struct ipl_config {
     int32_t check_finite;     /* 1: ipl_step checks p and u for NaN/Inf after the last step */
     double pf_tau;            /* phase-field relaxation time; the reference hard-codes 0.58
                                  (collision.cu:218), i.e. mobility (pf_tau - 0.5) c^2 dt / 3 */
-    int32_t reserved[6];
+    int32_t wall_function;    /* 1: Spalding log-law wall shear on the cells next to every wall,
+                                 added to F before Force (wall_function, collision.cu:28-55, called
+                                 for the six faces at LockExchange.cu:214-219); fused path */
+    int32_t reserved0;
+    double wall_nu;           /* kinematic viscosity of the log law (items[IDX_nu]) */
+    int32_t reserved[2];
 } ipl_config;
 
 /* Initial state, canonical cell order, length = number of calc cells.  NULL entries default to

@@ -86,6 +86,8 @@ class Case:
     f16_scale: float = 0.0
     check_finite: int = 0
     pf_tau: float = 0.58
+    wall_function: int = 0
+    wall_nu: float = 0.0
     solid: Optional[np.ndarray] = None
     init: Dict[str, np.ndarray] = dataclasses.field(default_factory=dict)
     steps: int = 100
@@ -148,6 +150,8 @@ class Case:
         cfg.f16_scale = self.f16_scale
         cfg.check_finite = self.check_finite
         cfg.pf_tau = self.pf_tau
+        cfg.wall_function = self.wall_function
+        cfg.wall_nu = self.wall_nu
         ini = _abi.IplInit()
         for key in ("p", "u", "v", "w", "S", "phi", "tau", "taus"):
             if key in self.init and self.init[key] is not None:

@@ -106,6 +106,7 @@ public:
         G.grad_phi = cfg_.scalar == IPL_SCALAR_PHASE_FIELD;
         G.need_grad = G.grad_rho || G.grad_phi;
         G.half_force = 1;
+        G.wall_fn = cfg_.wall_function != 0;
         double scale = 1.0;
         if (std::is_same<ST, __half>::value) scale = cfg_.f16_scale > 0 ? cfg_.f16_scale : 8192.0;
         G.scale = (float)scale;
@@ -130,6 +131,7 @@ public:
         P_.inv_pfW = cfg_.pf_thickness > 0 ? 1.0 / cfg_.pf_thickness : 0.0;
         P_.pf_itau = cfg_.pf_tau > 0 ? 1.0 / cfg_.pf_tau : 0.0;
         P_.pf_fcoef = cfg_.pf_tau - 0.5;
+        P_.wall_nu = cfg_.wall_nu;
         K_ = fusedk::make_coef<MT>(P_);
         const int Q = gr_.Q, Qs = gr_.Qs;
         const int64_t NC = G.ncell, N = gr_.N;
@@ -216,6 +218,15 @@ public:
             IPL_HIP_TRY(hipMemcpyAsync(diw, iw.data(), G.n[0] * sizeof(MT), hipMemcpyHostToDevice, st));
             IPL_HIP_TRY(hipStreamSynchronize(st));
             tb_.inv_wx2 = diw;
+        }
+        for (int a = 0; a < 3; ++a) {  // cell widths per axis index (wall function)
+            std::vector<MT> wv(G.n[a]);
+            for (int x = 0; x < G.n[a]; ++x) wv[x] = (MT)cfg_widths_[a][x];
+            MT* dwv;
+            if (!(r = mem_.alloc_n(&dwv, G.n[a]))) return r;
+            IPL_HIP_TRY(hipMemcpyAsync(dwv, wv.data(), G.n[a] * sizeof(MT), hipMemcpyHostToDevice, st));
+            IPL_HIP_TRY(hipStreamSynchronize(st));
+            tb_.wid[a] = dwv;
         }
         // per-cell volume (FP64) for the conservation sums, internal order
         std::vector<double> vol(NC);

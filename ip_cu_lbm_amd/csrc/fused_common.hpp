@@ -22,6 +22,7 @@ struct FGeom {
     int need_grad;   // the scalar-field pass runs (rho and/or phi gradients are needed)
     int grad_rho;    // rho varies: the pressure-gradient term of Force is nonzero
     int grad_phi;    // phase field: grad phi and its Laplacian (Force, col_PF)
+    int wall_fn;     // wall-function shear on the cells next to walls (collision.cu:28-55)
     int half_force;  // u = sum e c f + F_old dt / (2 rho) (update_rho); 0 after an immersed-boundary
                      // sub-step, whose update_velIBM leaves u = sum e f (ibm.cu:356-360)
     int scalar;
@@ -35,6 +36,7 @@ struct FTables {
     const MT* rdel;    // [ns] dz / width along s
     const MT* invd[3]; // per axis index: 1 / (pos[+1] - pos[-1] + dz/1000), seam-corrected
     const MT* inv_wx2; // per x index: 1 / width_x^2
+    const MT* wid[3];  // per axis index: cell width (wall function)
 };
 
 template <typename ST, typename MT>
@@ -60,6 +62,7 @@ struct FParams {
     double press;                            // c^2 / 3
     double st_beta, kap, inv_dz2;            // 4 * 12 sigma / W, 3 sigma W / 2, 1/dz^2
     double inv_dt, inv_pfW, pf_itau, pf_fcoef; // 1/dt, 1/W, 1/tau_phi, tau_phi - 1/2
+    double wall_nu;                          // wall function: kinematic viscosity
 };
 
 // The coefficients of the step in the arithmetic type, cast once on the host (the kernels then
@@ -69,6 +72,7 @@ struct FCoef {
     MT c, inv_c, inv_c2, half_dt, fk_coef, dt;
     MT uc;                             // velocity scale of sum e f: c, or 1 after an IB sub-step
     MT itau, om_s, rnorm[27];
+    MT wall_nu, dz;                    // kinematic viscosity of the wall function's log law; dz
     MT rho_L, drho, beta_s, gravity;   // drho = rho_H - rho_L (in MT)
     MT press, st_beta, kap, inv_dz2;
     MT inv_dt, inv_pfW, pf_itau, pf_fcoef;
@@ -87,6 +91,7 @@ inline FCoef<MT> make_coef(const FParams& P) {
     k.press = (MT)P.press; k.st_beta = (MT)P.st_beta; k.kap = (MT)P.kap; k.inv_dz2 = (MT)P.inv_dz2;
     k.inv_dt = (MT)P.inv_dt; k.inv_pfW = (MT)P.inv_pfW; k.pf_itau = (MT)P.pf_itau; k.pf_fcoef = (MT)P.pf_fcoef;
     k.boussinesq = P.boussinesq; k.collision = P.collision;
+    k.wall_nu = (MT)P.wall_nu; k.dz = (MT)P.dz;
     return k;
 }
 

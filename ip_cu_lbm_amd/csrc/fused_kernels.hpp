@@ -545,6 +545,59 @@ __device__ inline uint32_t nb_index(const FGeom& G, const LineNb& nb, int i) {
     else return nb.template base<0, OFF>() + (uint32_t)i;
 }
 
+// wall_function (collision.cu:28-55) for one wall: u_tau of Spalding's law by 15 Newton steps
+// from the cell's speed and its distance y_dist to the wall, then the shear on the two
+// tangential components (vt1, vt2; vn is the normal one) into Ft1, Ft2
+template <typename MT>
+__device__ inline void spalding_shear(MT vt1, MT vt2, MT vn, MT y_dist, MT wid_x, MT wid_y, MT dz, MT rho, MT nu,
+                                      MT& Ft1, MT& Ft2) {
+    const MT volume = wid_x * wid_y * dz;
+    const MT abs_u = sqrt(vt1 * vt1 + vt2 * vt2 + vn * vn);
+    const MT eta = abs_u * MT(1e-7) + MT(1e-20);
+    const MT kap = MT(0.41), Beta = MT(5.5);
+    const MT area = volume / (y_dist * MT(2.0));
+    const MT ekb = exp(-kap * Beta);
+    MT utau = abs_u * MT(0.11);
+    for (int i = 0; i < 15; ++i) {
+        const MT tmp = kap * abs_u / (utau + eta);
+        const MT et = exp(tmp);
+        const MT eps = -((abs_u / (utau + eta) - y_dist * utau / nu +
+                          ekb * (et - MT(1.0) - tmp - tmp * tmp / MT(2.0) - tmp * tmp * tmp / MT(6.0))) /
+                         (-y_dist / nu - abs_u / (utau * utau + eta) +
+                          ekb * tmp / (utau + eta) * (-et + MT(1.0) + tmp + tmp * tmp / MT(2.0)) + eta));
+        utau += eps;
+    }
+    const MT shear = utau * utau * rho * (area / volume) / (abs_u + eta);
+    Ft1 += -vt1 * shear;
+    Ft2 += -vt2 * shear;
+}
+
+// the driver's six wall-function calls (LockExchange.cu:214-219) for the cell at index ii of the
+// line (c1, c2): faces +x, -x, +z, -z, +y, -y, each on the cells next to a (non-periodic) wall
+template <typename MT, int S, bool D3>
+__device__ inline void wall_shear(const FGeom& G, const FTables<MT>& tb, const FCoef<MT>& P, int ii, int c1, int c2,
+                                  MT u, MT v, MT w, MT rho, MT& Fx, MT& Fy, MT& Fz) {
+    constexpr int o1 = LineAxes<S>::o1;
+    int c[3];
+    c[S] = ii;
+    c[o1] = c1;
+    c[LineAxes<S>::o2] = c2;
+    const MT wx = *elem(tb.wid[0], (uint32_t)c[0]), wy = *elem(tb.wid[1], (uint32_t)c[1]);
+    const MT dz = P.dz;
+    if (!G.periodic[0]) {
+        const int hits = (c[0] == G.n[0] - 1) + (c[0] == 0);
+        for (int h = 0; h < hits; ++h) spalding_shear<MT>(v, w, u, wx / MT(2.0), wx, wy, dz, rho, P.wall_nu, Fy, Fz);
+    }
+    if (!G.periodic[2]) {
+        const int hits = (c[2] == G.n[2] - 1) + (c[2] == 0);
+        for (int h = 0; h < hits; ++h) spalding_shear<MT>(u, v, w, dz / MT(2.0), wx, wy, dz, rho, P.wall_nu, Fx, Fy);
+    }
+    if (D3 && !G.periodic[1]) {
+        const int hits = (c[1] == G.n[1] - 1) + (c[1] == 0);
+        for (int h = 0; h < hits; ++h) spalding_shear<MT>(u, w, v, wy / MT(2.0), wx, wy, dz, rho, P.wall_nu, Fx, Fz);
+    }
+}
+
 // (q[+] - q[-]) / (pos[+] - pos[-] + dz/1000) along axis A (Force, collision.cu:66-69); the
 // reciprocal, with the period added across a periodic seam, is a host table
 template <typename MT, int S, int A>
@@ -617,6 +670,7 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
         // ---- Force (collision.cu:57-112) ----
         constexpr bool d3 = Lattice<Q>::dim == 3;
         MT Fx = 0, Fy = 0, Fz = 0;
+        if (G.wall_fn) wall_shear<MT, S, d3>(G, tb, P, ii, c1, c2, u, v, w, rho, Fx, Fy, Fz);
         MT gpx = 0, gpy = 0, gpz = 0;  // grad phi (col_PF reuses it)
         if (G.grad_phi) {
             gpx = grad_axis<MT, S, 0>(G, nb, tb, b.phi_n, c1, c2, ii);

@@ -57,6 +57,8 @@ def load():
         lib.orc_hydrostatic_pressure.argtypes = [c_void_p, c_int, c_double, DP]
         lib.orc_wall_cells.argtypes = [c_void_p, c_int, POINTER(c_int32)]
         lib.orc_wall_cells.restype = c_int
+        lib.orc_spalding_utau.argtypes = [c_double, c_double, c_double]
+        lib.orc_spalding_utau.restype = c_double
         _LIB = lib
     return _LIB
 
@@ -179,6 +181,10 @@ class Oracle:
         out = np.empty(n, dtype=np.int32)
         self._lib.orc_table(self._h, which, out.ctypes.data_as(POINTER(c_int32)), n)
         return out
+
+
+def spalding_utau(abs_u: float, y_dist: float, nu: float) -> float:
+    return load().orc_spalding_utau(float(abs_u), float(y_dist), float(nu))
 
 
 def peskin4(r: float) -> float:

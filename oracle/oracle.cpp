@@ -1369,7 +1369,63 @@ void open_boundary(Oracle& o) {
     }
 }
 
+// u_tau of Spalding's law y+ = u+ + e^{-kB}(e^{k u+} - 1 - k u+ - (k u+)^2/2 - (k u+)^3/6),
+// u+ = |u| / u_tau, y+ = y u_tau / nu: 15 Newton steps from 0.11 |u| (collision.cu:37-50)
+double spalding_utau(double abs_u, double eta, double y_dist, double nu) {
+    const double kap = 0.41, Beta = 5.5;
+    double utau = abs_u * 0.11;
+    for (int i = 0; i < 15; ++i) {
+        const double tmp = kap * abs_u / (utau + eta);
+        const double eps = -((abs_u / (utau + eta) - y_dist * utau / nu +
+                              std::exp(-kap * Beta) * (std::exp(tmp) - 1.0 - tmp - std::pow(tmp, 2) / 2.0 - std::pow(tmp, 3) / 6.0)) /
+                             (-y_dist / nu - abs_u / (std::pow(utau, 2) + eta) +
+                              std::exp(-kap * Beta) * tmp / (utau + eta) * (-std::exp(tmp) + 1.0 + tmp + std::pow(tmp, 2) / 2.0) + eta));
+        utau += eps;
+    }
+    return utau;
+}
+
+// wall_function (collision.cu:28-55) on one face's wall cells: Spalding's law solved for u_tau by
+// 15 Newton steps, wall shear added to the two tangential force components.  (x, y, z) picks the
+// wall normal; vx, vy are the tangential velocities, vz the normal one; Fa, Fb receive the shear.
+void wall_function(Oracle& o, int x, int y, int z, const std::vector<int>& wall, const std::vector<double>& vx,
+                   const std::vector<double>& vy, const std::vector<double>& vz, std::vector<double>& Fa,
+                   std::vector<double>& Fb) {
+    for (size_t n = 0; n < wall.size(); ++n) {
+        const int id = wall[n];
+        const double volume = o.del[0][id] * o.del[1][id] * o.dz;
+        const double abs_u = std::sqrt(vx[id] * vx[id] + vy[id] * vy[id] + vz[id] * vz[id]);
+        const double eta = abs_u * std::pow(10.0, -7) + std::pow(10.0, -20);
+        const double kap = 0.41, Beta = 5.5;
+        const double y_dist = x * o.del[0][id] / 2.0 + y * o.del[1][id] / 2.0 + z * o.dz / 2.0;
+        const double area = volume / (y_dist * 2.0);
+        (void)kap; (void)Beta;
+        const double utau = spalding_utau(abs_u, eta, y_dist, o.cfg.wall_nu);
+        Fa[id] += -vx[id] / (abs_u + eta) * std::pow(utau, 2.0) * o.rho[id] * (area / volume);
+        Fb[id] += -vy[id] / (abs_u + eta) * std::pow(utau, 2.0) * o.rho[id] * (area / volume);
+    }
+}
+
+// the driver's six calls (LockExchange.cu:214-219): faces +x, -x, +z, -z, +y, -y
+void wall_functions(Oracle& o) {
+    int dirs[7];
+    dirs[1] = 1; dirs[3] = 3;
+    if (o.Q == 9) { dirs[2] = 0; dirs[4] = 0; dirs[5] = 2; dirs[6] = 4; }
+    else { dirs[2] = 2; dirs[4] = 4; dirs[5] = 5; dirs[6] = 6; }
+    std::vector<int> wall[7];
+    for (int f = 1; f <= 6; ++f)
+        for (int i = 0; i < o.N; ++i)
+            if (o.neib[(size_t)i * o.Q + dirs[f]] >= o.N) wall[f].push_back(i);
+    wall_function(o, 1, 0, 0, wall[1], o.v, o.w, o.u, o.Fy, o.Fz);
+    wall_function(o, 1, 0, 0, wall[3], o.v, o.w, o.u, o.Fy, o.Fz);
+    wall_function(o, 0, 0, 1, wall[5], o.u, o.v, o.w, o.Fx, o.Fy);
+    wall_function(o, 0, 0, 1, wall[6], o.u, o.v, o.w, o.Fx, o.Fy);
+    wall_function(o, 0, 1, 0, wall[2], o.u, o.w, o.v, o.Fx, o.Fz);
+    wall_function(o, 0, 1, 0, wall[4], o.u, o.w, o.v, o.Fx, o.Fz);
+}
+
 void step(Oracle& o) {
+    if (o.cfg.wall_function) wall_functions(o);
     for (int s = IPL_STAGE_EQU_F; s <= IPL_STAGE_RESET_F; ++s) stage(o, s);
     if (o.ib.active) ib_substep(o);
     if (!o.ob_cells.empty()) open_boundary(o);
@@ -1480,6 +1536,10 @@ int64_t orc_table(void* h, int which, int32_t* out, int64_t count) {
 }
 
 void orc_lagrange(const double* x, double xa, int n, double* coeff) { lagrange(coeff, x, xa, n); }
+
+double orc_spalding_utau(double abs_u, double y_dist, double nu) {
+    return spalding_utau(abs_u, abs_u * std::pow(10.0, -7) + std::pow(10.0, -20), y_dist, nu);
+}
 
 int orc_set_open_boundary(void* h, const int* cells, int n, const double* p, const double* u, const double* v,
                           const double* w) {

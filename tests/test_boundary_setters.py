@@ -89,3 +89,22 @@ def test_open_boundary_keeps_uniform_stream(oracle_lib):
         t = ex * U
         assert np.abs(f[k, cells] - w * (3 * t + 4.5 * t * t - 1.5 * U * U)).max() < 1e-15
     o.close()
+
+
+@pytest.mark.parametrize("abs_u,y,nu", [(0.01, 0.5, 1e-3), (0.05, 0.5, 1e-4), (0.002, 1.0, 1e-5), (0.1, 0.25, 5e-5)])
+def test_wall_function_newton_solves_spalding_law(oracle_lib, abs_u, y, nu):
+    """The wall function's u_tau (15 Newton steps, collision.cu:37-50) is the root of Spalding's
+    law y+ = u+ + e^{-kB}(e^{k u+} - 1 - k u+ - (k u+)^2/2 - (k u+)^3/6), k = 0.41, B = 5.5, as
+    scipy's bracketing solver finds it, within the offset of the reference's regularisation
+    (eta = 1e-7 |u| added to u_tau in u+ and to the derivative): 2e-6 relative, for y+ from 2 to
+    37 where 15 steps converge (far into the log layer they do not, in the reference as here)."""
+    from scipy.optimize import brentq
+    k, B = 0.41, 5.5
+
+    def resid(ut):
+        up = abs_u / ut
+        return y * ut / nu - (up + np.exp(-k * B) * (np.exp(k * up) - 1 - k * up - (k * up) ** 2 / 2 - (k * up) ** 3 / 6))
+
+    ref = brentq(resid, 1e-8 * abs_u, abs_u, xtol=1e-16, rtol=1e-15)
+    got = oracle_lib.spalding_utau(abs_u, y, nu)
+    assert abs(got - ref) <= 2e-6 * ref, (got, ref)

@@ -77,3 +77,35 @@ def test_open_boundary_fp32_and_removal(oracle_lib):
         a.step(30)
         b.step(30)
         assert np.array_equal(a.read("u"), b.read("u"))
+
+
+@pytest.mark.parametrize("make", [
+    lambda st: cases.lock_exchange(nx=32, ny=6, nz=16, storage=st, path=PATH_FUSED).replace(periodic=(0, 0, 0)),
+    lambda st: cases.lattice_taylor_green(nx=48, nz=24, storage=st, path=PATH_FUSED).replace(periodic=(0, 0, 0)),
+    lambda st: cases.lock_exchange(nx=24, ny=6, nz=12, storage=st, path=PATH_FUSED).replace(
+        periodic=(0, 0, 0), flux_axes=_abi.FLUX_NONE, wall_bc=_abi.BC_BOUNCE_BACK),
+], ids=["le_stretched_z", "tg_2d_stretched_x", "le_uniform_bb"])
+@pytest.mark.parametrize("storage", [F64, F32])
+def test_wall_function_fp64(oracle_lib, make, storage):
+    """Spalding wall shear on the cells next to every wall (wall_function, collision.cu:28-55, the
+    driver's six calls at LockExchange.cu:214-219) vs the oracle: FP64 <= 1e-12, FP32 <= 1e-5."""
+    case = make(storage).replace(wall_function=1, wall_nu=0.005)
+    X, _, Z = case.coords()
+    case.init = dict(case.init, u=np.asarray(case.init.get("u", 0.0)) + 0.02 * np.cos(Z.ravel() / 3.0))
+    fields = ("p", "u", "v", "w") + (("S",) if case.scalar == _abi.SCALAR_SALINITY else ())
+    with Simulation(case) as sim:
+        sim.step(80)
+        g = {f: sim.read(f) for f in fields}
+    o = oracle_lib.Oracle(case)
+    o.step(80)
+    r = {f: o.read(f) for f in fields}
+    o.close()
+    vel = rel_l2(np.stack([g[c] for c in "uvw"]), np.stack([r[c] for c in "uvw"]))
+    errs = {"vel": vel, "p": rel_l2(g["p"], r["p"])}
+    if "S" in g:
+        errs["S"] = rel_l2(g["S"], r["S"])
+    print(case.name, storage, errs, flush=True)
+    tol = 1e-12 if storage == F64 else 1e-5
+    assert errs["vel"] <= tol and errs.get("S", 0.0) <= tol
+    if storage == F64:
+        assert errs["p"] <= tol
