@@ -98,7 +98,11 @@ typedef struct ipl_config {
                                  for the six faces at LockExchange.cu:214-219); fused path */
     int32_t reserved0;
     double wall_nu;           /* kinematic viscosity of the log law (items[IDX_nu]) */
-    int32_t reserved[2];
+    int32_t les;              /* 1: coherent-structure LES sets tau and tau_s of every cell after
+                                 each step (LES, basic.cu:285-339, called after update_rho at
+                                 Boegman2005run18IB.cu:339); fused path */
+    int32_t reserved1;
+    double mu_L, mu_H;        /* LES: dynamic viscosity of phi = 0 / phi = 1 (muL, muH) */
 } ipl_config;
 
 /* Initial state, canonical cell order, length = number of calc cells.  NULL entries default to

@@ -88,6 +88,9 @@ class Case:
     pf_tau: float = 0.58
     wall_function: int = 0
     wall_nu: float = 0.0
+    les: int = 0
+    mu_L: float = 0.0
+    mu_H: float = 0.0
     solid: Optional[np.ndarray] = None
     init: Dict[str, np.ndarray] = dataclasses.field(default_factory=dict)
     steps: int = 100
@@ -152,6 +155,9 @@ class Case:
         cfg.pf_tau = self.pf_tau
         cfg.wall_function = self.wall_function
         cfg.wall_nu = self.wall_nu
+        cfg.les = self.les
+        cfg.mu_L = self.mu_L
+        cfg.mu_H = self.mu_H
         ini = _abi.IplInit()
         for key in ("p", "u", "v", "w", "S", "phi", "tau", "taus"):
             if key in self.init and self.init[key] is not None:

@@ -286,6 +286,7 @@ int ipl_estimate_device_bytes(const ipl_config* cfg, int64_t* bytes) {
         int64_t b = 2 * (int64_t)(Q + Qs) * s * n + 6 * m * n + 8 * n + 7 * 8 * n;
         if (cfg->scalar != IPL_SCALAR_NONE) b += 2 * m * n;
         if (cfg->scalar == IPL_SCALAR_PHASE_FIELD) b += 4 * m * n;
+        if (cfg->les) b += 6 * m * n;  // velocity, viscosity, tau and tau_s rows
         *bytes = b + per_axis + (1 << 20);
     } else {
         // plain: five population sets per field, neighbour table, ghosts (~n^(2/3) of n), fields

@@ -107,6 +107,7 @@ public:
         G.need_grad = G.grad_rho || G.grad_phi;
         G.half_force = 1;
         G.wall_fn = cfg_.wall_function != 0;
+        G.les = cfg_.les != 0;
         double scale = 1.0;
         if (std::is_same<ST, __half>::value) scale = cfg_.f16_scale > 0 ? cfg_.f16_scale : 8192.0;
         G.scale = (float)scale;
@@ -132,6 +133,8 @@ public:
         P_.pf_itau = cfg_.pf_tau > 0 ? 1.0 / cfg_.pf_tau : 0.0;
         P_.pf_fcoef = cfg_.pf_tau - 0.5;
         P_.wall_nu = cfg_.wall_nu;
+        for (int r = 0; r < gr_.Q; ++r) P_.inv_n2[r] = 1.0 / moment_row_norm2(gr_.Q, r);
+        P_.mu_L = cfg_.mu_L; P_.mu_H = cfg_.mu_H;
         K_ = fusedk::make_coef<MT>(P_);
         const int Q = gr_.Q, Qs = gr_.Qs;
         const int64_t NC = G.ncell, N = gr_.N;
@@ -235,6 +238,20 @@ public:
             vol[perm_[id]] = cfg_widths_[0][x] * cfg_widths_[1][y] * cfg_widths_[2][z] / (gr_.dz * gr_.dz * gr_.dz);
         }
         if (!(r = mem_.alloc_n(&vol_, (size_t)NC))) return r;
+        if (G.les) {  // LES rows; tau, tau_s start from the initial per-cell values
+            if (!(r = mem_.alloc_n(&les_vel_, (size_t)3 * NC))) return r;
+            if (!(r = mem_.alloc_n(&les_nu_, (size_t)NC))) return r;
+            if (!(r = mem_.alloc_n(&les_tau_, (size_t)NC))) return r;
+            if (!(r = mem_.alloc_n(&les_taus_, (size_t)NC))) return r;
+            MT* dst[2] = {les_tau_, les_taus_};
+            const std::vector<double>* src[2] = {&hs.tau, &hs.taus};
+            for (int q = 0; q < 2; ++q) {
+                std::vector<MT> t(NC, MT(1));
+                for (int64_t id = 0; id < N; ++id) t[perm_[id]] = (MT)(*src[q])[id];
+                IPL_HIP_TRY(hipMemcpyAsync(dst[q], t.data(), NC * sizeof(MT), hipMemcpyHostToDevice, st));
+                IPL_HIP_TRY(hipStreamSynchronize(st));
+            }
+        }
         if (!(r = mem_.alloc_n(&partial_, (size_t)3 * 1024 + 3))) return r;
         IPL_HIP_TRY(hipMemcpyAsync(vol_, vol.data(), NC * sizeof(double), hipMemcpyHostToDevice, st));
         IPL_HIP_TRY(hipStreamSynchronize(st));
@@ -271,11 +288,15 @@ public:
             b.F_old = Fa_; b.F_new = Fb_;
             b.rho_n = rho_n_; b.phi_n = phi_n_;
             b.phiold = phiold_; b.uold = uold_; b.vold = vold_; b.wold = wold_;
+            b.tau = les_tau_; b.taus = les_taus_;
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 0], stream));
             if (geo_.need_grad) fusedk::fused_launch_scalar_fields<ST, MT, QS>(geo_, ga_, rho_n_, phi_n_, K_, stream);
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 1], stream));
             fusedk::fused_launch_step<ST, MT, Q, QS>(geo_, tb_, b, K_, gr_.npts, block_, lds_bytes_, stream);
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 2], stream));
+            if (geo_.les)  // LES after update_rho, before the IB sub-step (Boegman2005run18IB.cu:338-339)
+                fusedk::fused_launch_les<ST, MT, Q, QS>(geo_, tb_, fb_, gb_, Fb_, les_vel_, les_nu_, les_tau_, les_taus_, K_,
+                                                         block_, stream);
             if (ib_.active()) {
                 // IB sub-step on f^{n+1}, with the Force field of this step (SphereFall.cu:372-383)
                 Status r = ib_.substep(fb_, gb_, Fb_, std::is_same<ST, double>::value ? IPL_F64 : IPL_F32);
@@ -374,6 +395,8 @@ public:
 
     Status ib_attach(const ipl_ib_config& cfg) override {
         if (std::is_same<ST, __half>::value) return Status::err(IPL_ERR_UNSUPPORTED, "the immersed boundary needs FP64 or FP32 storage");
+        if (geo_.les && cfg.method == IPL_IB_STRESS)  // its traction takes one tau for every marker
+            return Status::err(IPL_ERR_UNSUPPORTED, "the stress method with a per-cell (LES) tau");
         IbGrid g{};
         for (int a = 0; a < 3; ++a) { g.n[a] = geo_.n[a]; g.periodic[a] = geo_.periodic[a]; g.period[a] = gr_.period[a]; }
         g.s = geo_.s; g.o1 = geo_.o1; g.o2 = geo_.o2; g.ns = geo_.ns; g.ncell = geo_.ncell;
@@ -466,6 +489,14 @@ public:
         case IPL_FIELD_PHI: slot = 6; break;
         default: break;
         }
+        if ((field == IPL_FIELD_TAU || field == IPL_FIELD_TAUS) && geo_.les) {  // per-cell rows of the LES pass
+            std::vector<MT> t(NC);
+            IPL_HIP_TRY(hipMemcpyAsync(t.data(), field == IPL_FIELD_TAU ? les_tau_ : les_taus_, NC * sizeof(MT),
+                                       hipMemcpyDeviceToHost, stream));
+            IPL_HIP_TRY(hipStreamSynchronize(stream));
+            for (int64_t id = 0; id < N; ++id) host[id] = (double)t[perm_[id]];
+            return Status::ok();
+        }
         if (field == IPL_FIELD_FX || field == IPL_FIELD_FY || field == IPL_FIELD_FZ) {
             // the force of the last step (before resetF, collision.cu:258-264)
             std::vector<MT> t(NC);
@@ -545,6 +576,7 @@ private:
     ST *fa_ = nullptr, *fb_ = nullptr, *ga_ = nullptr, *gb_ = nullptr;
     MT *Fa_ = nullptr, *Fb_ = nullptr, *rho_n_ = nullptr, *phi_n_ = nullptr;
     MT *phiold_ = nullptr, *uold_ = nullptr, *vold_ = nullptr, *wold_ = nullptr;
+    MT *les_vel_ = nullptr, *les_nu_ = nullptr, *les_tau_ = nullptr, *les_taus_ = nullptr;  // LES rows
     double *vol_ = nullptr, *out_ = nullptr, *partial_ = nullptr;
     std::vector<int64_t> perm_;
     IbSolver ib_;
@@ -595,7 +627,7 @@ bool fused_supported(const Graph& g, const ipl_config& cfg, std::string& why) {
 
 std::unique_ptr<Engine> make_fused_engine(const Graph& g, const ipl_config& cfg, const HostState& hs,
                                           hipStream_t st, Status& status) {
-    for (int64_t i = 1; i < g.N; ++i)
+    for (int64_t i = 1; i < g.N && !cfg.les; ++i)
         if (hs.tau[i] != hs.tau[0] || hs.taus[i] != hs.taus[0]) {
             status = Status::err(IPL_ERR_UNSUPPORTED, "the fused path needs uniform tau and taus");
             return nullptr;

@@ -23,6 +23,7 @@ struct FGeom {
     int grad_rho;    // rho varies: the pressure-gradient term of Force is nonzero
     int grad_phi;    // phase field: grad phi and its Laplacian (Force, col_PF)
     int wall_fn;     // wall-function shear on the cells next to walls (collision.cu:28-55)
+    int les;         // per-cell tau, tau_s rows set by the LES pass (basic.cu:285-339)
     int half_force;  // u = sum e c f + F_old dt / (2 rho) (update_rho); 0 after an immersed-boundary
                      // sub-step, whose update_velIBM leaves u = sum e f (ibm.cu:356-360)
     int scalar;
@@ -47,6 +48,7 @@ struct FBufs {
     const MT* rho_n;              // [ncell] rho of step n (gradient pass), or null
     const MT* phi_n;              // [ncell] phi of step n, or null
     MT *phiold, *uold, *vold, *wold;
+    const MT *tau, *taus;         // [ncell] per-cell tau, tau_s (LES), or null
 };
 
 struct FParams {
@@ -63,6 +65,8 @@ struct FParams {
     double st_beta, kap, inv_dz2;            // 4 * 12 sigma / W, 3 sigma W / 2, 1/dz^2
     double inv_dt, inv_pfW, pf_itau, pf_fcoef; // 1/dt, 1/W, 1/tau_phi, tau_phi - 1/2
     double wall_nu;                          // wall function: kinematic viscosity
+    double inv_n2[27];                       // 1 / |moment row r|^2 (LES: per-cell stress rates)
+    double mu_L, mu_H;                       // LES: dynamic viscosities
 };
 
 // The coefficients of the step in the arithmetic type, cast once on the host (the kernels then
@@ -73,6 +77,7 @@ struct FCoef {
     MT uc;                             // velocity scale of sum e f: c, or 1 after an IB sub-step
     MT itau, om_s, rnorm[27];
     MT wall_nu, dz;                    // kinematic viscosity of the wall function's log law; dz
+    MT inv_n2[27], mu_L, mu_H;         // LES: 1 / |row r|^2, dynamic viscosities
     MT rho_L, drho, beta_s, gravity;   // drho = rho_H - rho_L (in MT)
     MT press, st_beta, kap, inv_dz2;
     MT inv_dt, inv_pfW, pf_itau, pf_fcoef;
@@ -92,6 +97,8 @@ inline FCoef<MT> make_coef(const FParams& P) {
     k.inv_dt = (MT)P.inv_dt; k.inv_pfW = (MT)P.inv_pfW; k.pf_itau = (MT)P.pf_itau; k.pf_fcoef = (MT)P.pf_fcoef;
     k.boussinesq = P.boussinesq; k.collision = P.collision;
     k.wall_nu = (MT)P.wall_nu; k.dz = (MT)P.dz;
+    for (int r = 0; r < 27; ++r) k.inv_n2[r] = (MT)P.inv_n2[r];
+    k.mu_L = (MT)P.mu_L; k.mu_H = (MT)P.mu_H;
     return k;
 }
 
@@ -102,6 +109,9 @@ template <typename ST, typename MT, int Q, int QS>
 size_t fused_lds_bytes(int s, int ns);
 template <typename ST, typename MT, int QS>
 void fused_launch_scalar_fields(const FGeom& G, const ST* g_src, MT* rho_n, MT* phi_n, const FCoef<MT>& P, hipStream_t st);
+template <typename ST, typename MT, int Q, int QS>
+void fused_launch_les(const FGeom& G, const FTables<MT>& tb, const ST* f, const ST* g, const MT* F, MT* vel, MT* nu,
+                      MT* tau, MT* taus, const FCoef<MT>& P, unsigned block, hipStream_t st);
 template <typename ST, typename MT, int Q, int QS>
 void fused_launch_readout(const FGeom& G, const ST* f_src, const ST* g_src, const MT* F_old, double* out,
                           const FParams& P, hipStream_t st);

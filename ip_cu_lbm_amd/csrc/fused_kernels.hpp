@@ -456,7 +456,7 @@ template <int DX, int DY, int DZ, typename MT> __device__ inline MT dot3(MT x, M
 // and odd parts of column k.  f is replaced by f*.
 template <typename MT, int Q>
 __device__ inline void collide_pairs(MT (&f)[Q], MT p, MT u, MT v, MT w, MT u2c, MT Fx, MT Fy, MT Fz, MT fkc, MT ic,
-                                     const FCoef<MT>& P) {
+                                     const FCoef<MT>& P, bool les, MT itau_c) {
     MT sv[Q], av[Q], fkv[Q];
     sv[0] = f[0] - MT(Lattice<Q>::w(0)) * (p - MT(1.5) * u2c);
     static_for<Q>([&](auto kc) {
@@ -475,7 +475,7 @@ __device__ inline void collide_pairs(MT (&f)[Q], MT p, MT u, MT v, MT w, MT u2c,
         }
     });
     if (P.collision == IPL_COLL_SRT) {
-        const MT itau = P.itau;
+        const MT itau = les ? itau_c : P.itau;
         f[0] = f[0] - sv[0] * itau;
         static_for<Q>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
@@ -505,7 +505,8 @@ __device__ inline void collide_pairs(MT (&f)[Q], MT p, MT u, MT v, MT w, MT u2c,
                 }
             }
         });
-        m[r] = acc * P.rnorm[r];
+        if constexpr (is_stress<Q>(r)) m[r] = acc * (les ? itau_c * P.inv_n2[r] : P.rnorm[r]);
+        else m[r] = acc * P.rnorm[r];
     });
     static_for<Q>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
@@ -698,7 +699,12 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
         // ---- equ_f, forcing term and collision ----
         const MT u2c = (u * u + v * v + w * w) * ic2;
         const MT fkc = P.fk_coef * irho;
-        collide_pairs<MT, Q>(Pf, p, u, v, w, u2c, Fx, Fy, Fz, fkc, ic, P);
+        MT itau_c = P.itau, om_c = P.om_s;  // per-cell rates when an LES pass sets tau, tau_s
+        if (G.les) {
+            itau_c = MT(1) / *elem(b.tau, n);
+            if constexpr (QS > 0) om_c = MT(1) - MT(1) / *elem(b.taus, n);
+        }
+        collide_pairs<MT, Q>(Pf, p, u, v, w, u2c, Fx, Fy, Fz, fkc, ic, P, G.les != 0, itau_c);
         // ---- scalar collision ----
         if constexpr (QS > 0) {
             if (G.scalar == IPL_SCALAR_SALINITY) {  // col_g_reg (collision.cu:167-188)
@@ -718,7 +724,7 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
                     if constexpr (d.y > 0) jy += gn; else if constexpr (d.y < 0) jy -= gn;
                     if constexpr (d.z > 0) jz += gn; else if constexpr (d.z < 0) jz -= gn;
                 });
-                const MT om = P.om_s;
+                const MT om = om_c;
                 static_for<QS>([&](auto kc) {
                     constexpr int k = decltype(kc)::value;
                     constexpr Dir d = Lattice<QS>::c(k);
@@ -901,6 +907,113 @@ size_t fused_lds_bytes(int s, int ns) {
     return (size_t)(nf + ng) * ns * sizeof(MT);
 }
 
+// ---- LES (basic.cu:285-339), after the step: velocities, then the model ---------------------
+// k_les_fields: u of every cell as update_rho leaves it (sum e c f + F dt / (2 rho), with the
+// step's force) and the molecular kinematic viscosity (mu_L + phi (mu_H - mu_L)) / rho
+template <typename ST, typename MT, int Q, int QS>
+__global__ void k_les_fields(FGeom G, const ST* __restrict__ f, const ST* __restrict__ g, const MT* __restrict__ F,
+                             MT* __restrict__ vel, MT* __restrict__ nu, FCoef<MT> P) {
+    const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (n >= G.ncell) return;
+    const uint32_t NC = (uint32_t)G.ncell, c = (uint32_t)n;
+    MT ex = 0, ey = 0, ez = 0;
+    static_for<Q>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        constexpr Dir d = Lattice<Q>::c(k);
+        if constexpr (d.x != 0 || d.y != 0 || d.z != 0) {
+            const MT fv = Store<ST, MT>::load(elem(f, (uint32_t)k * NC + c), Lattice<Q>::w(k), G.inv_scale);
+            if constexpr (d.x > 0) ex += fv; else if constexpr (d.x < 0) ex -= fv;
+            if constexpr (d.y > 0) ey += fv; else if constexpr (d.y < 0) ey -= fv;
+            if constexpr (d.z > 0) ez += fv; else if constexpr (d.z < 0) ez -= fv;
+        }
+    });
+    MT sc = 0;
+    if constexpr (QS > 0) {
+#pragma unroll
+        for (int k = 0; k < QS; ++k) sc += Store<ST, MT>::load(elem(g, (uint32_t)k * NC + c), Lattice<QS>::w(k), G.inv_scale);
+    }
+    const MT S_ = G.scalar == IPL_SCALAR_SALINITY ? sc : MT(0);
+    const MT phi = G.scalar == IPL_SCALAR_PHASE_FIELD ? sc : MT(0);
+    const MT rho = P.rho_L + P.drho * phi + S_ * P.beta_s;
+    const MT hr = P.dt / (MT(2.0) * rho);
+    *elem(vel, c) = ex * P.c + *elem(F, c) * hr;
+    *elem(vel, NC + c) = ey * P.c + *elem(F, NC + c) * hr;
+    *elem(vel, 2 * NC + c) = ez * P.c + *elem(F, 2 * NC + c) * hr;
+    *elem(nu, c) = (P.mu_L + phi * (P.mu_H - P.mu_L)) / rho;
+}
+
+// velocity component q of the neighbour along axis A at offset OFF: zero beyond a wall (a ghost
+// has zero velocity, basic.cu:226)
+template <int S, int A, int OFF, typename MT>
+__device__ inline MT les_nb(const FGeom& G, const LineNb& nb, const MT* vel, uint32_t q, int i) {
+    constexpr int o1 = LineAxes<S>::o1;
+    bool wall;
+    if constexpr (A == S) wall = (i + OFF < 0 || i + OFF >= G.ns) && !G.periodic[S];
+    else if constexpr (A == o1) wall = OFF < 0 ? nb.o1m : nb.o1p;
+    else wall = OFF < 0 ? nb.o2m : nb.o2p;
+    return wall ? MT(0) : *elem(vel, q * (uint32_t)G.ncell + nb_index<S, A, OFF>(G, nb, i));
+}
+
+// k_les: the coherent-structure model per cell, one workgroup per line; gradients
+// (v[+1] - v[-1]) / (pos[+1] - pos[-1] + dz/1000) with the reciprocal tables of Force.  In 2D the
+// reference takes the z differences over the same neighbour twice, so they vanish.
+template <typename MT, int Q, int S>
+__global__ void k_les(FGeom G, FTables<MT> tb, const MT* __restrict__ vel, const MT* __restrict__ nu,
+                      MT* __restrict__ tau, MT* __restrict__ taus, FCoef<MT> P) {
+    constexpr bool d3 = Lattice<Q>::dim == 3;
+    constexpr int o1 = LineAxes<S>::o1;
+    const int i = threadIdx.x;
+    if (i >= G.ns) return;
+    const int L = (int)blockIdx.x;
+    const int n2 = G.n[LineAxes<S>::o2];
+    const int c1 = L / n2, c2 = L - (L / n2) * n2;
+    const LineNb nb = line_neighbours<S>(G, c1, c2);
+    const uint32_t n = (uint32_t)L * (uint32_t)G.ns + (uint32_t)i;
+    int cc[3];
+    cc[S] = i; cc[o1] = c1; cc[LineAxes<S>::o2] = c2;
+    MT gr[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // gr[axis][component]
+    auto axis_grad = [&](auto ac) {
+        constexpr int A = decltype(ac)::value;
+        const MT inv = *elem(tb.invd[A], (uint32_t)cc[A]);
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            gr[A][q] = (les_nb<S, A, 1>(G, nb, vel, (uint32_t)q, i) - les_nb<S, A, -1>(G, nb, vel, (uint32_t)q, i)) * inv;
+    };
+    axis_grad(std::integral_constant<int, 0>{});
+    if constexpr (d3) {
+        axis_grad(std::integral_constant<int, 1>{});
+        axis_grad(std::integral_constant<int, 2>{});
+    }
+    const MT nxx = gr[0][0], nxy = gr[0][1], nxz = gr[0][2];
+    const MT nyx = gr[1][0], nyy = gr[1][1], nyz = gr[1][2];
+    const MT nzx = gr[2][0], nzy = gr[2][1], nzz = gr[2][2];
+    const MT arr[9] = {nxx, (nxy + nyx) / MT(2), (nxz + nzx) / MT(2), (nyx + nxy) / MT(2), nyy,
+                       (nyz + nzy) / MT(2), (nzx + nxz) / MT(2), (nzy + nyz) / MT(2), nzz};
+    const MT Qi = MT(-0.5) * (nxx * nxx + nxy * nyx + nxz * nzx + nyx * nxy + nyy * nyy + nyz * nzy +
+                              nzx * nxz + nzy * nyz + nzz * nzz);
+    const MT sum = nxx + nxy + nxz + nyx + nyy + nyz + nzx + nzy + nzz;
+    const MT E = MT(1) / (MT(2) * sum * sum + MT(1e-40));
+    const MT Fcs = fmax(MT(-1), fmin(Qi / E, MT(1)));
+    const MT af = fabs(Fcs);
+    const MT C = MT(0.05) * af * sqrt(af);
+    MT Ssum = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Ssum += sqrt(MT(2) * arr[k] * arr[k]);
+    const MT c2dt = P.c * P.c * P.dt;
+    const MT dz2 = P.dz * P.dz;
+    *elem(tau, n) = MT(3) * (*elem(nu, n) + C * Ssum * dz2) / c2dt + MT(0.5);
+    *elem(taus, n) = MT(3) * (MT(1e-9) + C * Ssum * dz2 / MT(1000)) / c2dt + MT(0.5);
+}
+
+template <typename ST, typename MT, int Q, int QS>
+void fused_launch_les(const FGeom& G, const FTables<MT>& tb, const ST* f, const ST* g, const MT* F, MT* vel, MT* nu,
+                      MT* tau, MT* taus, const FCoef<MT>& P, unsigned block, hipStream_t st) {
+    IPL_LAUNCH((k_les_fields<ST, MT, Q, QS>), (G.ncell + 255) / 256, 256, 0, st, G, f, g, F, vel, nu, P);
+    if (G.s == 0) IPL_LAUNCH((k_les<MT, Q, 0>), (unsigned)G.lines, block, 0, st, G, tb, vel, nu, tau, taus, P);
+    else if (G.s == 1) IPL_LAUNCH((k_les<MT, Q, 1>), (unsigned)G.lines, block, 0, st, G, tb, vel, nu, tau, taus, P);
+    else IPL_LAUNCH((k_les<MT, Q, 2>), (unsigned)G.lines, block, 0, st, G, tb, vel, nu, tau, taus, P);
+}
+
 template <typename ST, typename MT, int QS>
 void fused_launch_scalar_fields(const FGeom& G, const ST* g_src, MT* rho_n, MT* phi_n, const FCoef<MT>& P, hipStream_t st) {
     const int64_t items = (G.ncell + 3) / 4;
@@ -925,6 +1038,9 @@ void fused_launch_convert(const FGeom& G, const ST* src, double* dst, int q, boo
 
 // explicit instantiation of every launcher of one (storage, math, Q, QS) combination
 #define IPL_FUSED_INSTANTIATE(ST, MT, Q, QS)                                                                \
+    template void ipl::fusedk::fused_launch_les<ST, MT, Q, QS>(const FGeom&, const FTables<MT>&, const ST*, const ST*, \
+                                                              const MT*, MT*, MT*, MT*, MT*, const FCoef<MT>&,        \
+                                                              unsigned, hipStream_t);                                 \
     template void ipl::fusedk::fused_launch_step<ST, MT, Q, QS>(                                             \
         const FGeom&, const FTables<MT>&, const FBufs<ST, MT>&, const FCoef<MT>&, int, unsigned, size_t, hipStream_t); \
     template size_t ipl::fusedk::fused_lds_bytes<ST, MT, Q, QS>(int, int);                                    \

@@ -1406,6 +1406,43 @@ void wall_function(Oracle& o, int x, int y, int z, const std::vector<int>& wall,
     }
 }
 
+// LES (basic.cu:285-339): coherent-structure model.  Velocity gradients by central differences
+// over the neighbour table (a ghost has zero velocity and its owner's position; across a periodic
+// seam the period is added, as in Force), the model coefficient from Q / E clipped to [-1, 1],
+// and tau, tau_s of every calc cell from the molecular viscosity plus the eddy viscosity.  In 2D
+// the z differences use the same neighbour twice (up = down = 2) and vanish, as in the reference.
+void les(Oracle& o) {
+    const int Q = o.Q;
+    const int up = Q == 9 ? 2 : 5, down = Q == 9 ? 2 : 6;
+    const double dz = o.dz;
+    std::vector<double> tau(o.N), taus(o.N);
+    #pragma omp parallel for schedule(static)
+    for (int id = 0; id < o.N; ++id) {
+        const int* nb = &o.neib[(size_t)id * Q];
+        auto vel = [&](const std::vector<double>& a, int j) { return j < o.N ? a[j] : 0.0; };
+        auto grad = [&](const std::vector<double>& a, int jp, int jm, int axis) {
+            return (vel(a, jp) - vel(a, jm)) / (dpos(o, axis, jp, jm) + dz / 1000);
+        };
+        const double nxx = grad(o.u, nb[1], nb[3], 0), nxy = grad(o.v, nb[1], nb[3], 0), nxz = grad(o.w, nb[1], nb[3], 0);
+        double nyx = 0, nyy = 0, nyz = 0;
+        if (Q != 9) { nyx = grad(o.u, nb[2], nb[4], 1); nyy = grad(o.v, nb[2], nb[4], 1); nyz = grad(o.w, nb[2], nb[4], 1); }
+        const double nzx = grad(o.u, nb[up], nb[down], 2), nzy = grad(o.v, nb[up], nb[down], 2), nzz = grad(o.w, nb[up], nb[down], 2);
+        const double arr[9] = {(nxx + nxx) / 2.0, (nxy + nyx) / 2.0, (nxz + nzx) / 2.0, (nyx + nxy) / 2.0, (nyy + nyy) / 2.0,
+                               (nyz + nzy) / 2.0, (nzx + nxz) / 2.0, (nzy + nyz) / 2.0, (nzz + nzz) / 2.0};
+        const double Qi = -1.0 / 2.0 * (nxx * nxx + nxy * nyx + nxz * nzx + nyx * nxy + nyy * nyy + nyz * nzy +
+                                        nzx * nxz + nzy * nyz + nzz * nzz);
+        const double E = 1.0 / (2.0 * std::pow(nxx + nxy + nxz + nyx + nyy + nyz + nzx + nzy + nzz, 2) + std::pow(10.0, -40));
+        const double F_cs = std::max(-1.0, std::min(Qi / E, 1.0));
+        const double C = 0.05 * std::pow(std::fabs(F_cs), 3.0 / 2.0);
+        double S = 0;
+        for (int k = 0; k < 9; ++k) S += std::sqrt(2.0 * std::pow(arr[k], 2));
+        const double c2dt = o.c * o.c * o.dt;
+        tau[id] = 3.0 * ((o.cfg.mu_L + o.phi[id] * (o.cfg.mu_H - o.cfg.mu_L)) / o.rho[id] + C * S * std::pow(dz, 2)) / c2dt + 0.5;
+        taus[id] = 3.0 * (std::pow(10.0, -9) + C * S * std::pow(dz, 2) / 1000) / c2dt + 0.5;
+    }
+    for (int id = 0; id < o.N; ++id) { o.tau[id] = tau[id]; o.taus[id] = taus[id]; }
+}
+
 // the driver's six calls (LockExchange.cu:214-219): faces +x, -x, +z, -z, +y, -y
 void wall_functions(Oracle& o) {
     int dirs[7];
@@ -1426,7 +1463,10 @@ void wall_functions(Oracle& o) {
 
 void step(Oracle& o) {
     if (o.cfg.wall_function) wall_functions(o);
-    for (int s = IPL_STAGE_EQU_F; s <= IPL_STAGE_RESET_F; ++s) stage(o, s);
+    for (int s = IPL_STAGE_EQU_F; s <= IPL_STAGE_RESET_F; ++s) {
+        stage(o, s);
+        if (s == IPL_STAGE_UPDATE_RHO && o.cfg.les) les(o);
+    }
     if (o.ib.active) ib_substep(o);
     if (!o.ob_cells.empty()) open_boundary(o);
 }
