@@ -258,6 +258,18 @@ int ipl_set_graph(ipl_ctx* ctx, int32_t on);
 int ipl_set_open_boundary(ipl_ctx* ctx, const int32_t* cells, int64_t n, const double* p, const double* u,
                           const double* v, const double* w);
 
+/* ---- file writers (host side, after a readout of the fields) -------------------------------- */
+/* output() (basic.cu:3-30): "num_calc,N", the 15-column header and one "%f" row per calc cell
+ * (x, y, z, delx, dely, pressure, salinity, phi, rho, u, v, w, Fx, Fy, Fz), the format the
+ * reference's post-processing reads (data/Fakhari_LBM%04d.csv). */
+int ipl_write_fields_csv(ipl_ctx* ctx, const char* path);
+/* IB_csv (ibm.cu:3-14): "x,y,z,velx,vely,velz,ax,ay,az" and one row per marker (IB_point/ibm_%04d.csv). */
+int ipl_write_markers_csv(ipl_ctx* ctx, const char* path);
+/* The VTU of LockExchange/make_VTK.py: one VTK_HEXAHEDRON per calc cell (corners at the cell
+ * faces, shared between neighbours), cell data salinity, phi, rho, pressure, velocity, force;
+ * ASCII, written directly instead of through a CSV and pyvista. */
+int ipl_write_vtu(ipl_ctx* ctx, const char* path);
+
 /* ---- host-only set-up helpers (no device work) --------------------------------------------- */
 /* hydrostatic_pressure (basic.cu:239-252): the initial p of every calc cell (canonical order),
  * p = 3 g (nz dz / 2 - z) / c^2 / 1000 with boussinesq != 0, else 3 g (center_z - z) / c^2,

@@ -14,6 +14,7 @@ fallback, and the constructor raises if the library is missing.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -90,6 +91,18 @@ class Simulation:
         vals = [np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=np.float64), (n,))) for x in (p, u, v, w)]
         self._check(self._lib.ipl_set_open_boundary(self._h, cells.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n,
                                                     *[_dp(x) for x in vals]))
+
+    def write_fields_csv(self, path: str):
+        """The reference's output() file: num_calc line, 15-column header, one row per cell."""
+        self._check(self._lib.ipl_write_fields_csv(self._h, os.fsencode(path)))
+
+    def write_markers_csv(self, path: str):
+        """The reference's IB_csv file: marker positions, velocities and forces."""
+        self._check(self._lib.ipl_write_markers_csv(self._h, os.fsencode(path)))
+
+    def write_vtu(self, path: str):
+        """VTU of hexahedral cells with the cell data of make_VTK.py."""
+        self._check(self._lib.ipl_write_vtu(self._h, os.fsencode(path)))
 
     def step_timed(self, n: int) -> float:
         """Runs n steps and returns their device time in milliseconds (HIP events)."""

@@ -2,8 +2,10 @@
 #include "hip_compat.hpp"
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <new>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -12,6 +14,7 @@
 struct ipl_ctx {
     ipl_config cfg{};
     ipl::Graph graph;
+    std::vector<double> axis_pos[3], axis_wid[3];  // per axis index, for the file writers
     std::unique_ptr<ipl::Engine> engine;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -80,6 +83,11 @@ int ipl_create(const ipl_config* cfg, const ipl_init* init, ipl_ctx** out) {
         if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
         if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
         if (e != hipSuccess) return bail(ipl::Status::err(IPL_ERR_HIP, std::string("HIP setup: ") + hipGetErrorString(e)));
+    }
+    for (int a = 0; a < 3; ++a) {
+        const int na = a == 0 ? cfg->nx : (a == 1 ? cfg->ny : cfg->nz);
+        ctx->axis_pos[a].assign(cfg->pos[a], cfg->pos[a] + na);
+        ctx->axis_wid[a].assign(cfg->width[a], cfg->width[a] + na);
     }
     ipl::HostState hs;
     ipl::Status st = ipl::make_host_state(ctx->graph, *cfg, init, hs);
@@ -305,6 +313,128 @@ int ipl_set_open_boundary(ipl_ctx* ctx, const int32_t* cells, int64_t n, const d
     }
     ipl::Status s = ctx->engine->set_open_boundary(cells, n, vals.data());
     return s ? IPL_OK : fail(ctx, s);
+}
+
+namespace {
+
+// the fields of the reference's output() rows, canonical order: p, S, phi, rho, u, v, w, Fx, Fy, Fz
+ipl::Status read_output_fields(ipl_ctx* ctx, std::vector<double> (&v)[10]) {
+    static const int fields[10] = {IPL_FIELD_P, IPL_FIELD_S, IPL_FIELD_PHI, IPL_FIELD_RHO, IPL_FIELD_U,
+                                   IPL_FIELD_V, IPL_FIELD_W, IPL_FIELD_FX, IPL_FIELD_FY, IPL_FIELD_FZ};
+    const size_t N = (size_t)ctx->graph.N;
+    for (int q = 0; q < 10; ++q) {
+        v[q].assign(N, 0.0);
+        ipl::Status s = ctx->engine->read(fields[q], v[q].data(), N);
+        if (!s) return s;
+    }
+    return ipl::Status::ok();
+}
+
+}  // namespace
+
+int ipl_write_fields_csv(ipl_ctx* ctx, const char* path) {
+    if (!ctx || !path) return IPL_ERR_ARGUMENT;
+    std::vector<double> v[10];
+    ipl::Status s = read_output_fields(ctx, v);
+    if (!s) return fail(ctx, s);
+    FILE* fp = std::fopen(path, "w");
+    if (!fp) return fail(ctx, ipl::Status::err(IPL_ERR_ARGUMENT, std::string("cannot open ") + path));
+    const ipl::Graph& g = ctx->graph;
+    std::fprintf(fp, "num_calc,%d\n", (int)g.N);
+    std::fprintf(fp, "x       ,y       ,z       ,delx    ,dely    ,pressure,salinity ,phi     ,rho     ,u       ,v       ,w       ,Fx      ,Fy      ,Fz      \n");
+    for (int64_t i = 0; i < g.N; ++i)
+        std::fprintf(fp, "%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f\n", g.pos[0][i], g.pos[1][i], g.pos[2][i],
+                     g.del[0][i], g.del[1][i], v[0][i], v[1][i], v[2][i], v[3][i], v[4][i], v[5][i], v[6][i], v[7][i],
+                     v[8][i], v[9][i]);
+    const bool ok = std::fclose(fp) == 0;
+    return ok ? IPL_OK : fail(ctx, ipl::Status::err(IPL_ERR_ARGUMENT, std::string("write failed: ") + path));
+}
+
+int ipl_write_markers_csv(ipl_ctx* ctx, const char* path) {
+    if (!ctx || !path) return IPL_ERR_ARGUMENT;
+    double body[19];
+    ipl::Status s = ctx->engine->ib_body_state(body);
+    if (!s) return fail(ctx, s);
+    const int64_t M = ctx->engine->ib_marker_count();
+    std::vector<double> pos(3 * M), vel(3 * M), gw(3 * M);
+    if (!(s = ctx->engine->ib_read_markers(IPL_IB_POS, pos.data(), pos.size()))) return fail(ctx, s);
+    if (!(s = ctx->engine->ib_read_markers(IPL_IB_VEL, vel.data(), vel.size()))) return fail(ctx, s);
+    if (!(s = ctx->engine->ib_read_markers(IPL_IB_FORCE, gw.data(), gw.size()))) return fail(ctx, s);
+    std::ofstream file(path);
+    if (!file) return fail(ctx, ipl::Status::err(IPL_ERR_ARGUMENT, std::string("cannot open ") + path));
+    file << "x,y,z,velx,vely,velz,ax,ay,az\n";
+    for (int64_t i = 0; i < M; ++i)
+        file << pos[i * 3] << "," << pos[i * 3 + 1] << "," << pos[i * 3 + 2] << "," << vel[i * 3] << "," << vel[i * 3 + 1]
+             << "," << vel[i * 3 + 2] << "," << gw[i * 3] << "," << gw[i * 3 + 1] << "," << gw[i * 3 + 2] << "\n";
+    file.close();
+    return file ? IPL_OK : fail(ctx, ipl::Status::err(IPL_ERR_ARGUMENT, std::string("write failed: ") + path));
+}
+
+int ipl_write_vtu(ipl_ctx* ctx, const char* path) {
+    if (!ctx || !path) return IPL_ERR_ARGUMENT;
+    std::vector<double> v[10];
+    ipl::Status s = read_output_fields(ctx, v);
+    if (!s) return fail(ctx, s);
+    const ipl::Graph& g = ctx->graph;
+    const int64_t n[3] = {ctx->cfg.nx, ctx->cfg.ny, ctx->cfg.nz};
+    // corner coordinates per axis: the faces between cells
+    std::vector<double> corner[3];
+    for (int a = 0; a < 3; ++a) {
+        corner[a].resize((size_t)n[a] + 1);
+        corner[a][0] = ctx->axis_pos[a][0] - ctx->axis_wid[a][0] / 2.0;
+        for (int64_t k = 1; k <= n[a]; ++k) corner[a][k] = ctx->axis_pos[a][k - 1] + ctx->axis_wid[a][k - 1] / 2.0;
+    }
+    auto cid = [&](int64_t i, int64_t j, int64_t l) { return (i * (n[1] + 1) + j) * (n[2] + 1) + l; };
+    std::vector<int64_t> pid((size_t)((n[0] + 1) * (n[1] + 1) * (n[2] + 1)), -1);
+    std::vector<int64_t> conn((size_t)8 * g.N);
+    std::vector<int64_t> pts;
+    for (int64_t c = 0; c < g.N; ++c) {
+        const int64_t i = g.cell_ijk[3 * c], j = g.cell_ijk[3 * c + 1], l = g.cell_ijk[3 * c + 2];
+        // VTK_HEXAHEDRON order: bottom face (z-) counter-clockwise from (x-, y-), then the top face
+        const int64_t corners[8] = {cid(i, j, l), cid(i + 1, j, l), cid(i + 1, j + 1, l), cid(i, j + 1, l),
+                                    cid(i, j, l + 1), cid(i + 1, j, l + 1), cid(i + 1, j + 1, l + 1), cid(i, j + 1, l + 1)};
+        for (int q = 0; q < 8; ++q) {
+            int64_t& p = pid[(size_t)corners[q]];
+            if (p < 0) { p = (int64_t)pts.size(); pts.push_back(corners[q]); }
+            conn[(size_t)8 * c + q] = p;
+        }
+    }
+    FILE* fp = std::fopen(path, "w");
+    if (!fp) return fail(ctx, ipl::Status::err(IPL_ERR_ARGUMENT, std::string("cannot open ") + path));
+    std::fprintf(fp, "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n");
+    std::fprintf(fp, "<UnstructuredGrid>\n<Piece NumberOfPoints=\"%lld\" NumberOfCells=\"%lld\">\n", (long long)pts.size(), (long long)g.N);
+    std::fprintf(fp, "<Points>\n<DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n");
+    for (int64_t q : pts) {
+        const int64_t l = q % (n[2] + 1), j = (q / (n[2] + 1)) % (n[1] + 1), i = q / ((n[1] + 1) * (n[2] + 1));
+        std::fprintf(fp, "%.17g %.17g %.17g\n", corner[0][i], corner[1][j], corner[2][l]);
+    }
+    std::fprintf(fp, "</DataArray>\n</Points>\n<Cells>\n<DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n");
+    for (int64_t c = 0; c < g.N; ++c) {
+        for (int q = 0; q < 8; ++q) std::fprintf(fp, q ? " %lld" : "%lld", (long long)conn[(size_t)8 * c + q]);
+        std::fprintf(fp, "\n");
+    }
+    std::fprintf(fp, "</DataArray>\n<DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n");
+    for (int64_t c = 0; c < g.N; ++c) std::fprintf(fp, "%lld\n", (long long)(8 * (c + 1)));
+    std::fprintf(fp, "</DataArray>\n<DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n");
+    for (int64_t c = 0; c < g.N; ++c) std::fprintf(fp, "12\n");
+    std::fprintf(fp, "</DataArray>\n</Cells>\n<CellData>\n");
+    const char* names[4] = {"salinity", "phi", "rho", "pressure"};
+    const int slot[4] = {1, 2, 3, 0};
+    for (int q = 0; q < 4; ++q) {
+        std::fprintf(fp, "<DataArray type=\"Float64\" Name=\"%s\" format=\"ascii\">\n", names[q]);
+        for (int64_t c = 0; c < g.N; ++c) std::fprintf(fp, "%.17g\n", v[slot[q]][c]);
+        std::fprintf(fp, "</DataArray>\n");
+    }
+    const char* vnames[2] = {"velocity", "force"};
+    for (int q = 0; q < 2; ++q) {
+        std::fprintf(fp, "<DataArray type=\"Float64\" Name=\"%s\" NumberOfComponents=\"3\" format=\"ascii\">\n", vnames[q]);
+        for (int64_t c = 0; c < g.N; ++c)
+            std::fprintf(fp, "%.17g %.17g %.17g\n", v[4 + 3 * q][c], v[5 + 3 * q][c], v[6 + 3 * q][c]);
+        std::fprintf(fp, "</DataArray>\n");
+    }
+    std::fprintf(fp, "</CellData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n");
+    const bool ok = std::fclose(fp) == 0;
+    return ok ? IPL_OK : fail(ctx, ipl::Status::err(IPL_ERR_ARGUMENT, std::string("write failed: ") + path));
 }
 
 int ipl_hydrostatic_pressure(const ipl_config* cfg, int32_t boussinesq, double center_z, double* p_out, int64_t count) {

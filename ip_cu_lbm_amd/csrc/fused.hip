@@ -413,6 +413,7 @@ public:
     }
     Status ib_body_state(double* out) override { return ib_.body_state(out); }
     Status ib_read_markers(int which, double* out, size_t count) override { return ib_.read_markers(which, out, count); }
+    int64_t ib_marker_count() const override { return ib_.markers(); }
 
     Status set_open_boundary(const int32_t* cells, int64_t n, const double* vals) override {
         if (std::is_same<ST, __half>::value) return Status::err(IPL_ERR_UNSUPPORTED, "open boundary needs FP64 or FP32 storage");

@@ -609,7 +609,8 @@ __device__ inline MT grad_axis(const FGeom& G, const LineNb& nb, const FTables<M
     return (*elem(q, nb_index<S, A, 1>(G, nb, i)) - *elem(q, nb_index<S, A, -1>(G, nb, i))) * *elem(tb.invd[A], (uint32_t)x);
 }
 
-template <typename ST, typename MT, int Q, int QS, int NPTS, int S, bool FLUX>
+// EXT: the wall function and LES are compiled in (they add registers to every step otherwise)
+template <typename ST, typename MT, int Q, int QS, int NPTS, int S, bool FLUX, bool EXT>
 __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGeom G, FTables<MT> tb, FBufs<ST, MT> b, FCoef<MT> P) {
     IPL_DYNAMIC_LDS(MT, lds_f);
     MT* lds_g = lds_f + (sdir_count<Q, S>() + 1) * G.ns;
@@ -671,7 +672,9 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
         // ---- Force (collision.cu:57-112) ----
         constexpr bool d3 = Lattice<Q>::dim == 3;
         MT Fx = 0, Fy = 0, Fz = 0;
-        if (G.wall_fn) wall_shear<MT, S, d3>(G, tb, P, ii, c1, c2, u, v, w, rho, Fx, Fy, Fz);
+        if constexpr (EXT) {
+            if (G.wall_fn) wall_shear<MT, S, d3>(G, tb, P, ii, c1, c2, u, v, w, rho, Fx, Fy, Fz);
+        }
         MT gpx = 0, gpy = 0, gpz = 0;  // grad phi (col_PF reuses it)
         if (G.grad_phi) {
             gpx = grad_axis<MT, S, 0>(G, nb, tb, b.phi_n, c1, c2, ii);
@@ -700,11 +703,15 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
         const MT u2c = (u * u + v * v + w * w) * ic2;
         const MT fkc = P.fk_coef * irho;
         MT itau_c = P.itau, om_c = P.om_s;  // per-cell rates when an LES pass sets tau, tau_s
-        if (G.les) {
-            itau_c = MT(1) / *elem(b.tau, n);
-            if constexpr (QS > 0) om_c = MT(1) - MT(1) / *elem(b.taus, n);
+        bool les = false;
+        if constexpr (EXT) {
+            les = G.les != 0;
+            if (les) {
+                itau_c = MT(1) / *elem(b.tau, n);
+                if constexpr (QS > 0) om_c = MT(1) - MT(1) / *elem(b.taus, n);
+            }
         }
-        collide_pairs<MT, Q>(Pf, p, u, v, w, u2c, Fx, Fy, Fz, fkc, ic, P, G.les != 0, itau_c);
+        collide_pairs<MT, Q>(Pf, p, u, v, w, u2c, Fx, Fy, Fz, fkc, ic, P, les, itau_c);
         // ---- scalar collision ----
         if constexpr (QS > 0) {
             if (G.scalar == IPL_SCALAR_SALINITY) {  // col_g_reg (collision.cu:167-188)
@@ -877,7 +884,12 @@ template <typename ST, typename MT, int Q, int QS>
 void fused_launch_step(const FGeom& G, const FTables<MT>& tb, const FBufs<ST, MT>& b, const FCoef<MT>& P, int npts,
                        unsigned block, size_t lds, hipStream_t st) {
     const unsigned grid = (unsigned)G.lines;
-#define IPL_FSTEP(NP, SS, FL) IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS, FL>), grid, block, lds, st, G, tb, b, P)
+    const bool ext = G.wall_fn || G.les;
+#define IPL_FSTEP(NP, SS, FL)                                                                                     \
+    do {                                                                                                          \
+        if (ext) IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS, FL, true>), grid, block, lds, st, G, tb, b, P);  \
+        else IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS, FL, false>), grid, block, lds, st, G, tb, b, P);     \
+    } while (0)
     // the stream axis and the stencil are template arguments; a uniform grid has no stencil
     if (!G.flux) {
         if constexpr (Lattice<Q>::dim == 2) {

@@ -60,6 +60,7 @@ public:
 
     // slots of a marker's node block
     int slots() const { return S_; }
+    int markers() const { return M_; }
 
 private:
     IbGrid g_{};
