@@ -47,6 +47,31 @@ def axis_from_widths(widths: np.ndarray):
     return pos, widths
 
 
+def subdivided_axis(divs, coarse: float):
+    """Piecewise refinement of the drivers (divx / divy, SphereFall.cu:80-128): coarse column i of
+    width `coarse` is split into divs[i] equal cells, centred at (i + (2 d + 1) / (2 divs[i])) coarse.
+    Returns (pos, widths)."""
+    pos, wid = [], []
+    for i, d in enumerate(divs):
+        for k in range(int(d)):
+            pos.append((i + (2.0 * k + 1.0) / (2.0 * d)) * coarse)
+            wid.append(coarse / d)
+    return np.array(pos), np.array(wid)
+
+
+def sphere_fall_divs(n: int, coarse: float, inner=(0.03, 0.095), outer=(0.02, 0.105)):
+    """The falling-sphere driver's refinement pattern (SphereFall.cu:80-104): columns whose centre
+    lies in `inner` split in 4, in `outer` in 2, the rest kept."""
+    divs = []
+    for i in range(n):
+        x = (i + 0.5) * coarse
+        if outer[0] < x < outer[1]:
+            divs.append(4 if inner[0] < x < inner[1] else 2)
+        else:
+            divs.append(1)
+    return divs
+
+
 def stretched_widths(n: int, ratio: float, dz: float = 1.0) -> np.ndarray:
     """Widths growing geometrically by `ratio` per cell away from the axis centre (finest at
     the centre): width_i = dz * ratio**j, j = cells between i and the centre pair."""

@@ -83,3 +83,32 @@ def test_library_exports_every_declared_symbol():
     lib = _abi.load_library()
     for n in names:
         assert hasattr(lib, n), n
+
+
+def test_subdivided_axis_matches_driver_formula():
+    """subdivided_axis reproduces the drivers' refined columns (SphereFall.cu:80-128): centres at
+    (i + (2d+1)/(2 divx[i])) dx ratiox, widths dx ratiox / divx[i], contiguous cells, total length kept."""
+    coarse = 0.013  # column centres 0.0065, 0.0195, ..., 0.0975, 0.1105: none on a bound
+    divs = cases.sphere_fall_divs(9, coarse)
+    assert divs == [1, 1, 4, 4, 4, 4, 4, 2, 1]
+    pos, wid = cases.subdivided_axis(divs, coarse)
+    assert len(pos) == sum(divs)
+    k = 0
+    for i, d in enumerate(divs):
+        for dv in range(d):
+            assert abs(pos[k] - (i + (2.0 * dv + 1.0) / (2.0 * d)) * coarse) < 1e-15
+            assert abs(wid[k] - coarse / d) < 1e-18
+            k += 1
+    edges = pos - wid / 2
+    assert np.allclose(edges[1:], (pos + wid / 2)[:-1], atol=1e-15) and abs(wid.sum() - 9 * coarse) < 1e-14
+
+
+def test_refined_axis_builds_tables():
+    """A refined x axis (flux axis, dz = finest width) builds; the stencil weights of every cell are
+    a partition of unity."""
+    px, wx = cases.subdivided_axis([1, 2, 4, 4, 2, 1, 1, 1], 4.0)
+    pz, wz = cases.axis_from_widths(np.ones(6))
+    case = cases.Case(name="refined", lattice=9, n=(len(px), 1, 6), pos=[px, np.array([0.5]), pz],
+                      width=[wx, np.array([1.0]), wz], periodic=(1, 0, 0), flux_axes=_abi.FLUX_X)
+    w = _tables(case, 3, np.float64).reshape(3, 2, 5, -1)
+    assert np.abs(w[0].sum(axis=1) - 1).max() < 1e-13

@@ -197,3 +197,36 @@ def test_fused_matches_plain_fp64_lock_exchange():
 def test_fused_lock_exchange_fp16(oracle_lib):
     case = cases.lock_exchange(nx=32, ny=6, nz=16, storage=F16, path=PATH_FUSED)
     _compare(case, 150, ("vel", "S"), 2e-2, oracle_lib)
+
+
+def _refined_x(coarse=4.0):
+    """x refined as the falling-sphere driver does (SphereFall.cu:80-128), in lattice units: coarse
+    columns of 4 cells split in 4 near the middle and in 2 around it (finest width = dz)."""
+    divs = cases.sphere_fall_divs(10, coarse, inner=(12.0, 28.0), outer=(6.0, 34.0))
+    return cases.subdivided_axis(divs, coarse)
+
+
+def test_fused_refined_axis_fp64(oracle_lib):
+    px, wx = _refined_x()
+    pz, wz = cases.axis_from_widths(np.ones(16))
+    case = cases.Case(name="refined_x", lattice=9, n=(len(px), 1, 16), pos=[px, np.array([0.5]), pz],
+                      width=[wx, np.array([1.0]), wz], periodic=(1, 0, 0), wall_bc=_abi.BC_BOUNCE_BACK,
+                      flux_axes=_abi.FLUX_X, tau=0.6, path=PATH_FUSED, s_rates=np.array([0, 1.4, 1.4, 0, 1.2, 0, 1.2, 0, 0]))
+    X, _, Z = case.coords()
+    L = wx.sum()
+    case.init = {"u": 0.01 * np.sin(2 * np.pi * X / L) * np.cos(np.pi * Z / 16), "p": cases.uniform(11, case.n_cells, -1e-3, 1e-3)}
+    _compare(case, 150, FIELDS_FLOW, 1e-12, oracle_lib)
+
+
+def test_plain_refined_xy_corner_scheme_fp64(oracle_lib):
+    """x and y both refined (divx, divy): the reference's two-axis corner scheme, plain path."""
+    px, wx = cases.subdivided_axis([1, 2, 2, 1], 2.0)
+    py, wy = cases.subdivided_axis([1, 2, 1], 2.0)
+    pz, wz = cases.axis_from_widths(np.ones(5))
+    case = cases.Case(name="refined_xy", lattice=27, n=(len(px), len(py), 5), pos=[px, py, pz], width=[wx, wy, wz],
+                      wall_bc=_abi.BC_SLIP, flux_axes=_abi.FLUX_REFERENCE, tau=0.7, path=PATH_PLAIN,
+                      s_rates=np.full(27, 1.2))
+    X, Y, Z = case.coords()
+    case.init = {"u": 0.01 * np.sin(X / 2.0), "v": 0.01 * np.cos(Y / 2.0) * np.sin(Z),
+                 "p": cases.uniform(12, case.n_cells, -1e-3, 1e-3)}
+    _compare(case, 60, ("p", "vel"), 1e-12, oracle_lib)
