@@ -258,6 +258,11 @@ int ipl_set_graph(ipl_ctx* ctx, int32_t on);
 int ipl_set_open_boundary(ipl_ctx* ctx, const int32_t* cells, int64_t n, const double* p, const double* u,
                           const double* v, const double* w);
 
+/* Time-step change (reset_items_dt + reset_pressure, basic.cu:100-110 and 262-273; the driver's
+ * high-resolution start at LockExchange.cu:201-211): dt /= alpha, c *= alpha, p /= alpha^2 and
+ * every cell's f reset to feq(p, u) with the new c; tau is kept.  Fused path, no immersed body. */
+int ipl_change_dt(ipl_ctx* ctx, double alpha);
+
 /* ---- file writers (host side, after a readout of the fields) -------------------------------- */
 /* output() (basic.cu:3-30): "num_calc,N", the 15-column header and one "%f" row per calc cell
  * (x, y, z, delx, dely, pressure, salinity, phi, rho, u, v, w, Fx, Fy, Fz), the format the

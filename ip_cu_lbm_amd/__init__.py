@@ -92,6 +92,10 @@ class Simulation:
         self._check(self._lib.ipl_set_open_boundary(self._h, cells.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n,
                                                     *[_dp(x) for x in vals]))
 
+    def change_dt(self, alpha: float):
+        """dt / alpha, c * alpha, p / alpha^2, f reset to feq(p, u) (reset_items_dt + reset_pressure)."""
+        self._check(self._lib.ipl_change_dt(self._h, float(alpha)))
+
     def write_fields_csv(self, path: str):
         """The reference's output() file: num_calc line, 15-column header, one row per cell."""
         self._check(self._lib.ipl_write_fields_csv(self._h, os.fsencode(path)))

@@ -149,6 +149,8 @@ def load_library(path: str = None):
     lib.ipl_hydrostatic_pressure.restype = c_int32
     lib.ipl_wall_cells.argtypes = [POINTER(IplConfig), c_int32, POINTER(c_int32), POINTER(c_int64)]
     lib.ipl_wall_cells.restype = c_int32
+    lib.ipl_change_dt.argtypes = [vp, c_double]
+    lib.ipl_change_dt.restype = c_int32
     for name in ("ipl_write_fields_csv", "ipl_write_markers_csv", "ipl_write_vtu"):
         getattr(lib, name).argtypes = [vp, c_char_p]
         getattr(lib, name).restype = c_int32

@@ -437,6 +437,14 @@ int ipl_write_vtu(ipl_ctx* ctx, const char* path) {
     return ok ? IPL_OK : fail(ctx, ipl::Status::err(IPL_ERR_ARGUMENT, std::string("write failed: ") + path));
 }
 
+int ipl_change_dt(ipl_ctx* ctx, double alpha) {
+    if (!ctx) return IPL_ERR_ARGUMENT;
+    ipl::Status s = ctx->engine->change_dt(alpha);
+    if (!s) return fail(ctx, s);
+    ctx->cfg.dt /= alpha;
+    return IPL_OK;
+}
+
 int ipl_hydrostatic_pressure(const ipl_config* cfg, int32_t boussinesq, double center_z, double* p_out, int64_t count) {
     if (!cfg || !p_out) return IPL_ERR_ARGUMENT;
     ipl::Graph g;

@@ -91,6 +91,7 @@ public:
     virtual Status ib_body_state(double*) { return Status::err(IPL_ERR_ARGUMENT, "no body attached"); }
     virtual Status ib_read_markers(int, double*, size_t) { return Status::err(IPL_ERR_ARGUMENT, "no body attached"); }
     virtual int64_t ib_marker_count() const { return 0; }
+    virtual Status change_dt(double) { return Status::err(IPL_ERR_UNSUPPORTED, "time-step changes run on the fused path"); }
     // vals: (p, u, v, w) per listed cell
     virtual Status set_open_boundary(const int32_t*, int64_t, const double*) {
         return Status::err(IPL_ERR_UNSUPPORTED, "open boundaries run on the fused path");

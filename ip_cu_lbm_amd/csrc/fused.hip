@@ -119,23 +119,7 @@ public:
         P_.sigma = cfg_.sigma; P_.pf_thickness = cfg_.pf_thickness; P_.pf_tau = cfg_.pf_tau;
         P_.tau = hs.tau[0]; P_.taus = hs.taus[0];
         P_.boussinesq = cfg_.boussinesq; P_.collision = cfg_.collision;
-        P_.inv_c = 1.0 / P_.c; P_.inv_c2 = 1.0 / (P_.c * P_.c); P_.half_dt = P_.dt / 2.0;
-        P_.fk_coef = 3.0 * P_.dt / P_.c;
-        P_.itau = 1.0 / P_.tau; P_.om_s = 1.0 - 1.0 / P_.taus;
-        for (int r = 0; r < gr_.Q; ++r)
-            P_.rnorm[r] = (moment_row_is_stress(gr_.Q, r) ? P_.itau : P_.rates[r]) / moment_row_norm2(gr_.Q, r);
-        P_.press = P_.c * P_.c / 3.0;
-        P_.st_beta = cfg_.pf_thickness > 0 ? 4.0 * 12.0 * cfg_.sigma / cfg_.pf_thickness : 0.0;
-        P_.kap = 3.0 * cfg_.sigma * cfg_.pf_thickness / 2.0;
-        P_.inv_dz2 = 1.0 / (P_.dz * P_.dz);
-        P_.inv_dt = 1.0 / P_.dt;
-        P_.inv_pfW = cfg_.pf_thickness > 0 ? 1.0 / cfg_.pf_thickness : 0.0;
-        P_.pf_itau = cfg_.pf_tau > 0 ? 1.0 / cfg_.pf_tau : 0.0;
-        P_.pf_fcoef = cfg_.pf_tau - 0.5;
-        P_.wall_nu = cfg_.wall_nu;
-        for (int r = 0; r < gr_.Q; ++r) P_.inv_n2[r] = 1.0 / moment_row_norm2(gr_.Q, r);
-        P_.mu_L = cfg_.mu_L; P_.mu_H = cfg_.mu_H;
-        K_ = fusedk::make_coef<MT>(P_);
+        derive_coefficients();
         const int Q = gr_.Q, Qs = gr_.Qs;
         const int64_t NC = G.ncell, N = gr_.N;
         // internal <-> canonical permutation
@@ -304,7 +288,7 @@ public:
             }
             if (ob_n_ > 0)
                 IPL_LAUNCH((k_open_boundary<ST, Q>), (unsigned)((ob_n_ + 255) / 256), 256, 0, stream, fb_,
-                           (uint32_t)geo_.ncell, ob_idx_, ob_val_, ob_n_, gr_.c);
+                           (uint32_t)geo_.ncell, ob_idx_, ob_val_, ob_n_, P_.c);
             std::swap(fa_, fb_);
             std::swap(ga_, gb_);
             std::swap(Fa_, Fb_);
@@ -401,7 +385,7 @@ public:
         for (int a = 0; a < 3; ++a) { g.n[a] = geo_.n[a]; g.periodic[a] = geo_.periodic[a]; g.period[a] = gr_.period[a]; }
         g.s = geo_.s; g.o1 = geo_.o1; g.o2 = geo_.o2; g.ns = geo_.ns; g.ncell = geo_.ncell;
         g.dim = gr_.dim; g.Q = gr_.Q; g.Qs = gr_.Qs; g.scalar = cfg_.scalar;
-        g.dz = gr_.dz; g.dt = gr_.dt; g.c = gr_.c; g.tau = P_.tau;
+        g.dz = gr_.dz; g.dt = P_.dt; g.c = P_.c; g.tau = P_.tau;
         g.rho_L = cfg_.rho_L; g.rho_H = cfg_.rho_H; g.beta_s = cfg_.beta_s;
         Status r = ib_.attach(cfg, g, cfg_pos_, stream);
         if (r) {
@@ -414,6 +398,63 @@ public:
     Status ib_body_state(double* out) override { return ib_.body_state(out); }
     Status ib_read_markers(int which, double* out, size_t count) override { return ib_.read_markers(which, out, count); }
     int64_t ib_marker_count() const override { return ib_.markers(); }
+
+    // FP64 factors of the step from dz, dt, c and the rates, cast once into K_
+    void derive_coefficients() {
+        P_.inv_c = 1.0 / P_.c; P_.inv_c2 = 1.0 / (P_.c * P_.c); P_.half_dt = P_.dt / 2.0;
+        P_.fk_coef = 3.0 * P_.dt / P_.c;
+        P_.itau = 1.0 / P_.tau; P_.om_s = 1.0 - 1.0 / P_.taus;
+        for (int r = 0; r < gr_.Q; ++r)
+            P_.rnorm[r] = (moment_row_is_stress(gr_.Q, r) ? P_.itau : P_.rates[r]) / moment_row_norm2(gr_.Q, r);
+        P_.press = P_.c * P_.c / 3.0;
+        P_.st_beta = cfg_.pf_thickness > 0 ? 4.0 * 12.0 * cfg_.sigma / cfg_.pf_thickness : 0.0;
+        P_.kap = 3.0 * cfg_.sigma * cfg_.pf_thickness / 2.0;
+        P_.inv_dz2 = 1.0 / (P_.dz * P_.dz);
+        P_.inv_dt = 1.0 / P_.dt;
+        P_.inv_pfW = cfg_.pf_thickness > 0 ? 1.0 / cfg_.pf_thickness : 0.0;
+        P_.pf_itau = cfg_.pf_tau > 0 ? 1.0 / cfg_.pf_tau : 0.0;
+        P_.pf_fcoef = cfg_.pf_tau - 0.5;
+        P_.wall_nu = cfg_.wall_nu;
+        for (int r = 0; r < gr_.Q; ++r) P_.inv_n2[r] = 1.0 / moment_row_norm2(gr_.Q, r);
+        P_.mu_L = cfg_.mu_L; P_.mu_H = cfg_.mu_H;
+        K_ = fusedk::make_coef<MT>(P_);
+    }
+
+    // reset_items_dt + reset_pressure (basic.cu:100-110, 262-273): dt / alpha, c * alpha, then
+    // p / alpha^2 and f = feq(p, u) with the new c; u (a physical velocity) is kept
+    Status change_dt(double alpha) override {
+        if (!(alpha > 0)) return Status::err(IPL_ERR_ARGUMENT, "alpha must be positive");
+        if (ib_.active()) return Status::err(IPL_ERR_UNSUPPORTED, "a time-step change with an immersed body");
+        const int64_t N = gr_.N;
+        const int Q = gr_.Q;
+        std::vector<double> p(N), u(N), v(N), w(N);
+        Status r;
+        if (!(r = read(IPL_FIELD_P, p.data(), N)) || !(r = read(IPL_FIELD_U, u.data(), N)) ||
+            !(r = read(IPL_FIELD_V, v.data(), N)) || !(r = read(IPL_FIELD_W, w.data(), N)))
+            return r;
+        P_.dt /= alpha;
+        P_.c *= alpha;
+        derive_coefficients();
+        const double c = P_.c, c2 = c * c;
+        std::vector<double> f((size_t)Q * N);
+        for (int64_t i = 0; i < N; ++i) {
+            p[i] /= alpha * alpha;
+            const double u2 = (u[i] * u[i] + v[i] * v[i] + w[i] * w[i]) / c2;
+            for (int k = 0; k < Q; ++k) {
+                const Dir e = gr_.Q == 9 ? Lattice<9>::c(k) : Lattice<27>::c(k);
+                const double t = (e.x * c * u[i] + e.y * c * v[i] + e.z * c * w[i]) / c2;
+                const double wk = gr_.Q == 9 ? Lattice<9>::w(k) : (gr_.Q == 19 ? Lattice<19>::w(k) : Lattice<27>::w(k));
+                f[(size_t)k * N + i] = wk * (p[i] + 3.0 * t + 4.5 * t * t - 1.5 * u2);
+            }
+        }
+        drop_graph();  // the captured steps hold the old coefficients
+        if (!(r = write(IPL_FIELD_F, f.data(), f.size()))) return r;
+        // the next step starts from u itself (the reference's equ_f reads the kept velocity), so
+        // the force half-step that u = sum e c f + F dt / (2 rho) would add again is cleared
+        IPL_HIP_TRY(hipMemsetAsync(Fa_, 0, (size_t)3 * geo_.ncell * sizeof(MT), stream));
+        IPL_HIP_TRY(hipStreamSynchronize(stream));
+        return Status::ok();
+    }
 
     Status set_open_boundary(const int32_t* cells, int64_t n, const double* vals) override {
         if (std::is_same<ST, __half>::value) return Status::err(IPL_ERR_UNSUPPORTED, "open boundary needs FP64 or FP32 storage");

@@ -1577,6 +1577,25 @@ int64_t orc_table(void* h, int which, int32_t* out, int64_t count) {
 
 void orc_lagrange(const double* x, double xa, int n, double* coeff) { lagrange(coeff, x, xa, n); }
 
+// reset_items_dt + reset_pressure (basic.cu:100-110, 262-273): dt / alpha, c * alpha (the
+// lattice vectors are c times the integer directions), p / alpha^2, f = feq(p, u) with the new c
+int orc_change_dt(void* h, double alpha) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    o.dt /= alpha;
+    o.c *= alpha;
+    o.cfg.dt = o.dt;
+    const int Q = o.Q;
+    for (int id = 0; id < o.N; ++id) {
+        o.p[id] /= alpha * alpha;
+        for (int k = 0; k < Q; ++k) {
+            const double tmp = (o.L.cx[k] * o.c * o.u[id] + o.L.cy[k] * o.c * o.v[id] + o.L.cz[k] * o.c * o.w[id]) / (o.c * o.c);
+            o.fl.f[(size_t)id * Q + k] = o.L.w[k] * (o.p[id] + 3.0 * tmp + 4.5 * tmp * tmp -
+                1.5 * (o.u[id] * o.u[id] + o.v[id] * o.v[id] + o.w[id] * o.w[id]) / (o.c * o.c));
+        }
+    }
+    return 0;
+}
+
 double orc_spalding_utau(double abs_u, double y_dist, double nu) {
     return spalding_utau(abs_u, abs_u * std::pow(10.0, -7) + std::pow(10.0, -20), y_dist, nu);
 }

@@ -109,3 +109,28 @@ def test_wall_function_fp64(oracle_lib, make, storage):
     assert errs["vel"] <= tol and errs.get("S", 0.0) <= tol
     if storage == F64:
         assert errs["p"] <= tol
+
+
+@pytest.mark.parametrize("make", [
+    lambda: cases.lock_exchange(nx=24, ny=6, nz=12, path=PATH_FUSED),
+    lambda: cases.lattice_taylor_green(nx=48, nz=24, path=PATH_FUSED),
+], ids=["lock_exchange", "taylor_green"])
+def test_change_dt_fp64(oracle_lib, make):
+    """reset_items_dt + reset_pressure (basic.cu:100-110, 262-273) as the driver uses them
+    (LockExchange.cu:201-211): a high-resolution start (dt / 2), then back to the original dt."""
+    case = make()
+    fields = ("p", "u", "v", "w")
+    with Simulation(case) as sim:
+        o = oracle_lib.Oracle(case)
+        for k, alpha in ((20, 2.0), (30, 0.5), (25, None)):
+            sim.step(k)
+            o.step(k)
+            if alpha:
+                sim.change_dt(alpha)
+                o.change_dt(alpha)
+        g = {f: sim.read(f) for f in fields}
+        r = {f: o.read(f) for f in fields}
+        o.close()
+    errs = {"vel": rel_l2(np.stack([g[c] for c in "uvw"]), np.stack([r[c] for c in "uvw"])), "p": rel_l2(g["p"], r["p"])}
+    print(case.name, errs, flush=True)
+    assert max(errs.values()) <= 1e-12
