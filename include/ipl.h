@@ -212,7 +212,12 @@ typedef struct ipl_ib_config {
     double pos[3], vel[3];        /* initial body centre and velocity */
     double omega[3];              /* initial angular velocity, body frame (angleVB) */
     double quat[4];               /* initial orientation quaternion (q0, q1, q2, q3) */
-    int32_t reserved[8];
+    int32_t spm;                  /* 1: smoothed-profile forcing of the body interior before the
+                                     marker forcing (SPM_ellipse3D, SphereFall3D/SphereFall.cu:13-30,
+                                     called at :376); stress method */
+    int32_t reserved0;
+    double spm_a, spm_b;          /* SPM ellipsoid semi-axes (body frame x, y: a; z: b) */
+    int32_t reserved[2];
 } ipl_ib_config;
 
 /* Attaches the body to a context (once, before the first step); nodes of a marker must lie inside

@@ -72,7 +72,7 @@ class IplIbConfig(ctypes.Structure):
         ("spacing", c_double), ("rho_f", c_double), ("density", c_double), ("mass", c_double),
         ("inertia", c_double * 3), ("gravity", c_double * 3),
         ("pos", c_double * 3), ("vel", c_double * 3), ("omega", c_double * 3), ("quat", c_double * 4),
-        ("reserved", c_int32 * 8),
+        ("spm", c_int32), ("reserved0", c_int32), ("spm_a", c_double), ("spm_b", c_double), ("reserved", c_int32 * 2),
     ]
 
 

@@ -214,6 +214,9 @@ class Body:
     kernel: int = _abi.IB_KERNEL_REFERENCE
     iterations: int = 1
     moving: int = 1
+    spm: int = 0                  # smoothed-profile forcing of the interior (SPM_ellipse3D)
+    spm_a: float = 0.0
+    spm_b: float = 0.0
 
     def to_c(self):
         keep = []
@@ -225,6 +228,7 @@ class Body:
 
         c = _abi.IplIbConfig()
         c.method, c.kernel, c.iterations, c.moving = self.method, self.kernel, self.iterations, self.moving
+        c.spm, c.spm_a, c.spm_b = self.spm, self.spm_a, self.spm_b
         c.n_markers = len(self.markers)
         c.marker_pos = dp(self.markers)
         c.marker_normal = dp(self.normals)

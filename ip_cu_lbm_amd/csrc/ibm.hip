@@ -24,6 +24,8 @@ struct IbArgs {
     int method, kernel, moving;
     double spread, wscale, gwscale;
     double sina, inv_c, inv_c2, half_itau;  // stress method: sin of the split angle, 1/c, 1/c^2, 1/(2 tau)
+    int spm, bpad;                          // smoothed-profile forcing; extra box cells it needs
+    double spm_a, spm_b;
     double rho_f, density, mass, inertia[3], gravity[3];
     int64_t vmax;
     const double* pos[3];
@@ -173,7 +175,7 @@ __global__ void __launch_bounds__(kIbRankThreads) k_ib_rank(IbArgs A, const int*
     if (t == 0) {
         int64_t vol = 1;
         for (int a = 0; a < 3; ++a) {
-            int l = work[a][0] + A.lo, h = work[3 + a][0] + A.hi;
+            int l = work[a][0] + A.lo - A.bpad, h = work[3 + a][0] + A.hi + A.bpad;
             if (A.periodic[a]) {
                 if (h - l + 1 >= A.n[a]) { l = 0; h = A.n[a] - 1; }
                 else if (l < 0) l += A.n[a];
@@ -252,6 +254,65 @@ __global__ void k_ib_ubase(IbArgs A, const ST* __restrict__ f, const ST* __restr
     Ubox[3 * b + 0] = jx + (double)F[idx] * A.dt / (2.0 * rho);
     Ubox[3 * b + 1] = jy + (double)F[A.ncell + idx] * A.dt / (2.0 * rho);
     Ubox[3 * b + 2] = jz + (double)F[2 * (size_t)A.ncell + idx] * A.dt / (2.0 * rho);
+}
+
+// ---- k_ib_spm: SPM_ellipse3D (SphereFall3D/SphereFall.cu:13-30) on the cells of the box --------
+// u of the cell as update_rho leaves it (sum e c f + F dt / (2 rho), the step's force), the
+// body-frame ellipsoid distance, the driver's profile (profile_s22: transition width fixed at
+// 0.1), and f += w dt 3 (e c . F) / c^2 with F = profile (v_B - u) / dt
+__device__ inline double spm_profile(double r, double Rada) {
+    const double lim = 0.1;
+    const double in_line = 1.0 - lim + (lim / 2) * (lim / 2), out_line = 1.0 + lim + (lim / 2) * (lim / 2);
+    if (r < in_line) return 1.0;
+    if (r <= out_line) return 0.5 * (sin(3.14159 * r / lim / Rada) + 1.0);
+    return 0.0;
+}
+
+template <typename ST, typename MT, int Q>
+__global__ void k_ib_spm(IbArgs A, ST* __restrict__ f, const ST* __restrict__ g, const MT* __restrict__ F,
+                         const int* __restrict__ box, const double* __restrict__ body) {
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t vol = (int64_t)box[3] * box[4] * box[5];
+    if (b >= vol || b >= A.vmax) return;
+    int c[3];
+    c[2] = (int)(b % box[5]);
+    c[1] = (int)((b / box[5]) % box[4]);
+    c[0] = (int)(b / ((int64_t)box[4] * box[5]));
+    for (int a = 0; a < 3; ++a) {
+        c[a] += box[a];
+        if (c[a] >= A.n[a]) c[a] -= A.n[a];
+    }
+    const double* R = body + B_R;
+    const double d[3] = {A.pos[0][c[0]] - body[B_POS], A.pos[1][c[1]] - body[B_POS + 1], A.pos[2][c[2]] - body[B_POS + 2]};
+    const double X1 = R[0] * d[0] + R[3] * d[1] + R[6] * d[2];
+    const double Y1 = R[1] * d[0] + R[4] * d[1] + R[7] * d[2];
+    const double Z1 = R[2] * d[0] + R[5] * d[1] + R[8] * d[2];
+    const double distance = (X1 / A.spm_a) * (X1 / A.spm_a) + (Y1 / A.spm_a) * (Y1 / A.spm_a) + (Z1 / A.spm_b) * (Z1 / A.spm_b);
+    const double pr = spm_profile(distance, A.spm_a);
+    if (pr == 0.0) return;
+    const uint32_t idx = cell_of(A, c);
+    const double cl = A.c;
+    double jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const Dir e = Lattice<Q>::c(k);
+        const double fv = ldpop(f, (uint32_t)k * A.ncell + idx);
+        jx += e.x * cl * fv; jy += e.y * cl * fv; jz += e.z * cl * fv;
+    }
+    double sc = 0;
+    for (int k = 0; k < A.Qs; ++k) sc += ldpop(g, (uint32_t)k * A.ncell + idx);
+    const double phi = A.scalar == IPL_SCALAR_PHASE_FIELD ? sc : 0.0, S = A.scalar == IPL_SCALAR_SALINITY ? sc : 0.0;
+    const double rho = A.rho_L + A.drho * phi + S * A.beta_s;
+    const double u[3] = {jx + (double)F[idx] * A.dt / (2.0 * rho), jy + (double)F[A.ncell + idx] * A.dt / (2.0 * rho),
+                         jz + (double)F[2 * (size_t)A.ncell + idx] * A.dt / (2.0 * rho)};
+    double fs[3];
+    for (int a = 0; a < 3; ++a) fs[a] = pr * (body[B_VEL + a] - u[a]) / A.dt;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const Dir e = Lattice<Q>::c(k);
+        ST& fv = f[(uint32_t)k * A.ncell + idx];
+        fv = (ST)((double)fv + Lattice<Q>::w(k) * A.dt * 3.0 * (e.x * cl * fs[0] + e.y * cl * fs[1] + e.z * cl * fs[2]) / (cl * cl));
+    }
 }
 
 // ---- k_ib_force: one 64-lane workgroup per marker, lanes over the slots of its node block ----
@@ -674,6 +735,8 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
         return Status::err(IPL_ERR_ARGUMENT, "the stress method uses the reference weights");
     if (cfg.kernel != IPL_IB_KERNEL_REFERENCE && cfg.kernel != IPL_IB_KERNEL_PESKIN4)
         return Status::err(IPL_ERR_ARGUMENT, "IB kernel");
+    if (cfg.spm && (cfg.method != IPL_IB_STRESS || !(cfg.spm_a > 0) || !(cfg.spm_b > 0)))
+        return Status::err(IPL_ERR_ARGUMENT, "the smoothed profile needs the stress method and positive semi-axes");
     if (!(cfg.mass > 0) || !(cfg.density > 0) || !(cfg.spacing > 0))
         return Status::err(IPL_ERR_ARGUMENT, "body mass, density and marker spacing must be positive");
     if (cfg.moving && !(cfg.inertia[0] > 0 && cfg.inertia[1] > 0 && cfg.inertia[2] > 0))
@@ -726,9 +789,16 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
     }
     // box bound: the marker nodes span at most 2 rmax / dz + 3 cells per axis (widths >= dz);
     // a periodic axis may be covered whole when the body straddles its seam
+    // the smoothed profile reaches sqrt(1.1025) semi-axes from the centre: pad the box beyond the
+    // marker node blocks by what that exceeds of the marker radius
+    bpad_ = 0;
+    if (cfg.spm) {
+        const double reach = std::sqrt(1.1025) * std::max(cfg.spm_a, cfg.spm_b);
+        bpad_ = std::max(0, (int)std::ceil((reach - rmax) / g.dz) + 1);
+    }
     vmax_ = 1;
     for (int a = 0; a < 3; ++a) {
-        int ext = (int)std::ceil(2.0 * rmax / g.dz) + 3 + (reach_hi_ - reach_lo_);
+        int ext = (int)std::ceil(2.0 * rmax / g.dz) + 3 + (reach_hi_ - reach_lo_) + 2 * bpad_;
         if (g.dim == 2 && a == 1) ext = 1;
         if (g.periodic[a] || ext > g.n[a]) ext = std::min(g.periodic[a] ? g.n[a] : ext, g.n[a]);
         box_max_[a] = ext;
@@ -773,7 +843,7 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
 namespace {
 
 ibk::IbArgs make_args(const IbGrid& g, const ipl_ib_config& cfg, int M, int S, int lo, int hi, int64_t vmax,
-                      double* const pos[3]) {
+                      double* const pos[3], int bpad) {
     ibk::IbArgs A{};
     for (int a = 0; a < 3; ++a) { A.n[a] = g.n[a]; A.periodic[a] = g.periodic[a]; A.pos[a] = pos[a]; }
     A.s = g.s; A.o1 = g.o1; A.o2 = g.o2; A.ns = g.ns;
@@ -800,6 +870,7 @@ ibk::IbArgs make_args(const IbGrid& g, const ipl_ib_config& cfg, int M, int S, i
     A.rho_f = cfg.rho_f; A.density = cfg.density; A.mass = cfg.mass;
     for (int a = 0; a < 3; ++a) { A.inertia[a] = cfg.inertia[a]; A.gravity[a] = cfg.gravity[a]; }
     A.vmax = vmax;
+    A.spm = cfg.spm; A.bpad = bpad; A.spm_a = cfg.spm_a; A.spm_b = cfg.spm_b;
     return A;
 }
 
@@ -832,12 +903,19 @@ void apply_q(int Q, const ibk::IbArgs& A, ST* f, const int* box, const double* F
 
 Status IbSolver::substep(void* f, const void* g, const void* F, int storage) {
     if (M_ == 0) return Status::ok();
-    const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_);
+    const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_, bpad_);
     constexpr int kPad = 2 * ibk::kIbRankParts;
     const int mpad = (M_ + kPad - 1) / kPad * kPad;
     IPL_LAUNCH_SYNC(ibk::k_ib_rank, (unsigned)((M_ + ibk::kIbRankMarkers - 1) / ibk::kIbRankMarkers), ibk::kIbRankThreads,
                     (size_t)mpad * sizeof(unsigned long long), st_, A, node_, sorted_, box_, range_, mpad);
     const int iters = cfg_.method == IPL_IB_DIRECT ? std::max(1, cfg_.iterations) : 1;
+    if (cfg_.spm) {  // before the marker forcing (SphereFall.cu:376-377)
+        const unsigned blocks = (unsigned)((vmax_ + 255) / 256);
+#define IPL_IBS(ST, MT, QQ) IPL_LAUNCH((ibk::k_ib_spm<ST, MT, QQ>), blocks, 256, 0, st_, A, (ST*)f, (const ST*)g, (const MT*)F, box_, body_)
+        if (storage == IPL_F64) { if (g_.Q == 9) IPL_IBS(double, double, 9); else if (g_.Q == 19) IPL_IBS(double, double, 19); else IPL_IBS(double, double, 27); }
+        else { if (g_.Q == 9) IPL_IBS(float, float, 9); else if (g_.Q == 19) IPL_IBS(float, float, 19); else IPL_IBS(float, float, 27); }
+#undef IPL_IBS
+    }
     if (cfg_.method == IPL_IB_DIRECT) {
         if (storage == IPL_F64) ubase_q<double, double>(g_.Q, A, (const double*)f, (const double*)g, (const double*)F, box_, Ubox_, st_);
         else ubase_q<float, float>(g_.Q, A, (const float*)f, (const float*)g, (const float*)F, box_, Ubox_, st_);

@@ -67,6 +67,7 @@ private:
     ipl_ib_config cfg_{};
     hipStream_t st_ = nullptr;
     int M_ = 0, S_ = 0, reach_lo_ = -1, reach_hi_ = 1;
+    int bpad_ = 0;  // extra box cells on every side for the smoothed-profile forcing
     int64_t vmax_ = 0;
     int box_max_[3] = {0, 0, 0};
     DeviceArena mem_;

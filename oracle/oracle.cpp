@@ -1273,10 +1273,41 @@ void ib_points(Oracle& o) {
 
 // The IB sub-step after update_rho and resetF (SphereFall.cu:372-383), then resetF
 // (SphereFall.cu:433).
+// profile_s22 (SphereFall3D/SphereFall.cu:3-11): 1 inside, 0 outside, the driver's sine ramp
+// between; the transition width is fixed at 0.1 whatever the argument
+double spm_profile(double r, double Rada) {
+    const double lim = 0.1;
+    const double in_line = 1.0 - lim + std::pow(lim / 2, 2), out_line = 1.0 + lim + std::pow(lim / 2, 2);
+    return 0 * (out_line < r) + (0.5 * (std::sin(3.14159 * r / lim / Rada) + 1.0)) * (in_line <= r && r <= out_line) +
+           1.0 * (r < in_line);
+}
+
+// SPM_ellipse3D (SphereFall3D/SphereFall.cu:13-30): forcing (v_B - u) / dt on the body interior,
+// weighted by the profile of the body-frame ellipsoid distance, added to f
+void ib_spm(Oracle& o) {
+    const auto& B = o.ib;
+    const double a = B.cfg.spm_a, b = B.cfg.spm_b, c2 = o.c * o.c;
+    const int Q = o.Q;
+    for (int id = 0; id < o.N; ++id) {
+        const double d[3] = {o.pos[0][id] - B.posB[0], o.pos[1][id] - B.posB[1], o.pos[2][id] - B.posB[2]};
+        const double X1 = B.quaS[0] * d[0] + B.quaS[3] * d[1] + B.quaS[6] * d[2];
+        const double Y1 = B.quaS[1] * d[0] + B.quaS[4] * d[1] + B.quaS[7] * d[2];
+        const double Z1 = B.quaS[2] * d[0] + B.quaS[5] * d[1] + B.quaS[8] * d[2];
+        const double distance = std::pow(X1 / a, 2) + std::pow(Y1 / a, 2) + std::pow(Z1 / b, 2);
+        const double pr = spm_profile(distance, a);
+        if (pr == 0.0) continue;
+        const double fx = pr * (B.velB[0] - o.u[id]) / o.dt, fy = pr * (B.velB[1] - o.v[id]) / o.dt,
+                     fz = pr * (B.velB[2] - o.w[id]) / o.dt;
+        for (int k = 0; k < Q; ++k)
+            o.fl.f[(size_t)id * Q + k] += o.L.w[k] * o.dt * 3.0 * (o.L.cx[k] * o.c * fx + o.L.cy[k] * o.c * fy + o.L.cz[k] * o.c * fz) / c2;
+    }
+}
+
 void ib_substep(Oracle& o) {
     auto& B = o.ib;
     reset_force(o);
     std::fill(B.Gw.begin(), B.Gw.end(), 0.0);
+    if (B.cfg.spm) ib_spm(o);
     if (B.cfg.method == IPL_IB_STRESS) ib_stress(o);
     else ib_direct(o);
     ib_update_vel(o);

@@ -105,3 +105,16 @@ def test_ib_deterministic():
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][2], out[1][2])
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_ib_smoothed_profile_fp64(oracle_lib, q):
+    """Stress method with the driver's smoothed-profile interior forcing (SPM_ellipse3D,
+    SphereFall3D/SphereFall.cu:13-30, called before get_IBMGw2 at :376): semi-axes R, R as there."""
+    c = _sphere(q=q)
+    c.body.spm, c.body.spm_a, c.body.spm_b = 1, 4.0, 4.0
+    gb, ob = _compare(c, 30, oracle_lib, 1e-12)
+    c0 = _sphere(q=q)
+    with Simulation(c0) as s:
+        s.step(30)
+        assert np.abs(s.body_state()["force"] - gb["force"]).max() > 1e-8  # the interior forcing acts
