@@ -58,6 +58,7 @@ def load():
         lib.orc_wall_cells.argtypes = [c_void_p, c_int, POINTER(c_int32)]
         lib.orc_wall_cells.restype = c_int
         lib.orc_change_dt.argtypes = [c_void_p, c_double]
+        lib.orc_les.argtypes = [c_void_p]
         lib.orc_spalding_utau.argtypes = [c_double, c_double, c_double]
         lib.orc_spalding_utau.restype = c_double
         _LIB = lib
@@ -171,6 +172,9 @@ class Oracle:
         out = np.empty(self.n_calc)
         self._lib.orc_hydrostatic_pressure(self._h, 1 if boussinesq else 0, float(center_z), _dp(out))
         return out
+
+    def les(self):
+        self._lib.orc_les(self._h)
 
     def change_dt(self, alpha: float):
         self._lib.orc_change_dt(self._h, float(alpha))

@@ -1627,6 +1627,12 @@ int orc_change_dt(void* h, double alpha) {
     return 0;
 }
 
+// the LES pass alone on the current macroscopic fields (for its closed-form check)
+int orc_les(void* h) {
+    les(*static_cast<Oracle*>(h));
+    return 0;
+}
+
 double orc_spalding_utau(double abs_u, double y_dist, double nu) {
     return spalding_utau(abs_u, abs_u * std::pow(10.0, -7) + std::pow(10.0, -20), y_dist, nu);
 }
@@ -1733,6 +1739,24 @@ int orc_lattice(int Q, int32_t* c, double* w) {
  * is a fixed point; MRT with all rates 1/tau equals SRT; uniform grid = exact shift; periodic
  * translation invariance; volume-weighted mass conservation with walls; Poiseuille profile with
  * bounce-back; slip keeps a plug flow; set_bound2D tables against the mirror rule.
+ * Immersed boundary (tests/test_oracle_ibm.py): weightFunction's Peskin moments (sum 1, first
+ * moment 0, sum of squares 3/8, support); set_quaternionS against scipy's rotation; the marker
+ * generator against the reference's sphere_points_n7.csv; the stress method under a uniform p
+ * gives the closed-form pressure traction and leaves the fluid at rest; direct forcing in a
+ * uniform stream gives (u0/dt) rho_f dIBM^2 dz per marker and the matching momentum loss.
+ * update_IBbody / update_IBpoint / search_IBlattice: pinned through the GPU comparison only (two
+ * independent restatements agreeing at 1e-12) and the falling direction; parity unpinned
+ * against the reference itself.
+ * Open boundary (set_velocity1 + set_wall_boundary1): a uniform stream held by it stays uniform
+ * and the open cells hold exactly the reference's equilibrium (tests/test_boundary_setters.py).
+ * hydrostatic_pressure: closed form on a stretched grid.  set_walln: brute force over
+ * coordinates, with and without solid cells.
+ * wall_function: its Newton solve against scipy's root of Spalding's law (within the reference's
+ * regularisation); the face mapping and the force application are pinned only through the GPU
+ * comparison: parity unpinned against the reference itself.
+ * LES: a linear velocity field (exact central differences) gives the closed-form tau and tau_s
+ * at every interior cell (tests/test_boundary_setters.py::test_les_closed_form).
+ * SPM_ellipse3D, reset_items_dt + reset_pressure: parity unpinned in the same sense.
  * Parity unpinned: the reference's own FP32 trajectories on NVIDIA hardware (CUDA device math
  * and atomicAdd ordering cannot run here); the 3D two-axis corner scheme (set_out num3 /
  * set_fin) is pinned only by mass conservation and the uniform-grid reduction.

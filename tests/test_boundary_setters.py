@@ -108,3 +108,37 @@ def test_wall_function_newton_solves_spalding_law(oracle_lib, abs_u, y, nu):
     ref = brentq(resid, 1e-8 * abs_u, abs_u, xtol=1e-16, rtol=1e-15)
     got = oracle_lib.spalding_utau(abs_u, y, nu)
     assert abs(got - ref) <= 2e-6 * ref, (got, ref)
+
+
+def test_les_closed_form(oracle_lib):
+    """LES (basic.cu:285-339) on u = a x, v = b y, w = c z: the central differences are exact, so
+    at every interior cell nabla = diag(a, b, c), Q = -(a^2 + b^2 + c^2)/2, E = 1/(2 (a+b+c)^2),
+    F = clip(Q/E), C = 0.05 |F|^(3/2), S = sqrt(2) (|a| + |b| + |c|), tau = 3 (nu + C S dz^2)/(c^2 dt)
+    + 1/2 and tau_s = 3 (1e-9 + C S dz^2 / 1000)/(c^2 dt) + 1/2, with nu = mu_L / rho."""
+    a, b, c = 0.3, -0.2, 0.5
+    case = cases.lock_exchange(nx=8, ny=7, nz=9).replace(flux_axes=_abi.FLUX_NONE, les=1, mu_L=0.01, mu_H=0.01,
+                                                        scalar=_abi.SCALAR_NONE, scalar_lattice=0, gravity=0.0)
+    pos = [cases.axis_from_widths(np.ones(n))[0] for n in case.n]
+    case = case.replace(pos=pos, width=[np.ones(n) for n in case.n])
+    X, Y, Z = case.coords()
+    case.init = {"u": a * X.ravel(), "v": b * Y.ravel(), "w": c * Z.ravel()}
+    o = oracle_lib.Oracle(case)
+    o.les()
+    tau, taus = o.read("tau").reshape(case.n), o.read("taus").reshape(case.n)
+    o.close()
+    Q = -(a * a + b * b + c * c) / 2
+    E = 1 / (2 * (a + b + c) ** 2 + 1e-40)
+    F = max(-1.0, min(Q / E, 1.0))
+    C = 0.05 * abs(F) ** 1.5
+    S = np.sqrt(2) * (abs(a) + abs(b) + abs(c))
+    # the grid's own dz/1000 guard in the differences: (v+ - v-)/(2 + dz/1000)
+    g = 2.0 / (2.0 + 1e-3)
+    Qg, Eg = Q * g * g, 1 / (2 * ((a + b + c) * g) ** 2 + 1e-40)
+    Cg = 0.05 * abs(max(-1.0, min(Qg / Eg, 1.0))) ** 1.5
+    Sg = S * g
+    want_tau = 3 * (0.01 / 1.0 + Cg * Sg) + 0.5
+    want_taus = 3 * (1e-9 + Cg * Sg / 1000) + 0.5
+    inner = (slice(1, -1),) * 3
+    assert np.abs(tau[inner] - want_tau).max() < 1e-14
+    assert np.abs(taus[inner] - want_taus).max() < 1e-15
+    assert Cg * Sg > 1e-6  # the eddy part is visible at this size
