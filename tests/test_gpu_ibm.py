@@ -118,3 +118,18 @@ def test_ib_smoothed_profile_fp64(oracle_lib, q):
     with Simulation(c0) as s:
         s.step(30)
         assert np.abs(s.body_state()["force"] - gb["force"]).max() > 1e-8  # the interior forcing acts
+
+
+def test_ib_most_markers_fp64(oracle_lib):
+    """4096 markers, the most one body takes (the rank kernel's LDS), against the oracle; 4097 is
+    refused with an error."""
+    n = (28, 28, 30)
+    c = cases.falling_sphere(n=n, diameter=14, markers=4096, storage=F64, path=PATH_FUSED, tau=0.7, gravity=1e-4,
+                             method=_abi.IB_STRESS, kernel=_abi.IB_KERNEL_REFERENCE)
+    c.body.pos = (14.2, 13.9, 15.3)
+    _compare(c, 12, oracle_lib, 1e-12)
+    over = cases.falling_sphere(n=n, diameter=14, markers=4097, storage=F64, path=PATH_FUSED,
+                                method=_abi.IB_STRESS, kernel=_abi.IB_KERNEL_REFERENCE)
+    over.body.pos = (14.2, 13.9, 15.3)
+    with pytest.raises(_abi.IplError):
+        Simulation(over)

@@ -230,3 +230,17 @@ def test_plain_refined_xy_corner_scheme_fp64(oracle_lib):
     case.init = {"u": 0.01 * np.sin(X / 2.0), "v": 0.01 * np.cos(Y / 2.0) * np.sin(Z),
                  "p": cases.uniform(12, case.n_cells, -1e-3, 1e-3)}
     _compare(case, 60, ("p", "vel"), 1e-12, oracle_lib)
+
+
+def test_fused_longest_line_fp32(oracle_lib):
+    """Lines of 512 cells, the longest the fused path takes (LDS rows of 28 x 512 x 4 B):
+    FP32 within 1e-5 of the FP64 oracle; one cell more and AUTO picks the plain path."""
+    case = cases.lock_exchange(nx=4, ny=3, nz=512, storage=F32, path=_abi.PATH_AUTO)
+    with Simulation(case) as sim:
+        assert sim.path == PATH_FUSED
+    _compare(case, 40, ("vel", "S"), 1e-5, oracle_lib)
+    longer = cases.lock_exchange(nx=2, ny=2, nz=513, storage=F32, path=_abi.PATH_AUTO)
+    with Simulation(longer) as sim:
+        assert sim.path == PATH_PLAIN
+    with pytest.raises(_abi.IplError):
+        Simulation(longer.replace(path=PATH_FUSED))
