@@ -1744,9 +1744,9 @@ int orc_lattice(int Q, int32_t* c, double* w) {
  * generator against the reference's sphere_points_n7.csv; the stress method under a uniform p
  * gives the closed-form pressure traction and leaves the fluid at rest; direct forcing in a
  * uniform stream gives (u0/dt) rho_f dIBM^2 dz per marker and the matching momentum loss.
- * update_IBbody / update_IBpoint / search_IBlattice: pinned through the GPU comparison only (two
- * independent restatements agreeing at 1e-12) and the falling direction; parity unpinned
- * against the reference itself.
+ * update_IBbody / update_IBpoint / search_IBlattice: the first step of a body in fluid at rest
+ * (buoyancy only) against the AB2 closed form, rigid marker motion and brute-force nearest cells;
+ * later steps (torque, quaternion AB2) only through the GPU comparison.
  * Open boundary (set_velocity1 + set_wall_boundary1): a uniform stream held by it stays uniform
  * and the open cells hold exactly the reference's equilibrium (tests/test_boundary_setters.py).
  * hydrostatic_pressure: closed form on a stretched grid.  set_walln: brute force over

@@ -108,3 +108,35 @@ def test_direct_forcing_uniform_stream(oracle_lib, kernel):
     du = o.read("u").sum() - N * u0
     assert abs(du + M * per) < 1e-12
     o.close()
+
+
+def test_body_update_first_step_closed_form(oracle_lib):
+    """update_IBbody / update_IBpoint / search_IBlattice (ibm.cu:367-463) on a body in fluid at
+    rest with p = 0: the stress traction vanishes, so the first step sees only buoyancy
+    F = (1 - rho_f/rho_B) m g.  AB2 from rest gives v1 = dt 3F/(2m), x1 = x0 + dt 3 v1 / 2; the
+    markers move rigidly with the body (R = I) and their nearest cells follow."""
+    g = (0.0, 0.0, -3e-4)
+    c = _box_case(n=(16, 16, 20), periodic=(0, 0, 0), body_kw={"method": _abi.IB_STRESS, "kernel": _abi.IB_KERNEL_REFERENCE,
+                                                                "gravity": g, "moving": 1})
+    b0 = c.body
+    o = oracle_lib.Oracle(c)
+    x0 = o.markers("pos").copy()
+    o.step(1)
+    st = o.body_state()
+    m, dens = b0.mass, b0.density
+    F = (1 - b0.rho_f / dens) * m * np.array(g)
+    v1 = c.dt * 3 * F / (2 * m)
+    x1 = np.array(b0.pos) + c.dt * 3 * v1 / 2
+    assert np.abs(st["force"] - F).max() < 1e-18
+    assert np.abs(st["vel"] - v1).max() < 1e-18
+    assert np.abs(st["pos"] - x1).max() < 1e-15
+    assert np.abs(st["omega"]).max() == 0.0
+    assert np.abs(o.markers("pos") - (x0 + (x1 - np.array(b0.pos)))).max() < 1e-14
+    # nearest cells: brute force over the cell centres
+    X, Y, Z = c.coords()
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    node = o.markers("node").astype(int)
+    for mk in range(0, len(node), 7):
+        d = ((pts - o.markers("pos")[mk]) ** 2).sum(1)
+        assert d[node[mk]] <= d.min() + 1e-12
+    o.close()
