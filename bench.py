@@ -84,12 +84,12 @@ def measure_serving(n_jobs=256):
     """The serving mix: 256 seeded Poisson arrivals (mean gap 20 ms) of D2Q9/D3Q19 FP32 jobs of
     mixed sizes (2D sides 64-512, 3D sides 32-128, 500-5000 steps, ~30 % with salinity, ~10 % with
     an immersed body), admitted under a 200 GB device budget and advanced by continuous batching
-    in chunks of 200 steps.  Latency is completion - arrival; aggregate MLUPS is all jobs' cell
+    in chunks of 400 steps.  Latency is completion - arrival; aggregate MLUPS is all jobs' cell
     updates over the makespan (first arrival to last completion), busy MLUPS over the time spent
     in batch steps."""
     from ip_cu_lbm_amd import serve
     jobs = serve.serving_mix(n_jobs=n_jobs)
-    res = serve.Server(budget_bytes=200e9, chunk=200).run(jobs)
+    res = serve.Server(budget_bytes=200e9, chunk=400).run(jobs)
     res["workload"] = f"serving_mix_{n_jobs}_jobs_fp32"
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res.items()}
 

@@ -106,7 +106,7 @@ class Server:
     threads while the batch steps on the device, so set-up of new arrivals overlaps the running
     jobs' work; a job joins the batch at the first chunk boundary after its set-up ends."""
 
-    def __init__(self, budget_bytes: float = 200e9, chunk: int = 200, library: str = None, workers: int = 4):
+    def __init__(self, budget_bytes: float = 200e9, chunk: int = 400, library: str = None, workers: int = 4):
         self.budget = int(budget_bytes)
         self.chunk = chunk
         self.library = library
