@@ -328,35 +328,69 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
                                                       const double* __restrict__ posw, const double* __restrict__ nB,
                                                       const double* __restrict__ velw, double* __restrict__ Gw,
                                                       double* __restrict__ contrib, const int* __restrict__ box,
-                                                      const double* __restrict__ Fbox, const double* __restrict__ Ubox) {
+                                                      const double* __restrict__ Fbox, const double* __restrict__ Ubox,
+                                                      double* __restrict__ wcache, int* __restrict__ bcache) {
     constexpr int kRows = Q > 20 ? Q : 20;  // the slot sums, then the 20 moment sums
     __shared__ double red[kIbLanes * kRows];
     __shared__ double share[3];
     const int m = blockIdx.x, t = threadIdx.x;
-    const int base[3] = {node[3 * m], node[3 * m + 1], node[3 * m + 2]};
-    const double xw[3] = {posw[3 * m], posw[3 * m + 1], posw[3 * m + 2]};
     const double U[3] = {velw[3 * m], velw[3 * m + 1], velw[3 * m + 2]};
-    double W[3][5];
-    axis_weights(A, base, xw, W);
     const double c = A.c, c2 = c * c;
     double* out = contrib + (size_t)m * A.S * 3;
     const int nr = A.method == IPL_IB_STRESS ? Q : 3;
+    // the lane's slots (at most two: S <= 2 x 64): node weight, node valid (not a ghost) and its
+    // cell (stress) or box index (direct forcing).  Direct forcing's later passes read them back
+    // from the first pass's cache: the markers do not move between passes.
+    double wsl[2] = {0.0, 0.0};
+    bool oksl[2] = {false, false};
+    int64_t bsl[2] = {-1, -1};
+    uint32_t isl[2] = {0, 0};
+    const bool direct = A.method == IPL_IB_DIRECT;
+    if (direct && it > 0) {
+        for (int j = 0; j < 2; ++j) {
+            const int s = t + j * kIbLanes;
+            if (s >= A.S) break;
+            const int bc = bcache[(size_t)m * A.S + s];
+            wsl[j] = wcache[(size_t)m * A.S + s];
+            oksl[j] = bc != -2;
+            bsl[j] = bc;
+        }
+    } else {
+        const int base[3] = {node[3 * m], node[3 * m + 1], node[3 * m + 2]};
+        const double xw[3] = {posw[3 * m], posw[3 * m + 1], posw[3 * m + 2]};
+        double W[3][5];
+        axis_weights(A, base, xw, W);
+        for (int j = 0; j < 2; ++j) {
+            const int s = t + j * kIbLanes;
+            if (s >= A.S) break;
+            int d[3], cc[3];
+            double pp[3];
+            slot_offset(A, s, d);
+            oksl[j] = node_at(A, base, d, cc, pp);
+            wsl[j] = slot_weight(A, W, s);
+            if (oksl[j]) {
+                if (direct) bsl[j] = box_index(A, box, cc);
+                else isl[j] = cell_of(A, cc);
+            }
+            if (direct) {
+                wcache[(size_t)m * A.S + s] = wsl[j];
+                bcache[(size_t)m * A.S + s] = oksl[j] ? (int)bsl[j] : -2;
+            }
+        }
+    }
     // 1. per-lane partial sums over the lane's slots
     double part[Q > 3 ? Q : 3];
 #pragma unroll
     for (int k = 0; k < (Q > 3 ? Q : 3); ++k) part[k] = 0.0;
-    for (int s = t; s < A.S; s += kIbLanes) {
-        int d[3], cc[3];
-        double pp[3];
-        slot_offset(A, s, d);
-        if (!node_at(A, base, d, cc, pp)) continue;
-        const double wgt = slot_weight(A, W, s);
-        if (A.method == IPL_IB_STRESS) {
-            const uint32_t idx = cell_of(A, cc);
+    for (int j = 0; j < 2; ++j) {
+        if (!oksl[j]) continue;
+        const double wgt = wsl[j];
+        if (!direct) {
+            const uint32_t idx = isl[j];
 #pragma unroll
             for (int k = 0; k < Q; ++k) part[k] += wgt * ldpop(f, (uint32_t)k * A.ncell + idx);
         } else {
-            const int64_t b = box_index(A, box, cc);
+            const int64_t b = bsl[j];
             if (b < 0 || b >= A.vmax) continue;  // cannot happen: the box holds every node block
             for (int a = 0; a < 3; ++a) {
                 double u = Ubox[3 * b + a];
@@ -451,12 +485,11 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
         if (t == 0)
             for (int a = 0; a < 3; ++a) Gw[3 * m + a] = tr[1][a] * A.gwscale;
         const double gs0 = -tr[0][0], gs1 = -tr[0][1], gs2 = -tr[0][2];
-        for (int s = t; s < A.S; s += kIbLanes) {
-            int d[3], cc[3];
-            double pp[3];
-            slot_offset(A, s, d);
-            const bool ok = node_at(A, base, d, cc, pp);
-            const double wgt = slot_weight(A, W, s);
+        for (int j = 0; j < 2; ++j) {
+            const int s = t + j * kIbLanes;
+            if (s >= A.S) break;
+            const bool ok = oksl[j];
+            const double wgt = wsl[j];
             out[3 * s + 0] = ok ? gs0 * A.spread * wgt : 0.0;
             out[3 * s + 1] = ok ? gs1 * A.spread * wgt : 0.0;
             out[3 * s + 2] = ok ? gs2 * A.spread * wgt : 0.0;
@@ -473,12 +506,11 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
     __syncthreads();
     // 3. contribution of every slot to the spread force
     const double g0 = share[0], g1 = share[1], g2 = share[2];
-    for (int s = t; s < A.S; s += kIbLanes) {
-        int d[3], cc[3];
-        double pp[3];
-        slot_offset(A, s, d);
-        const bool ok = node_at(A, base, d, cc, pp);
-        const double wgt = slot_weight(A, W, s);
+    for (int j = 0; j < 2; ++j) {
+        const int s = t + j * kIbLanes;
+        if (s >= A.S) break;
+        const bool ok = oksl[j];
+        const double wgt = wsl[j];
         out[3 * s + 0] = ok ? g0 * A.spread * wgt : 0.0;
         out[3 * s + 1] = ok ? g1 * A.spread * wgt : 0.0;
         out[3 * s + 2] = ok ? g2 * A.spread * wgt : 0.0;
@@ -820,6 +852,10 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
     if (!(r = mem_.alloc_n(&Gw_, 3 * M_))) return r;
     if (!(r = mem_.alloc_n(&node_, 3 * M_))) return r;
     if (!(r = mem_.alloc_n(&contrib_, (size_t)3 * M_ * S_))) return r;
+    if (cfg.method == IPL_IB_DIRECT) {  // slot weights and box indices of the first forcing pass
+        if (!(r = mem_.alloc_n(&wcache_, (size_t)M_ * S_))) return r;
+        if (!(r = mem_.alloc_n(&bcache_, (size_t)M_ * S_))) return r;
+    }
     if (!(r = mem_.alloc_n(&Fbox_, (size_t)3 * vmax_))) return r;
     if (!(r = mem_.alloc_n(&Ubox_, (size_t)3 * vmax_))) return r;
     if (!(r = mem_.alloc_n(&range_, (size_t)vmax_))) return r;
@@ -877,8 +913,8 @@ ibk::IbArgs make_args(const IbGrid& g, const ipl_ib_config& cfg, int M, int S, i
 template <typename ST, typename MT>
 void force_q(int Q, const ibk::IbArgs& A, int it, const ST* f, const int* node, const double* posw, const double* nB,
              const double* velw, double* Gw, double* contrib, const int* box, const double* Fbox, const double* Ubox,
-             hipStream_t st) {
-#define IPL_IBF(QQ) IPL_LAUNCH_SYNC((ibk::k_ib_force<ST, MT, QQ>), A.M, ibk::kIbLanes, 0, st, A, it, f, node, posw, nB, velw, Gw, contrib, box, Fbox, Ubox)
+             double* wcache, int* bcache, hipStream_t st) {
+#define IPL_IBF(QQ) IPL_LAUNCH_SYNC((ibk::k_ib_force<ST, MT, QQ>), A.M, ibk::kIbLanes, 0, st, A, it, f, node, posw, nB, velw, Gw, contrib, box, Fbox, Ubox, wcache, bcache)
     if (Q == 9) IPL_IBF(9); else if (Q == 19) IPL_IBF(19); else IPL_IBF(27);
 #undef IPL_IBF
 }
@@ -922,9 +958,11 @@ Status IbSolver::substep(void* f, const void* g, const void* F, int storage) {
     }
     for (int it = 0; it < iters; ++it) {
         if (storage == IPL_F64)
-            force_q<double, double>(g_.Q, A, it, (const double*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box_, Fbox_, Ubox_, st_);
+            force_q<double, double>(g_.Q, A, it, (const double*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box_, Fbox_, Ubox_,
+                                    wcache_, bcache_, st_);
         else
-            force_q<float, float>(g_.Q, A, it, (const float*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box_, Fbox_, Ubox_, st_);
+            force_q<float, float>(g_.Q, A, it, (const float*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box_, Fbox_, Ubox_,
+                                  wcache_, bcache_, st_);
         IPL_LAUNCH_SYNC(ibk::k_ib_gather, (unsigned)vmax_, ibk::kIbGatherLanes, 0, st_, A, it, sorted_, range_, contrib_, box_, Fbox_);
     }
     if (storage == IPL_F64) apply_q<double>(g_.Q, A, (double*)f, box_, Fbox_, range_, st_);

@@ -74,6 +74,8 @@ private:
     double *pos_[3] = {nullptr, nullptr, nullptr};
     double *oposw_ = nullptr, *onB_ = nullptr, *posw_ = nullptr, *nB_ = nullptr, *velw_ = nullptr, *Gw_ = nullptr;
     double *contrib_ = nullptr, *Fbox_ = nullptr, *Ubox_ = nullptr, *body_ = nullptr;
+    double* wcache_ = nullptr;  // direct forcing: slot weights of the first pass [M][S]
+    int* bcache_ = nullptr;     // and their box indices (-2: ghost node)
     int *node_ = nullptr, *box_ = nullptr;
     int2* range_ = nullptr;
     unsigned long long* sorted_ = nullptr;
