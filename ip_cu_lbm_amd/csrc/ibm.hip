@@ -329,7 +329,8 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
                                                       const double* __restrict__ velw, double* __restrict__ Gw,
                                                       double* __restrict__ contrib, const int* __restrict__ box,
                                                       const double* __restrict__ Fbox, const double* __restrict__ Ubox,
-                                                      double* __restrict__ wcache, int* __restrict__ bcache) {
+                                                      double* __restrict__ wcache, int* __restrict__ bcache,
+                                                      int* __restrict__ mark) {
     constexpr int kRows = Q > 20 ? Q : 20;  // the slot sums, then the 20 moment sums
     __shared__ double red[kIbLanes * kRows];
     __shared__ double share[3];
@@ -369,8 +370,10 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
             oksl[j] = node_at(A, base, d, cc, pp);
             wsl[j] = slot_weight(A, W, s);
             if (oksl[j]) {
-                if (direct) bsl[j] = box_index(A, box, cc);
+                const int64_t bx = box_index(A, box, cc);
+                if (direct) bsl[j] = bx;
                 else isl[j] = cell_of(A, cc);
+                if (bx >= 0 && bx < A.vmax) mark[bx] = 1;  // the gather visits marked cells only
             }
             if (direct) {
                 wcache[(size_t)m * A.S + s] = wsl[j];
@@ -525,13 +528,18 @@ constexpr int kIbGatherLanes = 64;
 
 __global__ void __launch_bounds__(kIbGatherLanes) k_ib_gather(IbArgs A, int it, const unsigned long long* __restrict__ sorted,
                                                                const int2* __restrict__ range, const double* __restrict__ contrib,
-                                                               const int* __restrict__ box, double* __restrict__ Fbox) {
+                                                               const int* __restrict__ box, double* __restrict__ Fbox,
+                                                               const int* __restrict__ mark) {
     __shared__ double red[3][kIbGatherLanes];
     __shared__ int hit;
     const int64_t b = blockIdx.x;
     const int t = threadIdx.x;
     const int64_t vol = (int64_t)box[3] * box[4] * box[5];
     if (b >= vol || b >= A.vmax) return;  // whole workgroup: b is uniform
+    if (!mark[b]) {  // no marker block covers this cell (marked by the first force pass)
+        if (t == 0 && it == 0) { Fbox[3 * b] = 0.0; Fbox[3 * b + 1] = 0.0; Fbox[3 * b + 2] = 0.0; }
+        return;
+    }
     int c[3];
     c[2] = (int)(b % box[5]);
     c[1] = (int)((b / box[5]) % box[4]);
@@ -587,9 +595,9 @@ __global__ void __launch_bounds__(kIbGatherLanes) k_ib_gather(IbArgs A, int it, 
 // It also clears the marker runs of every box cell for the next step's rank kernel.
 template <typename ST, int Q>
 __global__ void k_ib_apply(IbArgs A, ST* __restrict__ f, const int* __restrict__ box, const double* __restrict__ Fbox,
-                           int2* __restrict__ range) {
+                           int2* __restrict__ range, int* __restrict__ mark) {
     const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (b < A.vmax) range[b] = make_int2(0, 0);
+    if (b < A.vmax) { range[b] = make_int2(0, 0); mark[b] = 0; }
     const int64_t vol = (int64_t)box[3] * box[4] * box[5];
     if (b >= vol || b >= A.vmax) return;
     const double F[3] = {Fbox[3 * b], Fbox[3 * b + 1], Fbox[3 * b + 2]};
@@ -859,6 +867,7 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
     if (!(r = mem_.alloc_n(&Fbox_, (size_t)3 * vmax_))) return r;
     if (!(r = mem_.alloc_n(&Ubox_, (size_t)3 * vmax_))) return r;
     if (!(r = mem_.alloc_n(&range_, (size_t)vmax_))) return r;
+    if (!(r = mem_.alloc_n(&mark_, (size_t)vmax_))) return r;
     if (!(r = mem_.alloc_n(&body_, (size_t)ibk::B_SIZE))) return r;
     if (!(r = mem_.alloc_n(&box_, 8))) return r;
     if (!(r = mem_.alloc_n(&sorted_, (size_t)M_))) return r;
@@ -872,6 +881,7 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
     IPL_HIP_TRY(hipMemcpyAsync(body_, body.data(), ibk::B_SIZE * sizeof(double), hipMemcpyHostToDevice, st));
     IPL_HIP_TRY(hipMemsetAsync(box_, 0, 8 * sizeof(int), st));
     IPL_HIP_TRY(hipMemsetAsync(range_, 0, (size_t)vmax_ * sizeof(int2), st));  // later cleared by k_ib_apply
+    IPL_HIP_TRY(hipMemsetAsync(mark_, 0, (size_t)vmax_ * sizeof(int), st));
     IPL_HIP_TRY(hipStreamSynchronize(st));
     return Status::ok();
 }
@@ -913,8 +923,8 @@ ibk::IbArgs make_args(const IbGrid& g, const ipl_ib_config& cfg, int M, int S, i
 template <typename ST, typename MT>
 void force_q(int Q, const ibk::IbArgs& A, int it, const ST* f, const int* node, const double* posw, const double* nB,
              const double* velw, double* Gw, double* contrib, const int* box, const double* Fbox, const double* Ubox,
-             double* wcache, int* bcache, hipStream_t st) {
-#define IPL_IBF(QQ) IPL_LAUNCH_SYNC((ibk::k_ib_force<ST, MT, QQ>), A.M, ibk::kIbLanes, 0, st, A, it, f, node, posw, nB, velw, Gw, contrib, box, Fbox, Ubox, wcache, bcache)
+             double* wcache, int* bcache, int* mark, hipStream_t st) {
+#define IPL_IBF(QQ) IPL_LAUNCH_SYNC((ibk::k_ib_force<ST, MT, QQ>), A.M, ibk::kIbLanes, 0, st, A, it, f, node, posw, nB, velw, Gw, contrib, box, Fbox, Ubox, wcache, bcache, mark)
     if (Q == 9) IPL_IBF(9); else if (Q == 19) IPL_IBF(19); else IPL_IBF(27);
 #undef IPL_IBF
 }
@@ -928,11 +938,11 @@ void ubase_q(int Q, const ibk::IbArgs& A, const ST* f, const ST* g, const MT* F,
 }
 
 template <typename ST>
-void apply_q(int Q, const ibk::IbArgs& A, ST* f, const int* box, const double* Fbox, int2* range, hipStream_t st) {
+void apply_q(int Q, const ibk::IbArgs& A, ST* f, const int* box, const double* Fbox, int2* range, int* mark, hipStream_t st) {
     const unsigned blocks = (unsigned)((A.vmax + 255) / 256);
-    if (Q == 9) IPL_LAUNCH((ibk::k_ib_apply<ST, 9>), blocks, 256, 0, st, A, f, box, Fbox, range);
-    else if (Q == 19) IPL_LAUNCH((ibk::k_ib_apply<ST, 19>), blocks, 256, 0, st, A, f, box, Fbox, range);
-    else IPL_LAUNCH((ibk::k_ib_apply<ST, 27>), blocks, 256, 0, st, A, f, box, Fbox, range);
+    if (Q == 9) IPL_LAUNCH((ibk::k_ib_apply<ST, 9>), blocks, 256, 0, st, A, f, box, Fbox, range, mark);
+    else if (Q == 19) IPL_LAUNCH((ibk::k_ib_apply<ST, 19>), blocks, 256, 0, st, A, f, box, Fbox, range, mark);
+    else IPL_LAUNCH((ibk::k_ib_apply<ST, 27>), blocks, 256, 0, st, A, f, box, Fbox, range, mark);
 }
 
 }  // namespace
@@ -959,14 +969,15 @@ Status IbSolver::substep(void* f, const void* g, const void* F, int storage) {
     for (int it = 0; it < iters; ++it) {
         if (storage == IPL_F64)
             force_q<double, double>(g_.Q, A, it, (const double*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box_, Fbox_, Ubox_,
-                                    wcache_, bcache_, st_);
+                                    wcache_, bcache_, mark_, st_);
         else
             force_q<float, float>(g_.Q, A, it, (const float*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box_, Fbox_, Ubox_,
-                                  wcache_, bcache_, st_);
-        IPL_LAUNCH_SYNC(ibk::k_ib_gather, (unsigned)vmax_, ibk::kIbGatherLanes, 0, st_, A, it, sorted_, range_, contrib_, box_, Fbox_);
+                                  wcache_, bcache_, mark_, st_);
+        IPL_LAUNCH_SYNC(ibk::k_ib_gather, (unsigned)vmax_, ibk::kIbGatherLanes, 0, st_, A, it, sorted_, range_, contrib_, box_, Fbox_,
+                        mark_);
     }
-    if (storage == IPL_F64) apply_q<double>(g_.Q, A, (double*)f, box_, Fbox_, range_, st_);
-    else apply_q<float>(g_.Q, A, (float*)f, box_, Fbox_, range_, st_);
+    if (storage == IPL_F64) apply_q<double>(g_.Q, A, (double*)f, box_, Fbox_, range_, mark_, st_);
+    else apply_q<float>(g_.Q, A, (float*)f, box_, Fbox_, range_, mark_, st_);
     IPL_LAUNCH_SYNC(ibk::k_ib_body, 1, ibk::kIbBodyThreads, 0, st_, A, Gw_, posw_, body_);
     if (cfg_.moving) {
         const unsigned blocks = (unsigned)((M_ + 63) / 64);

@@ -78,6 +78,7 @@ private:
     int* bcache_ = nullptr;     // and their box indices (-2: ghost node)
     int *node_ = nullptr, *box_ = nullptr;
     int2* range_ = nullptr;
+    int* mark_ = nullptr;       // box cells some marker block covers (set by the first force pass)
     unsigned long long* sorted_ = nullptr;
 };
 
