@@ -108,6 +108,7 @@ public:
         G.half_force = 1;
         G.wall_fn = cfg_.wall_function != 0;
         G.les = cfg_.les != 0;
+        G.zero_force = cfg_.gravity == 0.0 && !G.grad_rho && !G.grad_phi && !G.wall_fn;
         double scale = 1.0;
         if (std::is_same<ST, __half>::value) scale = cfg_.f16_scale > 0 ? cfg_.f16_scale : 8192.0;
         G.scale = (float)scale;

@@ -24,6 +24,8 @@ struct FGeom {
     int grad_phi;    // phase field: grad phi and its Laplacian (Force, col_PF)
     int wall_fn;     // wall-function shear on the cells next to walls (collision.cu:28-55)
     int les;         // per-cell tau, tau_s rows set by the LES pass (basic.cu:285-339)
+    int zero_force;  // no gravity, gradient or wall term: F stays zero, its rows are neither
+                     // read nor written by the step (they hold the zeros of the set-up)
     int half_force;  // u = sum e c f + F_old dt / (2 rho) (update_rho); 0 after an immersed-boundary
                      // sub-step, whose update_velIBM leaves u = sum e f (ibm.cu:356-360)
     int scalar;

@@ -660,7 +660,7 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
         // u = sum e c f + F_old dt / (2 rho); after an immersed-boundary sub-step u = sum e f
         // (uc = 1, no force term).  Only the loads are conditional, so the code stays one path.
         MT F0 = MT(0), F1 = MT(0), F2 = MT(0);
-        if (G.half_force) {
+        if (G.half_force && !G.zero_force) {
             F0 = *elem(b.F_old, n);
             F1 = *elem(b.F_old, NC + n);
             F2 = *elem(b.F_old, 2 * NC + n);
@@ -698,7 +698,7 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
             if constexpr (d3) Fy += pp * grad_axis<MT, S, 1>(G, nb, tb, b.rho_n, c1, c2, ii);
             Fz += pp * grad_axis<MT, S, 2>(G, nb, tb, b.rho_n, c1, c2, ii);
         }
-        *elem(b.F_new, n) = Fx; *elem(b.F_new, NC + n) = Fy; *elem(b.F_new, 2 * NC + n) = Fz;
+        if (!G.zero_force) { *elem(b.F_new, n) = Fx; *elem(b.F_new, NC + n) = Fy; *elem(b.F_new, 2 * NC + n) = Fz; }
         // ---- equ_f, forcing term and collision ----
         const MT u2c = (u * u + v * v + w * w) * ic2;
         const MT fkc = P.fk_coef * irho;
