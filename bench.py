@@ -80,6 +80,44 @@ def measure_sphere(steps, warmup):
             "body_vel_z": float(body["vel"][2]), "body_force_z": float(body["force"][2])}
 
 
+def measure_sphere_precision(steps=20000, every=1000):
+    """The falling-sphere workload run to `steps` in FP32 and in FP64 storage (the FP64 GPU path
+    matches the FP64 oracle to 1e-12 on the reduced cases; the oracle itself cannot run 2.8e11
+    cell updates here).  Every `every` steps: the body velocity and the hydrodynamic drag along z
+    (the body force minus the constant (1 - rho_f/rho_B) m g).  Reported at the terminal state
+    (the step of largest |v|, before the sphere reaches the bottom wall): settling velocity, drag,
+    particle Reynolds number |v| D / nu with nu = (tau - 1/2) c^2 dt / 3, and the FP32 errors
+    against FP64; plus the FP32 velocity-field error at the end."""
+    import numpy as np
+    from ip_cu_lbm_amd import F32, F64, Simulation, cases
+    out = {}
+    for name, st in (("fp64", F64), ("fp32", F32)):
+        case = cases.falling_sphere(storage=st)
+        bdy = case.body
+        w_net = (1.0 - bdy.rho_f / bdy.density) * bdy.mass * bdy.gravity[2]
+        hist = []
+        with Simulation(case) as sim:
+            for k in range(0, steps, every):
+                sim.step(every)
+                b = sim.body_state()
+                hist.append((k + every, float(b["vel"][2]), float(b["force"][2]) - w_net))
+                log(f"[sphere {name}] step {k + every}: v_z {hist[-1][1]:.6e}, drag_z {hist[-1][2]:.6e}")
+            u = np.stack([sim.read(c) for c in "uvw"])
+        out[name] = {"hist": hist, "u": u, "case": case}
+    c = out["fp64"]["case"]
+    nu = (c.tau - 0.5) * (c.dz / c.dt) ** 2 * c.dt / 3.0
+    d = float(2 * np.abs(c.body.markers).max())
+    h64, h32 = out["fp64"]["hist"], out["fp32"]["hist"]
+    t = int(np.argmax([abs(v) for _, v, _ in h64]))
+    v64, v32, f64, f32 = h64[t][1], h32[t][1], h64[t][2], h32[t][2]
+    du = float(np.linalg.norm(out["fp32"]["u"] - out["fp64"]["u"]) / np.linalg.norm(out["fp64"]["u"]))
+    return {"workload": "falling_sphere_192x192x384_d3q19_ibm_mdf3_peskin4", "steps": steps, "terminal_step": h64[t][0],
+            "settling_velocity_fp64": v64, "settling_velocity_fp32": v32, "drag_fp64": f64, "drag_fp32": f32,
+            "re_p": abs(v64) * d / nu, "settling_rel_fp32_vs_fp64": abs(v32 - v64) / abs(v64),
+            "drag_rel_fp32_vs_fp64": abs(f32 - f64) / abs(f64), "velocity_rel_l2_fp32_vs_fp64_at_end": du,
+            "history_fp64": h64, "history_fp32": h32}
+
+
 def measure_serving(n_jobs=256):
     """The serving mix: 256 seeded Poisson arrivals (mean gap 20 ms) of D2Q9/D3Q19 FP32 jobs of
     mixed sizes (2D sides 64-512, 3D sides 32-128, 500-5000 steps, ~30 % with salinity, ~10 % with
@@ -158,6 +196,8 @@ def main(argv=None):
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 plain / fused comparison runs")
     ap.add_argument("--only-headline", action="store_true", help="run just the headline workload (for profilers)")
     ap.add_argument("--only-sphere", action="store_true", help="run just the falling-sphere workload (for profilers)")
+    ap.add_argument("--sphere-precision", type=int, metavar="STEPS", default=0,
+                    help="run the falling sphere STEPS steps in FP32 and FP64 and print their settling velocity and drag")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the traffic file from two rocprofv3 --pmc output directories and exit")
     a = ap.parse_args(argv)
@@ -173,6 +213,10 @@ def main(argv=None):
 
     if a.pmc_summary:
         pmc_summary(a.pmc_summary[0], a.pmc_summary[1], 512 * 32 * 64)
+        return 0
+
+    if a.sphere_precision:
+        print(json.dumps(measure_sphere_precision(a.sphere_precision)), flush=True)
         return 0
 
     if a.only_sphere:

@@ -18,18 +18,20 @@ namespace ipl {
 
 // Stream axis of a uniform grid (any axis streams exactly): the one whose lines fill their
 // wavefronts best (cells / cells rounded up to 64), z on ties, then x; lines of at most 512
-// cells.  Measured: 48 x 40 x 72 D3Q19 step 7.9 -> 6.7 us along x; full lines neutral.
+// cells whose LDS rows fit the 64 KiB budget.  Measured: 48 x 40 x 72 D3Q19 step 7.9 -> 6.7 us
+// along x; full lines neutral.
 int uniform_stream_axis(const Graph& g, const ipl_config& cfg) {
     const int order[3] = {2, 0, 1};
+    const size_t mt = cfg.storage == IPL_F64 ? 8 : 4;
     int best = 2;
     double best_fill = -1.0;
     for (int a : order) {
         const int n = g.n[a];
         if (n > 512 || (g.dim == 2 && a == 1)) continue;
+        if ((size_t)(g.Q + g.Qs + 2) * n * mt > 64 * 1024) continue;
         const double fill = (double)n / (double)(((n + 63) / 64) * 64);
         if (fill > best_fill + 1e-12) { best_fill = fill; best = a; }
     }
-    (void)cfg;
     return best;
 }
 

@@ -254,3 +254,13 @@ def test_fused_uniform_stream_axis_choice_fp64(oracle_lib, n, periodic):
     hydrostatic: with z periodic it stays near zero and its relative error is ill-conditioned."""
     case = cases.lock_exchange(nx=n[0], ny=n[1], nz=n[2], path=PATH_FUSED).replace(periodic=periodic, flux_axes=_abi.FLUX_NONE)
     _compare(case, 60, ("p", "vel", "S"), 1e-12, oracle_lib)
+
+
+def test_fused_uniform_stream_axis_lds_budget_fp64(oracle_lib):
+    """FP64 lines of 300 cells with a D3Q7 scalar need (19 + 7 + 2) * 300 * 8 = 67,200 bytes of
+    LDS, over the 64 KiB budget: the best-filled axis (z) is passed over and the grid streams
+    along x, still on the fused path."""
+    case = cases.lock_exchange(nx=20, ny=12, nz=300, path=PATH_FUSED).replace(periodic=(0, 1, 0), flux_axes=_abi.FLUX_NONE)
+    with Simulation(case) as sim:
+        assert sim.path == PATH_FUSED
+    _compare(case, 40, ("p", "vel", "S"), 1e-12, oracle_lib)
