@@ -289,11 +289,10 @@ int ipl_estimate_device_bytes(const ipl_config* cfg, int64_t* bytes) {
     const int64_t m = cfg->storage == IPL_F64 ? 8 : 4;
     const int64_t per_axis = 4 * (int64_t)(cfg->nx + cfg->ny + cfg->nz) * 8;
     if (cfg->path != IPL_PATH_PLAIN) {
-        // fused: ping-pong populations, force rows, scalar-field rows, phase-field history,
+        // fused: ping-pong populations, force rows, the phase field's scalar-field and history rows,
         // FP64 cell volumes and the FP64 readout staging rows
         int64_t b = 2 * (int64_t)(Q + Qs) * s * n + 6 * m * n + 8 * n + 7 * 8 * n;
-        if (cfg->scalar != IPL_SCALAR_NONE) b += 2 * m * n;
-        if (cfg->scalar == IPL_SCALAR_PHASE_FIELD) b += 4 * m * n;
+        if (cfg->scalar == IPL_SCALAR_PHASE_FIELD) b += 6 * m * n;
         if (cfg->les) b += 6 * m * n;  // velocity, viscosity, tau and tau_s rows
         *bytes = b + per_axis + (1 << 20);
     } else {

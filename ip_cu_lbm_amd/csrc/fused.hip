@@ -123,7 +123,10 @@ public:
         G.grad_rho = (cfg_.scalar == IPL_SCALAR_SALINITY && cfg_.beta_s != 0.0) ||
                      (cfg_.scalar == IPL_SCALAR_PHASE_FIELD && cfg_.rho_H != cfg_.rho_L);
         G.grad_phi = cfg_.scalar == IPL_SCALAR_PHASE_FIELD;
-        G.need_grad = G.grad_rho || G.grad_phi;
+        // salinity gradients come from the step itself; the phase field keeps the pass (its
+        // Laplacian and col_PF read phi of the neighbours as well)
+        G.inline_rho = G.grad_rho && !G.grad_phi && cfg_.scalar == IPL_SCALAR_SALINITY;
+        G.need_grad = (G.grad_rho || G.grad_phi) && !G.inline_rho;
         G.half_force = 1;
         G.wall_fn = cfg_.wall_function != 0;
         G.les = cfg_.les != 0;

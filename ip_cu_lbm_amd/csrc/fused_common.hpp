@@ -19,8 +19,10 @@ struct FGeom {
     int64_t lines, ncell;
     int wall_bc;
     int flux;        // 1 = interpolated flux along s
-    int need_grad;   // the scalar-field pass runs (rho and/or phi gradients are needed)
+    int need_grad;   // the scalar-field pass runs (phi, and rho beside it, for the gradients)
     int grad_rho;    // rho varies: the pressure-gradient term of Force is nonzero
+    int inline_rho;  // salinity: the step forms the neighbours' rho from their g rows itself
+                     // (along the line through LDS), and no scalar-field pass runs
     int grad_phi;    // phase field: grad phi and its Laplacian (Force, col_PF)
     int wall_fn;     // wall-function shear on the cells next to walls (collision.cu:28-55)
     int les;         // per-cell tau, tau_s rows set by the LES pass (basic.cu:285-339)

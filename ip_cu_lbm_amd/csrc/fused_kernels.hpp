@@ -609,6 +609,39 @@ __device__ inline MT grad_axis(const FGeom& G, const LineNb& nb, const FTables<M
     return (*elem(q, nb_index<S, A, 1>(G, nb, i)) - *elem(q, nb_index<S, A, -1>(G, nb, i))) * *elem(tb.invd[A], (uint32_t)x);
 }
 
+// rho of step n of a salinity cell from the sum of its g (update_rho, collision.cu:244-254)
+template <typename MT>
+__device__ inline MT rho_from_sum(const FCoef<MT>& P, MT sc) { return P.rho_L + sc * P.beta_s; }
+
+// the same at cell m from its g rows, summed in the order every other pass sums them
+template <typename ST, typename MT, int QS>
+__device__ inline MT rho_from_g(const FGeom& G, const FCoef<MT>& P, const ST* __restrict__ gsrc, uint32_t m) {
+    const uint32_t NC = (uint32_t)G.ncell;
+    MT sc = Store<ST, MT>::load(elem(gsrc, m), Lattice<QS>::w(0), G.inv_scale);
+#pragma unroll
+    for (int k = 1; k < QS; ++k) sc += Store<ST, MT>::load(elem(gsrc, k * NC + m), Lattice<QS>::w(k), G.inv_scale);
+    return rho_from_sum(P, sc);
+}
+
+// grad rho along axis A without the scalar-field pass: along the line from the workgroup's LDS
+// row of rho, across lines from the neighbour cells' g rows (their lines are processed by the
+// neighbouring workgroups of the same XCD at about the same time, so the rows come from L2)
+template <typename ST, typename MT, int QS, int S, int A>
+__device__ inline MT grad_rho_inline(const FGeom& G, const LineNb& nb, const FTables<MT>& tb, const FCoef<MT>& P,
+                                     const ST* __restrict__ gsrc, const MT* lds_r, int c1, int c2, int i) {
+    constexpr int o1 = LineAxes<S>::o1;
+    const int x = A == S ? i : (A == o1 ? c1 : c2);
+    MT hi, lo;
+    if constexpr (A == S) {
+        hi = lds_r[wrap_or_clamp(i + 1, G.ns, G.periodic[S])];
+        lo = lds_r[wrap_or_clamp(i - 1, G.ns, G.periodic[S])];
+    } else {
+        hi = rho_from_g<ST, MT, QS>(G, P, gsrc, nb_index<S, A, 1>(G, nb, i));
+        lo = rho_from_g<ST, MT, QS>(G, P, gsrc, nb_index<S, A, -1>(G, nb, i));
+    }
+    return (hi - lo) * *elem(tb.invd[A], (uint32_t)x);
+}
+
 // EXT: the wall function and LES are compiled in (they add registers to every step otherwise)
 template <typename ST, typename MT, int Q, int QS, int NPTS, int S, bool FLUX, bool EXT>
 __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGeom G, FTables<MT> tb, FBufs<ST, MT> b, FCoef<MT> P) {
@@ -629,14 +662,27 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
     const LineNb nb = line_neighbours<S>(G, c1, c2);
 
     MT Pf[Q], Pg[QS > 0 ? QS : 1];
-    if (active) {
-        MT g[QS > 0 ? QS : 1];
-#pragma unroll
-        for (int k = 0; k < Q; ++k) Pf[k] = Store<ST, MT>::load(elem(fsrc, k * NC + n), Lattice<Q>::w(k), G.inv_scale);
-        if constexpr (QS > 0) {
+    MT g[QS > 0 ? QS : 1];
+    MT* lds_r = lds_g;  // [ns] rho of the line (inline_rho), after the g rows
+    if constexpr (QS > 0) lds_r += (sdir_count<QS, S>() + 1) * G.ns;
+    if constexpr (QS > 0) {
+        if (active) {
 #pragma unroll
             for (int k = 0; k < QS; ++k) g[k] = Store<ST, MT>::load(elem(gsrc, k * NC + n), Lattice<QS>::w(k), G.inv_scale);
         }
+        if (G.inline_rho) {  // uniform over the launch: every work-item reaches the barrier
+            if (active) {
+                MT sc = g[0];
+#pragma unroll
+                for (int k = 1; k < QS; ++k) sc += g[k];
+                lds_r[i] = rho_from_sum(P, sc);
+            }
+            __syncthreads();
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < Q; ++k) Pf[k] = Store<ST, MT>::load(elem(fsrc, k * NC + n), Lattice<Q>::w(k), G.inv_scale);
         // ---- macroscopic fields of step n (update_rho, collision.cu:244-254) ----
         MT p = Pf[0], ex = 0, ey = 0, ez = 0;
         static_for<Q - 1>([&](auto kc) {
@@ -694,9 +740,18 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
         else Fz += -rho * P.gravity;
         if (G.grad_rho) {
             const MT pp = -p * P.press;
-            Fx += pp * grad_axis<MT, S, 0>(G, nb, tb, b.rho_n, c1, c2, ii);
-            if constexpr (d3) Fy += pp * grad_axis<MT, S, 1>(G, nb, tb, b.rho_n, c1, c2, ii);
-            Fz += pp * grad_axis<MT, S, 2>(G, nb, tb, b.rho_n, c1, c2, ii);
+            if constexpr (QS > 0) {
+                if (G.inline_rho) {
+                    Fx += pp * grad_rho_inline<ST, MT, QS, S, 0>(G, nb, tb, P, gsrc, lds_r, c1, c2, ii);
+                    if constexpr (d3) Fy += pp * grad_rho_inline<ST, MT, QS, S, 1>(G, nb, tb, P, gsrc, lds_r, c1, c2, ii);
+                    Fz += pp * grad_rho_inline<ST, MT, QS, S, 2>(G, nb, tb, P, gsrc, lds_r, c1, c2, ii);
+                }
+            }
+            if (!G.inline_rho) {
+                Fx += pp * grad_axis<MT, S, 0>(G, nb, tb, b.rho_n, c1, c2, ii);
+                if constexpr (d3) Fy += pp * grad_axis<MT, S, 1>(G, nb, tb, b.rho_n, c1, c2, ii);
+                Fz += pp * grad_axis<MT, S, 2>(G, nb, tb, b.rho_n, c1, c2, ii);
+            }
         }
         if (!G.zero_force) { *elem(b.F_new, n) = Fx; *elem(b.F_new, NC + n) = Fy; *elem(b.F_new, 2 * NC + n) = Fz; }
         // ---- equ_f, forcing term and collision ----
@@ -913,7 +968,7 @@ size_t fused_lds_bytes(int s, int ns) {
     int nf = 1, ng = 0;
     for (int k = 0; k < Q; ++k) nf += comp<Q>(k, s) != 0;
     if constexpr (QS > 0) {
-        ng = 1;
+        ng = 2;  // + the line's rho (inline_rho)
         for (int k = 0; k < QS; ++k) ng += comp<QS>(k, s) != 0;
     }
     return (size_t)(nf + ng) * ns * sizeof(MT);
