@@ -118,6 +118,38 @@ def measure_sphere_precision(steps=20000, every=1000):
             "history_fp64": h64, "history_fp32": h32}
 
 
+def measure_small_configs(steps):
+    """The two 2D workloads beside the headline (both fit the 256 MB last-level cache, so their
+    bandwidth figure is not an HBM one):
+    - phase-field solitary wave, D2Q9 MRT + D2Q9 Allen-Cahn phi, 1024 x 256, z stretched by 1.01
+      around the interface, fused path, FP32 and FP64 populations, plain FP64 beside them;
+      algorithmic bytes 2 x (9 + 9) x s per cell;
+    - Taylor-Green vortex in a stretched channel, D2Q9 MRT, 256 x 128, x stretched by 1.01,
+      fused and plain FP64 (2 x 9 x 8 B per cell)."""
+    from ip_cu_lbm_amd import F32, F64, PATH_FUSED, PATH_PLAIN, cases
+    out = {}
+    k = max(200, steps // 2)
+    sw32 = measure(cases.solitary_wave(storage=F32, path=PATH_FUSED), k, 50, profile_steps=100)
+    sw64 = measure(cases.solitary_wave(storage=F64, path=PATH_FUSED), k, 50, profile_steps=100)
+    swp = measure(cases.solitary_wave(storage=F64, path=PATH_PLAIN), max(50, k // 10), 10)
+    cells = sw32["n_cells"]
+    out["solitary_wave"] = {
+        "workload": "solitary_wave_1024x256_d2q9_phase_field", "cells": cells,
+        "fused_fp32_mlups": round(sw32["mlups"], 1), "fused_fp64_mlups": round(sw64["mlups"], 1),
+        "plain_fp64_mlups": round(swp["mlups"], 1),
+        "fused_fp32_kernel_us": round(sw32["profile"][0] * 1e3, 2),
+        "fused_fp32_kernel_gbps": round(2 * 18 * 4 * cells / (sw32["profile"][0] * 1e-3) / 1e9, 1),
+        "fused_fp32_aux_us_per_step": round(sw32["profile"][1] * 1e3, 2),
+        "speedup_fused_fp32_vs_plain_fp64": round(sw32["mlups"] / swp["mlups"], 2)}
+    tg64 = measure(cases.lattice_taylor_green(storage=F64, path=PATH_FUSED), k, 50, profile_steps=100)
+    tgp = measure(cases.lattice_taylor_green(storage=F64, path=PATH_PLAIN), max(50, k // 10), 10)
+    out["taylor_green"] = {
+        "workload": "taylor_green_256x128_d2q9_stretched_x", "cells": tg64["n_cells"],
+        "fused_fp64_mlups": round(tg64["mlups"], 1), "plain_fp64_mlups": round(tgp["mlups"], 1),
+        "fused_fp64_kernel_us": round(tg64["profile"][0] * 1e3, 2)}
+    return out
+
+
 def measure_serving(n_jobs=256):
     """The serving mix: 256 seeded Poisson arrivals (mean gap 20 ms) of D2Q9/D3Q19 FP32 jobs of
     mixed sizes (2D sides 64-512, 3D sides 32-128, 500-5000 steps, ~30 % with salinity, ~10 % with
@@ -245,6 +277,9 @@ def main(argv=None):
         sph = measure_sphere(max(100, a.steps // 10), 20)
         log(f"[bench] falling sphere: {sph['mlups']:.0f} MLUPS, fused kernel {sph['kernel_us']:.1f} us, "
             f"IB sub-step {sph['ib_us']:.1f} us per step")
+        sw = measure_small_configs(a.steps)
+        log(f"[bench] solitary wave: fused FP32 {sw['solitary_wave']['fused_fp32_mlups']:.0f} MLUPS, "
+            f"Taylor-Green: fused FP64 {sw['taylor_green']['fused_fp64_mlups']:.0f} MLUPS")
         srv = measure_serving()
         srv["aggregate_vs_single_large_case"] = round(srv["aggregate_mlups"] / head["mlups"], 3)
         log(f"[bench] serving mix: {srv['jobs']} jobs, p50 {srv['p50_s']:.2f} s, p99 {srv['p99_s']:.2f} s, "
@@ -258,6 +293,8 @@ def main(argv=None):
             "fused_fp64_kernel_gbps": round(2 * 26 * 8 * cells / (fused64["profile"][0] * 1e-3) / 1e9, 1),
             "fp16_storage": "disabled (see DESIGN.md)",
             "falling_sphere": sph,
+            "solitary_wave": sw["solitary_wave"],
+            "taylor_green": sw["taylor_green"],
             "serving": srv,
         }
 

@@ -70,10 +70,11 @@ int ipl_create(const ipl_config* cfg, const ipl_init* init, ipl_ctx** out) {
     }
     auto bail = [&](const ipl::Status& s) {
         g_create_error = s.msg;
+        ctx->engine.reset();  // its memory returns to the pool on the stream, destroyed after it
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
         if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-        ctx->engine.reset();
         delete ctx;
         return s.code;
     };
@@ -182,9 +183,10 @@ int ipl_info(ipl_ctx* ctx, int64_t* n_calc, int64_t* n_ghost, int64_t* device_by
 int ipl_destroy(ipl_ctx* ctx) {
     if (!ctx) return IPL_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    ctx->engine.reset();
+    ctx->engine.reset();  // frees on ctx->stream (stream-ordered), before the stream goes
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return IPL_OK;

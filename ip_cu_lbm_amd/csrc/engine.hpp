@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <memory>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/ipl.h"
@@ -31,29 +32,56 @@ struct Status {
                                       std::string(#expr) + ": " + hipGetErrorString(e_));      \
     } while (0)
 
-// Device allocation that frees itself.
+// Device allocation that frees itself.  Once bound to the owner's stream, memory comes from the
+// device's stream-ordered pool (hipMallocAsync) and returns to it with hipFreeAsync on that
+// stream: releasing a context waits for nothing but its own stream, where hipFree would wait for
+// every stream of the device (a server releases finished jobs while the others step).  The pool
+// keeps freed memory for the next allocation instead of unmapping it.
 class DeviceArena {
 public:
     ~DeviceArena() { release(); }
+    void bind(hipStream_t st) { stream_ = st; bound_ = true; }
     Status alloc(void** p, size_t bytes) {
         *p = nullptr;
         if (bytes == 0) bytes = 256;
-        IPL_HIP_TRY(hipMalloc(p, bytes));
+        if (bound_) {
+            keep_pool_memory();
+            IPL_HIP_TRY(hipMallocAsync(p, bytes, stream_));
+            IPL_HIP_TRY(hipStreamSynchronize(stream_));
+        } else {
+            IPL_HIP_TRY(hipMalloc(p, bytes));
+        }
         ptrs_.push_back(*p);
         total_ += bytes;
         return Status::ok();
     }
     template <typename T> Status alloc_n(T** p, size_t n) { return alloc((void**)p, n * sizeof(T)); }
     void release() {
-        for (void* p : ptrs_) (void)hipFree(p);
+        for (void* p : ptrs_) (void)(bound_ ? hipFreeAsync(p, stream_) : hipFree(p));
         ptrs_.clear();
         total_ = 0;
     }
     size_t total() const { return total_; }
 
 private:
+    // release threshold of the current device's default pool: keep what is freed
+    static void keep_pool_memory() {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return;
+        static std::once_flag once[64];
+        if (dev < 0 || dev >= 64) return;
+        std::call_once(once[dev], [dev] {
+            hipMemPool_t pool;
+            if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+                uint64_t keep = UINT64_MAX;
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            }
+        });
+    }
     std::vector<void*> ptrs_;
     size_t total_ = 0;
+    hipStream_t stream_ = nullptr;
+    bool bound_ = false;
 };
 
 // Host-side initial state expanded to all cells (calc + ghosts) in FP64.

@@ -105,6 +105,7 @@ public:
 
     Status init(const HostState& hs, hipStream_t st) {
         stream = st;
+        mem_.bind(st);
         FGeom& G = geo_;
         std::memset(&G, 0, sizeof(G));
         for (int a = 0; a < 3; ++a) { G.n[a] = gr_.n[a]; G.periodic[a] = cfg_.periodic[a]; }

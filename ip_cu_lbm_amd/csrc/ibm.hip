@@ -767,6 +767,7 @@ IbSolver::~IbSolver() = default;
 
 Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::vector<double>* pos, hipStream_t st) {
     if (M_ > 0) return Status::err(IPL_ERR_ARGUMENT, "a body is already attached");
+    mem_.bind(st);
     if (cfg.n_markers <= 0 || !cfg.marker_pos || !cfg.marker_normal)
         return Status::err(IPL_ERR_ARGUMENT, "markers missing");
     if (cfg.n_markers > kIbMaxMarkers) return Status::err(IPL_ERR_UNSUPPORTED, "more than 4096 markers");

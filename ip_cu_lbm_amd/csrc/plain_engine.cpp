@@ -97,6 +97,7 @@ public:
 
     Status init(const HostState& hs, hipStream_t st) {
         stream = st;
+        mem_.bind(st);
         const int64_t N = g_.N, Nt = g_.Ntot, G = g_.G;
         const int Q = g_.Q, Qs = g_.Qs;
         PlainView<T>& s = v_;

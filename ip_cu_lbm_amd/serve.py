@@ -104,7 +104,10 @@ class Server:
 
     Admitted jobs are set up (host-side lattice set-up and upload, ``ipl_create``) by `workers`
     threads while the batch steps on the device, so set-up of new arrivals overlaps the running
-    jobs' work; a job joins the batch at the first chunk boundary after its set-up ends."""
+    jobs' work; a job joins the batch at the first chunk boundary after its set-up ends.  A
+    finished job leaves the batch at once and is released (``ipl_destroy``: its graph, stream and
+    device memory) by a separate thread, also while the batch steps; its bytes return to the
+    budget when the release has ended."""
 
     def __init__(self, budget_bytes: float = 200e9, chunk: int = 400, library: str = None, workers: int = 4):
         self.budget = int(budget_bytes)
@@ -117,6 +120,12 @@ class Server:
         sim = Simulation(job.case, self.library)
         return sim, time.perf_counter() - t
 
+    @staticmethod
+    def _release(sim: Simulation) -> float:
+        t = time.perf_counter()
+        sim.close()
+        return time.perf_counter() - t
+
     def run(self, jobs: List[Job], log=None, on_done=None) -> dict:
         """Serves `jobs` to completion.  `on_done(job)` is called for each finished job while its
         simulation is still open (to read its fields), `log(str)` with one line per finished job."""
@@ -127,13 +136,19 @@ class Server:
                 raise ValueError(f"job {j.jid} needs {j.bytes} B, more than the whole budget")
         active: List[Job] = []
         creating = {}
+        releasing = {}
         used = 0
         done: List[Job] = []
         busy = create = close = 0.0
         t0 = time.perf_counter()
-        with Batch(self.library) as batch, cf.ThreadPoolExecutor(max_workers=self.workers) as pool:
+        with Batch(self.library) as batch, cf.ThreadPoolExecutor(max_workers=self.workers) as pool, \
+                cf.ThreadPoolExecutor(max_workers=1) as closer:
             try:
                 while pending or active or creating:
+                    for fut in [f for f in releasing if f.done()]:
+                        j = releasing.pop(fut)
+                        close += fut.result()
+                        used -= j.bytes
                     now = time.perf_counter() - t0
                     # admit in arrival order while the memory estimate fits
                     while pending and pending[0].arrival <= now and used + pending[0].bytes <= self.budget:
@@ -149,8 +164,10 @@ class Server:
                         active.append(j)
                     if not active:
                         nxt = pending[0].arrival - (time.perf_counter() - t0) if pending else 0.05
-                        if creating:
-                            cf.wait(list(creating), timeout=max(0.0, min(nxt, 0.05)), return_when=cf.FIRST_COMPLETED)
+                        waits = list(creating) + list(releasing)
+                        if waits:
+                            # an arrived job held back by the budget waits for a set-up or release
+                            cf.wait(waits, timeout=0.05 if nxt <= 0 else min(nxt, 0.05), return_when=cf.FIRST_COMPLETED)
                         elif nxt > 0:
                             time.sleep(min(nxt, 0.05))
                         continue
@@ -165,17 +182,19 @@ class Server:
                             j.finish = now
                             if on_done:
                                 on_done(j)
-                            tc = time.perf_counter()
                             batch.remove(j.sim)
-                            j.sim.close()
+                            releasing[closer.submit(self._release, j.sim)] = j
                             j.sim = None
-                            close += time.perf_counter() - tc
-                            used -= j.bytes
                             active.remove(j)
                             done.append(j)
                             if log:
                                 log(f"[serve] job {j.jid} {j.case.name} done: latency {j.latency:.3f} s")
+                for fut in list(releasing):
+                    close += fut.result()
+                    used -= releasing.pop(fut).bytes
             finally:
+                for fut in releasing:
+                    fut.exception()
                 for fut, j in creating.items():  # after an error: release what the workers set up
                     try:
                         fut.result()[0].close()

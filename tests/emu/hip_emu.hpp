@@ -104,6 +104,10 @@ template <typename T> inline hipError_t hipMalloc(T** p, size_t n) { return hipM
 inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
 inline hipError_t hipMallocAsync(void** p, size_t n, hipStream_t) { return hipMalloc(p, n); }
 inline hipError_t hipFreeAsync(void* p, hipStream_t) { return hipFree(p); }
+typedef void* hipMemPool_t;
+enum hipMemPoolAttr { hipMemPoolAttrReleaseThreshold = 4 };
+inline hipError_t hipDeviceGetDefaultMemPool(hipMemPool_t* m, int) { *m = nullptr; return hipSuccess; }
+inline hipError_t hipMemPoolSetAttribute(hipMemPool_t, hipMemPoolAttr, void*) { return hipSuccess; }
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) { std::memcpy(d, s, n); return hipSuccess; }
 inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { std::memcpy(d, s, n); return hipSuccess; }
 inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { std::memset(d, v, n); return hipSuccess; }
