@@ -228,6 +228,8 @@ def main(argv=None):
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 plain / fused comparison runs")
     ap.add_argument("--only-headline", action="store_true", help="run just the headline workload (for profilers)")
     ap.add_argument("--only-sphere", action="store_true", help="run just the falling-sphere workload (for profilers)")
+    ap.add_argument("--only-small", action="store_true",
+                    help="run just the solitary-wave and Taylor-Green workloads (for profilers)")
     ap.add_argument("--sphere-precision", type=int, metavar="STEPS", default=0,
                     help="run the falling sphere STEPS steps in FP32 and FP64 and print their settling velocity and drag")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -253,6 +255,10 @@ def main(argv=None):
 
     if a.only_sphere:
         print(json.dumps(measure_sphere(a.steps, a.warmup)), flush=True)
+        return 0
+
+    if a.only_small:
+        print(json.dumps(measure_small_configs(a.steps)), flush=True)
         return 0
 
     head_case = cases.lock_exchange(storage=F32, path=PATH_FUSED)

@@ -127,7 +127,10 @@ public:
         // salinity gradients come from the step itself; the phase field keeps the pass (its
         // Laplacian and col_PF read phi of the neighbours as well)
         G.inline_rho = G.grad_rho && !G.grad_phi && cfg_.scalar == IPL_SCALAR_SALINITY;
-        G.need_grad = (G.grad_rho || G.grad_phi) && !G.inline_rho;
+        // a 2D phase field gathers two lateral neighbours of nine rows each; in 3D (four of
+        // nineteen) the pass stays cheaper
+        G.inline_phi = G.grad_phi && gr_.Q == 9 && gr_.Qs == 9;
+        G.need_grad = (G.grad_rho || G.grad_phi) && !G.inline_rho && !G.inline_phi;
         G.half_force = 1;
         G.wall_fn = cfg_.wall_function != 0;
         G.les = cfg_.les != 0;

@@ -23,6 +23,8 @@ struct FGeom {
     int grad_rho;    // rho varies: the pressure-gradient term of Force is nonzero
     int inline_rho;  // salinity: the step forms the neighbours' rho from their g rows itself
                      // (along the line through LDS), and no scalar-field pass runs
+    int inline_phi;  // 2D phase field: the step forms the neighbours' phi (and rho) from their g
+                     // rows itself, like inline_rho, and no scalar-field pass runs
     int grad_phi;    // phase field: grad phi and its Laplacian (Force, col_PF)
     int wall_fn;     // wall-function shear on the cells next to walls (collision.cu:28-55)
     int les;         // per-cell tau, tau_s rows set by the LES pass (basic.cu:285-339)

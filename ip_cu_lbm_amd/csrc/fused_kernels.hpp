@@ -613,14 +613,39 @@ __device__ inline MT grad_axis(const FGeom& G, const LineNb& nb, const FTables<M
 template <typename MT>
 __device__ inline MT rho_from_sum(const FCoef<MT>& P, MT sc) { return P.rho_L + sc * P.beta_s; }
 
-// the same at cell m from its g rows, summed in the order every other pass sums them
+// rho of a phase-field cell from its phi (update_rho; the salinity term is zero)
+template <typename MT>
+__device__ inline MT rho_from_phi(const FCoef<MT>& P, MT phi) { return P.rho_L + P.drho * phi; }
+
+// the sum of the g rows of cell m (S or phi), in the order every other pass sums them
 template <typename ST, typename MT, int QS>
-__device__ inline MT rho_from_g(const FGeom& G, const FCoef<MT>& P, const ST* __restrict__ gsrc, uint32_t m) {
+__device__ inline MT scalar_from_g(const FGeom& G, const ST* __restrict__ gsrc, uint32_t m) {
     const uint32_t NC = (uint32_t)G.ncell;
     MT sc = Store<ST, MT>::load(elem(gsrc, m), Lattice<QS>::w(0), G.inv_scale);
 #pragma unroll
     for (int k = 1; k < QS; ++k) sc += Store<ST, MT>::load(elem(gsrc, k * NC + m), Lattice<QS>::w(k), G.inv_scale);
-    return rho_from_sum(P, sc);
+    return sc;
+}
+
+// rho of a salinity cell m from its g rows
+template <typename ST, typename MT, int QS>
+__device__ inline MT rho_from_g(const FGeom& G, const FCoef<MT>& P, const ST* __restrict__ gsrc, uint32_t m) {
+    return rho_from_sum(P, scalar_from_g<ST, MT, QS>(G, gsrc, m));
+}
+
+// phi of the two neighbours of the work-item's cell along axis A (phase field, no scalar-field
+// pass): along the line from the workgroup's LDS row of phi, across lines from the neighbour
+// cells' g rows
+template <typename ST, typename MT, int QS, int S, int A>
+__device__ inline void phi_pair_inline(const FGeom& G, const LineNb& nb, const ST* __restrict__ gsrc,
+                                       const MT* lds_s, int i, MT& hi, MT& lo) {
+    if constexpr (A == S) {
+        hi = lds_s[wrap_or_clamp(i + 1, G.ns, G.periodic[S])];
+        lo = lds_s[wrap_or_clamp(i - 1, G.ns, G.periodic[S])];
+    } else {
+        hi = scalar_from_g<ST, MT, QS>(G, gsrc, nb_index<S, A, 1>(G, nb, i));
+        lo = scalar_from_g<ST, MT, QS>(G, gsrc, nb_index<S, A, -1>(G, nb, i));
+    }
 }
 
 // grad rho along axis A without the scalar-field pass: along the line from the workgroup's LDS
@@ -663,19 +688,19 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
 
     MT Pf[Q], Pg[QS > 0 ? QS : 1];
     MT g[QS > 0 ? QS : 1];
-    MT* lds_r = lds_g;  // [ns] rho of the line (inline_rho), after the g rows
+    MT* lds_r = lds_g;  // [ns] rho (inline_rho) or phi (inline_phi) of the line, after the g rows
     if constexpr (QS > 0) lds_r += (sdir_count<QS, S>() + 1) * G.ns;
     if constexpr (QS > 0) {
         if (active) {
 #pragma unroll
             for (int k = 0; k < QS; ++k) g[k] = Store<ST, MT>::load(elem(gsrc, k * NC + n), Lattice<QS>::w(k), G.inv_scale);
         }
-        if (G.inline_rho) {  // uniform over the launch: every work-item reaches the barrier
+        if (G.inline_rho || G.inline_phi) {  // uniform over the launch: every work-item reaches the barrier
             if (active) {
                 MT sc = g[0];
 #pragma unroll
                 for (int k = 1; k < QS; ++k) sc += g[k];
-                lds_r[i] = rho_from_sum(P, sc);
+                lds_r[i] = G.inline_phi ? sc : rho_from_sum(P, sc);
             }
             __syncthreads();
         }
@@ -722,7 +747,27 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
             if (G.wall_fn) wall_shear<MT, S, d3>(G, tb, P, ii, c1, c2, u, v, w, rho, Fx, Fy, Fz);
         }
         MT gpx = 0, gpy = 0, gpz = 0;  // grad phi (col_PF reuses it)
-        if (G.grad_phi) {
+        MT grx = 0, grz = 0;           // grad rho of a 2D phase field (inline_phi)
+        if constexpr (QS == 9 && !d3) {
+            if (G.inline_phi) {
+                // the same terms from the neighbours' phi formed here (x across lines, z along S
+                // or across); rho of a neighbour is rho_L + (rho_H - rho_L) phi
+                MT xh, xl, zh, zl;
+                phi_pair_inline<ST, MT, QS, S, 0>(G, nb, gsrc, lds_r, ii, xh, xl);
+                phi_pair_inline<ST, MT, QS, S, 2>(G, nb, gsrc, lds_r, ii, zh, zl);
+                const MT idx = *elem(tb.invd[0], (uint32_t)(S == 0 ? ii : (o1 == 0 ? c1 : c2)));
+                const MT idz = *elem(tb.invd[2], (uint32_t)(S == 2 ? ii : (o1 == 2 ? c1 : c2)));
+                gpx = (xh - xl) * idx;
+                gpz = (zh - zl) * idz;
+                const MT lap = (xh - MT(2) * phi + xl) * *elem(tb.inv_wx2, (uint32_t)(S == 0 ? ii : (o1 == 0 ? c1 : c2))) +
+                               (zh - MT(2) * phi + zl) * P.inv_dz2;
+                const MT st = P.st_beta * phi * (phi - MT(1.0)) * (phi - MT(0.5)) - P.kap * lap;
+                Fx += st * gpx; Fz += st * gpz;
+                grx = (rho_from_phi(P, xh) - rho_from_phi(P, xl)) * idx;
+                grz = (rho_from_phi(P, zh) - rho_from_phi(P, zl)) * idz;
+            }
+        }
+        if (G.grad_phi && !G.inline_phi) {
             gpx = grad_axis<MT, S, 0>(G, nb, tb, b.phi_n, c1, c2, ii);
             if constexpr (d3) gpy = grad_axis<MT, S, 1>(G, nb, tb, b.phi_n, c1, c2, ii);
             gpz = grad_axis<MT, S, 2>(G, nb, tb, b.phi_n, c1, c2, ii);
@@ -738,7 +783,12 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
         }
         if (P.boussinesq) Fz += -S_ * P.beta_s * P.gravity;
         else Fz += -rho * P.gravity;
-        if (G.grad_rho) {
+        if (G.grad_rho && G.inline_phi) {
+            const MT pp = -p * P.press;
+            Fx += pp * grx;
+            Fz += pp * grz;
+        }
+        if (G.grad_rho && !G.inline_phi) {
             const MT pp = -p * P.press;
             if constexpr (QS > 0) {
                 if (G.inline_rho) {
@@ -831,50 +881,54 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
     if constexpr (QS > 0) transport<ST, MT, QS, NPTS, S, FLUX>(G, nb, Wp, Wm, rn, b.g_dst, Pg, lds_g, ii, active);
 }
 
-// rho (and phi) of step n for the gradients of the next fused step; four consecutive cells per
-// work-item with 4-wide loads when the rows allow it (ncell a multiple of 4)
-template <typename T> struct alignas(4 * sizeof(T)) Vec4 { T v[4]; };
+// rho (and phi) of step n for the gradients of the next fused step; V consecutive cells per
+// work-item with V-wide loads when the rows allow it (ncell a multiple of V).  The host takes
+// V = 4 on large grids and fewer cells per work-item where that would leave the chip with too few
+// workgroups to hide the load latency (the 1024 x 256 phase-field wave has 262,144 cells).
+template <typename T, int V> struct alignas(V * sizeof(T)) VecN { T v[V]; };
 
-template <typename ST, typename MT, int QS>
+template <typename ST, typename MT, int QS, int V>
 __global__ void k_scalar_fields(FGeom G, const ST* g_src, MT* rho_n, MT* phi_n, FCoef<MT> P) {
-    const int64_t n0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * 4;
+    const int64_t n0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * V;
     if (n0 >= G.ncell) return;
     const int64_t NC = G.ncell;
-    MT sc[4] = {MT(0), MT(0), MT(0), MT(0)};
-    const bool vec = (NC & 3) == 0;
+    MT sc[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) sc[j] = MT(0);
+    const bool vec = (NC % V) == 0;
     if constexpr (QS > 0) {
 #pragma unroll
         for (int k = 0; k < QS; ++k) {
-            ST x[4];
+            ST x[V];
             if (vec) {
-                const Vec4<ST> v = *reinterpret_cast<const Vec4<ST>*>(g_src + k * NC + n0);
+                const VecN<ST, V> v = *reinterpret_cast<const VecN<ST, V>*>(g_src + k * NC + n0);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) x[j] = v.v[j];
+                for (int j = 0; j < V; ++j) x[j] = v.v[j];
             } else {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) x[j] = n0 + j < NC ? g_src[k * NC + n0 + j] : ST{};
+                for (int j = 0; j < V; ++j) x[j] = n0 + j < NC ? g_src[k * NC + n0 + j] : ST{};
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < V; ++j) {
                 const MT y = Store<ST, MT>::load(&x[j], Lattice<QS>::w(k), G.inv_scale);
                 sc[j] = k == 0 ? y : sc[j] + y;
             }
         }
     }
-    Vec4<MT> r, ph;
+    VecN<MT, V> r, ph;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < V; ++j) {
         const MT S = G.scalar == IPL_SCALAR_SALINITY ? sc[j] : MT(0);
         const MT phi = G.scalar == IPL_SCALAR_PHASE_FIELD ? sc[j] : MT(0);
         r.v[j] = P.rho_L + P.drho * phi + S * P.beta_s;
         ph.v[j] = phi;
     }
     if (vec) {
-        if (G.grad_rho) *reinterpret_cast<Vec4<MT>*>(rho_n + n0) = r;
-        if (G.grad_phi) *reinterpret_cast<Vec4<MT>*>(phi_n + n0) = ph;
+        if (G.grad_rho) *reinterpret_cast<VecN<MT, V>*>(rho_n + n0) = r;
+        if (G.grad_phi) *reinterpret_cast<VecN<MT, V>*>(phi_n + n0) = ph;
     } else {
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < V; ++j)
             if (n0 + j < NC) {
                 if (G.grad_rho) rho_n[n0 + j] = r.v[j];
                 if (G.grad_phi) phi_n[n0 + j] = ph.v[j];
@@ -968,7 +1022,7 @@ size_t fused_lds_bytes(int s, int ns) {
     int nf = 1, ng = 0;
     for (int k = 0; k < Q; ++k) nf += comp<Q>(k, s) != 0;
     if constexpr (QS > 0) {
-        ng = 2;  // + the line's rho (inline_rho)
+        ng = 2;  // + the line's rho (inline_rho) or phi (inline_phi)
         for (int k = 0; k < QS; ++k) ng += comp<QS>(k, s) != 0;
     }
     return (size_t)(nf + ng) * ns * sizeof(MT);
@@ -1083,8 +1137,17 @@ void fused_launch_les(const FGeom& G, const FTables<MT>& tb, const ST* f, const 
 
 template <typename ST, typename MT, int QS>
 void fused_launch_scalar_fields(const FGeom& G, const ST* g_src, MT* rho_n, MT* phi_n, const FCoef<MT>& P, hipStream_t st) {
-    const int64_t items = (G.ncell + 3) / 4;
-    IPL_LAUNCH((k_scalar_fields<ST, MT, QS>), (items + 255) / 256, 256, 0, st, G, g_src, rho_n, phi_n, P);
+    // at least 1024 workgroups of 256 work-items (four per CU) before cells are grouped
+    const int64_t NC = G.ncell;
+    if (NC >= (int64_t)4 * 256 * 1024) {
+        const int64_t items = (NC + 3) / 4;
+        IPL_LAUNCH((k_scalar_fields<ST, MT, QS, 4>), (items + 255) / 256, 256, 0, st, G, g_src, rho_n, phi_n, P);
+    } else if (NC >= (int64_t)2 * 256 * 1024) {
+        const int64_t items = (NC + 1) / 2;
+        IPL_LAUNCH((k_scalar_fields<ST, MT, QS, 2>), (items + 255) / 256, 256, 0, st, G, g_src, rho_n, phi_n, P);
+    } else {
+        IPL_LAUNCH((k_scalar_fields<ST, MT, QS, 1>), (NC + 255) / 256, 256, 0, st, G, g_src, rho_n, phi_n, P);
+    }
 }
 
 template <typename ST, typename MT, int Q, int QS>
