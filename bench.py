@@ -211,7 +211,15 @@ def pmc_summary(fetch_dir, write_dir, cells):
     ws = (cells * 4 / 1024.0) / w_copy if w_copy else 1.0
     total = (f_main * fs + w_main * ws) * 1024.0
     d = {"workload": WORKLOAD, "fetch_kib": f_main, "write_kib": w_main, "fetch_scale": fs, "write_scale": ws,
-         "hbm_bytes_per_launch": total, "algorithmic_bytes_per_launch": 2 * 26 * 4 * cells}
+         "hbm_bytes_per_launch": total, "algorithmic_bytes_per_launch": 2 * 26 * 4 * cells,
+         "commands": [
+             "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o run -- "
+             "python bench.py --only-headline --steps 20 --warmup 2",
+             "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o run -- "
+             "python bench.py --only-headline --steps 20 --warmup 2",
+             "python bench.py --pmc-summary gpurun_out/pmc_fetch gpurun_out/pmc_write"],
+         "copy_kernel": f"k_store_pops<float,float,*>: reads an FP64 row and writes an FP32 row of {cells:,} values "
+                        f"({cells * 8 // 1024} KiB read, {cells * 4 // 1024} KiB written)"}
     os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
     with open(TRAFFIC_FILE, "w") as fh:
         json.dump(d, fh, indent=1)
