@@ -150,6 +150,28 @@ def measure_small_configs(steps):
     return out
 
 
+def measure_tuner(steps=100):
+    """The line-axis autotuner (ip_cu_lbm_amd.tune) on uniform grids: every admissible axis and
+    the library's own choice timed over `steps` steps (graph replay, HIP events):
+    - the falling sphere 192 x 192 x 384, D3Q19 FP32 with its immersed boundary (whole step);
+    - a D3Q19 FP32 box 160 x 96 x 72 (walls, no scalar field);
+    - the 2D cylinder 512 x 512, D2Q9 FP32 with 400 markers."""
+    from ip_cu_lbm_amd import F32, _abi, cases, tune
+    out = []
+    box = cases.lock_exchange(nx=160, ny=96, nz=72, ratio=1.0, storage=F32, scalar_lattice=0).replace(
+        flux_axes=_abi.FLUX_NONE, scalar=_abi.SCALAR_NONE, gravity=0.0, name="box_160x96x72")
+    box.init = {k: v for k, v in box.init.items() if k != "S"}
+    for case in (cases.falling_sphere(storage=F32), box,
+                 cases.cylinder_2d(n=(512, 512), markers=400, storage=F32)):
+        r = tune.tune_line_axis(case, steps=steps, warmup=20)
+        best = r["ms_per_step"][tune.AXES[r["line_axis"]]]
+        out.append({"workload": case.name, "cells": case.n_cells, "tuned_axis": tune.AXES[r["line_axis"]],
+                    "ms_per_step": {a: round(t, 5) for a, t in r["ms_per_step"].items()},
+                    "auto_ms_per_step": round(r["auto_ms_per_step"], 5),
+                    "tuned_vs_auto": round(r["auto_ms_per_step"] / best, 3)})
+    return out
+
+
 def measure_serving(n_jobs=256):
     """The serving mix: 256 seeded Poisson arrivals (mean gap 20 ms) of D2Q9/D3Q19 FP32 jobs of
     mixed sizes (2D sides 64-512, 3D sides 32-128, 500-5000 steps, ~30 % with salinity, ~10 % with
@@ -236,6 +258,7 @@ def main(argv=None):
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 plain / fused comparison runs")
     ap.add_argument("--only-headline", action="store_true", help="run just the headline workload (for profilers)")
     ap.add_argument("--only-sphere", action="store_true", help="run just the falling-sphere workload (for profilers)")
+    ap.add_argument("--only-tuner", action="store_true", help="run just the line-axis tuner workloads")
     ap.add_argument("--only-small", action="store_true",
                     help="run just the solitary-wave and Taylor-Green workloads (for profilers)")
     ap.add_argument("--sphere-precision", type=int, metavar="STEPS", default=0,
@@ -269,6 +292,10 @@ def main(argv=None):
         print(json.dumps(measure_small_configs(a.steps)), flush=True)
         return 0
 
+    if a.only_tuner:
+        print(json.dumps(measure_tuner()), flush=True)
+        return 0
+
     head_case = cases.lock_exchange(storage=F32, path=PATH_FUSED)
     log(f"[bench] headline: {WORKLOAD} fused FP32, warmup {a.warmup}, steps {a.steps}")
     head = measure(head_case, a.steps, a.warmup, profile_steps=min(a.steps, 200))
@@ -294,6 +321,8 @@ def main(argv=None):
         sw = measure_small_configs(a.steps)
         log(f"[bench] solitary wave: fused FP32 {sw['solitary_wave']['fused_fp32_mlups']:.0f} MLUPS, "
             f"Taylor-Green: fused FP64 {sw['taylor_green']['fused_fp64_mlups']:.0f} MLUPS")
+        tuned = measure_tuner()
+        log("[bench] line-axis tuner: " + "; ".join(f"{t['workload']} {t['tuned_axis']} x{t['tuned_vs_auto']}" for t in tuned))
         srv = measure_serving()
         srv["aggregate_vs_single_large_case"] = round(srv["aggregate_mlups"] / head["mlups"], 3)
         log(f"[bench] serving mix: {srv['jobs']} jobs, p50 {srv['p50_s']:.2f} s, p99 {srv['p99_s']:.2f} s, "
@@ -309,6 +338,7 @@ def main(argv=None):
             "falling_sphere": sph,
             "solitary_wave": sw["solitary_wave"],
             "taylor_green": sw["taylor_green"],
+            "line_axis_tuner": tuned,
             "serving": srv,
         }
 

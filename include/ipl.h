@@ -101,7 +101,11 @@ typedef struct ipl_config {
     int32_t les;              /* 1: coherent-structure LES sets tau and tau_s of every cell after
                                  each step (LES, basic.cu:285-339, called after update_rho at
                                  Boegman2005run18IB.cu:339); fused path */
-    int32_t reserved1;
+    int32_t line_axis;        /* fused path, grids without a stretched axis: the axis its lines (one
+                                 workgroup each) run along, 1/2/3 = x/y/z; 0 = chosen by the
+                                 library (the axis whose lines fill their wavefronts best).  An
+                                 axis that cannot hold a line makes the fused path unavailable.
+                                 No counterpart in the reference (one thread per cell there). */
     double mu_L, mu_H;        /* LES: dynamic viscosity of phi = 0 / phi = 1 (muL, muH) */
 } ipl_config;
 

@@ -52,7 +52,7 @@ class IplConfig(ctypes.Structure):
         ("scalar", c_int32), ("scalar_lattice", c_int32), ("storage", c_int32), ("path", c_int32),
         ("f16_scale", c_double), ("check_finite", c_int32), ("pf_tau", c_double),
         ("wall_function", c_int32), ("reserved0", c_int32), ("wall_nu", c_double),
-        ("les", c_int32), ("reserved1", c_int32), ("mu_L", c_double), ("mu_H", c_double),
+        ("les", c_int32), ("line_axis", c_int32), ("mu_L", c_double), ("mu_H", c_double),
     ]
 
 

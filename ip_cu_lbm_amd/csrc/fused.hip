@@ -21,6 +21,7 @@ namespace ipl {
 // cells whose LDS rows fit the 64 KiB budget.  Measured: 48 x 40 x 72 D3Q19 step 7.9 -> 6.7 us
 // along x; full lines neutral.
 int uniform_stream_axis(const Graph& g, const ipl_config& cfg) {
+    if (cfg.line_axis >= 1 && cfg.line_axis <= 3) return cfg.line_axis - 1;  // the caller's (tuned) choice
     const int order[3] = {2, 0, 1};
     const size_t mt = cfg.storage == IPL_F64 ? 8 : 4;
     int best = 2;
@@ -674,7 +675,13 @@ private:
 bool fused_supported(const Graph& g, const ipl_config& cfg, std::string& why) {
     if (cfg.solid) { why = "cell masks need the plain path"; return false; }
     if (g.flux_mode == FLUX_MODE_CORNER) { why = "the two-axis corner scheme runs on the plain path"; return false; }
+    if (cfg.line_axis < 0 || cfg.line_axis > 3) { why = "line_axis must be 0 (automatic) or 1-3"; return false; }
+    if (cfg.line_axis && g.flux_mode == FLUX_MODE_ONE_AXIS && cfg.line_axis - 1 != g.flux_axis) {
+        why = "the lines of a stretched grid run along its stretched axis";
+        return false;
+    }
     const int s = g.flux_mode == FLUX_MODE_ONE_AXIS ? g.flux_axis : uniform_stream_axis(g, cfg);
+    if (cfg.line_axis && ((g.dim == 2 && s == 1) || g.n[s] < 2)) { why = "no lines along the requested axis"; return false; }
     if (g.n[s] > 512) { why = "stream-axis lines longer than 512 cells"; return false; }
     // every population and field array is addressed with 32-bit byte offsets from its base
     const size_t row_bytes = (size_t)g.N * (cfg.storage == IPL_F64 ? 8 : 4);
