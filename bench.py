@@ -118,6 +118,89 @@ def measure_sphere_precision(steps=20000, every=1000):
             "history_fp64": h64, "history_fp32": h32}
 
 
+def _rel(a, b):
+    import numpy as np
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def measure_precision(with_oracle=False):
+    """Relative L2 errors of every field after the configurations' full step counts:
+    - lock exchange 512 x 32 x 64 (D3Q19 + D3Q7 salinity), 5000 steps;
+    - phase-field solitary wave 1024 x 256 (D2Q9 + D2Q9 phi), 10000 steps.
+    For each: the fused FP32 path against the fused FP64 path, and the fused FP64 path against the
+    plain FP64 path (one kernel per reference stage).  Both FP64 GPU paths match the FP64 oracle to
+    1e-12 on the reduced cases of the GPU tests; with `with_oracle` the solitary wave's fused FP64
+    result is also compared with the oracle itself at full size (OpenMP on the host, minutes)."""
+    import numpy as np
+    from ip_cu_lbm_amd import F32, F64, PATH_FUSED, PATH_PLAIN, Simulation, cases
+    out = {}
+    for name, make, steps, fields in (
+            ("lock_exchange_512x32x64_d3q19_d3q7", cases.lock_exchange, 5000, ("p", "u", "v", "w", "S")),
+            ("solitary_wave_1024x256_d2q9_phase_field", cases.solitary_wave, 10000, ("p", "u", "w", "phi"))):
+        res = {}
+        for tag, st, path in (("fused_fp64", F64, PATH_FUSED), ("fused_fp32", F32, PATH_FUSED), ("plain_fp64", F64, PATH_PLAIN)):
+            with Simulation(make(storage=st, path=path)) as sim:
+                left = steps
+                while left > 0:
+                    k = min(1000, left)
+                    sim.step(k)
+                    left -= k
+                res[tag] = {f: sim.read(f) for f in fields}
+            log(f"[precision] {name} {tag}: {steps} steps done")
+        for r in res.values():  # the velocity vector (a near-zero component alone is ill-conditioned)
+            r["vel"] = np.concatenate([r[c] for c in ("u", "v", "w") if c in r])
+        keys = fields + ("vel",)
+        entry = {"steps": steps,
+                 "fp32_vs_fp64_rel_l2": {f: _rel(res["fused_fp32"][f], res["fused_fp64"][f]) for f in keys},
+                 "fused_fp64_vs_plain_fp64_rel_l2": {f: _rel(res["fused_fp64"][f], res["plain_fp64"][f]) for f in keys}}
+        if with_oracle and name.startswith("solitary"):
+            import oracle
+            oracle.build()
+            o = oracle.Oracle(make(storage=F64))
+            try:
+                for k in range(0, steps, 500):
+                    o.step(500)
+                    log(f"[precision] oracle {name}: step {k + 500}")
+                got = {f: o.read(f) for f in fields}
+                got["vel"] = np.concatenate([got["u"], got["w"]])
+                entry["fused_fp64_vs_oracle_rel_l2"] = {f: _rel(res["fused_fp64"][f], got[f]) for f in keys}
+            finally:
+                o.close()
+            entry["oracle_one_ulp_sensitivity"] = oracle_sensitivity(make(storage=F64), (100, 300, 500, 1000))
+        out[name] = entry
+        log(f"[precision] {name}: {json.dumps(entry)}")
+    return out
+
+
+def oracle_sensitivity(case, checkpoints):
+    """The oracle against itself with phi of one interface cell moved by one ulp: the relative L2
+    difference of p, the velocity and phi at each checkpoint.  It measures how fast the flow
+    amplifies round-off, which bounds how long two correct FP64 implementations that sum in
+    different orders can agree."""
+    import numpy as np
+    import oracle
+    pert = case.replace(init=dict(case.init))
+    phi = pert.init["phi"].copy()
+    i = int(np.argmax(np.abs(phi - 0.5) < 0.4))
+    phi[i] = np.nextafter(phi[i], 2.0)
+    pert.init["phi"] = phi
+    a, b = oracle.Oracle(case), oracle.Oracle(pert)
+    out, done = {}, 0
+    try:
+        for t in checkpoints:
+            a.step(t - done)
+            b.step(t - done)
+            done = t
+            va = np.concatenate([a.read("u"), a.read("w")])
+            vb = np.concatenate([b.read("u"), b.read("w")])
+            out[str(t)] = {"p": _rel(b.read("p"), a.read("p")), "vel": _rel(vb, va), "phi": _rel(b.read("phi"), a.read("phi"))}
+            log(f"[precision] one-ulp sensitivity, step {t}: {out[str(t)]}")
+    finally:
+        a.close()
+        b.close()
+    return out
+
+
 def measure_small_configs(steps):
     """The two 2D workloads beside the headline (both fit the 256 MB last-level cache, so their
     bandwidth figure is not an HBM one):
@@ -259,6 +342,10 @@ def main(argv=None):
     ap.add_argument("--only-headline", action="store_true", help="run just the headline workload (for profilers)")
     ap.add_argument("--only-sphere", action="store_true", help="run just the falling-sphere workload (for profilers)")
     ap.add_argument("--only-tuner", action="store_true", help="run just the line-axis tuner workloads")
+    ap.add_argument("--precision", action="store_true",
+                    help="FP32 vs FP64 and fused vs plain FP64 field errors of the lock exchange and solitary wave")
+    ap.add_argument("--precision-oracle", action="store_true",
+                    help="with --precision: also run the oracle on the full solitary wave (minutes)")
     ap.add_argument("--only-small", action="store_true",
                     help="run just the solitary-wave and Taylor-Green workloads (for profilers)")
     ap.add_argument("--sphere-precision", type=int, metavar="STEPS", default=0,
@@ -290,6 +377,10 @@ def main(argv=None):
 
     if a.only_small:
         print(json.dumps(measure_small_configs(a.steps)), flush=True)
+        return 0
+
+    if a.precision:
+        print(json.dumps(measure_precision(with_oracle=a.precision_oracle)), flush=True)
         return 0
 
     if a.only_tuner:
