@@ -169,7 +169,40 @@ def measure_precision(with_oracle=False):
             entry["oracle_one_ulp_sensitivity"] = oracle_sensitivity(make(storage=F64), (100, 300, 500, 1000))
         out[name] = entry
         log(f"[precision] {name}: {json.dumps(entry)}")
+    if with_oracle:
+        out["falling_sphere_192x192x384_d3q19_ibm_mdf3_peskin4"] = sphere_vs_oracle(steps=4)
     return out
+
+
+def sphere_vs_oracle(steps):
+    """The full falling sphere (14,155,776 cells, 1800 markers, multi-direct forcing with the
+    Peskin kernel) in FP64 on the fused path against the oracle after `steps` steps: fluid fields,
+    marker forces and the body state (the oracle's set-up takes minutes at this size)."""
+    import numpy as np
+    import oracle
+    from ip_cu_lbm_amd import F64, PATH_FUSED, Simulation, cases
+    case = cases.falling_sphere(storage=F64, path=PATH_FUSED)
+    with Simulation(case) as sim:
+        sim.step(steps)
+        g = {f: sim.read(f) for f in ("p", "u", "v", "w")}
+        gf, gb = sim.markers("force"), sim.body_state()
+    oracle.build()
+    log("[precision] falling sphere: oracle set-up ...")
+    o = oracle.Oracle(case)
+    try:
+        for k in range(steps):
+            o.step(1)
+            log(f"[precision] falling sphere: oracle step {k + 1}")
+        r = {f: o.read(f) for f in ("p", "u", "v", "w")}
+        of, ob = o.markers("force"), o.body_state()
+    finally:
+        o.close()
+    vel = lambda d: np.concatenate([d["u"], d["v"], d["w"]])
+    res = {"steps": steps, "p": _rel(g["p"], r["p"]), "vel": _rel(vel(g), vel(r)),
+           "marker_force": _rel(gf.ravel(), of.reshape(gf.shape).ravel()),
+           "body_force": _rel(gb["force"], ob["force"]), "body_vel": _rel(gb["vel"], ob["vel"])}
+    log(f"[precision] falling sphere vs oracle: {res}")
+    return res
 
 
 def oracle_sensitivity(case, checkpoints):

@@ -162,6 +162,13 @@ def test_fused_solitary_wave_fp64(oracle_lib):
     _compare(case, 150, ("p", "vel", "phi"), 1e-12, oracle_lib)
 
 
+def test_fused_lock_exchange_full_size_fp64(oracle_lib):
+    """The headline workload at its full size (512 x 32 x 64, z stretched by 1.02, D3Q19 + D3Q7
+    salinity) on the fused path against the oracle."""
+    case = cases.lock_exchange(path=PATH_FUSED)
+    _compare(case, 40, ("p", "vel", "S"), 1e-12, oracle_lib)
+
+
 def test_fused_solitary_wave_full_size_fp64(oracle_lib):
     """The full 1024 x 256 wave (stretched z, seeded interface noise) over 100 steps.  Longer
     horizons cannot hold 1e-12 for any two FP64 implementations that sum in different orders: the
