@@ -45,3 +45,73 @@ def test_estimate_device_bytes_fused_formula():
     assert 2 * (19 + 7) * 4 <= per_cell <= 2 * (19 + 7) * 4 + 160, per_cell
     big = estimate_device_bytes(cases.falling_sphere())
     assert 3.0e9 < big < 4.0e9
+
+
+def test_server_admission_and_release_accounting(monkeypatch):
+    """The serving loop with stand-ins for the library (no GPU): jobs are admitted in arrival order
+    under the budget, every admitted job is set up once, stepped only while open and released once
+    (on the release thread), and the bytes alive never exceed the budget."""
+    import threading
+    import time
+    import types
+
+    lock = threading.Lock()
+    state = {"live": 0, "peak": 0, "created": 0, "closed": 0}
+    per_job = 1000
+
+    class FakeSim:
+        def __init__(self, case, library=None):
+            time.sleep(0.002)
+            with lock:
+                state["live"] += per_job
+                state["peak"] = max(state["peak"], state["live"])
+                state["created"] += 1
+            self.open = True
+            self.steps = 0
+
+        def close(self):
+            time.sleep(0.003)
+            with lock:
+                assert self.open
+                self.open = False
+                state["live"] -= per_job
+                state["closed"] += 1
+
+    class FakeBatch:
+        def __init__(self, library=None):
+            self.members = []
+
+        def add(self, sim):
+            self.members.append(sim)
+
+        def remove(self, sim):
+            self.members.remove(sim)
+
+        def step(self, counts):
+            assert len(counts) == len(self.members)
+            for sim, k in zip(self.members, counts):
+                assert sim.open and k > 0
+                sim.steps += k
+            time.sleep(0.001)
+
+        def __enter__(self):
+            return self
+
+        def __exit__(self, *exc):
+            return False
+
+    monkeypatch.setattr(serve, "Simulation", FakeSim)
+    monkeypatch.setattr(serve, "Batch", FakeBatch)
+    monkeypatch.setattr(serve, "estimate_device_bytes", lambda case, library=None: per_job)
+    rng = np.random.default_rng(7)
+    jobs, t = [], 0.0
+    for j in range(40):
+        t += float(rng.exponential(0.002))
+        case = types.SimpleNamespace(n_cells=100 + j, name=f"fake{j}")
+        jobs.append(serve.Job(jid=j, case=case, steps=int(rng.integers(1, 1200)), arrival=t))
+    res = serve.Server(budget_bytes=3.5 * per_job, chunk=400, workers=2).run(jobs)
+    assert res["jobs"] == 40
+    assert state["created"] == state["closed"] == 40 and state["live"] == 0
+    assert state["peak"] <= 3 * per_job
+    assert all(j.done_steps == j.steps and j.sim is None and j.finish >= j.start >= j.arrival - 1e-9 for j in jobs)
+    assert res["aggregate_mlups"] > 0 and res["p99_s"] >= res["p50_s"] > 0
