@@ -177,7 +177,11 @@ enum {
     IPL_STAGE_UPDATE_SCALAR = 6,/* collision.cu:225-236 */
     IPL_STAGE_UPDATE_RHO = 7,   /* collision.cu:238-257 */
     IPL_STAGE_SET_WALL_RHO = 8, /* basic.cu:254-261 (rho, and phi with a phase field) */
-    IPL_STAGE_RESET_F = 9       /* resetF, collision.cu:258-264 */
+    IPL_STAGE_RESET_F = 9,      /* resetF, collision.cu:258-264 */
+    IPL_STAGE_EQU_G = 10        /* equ_g, collision.cu:16-27: the scalar populations of every calc
+                                   cell <- the equilibrium of its scalar (S, or phi with a phase
+                                   field) and u; linear on D3Q7.  Not part of the step (no driver
+                                   calls it); the reference writes its feq argument, here g */
 };
 int ipl_stage(ipl_ctx* ctx, int32_t stage);
 

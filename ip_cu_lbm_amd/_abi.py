@@ -32,7 +32,7 @@ FIELDS = {
 
 STAGES = {
     "equ_f": 0, "Force": 1, "collide": 2, "IP_f": 3, "scalar_collide": 4, "IP_g": 5,
-    "update_scalar": 6, "update_rho": 7, "set_wall_rho": 8, "resetF": 9,
+    "update_scalar": 6, "update_rho": 7, "set_wall_rho": 8, "resetF": 9, "equ_g": 10,
 }
 
 _DP = POINTER(c_double)

@@ -143,6 +143,27 @@ __global__ void k_collide(PlainView<T> s, Params P) {
     for (int k = 0; k < Q; ++k) { s.f[k * Nt + i] = 0; s.feq[k * Nt + i] = 0; }
 }
 
+// ---- equ_g (collision.cu:16-27): the scalar's equilibrium, written into its populations ----
+// (linear with 1/cs^2 = 4 on D3Q7, as in col_g_reg below)
+template <typename T, int QS>
+__global__ void k_equ_g(PlainView<T> s, Params P, const T* __restrict__ sc) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= s.N) return;
+    const T c = P.c, c2 = P.c * P.c;
+    constexpr bool linear = QS == 7;
+    const T cs2inv = linear ? T(4.0) : T(3.0);
+    const T u = s.u[i], v = s.v[i], w = s.w[i], S = sc[i];
+    const T u2 = u * u + v * v + w * w;
+#pragma unroll
+    for (int k = 0; k < QS; ++k) {
+        const Dir d = Lattice<QS>::c(k);
+        const T t = (T(d.x) * c * u + T(d.y) * c * v + T(d.z) * c * w) / c2;
+        T e = T(1.0) + cs2inv * t;
+        if (!linear) e += T(4.5) * t * t - T(1.5) * u2 / c2;
+        s.g[k * s.Ntot + i] = T(Lattice<QS>::w(k)) * S * e;
+    }
+}
+
 // ---- col_g_reg (collision.cu:167-188) ----------------------------------------------------
 template <typename T, int QS>
 __global__ void k_col_g_reg(PlainView<T> s, Params P) {
@@ -565,6 +586,12 @@ int run_stage(const PlainView<T>& s, const Params& P, int stage, hipStream_t st)
         }
         break;
     case IPL_STAGE_RESET_F: IPL_LAUNCH((k_reset_force<T>), bn, kBlock, 0, st, s); break;
+    case IPL_STAGE_EQU_G:
+        if constexpr (QS > 0) {
+            IPL_LAUNCH((k_equ_g<T, QS>), bn, kBlock, 0, st, s, P, P.scalar == IPL_SCALAR_SALINITY ? s.S : s.phi);
+            break;
+        }
+        return IPL_ERR_UNSUPPORTED;
     default: return IPL_ERR_ARGUMENT;
     }
     return IPL_OK;

@@ -609,6 +609,27 @@ void collide(Oracle& o) {
     }
 }
 
+// equ_g (collision.cu:16-27): w_k S (1 + 3t + 4.5t^2 - 1.5u^2/c^2), t = (c_k . u)/c^2, into the
+// scalar populations of every calc cell (no driver calls it; the reference writes its feq
+// argument).  S is the salinity, or phi with a phase field; linear with 1/cs^2 = 4 on D3Q7.
+void equ_g(Oracle& o) {
+    PopSet& P = o.sc; const int Q = P.Q; const Lattice& L = P.L;
+    const double c2 = o.c * o.c;
+    const bool linear = Q == 7;
+    const double cs2inv = linear ? 4.0 : 3.0;
+    const std::vector<double>& S = o.cfg.scalar == IPL_SCALAR_SALINITY ? o.S : o.phi;
+    #pragma omp parallel for schedule(static)
+    for (int id = 0; id < o.N; ++id) {
+        const double u2 = o.u[id] * o.u[id] + o.v[id] * o.v[id] + o.w[id] * o.w[id];
+        for (int k = 0; k < Q; ++k) {
+            const double t = (L.cx[k] * o.c * o.u[id] + L.cy[k] * o.c * o.v[id] + L.cz[k] * o.c * o.w[id]) / c2;
+            double e = 1.0 + cs2inv * t;
+            if (!linear) e += 4.5 * t * t - 1.5 * u2 / c2;
+            P.f[(size_t)id * Q + k] = L.w[k] * S[id] * e;
+        }
+    }
+}
+
 // col_g_reg (collision.cu:167-188): regularized scalar collision.  On D3Q7 the equilibrium
 // is linear and 1/cs^2 = 4 (builder extension); otherwise the reference's form.
 void col_g_reg(Oracle& o) {
@@ -1378,6 +1399,7 @@ void stage(Oracle& o, int s) {
         if (o.cfg.scalar == IPL_SCALAR_PHASE_FIELD) set_wall_rho(o, o.phi);
         break;
     case IPL_STAGE_RESET_F: reset_force(o); break;
+    case IPL_STAGE_EQU_G: if (o.has_scalar) equ_g(o); break;
     }
 }
 

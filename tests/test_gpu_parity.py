@@ -280,3 +280,24 @@ def test_fused_uniform_stream_axis_lds_budget_fp64(oracle_lib):
     with Simulation(case) as sim:
         assert sim.path == PATH_FUSED
     _compare(case, 40, ("p", "vel", "S"), 1e-12, oracle_lib)
+
+
+@pytest.mark.parametrize("lattice,scalar_lattice", [(9, 0), (19, 7), (19, 0)], ids=["d2q9", "d3q7", "d3q19"])
+def test_stage_equ_g_fp64(oracle_lib, lattice, scalar_lattice):
+    """equ_g as a stage of the plain path (scalar populations <- the equilibrium of S and u)
+    against the oracle's, on the same written velocity field; the fused path has no stages."""
+    from test_oracle import _random_velocity, _scalar_case
+    case = _scalar_case(lattice, scalar_lattice).replace(path=PATH_PLAIN)
+    o = oracle_lib.Oracle(case)
+    with Simulation(case) as sim:
+        vel = _random_velocity(o, o.n_calc, 5)
+        for a, x in vel.items():
+            sim.write(a, x)
+        sim.stage("equ_g")
+        o.stage("equ_g")
+        got, want = sim.read("g"), o.read("g")
+    o.close()
+    assert rel_l2(got, want) < 1e-14
+    with Simulation(_scalar_case(lattice, scalar_lattice).replace(path=PATH_FUSED)) as fused:
+        with pytest.raises(Exception):
+            fused.stage("equ_g")

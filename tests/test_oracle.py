@@ -191,3 +191,55 @@ def test_salinity_diffusion_closed_form(oracle_lib):
         o.step(nstep)
         amp = np.abs(o.read("S") - 1.0).max()
         assert abs(amp / (0.1 * np.exp(-D * k * k * nstep)) - 1) < 0.01
+
+
+def _scalar_case(lattice, scalar_lattice):
+    if lattice == 9:
+        c = cases.lattice_taylor_green(nx=12, nz=8)
+        X, _, Z = c.coords()
+        return c.replace(scalar=_abi.SCALAR_SALINITY, scalar_lattice=0, taus=0.52, beta_s=1e-3,
+                         init=dict(c.init, S=0.5 + 0.1 * np.sin(X / 3.0) * np.cos(Z / 2.0)))
+    return cases.lock_exchange(nx=6, ny=5, nz=7, scalar_lattice=scalar_lattice)
+
+
+def _random_velocity(o, n, seed):
+    rng = np.random.default_rng(seed)
+    vel = {a: 0.05 * (rng.random(n) - 0.5) for a in "uvw"}
+    for a, x in vel.items():
+        o.write(a, x)
+    return vel
+
+
+@pytest.mark.parametrize("lattice,scalar_lattice", [(9, 0), (19, 7), (19, 0)], ids=["d2q9", "d3q7", "d3q19"])
+def test_equ_g_moments(oracle_lib, lattice, scalar_lattice):
+    """equ_g (collision.cu:16-27) into the scalar populations: the zeroth moment is S and the first
+    S u for the reference's form and the linear D3Q7 one (the quadratic terms cancel in both);
+    the second moment of the reference's form is S (c^2/3 I + u u).  A wrong weight, sign or
+    velocity factor breaks one of them."""
+    case = _scalar_case(lattice, scalar_lattice)
+    o = oracle_lib.Oracle(case)
+    try:
+        n = o.n_calc
+        vel = _random_velocity(o, n, 3)
+        if lattice == 9:
+            vel["v"] = np.zeros(n)
+            o.write("v", vel["v"])
+        S = o.read("S")
+        o.stage("equ_g")
+        g = o.read("g")
+    finally:
+        o.close()
+    qs = scalar_lattice or lattice
+    c, w = oracle_lib.lattice(qs)
+    speed = case.dz / case.dt
+    assert np.abs(g.sum(axis=0) - S).max() < 1e-14
+    for a, name in enumerate("uvw"):
+        assert np.abs(speed * (c[:, a] @ g) - S * vel[name]).max() < 1e-14, name
+    if qs != 7:
+        for a, na in enumerate("uvw"):
+            for b, nb in enumerate("uvw"):
+                if lattice == 9 and 1 in (a, b):
+                    continue
+                m2 = speed * speed * ((c[:, a] * c[:, b]) @ g)
+                want = S * ((speed * speed / 3.0 if a == b else 0.0) + vel[na] * vel[nb])
+                assert np.abs(m2 - want).max() < 1e-14, (na, nb)
