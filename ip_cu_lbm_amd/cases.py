@@ -433,6 +433,61 @@ def solitary_wave(nx: int = 1024, nz: int = 256, ratio: float = 1.01, width: flo
     return case
 
 
+@dataclasses.dataclass
+class Items:
+    """The run constants a driver reads from its input/initial.csv and derives from them
+    (input_items and Items::setc, function/basic.cu:57-98)."""
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dt: float
+    ratiox: float
+    ratioy: float
+    gz: float
+    nu: float              # kinematic viscosity, m^2/s (the file holds it in 1e-6 m^2/s)
+    save_interval: int
+    total_count: int       # steps: 1 + save_interval * (the file's count of saves)
+    num_velocity: int      # 9 when the grid is one cell deep in y, else 27
+    c: float               # dx / dt
+    tau: float             # 3e-6 / (c^2 dt) + 1/2: setc fixes nu = 1e-6 here, not the file's nu
+    taus: float            # 1.4 * 3 * 10^-alpha / (c^2 dt) + 1/2
+
+    def case_fields(self) -> dict:
+        """The ``Case`` fields these constants set (the driver-specific geometry, widths and
+        fields are not in the file)."""
+        return {"lattice": self.num_velocity, "n": (self.nx, self.ny, self.nz), "dz": self.dx,
+                "dt": self.dt, "tau": self.tau, "taus": self.taus, "wall_nu": self.nu,
+                "steps": self.total_count}
+
+
+def read_initial_csv(path: str, alpha: int = 9) -> Items:
+    """Parses a driver's input/initial.csv (its first line: nx, ny, nz, dx, dt, ratiox, ratioy,
+    gz, nu, save_interval, total_count) and applies input_items' and setc(alpha)'s derivations
+    (basic.cu:57-77; the drivers call setc(9), e.g. LockExchange.cu:16)."""
+    with open(path) as fh:
+        first = fh.readline()
+    v = [x.strip() for x in first.split(",")]
+    nx, ny, nz = int(v[0]), int(v[1]), int(v[2])
+    dx, dt, ratiox, ratioy, gz, nu = (float(x) for x in v[3:9])
+    save_interval, count = int(v[9]), int(v[10])
+    total = 1 + save_interval * count
+    nu *= 10.0 ** -6
+    nx = int(nx / ratiox)
+    ny = int(ny / ratioy)
+    if ny < 1:
+        ny, ratioy = 1, 1.0
+    if ny == 1:
+        q, ratioy = 9, 1.0
+    else:
+        q = 27
+    c = dx / dt
+    tau = 1.0 * 3.0 * 10.0 ** -6 / c / c / dt + 0.5
+    taus = 1.4 * 3.0 * 10.0 ** -alpha / c / c / dt + 0.5
+    return Items(nx=nx, ny=ny, nz=nz, dx=dx, dt=dt, ratiox=ratiox, ratioy=ratioy, gz=gz, nu=nu,
+                 save_interval=save_interval, total_count=total, num_velocity=q, c=c, tau=tau, taus=taus)
+
+
 CASES = {
     "taylor_green": lattice_taylor_green,
     "lock_exchange": lock_exchange,

@@ -112,3 +112,25 @@ def test_refined_axis_builds_tables():
                       width=[wx, np.array([1.0]), wz], periodic=(1, 0, 0), flux_axes=_abi.FLUX_X)
     w = _tables(case, 3, np.float64).reshape(3, 2, 5, -1)
     assert np.abs(w[0].sum(axis=1) - 1).max() < 1e-13
+
+
+def test_read_initial_csv_reference_files():
+    """The reference's input/initial.csv of three drivers (fixtures copied from LockExchange,
+    Horn2001S and Boegman2005run18IB) through input_items + setc(9) (basic.cu:57-77): grid
+    reduction by the ratios, the lattice choice, the step count and the relaxation times."""
+    import os
+    from ip_cu_lbm_amd import cases as C
+    g = os.path.join(os.path.dirname(__file__), "golden")
+    le = C.read_initial_csv(os.path.join(g, "initial_LockExchange.csv"))
+    assert (le.nx, le.ny, le.nz, le.num_velocity) == (800, 1, 100, 9)
+    assert le.total_count == 1 + 1000 * 30 and le.save_interval == 1000
+    assert le.c == 1.0 and abs(le.nu - 1e-6) < 1e-21 and le.gz == -9.81
+    assert abs(le.tau - (3e-6 / 0.001 + 0.5)) < 1e-15            # 0.503
+    assert abs(le.taus - (1.4 * 3e-9 / 0.001 + 0.5)) < 1e-15
+    horn = C.read_initial_csv(os.path.join(g, "initial_Horn2001S.csv"))
+    assert (horn.nx, horn.ny, horn.nz, horn.num_velocity, horn.ratioy) == (200, 1, 58, 9, 1.0)  # 1200/6; 1/12 -> 1
+    assert abs(horn.c - 2.0) < 1e-15 and abs(horn.tau - (3e-6 / 4.0 / 0.0025 + 0.5)) < 1e-15
+    boeg = C.read_initial_csv(os.path.join(g, "initial_Boegman2005run18IB.csv"))
+    assert (boeg.nx, boeg.nz) == (50, 145) and abs(boeg.c - 2.0) < 1e-15
+    f = boeg.case_fields()
+    assert f["lattice"] == 9 and f["n"] == (50, 1, 145) and f["dz"] == 0.002 and f["steps"] == 30001
