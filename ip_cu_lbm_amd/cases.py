@@ -488,6 +488,22 @@ def read_initial_csv(path: str, alpha: int = 9) -> Items:
                  save_interval=save_interval, total_count=total, num_velocity=q, c=c, tau=tau, taus=taus)
 
 
+def read_rates_csv(path: str, q: int) -> np.ndarray:
+    """The relaxation rates of a driver's input/S_D3Q{q}.csv as ``Case.s_rates`` (set_M,
+    basic.cu:171-189): the first q values of the first line, with the stress rows zeroed as set_M
+    zeroes them (rows 7, 8 of D2Q9; 5-9 of D3Q27), since they relax at 1/tau of each cell."""
+    with open(path) as fh:
+        vals = [float(x) for x in fh.readline().split(",")[:q] if x.strip()]
+    if len(vals) != q:
+        raise ValueError(f"{path}: {len(vals)} rates, expected {q}")
+    r = np.array(vals)
+    if q == 9:
+        r[7] = r[8] = 0.0
+    elif q == 27:
+        r[5:10] = 0.0
+    return r
+
+
 CASES = {
     "taylor_green": lattice_taylor_green,
     "lock_exchange": lock_exchange,

@@ -134,3 +134,17 @@ def test_read_initial_csv_reference_files():
     assert (boeg.nx, boeg.nz) == (50, 145) and abs(boeg.c - 2.0) < 1e-15
     f = boeg.case_fields()
     assert f["lattice"] == 9 and f["n"] == (50, 1, 145) and f["dz"] == 0.002 and f["steps"] == 30001
+
+
+def test_read_rates_csv_reference_files():
+    """S_D3Q9.csv / S_D3Q27.csv of the reference (fixtures) as relaxation rates, stress rows zeroed
+    as set_M does (basic.cu:184-185)."""
+    import os
+    import numpy as np
+    from ip_cu_lbm_amd import cases as C
+    g = os.path.join(os.path.dirname(__file__), "golden")
+    raw9 = np.array([float(x) for x in open(os.path.join(g, "S_D3Q9.csv")).readline().split(",")[:9]])
+    r9 = C.read_rates_csv(os.path.join(g, "S_D3Q9.csv"), 9)
+    assert np.array_equal(r9[:7], raw9[:7]) and r9[7] == r9[8] == 0.0
+    r27 = C.read_rates_csv(os.path.join(g, "S_D3Q27.csv"), 27)
+    assert r27.shape == (27,) and np.all(r27[5:10] == 0.0)
