@@ -20,6 +20,13 @@ counter summary for this workload exists under profiles/ (see --pmc-summary), el
 The CPU baseline is the FP64 oracle (a C++/OpenMP restatement of the reference kernels) on a
 bounded number of steps of the same workload, on the host's cores.
 
+Beside the headline the line carries: the fused FP64 and plain FP64/FP32 paths on the same
+workload, the falling sphere with its immersed boundary, the phase-field solitary wave and the
+stretched Taylor-Green vortex, the line-axis tuner's timings and the serving mix (p50/p99 latency,
+aggregate MLUPS).  --precision runs the lock exchange and the solitary wave to their full step
+counts in FP32 and FP64 (and, with --precision-oracle, against the oracle), --sphere-precision
+the falling sphere.
+
 Only one GPU is used; --gpus above 1 exits with a message.
 """
 from __future__ import annotations
