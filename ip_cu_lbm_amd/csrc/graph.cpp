@@ -198,7 +198,7 @@ bool build_graph(const ipl_config& cfg, Graph& g, std::string& err) {
             }
     // which calc cells get a ghost, and their ghost index
     const DirTable& d = g.dirs;
-    std::vector<int64_t> nb((size_t)N * Q);
+    std::vector<int32_t> nb((size_t)N * Q);  // N < 2^30 (checked above)
     std::vector<int64_t> ghost_of(N, -1);
     int64_t G = 0;
     for (int64_t id = 0; id < N; ++id) {
@@ -211,7 +211,7 @@ bool build_graph(const ipl_config& cfg, Graph& g, std::string& err) {
                     if (cfg.periodic[a]) t[a] = (t[a] + g.n[a]) % g.n[a];
                     else ok = false;
                 }
-            const int64_t v = ok ? lnum[((size_t)t[0] * ny + t[1]) * nz + t[2]] : -1;
+            const int32_t v = ok ? (int32_t)lnum[((size_t)t[0] * ny + t[1]) * nz + t[2]] : -1;
             nb[(size_t)id * Q + k] = v;
             if (v < 0) missing = true;
         }
