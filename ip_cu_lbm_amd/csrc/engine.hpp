@@ -7,6 +7,7 @@
 #include <memory>
 #include <string>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "../../include/ipl.h"
@@ -51,13 +52,13 @@ public:
         } else {
             IPL_HIP_TRY(hipMalloc(p, bytes));
         }
-        ptrs_.push_back(*p);
+        ptrs_.push_back({*p, bound_});
         total_ += bytes;
         return Status::ok();
     }
     template <typename T> Status alloc_n(T** p, size_t n) { return alloc((void**)p, n * sizeof(T)); }
-    void release() {
-        for (void* p : ptrs_) (void)(bound_ ? hipFreeAsync(p, stream_) : hipFree(p));
+    void release() {  // each buffer the way it was allocated
+        for (const auto& [p, pooled] : ptrs_) (void)(pooled ? hipFreeAsync(p, stream_) : hipFree(p));
         ptrs_.clear();
         total_ = 0;
     }
@@ -78,7 +79,7 @@ private:
             }
         });
     }
-    std::vector<void*> ptrs_;
+    std::vector<std::pair<void*, bool>> ptrs_;  // (buffer, from the stream-ordered pool)
     size_t total_ = 0;
     hipStream_t stream_ = nullptr;
     bool bound_ = false;
