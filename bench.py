@@ -465,7 +465,7 @@ def main(argv=None):
             "plain_fp32_mlups": round(plain32["mlups"], 1),
             "speedup_fused_fp32_vs_plain_fp64": round(head["mlups"] / plain64["mlups"], 2),
             "fused_fp64_kernel_gbps": round(2 * 26 * 8 * cells / (fused64["profile"][0] * 1e-3) / 1e9, 1),
-            "fp16_storage": "disabled (see DESIGN.md)",
+            "fp16_storage": "disabled: the library refuses IPL_F16 until the fault of its first GPU run is explained",
             "falling_sphere": sph,
             "solitary_wave": sw["solitary_wave"],
             "taylor_green": sw["taylor_green"],

@@ -244,7 +244,7 @@ int ipl_ib_read_markers(ipl_ctx* ctx, int32_t which, double* out, size_t count);
  * steps of every member on its own stream (the fused path replays a captured HIP graph of its
  * step), so the jobs' work overlaps on the device, then waits for all of them.  Members keep
  * their own buffers: a job's results are the same alone and in any mix.  The reference runs one
- * case per process and has no counterpart; this is the north star's batching / serving layer. */
+ * case per process and has no counterpart; batches are what a job server steps. */
 typedef struct ipl_batch ipl_batch;
 int ipl_batch_create(ipl_batch** out);
 int ipl_batch_add(ipl_batch* batch, ipl_ctx* ctx);
