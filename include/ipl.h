@@ -260,6 +260,14 @@ const char* ipl_batch_last_error_message(ipl_batch* batch);
 int ipl_estimate_device_bytes(const ipl_config* cfg, int64_t* bytes);
 /* 0: launch every step directly; 1 (default): replay captured graphs where the path supports it. */
 int ipl_set_graph(ipl_ctx* ctx, int32_t on);
+/* Captures the step graphs of the fused path now (one per ping-pong buffer parity; recorded, not
+ * run) instead of at the first ipl_step that uses them, so that later steps, e.g. a timed region
+ * or a serving chunk, pay no capture.  A context captures each graph once unless its set-up
+ * changes (immersed body attached, open boundary, dt change).  No-op on the plain path.  The
+ * reference launches every kernel directly and has no counterpart (LockExchange.cu:220-278). */
+int ipl_prepare(ipl_ctx* ctx);
+/* Number of graph captures the context has made so far (diagnostics). */
+int ipl_graph_captures(ipl_ctx* ctx, int32_t* count);
 
 /* Open (inflow / outflow) boundary, replacing set_velocity1 + set_wall_boundary1
  * (Karman_vortex_2D/Karman_vortexxz.cu:38-57, called at :391-392 and :425-430): at the end of

@@ -83,6 +83,16 @@ class Simulation:
         """Replay captured HIP graphs of the step (default on) or launch every step directly."""
         self._check(self._lib.ipl_set_graph(self._h, 1 if on else 0))
 
+    def prepare(self):
+        """Captures the step graphs now (recorded, not run), so later steps pay no capture."""
+        self._check(self._lib.ipl_prepare(self._h))
+
+    def graph_captures(self) -> int:
+        """Number of graph captures this context has made."""
+        n = ctypes.c_int32()
+        self._check(self._lib.ipl_graph_captures(self._h, ctypes.byref(n)))
+        return n.value
+
     def set_open_boundary(self, cells, p, u, v, w):
         """Inflow / outflow cells (canonical indices) reset to feq(p, u, v, w) after every step
         (set_velocity1 + set_wall_boundary1); scalars or per-cell arrays; empty cells remove it."""

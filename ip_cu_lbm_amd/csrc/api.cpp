@@ -277,6 +277,18 @@ int ipl_set_graph(ipl_ctx* ctx, int32_t on) {
     return IPL_OK;
 }
 
+int ipl_prepare(ipl_ctx* ctx) {
+    if (!ctx) return IPL_ERR_ARGUMENT;
+    ipl::Status s = ctx->engine->prepare();
+    return s ? IPL_OK : fail(ctx, s);
+}
+
+int ipl_graph_captures(ipl_ctx* ctx, int32_t* count) {
+    if (!ctx || !count) return IPL_ERR_ARGUMENT;
+    *count = ctx->engine->graph_captures();
+    return IPL_OK;
+}
+
 int ipl_estimate_device_bytes(const ipl_config* cfg, int64_t* bytes) {
     if (!cfg || !bytes) return IPL_ERR_ARGUMENT;
     int64_t n = (int64_t)cfg->nx * cfg->ny * cfg->nz;

@@ -102,6 +102,10 @@ public:
     virtual Status post_step() { return Status::ok(); }
     // replay steps from a captured HIP graph where the engine supports it
     virtual void set_graph(bool) {}
+    // capture the step graphs ahead of the first step (no device work)
+    virtual Status prepare() { return Status::ok(); }
+    // number of graph captures so far
+    virtual int graph_captures() const { return 0; }
     virtual Status stage(int s) = 0;
     virtual Status read(int field, double* host, size_t count) = 0;
     virtual Status write(int field, const double* host, size_t count) = 0;

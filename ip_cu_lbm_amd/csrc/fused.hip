@@ -160,6 +160,7 @@ public:
         // populations
         if (!(r = mem_.alloc_n(&fa_, (size_t)Q * NC))) return r;
         if (!(r = mem_.alloc_n(&fb_, (size_t)Q * NC))) return r;
+        f_first_ = fa_;
         if (Qs) {
             if (!(r = mem_.alloc_n(&ga_, (size_t)Qs * NC))) return r;
             if (!(r = mem_.alloc_n(&gb_, (size_t)Qs * NC))) return r;
@@ -351,10 +352,12 @@ public:
         return Status::err(IPL_ERR_UNSUPPORTED, "lattice combination not built for the fused path");
     }
 
-    // Steps are replayed from a HIP graph of kGraphSteps steps, captured on first use: an even
-    // count, so the ping-pong buffers are back in their captured roles after every replay.  The
-    // step records kernel launches only (no memset or copy nodes).  Remaining steps are launched
-    // directly.
+    // Steps are replayed from HIP graphs of kGraphSteps steps: an even count, so the ping-pong
+    // buffers are back in their captured roles after every replay.  There is one graph per
+    // buffer parity (which of the two population sets is read first), each captured once on
+    // first use or by prepare(), so an odd number of direct steps never forces a new capture.
+    // The step records kernel launches only (no memset or copy nodes).  Remaining steps are
+    // launched directly.
     static constexpr int kGraphSteps = 20;
 
     Status step(int nsteps) override {
@@ -363,17 +366,19 @@ public:
         return r;
     }
 
+    // 0 while the buffers hold their set-up roles (fa_ is the first allocation), 1 when swapped
+    int parity() const { return fa_ == f_first_ ? 0 : 1; }
+
     Status launch_steps(int nsteps) override {
         int left = nsteps;
         if (use_graph_ && left >= kGraphSteps) {
-            // the graph fixes which buffer is read first: re-capture after an odd direct count
-            if (graph_exec_ && graph_src_ != fa_) drop_graph();
-            if (!graph_exec_) {
-                Status r = capture_graph();
+            const int par = parity();
+            if (!graph_exec_[par]) {
+                Status r = capture_graph(par);
                 if (!r) return r;
             }
             while (left >= kGraphSteps) {
-                IPL_HIP_TRY(hipGraphLaunch(graph_exec_, stream));
+                IPL_HIP_TRY(hipGraphLaunch(graph_exec_[par], stream));
                 left -= kGraphSteps;
             }
         }
@@ -383,27 +388,45 @@ public:
 
     Status post_step() override { return ib_.active() ? ib_.check() : Status::ok(); }
 
-    Status capture_graph() {
-        graph_src_ = fa_;
+    // Captures the graph whose first step reads the buffers of parity `par` (no device work: the
+    // launches are recorded, not run).  For the other parity the buffer roles are swapped for
+    // the capture and swapped back after it.
+    Status capture_graph(int par) {
+        const bool swap = par != parity();
+        auto swap_roles = [&] { std::swap(fa_, fb_); std::swap(ga_, gb_); std::swap(Fa_, Fb_); };
         IPL_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        if (swap) swap_roles();
         Status r = dispatch([&](auto q, auto qs) { return this->template step_q<decltype(q)::value, decltype(qs)::value>(kGraphSteps); });
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(stream, &g);
+        if (swap) swap_roles();
         if (!r) { if (g) (void)hipGraphDestroy(g); return r; }
         if (e != hipSuccess) return Status::err(IPL_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-        const hipError_t e2 = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+        const hipError_t e2 = hipGraphInstantiate(&graph_exec_[par], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
-        if (e2 != hipSuccess) { graph_exec_ = nullptr; return Status::err(IPL_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e2)); }
+        if (e2 != hipSuccess) { graph_exec_[par] = nullptr; return Status::err(IPL_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e2)); }
+        ++captures_;
         return Status::ok();
     }
+
+    // both graphs now, so that the next steps (a timed region, a serving chunk) pay no capture
+    Status prepare() override {
+        if (!use_graph_) return Status::ok();
+        for (int par = 0; par < 2; ++par)
+            if (!graph_exec_[par]) {
+                Status r = capture_graph(par);
+                if (!r) return r;
+            }
+        return Status::ok();
+    }
+
+    int graph_captures() const override { return captures_; }
 
     void set_graph(bool on) override {
         use_graph_ = on;
     }
 
-    ~FusedEngine() override {
-        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-    }
+    ~FusedEngine() override { drop_graph(); }
 
     Status ib_attach(const ipl_ib_config& cfg) override {
         if (std::is_same<ST, __half>::value) return Status::err(IPL_ERR_UNSUPPORTED, "the immersed boundary needs FP64 or FP32 storage");
@@ -650,8 +673,9 @@ private:
     double *vol_ = nullptr, *out_ = nullptr, *partial_ = nullptr;
     std::vector<int64_t> perm_;
     IbSolver ib_;
-    hipGraphExec_t graph_exec_ = nullptr;
-    const ST* graph_src_ = nullptr;
+    hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};  // per buffer parity
+    const ST* f_first_ = nullptr;                          // fa_ as set up: parity 0
+    int captures_ = 0;
 #ifdef IPL_CPU_EMULATION
     bool use_graph_ = false;
 #else
@@ -664,8 +688,10 @@ private:
     int64_t ob_n_ = 0, ob_cap_ = 0;
 
     void drop_graph() {
-        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-        graph_exec_ = nullptr;
+        for (auto& ge : graph_exec_) {
+            if (ge) (void)hipGraphExecDestroy(ge);
+            ge = nullptr;
+        }
     }
     size_t lds_bytes_ = 0;
 };

@@ -118,6 +118,11 @@ class Server:
     def _create(self, job: Job):
         t = time.perf_counter()
         sim = Simulation(job.case, self.library)
+        try:
+            sim.prepare()  # the job's step graphs, captured here rather than in the stepping loop
+        except Exception:
+            sim.close()
+            raise
         return sim, time.perf_counter() - t
 
     @staticmethod
@@ -135,6 +140,7 @@ class Server:
             if j.bytes > self.budget:
                 raise ValueError(f"job {j.jid} needs {j.bytes} B, more than the whole budget")
         active: List[Job] = []
+        captures: List[int] = []
         creating = {}
         releasing = {}
         used = 0
@@ -180,6 +186,7 @@ class Server:
                         j.done_steps += k
                         if j.done_steps >= j.steps:
                             j.finish = now
+                            captures.append(j.sim.graph_captures())
                             if on_done:
                                 on_done(j)
                             batch.remove(j.sim)
@@ -211,4 +218,4 @@ class Server:
                 "setup_s": float(create), "close_s": float(close),
                 "aggregate_mlups": updates / makespan / 1e6, "busy_mlups": updates / busy / 1e6,
                 "cell_updates": updates, "budget_gb": self.budget / 1e9, "chunk": self.chunk,
-                "setup_workers": self.workers}
+                "setup_workers": self.workers, "max_graph_captures_per_job": int(max(captures))}

@@ -41,6 +41,7 @@ def _time(case: "cases.Case", steps: int, warmup: int, library: Optional[str]) -
         if sim.path != _abi.PATH_FUSED:
             return None
         sim.step(warmup)
+        sim.prepare()
         return sim.step_timed(steps) / steps
 
 

@@ -33,3 +33,19 @@ def test_struct_layout_matches_c(tmp_path, struct, cls):
     assert c["size"] == ctypes.sizeof(cls)
     for n in names:
         assert c[n] == getattr(cls, n).offset, n
+
+
+def test_library_exports_every_declared_entry_point():
+    """Every function include/ipl.h declares is exported by the built libipl.so (the library is
+    loaded, not called: no device work)."""
+    import re
+    header = open(os.path.join(ROOT, "include", "ipl.h")).read()
+    header = re.sub(r"/\*.*?\*/", "", header, flags=re.S)
+    declared = set(re.findall(r"\b(ipl_[a-z0-9_]+)\s*\(", header))
+    assert len(declared) > 30
+    lib = _abi.load_library()
+    missing = [name for name in sorted(declared) if not hasattr(lib, name)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", lib._name], check=True, capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if l.strip()}
+    assert declared <= exported, sorted(declared - exported)

@@ -37,14 +37,33 @@ def _small_cases():
 @pytest.mark.parametrize("name", ["le", "tg", "pf", "cyl", "sph"])
 def test_graph_replay_matches_direct_launches(name):
     """Graph replay (20-step graphs) vs direct launches, bit for bit, after each of several
-    replays; an odd direct count in between swaps the buffer roles and forces a re-capture."""
+    replays; an odd direct count in between swaps the buffer roles, which switches to the other
+    parity's graph: each of the two graphs is captured once."""
     case, fields = _small_cases()[name]
     with Simulation(case) as g, Simulation(case) as d:
         d.set_graph(False)
-        for n in (20, 20, 40, 7, 20, 60, 3, 21):
+        for n in (20, 20, 40, 7, 20, 60, 3, 21, 41, 20):
             g.step(n)
             d.step(n)
             _same(_state(g, fields), _state(d, fields), (name, n))
+        assert g.graph_captures() == 2
+        assert d.graph_captures() == 0
+
+
+def test_prepare_captures_both_graphs_once():
+    """ipl_prepare captures the graphs of both buffer parities before any step; later steps of
+    any count capture nothing more, and the results equal direct launches bit for bit."""
+    case, fields = _small_cases()["le"]
+    with Simulation(case) as g, Simulation(case) as d:
+        d.set_graph(False)
+        g.prepare()
+        assert g.graph_captures() == 2
+        g.prepare()
+        for n in (5, 20, 33, 40):
+            g.step(n)
+            d.step(n)
+        assert g.graph_captures() == 2
+        _same(_state(g, fields), _state(d, fields), "prepare")
         if case.body is not None:
             bg, bd = g.body_state(), d.body_state()
             for k in bg:

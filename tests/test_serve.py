@@ -68,6 +68,14 @@ def test_server_admission_and_release_accounting(monkeypatch):
                 state["created"] += 1
             self.open = True
             self.steps = 0
+            self.prepared = 0
+
+        def prepare(self):
+            assert self.open
+            self.prepared += 1
+
+        def graph_captures(self):
+            return 2 * self.prepared
 
         def close(self):
             time.sleep(0.003)
@@ -115,3 +123,4 @@ def test_server_admission_and_release_accounting(monkeypatch):
     assert state["peak"] <= 3 * per_job
     assert all(j.done_steps == j.steps and j.sim is None and j.finish >= j.start >= j.arrival - 1e-9 for j in jobs)
     assert res["aggregate_mlups"] > 0 and res["p99_s"] >= res["p50_s"] > 0
+    assert res["max_graph_captures_per_job"] == 2
