@@ -745,10 +745,15 @@ std::unique_ptr<Engine> make_fused_engine(const Graph& g, const ipl_config& cfg,
     if (cfg.storage == IPL_F64) return make((FusedEngine<double, double>*)nullptr);
     if (cfg.storage == IPL_F32) return make((FusedEngine<float, float>*)nullptr);
     if (cfg.storage == IPL_F16) {
+#ifdef IPL_CPU_EMULATION
+        // the CPU test build carries the FP16 instantiations, to examine that path on the host
+        return make((FusedEngine<__half, float>*)nullptr);
+#else
         // FP16 storage is switched off: its first GPU run (D3Q19 + D3Q7 lock exchange) ended in
         // a device memory fault whose cause is not yet shown; the CPU build of this code is clean.
         status = Status::err(IPL_ERR_UNSUPPORTED, "FP16 storage is disabled pending a GPU fault analysis");
         return nullptr;
+#endif
     }
     status = Status::err(IPL_ERR_ARGUMENT, "storage type");
     return nullptr;

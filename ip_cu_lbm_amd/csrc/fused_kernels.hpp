@@ -116,8 +116,12 @@ __device__ inline int wrap_or_clamp(int x, int n, int periodic) {
 // the form of the global instructions' scalar-base addressing (the host keeps every population and
 // field array below 4 GiB, so idx counts rows as well: idx = row * ncell + cell)
 template <typename T> __device__ inline T* elem(T* row, uint32_t idx) {
-    return reinterpret_cast<T*>(reinterpret_cast<typename std::conditional<std::is_const<T>::value, const char, char>::type*>(row) +
+    T* p = reinterpret_cast<T*>(reinterpret_cast<typename std::conditional<std::is_const<T>::value, const char, char>::type*>(row) +
                                 (uint64_t)(idx * (uint32_t)sizeof(T)));
+#ifdef IPL_CPU_EMULATION
+    ipl_emu::check_access(row, p, sizeof(T));  // the CPU test build checks every access
+#endif
+    return p;
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  Line L of the

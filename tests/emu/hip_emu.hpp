@@ -2,8 +2,8 @@
 //
 // Force-included (-include) when the library sources are compiled with g++ and
 // -DIPL_CPU_EMULATION.  Kernels become host functions; IPL_LAUNCH runs every work-item of
-// every workgroup in order, IPL_LAUNCH_SYNC runs a workgroup's work-items as threads that
-// meet at std::barrier for __syncthreads.  Device memory is host memory, so an out-of-bounds
+// every workgroup in order, IPL_LAUNCH_SYNC runs a workgroup's work-items as fibers that
+// yield at __syncthreads.  Device memory is host memory, so an out-of-bounds
 // index is caught by AddressSanitizer on the CPU instead of faulting a GPU.
 #pragma once
 #include <algorithm>
@@ -38,8 +38,14 @@ extern thread_local ipl_emu_idx3 threadIdx, blockIdx, blockDim, gridDim;
 extern thread_local std::barrier<>* ipl_emu_barrier;
 extern thread_local unsigned char* ipl_emu_lds;
 
+namespace ipl_emu {
+bool in_fiber();
+void fiber_barrier();
+}  // namespace ipl_emu
+// a barrier of the workgroup: the work-item's fiber yields to the scheduler (a no-op outside a
+// synchronising launch)
 inline void __syncthreads() {
-    if (ipl_emu_barrier) ipl_emu_barrier->arrive_and_wait();
+    if (ipl_emu::in_fiber()) ipl_emu::fiber_barrier();
 }
 
 template <typename T> inline T atomicAdd(T* p, T v) {
@@ -91,6 +97,15 @@ inline __half __float2half_rn(float x) {
     h.bits = (uint16_t)(s | (he << 10) | hm); return h;
 }
 
+// ---- device allocations: every access through the kernels' element helper is checked against
+// the allocation that holds its row base (an access that lands in another live allocation is
+// not seen by AddressSanitizer, whose red zones only border each block)
+namespace ipl_emu {
+void register_alloc(const void* p, size_t n);
+void unregister_alloc(const void* p);
+void check_access(const void* base, const void* addr, size_t n);
+}  // namespace ipl_emu
+
 // ---- runtime API subset ------------------------------------------------------------------
 typedef int hipError_t;
 enum { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorInvalidValue = 1 };
@@ -99,9 +114,18 @@ typedef void* hipEvent_t;
 enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
 #define hipStreamNonBlocking 1
 inline const char* hipGetErrorString(hipError_t e) { return e == hipSuccess ? "success" : "emulated error"; }
-inline hipError_t hipMalloc(void** p, size_t n) { *p = std::malloc(n); return *p ? hipSuccess : hipErrorOutOfMemory; }
+inline hipError_t hipMalloc(void** p, size_t n) {
+    *p = std::malloc(n);
+    if (!*p) return hipErrorOutOfMemory;
+    ipl_emu::register_alloc(*p, n);
+    return hipSuccess;
+}
 template <typename T> inline hipError_t hipMalloc(T** p, size_t n) { return hipMalloc((void**)p, n); }
-inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+inline hipError_t hipFree(void* p) {
+    ipl_emu::unregister_alloc(p);
+    std::free(p);
+    return hipSuccess;
+}
 inline hipError_t hipMallocAsync(void** p, size_t n, hipStream_t) { return hipMalloc(p, n); }
 inline hipError_t hipFreeAsync(void* p, hipStream_t) { return hipFree(p); }
 typedef void* hipMemPool_t;
