@@ -55,6 +55,13 @@ def test_ib_stress_method_fp64(oracle_lib, q):
     _compare(_sphere(q=q), 40, oracle_lib, 1e-12)
 
 
+def test_ib_stress_method_srt_fp64(oracle_lib):
+    """The falling sphere with SRT collision, as the reference's sphere driver runs it
+    (SphereFall.cu:356, col_f_SRT collision.cu:151-165)."""
+    c = _sphere()
+    _compare(c.replace(collision=_abi.COLL_SRT, name=c.name + "_srt"), 40, oracle_lib, 1e-12)
+
+
 def test_ib_direct_forcing_fp64(oracle_lib):
     _compare(_sphere(method=_abi.IB_DIRECT), 40, oracle_lib, 1e-12)
 

@@ -87,6 +87,49 @@ def test_solitary_wave_phase_field_fp64(oracle_lib):
     _compare(case, 150, ("p", "vel", "phi"), 1e-12, oracle_lib)
 
 
+def test_srt_collision_plain_and_fused_fp64(oracle_lib):
+    """col_f_SRT (collision.cu:151-165), the collision of the reference's sphere and ellipse drivers
+    (SphereFall.cu:356), on both paths: a stretched Taylor-Green channel and the lock exchange."""
+    for path in (PATH_PLAIN, _abi.PATH_FUSED):
+        tg = cases.lattice_taylor_green(nx=48, nz=24, path=path).replace(collision=_abi.COLL_SRT)
+        _compare(tg, 200, FIELDS_FLOW, 1e-12, oracle_lib)
+        le = cases.lock_exchange(nx=24, ny=6, nz=12, path=path).replace(collision=_abi.COLL_SRT)
+        _compare(le, 100, ("p", "vel", "S"), 1e-12, oracle_lib)
+
+
+def _sloped_bed(case, h0, slope):
+    """Cells below a sloping bed z < h0 + slope x are solid (lnum = -1), the reference's slope
+    geometry (Boegman2005run18IB.cu:119-124)."""
+    nx, ny, nz = case.n
+    x = case.pos[0][:, None, None]
+    z = case.pos[2][None, None, :]
+    solid = np.broadcast_to(z < h0 + slope * x, (nx, ny, nz)).astype(np.uint8)
+    return case.replace(solid=solid.ravel())
+
+
+@pytest.mark.parametrize("bc", [_abi.BC_BOUNCE_BACK, _abi.BC_SLIP])
+def test_cell_mask_sloped_bed_2d_fp64(oracle_lib, bc):
+    """A D2Q9 channel stretched in x over a sloping solid bed (ghosts behind every bed cell,
+    set_neibghor_wall), plain path, against the oracle."""
+    case = cases.lattice_taylor_green(nx=64, nz=32, path=PATH_PLAIN).replace(periodic=(0, 0, 0), wall_bc=bc)
+    case = _sloped_bed(case, 2.0, 0.12)
+    X, _, Z = case.coords()
+    case.init = {"u": 0.01 * np.sin(2 * np.pi * Z / 32.0), "w": 0.005 * np.cos(2 * np.pi * X / 64.0),
+                 "p": cases.uniform(11, case.n_cells, -1e-3, 1e-3)}
+    assert case.n_cells < 64 * 32
+    _compare(case, 200, FIELDS_FLOW, 1e-12, oracle_lib)
+
+
+def test_cell_mask_lock_exchange_over_a_slope_fp64(oracle_lib):
+    """The D3Q19 + D3Q7 lock exchange over a sloping bed (a Boegman-type slope), plain path."""
+    case = cases.lock_exchange(nx=32, ny=6, nz=16, path=PATH_PLAIN)
+    case = _sloped_bed(case, 1.5, 0.25)
+    X, Y, Z = case.coords()
+    case.init = {"S": 0.5 * (1.0 - np.tanh((X - 16.0) / 3.0)), "p": cases.uniform(12, case.n_cells, -1e-4, 1e-4)}
+    assert case.n_cells < 32 * 6 * 16
+    _compare(case, 120, ("p", "vel", "S"), 1e-12, oracle_lib)
+
+
 def test_taylor_green_fp32_storage(oracle_lib):
     case = cases.lattice_taylor_green(nx=64, nz=32, storage=F32, path=PATH_PLAIN)
     _compare(case, 200, ("vel",), 1e-5, oracle_lib)
