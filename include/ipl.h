@@ -158,6 +158,17 @@ int ipl_write(ipl_ctx* ctx, int32_t field, const double* host, size_t count);
  * monitor printed by output() (basic.cu:9-15). */
 int ipl_global_sums(ipl_ctx* ctx, double out[3]);
 
+/* The sums of ipl_global_sums without a device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The partial sums go to a file-scope device array of
+ * 3 x 1024 + 3 doubles (24,600 bytes) that every GPU the library runs on holds until the process
+ * ends; the reduction grid is fixed, so the sums of any context fit it, and a reduction that would
+ * need more is refused with IPL_ERR_ARGUMENT.  The context's device must be current.  Every call
+ * on one GPU shares that GPU's array (its contents are the last call's): calls that overlap on
+ * one GPU, from different threads or contexts, race.  Same kernels and order as ipl_global_sums,
+ * so the two return the same bits. */
+int ipl_global_sums_noalloc(ipl_ctx* ctx, double out[3]);
+
 /* Number of calc cells, ghost cells and population storage bytes of a context. */
 int ipl_info(ipl_ctx* ctx, int64_t* n_calc, int64_t* n_ghost, int64_t* device_bytes, int32_t* path);
 

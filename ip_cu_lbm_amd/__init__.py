@@ -158,6 +158,12 @@ class Simulation:
         self._check(self._lib.ipl_global_sums(self._h, _dp(out)))
         return out
 
+    def global_sums_noalloc(self) -> np.ndarray:
+        """The same sums without a device allocation (the library's per-GPU scratch array)."""
+        out = np.zeros(3)
+        self._check(self._lib.ipl_global_sums_noalloc(self._h, _dp(out)))
+        return out
+
     # ---- immersed boundary ------------------------------------------------------------------
     def ib_attach(self, body: "cases.Body"):
         cfg, keep = body.to_c()

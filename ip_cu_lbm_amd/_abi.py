@@ -107,6 +107,8 @@ def load_library(path: str = None):
     lib.ipl_write.restype = c_int32
     lib.ipl_global_sums.argtypes = [vp, _DP]
     lib.ipl_global_sums.restype = c_int32
+    lib.ipl_global_sums_noalloc.argtypes = [vp, _DP]
+    lib.ipl_global_sums_noalloc.restype = c_int32
     lib.ipl_info.argtypes = [vp, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64), POINTER(c_int32)]
     lib.ipl_info.restype = c_int32
     lib.ipl_destroy.argtypes = [vp]

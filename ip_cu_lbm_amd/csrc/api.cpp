@@ -167,7 +167,22 @@ int ipl_write(ipl_ctx* ctx, int32_t field, const double* host, size_t count) {
 
 int ipl_global_sums(ipl_ctx* ctx, double out[3]) {
     if (!ctx || !out) return IPL_ERR_ARGUMENT;
-    ipl::Status s = ctx->engine->global_sums(out);
+    ipl::Status s = ctx->engine->global_sums(out, nullptr);
+    return s ? IPL_OK : fail(ctx, s);
+}
+
+int ipl_global_sums_noalloc(ipl_ctx* ctx, double out[3]) {
+    if (!ctx || !out) return IPL_ERR_ARGUMENT;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != ctx->device)
+        return fail(ctx, ipl::Status::err(IPL_ERR_ARGUMENT, "the context's device must be current"));
+    double* scratch = nullptr;
+    size_t capacity = 0;
+    ipl::Status s = ipl::noalloc_scratch(&scratch, &capacity);
+    if (!s) return fail(ctx, s);
+    if (ipl::weighted_sums_scratch_doubles() > capacity)
+        return fail(ctx, ipl::Status::err(IPL_ERR_ARGUMENT, "the reduction needs more scratch than the library holds"));
+    s = ctx->engine->global_sums(out, scratch);
     return s ? IPL_OK : fail(ctx, s);
 }
 

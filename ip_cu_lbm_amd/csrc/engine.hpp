@@ -109,7 +109,9 @@ public:
     virtual Status stage(int s) = 0;
     virtual Status read(int field, double* host, size_t count) = 0;
     virtual Status write(int field, const double* host, size_t count) = 0;
-    virtual Status global_sums(double out[3]) = 0;
+    // scratch: device buffer of at least weighted_sums_scratch_doubles() doubles for the partial
+    // sums, or null for the engine's own
+    virtual Status global_sums(double out[3], double* scratch) = 0;
     virtual Status check_finite(bool* finite) = 0;
     // Runs nsteps steps timing every launch with HIP events on the engine's stream:
     // out[0] = mean duration of the dominant kernel's launch (ms), out[1] = mean per-step time of
@@ -141,10 +143,14 @@ std::unique_ptr<Engine> make_fused_engine(const Graph& g, const ipl_config& cfg,
 // Whether the fused path can run this configuration; fills why when it cannot.
 bool fused_supported(const Graph& g, const ipl_config& cfg, std::string& why);
 
-// Deterministic FP64 reductions on the device (fixed block partition, fixed order).
+// Deterministic FP64 reductions on the device (fixed block partition, fixed order).  scratch:
+// weighted_sums_scratch_doubles() doubles for the partial sums, or null to allocate them.
 template <typename T>
 Status device_weighted_sums(const T* const* fields, int nfields, const T* wx, const T* wy, const T* wz,
-                            int64_t n, double inv_vol, double* out, hipStream_t st);
+                            int64_t n, double inv_vol, double* out, hipStream_t st, double* scratch);
+size_t weighted_sums_scratch_doubles();
+// The device address and capacity (doubles) of the current GPU's file-scope scratch array.
+Status noalloc_scratch(double** out, size_t* capacity);
 template <typename T>
 Status device_all_finite(const T* const* fields, int nfields, int64_t n, bool* finite, hipStream_t st);
 

@@ -629,16 +629,17 @@ public:
         return Status::err(IPL_ERR_UNSUPPORTED, "the fused path writes populations only");
     }
 
-    Status global_sums(double out3[3]) override {
+    Status global_sums(double out3[3], double* scratch) override {
         Status r = readout();
         if (!r) return r;
+        double* partial = scratch ? scratch : partial_;  // 1024 partials, then the result
         // fields p (slot 0), S (slot 5), phi (slot 6) are not contiguous: sum each separately
         double res[3];
         const int slots[3] = {0, 5, 6};
         for (int q = 0; q < 3; ++q) {
-            IPL_LAUNCH_SYNC(k_wsum, 1024, 256, 0, stream, out_ + (size_t)slots[q] * geo_.ncell, geo_.ncell, vol_, partial_);
-            IPL_LAUNCH_SYNC(k_wsum_final, 1, 256, 0, stream, partial_, partial_ + 3 * 1024);
-            IPL_HIP_TRY(hipMemcpyAsync(&res[q], partial_ + 3 * 1024, sizeof(double), hipMemcpyDeviceToHost, stream));
+            IPL_LAUNCH_SYNC(k_wsum, 1024, 256, 0, stream, out_ + (size_t)slots[q] * geo_.ncell, geo_.ncell, vol_, partial);
+            IPL_LAUNCH_SYNC(k_wsum_final, 1, 256, 0, stream, partial, partial + 3 * 1024);
+            IPL_HIP_TRY(hipMemcpyAsync(&res[q], partial + 3 * 1024, sizeof(double), hipMemcpyDeviceToHost, stream));
             IPL_HIP_TRY(hipStreamSynchronize(stream));
         }
         for (int q = 0; q < 3; ++q) out3[q] = res[q];

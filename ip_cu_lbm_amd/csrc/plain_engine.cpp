@@ -264,10 +264,11 @@ public:
         return Status::ok();
     }
 
-    Status global_sums(double out[3]) override {
+    Status global_sums(double out[3], double* scratch) override {
         const T* f[3] = {v_.p, v_.S, v_.phi};
         const double dz = g_.dz;
-        return device_weighted_sums<T>(f, 3, v_.del[0], v_.del[1], v_.del[2], g_.N, 1.0 / (dz * dz * dz), out, stream);
+        return device_weighted_sums<T>(f, 3, v_.del[0], v_.del[1], v_.del[2], g_.N, 1.0 / (dz * dz * dz), out, stream,
+                                       scratch);
     }
 
     Status check_finite(bool* finite) override {

@@ -4,6 +4,8 @@
 // drivers do at save intervals (LockExchange.cu:259-261).
 #include "hip_compat.hpp"
 
+#include <mutex>
+
 #include "engine.hpp"
 
 namespace ipl {
@@ -12,6 +14,12 @@ namespace {
 
 constexpr int kRedBlock = 256;
 constexpr int kRedGrid = 1024;
+
+// Scratch of the allocation-free sums (ipl_global_sums_noalloc): the partial sums of the fixed
+// reduction grid for up to three fields, then the three results.  Every GPU the library runs on
+// holds one copy until the process ends; calls that overlap on one GPU share it.
+constexpr size_t kNoAllocScratchDoubles = 3 * kRedGrid + 3;
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
 
 template <typename T>
 __global__ void k_partial_wsum(const T* a0, const T* a1, const T* a2, const T* wx, const T* wy, const T* wz,
@@ -67,17 +75,38 @@ __global__ void k_zero_int(int* p) { *p = 0; }
 
 }  // namespace
 
+Status noalloc_scratch(double** out, size_t* capacity) {
+    // the device address of this GPU's copy of `scratch`, looked up once per device with that
+    // device current (the array's name is a host-side stand-in, not a device address)
+    static std::mutex m;
+    static double* ptr[64] = {};
+    int dev = 0;
+    IPL_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return Status::err(IPL_ERR_HIP, "device index beyond 63");
+    std::lock_guard<std::mutex> lock(m);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        IPL_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptr[dev] = static_cast<double*>(p);
+    }
+    *out = ptr[dev];
+    *capacity = kNoAllocScratchDoubles;
+    return Status::ok();
+}
+
+size_t weighted_sums_scratch_doubles() { return 3 * (size_t)kRedGrid + 3; }
+
 template <typename T>
 Status device_weighted_sums(const T* const* fields, int nfields, const T* wx, const T* wy, const T* wz, int64_t n,
-                            double inv_vol, double* out, hipStream_t st) {
-    double* buf = nullptr;
-    IPL_HIP_TRY(hipMallocAsync((void**)&buf, (3 * kRedGrid + 3) * sizeof(double), st));
+                            double inv_vol, double* out, hipStream_t st, double* scratch_buf) {
+    double* buf = scratch_buf;  // the caller's scratch (every slot is written before it is read), or our own
+    if (!buf) IPL_HIP_TRY(hipMallocAsync((void**)&buf, (3 * kRedGrid + 3) * sizeof(double), st));
     IPL_LAUNCH_SYNC((k_partial_wsum<T>), kRedGrid, kRedBlock, 0, st, nfields > 0 ? fields[0] : nullptr, nfields > 1 ? fields[1] : nullptr,
                                                         nfields > 2 ? fields[2] : nullptr, wx, wy, wz, n, buf);
     IPL_LAUNCH_SYNC(k_final_sum, 1, kRedBlock, 0, st, buf, 3, inv_vol, buf + 3 * kRedGrid);
     double h[3];
     IPL_HIP_TRY(hipMemcpyAsync(h, buf + 3 * kRedGrid, sizeof(h), hipMemcpyDeviceToHost, st));
-    IPL_HIP_TRY(hipFreeAsync(buf, st));
+    if (!scratch_buf) IPL_HIP_TRY(hipFreeAsync(buf, st));
     IPL_HIP_TRY(hipStreamSynchronize(st));
     for (int q = 0; q < nfields && q < 3; ++q) out[q] = h[q];
     return Status::ok();
@@ -100,9 +129,9 @@ Status device_all_finite(const T* const* fields, int nfields, int64_t n, bool* f
 }
 
 template Status device_weighted_sums<double>(const double* const*, int, const double*, const double*, const double*,
-                                             int64_t, double, double*, hipStream_t);
+                                             int64_t, double, double*, hipStream_t, double*);
 template Status device_weighted_sums<float>(const float* const*, int, const float*, const float*, const float*,
-                                            int64_t, double, double*, hipStream_t);
+                                            int64_t, double, double*, hipStream_t, double*);
 template Status device_all_finite<double>(const double* const*, int, int64_t, bool*, hipStream_t);
 template Status device_all_finite<float>(const float* const*, int, int64_t, bool*, hipStream_t);
 

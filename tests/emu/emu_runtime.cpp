@@ -102,6 +102,9 @@ void resume(unsigned t) {
 }  // namespace
 
 bool in_fiber() { return g_sched.active; }
+
+std::atomic<long long> g_allocs{0};
+void count_alloc() { ++g_allocs; }
 void fiber_barrier() { to_main(); }
 
 void register_alloc(const void* p, size_t n) {
@@ -194,3 +197,6 @@ void launch_sync(unsigned grid, unsigned block, size_t lds, const std::function<
 }
 
 }  // namespace ipl_emu
+
+// number of device allocations made so far (hipMalloc and hipMallocAsync)
+extern "C" long long ipl_emu_alloc_count() { return ipl_emu::g_allocs.load(); }

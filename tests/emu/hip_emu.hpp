@@ -24,6 +24,7 @@
 #define __constant__
 #define __shared__ static
 #define __launch_bounds__(...)
+#define __align__(n) alignas(n)
 
 using std::isfinite;
 using std::sqrt;
@@ -114,7 +115,9 @@ typedef void* hipEvent_t;
 enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
 #define hipStreamNonBlocking 1
 inline const char* hipGetErrorString(hipError_t e) { return e == hipSuccess ? "success" : "emulated error"; }
+namespace ipl_emu { void count_alloc(); }
 inline hipError_t hipMalloc(void** p, size_t n) {
+    ipl_emu::count_alloc();
     *p = std::malloc(n);
     if (!*p) return hipErrorOutOfMemory;
     ipl_emu::register_alloc(*p, n);
@@ -145,6 +148,10 @@ inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms = 0; return hipSuccess; }
 inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
+// a file-scope __device__ array is a host array here: its address is the symbol's
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return hipSuccess; }
+extern "C" long long ipl_emu_alloc_count();
 // graphs are not emulated: capture reports an error, so the engines launch every step directly
 typedef void* hipGraph_t;
 typedef void* hipGraphExec_t;
