@@ -15,6 +15,20 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 _LIB = None
+_MUTANTS = {}
+
+
+def build_mutant(n: int) -> str:
+    """The oracle built with planted mistake number n (ORC_MUTANT, see oracle.cpp): test
+    infrastructure for showing that a pin fails on that mistake."""
+    out = os.path.join(_HERE, "_build", f"liboracle_mutant{n}.so")
+    src = os.path.join(_HERE, "oracle.cpp")
+    if not os.path.exists(out) or os.path.getmtime(out) < os.path.getmtime(src):
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        subprocess.run(["g++", "-O2", "-fopenmp", "-fPIC", "-std=c++17", "-shared", f"-DORC_MUTANT={int(n)}", src,
+                        "-o", out + ".tmp"], check=True)
+        os.replace(out + ".tmp", out)
+    return out
 
 
 def build(force: bool = False) -> str:
@@ -24,12 +38,15 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
-def load():
+def load(path: str = None):
+    """The oracle library (the normal build, or `path`, e.g. a mutant of build_mutant)."""
     global _LIB
-    if _LIB is None:
-        if not os.path.exists(LIB_PATH):
+    if path is not None and path in _MUTANTS:
+        return _MUTANTS[path]
+    if path is not None or _LIB is None:
+        if path is None and not os.path.exists(LIB_PATH):
             build()
-        lib = ctypes.CDLL(LIB_PATH)
+        lib = ctypes.CDLL(path or LIB_PATH)
         DP = POINTER(c_double)
         lib.orc_create.argtypes = [c_void_p, c_void_p, c_char_p, c_int]
         lib.orc_create.restype = c_void_p
@@ -61,6 +78,9 @@ def load():
         lib.orc_les.argtypes = [c_void_p]
         lib.orc_spalding_utau.argtypes = [c_double, c_double, c_double]
         lib.orc_spalding_utau.restype = c_double
+        if path is not None:
+            _MUTANTS[path] = lib
+            return lib
         _LIB = lib
     return _LIB
 
@@ -72,8 +92,8 @@ def _dp(a):
 class Oracle:
     """FP64 reference-algorithm run of a Case (same inputs as the HIP library)."""
 
-    def __init__(self, case):
-        lib = load()
+    def __init__(self, case, lib_path: str = None):
+        lib = load(lib_path)
         cfg, ini, keep = case.to_c()
         err = ctypes.create_string_buffer(256)
         h = lib.orc_create(ctypes.byref(cfg), ctypes.byref(ini), err, 256)

@@ -19,6 +19,10 @@
 //    parameters instead of the literals 9.81 / 0.824493.
 //
 // Parity status per function is listed at the end of this file (PARITY block).
+//
+// ORC_MUTANT (a compile-time number, 0 in every normal build) plants one known mistake, so that
+// tests/test_oracle_pins.py can show that each closed-form pin fails on a plausible error in the
+// function it pins.  The mutants are listed where they act; none is in the normal build.
 
 #include "../include/ipl.h"
 #include <algorithm>
@@ -30,6 +34,11 @@
 #include <vector>
 
 namespace {
+
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+constexpr int kMutant = ORC_MUTANT;
 
 // ------------------------------------------------------------------------------------------
 // Lattice constants: Items::setc (basic.cu:70-98).  D2Q9 lives in the x-z plane.  D3Q19 and
@@ -560,7 +569,11 @@ void force(Oracle& o) {
         // (collision.cu:70-71), also where 2/4 are the y directions of a 3D lattice
         const double lap = (ph[nb[1]] - 2 * ph[id] + ph[nb[3]]) / (o.del[0][id] * o.del[0][id]) +
                            (ph[nb[2]] - 2 * ph[id] + ph[nb[4]]) / (o.dz * o.dz);
-        const double st = 4 * beta * ph[id] * (ph[id] - 1.0) * (ph[id] - 0.5) - kap * lap;
+        double st = 4 * beta * ph[id] * (ph[id] - 1.0) * (ph[id] - 0.5) - kap * lap;
+        if (kMutant == 2) st = 4 * beta * ph[id] * (ph[id] - 1.0) * (ph[id] - 0.5) + kap * lap;  // mutant 2: sign of kappa
+        if (kMutant == 12)  // mutant 12: the x term of the Laplacian over dz^2
+            st = 4 * beta * ph[id] * (ph[id] - 1.0) * (ph[id] - 0.5) -
+                 kap * ((ph[nb[1]] - 2 * ph[id] + ph[nb[3]]) / (o.dz * o.dz) + (ph[nb[2]] - 2 * ph[id] + ph[nb[4]]) / (o.dz * o.dz));
         o.Fx[id] += st * nx_; o.Fy[id] += st * ny_; o.Fz[id] += st * nz_;
         if (o.cfg.boussinesq) o.Fz[id] += -o.S[id] * o.cfg.beta_s * o.cfg.gravity;
         else o.Fz[id] += -o.rho[id] * o.cfg.gravity;
@@ -568,13 +581,14 @@ void force(Oracle& o) {
         ny_ = 0;
         if (o.dim == 3) ny_ = (rh[nb[2]] - rh[nb[4]]) / (dpos(o, 1, nb[2], nb[4]) + eps);
         nz_ = (rh[nb[up]] - rh[nb[down]]) / (dpos(o, 2, nb[up], nb[down]) + eps);
-        o.Fx[id] += -o.p[id] * c2 / 3.0 * nx_;
-        o.Fy[id] += -o.p[id] * c2 / 3.0 * ny_;
-        o.Fz[id] += -o.p[id] * c2 / 3.0 * nz_;
+        const double psg = kMutant == 1 ? 1.0 : -1.0;  // mutant 1: sign of the pressure-gradient term
+        o.Fx[id] += psg * o.p[id] * c2 / 3.0 * nx_;
+        o.Fy[id] += psg * o.p[id] * c2 / 3.0 * ny_;
+        o.Fz[id] += psg * o.p[id] * c2 / 3.0 * nz_;
+        const double den = kMutant == 3 ? o.rho[id] * c2 : o.rho[id] * c2 / 3.0;  // mutant 3: 1/3 dropped
         for (int k = 0; k < Q; ++k)
             o.fl.Fk[(size_t)id * Q + k] = o.L.w[k] * o.dt *
-                (o.L.cx[k] * o.c * o.Fx[id] + o.L.cy[k] * o.c * o.Fy[id] + o.L.cz[k] * o.c * o.Fz[id]) /
-                (o.rho[id] * c2 / 3.0);
+                (o.L.cx[k] * o.c * o.Fx[id] + o.L.cy[k] * o.c * o.Fy[id] + o.L.cz[k] * o.c * o.Fz[id]) / den;
     }
 }
 
@@ -636,14 +650,15 @@ void col_g_reg(Oracle& o) {
     PopSet& P = o.sc; const int Q = P.Q; const Lattice& L = P.L;
     const double c2 = o.c * o.c;
     const bool linear = Q == 7;
-    const double cs2inv = linear ? 4.0 : 3.0;
+    const double cs2inv = linear && kMutant != 7 ? 4.0 : 3.0;  // mutant 7: 1/cs^2 = 3 on D3Q7
     #pragma omp parallel for schedule(static)
     for (int id = 0; id < o.N; ++id) {
         double* g = &P.f[(size_t)id * Q]; double* geq = &P.feq[(size_t)id * Q]; double* gt = &P.ftmp[(size_t)id * Q];
         double gkx = 0, gky = 0, gkz = 0;
         const double u2 = o.u[id] * o.u[id] + o.v[id] * o.v[id] + o.w[id] * o.w[id];
         for (int k = 0; k < Q; ++k) {
-            const double t = (L.cx[k] * o.c * o.u[id] + L.cy[k] * o.c * o.v[id] + L.cz[k] * o.c * o.w[id]) / c2;
+            double t = (L.cx[k] * o.c * o.u[id] + L.cy[k] * o.c * o.v[id] + L.cz[k] * o.c * o.w[id]) / c2;
+            if (kMutant == 6) t = 0;  // mutant 6: the equilibrium without the velocity
             double e = 1.0 + cs2inv * t;
             if (!linear) e += 4.5 * t * t - 1.5 * u2 / c2;
             geq[k] = L.w[k] * o.S[id] * e;
@@ -684,9 +699,11 @@ void col_PF(Oracle& o) {
             const double cxk = L.cx[k] * o.c, cyk = L.cy[k] * o.c, czk = L.cz[k] * o.c;
             const double t = (cxk * o.u[id] + cyk * o.v[id] + czk * o.w[id]) / c2;
             geq[k] = L.w[k] * ph[id] * (1.0 + 3.0 * t + 4.5 * t * t - 1.5 * u2 / c2);
+            const double ssg = kMutant == 5 ? -1.0 : 1.0;  // mutant 5: sign of the sharpening term
+            const double tsg = kMutant == 4 ? 0.0 : 1.0;   // mutant 4: d(phi u)/dt dropped
             const double Fkp = o.dt * L.w[k] * (
-                (1.0 - 4.0 * (ph[id] - 0.5) * (ph[id] - 0.5)) / o.cfg.pf_thickness * (cxk * nx_ + cyk * ny_ + czk * nz_) / nrm
-                + 3 * (cxk * nabu + cyk * nabv + czk * nabw) / c2);
+                ssg * (1.0 - 4.0 * (ph[id] - 0.5) * (ph[id] - 0.5)) / o.cfg.pf_thickness * (cxk * nx_ + cyk * ny_ + czk * nz_) / nrm
+                + tsg * 3 * (cxk * nabu + cyk * nabv + czk * nabw) / c2);
             const double tp = o.cfg.pf_tau;  // 0.58 in the reference
             g[k] = g[k] - (g[k] - geq[k] - Fkp / 1.0 * (tp - 0.5)) / tp + Fkp * 0;
             gt[k] = g[k]; geq[k] = 0; g[k] = 0;
@@ -824,9 +841,10 @@ void flux_corner(Oracle& o, PopSet& P, int id, int kk) {
         for (int j = 0; j < 5; ++j) fxy += cxw[i] * cyw[j] * P.ftmp[(size_t)idf[i + j * 5] * Q + kk];
     const double dz = o.dz;
     const std::vector<double>& dX = o.del[0]; const std::vector<double>& dY = o.del[1];
-    P.fout[(size_t)id * Q + kk] = fx * dz / dX[id] + fy * dz / dY[id] - fxy * (dz * dz) / (dX[id] * dY[id]);
+    P.fout[(size_t)id * Q + kk] = fx * dz / dX[id] + fy * dz / dY[id] - (kMutant == 8 ? 0.0 : 1.0) * fxy * (dz * dz) / (dX[id] * dY[id]);
     const double a0 = fxy * (dz * dz) / (dX[idf[13]] * dY[idf[17]]);
-    const double a1 = (fy - fxy * dz / dX[idf[17]]) * dz / dY[idf[17]];
+    // mutant 9: the x width of the -y neighbour's flux taken at the +x neighbour
+    const double a1 = (fy - fxy * dz / dX[kMutant == 9 ? idf[13] : idf[17]]) * dz / dY[idf[17]];
     const double a2 = (fx - fxy * dz / dY[idf[13]]) * dz / dX[idf[13]];
     double* fin = P.feq.data(); double* finy = P.f.data();
     #pragma omp atomic
@@ -1311,9 +1329,14 @@ void ib_spm(Oracle& o) {
     const int Q = o.Q;
     for (int id = 0; id < o.N; ++id) {
         const double d[3] = {o.pos[0][id] - B.posB[0], o.pos[1][id] - B.posB[1], o.pos[2][id] - B.posB[2]};
-        const double X1 = B.quaS[0] * d[0] + B.quaS[3] * d[1] + B.quaS[6] * d[2];
-        const double Y1 = B.quaS[1] * d[0] + B.quaS[4] * d[1] + B.quaS[7] * d[2];
-        const double Z1 = B.quaS[2] * d[0] + B.quaS[5] * d[1] + B.quaS[8] * d[2];
+        double X1 = B.quaS[0] * d[0] + B.quaS[3] * d[1] + B.quaS[6] * d[2];
+        double Y1 = B.quaS[1] * d[0] + B.quaS[4] * d[1] + B.quaS[7] * d[2];
+        double Z1 = B.quaS[2] * d[0] + B.quaS[5] * d[1] + B.quaS[8] * d[2];
+        if (kMutant == 10) {  // mutant 10: the rotation applied untransposed
+            X1 = B.quaS[0] * d[0] + B.quaS[1] * d[1] + B.quaS[2] * d[2];
+            Y1 = B.quaS[3] * d[0] + B.quaS[4] * d[1] + B.quaS[5] * d[2];
+            Z1 = B.quaS[6] * d[0] + B.quaS[7] * d[1] + B.quaS[8] * d[2];
+        }
         const double distance = std::pow(X1 / a, 2) + std::pow(Y1 / a, 2) + std::pow(Z1 / b, 2);
         const double pr = spm_profile(distance, a);
         if (pr == 0.0) continue;
@@ -1639,7 +1662,7 @@ int orc_change_dt(void* h, double alpha) {
     o.cfg.dt = o.dt;
     const int Q = o.Q;
     for (int id = 0; id < o.N; ++id) {
-        o.p[id] /= alpha * alpha;
+        o.p[id] /= kMutant == 11 ? alpha : alpha * alpha;  // mutant 11: p / alpha
         for (int k = 0; k < Q; ++k) {
             const double tmp = (o.L.cx[k] * o.c * o.u[id] + o.L.cy[k] * o.c * o.v[id] + o.L.cz[k] * o.c * o.w[id]) / (o.c * o.c);
             o.fl.f[(size_t)id * Q + k] = o.L.w[k] * (o.p[id] + 3.0 * tmp + 4.5 * tmp * tmp -
@@ -1778,8 +1801,20 @@ int orc_lattice(int Q, int32_t* c, double* w) {
  * comparison: parity unpinned against the reference itself.
  * LES: a linear velocity field (exact central differences) gives the closed-form tau and tau_s
  * at every interior cell (tests/test_boundary_setters.py::test_les_closed_form).
- * SPM_ellipse3D, reset_items_dt + reset_pressure: parity unpinned in the same sense.
+ * tests/test_oracle_pins.py, each pin also shown to fail on a planted mistake (ORC_MUTANT):
+ *   Force: Guo forcing, u_z = -(n + 1/2) g dt in a periodic box; the pressure-gradient term on a
+ *     linear rho over a stretched axis; the chemical potential of the tanh profile (zero at
+ *     second order for the equilibrium width, closed form for twice the width, x over delX^2);
+ *   col_PF: a flat interface stays put with sum phi vol conserved; a slab translates with a
+ *     uniform stream; in an accelerating stream it follows the fluid's lattice momentum (the
+ *     d(phi u)/dt term);
+ *   col_g_reg: an advected sine mode decays as exp(-D k^2 t) and moves at u (D3Q7 and same-Q);
+ *   corner scheme (set_out num3, set_fin): equals the one-axis scheme along x (y uniform) and
+ *     along y (x uniform), exact streaming on a uniform grid, sum p vol conserved with slip walls
+ *     on a grid refined in x and y;
+ *   SPM_ellipse3D: v_B exactly inside a rotated ellipsoid (scipy's body frame), rest outside;
+ *   reset_items_dt + reset_pressure: p c^2, momentum and momentum flux kept, shear-wave decay
+ *     at the kept lattice viscosity.
  * Parity unpinned: the reference's own FP32 trajectories on NVIDIA hardware (CUDA device math
- * and atomicAdd ordering cannot run here); the 3D two-axis corner scheme (set_out num3 /
- * set_fin) is pinned only by mass conservation and the uniform-grid reduction.
+ * and atomicAdd ordering cannot run here).
  */

@@ -1,0 +1,385 @@
+"""Closed-form and invariant pins of the oracle functions that the reference's fixtures do not
+cover: Force (Guo forcing, the pressure-gradient term, surface tension), col_PF, col_g_reg under
+flow, the 3D two-axis corner scheme (set_out num3 / set_fin), SPM_ellipse3D and the dt change
+(reset_items_dt + reset_pressure).
+
+Each pin is a function of the oracle library that returns its measured errors and a verdict.  The
+test of a pin runs it on the oracle; the mutation test runs it on oracle builds with one planted
+mistake each (ORC_MUTANT, oracle/oracle.cpp) and requires the pin to fail there, so that every
+pin is shown to catch a plausible error (a dropped term, a wrong sign, a swapped index) in the
+function it pins."""
+import numpy as np
+import pytest
+from scipy.spatial.transform import Rotation
+
+import oracle
+from ip_cu_lbm_amd import _abi, cases
+
+
+def _oracle(case, lib):
+    return oracle.Oracle(case, lib)
+
+
+# ---- Force (collision.cu:57-112) -------------------------------------------------------------
+
+def pin_guo_forcing(lib=None):
+    """A uniform body force in a periodic box at rest (non-Boussinesq gravity, uniform rho,
+    conserved moments at rate 0 as in the reference's S files): Fk = w dt (c.F) / (rho c^2 / 3)
+    adds dt F / rho of momentum per step and update_rho adds the half step, so u_z =
+    -(n + 1/2) g dt exactly after n steps, and u_x = u_y = 0."""
+    g = 3e-5
+    rates = np.full(27, 1.1)
+    rates[[0, 3, 5, 7]] = 0.0  # density and momentum rows of the D3Q19 basis: conserved, rate 0
+    c = cases.lock_exchange(nx=8, ny=6, nz=10, ratio=1.0, scalar_lattice=0).replace(
+        periodic=(1, 1, 1), flux_axes=_abi.FLUX_NONE, scalar=_abi.SCALAR_NONE, boussinesq=0, gravity=g, s_rates=rates)
+    c.init = {}
+    o = _oracle(c, lib)
+    err = 0.0
+    done = 0
+    for n in (1, 5, 40):
+        o.step(n - done)
+        done = n
+        w = o.read("w")
+        err = max(err, np.abs(w + (n + 0.5) * g * c.dt).max() / ((n + 0.5) * g * c.dt),
+                  np.abs(o.read("u")).max() / (g * c.dt))
+    o.close()
+    return {"err": err, "ok": err < 1e-12}
+
+
+def pin_pressure_gradient_term(lib=None):
+    """Salinity linear in x on an x-stretched grid with a uniform p: rho = rho_L + beta_s S is
+    linear, so the central difference is exact and Force gives F_x = -p c^2/3 beta_s b D/(D + dz/1000)
+    with D the position difference of the two neighbours (the wall cell's own position beyond a
+    wall), F_y = 0 and the Boussinesq buoyancy F_z = -S beta_s g."""
+    nx, ny, nz = 12, 4, 6
+    px, wx = cases.axis_from_widths(cases.stretched_widths(nx, 1.1))
+    c = cases.lock_exchange(nx=nx, ny=ny, nz=nz, ratio=1.0).replace(
+        pos=[px, np.arange(ny) + 0.5, np.arange(nz) + 0.5], width=[wx, np.ones(ny), np.ones(nz)], flux_axes=_abi.FLUX_X)
+    X, _, _ = c.coords()
+    p0, a, b = 2e-3, 0.3, 0.05
+    c.init = {"S": a + b * X, "p": np.full(c.n_cells, p0)}
+    o = _oracle(c, lib)
+    for st in ("update_rho", "resetF", "Force"):
+        o.stage(st)
+    Fx, Fy, Fz, S = o.read("Fx"), o.read("Fy"), o.read("Fz"), o.read("S")
+    o.close()
+    ix = np.repeat(np.arange(nx), ny * nz)
+    D = px[np.minimum(ix + 1, nx - 1)] - px[np.maximum(ix - 1, 0)]
+    cc = c.dz / c.dt
+    want_x = -p0 * cc ** 2 / 3 * c.beta_s * b * D / (D + c.dz / 1000)
+    want_z = -S * c.beta_s * c.gravity
+    err = max(np.abs(Fx - want_x).max() / np.abs(want_x).max(), np.abs(Fz - want_z).max() / np.abs(want_z).max(),
+              np.abs(Fy).max())
+    return {"err": err, "ok": err < 1e-10}
+
+
+def _chemical_potential(lib, W, W2, axis, spacing=1.0, sigma=1e-3):
+    """mu = F / grad phi after the Force stage for phi = (1 + tanh(2 (r - r0) / W2)) / 2 along the
+    axis (2D, uniform grid of cell width `spacing` along the axis), against the closed form
+    mu = (24 sigma / W) (1 - W^2 / W2^2) phi (1 - phi) (1 - 2 phi) of the continuous profile, which
+    is 0 for W2 = W (the equilibrium profile: 4 beta phi (phi-1)(phi-1/2) = kappa phi'' with
+    beta = 12 sigma / W, kappa = 3 sigma W / 2).  Returns the largest deviation over the interface
+    and the largest closed-form value, both in units of sigma / W."""
+    n = int(16 * W / spacing)
+    c = cases.solitary_wave(nx=n if axis == 0 else 8, nz=n if axis == 2 else 8, ratio=1.0, width=W, sigma=sigma,
+                            rho_ratio=1.0, gravity=0.0).replace(periodic=(1, 0, 1), flux_axes=_abi.FLUX_NONE)
+    if spacing != 1.0:
+        pos, wid = cases.axis_from_widths(np.full(n, spacing))
+        P, Wd = list(c.pos), list(c.width)
+        P[axis], Wd[axis] = pos, wid
+        c = c.replace(pos=P, width=Wd, flux_axes=_abi.FLUX_X if axis == 0 else _abi.FLUX_Z)
+    X, _, Z = c.coords()
+    r = X if axis == 0 else Z
+    phi = 0.5 * (1 + np.tanh(2 * (r - r.mean()) / W2))
+    c.init = {"phi": phi}
+    o = _oracle(c, lib)
+    for st in ("update_rho", "resetF", "Force"):
+        o.stage(st)
+    F = o.read("Fx" if axis == 0 else "Fz")
+    ph = o.read("phi")
+    o.close()
+    Pm = ph.reshape(c.n[0], c.n[2])
+    a = 0 if axis == 0 else 1
+    grad = ((np.roll(Pm, -1, a) - np.roll(Pm, 1, a)) / (2 * spacing + c.dz / 1000)).ravel()
+    core = np.abs(phi - 0.5) < 0.45
+    mu = F[core] / grad[core]
+    want = (24 * sigma / W) * (1 - W * W / (W2 * W2)) * phi[core] * (1 - phi[core]) * (1 - 2 * phi[core])
+    unit = sigma / W
+    return np.abs(mu - want).max() / unit, np.abs(want).max() / unit
+
+
+def pin_surface_tension(lib=None):
+    """The equilibrium profile's chemical potential vanishes at second order (it shrinks about
+    threefold from W = 4 to W = 8 cells), and a profile twice as wide gives the closed-form mu
+    within 2 % of its peak, along x (over cells two lattice spacings wide: the x term of the
+    Laplacian is over delX^2) and along z."""
+    res = {}
+    ok = True
+    for axis, spacing in ((0, 1.0), (2, 1.0), (0, 2.0)):
+        e4, _ = _chemical_potential(lib, 4.0 * spacing, 4.0 * spacing, axis, spacing)
+        e8, _ = _chemical_potential(lib, 8.0 * spacing, 8.0 * spacing, axis, spacing)
+        ew, peak = _chemical_potential(lib, 4.0 * spacing, 8.0 * spacing, axis, spacing)
+        res[f"axis{axis}_dx{spacing}"] = (e4, e8, ew / peak)
+        ok = ok and e4 < 0.3 and e8 < e4 / 2.5 and ew / peak < 0.02
+    return {"err": res, "ok": ok}
+
+
+# ---- col_PF (collision.cu:190-223) ---------------------------------------------------------
+
+def pin_phase_field_flat_interface(lib=None):
+    """A flat equilibrium interface in fluid at rest (no surface tension, equal densities) stays
+    where it is: after 2000 steps the phase field differs from the initial tanh profile by less
+    than 0.01, its centre has not moved and sum phi vol is conserved to round-off."""
+    W = 4.0
+    c = cases.solitary_wave(nx=8, nz=64, ratio=1.0, width=W, sigma=0.0, rho_ratio=1.0, gravity=0.0).replace(
+        flux_axes=_abi.FLUX_NONE)
+    _, _, Z = c.coords()
+    phi0 = 0.5 * (1 - np.tanh(2 * (Z - 30.3) / W))
+    c.init = {"phi": phi0}
+    o = _oracle(c, lib)
+    s0 = o.global_sums()[2]
+    o.step(2000)
+    ph = o.read("phi")
+    s1 = o.global_sums()[2]
+    o.close()
+    dphi = np.abs(ph - phi0).max()
+    ctr = lambda p: (Z * p * (1 - p)).sum() / (p * (1 - p)).sum()
+    err = {"dphi": dphi, "mass": abs(s1 - s0) / s0, "centre": abs(ctr(ph) - ctr(phi0))}
+    return {"err": err, "ok": dphi < 0.01 and err["mass"] < 1e-12 and err["centre"] < 1e-3}
+
+
+def pin_phase_field_translation(lib=None):
+    """A slab of phi in a uniform stream u0 (periodic, no surface tension, equal densities): the
+    continuous conservative Allen-Cahn equation translates it unchanged, so after T steps its
+    centroid has moved by u0 T dt (within 1e-3 of a cell) and its profile by less than 0.01."""
+    W, nx, u0, T = 4.0, 96, 0.01, 1000
+    c = cases.solitary_wave(nx=nx, nz=4, ratio=1.0, width=W, sigma=0.0, rho_ratio=1.0, gravity=0.0).replace(
+        periodic=(1, 0, 1), flux_axes=_abi.FLUX_NONE)
+    X, _, _ = c.coords()
+    slab = lambda x0: 0.5 * (np.tanh(2 * (X - x0) / W) - np.tanh(2 * (X - x0 - 20) / W))
+    phi0 = slab(30.0)
+    c.init = {"phi": phi0, "u": np.full(c.n_cells, u0)}
+    o = _oracle(c, lib)
+    o.step(T)
+    ph = o.read("phi")
+    o.close()
+    cen = lambda p: (X * p).sum() / p.sum()
+    shift = cen(ph) - cen(phi0)
+    prof = np.abs(ph - slab(30.0 + u0 * T * c.dt)).max()
+    return {"err": {"shift": shift - u0 * T * c.dt, "profile": prof}, "ok": abs(shift - u0 * T * c.dt) < 1e-3 and prof < 0.01}
+
+
+def pin_phase_field_accelerating_stream(lib=None):
+    """A slab of phi carried by a uniformly accelerating fluid (periodic, gravity g on equal
+    densities, no surface tension).  Per step the slab moves by the post-collision first moment of
+    g over sum phi, which col_PF sets from phi u and, through its d(phi u)/dt source, follows the
+    fluid's own lattice momentum: a (m + 1) at step m, a = -g dt (Guo forcing, momentum rate 0).
+    So after T steps the centroid has moved by a T (T + 1) / 2, up to a start-up transient of
+    a / 2; without the d(phi u)/dt term the slab would drift by a further (tau_phi - 1/2) a T."""
+    W, nz, g, T = 4.0, 128, 1e-4, 800
+    c = cases.solitary_wave(nx=4, nz=nz, ratio=1.0, width=W, sigma=0.0, rho_ratio=1.0, gravity=g).replace(
+        periodic=(1, 0, 1), flux_axes=_abi.FLUX_NONE)
+    _, _, Z = c.coords()
+    phi0 = 0.5 * (np.tanh(2 * (Z - 80.0) / W) - np.tanh(2 * (Z - 100.0) / W))
+    c.init = {"phi": phi0}
+    o = _oracle(c, lib)
+    o.step(T)
+    ph = o.read("phi")
+    o.close()
+    cen = lambda p: (Z * p).sum() / p.sum()
+    a = -g * c.dt
+    dev = (cen(ph) - cen(phi0)) - a * T * (T + 1) / 2
+    drift = (c.pf_tau - 0.5) * abs(a) * T  # what a dropped d(phi u)/dt term adds
+    return {"err": {"deviation": dev, "drift_of_dropped_term": drift}, "ok": abs(dev) < 0.1 * drift}
+
+
+# ---- col_g_reg (collision.cu:167-188) ------------------------------------------------------
+
+def pin_scalar_advection_diffusion(lib=None):
+    """A sine mode of S in a uniform stream (periodic box, no buoyancy) obeys the advection-
+    diffusion equation: after T steps its amplitude is exp(-D k^2 T dt), D = cs^2 (tau_s - 1/2)
+    c^2 dt, within 2e-3, and its phase has moved by k u T dt within 2e-3 rad, on D3Q7 (1/cs^2 = 4)
+    and on the flow's own D3Q19 lattice (1/cs^2 = 3)."""
+    res = {}
+    ok = True
+    for ql in (7, 0):
+        nx, D, A, u0, w0, T = 64, 0.05, 0.1, 0.02, 0.01, 800
+        c = cases.lock_exchange(nx=nx, ny=4, nz=4, ratio=1.0, scalar_lattice=ql, diffusivity=D).replace(
+            periodic=(1, 1, 1), flux_axes=_abi.FLUX_NONE, gravity=0.0, beta_s=0.0)
+        X, _, _ = c.coords()
+        k = 2 * np.pi / nx
+        c.init = {"S": 1.0 + A * np.sin(k * X), "u": np.full(c.n_cells, u0), "w": np.full(c.n_cells, w0)}
+        o = _oracle(c, lib)
+        o.step(T)
+        S = o.read("S")
+        o.close()
+        s = ((S - 1.0) * np.sin(k * X)).sum() * 2 / len(X)
+        co = ((S - 1.0) * np.cos(k * X)).sum() * 2 / len(X)
+        amp, phase = np.hypot(s, co) / A, np.arctan2(-co, s)
+        ea = abs(amp / np.exp(-D * k * k * T) - 1)
+        ep = abs(phase - k * u0 * T)
+        res[f"q{ql or 19}"] = (ea, ep)
+        ok = ok and ea < 2e-3 and ep < 2e-3
+    return {"err": res, "ok": ok}
+
+
+# ---- the 3D two-axis corner scheme (adv.cu:100-175) ----------------------------------------
+
+def _corner_box(q, rx, ry, flux, bc=_abi.BC_SLIP, nx=10, ny=8, nz=6):
+    px, wx = cases.axis_from_widths(cases.stretched_widths(nx, rx))
+    py, wy = cases.axis_from_widths(cases.stretched_widths(ny, ry))
+    pz, wz = cases.axis_from_widths(np.ones(nz))
+    c = cases.Case(name=f"corner_q{q}", lattice=q, n=(nx, ny, nz), pos=[px, py, pz], width=[wx, wy, wz],
+                   wall_bc=bc, flux_axes=flux, tau=0.7, s_rates=np.full(27, 1.2))
+    X, Y, Z = c.coords()
+    c.init = {"u": 0.01 * np.sin(X / 3.0) * np.cos(Z), "v": 0.01 * np.cos(Y / 2.0), "w": 0.005 * np.sin(X + Y),
+              "p": cases.uniform(7, c.n_cells, -1e-3, 1e-3)}
+    return c
+
+
+def _run(c, lib, n=30):
+    o = _oracle(c, lib)
+    s0 = o.global_sums()[0]
+    o.step(n)
+    r = {f: o.read(f) for f in ("p", "u", "v", "w")}
+    s1 = o.global_sums()[0]
+    o.close()
+    return r, s0, s1
+
+
+def pin_corner_scheme(lib=None):
+    """The reference's corner scheme on a grid refined in x only equals the one-axis flux scheme
+    along x (1e-13), refined in y only the one along y, uniform in both exact streaming; and on a
+    grid refined in x and y with slip walls sum p vol is conserved to round-off.  D3Q27 and D3Q19."""
+    rel = lambda a, b: max(np.linalg.norm(a[f] - b[f]) / np.linalg.norm(b[f]) for f in a)
+    res = {}
+    ok = True
+    for q in (27, 19):
+        for rx, ry, one in ((1.15, 1.0, _abi.FLUX_X), (1.0, 1.15, _abi.FLUX_Y), (1.0, 1.0, _abi.FLUX_NONE)):
+            a, _, _ = _run(_corner_box(q, rx, ry, _abi.FLUX_REFERENCE), lib)
+            b, _, _ = _run(_corner_box(q, rx, ry, one), None)
+            res[f"q{q}_{rx}_{ry}"] = rel(a, b)
+            ok = ok and res[f"q{q}_{rx}_{ry}"] < 1e-13
+        _, s0, s1 = _run(_corner_box(q, 1.1, 1.15, _abi.FLUX_REFERENCE), lib)
+        res[f"q{q}_mass"] = abs(s1 - s0)
+        ok = ok and abs(s1 - s0) < 1e-15
+    return {"err": res, "ok": ok}
+
+
+# ---- SPM_ellipse3D (SphereFall3D/SphereFall.cu:13-30) ---------------------------------------
+
+def pin_smoothed_profile(lib=None):
+    """A fixed body moving at v_B in fluid at rest, with the interior forcing on a rotated
+    ellipsoid (semi-axes a, a, b in the body frame, body frame from scipy's rotation of the
+    quaternion): after one step every cell well inside the ellipsoid (squared normalised distance
+    < 0.9, profile 1) moves at v_B exactly, every cell between the ellipsoid and the marker shell
+    (distance > 1.11, profile 0) and every cell beyond the shell stays at rest."""
+    n, R = (26, 24, 28), 8.0
+    c = cases.falling_sphere(n=n, diameter=2 * R, markers=600, tau=0.7, gravity=0.0, method=_abi.IB_STRESS,
+                             kernel=_abi.IB_KERNEL_REFERENCE).replace(lattice=19)
+    c.init = {}
+    b = c.body
+    b.pos = (13.2, 11.9, 14.3)
+    th, ax = 0.7, np.array([1.0, 2.0, -0.5]) / np.linalg.norm([1.0, 2.0, -0.5])
+    b.quat = (np.cos(th / 2), *(np.sin(th / 2) * ax))
+    vB = np.array([0.01, -0.004, 0.02])
+    b.vel = tuple(vB)
+    b.moving = 0
+    b.spm, b.spm_a, b.spm_b = 1, 5.0, 3.5
+    o = _oracle(c, lib)
+    o.step(1)
+    u = np.stack([o.read(x) for x in "uvw"], axis=1)
+    o.close()
+    X, Y, Z = c.coords()
+    d = np.stack([X - b.pos[0], Y - b.pos[1], Z - b.pos[2]], axis=1)
+    Rm = Rotation.from_quat([b.quat[1], b.quat[2], b.quat[3], b.quat[0]]).as_matrix()
+    body = d @ Rm  # R^T d: body-frame coordinates
+    dist = (body[:, 0] / b.spm_a) ** 2 + (body[:, 1] / b.spm_a) ** 2 + (body[:, 2] / b.spm_b) ** 2
+    r = np.linalg.norm(d, axis=1)
+    inside, between, far = (dist < 0.9) & (r < R - 2.5), (dist > 1.11) & (r < R - 2.5), r > R + 3
+    err = {"inside": np.abs(u[inside] - vB).max(), "between": np.abs(u[between]).max(), "far": np.abs(u[far]).max()}
+    ok = inside.sum() > 100 and between.sum() > 100 and max(err.values()) < 1e-15
+    return {"err": err, "ok": ok}
+
+
+# ---- reset_items_dt + reset_pressure (basic.cu:100-110, 262-273) ----------------------------
+
+def pin_dt_change(lib=None):
+    """After dt / alpha (c * alpha, tau kept) the populations are the equilibrium of the same
+    physical state: p c^2 and the momentum sum e c f = u are unchanged, the momentum flux is
+    p c^2/3 I + u u; and a shear wave then decays by exp(-nu k^2 n) per n steps with the lattice
+    viscosity nu = (tau - 1/2)/3 that tau keeps (within 1 %), for alpha = 2 and 1/2."""
+    c = cases.lattice_taylor_green(nx=16, nz=32, ratio=1.0).replace(periodic=(1, 0, 1), flux_axes=_abi.FLUX_NONE,
+                                                                     s_rates=np.full(9, 1 / 0.6))
+    X, _, Z = c.coords()
+    k = 2 * np.pi / 32
+    c.init = {"u": 0.01 * np.sin(k * Z), "w": 0.002 * np.sin(2 * np.pi * X / 16), "p": 1e-3 * np.cos(k * Z)}
+    cx, _ = oracle.lattice(9)
+    ex, ez = cx[:, 0][:, None], cx[:, 2][:, None]
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    res = {}
+    ok = True
+    for alpha in (2.0, 0.5):
+        o = _oracle(c, lib)
+        o.step(10)
+        p0, u0, w0 = o.read("p"), o.read("u"), o.read("w")
+        c0 = c.dz / c.dt
+        o.change_dt(alpha)
+        c1 = c0 * alpha
+        f = o.read("f")
+        e = {"pc2": rel(f.sum(0) * c1 ** 2, p0 * c0 ** 2), "mom": rel((ex * c1 * f).sum(0), u0),
+             "Pxx": rel((ex * ex * c1 ** 2 * f).sum(0), p0 * c0 ** 2 / 3 + u0 * u0),
+             "Pxz": np.abs((ex * ez * c1 ** 2 * f).sum(0) - u0 * w0).max() / np.abs(u0 * w0).max()}
+        a0 = (o.read("u") * np.sin(k * Z)).sum()
+        o.step(200)
+        a1 = (o.read("u") * np.sin(k * Z)).sum()
+        o.close()
+        e["decay"] = abs(a1 / a0 / np.exp(-(0.6 - 0.5) / 3 * k * k * 200) - 1)
+        res[alpha] = e
+        ok = ok and max(e["pc2"], e["mom"], e["Pxx"], e["Pxz"]) < 1e-12 and e["decay"] < 0.01
+    return {"err": res, "ok": ok}
+
+
+PINS = {
+    "guo_forcing": pin_guo_forcing,
+    "pressure_gradient_term": pin_pressure_gradient_term,
+    "surface_tension": pin_surface_tension,
+    "phase_field_flat_interface": pin_phase_field_flat_interface,
+    "phase_field_translation": pin_phase_field_translation,
+    "phase_field_accelerating_stream": pin_phase_field_accelerating_stream,
+    "scalar_advection_diffusion": pin_scalar_advection_diffusion,
+    "corner_scheme": pin_corner_scheme,
+    "smoothed_profile": pin_smoothed_profile,
+    "dt_change": pin_dt_change,
+}
+
+# planted mistake (oracle.cpp ORC_MUTANT) -> the pin that must catch it
+MUTANTS = {
+    1: "pressure_gradient_term",       # sign of -p c^2/3 grad rho
+    2: "surface_tension",              # sign of kappa
+    3: "guo_forcing",                  # Fk over rho c^2 instead of rho c^2 / 3
+    4: "phase_field_accelerating_stream",  # d(phi u)/dt dropped from col_PF's source
+    5: "phase_field_flat_interface",   # sign of col_PF's sharpening term
+    6: "scalar_advection_diffusion",   # col_g_reg's equilibrium without the velocity
+    7: "scalar_advection_diffusion",   # 1/cs^2 = 3 on D3Q7
+    8: "corner_scheme",                # the Fxy term dropped from the corner fout
+    9: "corner_scheme",                # a swapped width index in the corner flux
+    10: "smoothed_profile",            # the body rotation untransposed
+    11: "dt_change",                   # p / alpha instead of p / alpha^2
+    12: "surface_tension",             # the Laplacian's x term over dz^2 instead of delX^2
+}
+
+
+@pytest.mark.parametrize("name", list(PINS))
+def test_pin_holds_on_the_oracle(name):
+    r = PINS[name]()
+    print(name, r["err"])
+    assert r["ok"], (name, r["err"])
+
+
+@pytest.mark.parametrize("mutant", sorted(MUTANTS))
+def test_pin_catches_planted_mistake(mutant):
+    lib = oracle.build_mutant(mutant)
+    r = PINS[MUTANTS[mutant]](lib)
+    print(mutant, MUTANTS[mutant], r["err"])
+    assert not r["ok"], (mutant, MUTANTS[mutant], r["err"])
