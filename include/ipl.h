@@ -231,13 +231,37 @@ typedef struct ipl_ib_config {
     double pos[3], vel[3];        /* initial body centre and velocity */
     double omega[3];              /* initial angular velocity, body frame (angleVB) */
     double quat[4];               /* initial orientation quaternion (q0, q1, q2, q3) */
-    int32_t spm;                  /* 1: smoothed-profile forcing of the body interior before the
-                                     marker forcing (SPM_ellipse3D, SphereFall3D/SphereFall.cu:13-30,
-                                     called at :376); stress method */
+    int32_t spm;                  /* smoothed-profile forcing of the body interior before the marker
+                                     forcing, f += w dt 3 (e c . F) / c^2 with F = profile (v - u) / dt:
+                                     1 = SPM_ellipse3D (SphereFall3D/SphereFall.cu:13-30, called at
+                                     :376; stress method): ellipsoid a, a, b in the body frame;
+                                     2 = SPM_ellipse (ibm.cu:89-122): ellipse a, b over the body-frame
+                                     x and z (x and y on a 3D lattice), the reference's rotation as
+                                     written (untransposed), sine ramp of width dz/a;
+                                     3 = SPM (ibm.cu:68-87): circle of radius a about the body centre in
+                                     the x-z plane (x-y on a 3D lattice), ramp over +-dz/2 about
+                                     a - dz/2, v = the velocity of marker 0, F over dz.
+                                     The library's two forms are commented out in every driver. */
     int32_t reserved0;
-    double spm_a, spm_b;          /* SPM ellipsoid semi-axes (body frame x, y: a; z: b) */
+    double spm_a, spm_b;          /* SPM semi-axes (form 1: body frame x, y: a; z: b; form 3: radius a) */
     int32_t reserved[2];
 } ipl_ib_config;
+
+/* The immersed-boundary kernels of the drivers' loop one at a time, for stage-wise parity
+ * (SphereFall.cu:372-383).  IPL_IB_STAGE_FLOW runs one step without its IB sub-step (every kernel
+ * before get_IBMGw2; with an open boundary its reset comes at the end of this stage); the other
+ * stages then act on the current state in the drivers' order: SPM (the body's smoothed-profile
+ * forcing, SPM_ellipse3D, SphereFall.cu:13-30), FORCE (get_IBMGw2, ibm.cu:124-274, or
+ * IB_directForcing, ibm.cu:276-344, repeated for multi-direct forcing, spread onto the grid),
+ * UPDATE_VEL (update_velIBM, ibm.cu:346-365, with the force FORCE spread), BODY (update_IBbody,
+ * ibm.cu:367-424) and POINTS (update_IBpoint + search_IBlattice, ibm.cu:426-463, of a moving
+ * body).  FLOW then SPM (if configured), FORCE, UPDATE_VEL, BODY, POINTS is one ipl_step.  Fused
+ * path, body attached. */
+enum {
+    IPL_IB_STAGE_FLOW = 0, IPL_IB_STAGE_SPM = 1, IPL_IB_STAGE_FORCE = 2, IPL_IB_STAGE_UPDATE_VEL = 3,
+    IPL_IB_STAGE_BODY = 4, IPL_IB_STAGE_POINTS = 5
+};
+int ipl_ib_stage(ipl_ctx* ctx, int32_t stage);
 
 /* Attaches the body to a context (once, before the first step); nodes of a marker must lie inside
  * the box: the reference's ghost positions make the 3-point Lagrange weights singular there. */
@@ -302,6 +326,11 @@ int ipl_change_dt(ipl_ctx* ctx, double alpha);
 int ipl_write_fields_csv(ipl_ctx* ctx, const char* path);
 /* IB_csv (ibm.cu:3-14): "x,y,z,velx,vely,velz,ax,ay,az" and one row per marker (IB_point/ibm_%04d.csv). */
 int ipl_write_markers_csv(ipl_ctx* ctx, const char* path);
+/* out_C_D (SphereFall3D/SphereFall.cu:32-42, Karman_vortex_2D/Karman_vortex2D.cu:81-91): the
+ * "C_D,C_time" header and one row per record, the two columns in the C++ stream's default format
+ * (6 significant digits).  The drivers record a drag coefficient (Karman_vortex2D.cu:378-379) or
+ * the body's settling velocity (SphereFall.cu:424) at every save interval; n = 0 writes the header. */
+int ipl_write_cd_csv(const char* path, const double* c_d, const double* time, int64_t n);
 /* The VTU of LockExchange/make_VTK.py: one VTK_HEXAHEDRON per calc cell (corners at the cell
  * faces, shared between neighbours), cell data salinity, phi, rho, pressure, velocity, force;
  * ASCII, written directly instead of through a CSV and pyvista. */

@@ -171,6 +171,11 @@ class Simulation:
         self._n_markers = len(body.markers)
         del keep
 
+    def ib_stage(self, name: str):
+        """One stage of the immersed-boundary step: 'flow' (a step without its IB sub-step), then
+        'spm', 'force', 'update_vel', 'body', 'points' in the drivers' order (ipl_ib_stage)."""
+        self._check(self._lib.ipl_ib_stage(self._h, _abi.IB_STAGES[name]))
+
     def body_state(self) -> dict:
         """FP64 body state: pos, vel, omega (body frame), quat, force, torque."""
         out = np.zeros(19)
@@ -279,3 +284,30 @@ def estimate_device_bytes(case: "cases.Case", library: str = None) -> int:
     if rc != 0:
         raise IplError(rc, "ipl_estimate_device_bytes")
     return int(out.value)
+
+
+def write_cd_csv(path: str, c_d, time, library: str = None):
+    """The drivers' C_D.csv (out_C_D, SphereFall3D/SphereFall.cu:32-42): header "C_D,C_time" and one
+    row per record, e.g. the settling velocity or a drag coefficient at each save interval."""
+    lib = _abi.load_library(library)
+    a = np.ascontiguousarray(c_d, dtype=np.float64).ravel()
+    t = np.ascontiguousarray(time, dtype=np.float64).ravel()
+    if a.size != t.size:
+        raise ValueError("C_D and time records differ in length")
+    rc = lib.ipl_write_cd_csv(os.fsencode(path), _dp(a), _dp(t), a.size)
+    if rc != 0:
+        raise IplError(rc, f"ipl_write_cd_csv {path}")
+
+
+def settling_history(sim: "Simulation", steps: int, every: int, dt: float):
+    """Runs `steps` steps and records, every `every` steps, the body's z velocity and the time, as
+    the falling-sphere driver does for its C_D.csv (SphereFall3D/SphereFall.cu:424, 437)."""
+    vel, times = [], []
+    done = 0
+    while done < steps:
+        k = min(every, steps - done)
+        sim.step(k)
+        done += k
+        vel.append(float(sim.body_state()["vel"][2]))
+        times.append(done * dt)
+    return np.array(vel), np.array(times)

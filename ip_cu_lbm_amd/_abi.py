@@ -30,6 +30,8 @@ FIELDS = {
     "Fx": 7, "Fy": 8, "Fz": 9, "tau": 10, "taus": 11, "f": 12, "g": 13,
 }
 
+IB_STAGES = {"flow": 0, "spm": 1, "force": 2, "update_vel": 3, "body": 4, "points": 5}
+
 STAGES = {
     "equ_f": 0, "Force": 1, "collide": 2, "IP_f": 3, "scalar_collide": 4, "IP_g": 5,
     "update_scalar": 6, "update_rho": 7, "set_wall_rho": 8, "resetF": 9, "equ_g": 10,
@@ -125,6 +127,8 @@ def load_library(path: str = None):
     lib.ipl_build_tables.restype = c_int32
     lib.ipl_ib_attach.argtypes = [vp, POINTER(IplIbConfig)]
     lib.ipl_ib_attach.restype = c_int32
+    lib.ipl_ib_stage.argtypes = [vp, c_int32]
+    lib.ipl_ib_stage.restype = c_int32
     lib.ipl_ib_body_state.argtypes = [vp, _DP]
     lib.ipl_ib_body_state.restype = c_int32
     lib.ipl_ib_read_markers.argtypes = [vp, c_int32, _DP, c_size_t]
@@ -155,6 +159,8 @@ def load_library(path: str = None):
     lib.ipl_hydrostatic_pressure.restype = c_int32
     lib.ipl_wall_cells.argtypes = [POINTER(IplConfig), c_int32, POINTER(c_int32), POINTER(c_int64)]
     lib.ipl_wall_cells.restype = c_int32
+    lib.ipl_write_cd_csv.argtypes = [c_char_p, _DP, _DP, c_int64]
+    lib.ipl_write_cd_csv.restype = c_int32
     lib.ipl_change_dt.argtypes = [vp, c_double]
     lib.ipl_change_dt.restype = c_int32
     for name in ("ipl_write_fields_csv", "ipl_write_markers_csv", "ipl_write_vtu"):

@@ -213,6 +213,16 @@ int ipl_ib_attach(ipl_ctx* ctx, const ipl_ib_config* ib) {
     return s ? IPL_OK : fail(ctx, s);
 }
 
+int ipl_ib_stage(ipl_ctx* ctx, int32_t stage) {
+    if (!ctx) return IPL_ERR_ARGUMENT;
+    ipl::Status s = ctx->engine->ib_stage(stage);
+    if (!s) return fail(ctx, s);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(ctx, ipl::Status::err(IPL_ERR_HIP, hipGetErrorString(e)));
+    s = ctx->engine->post_step();
+    return s ? IPL_OK : fail(ctx, s);
+}
+
 int ipl_ib_body_state(ipl_ctx* ctx, double out[19]) {
     if (!ctx || !out) return IPL_ERR_ARGUMENT;
     ipl::Status s = ctx->engine->ib_body_state(out);
@@ -396,6 +406,16 @@ int ipl_write_markers_csv(ipl_ctx* ctx, const char* path) {
              << "," << vel[i * 3 + 2] << "," << gw[i * 3] << "," << gw[i * 3 + 1] << "," << gw[i * 3 + 2] << "\n";
     file.close();
     return file ? IPL_OK : fail(ctx, ipl::Status::err(IPL_ERR_ARGUMENT, std::string("write failed: ") + path));
+}
+
+int ipl_write_cd_csv(const char* path, const double* c_d, const double* time, int64_t n) {
+    if (!path || n < 0 || (n > 0 && (!c_d || !time))) return IPL_ERR_ARGUMENT;
+    std::ofstream file(path);
+    if (!file) return IPL_ERR_ARGUMENT;
+    file << "C_D,C_time\n";
+    for (int64_t i = 0; i < n; ++i) file << c_d[i] << "," << time[i] << "\n";  // the stream's default format
+    file.close();
+    return file ? IPL_OK : IPL_ERR_ARGUMENT;
 }
 
 int ipl_write_vtu(ipl_ctx* ctx, const char* path) {

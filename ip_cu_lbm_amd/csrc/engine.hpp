@@ -126,6 +126,7 @@ public:
     virtual Status ib_body_state(double*) { return Status::err(IPL_ERR_ARGUMENT, "no body attached"); }
     virtual Status ib_read_markers(int, double*, size_t) { return Status::err(IPL_ERR_ARGUMENT, "no body attached"); }
     virtual int64_t ib_marker_count() const { return 0; }
+    virtual Status ib_stage(int) { return Status::err(IPL_ERR_UNSUPPORTED, "the immersed boundary runs on the fused path"); }
     virtual Status change_dt(double) { return Status::err(IPL_ERR_UNSUPPORTED, "time-step changes run on the fused path"); }
     // vals: (p, u, v, w) per listed cell
     virtual Status set_open_boundary(const int32_t*, int64_t, const double*) {

@@ -310,7 +310,7 @@ public:
             if (geo_.les)  // LES after update_rho, before the IB sub-step (Boegman2005run18IB.cu:338-339)
                 fusedk::fused_launch_les<ST, MT, Q, QS>(geo_, tb_, fb_, gb_, Fb_, les_vel_, les_nu_, les_tau_, les_taus_, K_,
                                                          block_, stream);
-            if (ib_.active()) {
+            if (ib_.active() && !flow_only_) {
                 // IB sub-step on f^{n+1}, with the Force field of this step (SphereFall.cu:372-383)
                 Status r = ib_.substep(fb_, gb_, Fb_, std::is_same<ST, double>::value ? IPL_F64 : IPL_F32);
                 if (!r) return r;
@@ -440,8 +440,7 @@ public:
         g.rho_L = cfg_.rho_L; g.rho_H = cfg_.rho_H; g.beta_s = cfg_.beta_s;
         Status r = ib_.attach(cfg, g, cfg_pos_, stream);
         if (r) {
-            geo_.half_force = 0;
-            K_.uc = MT(1);
+            geo_.half_force = 0;  // u = sum c e f after update_velIBM, no force half step
             drop_graph();  // the captured step has no IB sub-step
         }
         return r;
@@ -449,6 +448,17 @@ public:
     Status ib_body_state(double* out) override { return ib_.body_state(out); }
     Status ib_read_markers(int which, double* out, size_t count) override { return ib_.read_markers(which, out, count); }
     int64_t ib_marker_count() const override { return ib_.markers(); }
+    Status ib_stage(int which) override {
+        if (!ib_.active()) return Status::err(IPL_ERR_ARGUMENT, "no body attached");
+        if (which == IPL_IB_STAGE_FLOW) {  // one step, launched directly, without its IB sub-step
+            flow_only_ = true;
+            Status r = dispatch([&](auto q, auto qs) { return this->template step_q<decltype(q)::value, decltype(qs)::value>(1); });
+            flow_only_ = false;
+            return r;
+        }
+        // the state of the last step: f, g and its Force field
+        return ib_.stage(which, fa_, ga_, Fa_, std::is_same<ST, double>::value ? IPL_F64 : IPL_F32);
+    }
 
     // FP64 factors of the step from dz, dt, c and the rates, cast once into K_
     void derive_coefficients() {
@@ -674,6 +684,7 @@ private:
     double *vol_ = nullptr, *out_ = nullptr, *partial_ = nullptr;
     std::vector<int64_t> perm_;
     IbSolver ib_;
+    bool flow_only_ = false;                               // step_q without the IB sub-step
     hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};  // per buffer parity
     const ST* f_first_ = nullptr;                          // fa_ as set up: parity 0
     int captures_ = 0;

@@ -80,7 +80,7 @@ struct FParams {
 template <typename MT>
 struct FCoef {
     MT c, inv_c, inv_c2, half_dt, fk_coef, dt;
-    MT uc;                             // velocity scale of sum e f: c, or 1 after an IB sub-step
+    MT uc;                             // velocity scale of sum e f: c
     MT itau, om_s, rnorm[27];
     MT wall_nu, dz;                    // kinematic viscosity of the wall function's log law; dz
     MT inv_n2[27], mu_L, mu_H;         // LES: 1 / |row r|^2, dynamic viscosities

@@ -732,8 +732,8 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
         const MT phi = G.scalar == IPL_SCALAR_PHASE_FIELD ? sc : MT(0);
         const MT rho = P.rho_L + P.drho * phi + S_ * P.beta_s;
         const MT irho = MT(1) / rho;
-        // u = sum e c f + F_old dt / (2 rho); after an immersed-boundary sub-step u = sum e f
-        // (uc = 1, no force term).  Only the loads are conditional, so the code stays one path.
+        // u = sum e c f + F_old dt / (2 rho); after an immersed-boundary sub-step u = sum e c f
+        // (update_velIBM: no force term).  Only the loads are conditional, so the code stays one path.
         MT F0 = MT(0), F1 = MT(0), F2 = MT(0);
         if (G.half_force && !G.zero_force) {
             F0 = *elem(b.F_old, n);
@@ -967,10 +967,10 @@ __global__ void k_readout(FGeom G, const ST* f_src, const ST* g_src, const MT* F
         out[1 * NC + n] = jx + F_old[n] * MT(P.dt) / (MT(2.0) * rho);
         out[2 * NC + n] = jy + F_old[NC + n] * MT(P.dt) / (MT(2.0) * rho);
         out[3 * NC + n] = jz + F_old[2 * NC + n] * MT(P.dt) / (MT(2.0) * rho);
-    } else {  // update_velIBM: u = sum e f (ibm.cu:356-360)
-        out[1 * NC + n] = jx / c;
-        out[2 * NC + n] = jy / c;
-        out[3 * NC + n] = jz / c;
+    } else {  // update_velIBM: u = sum c e f (ibm.cu:356-360)
+        out[1 * NC + n] = jx;
+        out[2 * NC + n] = jy;
+        out[3 * NC + n] = jz;
     }
     out[4 * NC + n] = rho;
     out[5 * NC + n] = S;

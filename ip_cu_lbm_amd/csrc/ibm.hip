@@ -256,7 +256,7 @@ __global__ void k_ib_ubase(IbArgs A, const ST* __restrict__ f, const ST* __restr
     Ubox[3 * b + 2] = jz + (double)F[2 * (size_t)A.ncell + idx] * A.dt / (2.0 * rho);
 }
 
-// ---- k_ib_spm: SPM_ellipse3D (SphereFall3D/SphereFall.cu:13-30) on the cells of the box --------
+// ---- k_ib_spm: smoothed-profile forcing of the body interior ----------------------------------
 // u of the cell as update_rho leaves it (sum e c f + F dt / (2 rho), the step's force), the
 // body-frame ellipsoid distance, the driver's profile (profile_s22: transition width fixed at
 // 0.1), and f += w dt 3 (e c . F) / c^2 with F = profile (v_B - u) / dt
@@ -268,27 +268,74 @@ __device__ inline double spm_profile(double r, double Rada) {
     return 0.0;
 }
 
+// profile_s2 (ibm.cu:50-58) of SPM_ellipse: 1 inside, 0 outside, a sine ramp of width lim between
+__device__ inline double spm_profile_s2(double lim, double r) {
+    const double in_line = 1.0 - lim + (lim / 2) * (lim / 2), out_line = 1.0 + lim + (lim / 2) * (lim / 2);
+    if (r < in_line) return 1.0;
+    if (r <= out_line) return 0.5 * (sin(3.14159 * r / lim) + 1.0);
+    return 0.0;
+}
+
+// profile_s (ibm.cu:44-48) of SPM: 1 more than dz/2 inside the circle of radius R - dz/2, 0 more
+// than dz/2 outside, a sine ramp between
+__device__ inline double spm_profile_s(double dz, double R, double dist) {
+    const double r = R - 0.5 * dz - dist;
+    if (r > 0.5 * dz) return 1.0;
+    if (r >= -0.5 * dz) return 0.5 * (sin(3.14159 * r / dz) + 1.0);
+    return 0.0;
+}
+
+// Smoothed-profile forcing of the body interior, f += w dt 3 (e c . F) / c^2 with F = profile
+// (v - u) / dt (u of update_rho): spm = 1 SPM_ellipse3D (SphereFall3D/SphereFall.cu:13-30) on the
+// cells of the box, the ellipsoid (a, a, b) in the body frame R^T d; spm = 2 SPM_ellipse (ibm.cu:89-122)
+// on every cell, the ellipse (a, b) over the body-frame x and z of R d (x and y on a 3D lattice:
+// its "xy face"), v the body velocity; spm = 3 SPM (ibm.cu:68-87) on every cell, the circle of
+// radius a about the body centre in the plane of the lattice's first and last axes (x-z in 2D,
+// x-y in 3D), v the velocity of marker 0, F over dz.
 template <typename ST, typename MT, int Q>
 __global__ void k_ib_spm(IbArgs A, ST* __restrict__ f, const ST* __restrict__ g, const MT* __restrict__ F,
-                         const int* __restrict__ box, const double* __restrict__ body) {
+                         const int* __restrict__ box, const double* __restrict__ body, const double* __restrict__ velw) {
     const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int64_t vol = (int64_t)box[3] * box[4] * box[5];
-    if (b >= vol || b >= A.vmax) return;
     int c[3];
-    c[2] = (int)(b % box[5]);
-    c[1] = (int)((b / box[5]) % box[4]);
-    c[0] = (int)(b / ((int64_t)box[4] * box[5]));
-    for (int a = 0; a < 3; ++a) {
-        c[a] += box[a];
-        if (c[a] >= A.n[a]) c[a] -= A.n[a];
+    if (A.spm == 1) {
+        const int64_t vol = (int64_t)box[3] * box[4] * box[5];
+        if (b >= vol || b >= A.vmax) return;
+        c[2] = (int)(b % box[5]);
+        c[1] = (int)((b / box[5]) % box[4]);
+        c[0] = (int)(b / ((int64_t)box[4] * box[5]));
+        for (int a = 0; a < 3; ++a) {
+            c[a] += box[a];
+            if (c[a] >= A.n[a]) c[a] -= A.n[a];
+        }
+    } else {  // every cell, from its internal index
+        if (b >= (int64_t)A.ncell) return;
+        c[A.s] = (int)(b % A.ns);
+        const int64_t line = b / A.ns;
+        c[A.o2] = (int)(line % A.n[A.o2]);
+        c[A.o1] = (int)(line / A.n[A.o2]);
     }
     const double* R = body + B_R;
     const double d[3] = {A.pos[0][c[0]] - body[B_POS], A.pos[1][c[1]] - body[B_POS + 1], A.pos[2][c[2]] - body[B_POS + 2]};
-    const double X1 = R[0] * d[0] + R[3] * d[1] + R[6] * d[2];
-    const double Y1 = R[1] * d[0] + R[4] * d[1] + R[7] * d[2];
-    const double Z1 = R[2] * d[0] + R[5] * d[1] + R[8] * d[2];
-    const double distance = (X1 / A.spm_a) * (X1 / A.spm_a) + (Y1 / A.spm_a) * (Y1 / A.spm_a) + (Z1 / A.spm_b) * (Z1 / A.spm_b);
-    const double pr = spm_profile(distance, A.spm_a);
+    double pr, vb[3];
+    if (A.spm == 1) {
+        const double X1 = R[0] * d[0] + R[3] * d[1] + R[6] * d[2];
+        const double Y1 = R[1] * d[0] + R[4] * d[1] + R[7] * d[2];
+        const double Z1 = R[2] * d[0] + R[5] * d[1] + R[8] * d[2];
+        const double distance = (X1 / A.spm_a) * (X1 / A.spm_a) + (Y1 / A.spm_a) * (Y1 / A.spm_a) + (Z1 / A.spm_b) * (Z1 / A.spm_b);
+        pr = spm_profile(distance, A.spm_a);
+        for (int a = 0; a < 3; ++a) vb[a] = body[B_VEL + a];
+    } else if (A.spm == 2) {
+        const double X1 = R[0] * d[0] + R[1] * d[1] + R[2] * d[2];
+        double Z1 = R[6] * d[0] + R[7] * d[1] + R[8] * d[2];
+        if (A.dim == 3) Z1 = R[3] * d[0] + R[4] * d[1] + R[5] * d[2];  // the xy face
+        const double distance = (X1 / A.spm_a) * (X1 / A.spm_a) + (Z1 / A.spm_b) * (Z1 / A.spm_b);
+        pr = spm_profile_s2(A.dz / A.spm_a, distance);
+        for (int a = 0; a < 3; ++a) vb[a] = body[B_VEL + a];
+    } else {
+        const int t = A.dim == 3 ? 1 : 2;
+        pr = spm_profile_s(A.dz, A.spm_a, sqrt(d[0] * d[0] + d[t] * d[t]));
+        for (int a = 0; a < 3; ++a) vb[a] = velw[a];
+    }
     if (pr == 0.0) return;
     const uint32_t idx = cell_of(A, c);
     const double cl = A.c;
@@ -305,8 +352,9 @@ __global__ void k_ib_spm(IbArgs A, ST* __restrict__ f, const ST* __restrict__ g,
     const double rho = A.rho_L + A.drho * phi + S * A.beta_s;
     const double u[3] = {jx + (double)F[idx] * A.dt / (2.0 * rho), jy + (double)F[A.ncell + idx] * A.dt / (2.0 * rho),
                          jz + (double)F[2 * (size_t)A.ncell + idx] * A.dt / (2.0 * rho)};
+    const double per = A.spm == 3 ? A.dz : A.dt;
     double fs[3];
-    for (int a = 0; a < 3; ++a) fs[a] = pr * (body[B_VEL + a] - u[a]) / A.dt;
+    for (int a = 0; a < 3; ++a) fs[a] = pr * (vb[a] - u[a]) / per;
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         const Dir e = Lattice<Q>::c(k);
@@ -319,7 +367,7 @@ __global__ void k_ib_spm(IbArgs A, ST* __restrict__ f, const ST* __restrict__ g,
 // Stress method (get_IBMGw2, ibm.cu:124-274): f interpolated at the marker, split into outer (fw1)
 // and inner (fw2) populations by the normal; minus the traction of (fw1 - fw2) spread, marker force
 // Gw from the traction of fw1.  Direct forcing (IB_directForcing, ibm.cu:276-344): node velocity
-// (plus dt F_ib / 3 of earlier passes) interpolated, (U_w - u)/dt spread.  Slot sums are reduced in
+// (plus c^2 dt F_ib / 3 of earlier passes) interpolated, (U_w - u)/dt spread.  Slot sums are reduced in
 // a fixed tree, so every run adds in the same order.
 constexpr int kIbLanes = 64;
 
@@ -397,7 +445,7 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
             if (b < 0 || b >= A.vmax) continue;  // cannot happen: the box holds every node block
             for (int a = 0; a < 3; ++a) {
                 double u = Ubox[3 * b + a];
-                if (it > 0) u += A.dt * Fbox[3 * b + a] / 3.0;
+                if (it > 0) u += A.c * A.c * A.dt * Fbox[3 * b + a] / 3.0;  // what update_velIBM adds
                 part[a] += wgt * u;
             }
         }
@@ -432,14 +480,15 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
                 }
             const double fk = red[t * kIbLanes], fo = red[op * kIbLanes];
             const double nv[3] = {nB[3 * m], nB[3 * m + 1], nB[3 * m + 2]};
-            const double cn = (nv[0] * ex + nv[1] * ey + nv[2] * ez) * A.inv_c;
-            const double lim = sqrt((double)(ex * ex + ey * ey + ez * ez)) * A.inv_c * A.sina;
-            const double bounce = fo + 6.0 * wk * (ex * U[0] + ey * U[1] + ez * U[2]) * A.inv_c2;
+            // the reference's lattice velocities are c e (items[IDX_cx], Items::setc, basic.cu:79-96)
+            const double ev[3] = {ex * A.c, ey * A.c, ez * A.c};
+            const double cn = (nv[0] * ev[0] + nv[1] * ev[1] + nv[2] * ev[2]) * A.inv_c;
+            const double lim = sqrt(ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2]) * A.inv_c * A.sina;
+            const double bounce = fo + 6.0 * wk * (ev[0] * U[0] + ev[1] * U[1] + ev[2] * U[2]) * A.inv_c2;
             const double beta = fabs(cn) / (lim + 1e-10);
             const double mixed = (1.0 - beta) * fk + beta * bounce;
             const double fw1 = cn <= 0 ? fk : (cn <= lim ? mixed : bounce);
             const double fw2 = cn >= 0 ? fk : (cn >= -lim ? mixed : bounce);
-            const double ev[3] = {(double)ex, (double)ey, (double)ez};
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const double g = h == 0 ? fw1 - fw2 : fw1;
@@ -461,7 +510,7 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
             __syncthreads();
         }
         // traction sigma . n with sigma_ab = -sum_k [g_k delta_ab / (2 tau)
-        //   + (1 - 1/(2 tau)) 3 (g_k (e_a - U_a)(e_b - U_b) - (g_k - w_k) U_a U_b) / c^2]
+        //   + (1 - 1/(2 tau)) 3 (g_k (c e_a - U_a)(c e_b - U_b) - (g_k - w_k) U_a U_b) / c^2]
         // = -[G delta_ab / (2 tau) + (1 - 1/(2 tau)) 3 / c^2 (G_ab - G_a U_b - U_a G_b + U_a U_b)]
         // (sum_k w_k = 1), the same on every lane
         const double nv[3] = {nB[3 * m], nB[3 * m + 1], nB[3 * m + 2]};
@@ -591,7 +640,7 @@ __global__ void __launch_bounds__(kIbGatherLanes) k_ib_gather(IbArgs A, int it, 
         for (int a = 0; a < 3; ++a) Fbox[3 * b + a] = (it == 0 ? 0.0 : Fbox[3 * b + a]) + red[a][0];
 }
 
-// ---- k_ib_apply: update_velIBM (ibm.cu:346-365) on the box ------------------------------------
+// ---- k_ib_apply: update_velIBM (ibm.cu:346-365) on the box: f += w dt (c e . F) ---------------
 // It also clears the marker runs of every box cell for the next step's rank kernel.
 template <typename ST, int Q>
 __global__ void k_ib_apply(IbArgs A, ST* __restrict__ f, const int* __restrict__ box, const double* __restrict__ Fbox,
@@ -615,7 +664,7 @@ __global__ void k_ib_apply(IbArgs A, ST* __restrict__ f, const int* __restrict__
     for (int k = 0; k < Q; ++k) {
         const Dir e = Lattice<Q>::c(k);
         ST& fv = f[(uint32_t)k * A.ncell + idx];
-        fv = (ST)((double)fv + Lattice<Q>::w(k) * A.dt * (e.x * F[0] + e.y * F[1] + e.z * F[2]));
+        fv = (ST)((double)fv + Lattice<Q>::w(k) * A.dt * A.c * (e.x * F[0] + e.y * F[1] + e.z * F[2]));
     }
 }
 
@@ -776,8 +825,11 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
         return Status::err(IPL_ERR_ARGUMENT, "the stress method uses the reference weights");
     if (cfg.kernel != IPL_IB_KERNEL_REFERENCE && cfg.kernel != IPL_IB_KERNEL_PESKIN4)
         return Status::err(IPL_ERR_ARGUMENT, "IB kernel");
-    if (cfg.spm && (cfg.method != IPL_IB_STRESS || !(cfg.spm_a > 0) || !(cfg.spm_b > 0)))
-        return Status::err(IPL_ERR_ARGUMENT, "the smoothed profile needs the stress method and positive semi-axes");
+    if (cfg.spm < 0 || cfg.spm > 3) return Status::err(IPL_ERR_ARGUMENT, "smoothed-profile form");
+    if (cfg.spm == 1 && cfg.method != IPL_IB_STRESS)
+        return Status::err(IPL_ERR_ARGUMENT, "SPM_ellipse3D runs with the stress method");
+    if (cfg.spm && (!(cfg.spm_a > 0) || (cfg.spm != 3 && !(cfg.spm_b > 0))))
+        return Status::err(IPL_ERR_ARGUMENT, "the smoothed profile needs positive semi-axes (radius)");
     if (!(cfg.mass > 0) || !(cfg.density > 0) || !(cfg.spacing > 0))
         return Status::err(IPL_ERR_ARGUMENT, "body mass, density and marker spacing must be positive");
     if (cfg.moving && !(cfg.inertia[0] > 0 && cfg.inertia[1] > 0 && cfg.inertia[2] > 0))
@@ -833,7 +885,7 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
     // the smoothed profile reaches sqrt(1.1025) semi-axes from the centre: pad the box beyond the
     // marker node blocks by what that exceeds of the marker radius
     bpad_ = 0;
-    if (cfg.spm) {
+    if (cfg.spm == 1) {  // the other forms run over every cell
         const double reach = std::sqrt(1.1025) * std::max(cfg.spm_a, cfg.spm_b);
         bpad_ = std::max(0, (int)std::ceil((reach - rmax) / g.dz) + 1);
     }
@@ -950,19 +1002,60 @@ void apply_q(int Q, const ibk::IbArgs& A, ST* f, const int* box, const double* F
 
 Status IbSolver::substep(void* f, const void* g, const void* F, int storage) {
     if (M_ == 0) return Status::ok();
+    Status r = rank();
+    if (r && cfg_.spm) r = spm(f, g, F, storage);  // before the marker forcing (SphereFall.cu:376-377)
+    if (r) r = forcing(f, g, F, storage);
+    if (r) r = update_vel(f, storage);
+    if (r) r = body();
+    if (r) r = points();
+    return r;
+}
+
+// One stage of the sub-step on the current state (ipl_ib_stage): the stages that read the marker
+// runs and the box rank the markers first, as the sub-step does once.
+Status IbSolver::stage(int which, void* f, const void* g, const void* F, int storage) {
+    if (M_ == 0) return Status::err(IPL_ERR_ARGUMENT, "no body attached");
+    switch (which) {
+    case IPL_IB_STAGE_SPM:
+        if (!cfg_.spm) return Status::err(IPL_ERR_ARGUMENT, "the body has no smoothed-profile forcing");
+        { Status r = rank(); return r ? spm(f, g, F, storage) : r; }
+    case IPL_IB_STAGE_FORCE: { Status r = rank(); return r ? forcing(f, g, F, storage) : r; }
+    case IPL_IB_STAGE_UPDATE_VEL: return update_vel(f, storage);
+    case IPL_IB_STAGE_BODY: return body();
+    case IPL_IB_STAGE_POINTS: return points();
+    default: return Status::err(IPL_ERR_ARGUMENT, "IB stage id");
+    }
+}
+
+// markers sorted by nearest cell, the box of their node blocks and each box cell's marker run
+Status IbSolver::rank() {
     const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_, bpad_);
     constexpr int kPad = 2 * ibk::kIbRankParts;
     const int mpad = (M_ + kPad - 1) / kPad * kPad;
     IPL_LAUNCH_SYNC(ibk::k_ib_rank, (unsigned)((M_ + ibk::kIbRankMarkers - 1) / ibk::kIbRankMarkers), ibk::kIbRankThreads,
                     (size_t)mpad * sizeof(unsigned long long), st_, A, node_, sorted_, box_, range_, mpad);
-    const int iters = cfg_.method == IPL_IB_DIRECT ? std::max(1, cfg_.iterations) : 1;
-    if (cfg_.spm) {  // before the marker forcing (SphereFall.cu:376-377)
-        const unsigned blocks = (unsigned)((vmax_ + 255) / 256);
-#define IPL_IBS(ST, MT, QQ) IPL_LAUNCH((ibk::k_ib_spm<ST, MT, QQ>), blocks, 256, 0, st_, A, (ST*)f, (const ST*)g, (const MT*)F, box_, body_)
-        if (storage == IPL_F64) { if (g_.Q == 9) IPL_IBS(double, double, 9); else if (g_.Q == 19) IPL_IBS(double, double, 19); else IPL_IBS(double, double, 27); }
-        else { if (g_.Q == 9) IPL_IBS(float, float, 9); else if (g_.Q == 19) IPL_IBS(float, float, 19); else IPL_IBS(float, float, 27); }
+    IPL_HIP_TRY(hipGetLastError());
+    return Status::ok();
+}
+
+// SPM_ellipse3D (SphereFall3D/SphereFall.cu:13-30) on the box
+Status IbSolver::spm(void* f, const void* g, const void* F, int storage) {
+    const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_, bpad_);
+    const int64_t items = cfg_.spm == 1 ? vmax_ : (int64_t)g_.ncell;  // the box, or every cell
+    const unsigned blocks = (unsigned)((items + 255) / 256);
+#define IPL_IBS(ST, MT, QQ) IPL_LAUNCH((ibk::k_ib_spm<ST, MT, QQ>), blocks, 256, 0, st_, A, (ST*)f, (const ST*)g, (const MT*)F, box_, body_, velw_)
+    if (storage == IPL_F64) { if (g_.Q == 9) IPL_IBS(double, double, 9); else if (g_.Q == 19) IPL_IBS(double, double, 19); else IPL_IBS(double, double, 27); }
+    else { if (g_.Q == 9) IPL_IBS(float, float, 9); else if (g_.Q == 19) IPL_IBS(float, float, 19); else IPL_IBS(float, float, 27); }
 #undef IPL_IBS
-    }
+    IPL_HIP_TRY(hipGetLastError());
+    return Status::ok();
+}
+
+// marker forces (get_IBMGw2 / IB_directForcing, repeated for multi-direct forcing) and their
+// spread onto the box (needs rank() of the current marker nodes)
+Status IbSolver::forcing(void* f, const void* g, const void* F, int storage) {
+    const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_, bpad_);
+    const int iters = cfg_.method == IPL_IB_DIRECT ? std::max(1, cfg_.iterations) : 1;
     if (cfg_.method == IPL_IB_DIRECT) {
         if (storage == IPL_F64) ubase_q<double, double>(g_.Q, A, (const double*)f, (const double*)g, (const double*)F, box_, Ubox_, st_);
         else ubase_q<float, float>(g_.Q, A, (const float*)f, (const float*)g, (const float*)F, box_, Ubox_, st_);
@@ -977,9 +1070,30 @@ Status IbSolver::substep(void* f, const void* g, const void* F, int storage) {
         IPL_LAUNCH_SYNC(ibk::k_ib_gather, (unsigned)vmax_, ibk::kIbGatherLanes, 0, st_, A, it, sorted_, range_, contrib_, box_, Fbox_,
                         mark_);
     }
+    IPL_HIP_TRY(hipGetLastError());
+    return Status::ok();
+}
+
+// update_velIBM (ibm.cu:346-365) with the spread force of forcing(); clears the marker runs
+Status IbSolver::update_vel(void* f, int storage) {
+    const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_, bpad_);
     if (storage == IPL_F64) apply_q<double>(g_.Q, A, (double*)f, box_, Fbox_, range_, mark_, st_);
     else apply_q<float>(g_.Q, A, (float*)f, box_, Fbox_, range_, mark_, st_);
+    IPL_HIP_TRY(hipGetLastError());
+    return Status::ok();
+}
+
+// update_IBbody (ibm.cu:367-424)
+Status IbSolver::body() {
+    const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_, bpad_);
     IPL_LAUNCH_SYNC(ibk::k_ib_body, 1, ibk::kIbBodyThreads, 0, st_, A, Gw_, posw_, body_);
+    IPL_HIP_TRY(hipGetLastError());
+    return Status::ok();
+}
+
+// update_IBpoint + search_IBlattice (ibm.cu:426-463) of a moving body
+Status IbSolver::points() {
+    const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_, bpad_);
     if (cfg_.moving) {
         const unsigned blocks = (unsigned)((M_ + 63) / 64);
         if (g_.Q == 9) IPL_LAUNCH((ibk::k_ib_points<9>), blocks, 64, 0, st_, A, body_, oposw_, onB_, posw_, nB_, velw_, node_);

@@ -52,6 +52,8 @@ public:
     // The IB sub-step on the populations f of the fused path (storage F64 or F32), with F the Force
     // field of the step (same type as the arithmetic) and g the scalar populations (or null).
     Status substep(void* f, const void* g, const void* F, int storage);
+    // One stage of the sub-step (IPL_IB_STAGE_*) on the current state.
+    Status stage(int which, void* f, const void* g, const void* F, int storage);
     Status body_state(double out[19]);
     // IPL_ERR_UNSUPPORTED when a marker block left the bound the box was sized for
     Status check();
@@ -63,6 +65,13 @@ public:
     int markers() const { return M_; }
 
 private:
+    Status rank();
+    Status spm(void* f, const void* g, const void* F, int storage);
+    Status forcing(void* f, const void* g, const void* F, int storage);
+    Status update_vel(void* f, int storage);
+    Status body();
+    Status points();
+
     IbGrid g_{};
     ipl_ib_config cfg_{};
     hipStream_t st_ = nullptr;

@@ -55,6 +55,8 @@ def load(path: str = None):
         lib.orc_time_steps.argtypes = [c_void_p, c_int]
         lib.orc_time_steps.restype = c_double
         lib.orc_stage.argtypes = [c_void_p, c_int]
+        lib.orc_ib_stage.argtypes = [c_void_p, c_int]
+        lib.orc_ib_stage.restype = c_int
         lib.orc_info.argtypes = [c_void_p, POINTER(c_int64), POINTER(c_int64)]
         lib.orc_read.argtypes = [c_void_p, c_int, DP, c_size_t]
         lib.orc_write.argtypes = [c_void_p, c_int, DP, c_size_t]
@@ -154,6 +156,12 @@ class Oracle:
         from ip_cu_lbm_amd._abi import STAGES
         sid = STAGES[name_or_id] if isinstance(name_or_id, str) else int(name_or_id)
         self._lib.orc_stage(self._h, sid)
+
+    def ib_stage(self, name: str):
+        from ip_cu_lbm_amd._abi import IB_STAGES
+        rc = self._lib.orc_ib_stage(self._h, IB_STAGES[name])
+        if rc != 0:
+            raise ValueError(f"oracle IB stage {name} failed ({rc})")
 
     def read(self, field: str) -> np.ndarray:
         from ip_cu_lbm_amd._abi import FIELDS

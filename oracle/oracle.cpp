@@ -1132,8 +1132,9 @@ void ib_stress(Oracle& o) {
         });
         const double U[3] = {B.velw[3 * m], B.velw[3 * m + 1], B.velw[3 * m + 2]};
         const double nv[3] = {B.nB[3 * m], B.nB[3 * m + 1], B.nB[3 * m + 2]};
+        // the reference's lattice velocities are c e (items[IDX_cx], Items::setc, basic.cu:79-96)
         for (int k = 0; k < Q; ++k) {
-            const double e[3] = {(double)o.L.cx[k], (double)o.L.cy[k], (double)o.L.cz[k]};
+            const double e[3] = {c * o.L.cx[k], c * o.L.cy[k], c * o.L.cz[k]};
             const double cn = (nv[0] * e[0] + nv[1] * e[1] + nv[2] * e[2]) / c;
             const double lim = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) / c * std::sin(alpha);
             const double bounce = fw[o.L.kinv[k]] + 6.0 * o.L.w[k] * (e[0] * U[0] + e[1] * U[1] + e[2] * U[2]) / c2;
@@ -1144,13 +1145,13 @@ void ib_stress(Oracle& o) {
         }
         const double tau = o.tau[B.lattice_id[m]];
         // traction sigma . n of the populations g (ibm.cu:207-227): sigma_ab = -sum_k [ (1/(2 tau)) g_k delta_ab
-        //   + (1 - 1/(2 tau)) 3 (g_k (e_a - U_a)(e_b - U_b) - (g_k - w_k) U_a U_b) / c^2 ]
+        //   + (1 - 1/(2 tau)) 3 (g_k (c e_a - U_a)(c e_b - U_b) - (g_k - w_k) U_a U_b) / c^2 ]
         auto traction = [&](const std::vector<double>& g1, const std::vector<double>* g2, double out[3]) {
             for (int a = 0; a < 3; ++a) {
                 double sg[3] = {0, 0, 0};
                 for (int k = 0; k < Q; ++k) {
                     const double gk = g2 ? g1[k] - (*g2)[k] : g1[k];
-                    const double e[3] = {(double)o.L.cx[k], (double)o.L.cy[k], (double)o.L.cz[k]};
+                    const double e[3] = {c * o.L.cx[k], c * o.L.cy[k], c * o.L.cz[k]};
                     for (int b = 0; b < 3; ++b)
                         sg[b] -= 0.5 / tau * gk * (a == b ? 1.0 : 0.0) +
                                  (1.0 - 0.5 / tau) * 3 * (gk * (e[a] - U[a]) * (e[b] - U[b]) - (gk - o.L.w[k]) * U[a] * U[b]) / c2;
@@ -1175,8 +1176,8 @@ void ib_stress(Oracle& o) {
 
 // IB_directForcing (ibm.cu:276-344): node velocity interpolated at the marker, force
 // (U_w - u)/dt spread to F.  With iterations > 1 (multi-direct forcing, builder extension) the
-// node velocities are corrected by the spread force, du = dt dF / 3 (what update_velIBM does to
-// them, sum_k w_k e_k e_k = 1/3), and the pass is repeated.
+// node velocities are corrected by the spread force, du = c^2 dt dF / 3 (what update_velIBM does
+// to them, sum_k w_k (c e_k)(c e_k) = c^2 / 3), and the pass is repeated.
 void ib_direct(Oracle& o) {
     auto& B = o.ib;
     const double c2 = o.c * o.c;
@@ -1207,21 +1208,23 @@ void ib_direct(Oracle& o) {
         }
         for (int id = 0; id < o.N; ++id) {
             o.Fx[id] += dF[0][id]; o.Fy[id] += dF[1][id]; o.Fz[id] += dF[2][id];
-            for (int a = 0; a < 3; ++a) vel[a][id] += o.dt * dF[a][id] / 3.0;
+            for (int a = 0; a < 3; ++a) vel[a][id] += c2 * o.dt * dF[a][id] / 3.0;
         }
     }
 }
 
-// update_velIBM (ibm.cu:346-365): f += w dt (e . F); p and u recomputed from f (no half-force term)
+// update_velIBM (ibm.cu:346-365): f += w dt (c e . F); p and u = sum c e f recomputed from f (no
+// half-force term)
 void ib_update_vel(Oracle& o) {
     const int Q = o.Q;
+    const double c = o.c;
     #pragma omp parallel for schedule(static)
     for (int id = 0; id < o.N; ++id) {
         double p = 0, ux = 0, uy = 0, uz = 0;
         for (int k = 0; k < Q; ++k) {
             double& f = o.fl.f[(size_t)id * Q + k];
-            f = f + o.L.w[k] * o.dt * (o.L.cx[k] * o.Fx[id] + o.L.cy[k] * o.Fy[id] + o.L.cz[k] * o.Fz[id]);
-            p += f; ux += o.L.cx[k] * f; uy += o.L.cy[k] * f; uz += o.L.cz[k] * f;
+            f = f + o.L.w[k] * o.dt * (c * o.L.cx[k] * o.Fx[id] + c * o.L.cy[k] * o.Fy[id] + c * o.L.cz[k] * o.Fz[id]);
+            p += f; ux += c * o.L.cx[k] * f; uy += c * o.L.cy[k] * f; uz += c * o.L.cz[k] * f;
         }
         o.p[id] = p; o.u[id] = ux; o.v[id] = uy; o.w[id] = uz;
     }
@@ -1321,10 +1324,62 @@ double spm_profile(double r, double Rada) {
            1.0 * (r < in_line);
 }
 
+// profile_s2 (ibm.cu:50-58), the profile of SPM_ellipse
+double spm_profile_s2(double lim, double r) {
+    const double in_line = 1.0 - lim + std::pow(lim / 2, 2), out_line = 1.0 + lim + std::pow(lim / 2, 2);
+    return 0 * (out_line < r) + (0.5 * (std::sin(3.14159 * r / lim) + 1.0)) * (in_line <= r && r <= out_line) +
+           1.0 * (r < in_line);
+}
+
+// profile_s (ibm.cu:44-48), the profile of SPM; dist = distance from the circle's centre
+double spm_profile_s(double dz, double R, double dist) {
+    const double r = R - 0.5 * dz - dist;
+    return 0 * (r < -0.5 * dz) + (0.5 * (std::sin(3.14159 * r / dz) + 1.0)) * (std::fabs(r) <= 0.5 * dz) + 1.0 * (r > 0.5 * dz);
+}
+
+// The library's SPM_ellipse (ibm.cu:89-122, spm = 2) and SPM (ibm.cu:68-87, spm = 3): the second
+// assignment of X1, Y1, Z1 (R d, untransposed) is the one SPM_ellipse uses, and on a 3D lattice
+// its ellipse lies in the xy face; SPM's circle is in the x-y plane of its posx / posy arguments,
+// here the lattice's first and last axes (x-z in 2D, where x-y would be a stripe), with the
+// velocity of marker 0 and the force over dz.  f += w dt 3 (e c . F) / c^2 (SPM writes
+// f = ftmp + ..., ftmp = f: the same once ftmp holds f).
+void ib_spm_library(Oracle& o) {
+    const auto& B = o.ib;
+    const double a = B.cfg.spm_a, b = B.cfg.spm_b, c2 = o.c * o.c;
+    const int Q = o.Q;
+    for (int id = 0; id < o.N; ++id) {
+        const double d[3] = {o.pos[0][id] - B.posB[0], o.pos[1][id] - B.posB[1], o.pos[2][id] - B.posB[2]};
+        double pr, v[3];
+        if (B.cfg.spm == 2) {
+            double X1 = B.quaS[0] * d[0] + B.quaS[1] * d[1] + B.quaS[2] * d[2];
+            double Y1 = B.quaS[3] * d[0] + B.quaS[4] * d[1] + B.quaS[5] * d[2];
+            double Z1 = B.quaS[6] * d[0] + B.quaS[7] * d[1] + B.quaS[8] * d[2];
+            if (kMutant == 13) {  // mutant 13: the transposed rotation (SPM_ellipse3D's) here
+                X1 = B.quaS[0] * d[0] + B.quaS[3] * d[1] + B.quaS[6] * d[2];
+                Y1 = B.quaS[1] * d[0] + B.quaS[4] * d[1] + B.quaS[7] * d[2];
+                Z1 = B.quaS[2] * d[0] + B.quaS[5] * d[1] + B.quaS[8] * d[2];
+            }
+            if (o.dim == 3) Z1 = Y1;  // 3D (xy face)
+            pr = spm_profile_s2(o.dz / a, std::pow(X1 / a, 2) + std::pow(Z1 / b, 2));
+            for (int q = 0; q < 3; ++q) v[q] = B.velB[q];
+        } else {
+            const int t = o.dim == 3 ? 1 : 2;
+            pr = spm_profile_s(o.dz, a, std::sqrt(std::pow(d[0], 2) + std::pow(d[t], 2)));
+            for (int q = 0; q < 3; ++q) v[q] = kMutant == 14 ? B.velB[q] : B.velw[q];  // mutant 14: body velocity
+        }
+        if (pr == 0.0) continue;
+        const double per = B.cfg.spm == 3 && kMutant != 15 ? o.dz : o.dt;  // mutant 15: SPM's force over dt
+        const double fx = pr * (v[0] - o.u[id]) / per, fy = pr * (v[1] - o.v[id]) / per, fz = pr * (v[2] - o.w[id]) / per;
+        for (int k = 0; k < Q; ++k)
+            o.fl.f[(size_t)id * Q + k] += o.L.w[k] * o.dt * 3.0 * (o.L.cx[k] * o.c * fx + o.L.cy[k] * o.c * fy + o.L.cz[k] * o.c * fz) / c2;
+    }
+}
+
 // SPM_ellipse3D (SphereFall3D/SphereFall.cu:13-30): forcing (v_B - u) / dt on the body interior,
 // weighted by the profile of the body-frame ellipsoid distance, added to f
 void ib_spm(Oracle& o) {
     const auto& B = o.ib;
+    if (B.cfg.spm != 1) { ib_spm_library(o); return; }
     const double a = B.cfg.spm_a, b = B.cfg.spm_b, c2 = o.c * o.c;
     const int Q = o.Q;
     for (int id = 0; id < o.N; ++id) {
@@ -1537,14 +1592,36 @@ void wall_functions(Oracle& o) {
     wall_function(o, 0, 1, 0, wall[4], o.u, o.w, o.v, o.Fx, o.Fz);
 }
 
-void step(Oracle& o) {
+void step(Oracle& o, bool with_ib = true) {
     if (o.cfg.wall_function) wall_functions(o);
     for (int s = IPL_STAGE_EQU_F; s <= IPL_STAGE_RESET_F; ++s) {
         stage(o, s);
         if (s == IPL_STAGE_UPDATE_RHO && o.cfg.les) les(o);
     }
-    if (o.ib.active) ib_substep(o);
+    if (o.ib.active && with_ib) ib_substep(o);
     if (!o.ob_cells.empty()) open_boundary(o);
+}
+
+// One stage of the IB sub-step (ipl_ib_stage): FLOW = a step without it, then the kernels of
+// ib_substep in its order; FORCE clears F and Gw first and POINTS clears F after, as ib_substep.
+int ib_stage(Oracle& o, int which) {
+    auto& B = o.ib;
+    if (!B.active) return IPL_ERR_ARGUMENT;
+    switch (which) {
+    case IPL_IB_STAGE_FLOW: step(o, false); break;
+    case IPL_IB_STAGE_SPM: if (!B.cfg.spm) return IPL_ERR_ARGUMENT; ib_spm(o); break;
+    case IPL_IB_STAGE_FORCE:
+        reset_force(o);
+        std::fill(B.Gw.begin(), B.Gw.end(), 0.0);
+        if (B.cfg.method == IPL_IB_STRESS) ib_stress(o);
+        else ib_direct(o);
+        break;
+    case IPL_IB_STAGE_UPDATE_VEL: ib_update_vel(o); break;
+    case IPL_IB_STAGE_BODY: ib_body(o); break;
+    case IPL_IB_STAGE_POINTS: if (B.cfg.moving) ib_points(o); reset_force(o); break;
+    default: return IPL_ERR_ARGUMENT;
+    }
+    return 0;
 }
 
 }  // namespace
@@ -1576,6 +1653,8 @@ double orc_time_steps(void* h, int n) {
     const auto t1 = std::chrono::steady_clock::now();
     return std::chrono::duration<double>(t1 - t0).count();
 }
+
+int orc_ib_stage(void* h, int which) { return ib_stage(*static_cast<Oracle*>(h), which); }
 
 int orc_stage(void* h, int s) {
     stage(*static_cast<Oracle*>(h), s);

@@ -149,3 +149,92 @@ def test_ib_lines_along_y_fp64(oracle_lib):
                              density_ratio=8.0)
     c.body.pos = (10.3, 20.2, 12.4)
     _compare(c, 30, oracle_lib, 1e-12, gw_tol=1e-10)
+
+
+def _ib_stagewise(case, oracle_lib, tol, library=None):
+    """ipl_ib_stage against the oracle's stages: after a few whole steps, the flow part of a step,
+    then each IB kernel in the drivers' order (SphereFall.cu:372-383), comparing what that kernel
+    writes: the populations after FLOW, SPM and UPDATE_VEL, marker forces after FORCE, the body
+    after BODY, marker positions and nearest cells after POINTS; then whole steps again."""
+    stages = ["flow"] + (["spm"] if case.body.spm else []) + ["force", "update_vel", "body", "points"]
+    with Simulation(case, library) as sim:
+        o = oracle_lib.Oracle(case)
+        try:
+            sim.step(5)
+            o.step(5)
+            for rnd in range(2):
+                for st in stages:
+                    sim.ib_stage(st)
+                    o.ib_stage(st)
+                    errs = {}
+                    if st in ("flow", "spm", "update_vel"):  # these write the populations
+                        errs["f"] = rel_l2(sim.read("f"), o.read("f"))
+                    if st == "force":
+                        errs["Gw"] = rel_l2(sim.markers("force"), o.markers("force"))
+                    if st == "body":
+                        gb, ob = sim.body_state(), o.body_state()
+                        errs["body_force"] = rel_l2(gb["force"], ob["force"])
+                        errs["body_vel"] = rel_l2(gb["vel"], ob["vel"])
+                    if st == "points":
+                        errs["posw"] = rel_l2(sim.markers("pos"), o.markers("pos"))
+                        assert np.array_equal(sim.markers("node"), o.markers("node"))
+                    print(case.name, rnd, st, errs, flush=True)
+                    for k, e in errs.items():
+                        assert e <= tol, (rnd, st, k, e)
+            sim.step(3)
+            o.step(3)
+            assert rel_l2(np.stack([sim.read(x) for x in "uvw"]), np.stack([o.read(x) for x in "uvw"])) <= tol
+        finally:
+            o.close()
+
+
+def test_ib_stages_stress_method_with_smoothed_profile_fp64(oracle_lib):
+    c = _sphere()
+    c.body.spm, c.body.spm_a, c.body.spm_b = 1, 4.0, 4.0
+    _ib_stagewise(c, oracle_lib, 1e-12)
+
+
+def test_ib_stages_multi_direct_forcing_fp64(oracle_lib):
+    _ib_stagewise(_sphere(method=_abi.IB_DIRECT, kernel=_abi.IB_KERNEL_PESKIN4, iters=3, density_ratio=8.0), oracle_lib, 1e-10)
+
+
+def test_ib_stages_cylinder_2d_fp64(oracle_lib):
+    c = cases.cylinder_2d(n=(48, 56), markers=140, storage=F64, path=PATH_FUSED, density_ratio=3.0, gravity=2e-4)
+    _ib_stagewise(c, oracle_lib, 1e-12)
+
+
+@pytest.mark.parametrize("method", [_abi.IB_STRESS, _abi.IB_DIRECT])
+def test_ib_lattice_speed_not_one_fp64(oracle_lib, method):
+    """dt = dz / 2, so the lattice speed c = dz / dt = 2: the reference's lattice velocities are
+    c e (Items::setc, basic.cu:79-96) in the stress method's split and traction, in
+    update_velIBM and in the velocity it leaves (ibm.cu:199-227, 352-358).  Stress method, and
+    multi-direct forcing with the Peskin kernel."""
+    if method == _abi.IB_STRESS:
+        c = _sphere()
+    else:
+        c = _sphere(method=_abi.IB_DIRECT, kernel=_abi.IB_KERNEL_PESKIN4, iters=3, density_ratio=8.0)
+    c = c.replace(dt=0.5, name=c.name + "_c2")
+    c.init = {"u": 0.02 * np.sin(2 * np.pi * c.coords()[2] / c.n[2]), "p": cases.uniform(5, c.n_cells, -1e-4, 1e-4)}
+    _compare(c, 30, oracle_lib, 1e-12 if method == _abi.IB_STRESS else 1e-10)
+
+
+def test_ib_stages_lattice_speed_not_one_fp64(oracle_lib):
+    c = cases.cylinder_2d(n=(48, 56), markers=140, storage=F64, path=PATH_FUSED, density_ratio=3.0,
+                          gravity=2e-4).replace(dt=0.5)
+    _ib_stagewise(c, oracle_lib, 1e-12)
+
+
+@pytest.mark.parametrize("form", [2, 3])
+def test_ib_library_smoothed_profile_forms_fp64(oracle_lib, form):
+    """The library's SPM_ellipse (form 2, ibm.cu:89-122) and SPM (form 3, ibm.cu:68-87) on a moving,
+    spinning sphere (D3Q19, c = 2) and, for SPM_ellipse, a 2D cylinder turned about y."""
+    c = _sphere().replace(dt=0.5)
+    c.body.omega = (0.0, 0.0, 0.002)
+    c.body.spm, c.body.spm_a, c.body.spm_b = form, 3.0, 2.0
+    c = c.replace(name=f"{c.name}_spm{form}")
+    _compare(c, 25, oracle_lib, 1e-12)
+    if form == 2:
+        cy = cases.cylinder_2d(n=(48, 56), markers=140, storage=F64, path=PATH_FUSED, density_ratio=3.0, gravity=2e-4)
+        cy.body.quat = (np.cos(0.3), 0.0, np.sin(0.3), 0.0)
+        cy.body.spm, cy.body.spm_a, cy.body.spm_b = 2, 5.0, 3.0
+        _compare(cy.replace(name="cylinder_spm2"), 25, oracle_lib, 1e-12)

@@ -78,3 +78,20 @@ def test_markers_csv(tmp_path):
     want = np.concatenate([pos, vel, gw], axis=1)
     # std::ostream's default six significant digits
     assert np.all(np.abs(rows - want) <= 1e-5 * np.abs(want) + 1e-300)
+
+
+def test_settling_history_cd_csv(tmp_path):
+    """The falling-sphere driver's C_D.csv: the body's z velocity at every save interval
+    (SphereFall.cu:424, 437), recorded from the library's FP64 body state and read back."""
+    from ip_cu_lbm_amd import settling_history, write_cd_csv
+    c = cases.falling_sphere(n=(20, 18, 30), diameter=8.0, marker_set="3", storage=F64, path=PATH_FUSED, tau=0.7,
+                             gravity=2e-4)
+    with Simulation(c) as sim:
+        vel, t = settling_history(sim, 60, 20, c.dt)
+        final = sim.body_state()["vel"][2]
+    assert len(vel) == 3 and vel[-1] == final and vel[-1] < vel[0] < 0  # falling, accelerating
+    assert np.allclose(t, [20 * c.dt, 40 * c.dt, 60 * c.dt])
+    path = tmp_path / "C_D.csv"
+    write_cd_csv(str(path), vel, t)
+    rows = np.array([[float(x) for x in l.split(",")] for l in path.read_text().splitlines()[1:]])
+    assert np.allclose(rows[:, 0], vel, rtol=1e-5) and np.allclose(rows[:, 1], t, rtol=1e-5)

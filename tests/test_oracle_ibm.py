@@ -52,9 +52,9 @@ def test_fibonacci_sphere():
     assert np.abs(p.mean(axis=0)).max() < 1e-3
 
 
-def _box_case(n=(16, 16, 16), periodic=(1, 1, 1), init=None, body_kw=None):
+def _box_case(n=(16, 16, 16), periodic=(1, 1, 1), init=None, body_kw=None, dt=1.0):
     c = cases.falling_sphere(n=n, diameter=6.0, marker_set="3", storage=F64, path=PATH_FUSED, tau=0.8, gravity=0.0)
-    c = c.replace(periodic=periodic)
+    c = c.replace(periodic=periodic, dt=dt)
     c.body.pos = (n[0] / 2 + 0.21, n[1] / 2 - 0.13, n[2] / 2 + 0.07)
     c.body.moving = 0
     for k, v in (body_kw or {}).items():
@@ -63,16 +63,18 @@ def _box_case(n=(16, 16, 16), periodic=(1, 1, 1), init=None, body_kw=None):
     return c
 
 
-def test_stress_method_uniform_pressure_gives_pressure_traction(oracle_lib):
+@pytest.mark.parametrize("dt", [1.0, 0.5])
+def test_stress_method_uniform_pressure_gives_pressure_traction(oracle_lib, dt):
     """At rest under a uniform p the populations are w p and the normal split leaves them
-    unchanged, so sigma = -p' I with p' = p times the sum of the marker's node weights; in 3D
-    the x weights are the Peskin function truncated to three nodes and not renormalised
-    (ibm.cu:146-148, 158-170), the y and z Lagrange weights sum to one.  Every marker force is then
-    Gw = -p' n rho_f c^2/3 dIBM^2 (ibm.cu:254-272), and the spread (fw1 - fw2 = 0) leaves the fluid
-    at rest."""
+    unchanged, so sigma = -p' I with p' = p times the sum of the marker's node weights (the
+    traction's second moment is over the lattice velocities c e: sum w (c e)(c e) 3/c^2 = I, for
+    c = 1 and c = 2); in 3D the x weights are the Peskin function truncated to three nodes and not
+    renormalised (ibm.cu:146-148, 158-170), the y and z Lagrange weights sum to one.  Every marker
+    force is then Gw = -p' n rho_f c^2/3 dIBM^2 (ibm.cu:254-272), and the spread (fw1 - fw2 = 0)
+    leaves the fluid at rest."""
     p0 = 2.5e-3
     c = _box_case(init={"p": np.full(16 ** 3, p0)},
-                  body_kw={"method": _abi.IB_STRESS, "kernel": _abi.IB_KERNEL_REFERENCE})
+                  body_kw={"method": _abi.IB_STRESS, "kernel": _abi.IB_KERNEL_REFERENCE}, dt=dt)
     o = oracle_lib.Oracle(c)
     o.step(1)
     G = o.markers("force")
@@ -80,33 +82,38 @@ def test_stress_method_uniform_pressure_gives_pressure_traction(oracle_lib):
     xw = o.markers("pos")[:, 0]
     ix = (o.markers("node") // (16 * 16)).astype(int)
     sx = np.array([sum(oracle_lib.peskin4(abs(x - (i + d + 0.5))) for d in (-1, 0, 1)) for x, i in zip(xw, ix)])
-    expect = -p0 * sx[:, None] * n * (c.body.rho_f / 3.0 * c.body.spacing ** 2)
-    assert np.abs(G - expect).max() < 1e-16
+    cc = c.dz / c.dt
+    expect = -p0 * sx[:, None] * n * (c.body.rho_f * cc * cc / 3.0 * c.body.spacing ** 2)
+    assert np.abs(G - expect).max() < 1e-16 * cc * cc
     assert np.abs(o.read("u")).max() < 1e-18
     b = o.body_state()
     assert np.abs(b["torque"]).max() < 1e-15  # n parallel to r
     o.close()
 
 
-@pytest.mark.parametrize("kernel", [_abi.IB_KERNEL_REFERENCE, _abi.IB_KERNEL_PESKIN4])
-def test_direct_forcing_uniform_stream(oracle_lib, kernel):
+@pytest.mark.parametrize("kernel,dt", [(_abi.IB_KERNEL_REFERENCE, 1.0), (_abi.IB_KERNEL_PESKIN4, 1.0),
+                                       (_abi.IB_KERNEL_REFERENCE, 0.5)])
+def test_direct_forcing_uniform_stream(oracle_lib, kernel, dt):
     """A fixed body in a uniform stream u0 (fully periodic box): the interpolated velocity is u0
     (weights sum to one), so each marker force is Gw = (u0/dt) rho_f dIBM^2 dz and the body force
-    their sum (IB_directForcing, ibm.cu:331-342); the fluid gains momentum -sum Gw dt."""
+    their sum (IB_directForcing, ibm.cu:331-342); the spread force 3/c^2 (dIBM/dz)^2 gw, through
+    update_velIBM's f += w dt (c e . F) (ibm.cu:352-353), takes sum Gw dt / (rho_f dz) of velocity
+    out of the fluid: the velocity of a marker's worth of cells, for c = 1 and c = 2."""
     u0 = 0.02
     N = 16 ** 3
-    c = _box_case(init={"u": np.full(N, u0)}, body_kw={"method": _abi.IB_DIRECT, "kernel": kernel, "iterations": 1})
+    c = _box_case(init={"u": np.full(N, u0)}, body_kw={"method": _abi.IB_DIRECT, "kernel": kernel, "iterations": 1},
+                  dt=dt)
     o = oracle_lib.Oracle(c)
     o.step(1)
     M = len(c.body.markers)
-    per = u0 * c.body.rho_f * c.body.spacing ** 2 * 1.0
+    per = u0 / c.dt * c.body.rho_f * c.body.spacing ** 2 * c.dz
     G = o.markers("force")
-    assert np.abs(G[:, 0] - per).max() < 1e-15
-    assert np.abs(G[:, 1:]).max() < 1e-17
-    assert abs(o.body_state()["force"][0] - M * per) < 1e-13
-    # momentum: sum of u over cells drops by sum Gw dt / rho_f (unit cell volume)
+    assert np.abs(G[:, 0] - per).max() < 1e-15 / c.dt
+    assert np.abs(G[:, 1:]).max() < 1e-17 / c.dt
+    assert abs(o.body_state()["force"][0] - M * per) < 1e-13 / c.dt
+    # momentum: the sum of u over cells drops by sum Gw dt / (rho_f dz) (unit cell volume)
     du = o.read("u").sum() - N * u0
-    assert abs(du + M * per) < 1e-12
+    assert abs(du + M * per * c.dt / (c.body.rho_f * c.dz)) < 1e-12
     o.close()
 
 

@@ -302,6 +302,68 @@ def pin_smoothed_profile(lib=None):
     return {"err": err, "ok": ok}
 
 
+def pin_smoothed_profile_library_forms(lib=None):
+    """The library's SPM_ellipse (form 2) and SPM (form 3) on a fixed body in fluid at rest, one
+    step, cells well away from the marker shell:
+    - form 2, a D2Q9 cylinder turned about y: inside the ellipse (a, b) over the body-frame x and
+      z of R d (scipy's R of the quaternion, applied untransposed as the reference writes it) the
+      fluid moves at v_B exactly, outside it stays at rest;
+    - form 3, a D3Q19 sphere moving and spinning about z with dt = dz / 2 (no gravity, no traction
+      at rest: it keeps its motion): marker velocities are zero until the first update_IBpoint, so
+      in the second step, within a - dz of the centre in x-y, the fluid moves at (dt / dz) times the
+      velocity of marker 0 after the first (F is over dz, the update adds dt F), and beyond a + dz
+      it stays at rest."""
+    res, ok = {}, True
+    # form 2
+    R0 = 10.0
+    c = cases.cylinder_2d(n=(44, 40), diameter=2 * R0, markers=240, gravity=0.0, storage=_abi.F64)
+    c.init = {}
+    b = c.body
+    b.pos = (22.3, 0.5, 19.8)
+    th = 0.6
+    b.quat = (np.cos(th / 2), 0.0, np.sin(th / 2), 0.0)
+    vB = np.array([0.01, 0.0, -0.02])
+    b.vel, b.moving = tuple(vB), 0
+    b.spm, b.spm_a, b.spm_b = 2, 6.5, 3.5
+    o = _oracle(c, lib)
+    o.step(1)
+    u = np.stack([o.read(x) for x in "uvw"], axis=1)
+    o.close()
+    X, Y, Z = c.coords()
+    d = np.stack([X - b.pos[0], Y - b.pos[1], Z - b.pos[2]], axis=1)
+    Rm = Rotation.from_quat([b.quat[1], b.quat[2], b.quat[3], b.quat[0]]).as_matrix()
+    rd = d @ Rm.T  # R d
+    dist = (rd[:, 0] / b.spm_a) ** 2 + (rd[:, 2] / b.spm_b) ** 2
+    r = np.linalg.norm(d, axis=1)
+    inside, between = (dist < 0.7) & (r < R0 - 2.5), (dist > 1.4) & (r < R0 - 2.5)
+    res["ellipse"] = (int(inside.sum()), int(between.sum()), np.abs(u[inside] - vB).max(), np.abs(u[between]).max())
+    ok = ok and inside.sum() > 30 and between.sum() > 30 and max(res["ellipse"][2:]) < 1e-15
+    # form 3
+    n, R0 = (26, 24, 20), 8.0
+    c = cases.falling_sphere(n=n, diameter=2 * R0, markers=600, tau=0.7, gravity=0.0, method=_abi.IB_STRESS,
+                             kernel=_abi.IB_KERNEL_REFERENCE).replace(lattice=19, dt=0.5)
+    c.init = {}
+    b = c.body
+    b.pos = (13.2, 11.9, 10.3)
+    b.vel, b.omega, b.moving = (0.01, -0.004, 0.02), (0.0, 0.0, 0.003), 1
+    b.spm, b.spm_a, b.spm_b = 3, 4.0, 1.0
+    o = _oracle(c, lib)
+    o.step(1)
+    v0 = o.markers("vel")[0]
+    o.step(1)
+    u = np.stack([o.read(x) for x in "uvw"], axis=1)
+    o.close()
+    X, Y, Z = c.coords()
+    d = np.stack([X - b.pos[0], Y - b.pos[1], Z - b.pos[2]], axis=1)
+    rxy, r = np.hypot(d[:, 0], d[:, 1]), np.linalg.norm(d, axis=1)
+    inside, between = (rxy < b.spm_a - c.dz) & (r < R0 - 2.5), (rxy > b.spm_a + c.dz) & (r < R0 - 2.5)
+    want = (c.dt / c.dz) * v0
+    res["circle"] = (int(inside.sum()), int(between.sum()), np.abs(u[inside] - want).max() / np.abs(want).max(),
+                     np.abs(u[between]).max())
+    ok = ok and inside.sum() > 30 and between.sum() > 30 and max(res["circle"][2:]) < 1e-13
+    return {"err": res, "ok": ok}
+
+
 # ---- reset_items_dt + reset_pressure (basic.cu:100-110, 262-273) ----------------------------
 
 def pin_dt_change(lib=None):
@@ -350,6 +412,7 @@ PINS = {
     "scalar_advection_diffusion": pin_scalar_advection_diffusion,
     "corner_scheme": pin_corner_scheme,
     "smoothed_profile": pin_smoothed_profile,
+    "smoothed_profile_library_forms": pin_smoothed_profile_library_forms,
     "dt_change": pin_dt_change,
 }
 
@@ -367,6 +430,9 @@ MUTANTS = {
     10: "smoothed_profile",            # the body rotation untransposed
     11: "dt_change",                   # p / alpha instead of p / alpha^2
     12: "surface_tension",             # the Laplacian's x term over dz^2 instead of delX^2
+    13: "smoothed_profile_library_forms",  # SPM_ellipse with the transposed rotation
+    14: "smoothed_profile_library_forms",  # SPM with the body's velocity instead of marker 0's
+    15: "smoothed_profile_library_forms",  # SPM's force over dt instead of dz
 }
 
 
