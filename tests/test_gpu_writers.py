@@ -85,7 +85,7 @@ def test_settling_history_cd_csv(tmp_path):
     (SphereFall.cu:424, 437), recorded from the library's FP64 body state and read back."""
     from ip_cu_lbm_amd import settling_history, write_cd_csv
     c = cases.falling_sphere(n=(20, 18, 30), diameter=8.0, marker_set="3", storage=F64, path=PATH_FUSED, tau=0.7,
-                             gravity=2e-4)
+                             gravity=2e-4, method=_abi.IB_STRESS, kernel=_abi.IB_KERNEL_REFERENCE, density_ratio=4.0)
     with Simulation(c) as sim:
         vel, t = settling_history(sim, 60, 20, c.dt)
         final = sim.body_state()["vel"][2]
