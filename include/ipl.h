@@ -169,6 +169,11 @@ int ipl_global_sums(ipl_ctx* ctx, double out[3]);
  * so the two return the same bits. */
 int ipl_global_sums_noalloc(ipl_ctx* ctx, double out[3]);
 
+/* Why the context runs on the plain path: the fused path's refusal under IPL_PATH_AUTO (a cell mask,
+ * the two-axis corner scheme, lines over 512 cells, a non-uniform tau, ...), or that the plain path
+ * was requested; "" on the fused path.  The string lives as long as the context. */
+const char* ipl_path_reason(ipl_ctx* ctx);
+
 /* Number of calc cells, ghost cells and population storage bytes of a context. */
 int ipl_info(ipl_ctx* ctx, int64_t* n_calc, int64_t* n_ghost, int64_t* device_bytes, int32_t* path);
 
@@ -291,8 +296,12 @@ int ipl_batch_destroy(ipl_batch* batch);       /* the members stay valid */
 const char* ipl_batch_last_error_message(ipl_batch* batch);
 
 /* Device bytes ipl_create would allocate for cfg (population, force and field buffers, tables and
- * staging), for admission against a memory budget; no device work. */
+ * staging) on the path it would take (the fused path's own checks decide, as in ipl_create), for
+ * admission against a memory budget; no device work. */
 int ipl_estimate_device_bytes(const ipl_config* cfg, int64_t* bytes);
+/* Device bytes ipl_ib_attach would add for the body ib on the grid of cfg (markers, node-block
+ * box and body state); no device work. */
+int ipl_estimate_ib_bytes(const ipl_config* cfg, const ipl_ib_config* ib, int64_t* bytes);
 /* 0: launch every step directly; 1 (default): replay captured graphs where the path supports it. */
 int ipl_set_graph(ipl_ctx* ctx, int32_t on);
 /* Captures the step graphs of the fused path now (one per ping-pong buffer parity; recorded, not

@@ -49,8 +49,12 @@ class Simulation:
         n, g, b, p = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
         self._lib.ipl_info(h, ctypes.byref(n), ctypes.byref(g), ctypes.byref(b), ctypes.byref(p))
         self.n_calc, self.n_ghost, self.device_bytes, self.path = n.value, g.value, b.value, p.value
+        # why the plain path runs this case ("" on the fused path)
+        self.path_reason = self._lib.ipl_path_reason(h).decode()
         if case.body is not None:
             self.ib_attach(case.body)
+            self._lib.ipl_info(h, None, None, ctypes.byref(b), None)
+            self.device_bytes = b.value
 
     # ---- lifecycle --------------------------------------------------------------------------
     def close(self):
@@ -280,10 +284,16 @@ def estimate_device_bytes(case: "cases.Case", library: str = None) -> int:
     cfg, _ini, keep = case.to_c()
     out = ctypes.c_int64()
     rc = lib.ipl_estimate_device_bytes(ctypes.byref(cfg), ctypes.byref(out))
+    total = int(out.value)
+    if rc == 0 and getattr(case, "body", None) is not None:
+        bcfg, bkeep = case.body.to_c()
+        rc = lib.ipl_estimate_ib_bytes(ctypes.byref(cfg), ctypes.byref(bcfg), ctypes.byref(out))
+        total += int(out.value)
+        del bkeep
     del keep
     if rc != 0:
         raise IplError(rc, "ipl_estimate_device_bytes")
-    return int(out.value)
+    return total
 
 
 def write_cd_csv(path: str, c_d, time, library: str = None):

@@ -119,6 +119,8 @@ def load_library(path: str = None):
     lib.ipl_error_string.restype = c_char_p
     lib.ipl_last_error_message.argtypes = [vp]
     lib.ipl_last_error_message.restype = c_char_p
+    lib.ipl_path_reason.argtypes = [vp]
+    lib.ipl_path_reason.restype = c_char_p
     lib.ipl_profile_steps.argtypes = [vp, c_int32, _DP]
     lib.ipl_profile_steps.restype = c_int32
     lib.ipl_stage.argtypes = [vp, c_int32]
@@ -147,6 +149,8 @@ def load_library(path: str = None):
     lib.ipl_batch_last_error_message.restype = c_char_p
     lib.ipl_estimate_device_bytes.argtypes = [POINTER(IplConfig), POINTER(c_int64)]
     lib.ipl_estimate_device_bytes.restype = c_int32
+    lib.ipl_estimate_ib_bytes.argtypes = [POINTER(IplConfig), POINTER(IplIbConfig), POINTER(c_int64)]
+    lib.ipl_estimate_ib_bytes.restype = c_int32
     lib.ipl_set_graph.argtypes = [vp, c_int32]
     lib.ipl_set_graph.restype = c_int32
     lib.ipl_prepare.argtypes = [vp]

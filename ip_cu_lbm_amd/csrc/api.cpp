@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "ibm.hpp"
 
 struct ipl_ctx {
     ipl_config cfg{};
@@ -19,6 +20,7 @@ struct ipl_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string last_error;
+    std::string path_reason;  // why the automatic choice took the plain path ("" on the fused path)
     int device = 0;
 };
 
@@ -48,6 +50,8 @@ const char* ipl_error_string(int code) {
     default: return "unknown error";
     }
 }
+
+const char* ipl_path_reason(ipl_ctx* ctx) { return ctx ? ctx->path_reason.c_str() : ""; }
 
 const char* ipl_last_error_message(ipl_ctx* ctx) {
     return ctx ? ctx->last_error.c_str() : g_create_error.c_str();
@@ -101,8 +105,12 @@ int ipl_create(const ipl_config* cfg, const ipl_init* init, ipl_ctx** out) {
     if (req == IPL_PATH_FUSED || (req == IPL_PATH_AUTO && fused_ok)) {
         ctx->engine = ipl::make_fused_engine(ctx->graph, *cfg, hs, ctx->stream, st);
         if (!ctx->engine && req == IPL_PATH_FUSED) return bail(st);
+        if (!ctx->engine) why = st.msg;
     }
-    if (!ctx->engine) ctx->engine = ipl::make_plain_engine(ctx->graph, *cfg, hs, ctx->stream, st);
+    if (!ctx->engine) {
+        ctx->path_reason = req == IPL_PATH_PLAIN ? "the plain path was requested" : "fused path: " + why;
+        ctx->engine = ipl::make_plain_engine(ctx->graph, *cfg, hs, ctx->stream, st);
+    }
     if (!ctx->engine) return bail(st ? ipl::Status::err(IPL_ERR_UNSUPPORTED, "no engine") : st);
     *out = ctx;
     return IPL_OK;
@@ -316,18 +324,21 @@ int ipl_graph_captures(ipl_ctx* ctx, int32_t* count) {
 
 int ipl_estimate_device_bytes(const ipl_config* cfg, int64_t* bytes) {
     if (!cfg || !bytes) return IPL_ERR_ARGUMENT;
-    int64_t n = (int64_t)cfg->nx * cfg->ny * cfg->nz;
-    if (cfg->solid) {
-        int64_t solid = 0;
-        for (int64_t i = 0; i < n; ++i) solid += cfg->solid[i] != 0;
-        n -= solid;
+    // the path ipl_create would take: the fused path's checks on the graph's header (no tables)
+    ipl::Graph hdr;
+    std::string why;
+    if (!ipl::graph_header(*cfg, hdr, why)) {
+        g_create_error = why;
+        return IPL_ERR_INVALID_CONFIG;
     }
+    const bool fused = cfg->path != IPL_PATH_PLAIN && ipl::fused_supported(hdr, *cfg, why);
+    const int64_t n = hdr.N;
     const int Q = cfg->lattice;
     const int Qs = cfg->scalar == IPL_SCALAR_NONE ? 0 : (cfg->scalar_lattice ? cfg->scalar_lattice : Q);
     const int64_t s = cfg->storage == IPL_F64 ? 8 : (cfg->storage == IPL_F32 ? 4 : 2);
     const int64_t m = cfg->storage == IPL_F64 ? 8 : 4;
     const int64_t per_axis = 4 * (int64_t)(cfg->nx + cfg->ny + cfg->nz) * 8;
-    if (cfg->path != IPL_PATH_PLAIN) {
+    if (fused) {
         // fused: ping-pong populations, force rows, the phase field's scalar-field and history rows,
         // FP64 cell volumes and the FP64 readout staging rows
         int64_t b = 2 * (int64_t)(Q + Qs) * s * n + 6 * m * n + 8 * n + 7 * 8 * n;
@@ -339,6 +350,14 @@ int ipl_estimate_device_bytes(const ipl_config* cfg, int64_t* bytes) {
         const int64_t nt = n + 6 * (int64_t)std::cbrt((double)n) * (int64_t)std::cbrt((double)n);
         *bytes = 5 * (int64_t)(Q + Qs) * s * nt + 4 * (int64_t)Q * nt + 20 * m * nt + per_axis + (1 << 20);
     }
+    return IPL_OK;
+}
+
+int ipl_estimate_ib_bytes(const ipl_config* cfg, const ipl_ib_config* ib, int64_t* bytes) {
+    if (!cfg || !ib || !bytes || ib->n_markers < 0 || (ib->n_markers > 0 && !ib->marker_pos)) return IPL_ERR_ARGUMENT;
+    const int n[3] = {cfg->nx, cfg->ny, cfg->nz};
+    const int dim = cfg->lattice == 9 ? 2 : 3;
+    *bytes = ipl::IbSolver::estimate_bytes(*ib, n, cfg->periodic, dim, cfg->dz);
     return IPL_OK;
 }
 

@@ -555,7 +555,9 @@ public:
             IPL_HIP_TRY(hipEventSynchronize(ev[4 * (nsteps - 1) + 2]));
             for (int s = 0; s < nsteps; ++s) {
                 float a = 0, m = 0;
-                IPL_HIP_TRY(hipEventElapsedTime(&a, ev[4 * s + 0], ev[4 * s + 1]));
+                // the first bracket holds the scalar-field pass, when the step has one; empty, it
+                // would time the two events alone
+                if (geo_.need_grad) IPL_HIP_TRY(hipEventElapsedTime(&a, ev[4 * s + 0], ev[4 * s + 1]));
                 IPL_HIP_TRY(hipEventElapsedTime(&m, ev[4 * s + 1], ev[4 * s + 2]));
                 aux_ms += a; main_ms += m;
             }
@@ -663,7 +665,7 @@ public:
         return device_all_finite<double>(f, 4, geo_.ncell, finite, stream);
     }
 
-    int64_t device_bytes() const override { return (int64_t)mem_.total(); }
+    int64_t device_bytes() const override { return (int64_t)mem_.total() + ib_.device_bytes(); }
     int path() const override { return IPL_PATH_FUSED; }
 
     size_t lds_bytes() const { return lds_bytes_; }

@@ -134,7 +134,7 @@ void fix_positions(const Graph& g, const ipl_config& cfg, int axis, double* x, i
 
 }  // namespace
 
-bool build_graph(const ipl_config& cfg, Graph& g, std::string& err) {
+bool graph_header(const ipl_config& cfg, Graph& g, std::string& err) {
     const int Q = cfg.lattice;
     if (Q != 9 && Q != 19 && Q != 27) { err = "lattice must be 9, 19 or 27"; return false; }
     g.Q = Q;
@@ -177,6 +177,26 @@ bool build_graph(const ipl_config& cfg, Graph& g, std::string& err) {
             }
         if (cfg.periodic[a] && g.n[a] < g.npts) { err = "periodic flux axis shorter than the stencil"; return false; }
     }
+    g.Qs = 0;
+    if (cfg.scalar != IPL_SCALAR_NONE) {
+        const int Qs = cfg.scalar_lattice ? cfg.scalar_lattice : Q;
+        if (!(Qs == Q || (Qs == 7 && g.dim == 3))) { err = "scalar lattice must equal the flow lattice or be 7 in 3D"; return false; }
+        if (cfg.scalar == IPL_SCALAR_PHASE_FIELD && Qs != Q) { err = "the phase field uses the flow lattice"; return false; }
+        if (g.flux_mode == FLUX_MODE_CORNER && Qs == 7) { err = "a D3Q7 scalar needs a single flux axis"; return false; }
+        g.Qs = Qs;
+    }
+    int64_t N = 0;
+    const size_t ncell = (size_t)cfg.nx * cfg.ny * cfg.nz;
+    for (size_t q = 0; q < ncell; ++q) N += !(cfg.solid && cfg.solid[q]);
+    if (N == 0) { err = "no calc cells"; return false; }
+    if (N > (int64_t)0x3fffffff) { err = "too many cells for 32-bit cell indices"; return false; }
+    g.N = N;
+    return true;
+}
+
+bool build_graph(const ipl_config& cfg, Graph& g, std::string& err) {
+    if (!graph_header(cfg, g, err)) return false;
+    const int Q = g.Q;
 
     // calc cells in lnum order
     const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
@@ -185,9 +205,6 @@ bool build_graph(const ipl_config& cfg, Graph& g, std::string& err) {
     int64_t N = 0;
     for (size_t q = 0; q < ncell; ++q)
         if (!(cfg.solid && cfg.solid[q])) lnum[q] = N++;
-    if (N == 0) { err = "no calc cells"; return false; }
-    if (N > (int64_t)0x3fffffff) { err = "too many cells for 32-bit cell indices"; return false; }
-    g.N = N;
     g.cell_ijk.resize((size_t)3 * N);
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
@@ -247,13 +264,8 @@ bool build_graph(const ipl_config& cfg, Graph& g, std::string& err) {
     }
     // slip tables
     slip_tables(g, g.dirs, g.nextB, g.nextK);
-    g.Qs = 0;
-    if (cfg.scalar != IPL_SCALAR_NONE) {
-        const int Qs = cfg.scalar_lattice ? cfg.scalar_lattice : Q;
-        if (!(Qs == Q || (Qs == 7 && g.dim == 3))) { err = "scalar lattice must equal the flow lattice or be 7 in 3D"; return false; }
-        if (cfg.scalar == IPL_SCALAR_PHASE_FIELD && Qs != Q) { err = "the phase field uses the flow lattice"; return false; }
-        if (g.flux_mode == FLUX_MODE_CORNER && Qs == 7) { err = "a D3Q7 scalar needs a single flux axis"; return false; }
-        g.Qs = Qs;
+    if (g.Qs) {
+        const int Qs = g.Qs;
         g.sdirs = make_dirs(Qs);
         slip_tables(g, g.sdirs, g.snextB, g.snextK);
     }
@@ -261,7 +273,9 @@ bool build_graph(const ipl_config& cfg, Graph& g, std::string& err) {
     const int np = g.npts;
     g.weights.assign((size_t)3 * 2 * np * N, 0.0);
     for (int a = 0; a < 3; ++a) {
-        if (!flux_axis_on[a]) continue;
+        const bool flux_axis_on = (g.flux_mode == FLUX_MODE_ONE_AXIS && g.flux_axis == a) ||
+                                  (g.flux_mode == FLUX_MODE_CORNER && a < 2);
+        if (!flux_axis_on) continue;
         g.axis_has_weights[a] = true;
         for (int si = 0; si < 2; ++si) {
             const int s = si == 0 ? 1 : -1;

@@ -53,6 +53,9 @@ struct Graph {
 
 // Builds the graph; returns false and fills err on an invalid configuration.
 bool build_graph(const ipl_config& cfg, Graph& g, std::string& err);
+// The checks and the scalar fields of build_graph (lattices, sizes, flux mode, stencil, calc-cell
+// count) without its tables: enough to decide the path and size a context.
+bool graph_header(const ipl_config& cfg, Graph& g, std::string& err);
 
 // Lagrange_interpolation (adv.cu:40-48) with the same product order.
 void lagrange_weights(double* coeff, const double* x, double xa, int n);

@@ -814,6 +814,56 @@ void rotation_host(const double* q, double* R) {
 
 IbSolver::~IbSolver() = default;
 
+// The box that holds every marker's node block: the marker nodes span at most 2 rmax / dz + 3
+// cells per axis (widths >= dz); a periodic axis may be covered whole when the body straddles its
+// seam.  The smoothed profile of SPM_ellipse3D reaches sqrt(1.1025) semi-axes from the centre: the
+// box is padded beyond the node blocks by what that exceeds of the marker radius (the library's
+// other forms run over every cell).  Returns the cell count; ext and pad receive the extents.
+int64_t IbSolver::box_extent(const ipl_ib_config& cfg, const int n[3], const int periodic[3], int dim, double dz,
+                             double rmax, int ext[3], int* pad) {
+    const bool pk4 = cfg.method == IPL_IB_DIRECT && cfg.kernel == IPL_IB_KERNEL_PESKIN4;
+    const int span = pk4 ? 4 : 2;  // reach_hi - reach_lo
+    int bpad = 0;
+    if (cfg.spm == 1) {
+        const double reach = std::sqrt(1.1025) * std::max(cfg.spm_a, cfg.spm_b);
+        bpad = std::max(0, (int)std::ceil((reach - rmax) / dz) + 1);
+    }
+    int64_t vmax = 1;
+    for (int a = 0; a < 3; ++a) {
+        int e = (int)std::ceil(2.0 * rmax / dz) + 3 + span + 2 * bpad;
+        if (dim == 2 && a == 1) e = 1;
+        if (periodic[a] || e > n[a]) e = std::min(periodic[a] ? n[a] : e, n[a]);
+        ext[a] = e;
+        vmax *= e;
+    }
+    if (pad) *pad = bpad;
+    return vmax;
+}
+
+// The device bytes attach() allocates, in the order it allocates them (each buffer at least 256 B,
+// as the arena rounds empty requests).
+int64_t IbSolver::estimate_bytes(const ipl_ib_config& cfg, const int n[3], const int periodic[3], int dim, double dz) {
+    const int64_t M = cfg.n_markers;
+    double rmax = 0;
+    for (int64_t m = 0; m < M; ++m) {
+        const double* op = cfg.marker_pos + 3 * m;
+        rmax = std::max(rmax, std::sqrt(op[0] * op[0] + op[1] * op[1] + op[2] * op[2]));
+    }
+    int ext[3];
+    const int64_t vmax = box_extent(cfg, n, periodic, dim, dz, rmax, ext, nullptr);
+    const bool pk4 = cfg.method == IPL_IB_DIRECT && cfg.kernel == IPL_IB_KERNEL_PESKIN4;
+    const int np = pk4 ? 5 : 3;
+    const int64_t S = (int64_t)np * np * (dim == 3 ? np : 1);
+    auto b = [](int64_t bytes) { return std::max<int64_t>(bytes, 256); };
+    int64_t t = 0;
+    for (int a = 0; a < 3; ++a) t += b((int64_t)(n[a] + 1) * 8);       // axis coordinates
+    t += 6 * b(3 * M * 8) + b(3 * M * 4) + b(3 * M * S * 8);            // marker arrays, nodes, slot contributions
+    if (cfg.method == IPL_IB_DIRECT) t += b(M * S * 8) + b(M * S * 4);  // first-pass slot weights and box indices
+    t += 2 * b(3 * vmax * 8) + b(vmax * 8) + b(vmax * 4);               // box force, velocity, runs, marks
+    t += b(ibk::B_SIZE * 8) + b(8 * 4) + b(M * 8);                      // body, box, sorted keys
+    return t;
+}
+
 Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::vector<double>* pos, hipStream_t st) {
     if (M_ > 0) return Status::err(IPL_ERR_ARGUMENT, "a body is already attached");
     mem_.bind(st);
@@ -880,23 +930,7 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
             node[3 * m + a] = best;
         }
     }
-    // box bound: the marker nodes span at most 2 rmax / dz + 3 cells per axis (widths >= dz);
-    // a periodic axis may be covered whole when the body straddles its seam
-    // the smoothed profile reaches sqrt(1.1025) semi-axes from the centre: pad the box beyond the
-    // marker node blocks by what that exceeds of the marker radius
-    bpad_ = 0;
-    if (cfg.spm == 1) {  // the other forms run over every cell
-        const double reach = std::sqrt(1.1025) * std::max(cfg.spm_a, cfg.spm_b);
-        bpad_ = std::max(0, (int)std::ceil((reach - rmax) / g.dz) + 1);
-    }
-    vmax_ = 1;
-    for (int a = 0; a < 3; ++a) {
-        int ext = (int)std::ceil(2.0 * rmax / g.dz) + 3 + (reach_hi_ - reach_lo_) + 2 * bpad_;
-        if (g.dim == 2 && a == 1) ext = 1;
-        if (g.periodic[a] || ext > g.n[a]) ext = std::min(g.periodic[a] ? g.n[a] : ext, g.n[a]);
-        box_max_[a] = ext;
-        vmax_ *= ext;
-    }
+    vmax_ = box_extent(cfg, g.n, g.periodic, g.dim, g.dz, rmax, box_max_, &bpad_);
     std::vector<double> body(ibk::B_SIZE, 0.0);
     for (int a = 0; a < 3; ++a) {
         body[ibk::B_POS + a] = cfg.pos[a];

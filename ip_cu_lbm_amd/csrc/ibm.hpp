@@ -59,6 +59,11 @@ public:
     Status check();
     Status read_markers(int which, double* out, size_t count);
     bool active() const { return M_ > 0; }
+    int64_t device_bytes() const { return (int64_t)mem_.total(); }
+    // device bytes attach() allocates for this body on a grid of n cells per axis
+    static int64_t estimate_bytes(const ipl_ib_config& cfg, const int n[3], const int periodic[3], int dim, double dz);
+    static int64_t box_extent(const ipl_ib_config& cfg, const int n[3], const int periodic[3], int dim, double dz,
+                              double rmax, int ext[3], int* pad);
 
     // slots of a marker's node block
     int slots() const { return S_; }

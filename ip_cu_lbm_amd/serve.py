@@ -141,6 +141,7 @@ class Server:
                 raise ValueError(f"job {j.jid} needs {j.bytes} B, more than the whole budget")
         active: List[Job] = []
         captures: List[int] = []
+        plain_jobs = []  # (job id, why the fused path refused it)
         creating = {}
         releasing = {}
         used = 0
@@ -165,6 +166,10 @@ class Server:
                         j = creating.pop(fut)
                         j.sim, dt = fut.result()
                         create += dt
+                        if getattr(j.sim, "path", _abi.PATH_FUSED) == _abi.PATH_PLAIN:
+                            plain_jobs.append((j.jid, j.sim.path_reason))
+                            if log:
+                                log(f"[serve] job {j.jid} {j.case.name} runs on the plain path: {j.sim.path_reason}")
                         j.start = time.perf_counter() - t0
                         batch.add(j.sim)
                         active.append(j)
@@ -218,4 +223,5 @@ class Server:
                 "setup_s": float(create), "close_s": float(close),
                 "aggregate_mlups": updates / makespan / 1e6, "busy_mlups": updates / busy / 1e6,
                 "cell_updates": updates, "budget_gb": self.budget / 1e9, "chunk": self.chunk,
-                "setup_workers": self.workers, "max_graph_captures_per_job": int(max(captures))}
+                "setup_workers": self.workers, "max_graph_captures_per_job": int(max(captures)),
+                "plain_path_jobs": len(plain_jobs), "plain_path_reasons": sorted({r for _, r in plain_jobs})}

@@ -8,7 +8,7 @@ between the replays and checks the state after every one of them."""
 import numpy as np
 import pytest
 
-from ip_cu_lbm_amd import F32, F64, PATH_FUSED, Batch, Simulation, _abi, cases, estimate_device_bytes, serve
+from ip_cu_lbm_amd import F32, F64, PATH_FUSED, PATH_PLAIN, Batch, Simulation, _abi, cases, estimate_device_bytes, serve
 
 pytestmark = pytest.mark.gpu
 
@@ -168,3 +168,15 @@ def test_cylinder_2d_stress_fp64(oracle_lib):
         print("cylinder", method, kernel, iters, errs, flush=True)
         assert errs[0] <= 1e-12 and errs[2] <= 1e-12 and errs[3] <= 1e-12
         assert errs[1] <= (1e-10 if iters > 1 else 1e-12)
+
+
+def test_path_reason_names_the_fallback():
+    """Under the automatic path a configuration the fused path refuses runs on the plain path, and
+    the context says why (ipl_path_reason); a fused context has no reason."""
+    base = cases.lattice_taylor_green(nx=32, nz=16)
+    solid = np.zeros(base.n, dtype=np.uint8)
+    solid[5:8, :, :3] = 1
+    with Simulation(base.replace(solid=solid.ravel(), init={})) as s:
+        assert s.path == PATH_PLAIN and "mask" in s.path_reason
+    with Simulation(base) as s:
+        assert s.path == PATH_FUSED and s.path_reason == ""

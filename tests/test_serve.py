@@ -124,3 +124,26 @@ def test_server_admission_and_release_accounting(monkeypatch):
     assert all(j.done_steps == j.steps and j.sim is None and j.finish >= j.start >= j.arrival - 1e-9 for j in jobs)
     assert res["aggregate_mlups"] > 0 and res["p99_s"] >= res["p50_s"] > 0
     assert res["max_graph_captures_per_job"] == 2
+
+
+def test_estimate_follows_the_path_ipl_create_takes():
+    """Under the automatic path the estimate uses the fused formula only where the fused path
+    accepts the configuration: a cell mask or the two-axis corner scheme falls back to the plain
+    path, and so does its estimate (a larger one)."""
+    base = cases.lattice_taylor_green(nx=64, nz=32)
+    solid = np.zeros(base.n, dtype=np.uint8)
+    solid[10:14, :, :4] = 1
+    masked = base.replace(solid=solid.ravel(), init={})
+    assert estimate_device_bytes(masked) == estimate_device_bytes(masked.replace(path=_abi.PATH_PLAIN))
+    assert estimate_device_bytes(base.replace(init={})) < estimate_device_bytes(masked)
+    corner = cases.lock_exchange(nx=12, ny=10, nz=8, scalar_lattice=0).replace(
+        lattice=27, scalar=_abi.SCALAR_NONE, flux_axes=_abi.FLUX_REFERENCE, init={})
+    assert estimate_device_bytes(corner) == estimate_device_bytes(corner.replace(path=_abi.PATH_PLAIN))
+
+
+def test_estimate_counts_the_body():
+    """A case with an immersed body adds the IB solver's markers, node-block box and body state."""
+    c = cases.falling_sphere(n=(40, 40, 60), diameter=10.0, markers=800)
+    plain = estimate_device_bytes(c.replace(body=None))
+    with_body = estimate_device_bytes(c)
+    assert with_body > plain + 800 * 3 * 8 * 6
