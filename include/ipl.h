@@ -280,11 +280,17 @@ enum ipl_ib_array { IPL_IB_POS = 0, IPL_IB_VEL = 1, IPL_IB_FORCE = 2, IPL_IB_NOR
 int ipl_ib_read_markers(ipl_ctx* ctx, int32_t which, double* out, size_t count);
 
 /* ---- batches: many independent simulations advanced together -----------------------------
- * A batch holds contexts (jobs) of any sizes and configurations.  ipl_batch_step launches the
- * steps of every member on its own stream (the fused path replays a captured HIP graph of its
- * step), so the jobs' work overlaps on the device, then waits for all of them.  Members keep
- * their own buffers: a job's results are the same alone and in any mix.  The reference runs one
- * case per process and has no counterpart; batches are what a job server steps. */
+ * A batch holds contexts (jobs) of any sizes and configurations.  With batched launches (the
+ * default) the members whose step is the fused kernel alone are grouped by kernel variant
+ * (storage, lattices, stencil, line axis, line length, LDS) and every group advances by one
+ * launch per step whose workgroups cover the lines of all its jobs (a job table on the device, the
+ * launches of two steps captured in a HIP graph that is replayed and kept while the groups stay
+ * the same).  Members with more in their step (an immersed body, LES, the wall function, an open
+ * boundary, a 3D phase field) step alone on their own streams (graph replay), overlapping with
+ * the batched launches.  Members keep their own buffers: a job's results are the same alone and
+ * in any mix.  The reference runs one case per process and has no counterpart; batches are what a
+ * job server steps. */
+enum { IPL_BATCH_LAUNCH_PER_MEMBER = 0, IPL_BATCH_LAUNCH_BATCHED = 1 };
 typedef struct ipl_batch ipl_batch;
 int ipl_batch_create(ipl_batch** out);
 int ipl_batch_add(ipl_batch* batch, ipl_ctx* ctx);
@@ -293,6 +299,12 @@ int ipl_batch_remove(ipl_batch* batch, ipl_ctx* ctx);
  * nsteps[0] when all == 1. */
 int ipl_batch_step(ipl_batch* batch, const int32_t* nsteps, int32_t all);
 int ipl_batch_destroy(ipl_batch* batch);       /* the members stay valid */
+/* IPL_BATCH_LAUNCH_BATCHED (default) or IPL_BATCH_LAUNCH_PER_MEMBER (every member on its own stream). */
+int ipl_batch_set_launch(ipl_batch* batch, int32_t launch);
+/* Totals since ipl_batch_create (diagnostics): out[0] batched kernel launches (those inside
+ * graph replays included), out[1] member-steps they advanced, out[2] member-steps launched on the
+ * members' own streams, out[3] graphs the batched launches captured. */
+int ipl_batch_counters(ipl_batch* batch, int64_t out[4]);
 const char* ipl_batch_last_error_message(ipl_batch* batch);
 
 /* Device bytes ipl_create would allocate for cfg (population, force and field buffers, tables and

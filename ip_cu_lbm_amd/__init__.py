@@ -196,10 +196,12 @@ class Simulation:
 
 
 class Batch:
-    """Independent simulations advanced together (ipl_batch_*): every member's steps are launched
-    on its own stream before the batch waits for all of them."""
+    """Independent simulations advanced together (ipl_batch_*).  batched=True (default): members
+    whose step is the fused kernel alone advance in batched launches, one per kernel variant and
+    step; the others (and every member with batched=False) launch their steps on their own
+    streams before the batch waits for all of them."""
 
-    def __init__(self, library: str = None):
+    def __init__(self, library: str = None, batched: bool = True):
         self._lib = _abi.load_library(library)
         h = ctypes.c_void_p()
         rc = self._lib.ipl_batch_create(ctypes.byref(h))
@@ -207,6 +209,21 @@ class Batch:
             raise IplError(rc, "ipl_batch_create")
         self._h = h
         self.members = []
+        rc = self._lib.ipl_batch_set_launch(self._h, 1 if batched else 0)
+        if rc != 0:
+            raise IplError(rc, "ipl_batch_set_launch")
+
+    def counters(self) -> dict:
+        """Totals since creation: batched kernel launches, member-steps they advanced, member-steps
+        launched on the members' own streams, graphs captured for the batched launches."""
+        out = (ctypes.c_int64 * 4)()
+        rc = self._lib.ipl_batch_counters(self._h, out)
+        if rc != 0:
+            raise IplError(rc, "ipl_batch_counters")
+        return {"launches": out[0], "batched_steps": out[1], "own_steps": out[2], "graph_captures": out[3]}
+
+    def graph_captures(self) -> int:
+        return self.counters()["graph_captures"]
 
     def add(self, sim: Simulation):
         rc = self._lib.ipl_batch_add(self._h, sim._h)

@@ -143,6 +143,10 @@ def load_library(path: str = None):
     lib.ipl_batch_remove.restype = c_int32
     lib.ipl_batch_step.argtypes = [vp, POINTER(c_int32), c_int32]
     lib.ipl_batch_step.restype = c_int32
+    lib.ipl_batch_set_launch.argtypes = [vp, c_int32]
+    lib.ipl_batch_set_launch.restype = c_int32
+    lib.ipl_batch_counters.argtypes = [vp, POINTER(c_int64)]
+    lib.ipl_batch_counters.restype = c_int32
     lib.ipl_batch_destroy.argtypes = [vp]
     lib.ipl_batch_destroy.restype = c_int32
     lib.ipl_batch_last_error_message.argtypes = [vp]

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.hpp"
 #include "engine.hpp"
 #include "ibm.hpp"
 
@@ -248,6 +249,9 @@ int ipl_ib_read_markers(ipl_ctx* ctx, int32_t which, double* out, size_t count) 
 struct ipl_batch {
     std::vector<ipl_ctx*> members;
     std::string last_error;
+    int launch = IPL_BATCH_LAUNCH_BATCHED;
+    int64_t own_steps = 0;  // member-steps launched on the members' own streams
+    ipl::BatchRunner runner;
 };
 
 extern "C" {
@@ -279,12 +283,32 @@ int ipl_batch_remove(ipl_batch* b, ipl_ctx* ctx) {
 int ipl_batch_step(ipl_batch* b, const int32_t* nsteps, int32_t all) {
     if (!b || !nsteps) return IPL_ERR_ARGUMENT;
     const size_t n = b->members.size();
-    // launch every member first (each on its own stream), then wait for all of them
+    for (size_t i = 0; i < n; ++i)
+        if ((all ? nsteps[0] : nsteps[i]) < 0) return IPL_ERR_ARGUMENT;
+    // members whose step is the fused kernel alone join the batched launches; the others launch
+    // their own steps, each on its own stream, first
+    std::vector<ipl::BatchMember*> batched;
+    std::vector<int> counts;
     for (size_t i = 0; i < n; ++i) {
         const int k = all ? nsteps[0] : nsteps[i];
-        if (k < 0) return IPL_ERR_ARGUMENT;
         if (k == 0) continue;
+        ipl::BatchMember* m = b->launch == IPL_BATCH_LAUNCH_BATCHED ? b->members[i]->engine->batch_member() : nullptr;
+        if (m) {
+            batched.push_back(m);
+            counts.push_back(k);
+            continue;
+        }
         ipl::Status s = b->members[i]->engine->launch_steps(k);
+        if (!s) { b->last_error = s.msg; return s.code; }
+        b->own_steps += k;
+    }
+    if (!batched.empty()) {
+        for (size_t i = 0; i < n; ++i)  // their own streams are idle before the batched launches
+            if (b->members[i]->engine->batch_member() && (all ? nsteps[0] : nsteps[i]) > 0) {
+                hipError_t e = hipStreamSynchronize(b->members[i]->stream);
+                if (e != hipSuccess) { b->last_error = hipGetErrorString(e); return IPL_ERR_HIP; }
+            }
+        ipl::Status s = b->runner.run(batched, counts);
         if (!s) { b->last_error = s.msg; return s.code; }
     }
     for (size_t i = 0; i < n; ++i) {
@@ -294,6 +318,21 @@ int ipl_batch_step(ipl_batch* b, const int32_t* nsteps, int32_t all) {
         ipl::Status s = c->engine->post_step();
         if (!s) { b->last_error = s.msg; return s.code; }
     }
+    return IPL_OK;
+}
+
+int ipl_batch_set_launch(ipl_batch* b, int32_t launch) {
+    if (!b || (launch != IPL_BATCH_LAUNCH_PER_MEMBER && launch != IPL_BATCH_LAUNCH_BATCHED)) return IPL_ERR_ARGUMENT;
+    b->launch = launch;
+    return IPL_OK;
+}
+
+int ipl_batch_counters(ipl_batch* b, int64_t* out) {
+    if (!b || !out) return IPL_ERR_ARGUMENT;
+    out[0] = b->runner.launches();
+    out[1] = b->runner.member_steps();
+    out[2] = b->own_steps;
+    out[3] = b->runner.graph_captures();
     return IPL_OK;
 }
 

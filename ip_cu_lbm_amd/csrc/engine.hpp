@@ -7,6 +7,7 @@
 #include <memory>
 #include <string>
 #include <mutex>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -92,6 +93,33 @@ struct HostState {
 };
 Status make_host_state(const Graph& g, const ipl_config& cfg, const ipl_init* init, HostState& hs);
 
+// The template arguments, block and LDS size of a fused step launch: contexts with equal keys can
+// share one batched launch (fusedk::k_fused_step_batch).
+struct BatchKey {
+    int storage = 0, Q = 0, Qs = 0, npts = 0, s = 0, flux = 0, ext = 0;
+    unsigned block = 0;
+    size_t lds = 0;
+    auto tie() const { return std::make_tuple(storage, Q, Qs, npts, s, flux, ext, block, lds); }
+    bool operator==(const BatchKey& o) const { return tie() == o.tie(); }
+    bool operator<(const BatchKey& o) const { return tie() < o.tie(); }
+};
+
+// A fused-path context whose steps can run inside a batched launch.
+class BatchMember {
+public:
+    virtual ~BatchMember() = default;
+    virtual BatchKey batch_key() const = 0;
+    virtual int64_t batch_lines() const = 0;     // workgroups (lines) of one step
+    virtual size_t batch_job_bytes() const = 0;  // size of its fusedk::BatchJob
+    // writes its fusedk::BatchJob for launches that start from the current buffer roles
+    virtual void batch_fill(void* job, int first_block) const = 0;
+    // the buffer roles after nsteps batched steps
+    virtual void batch_advance(int nsteps) = 0;
+};
+
+// One batched step launch of the jobs (device memory, njobs fusedk::BatchJob of key's types).
+Status fused_batch_launch(const BatchKey& key, const void* jobs, int njobs, int64_t blocks, int step, hipStream_t st);
+
 class Engine {
 public:
     virtual ~Engine() = default;
@@ -127,6 +155,9 @@ public:
     virtual Status ib_read_markers(int, double*, size_t) { return Status::err(IPL_ERR_ARGUMENT, "no body attached"); }
     virtual int64_t ib_marker_count() const { return 0; }
     virtual Status ib_stage(int) { return Status::err(IPL_ERR_UNSUPPORTED, "the immersed boundary runs on the fused path"); }
+    // the engine as a member of batched launches, or null when its step has more than the fused
+    // kernel (an immersed body, LES, an open boundary, the 3D phase field's scalar pass)
+    virtual BatchMember* batch_member() { return nullptr; }
     virtual Status change_dt(double) { return Status::err(IPL_ERR_UNSUPPORTED, "time-step changes run on the fused path"); }
     // vals: (p, u, v, w) per listed cell
     virtual Status set_open_boundary(const int32_t*, int64_t, const double*) {

@@ -100,7 +100,7 @@ void convert_row(int Qlat, const FGeom& G, const ST* src, double* dbuf, int q, b
 }
 
 template <typename ST, typename MT>
-class FusedEngine final : public Engine {
+class FusedEngine final : public Engine, public BatchMember {
 public:
     FusedEngine(const Graph& g, const ipl_config& cfg) : gr_(g), cfg_(cfg) {}
 
@@ -293,15 +293,48 @@ public:
         return Status::ok();
     }
 
+    // the step's buffers with the current roles (swapped: the roles of the next step)
+    FBufs<ST, MT> bufs(bool swapped) const {
+        FBufs<ST, MT> b{};
+        b.f_src = swapped ? fb_ : fa_; b.f_dst = swapped ? fa_ : fb_;
+        b.g_src = swapped ? gb_ : ga_; b.g_dst = swapped ? ga_ : gb_;
+        b.F_old = swapped ? Fb_ : Fa_; b.F_new = swapped ? Fa_ : Fb_;
+        b.rho_n = rho_n_; b.phi_n = phi_n_;
+        b.phiold = phiold_; b.uold = uold_; b.vold = vold_; b.wold = wold_;
+        b.tau = les_tau_; b.taus = les_taus_;
+        return b;
+    }
+
+    // ---- batched launches (BatchMember) ----
+    BatchMember* batch_member() override {
+        if (ib_.active() || geo_.les || geo_.wall_fn || ob_n_ > 0 || geo_.need_grad || std::is_same<ST, __half>::value)
+            return nullptr;
+        return this;
+    }
+    BatchKey batch_key() const override {
+        BatchKey k;
+        k.storage = std::is_same<ST, double>::value ? IPL_F64 : (std::is_same<ST, float>::value ? IPL_F32 : IPL_F16);
+        k.Q = gr_.Q; k.Qs = gr_.Qs; k.npts = gr_.npts; k.s = geo_.s; k.flux = geo_.flux;
+        k.ext = geo_.wall_fn || geo_.les;
+        k.block = block_; k.lds = lds_bytes_;
+        return k;
+    }
+    int64_t batch_lines() const override { return geo_.lines; }
+    size_t batch_job_bytes() const override { return sizeof(fusedk::BatchJob<ST, MT>); }
+    void batch_fill(void* job, int first_block) const override {
+        fusedk::BatchJob<ST, MT> j{};
+        j.G = geo_; j.tb = tb_; j.b[0] = bufs(false); j.b[1] = bufs(true); j.P = K_;
+        j.first_block = first_block;
+        std::memcpy(job, &j, sizeof(j));
+    }
+    void batch_advance(int nsteps) override {
+        if (nsteps & 1) { std::swap(fa_, fb_); std::swap(ga_, gb_); std::swap(Fa_, Fb_); }
+    }
+
     template <int Q, int QS>
     Status step_q(int nsteps, std::vector<hipEvent_t>* ev = nullptr) {
         for (int s = 0; s < nsteps; ++s) {
-            FBufs<ST, MT> b{};
-            b.f_src = fa_; b.f_dst = fb_; b.g_src = ga_; b.g_dst = gb_;
-            b.F_old = Fa_; b.F_new = Fb_;
-            b.rho_n = rho_n_; b.phi_n = phi_n_;
-            b.phiold = phiold_; b.uold = uold_; b.vold = vold_; b.wold = wold_;
-            b.tau = les_tau_; b.taus = les_taus_;
+            const FBufs<ST, MT> b = bufs(false);
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 0], stream));
             if (geo_.need_grad) fusedk::fused_launch_scalar_fields<ST, MT, QS>(geo_, ga_, rho_n_, phi_n_, K_, stream);
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 1], stream));
@@ -711,6 +744,25 @@ private:
 };
 
 }  // namespace
+
+Status fused_batch_launch(const BatchKey& key, const void* jobs, int njobs, int64_t blocks, int step, hipStream_t st) {
+#define IPL_BATCH(STT, MTT, QQ, QSS)                                                                                  \
+    if (key.Q == QQ && key.Qs == QSS) {                                                                               \
+        fusedk::fused_launch_step_batch<STT, MTT, QQ, QSS>((const fusedk::BatchJob<STT, MTT>*)jobs, njobs, blocks, step, key.npts, \
+                                                           key.s, key.flux != 0, key.ext != 0, key.block, key.lds, st);   \
+        IPL_HIP_TRY(hipGetLastError());                                                                                \
+        return Status::ok();                                                                                           \
+    }
+    if (key.storage == IPL_F64) {
+        IPL_BATCH(double, double, 9, 0) IPL_BATCH(double, double, 9, 9) IPL_BATCH(double, double, 19, 0)
+        IPL_BATCH(double, double, 19, 7) IPL_BATCH(double, double, 19, 19) IPL_BATCH(double, double, 27, 0)
+    } else if (key.storage == IPL_F32) {
+        IPL_BATCH(float, float, 9, 0) IPL_BATCH(float, float, 9, 9) IPL_BATCH(float, float, 19, 0)
+        IPL_BATCH(float, float, 19, 7) IPL_BATCH(float, float, 19, 19) IPL_BATCH(float, float, 27, 0)
+    }
+#undef IPL_BATCH
+    return Status::err(IPL_ERR_UNSUPPORTED, "no batched step for this lattice and storage");
+}
 
 bool fused_supported(const Graph& g, const ipl_config& cfg, std::string& why) {
     if (cfg.solid) { why = "cell masks need the plain path"; return false; }

@@ -671,15 +671,15 @@ __device__ inline MT grad_rho_inline(const FGeom& G, const LineNb& nb, const FTa
     return (hi - lo) * *elem(tb.invd[A], (uint32_t)x);
 }
 
+// One line of the step: workgroup work-item i owns cell i of line L.
 // EXT: the wall function and LES are compiled in (they add registers to every step otherwise)
 template <typename ST, typename MT, int Q, int QS, int NPTS, int S, bool FLUX, bool EXT>
-__global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGeom G, FTables<MT> tb, FBufs<ST, MT> b, FCoef<MT> P) {
-    IPL_DYNAMIC_LDS(MT, lds_f);
+__device__ __forceinline__ void fused_line(const FGeom& G, const FTables<MT>& tb, const FBufs<ST, MT>& b, const FCoef<MT>& P,
+                                           int L, MT* lds_f) {
     MT* lds_g = lds_f + (sdir_count<Q, S>() + 1) * G.ns;
     constexpr int o1 = LineAxes<S>::o1, o2 = LineAxes<S>::o2;
     const int i = threadIdx.x;
     const bool active = i < G.ns;
-    const int L = xcd_line((int)blockIdx.x, (int)G.lines);
     const int n2 = G.n[o2];
     const int c1 = L / n2, c2 = L - (L / n2) * n2;
     const int ii = active ? i : 0;
@@ -885,6 +885,31 @@ __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGe
     if constexpr (QS > 0) transport<ST, MT, QS, NPTS, S, FLUX>(G, nb, Wp, Wm, rn, b.g_dst, Pg, lds_g, ii, active);
 }
 
+template <typename ST, typename MT, int Q, int QS, int NPTS, int S, bool FLUX, bool EXT>
+__global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGeom G, FTables<MT> tb, FBufs<ST, MT> b, FCoef<MT> P) {
+    IPL_DYNAMIC_LDS(MT, lds_f);
+    fused_line<ST, MT, Q, QS, NPTS, S, FLUX, EXT>(G, tb, b, P, xcd_line((int)blockIdx.x, (int)G.lines), lds_f);
+}
+
+// The step of many simulations in one launch: the workgroups of job j are blocks
+// jobs[j].first_block .. + lines - 1 (one per line, in the job's XCD order), and its buffer roles
+// at this launch are b[step & 1] (the roles swap every step).  Jobs of one launch share the
+// template arguments; each keeps its own geometry, tables, buffers and coefficients, so every line
+// computes exactly what the job's own k_fused_step launch would.
+template <typename ST, typename MT, int Q, int QS, int NPTS, int S, bool FLUX, bool EXT>
+__global__ void __launch_bounds__(kMaxLine) k_fused_step_batch(const BatchJob<ST, MT>* __restrict__ jobs, int njobs, int step) {
+    IPL_DYNAMIC_LDS(MT, lds_f);
+    const int bid = (int)blockIdx.x;
+    int lo = 0, hi = njobs - 1;  // the last job whose first block is at or before this one (uniform)
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].first_block <= bid) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchJob<ST, MT>& J = jobs[lo];
+    fused_line<ST, MT, Q, QS, NPTS, S, FLUX, EXT>(J.G, J.tb, J.b[step & 1], J.P, xcd_line(bid - J.first_block, (int)J.G.lines), lds_f);
+}
+
 // rho (and phi) of step n for the gradients of the next fused step; V consecutive cells per
 // work-item with V-wide loads when the rows allow it (ncell a multiple of V).  The host takes
 // V = 4 on large grids and fewer cells per work-item where that would leave the chip with too few
@@ -1018,6 +1043,29 @@ void fused_launch_step(const FGeom& G, const FTables<MT>& tb, const FBufs<ST, MT
         else { if (G.s == 0) IPL_FSTEP(3, 0, true); else if (G.s == 1) IPL_FSTEP(3, 1, true); else IPL_FSTEP(3, 2, true); }
     }
 #undef IPL_FSTEP
+}
+
+template <typename ST, typename MT, int Q, int QS>
+void fused_launch_step_batch(const BatchJob<ST, MT>* jobs, int njobs, int64_t blocks, int step, int npts, int s, bool flux,
+                             bool ext, unsigned block, size_t lds, hipStream_t st) {
+    const unsigned grid = (unsigned)blocks;
+    (void)ext;  // batched jobs have no wall function or LES (their members step alone)
+#define IPL_FSTEPB(NP, SS, FL) \
+    IPL_LAUNCH_SYNC((k_fused_step_batch<ST, MT, Q, QS, NP, SS, FL, false>), grid, block, lds, st, jobs, njobs, step)
+    if (!flux) {
+        if constexpr (Lattice<Q>::dim == 2) {
+            if (s == 0) IPL_FSTEPB(5, 0, false); else IPL_FSTEPB(5, 2, false);
+        } else {
+            if (s == 0) IPL_FSTEPB(5, 0, false); else if (s == 1) IPL_FSTEPB(5, 1, false); else IPL_FSTEPB(5, 2, false);
+        }
+    } else if constexpr (Lattice<Q>::dim == 2) {
+        if (npts == 5) { if (s == 0) IPL_FSTEPB(5, 0, true); else IPL_FSTEPB(5, 2, true); }
+        else { if (s == 0) IPL_FSTEPB(3, 0, true); else IPL_FSTEPB(3, 2, true); }
+    } else {
+        if (npts == 5) { if (s == 0) IPL_FSTEPB(5, 0, true); else if (s == 1) IPL_FSTEPB(5, 1, true); else IPL_FSTEPB(5, 2, true); }
+        else { if (s == 0) IPL_FSTEPB(3, 0, true); else if (s == 1) IPL_FSTEPB(3, 1, true); else IPL_FSTEPB(3, 2, true); }
+    }
+#undef IPL_FSTEPB
 }
 
 template <typename ST, typename MT, int Q, int QS>
@@ -1177,6 +1225,8 @@ void fused_launch_convert(const FGeom& G, const ST* src, double* dst, int q, boo
                                                               unsigned, hipStream_t);                                 \
     template void ipl::fusedk::fused_launch_step<ST, MT, Q, QS>(                                             \
         const FGeom&, const FTables<MT>&, const FBufs<ST, MT>&, const FCoef<MT>&, int, unsigned, size_t, hipStream_t); \
+    template void ipl::fusedk::fused_launch_step_batch<ST, MT, Q, QS>(const BatchJob<ST, MT>*, int, int64_t, int, int, int, \
+                                                                       bool, bool, unsigned, size_t, hipStream_t);    \
     template size_t ipl::fusedk::fused_lds_bytes<ST, MT, Q, QS>(int, int);                                    \
     template void ipl::fusedk::fused_launch_readout<ST, MT, Q, QS>(const FGeom&, const ST*, const ST*, const MT*, \
                                                                     double*, const FParams&, hipStream_t);

@@ -2,9 +2,11 @@
 
 Jobs arrive at seeded Poisson times.  The loop admits waiting jobs in arrival order while their
 device-memory estimate fits the budget, keeps every admitted job's lattice resident, advances all
-of them together in chunks of steps (``Batch``: each job's steps are launched on its own stream,
-from a captured HIP graph, so their work overlaps on the device), and retires a job as soon as its
-steps are done, which frees its memory for the next arrivals (continuous batching).  It reports
+of them together in chunks of steps (``Batch``: jobs whose step is the fused kernel alone advance
+in batched launches, one launch per kernel variant and step covering the lines of all of them,
+replayed from captured HIP graphs; the others launch their own steps on their own streams, so
+their work overlaps on the device), and retires a job as soon as its steps are done, which frees
+its memory for the next arrivals (continuous batching).  It reports
 each job's completion latency (completion - arrival) and the aggregate lattice updates per second.
 
 The mix follows the serving workload: D2Q9 or D3Q19 MRT cases, 2D sides log-uniform in [64, 512],
@@ -204,6 +206,7 @@ class Server:
                 for fut in list(releasing):
                     close += fut.result()
                     used -= releasing.pop(fut).bytes
+                counters = batch.counters() if hasattr(batch, "counters") else {}
             finally:
                 for fut in releasing:
                     fut.exception()
@@ -224,4 +227,5 @@ class Server:
                 "aggregate_mlups": updates / makespan / 1e6, "busy_mlups": updates / busy / 1e6,
                 "cell_updates": updates, "budget_gb": self.budget / 1e9, "chunk": self.chunk,
                 "setup_workers": self.workers, "max_graph_captures_per_job": int(max(captures)),
-                "plain_path_jobs": len(plain_jobs), "plain_path_reasons": sorted({r for _, r in plain_jobs})}
+                "plain_path_jobs": len(plain_jobs), "plain_path_reasons": sorted({r for _, r in plain_jobs}),
+                "batch_counters": counters}

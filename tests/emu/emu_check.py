@@ -4,7 +4,9 @@ with the AddressSanitizer runtime preloaded).
 
     python tests/emu/emu_check.py LIBRARY CASE [CASE ...]
 
-Each CASE is name:storage:steps, e.g. lock_exchange:f16:150."""
+Each CASE is name:storage:steps, e.g. lock_exchange:f16:150; the case "batch" (batch:mix:0) runs
+a batch of mixed jobs with uneven step counts and compares every member with the same job stepped
+alone (bit-exact)."""
 import json
 import os
 import sys
@@ -15,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, ROOT)
 
 import oracle  # noqa: E402
-from ip_cu_lbm_amd import F16, F32, F64, PATH_FUSED, PATH_PLAIN, Simulation, cases  # noqa: E402
+from ip_cu_lbm_amd import F16, F32, F64, PATH_FUSED, PATH_PLAIN, Batch, Simulation, cases  # noqa: E402
 
 STORAGE = {"f64": F64, "f32": F32, "f16": F16}
 
@@ -39,6 +41,42 @@ def make(name, storage):
     raise KeyError(name)
 
 
+def batch_check(lib):
+    """Members of two launch groups of the batched kernel (D2Q9 FP32 at three sizes, D3Q19+D3Q7
+    FP32 at two, D2Q9 FP64), an FP16 member and an immersed-boundary member (both stepped on their
+    own streams), uneven counts with zeros and odd counts: each member against itself alone."""
+    jobs = [cases.lattice_taylor_green(nx=24, nz=16, storage=F32, path=PATH_FUSED),
+            cases.lattice_taylor_green(nx=32, nz=12, storage=F32, path=PATH_FUSED),
+            cases.lock_exchange(nx=16, ny=4, nz=8, storage=F32, path=PATH_FUSED),
+            cases.lattice_taylor_green(nx=40, nz=20, storage=F32, path=PATH_FUSED),
+            cases.lock_exchange(nx=12, ny=6, nz=10, storage=F32, path=PATH_FUSED),
+            cases.lattice_taylor_green(nx=20, nz=12, storage=F64, path=PATH_FUSED),
+            cases.lattice_taylor_green(nx=16, nz=16, storage=F16, path=PATH_FUSED),
+            make("cylinder", F32)[0]]
+    counts = [[3, 2, 4], [1, 1, 1], [4, 0, 3], [0, 5, 2], [2, 2, 1], [3, 4, 0], [2, 1, 2], [1, 2, 1]]
+    fields = ("p", "u", "w")
+    alone = []
+    for c, k in zip(jobs, counts):
+        with Simulation(c, lib) as s:
+            s.step(sum(k))
+            alone.append([s.read(f) for f in fields])
+    sims = [Simulation(c, lib) for c in jobs]
+    try:
+        with Batch(lib) as b:
+            for s in sims:
+                b.add(s)
+            for r in range(3):
+                b.step([k[r] for k in counts])
+            ctr = b.counters()
+        got = [[s.read(f) for f in fields] for s in sims]
+    finally:
+        for s in sims:
+            s.close()
+    same = [all(np.array_equal(x, y) for x, y in zip(g, a)) for g, a in zip(got, alone)]
+    return {"identical": same, "finite": bool(all(np.all(np.isfinite(x)) for g in got for x in g)), "path": 2,
+            "counters": ctr, "batched_expected": sum(sum(k) for k in counts[:6]), "own_expected": sum(sum(k) for k in counts[6:])}
+
+
 def rel(a, b):
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
@@ -49,6 +87,10 @@ def main(argv):
     out = {}
     for spec in argv[1:]:
         name, st, steps = spec.split(":")
+        if name == "batch":
+            out[spec] = batch_check(lib)
+            print(f"[emu] {spec}: {out[spec]}", file=sys.stderr, flush=True)
+            continue
         case, fields = make(name, STORAGE[st])
         if "S" not in fields and case.init.get("S") is not None and case.scalar == 0:
             case.init = {k: v for k, v in case.init.items() if k != "S"}

@@ -21,6 +21,7 @@
 #define __global__
 #define __device__
 #define __host__
+#define __forceinline__ inline
 #define __constant__
 #define __shared__ static
 #define __launch_bounds__(...)

@@ -11,7 +11,8 @@ that their host and kernel code stays examined.
 Tolerances (relative L2 against the FP64 oracle): FP64 1e-12 on every field; FP32 1e-5 on the
 velocity vector; FP16 storage (deviation form, scaled by 2^13) 2e-3 on the lock exchange's
 velocity after 150 steps and 5e-3 on the solitary wave's after 60: measured 1.1e-3 and 3.0e-3,
-recorded here so that a change of the FP16 arithmetic shows."""
+recorded here so that a change of the FP16 arithmetic shows.  The batch case steps a mix of jobs
+in batched launches (k_fused_step_batch) and requires each to end bit-identical to itself alone."""
 import json
 import os
 import subprocess
@@ -25,7 +26,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests", "emu"))
 import build_emu  # noqa: E402
 
 CASES = ["lock_exchange:f64:60", "lock_exchange:f32:60", "lock_exchange:f16:150", "solitary_wave:f64:60",
-         "solitary_wave:f16:60", "lock_exchange_plain:f64:30", "cylinder:f64:20", "box_d3q27:f64:20"]
+         "solitary_wave:f16:60", "lock_exchange_plain:f64:30", "cylinder:f64:20", "box_d3q27:f64:20", "batch:mix:0"]
 
 
 @pytest.fixture(scope="module")
@@ -46,6 +47,12 @@ def test_emulation_matches_oracle(emu_results, spec):
     e = emu_results[spec]
     assert e["finite"], spec
     st = spec.split(":")[1]
+    if spec.startswith("batch"):  # batched launches: each member bit-identical to itself alone
+        assert all(e["identical"]), e
+        c = e["counters"]
+        assert c["batched_steps"] == e["batched_expected"] and c["own_steps"] == e["own_expected"], e
+        assert 0 < c["launches"] <= c["batched_steps"], e
+        return
     if st == "f64":
         for f, v in e.items():
             if f in ("finite", "path", "v"):  # v is near zero in these flows: its own norm is round-off

@@ -116,6 +116,85 @@ def test_batch_uniform_count():
         _same(_state(a, cs["le"][1]), _state(ref, cs["le"][1]), "uniform count")
 
 
+def _launch_mix():
+    """Members of several batched-launch groups: D2Q9 FP32 at three sizes whose lines round to one
+    workgroup size and one that does not, D3Q19+D3Q7 FP32 at two sizes, D2Q9 FP64, a phase-field
+    D2Q9+D2Q9 wave, and an immersed-boundary cylinder (stepped on its own stream)."""
+    f2 = ("p", "u", "w")
+    return [(cases.lattice_taylor_green(nx=96, nz=48, storage=F32, path=PATH_FUSED), f2, True),
+            (cases.lattice_taylor_green(nx=80, nz=72, storage=F32, path=PATH_FUSED), f2, True),
+            (cases.lattice_taylor_green(nx=120, nz=40, storage=F32, path=PATH_FUSED), f2, True),
+            (cases.lattice_taylor_green(nx=300, nz=200, storage=F32, path=PATH_FUSED), f2, True),
+            (cases.lock_exchange(nx=40, ny=6, nz=18, storage=F32, path=PATH_FUSED), ("p", "u", "v", "w", "S"), True),
+            (cases.lock_exchange(nx=36, ny=10, nz=20, storage=F32, path=PATH_FUSED), ("p", "u", "v", "w", "S"), True),
+            (cases.lattice_taylor_green(nx=64, nz=40, storage=F64, path=PATH_FUSED), f2, True),
+            (cases.solitary_wave(nx=64, nz=32, x0=20, sigma=1e-4, path=PATH_FUSED), ("p", "u", "w", "phi"), True),
+            (cases.cylinder_2d(n=(72, 80), markers=160, storage=F32, path=PATH_FUSED), f2, False)]
+
+
+def test_batched_launches_match_alone():
+    """Batched launches (one k_fused_step_batch launch per group and step) against each job
+    stepped alone, bit for bit, over rounds of uneven counts with zeros and odd counts (which run
+    as phases of decreasing membership and leave the buffer roles swapped); the counters show
+    which member-steps the batched launches advanced.  Per-member launches give the same."""
+    mix = _launch_mix()
+    rounds = [[30, 25, 40, 7, 19, 1, 13, 22, 9], [0, 31, 2, 40, 40, 40, 5, 0, 11], [17, 17, 17, 17, 3, 0, 28, 30, 6]]
+    alone = []
+    for i, (case, fields, _b) in enumerate(mix):
+        with Simulation(case) as s:
+            s.step(sum(r[i] for r in rounds))
+            alone.append(_state(s, fields))
+    for batched in (True, False):
+        sims = [Simulation(c) for c, _f, _b in mix]
+        try:
+            with Batch(batched=batched) as b:
+                for s in sims:
+                    b.add(s)
+                for r in rounds:
+                    b.step(r)
+                ctr = b.counters()
+            for i, (case, fields, _b) in enumerate(mix):
+                _same(_state(sims[i], fields), alone[i], (case.name, batched))
+        finally:
+            for s in sims:
+                s.close()
+        print(batched, ctr, flush=True)
+        in_batch = sum(r[i] for r in rounds for i in range(len(mix)) if mix[i][2])
+        total = sum(sum(r) for r in rounds)
+        if batched:
+            assert ctr["batched_steps"] == in_batch and ctr["own_steps"] == total - in_batch, ctr
+            assert 0 < ctr["launches"] < ctr["batched_steps"], ctr
+        else:
+            assert ctr["batched_steps"] == 0 and ctr["launches"] == 0 and ctr["own_steps"] == total, ctr
+
+
+def test_batched_graphs_reused_per_layout():
+    """A repeated pattern of step calls (same members, same counts) reuses the graphs captured on
+    its first pass: one per phase layout; results after every call equal direct launches."""
+    mix = [m for m in _launch_mix() if m[2]][:5]
+    pattern = [[20, 20, 20, 20, 20], [8, 8, 4, 4, 8], [6, 2, 6, 2, 6]]
+    sims = [Simulation(c) for c, _f, _b in mix]
+    refs = [Simulation(c) for c, _f, _b in mix]
+    try:
+        with Batch() as b:
+            for s in sims:
+                b.add(s)
+            captures = []
+            for rep in range(3):
+                for r in pattern:
+                    b.step(r)
+                    for ref, k in zip(refs, r):
+                        ref.step(k)
+                    for i, (case, fields, _b) in enumerate(mix):
+                        _same(_state(sims[i], fields), _state(refs[i], fields), (case.name, rep, r))
+                captures.append(b.counters()["graph_captures"])
+            print(captures, b.counters(), flush=True)
+            assert captures[0] > 0 and captures[2] == captures[1] == captures[0], captures
+    finally:
+        for s in sims + refs:
+            s.close()
+
+
 def test_estimate_covers_allocation():
     """The admission estimate is within 25 % above what the context actually allocates."""
     for name, (case, _f) in _small_cases().items():
