@@ -1,7 +1,7 @@
 // batch.cpp - batched step launches of many fused-path contexts (ipl_batch_step).
 //
 // Contexts whose step is the fused kernel alone are grouped by launch key (storage, lattices,
-// stencil, stream axis, workgroup size); each group advances with one launch of k_fused_step_batch
+// stencil, stream axis, workgroup size); each group advances with one launch of k_fused_step
 // per step, whose workgroups cover the lines of every job of the group.  The job table is uploaded
 // once per phase (a run of steps with one set of members), and the launches of two steps (the
 // buffer roles swap every step) are captured in a HIP graph that is replayed.  A graph holds the

@@ -16,7 +16,7 @@ public:
     // stream, and waits for them.  The members' own streams must be idle.
     Status run(const std::vector<BatchMember*>& members, const std::vector<int>& counts);
     int graph_captures() const { return captures_; }
-    int64_t launches() const { return launches_; }        // k_fused_step_batch launches (graph replays included)
+    int64_t launches() const { return launches_; }        // k_fused_step launches (graph replays included)
     int64_t member_steps() const { return member_steps_; }  // member-steps those launches advanced
     void set_graph(bool on) { use_graph_ = on; }
 

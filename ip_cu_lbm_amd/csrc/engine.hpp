@@ -94,7 +94,7 @@ struct HostState {
 Status make_host_state(const Graph& g, const ipl_config& cfg, const ipl_init* init, HostState& hs);
 
 // The template arguments, block and LDS size of a fused step launch: contexts with equal keys can
-// share one batched launch (fusedk::k_fused_step_batch).
+// share one batched launch (fusedk::k_fused_step).
 struct BatchKey {
     int storage = 0, Q = 0, Qs = 0, npts = 0, s = 0, flux = 0, ext = 0;
     unsigned block = 0;

@@ -266,6 +266,7 @@ public:
             }
         }
         if (!(r = mem_.alloc_n(&partial_, (size_t)3 * 1024 + 3))) return r;
+        if (!(r = mem_.alloc_n(&job_dev_, 1))) return r;
         IPL_HIP_TRY(hipMemcpyAsync(vol_, vol.data(), NC * sizeof(double), hipMemcpyHostToDevice, st));
         IPL_HIP_TRY(hipStreamSynchronize(st));
         block_ = ((G.ns + 63) / 64) * 64;
@@ -334,11 +335,12 @@ public:
     template <int Q, int QS>
     Status step_q(int nsteps, std::vector<hipEvent_t>* ev = nullptr) {
         for (int s = 0; s < nsteps; ++s) {
-            const FBufs<ST, MT> b = bufs(false);
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 0], stream));
             if (geo_.need_grad) fusedk::fused_launch_scalar_fields<ST, MT, QS>(geo_, ga_, rho_n_, phi_n_, K_, stream);
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 1], stream));
-            fusedk::fused_launch_step<ST, MT, Q, QS>(geo_, tb_, b, K_, gr_.npts, block_, lds_bytes_, stream);
+            // the one-job table (sync_job) holds both parities' buffer roles
+            fusedk::fused_launch_step<ST, MT, Q, QS>(job_dev_, 1, geo_.lines, parity(), gr_.npts, geo_.s, geo_.flux,
+                                                           geo_.wall_fn || geo_.les, block_, lds_bytes_, stream);
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 2], stream));
             if (geo_.les)  // LES after update_rho, before the IB sub-step (Boegman2005run18IB.cu:338-339)
                 fusedk::fused_launch_les<ST, MT, Q, QS>(geo_, tb_, fb_, gb_, Fb_, les_vel_, les_nu_, les_tau_, les_taus_, K_,
@@ -402,7 +404,35 @@ public:
     // 0 while the buffers hold their set-up roles (fa_ is the first allocation), 1 when swapped
     int parity() const { return fa_ == f_first_ ? 0 : 1; }
 
+    // This context's one-job table as the current state defines it: b[p] holds the buffer roles of
+    // parity p, so the table stays valid while the roles swap step by step.
+    void job_now(fusedk::BatchJob<ST, MT>* j) const {
+        std::memset((void*)j, 0, sizeof(*j));
+        const FBufs<ST, MT> cur = bufs(false), nxt = bufs(true);
+        const int p = parity();
+        std::memcpy((void*)&j->G, &geo_, sizeof(geo_));
+        std::memcpy((void*)&j->tb, &tb_, sizeof(tb_));
+        std::memcpy((void*)&j->b[p], &cur, sizeof(cur));
+        std::memcpy((void*)&j->b[p ^ 1], &nxt, sizeof(nxt));
+        std::memcpy((void*)&j->P, &K_, sizeof(K_));
+    }
+
+    // Uploads the table when the state it holds has changed since the last upload (set-up, new
+    // coefficients or fields); launches queued before read the old contents (stream order).
+    Status sync_job() {
+        fusedk::BatchJob<ST, MT> j;
+        job_now(&j);
+        if (job_set_ && std::memcmp((const void*)&j, (const void*)&job_host_, sizeof(j)) == 0) return Status::ok();
+        std::memcpy((void*)&job_host_, (const void*)&j, sizeof(j));
+        IPL_HIP_TRY(hipMemcpyAsync(job_dev_, &job_host_, sizeof(j), hipMemcpyHostToDevice, stream));
+        IPL_HIP_TRY(hipStreamSynchronize(stream));  // job_host_ is the copy's source until it has run
+        job_set_ = true;
+        return Status::ok();
+    }
+
     Status launch_steps(int nsteps) override {
+        Status rj = sync_job();
+        if (!rj) return rj;
         int left = nsteps;
         if (use_graph_ && left >= kGraphSteps) {
             const int par = parity();
@@ -444,6 +474,8 @@ public:
 
     // both graphs now, so that the next steps (a timed region, a serving chunk) pay no capture
     Status prepare() override {
+        Status rj = sync_job();
+        if (!rj) return rj;
         if (!use_graph_) return Status::ok();
         for (int par = 0; par < 2; ++par)
             if (!graph_exec_[par]) {
@@ -484,6 +516,8 @@ public:
     Status ib_stage(int which) override {
         if (!ib_.active()) return Status::err(IPL_ERR_ARGUMENT, "no body attached");
         if (which == IPL_IB_STAGE_FLOW) {  // one step, launched directly, without its IB sub-step
+            Status rj = sync_job();
+            if (!rj) return rj;
             flow_only_ = true;
             Status r = dispatch([&](auto q, auto qs) { return this->template step_q<decltype(q)::value, decltype(qs)::value>(1); });
             flow_only_ = false;
@@ -578,6 +612,8 @@ public:
     }
 
     Status profile(int nsteps, double out[3]) override {
+        Status rj = sync_job();
+        if (!rj) return rj;
         std::vector<hipEvent_t> ev((size_t)4 * nsteps, nullptr);
         for (auto& e : ev) IPL_HIP_TRY(hipEventCreate(&e));
         Status r = dispatch([&](auto q, auto qs) {
@@ -722,6 +758,9 @@ private:
     bool flow_only_ = false;                               // step_q without the IB sub-step
     hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};  // per buffer parity
     const ST* f_first_ = nullptr;                          // fa_ as set up: parity 0
+    fusedk::BatchJob<ST, MT>* job_dev_ = nullptr;          // the one-job table of this context's step launches
+    fusedk::BatchJob<ST, MT> job_host_{};                  // its contents as last uploaded
+    bool job_set_ = false;
     int captures_ = 0;
 #ifdef IPL_CPU_EMULATION
     bool use_graph_ = false;
@@ -748,7 +787,7 @@ private:
 Status fused_batch_launch(const BatchKey& key, const void* jobs, int njobs, int64_t blocks, int step, hipStream_t st) {
 #define IPL_BATCH(STT, MTT, QQ, QSS)                                                                                  \
     if (key.Q == QQ && key.Qs == QSS) {                                                                               \
-        fusedk::fused_launch_step_batch<STT, MTT, QQ, QSS>((const fusedk::BatchJob<STT, MTT>*)jobs, njobs, blocks, step, key.npts, \
+        fusedk::fused_launch_step<STT, MTT, QQ, QSS>((const fusedk::BatchJob<STT, MTT>*)jobs, njobs, blocks, step, key.npts, \
                                                            key.s, key.flux != 0, key.ext != 0, key.block, key.lds, st);   \
         IPL_HIP_TRY(hipGetLastError());                                                                                \
         return Status::ok();                                                                                           \

@@ -108,12 +108,9 @@ inline FCoef<MT> make_coef(const FParams& P) {
     return k;
 }
 
-template <typename ST, typename MT, int Q, int QS>
-void fused_launch_step(const FGeom& G, const FTables<MT>& tb, const FBufs<ST, MT>& b, const FCoef<MT>& P, int npts,
-                       unsigned block, size_t lds, hipStream_t st);
-
-// One job of a batched step launch (device memory): its geometry, tables, buffer roles at even and
-// odd launches, coefficients, and the first workgroup of its lines.
+// One job of a fused step launch (device memory): its geometry, tables, buffer roles at even and
+// odd launches, coefficients, and the first workgroup of its lines.  A context stepped alone keeps
+// a table of one job; batched launches build tables of many.
 template <typename ST, typename MT>
 struct BatchJob {
     FGeom G;
@@ -124,7 +121,7 @@ struct BatchJob {
     int pad_;
 };
 template <typename ST, typename MT, int Q, int QS>
-void fused_launch_step_batch(const BatchJob<ST, MT>* jobs, int njobs, int64_t blocks, int step, int npts, int s, bool flux,
+void fused_launch_step(const BatchJob<ST, MT>* jobs, int njobs, int64_t blocks, int step, int npts, int s, bool flux,
                              bool ext, unsigned block, size_t lds, hipStream_t st);
 template <typename ST, typename MT, int Q, int QS>
 size_t fused_lds_bytes(int s, int ns);

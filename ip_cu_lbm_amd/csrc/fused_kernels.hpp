@@ -61,21 +61,22 @@ constexpr int kMaxLine = 512;  // work-items per workgroup = cells per line, at 
 
 // ---- storage conversion ------------------------------------------------------------------
 template <typename ST, typename MT> struct Store;
+// (P: a pointer to the stored type in the generic or the global address space)
 template <typename MT> struct Store<double, MT> {
-    __device__ static MT load(const double* p, double, float) { return (MT)*p; }
-    __device__ static void store(double* p, MT v, double, float) { *p = (double)v; }
+    template <typename P> __device__ static MT load(P p, double, float) { return (MT)*p; }
+    template <typename P> __device__ static void store(P p, MT v, double, float) { *p = (double)v; }
 };
 template <typename MT> struct Store<float, MT> {
-    __device__ static MT load(const float* p, double, float) { return (MT)*p; }
-    __device__ static void store(float* p, MT v, double, float) { *p = (float)v; }
+    template <typename P> __device__ static MT load(P p, double, float) { return (MT)*p; }
+    template <typename P> __device__ static void store(P p, MT v, double, float) { *p = (float)v; }
 };
 // FP16: populations are already deviations from the rest state (sum f = p ~ 0, collision.cu:11);
 // they are stored divided by their lattice weight and scaled by a power of two.
 template <typename MT> struct Store<__half, MT> {
-    __device__ static MT load(const __half* p, double w, float inv_scale) {
+    template <typename P> __device__ static MT load(P p, double w, float inv_scale) {
         return (MT)(__half2float(*p) * ((float)w * inv_scale));
     }
-    __device__ static void store(__half* p, MT v, double w, float scale) {
+    template <typename P> __device__ static void store(P p, MT v, double w, float scale) {
         *p = __float2half_rn((float)v * (scale / (float)w));
     }
 };
@@ -115,13 +116,21 @@ __device__ inline int wrap_or_clamp(int x, int n, int periodic) {
 // pointer to element idx of an array: a 32-bit byte offset from a base held in scalar registers,
 // the form of the global instructions' scalar-base addressing (the host keeps every population and
 // field array below 4 GiB, so idx counts rows as well: idx = row * ncell + cell)
-template <typename T> __device__ inline T* elem(T* row, uint32_t idx) {
+// The pointer is typed global: the batched kernel loads its arrays' bases from the job table, and
+// the compiler treats a pointer loaded from memory as generic (flat instructions, 64-bit vector
+// addresses) unless the access itself names the global address space.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define IPL_GLOBAL __attribute__((address_space(1)))
+#else
+#define IPL_GLOBAL
+#endif
+template <typename T> __device__ inline IPL_GLOBAL T* elem(T* row, uint32_t idx) {
     T* p = reinterpret_cast<T*>(reinterpret_cast<typename std::conditional<std::is_const<T>::value, const char, char>::type*>(row) +
                                 (uint64_t)(idx * (uint32_t)sizeof(T)));
 #ifdef IPL_CPU_EMULATION
     ipl_emu::check_access(row, p, sizeof(T));  // the CPU test build checks every access
 #endif
-    return p;
+    return (IPL_GLOBAL T*)p;
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  Line L of the
@@ -885,29 +894,48 @@ __device__ __forceinline__ void fused_line(const FGeom& G, const FTables<MT>& tb
     if constexpr (QS > 0) transport<ST, MT, QS, NPTS, S, FLUX>(G, nb, Wp, Wm, rn, b.g_dst, Pg, lds_g, ii, active);
 }
 
-template <typename ST, typename MT, int Q, int QS, int NPTS, int S, bool FLUX, bool EXT>
-__global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY k_fused_step(FGeom G, FTables<MT> tb, FBufs<ST, MT> b, FCoef<MT> P) {
-    IPL_DYNAMIC_LDS(MT, lds_f);
-    fused_line<ST, MT, Q, QS, NPTS, S, FLUX, EXT>(G, tb, b, P, xcd_line((int)blockIdx.x, (int)G.lines), lds_f);
-}
-
-// The step of many simulations in one launch: the workgroups of job j are blocks
+// The fused step of one or many simulations in one launch.  The workgroups of job j are blocks
 // jobs[j].first_block .. + lines - 1 (one per line, in the job's XCD order), and its buffer roles
 // at this launch are b[step & 1] (the roles swap every step).  Jobs of one launch share the
-// template arguments; each keeps its own geometry, tables, buffers and coefficients, so every line
-// computes exactly what the job's own k_fused_step launch would.
+// template arguments; each keeps its own geometry, tables, buffers and coefficients.  A context
+// stepped alone launches this kernel with a table of one job, so a job runs the same instruction
+// stream alone and in any batch, and its results are bit-identical either way.
+//
+// The table is read through the constant address space: the kernel never writes it, and loads
+// the compiler knows to be invariant are issued as early, wide scalar loads (as kernel arguments
+// are) rather than one by one next to their uses.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define IPL_CONST_AS __attribute__((address_space(4)))
+#else
+#define IPL_CONST_AS
+#endif
+template <typename T> __device__ inline T load_invariant(const IPL_CONST_AS T* p) {
+    static_assert(sizeof(T) % 4 == 0, "copied as 32-bit words");
+    T v;
+    const IPL_CONST_AS uint32_t* src = reinterpret_cast<const IPL_CONST_AS uint32_t*>(p);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = src[i];
+    return v;
+}
+
 template <typename ST, typename MT, int Q, int QS, int NPTS, int S, bool FLUX, bool EXT>
-__global__ void __launch_bounds__(kMaxLine) k_fused_step_batch(const BatchJob<ST, MT>* __restrict__ jobs, int njobs, int step) {
+__global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY
+k_fused_step(const BatchJob<ST, MT>* __restrict__ jobs, int njobs, int step) {
     IPL_DYNAMIC_LDS(MT, lds_f);
+    const IPL_CONST_AS BatchJob<ST, MT>* cj = (const IPL_CONST_AS BatchJob<ST, MT>*)jobs;
     const int bid = (int)blockIdx.x;
     int lo = 0, hi = njobs - 1;  // the last job whose first block is at or before this one (uniform)
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].first_block <= bid) lo = mid;
+        if (cj[mid].first_block <= bid) lo = mid;
         else hi = mid - 1;
     }
-    const BatchJob<ST, MT>& J = jobs[lo];
-    fused_line<ST, MT, Q, QS, NPTS, S, FLUX, EXT>(J.G, J.tb, J.b[step & 1], J.P, xcd_line(bid - J.first_block, (int)J.G.lines), lds_f);
+    const FGeom G = load_invariant(&cj[lo].G);
+    const FTables<MT> tb = load_invariant(&cj[lo].tb);
+    const FBufs<ST, MT> b = load_invariant(&cj[lo].b[step & 1]);
+    const FCoef<MT> P = load_invariant(&cj[lo].P);
+    fused_line<ST, MT, Q, QS, NPTS, S, FLUX, EXT>(G, tb, b, P, xcd_line(bid - cj[lo].first_block, (int)G.lines), lds_f);
 }
 
 // rho (and phi) of step n for the gradients of the next fused step; V consecutive cells per
@@ -1019,39 +1047,15 @@ __global__ void k_store_pops(FGeom G, const double* src, ST* dst, int q) {
 
 // ---- launchers (declared in fused_common.hpp, instantiated in fused_q*.hip) ---------------
 template <typename ST, typename MT, int Q, int QS>
-void fused_launch_step(const FGeom& G, const FTables<MT>& tb, const FBufs<ST, MT>& b, const FCoef<MT>& P, int npts,
-                       unsigned block, size_t lds, hipStream_t st) {
-    const unsigned grid = (unsigned)G.lines;
-    const bool ext = G.wall_fn || G.les;
-#define IPL_FSTEP(NP, SS, FL)                                                                                     \
-    do {                                                                                                          \
-        if (ext) IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS, FL, true>), grid, block, lds, st, G, tb, b, P);  \
-        else IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS, FL, false>), grid, block, lds, st, G, tb, b, P);     \
-    } while (0)
-    // the stream axis and the stencil are template arguments; a uniform grid has no stencil
-    if (!G.flux) {
-        if constexpr (Lattice<Q>::dim == 2) {
-            if (G.s == 0) IPL_FSTEP(5, 0, false); else IPL_FSTEP(5, 2, false);
-        } else {
-            if (G.s == 0) IPL_FSTEP(5, 0, false); else if (G.s == 1) IPL_FSTEP(5, 1, false); else IPL_FSTEP(5, 2, false);
-        }
-    } else if constexpr (Lattice<Q>::dim == 2) {
-        if (npts == 5) { if (G.s == 0) IPL_FSTEP(5, 0, true); else IPL_FSTEP(5, 2, true); }
-        else { if (G.s == 0) IPL_FSTEP(3, 0, true); else IPL_FSTEP(3, 2, true); }
-    } else {
-        if (npts == 5) { if (G.s == 0) IPL_FSTEP(5, 0, true); else if (G.s == 1) IPL_FSTEP(5, 1, true); else IPL_FSTEP(5, 2, true); }
-        else { if (G.s == 0) IPL_FSTEP(3, 0, true); else if (G.s == 1) IPL_FSTEP(3, 1, true); else IPL_FSTEP(3, 2, true); }
-    }
-#undef IPL_FSTEP
-}
-
-template <typename ST, typename MT, int Q, int QS>
-void fused_launch_step_batch(const BatchJob<ST, MT>* jobs, int njobs, int64_t blocks, int step, int npts, int s, bool flux,
+void fused_launch_step(const BatchJob<ST, MT>* jobs, int njobs, int64_t blocks, int step, int npts, int s, bool flux,
                              bool ext, unsigned block, size_t lds, hipStream_t st) {
     const unsigned grid = (unsigned)blocks;
-    (void)ext;  // batched jobs have no wall function or LES (their members step alone)
-#define IPL_FSTEPB(NP, SS, FL) \
-    IPL_LAUNCH_SYNC((k_fused_step_batch<ST, MT, Q, QS, NP, SS, FL, false>), grid, block, lds, st, jobs, njobs, step)
+    // ext: the wall function or LES is compiled in (it adds registers to every step otherwise)
+#define IPL_FSTEPB(NP, SS, FL)                                                                                        \
+    do {                                                                                                              \
+        if (ext) IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS, FL, true>), grid, block, lds, st, jobs, njobs, step); \
+        else IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS, FL, false>), grid, block, lds, st, jobs, njobs, step); \
+    } while (0)
     if (!flux) {
         if constexpr (Lattice<Q>::dim == 2) {
             if (s == 0) IPL_FSTEPB(5, 0, false); else IPL_FSTEPB(5, 2, false);
@@ -1223,9 +1227,7 @@ void fused_launch_convert(const FGeom& G, const ST* src, double* dst, int q, boo
     template void ipl::fusedk::fused_launch_les<ST, MT, Q, QS>(const FGeom&, const FTables<MT>&, const ST*, const ST*, \
                                                               const MT*, MT*, MT*, MT*, MT*, const FCoef<MT>&,        \
                                                               unsigned, hipStream_t);                                 \
-    template void ipl::fusedk::fused_launch_step<ST, MT, Q, QS>(                                             \
-        const FGeom&, const FTables<MT>&, const FBufs<ST, MT>&, const FCoef<MT>&, int, unsigned, size_t, hipStream_t); \
-    template void ipl::fusedk::fused_launch_step_batch<ST, MT, Q, QS>(const BatchJob<ST, MT>*, int, int64_t, int, int, int, \
+    template void ipl::fusedk::fused_launch_step<ST, MT, Q, QS>(const BatchJob<ST, MT>*, int, int64_t, int, int, int, \
                                                                        bool, bool, unsigned, size_t, hipStream_t);    \
     template size_t ipl::fusedk::fused_lds_bytes<ST, MT, Q, QS>(int, int);                                    \
     template void ipl::fusedk::fused_launch_readout<ST, MT, Q, QS>(const FGeom&, const ST*, const ST*, const MT*, \

@@ -12,7 +12,7 @@ Tolerances (relative L2 against the FP64 oracle): FP64 1e-12 on every field; FP3
 velocity vector; FP16 storage (deviation form, scaled by 2^13) 2e-3 on the lock exchange's
 velocity after 150 steps and 5e-3 on the solitary wave's after 60: measured 1.1e-3 and 3.0e-3,
 recorded here so that a change of the FP16 arithmetic shows.  The batch case steps a mix of jobs
-in batched launches (k_fused_step_batch) and requires each to end bit-identical to itself alone."""
+in batched launches (k_fused_step) and requires each to end bit-identical to itself alone."""
 import json
 import os
 import subprocess

@@ -133,7 +133,7 @@ def _launch_mix():
 
 
 def test_batched_launches_match_alone():
-    """Batched launches (one k_fused_step_batch launch per group and step) against each job
+    """Batched launches (one k_fused_step launch per group and step) against each job
     stepped alone, bit for bit, over rounds of uneven counts with zeros and odd counts (which run
     as phases of decreasing membership and leave the buffer roles swapped); the counters show
     which member-steps the batched launches advanced.  Per-member launches give the same."""
