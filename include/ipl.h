@@ -96,7 +96,11 @@ typedef struct ipl_config {
     int32_t wall_function;    /* 1: Spalding log-law wall shear on the cells next to every wall,
                                  added to F before Force (wall_function, collision.cu:28-55, called
                                  for the six faces at LockExchange.cu:214-219); fused path */
-    int32_t reserved0;
+    int32_t workgroup;        /* fused path: work-items per workgroup of the step, a multiple of 64
+                                 (at most 512) holding as many line slots (a line's cells rounded up
+                                 to whole 64-wide wavefronts) as fit; 0 = one line per workgroup.
+                                 Every size gives the same results.  No counterpart in the
+                                 reference (blocks of 64 threads there, LockExchange.cu:195-196). */
     double wall_nu;           /* kinematic viscosity of the log law (items[IDX_nu]) */
     int32_t les;              /* 1: coherent-structure LES sets tau and tau_s of every cell after
                                  each step (LES, basic.cu:285-339, called after update_rho at
@@ -301,6 +305,9 @@ int ipl_batch_step(ipl_batch* batch, const int32_t* nsteps, int32_t all);
 int ipl_batch_destroy(ipl_batch* batch);       /* the members stay valid */
 /* IPL_BATCH_LAUNCH_BATCHED (default) or IPL_BATCH_LAUNCH_PER_MEMBER (every member on its own stream). */
 int ipl_batch_set_launch(ipl_batch* batch, int32_t launch);
+/* Members whose step has more than max_lines lines (workgroups) step on their own streams even
+ * with batched launches: their launches fill the device by themselves.  0: no limit. */
+int ipl_batch_set_max_lines(ipl_batch* batch, int64_t max_lines);
 /* Totals since ipl_batch_create (diagnostics): out[0] batched kernel launches (those inside
  * graph replays included), out[1] member-steps they advanced, out[2] member-steps launched on the
  * members' own streams, out[3] graphs the batched launches captured. */

@@ -201,7 +201,7 @@ class Batch:
     step; the others (and every member with batched=False) launch their steps on their own
     streams before the batch waits for all of them."""
 
-    def __init__(self, library: str = None, batched: bool = True):
+    def __init__(self, library: str = None, batched: bool = True, max_lines: int = None):
         self._lib = _abi.load_library(library)
         h = ctypes.c_void_p()
         rc = self._lib.ipl_batch_create(ctypes.byref(h))
@@ -212,6 +212,10 @@ class Batch:
         rc = self._lib.ipl_batch_set_launch(self._h, 1 if batched else 0)
         if rc != 0:
             raise IplError(rc, "ipl_batch_set_launch")
+        if max_lines is not None:  # members with more lines step on their own streams
+            rc = self._lib.ipl_batch_set_max_lines(self._h, int(max_lines))
+            if rc != 0:
+                raise IplError(rc, "ipl_batch_set_max_lines")
 
     def counters(self) -> dict:
         """Totals since creation: batched kernel launches, member-steps they advanced, member-steps

@@ -53,7 +53,7 @@ class IplConfig(ctypes.Structure):
         ("boussinesq", c_int32), ("sigma", c_double), ("pf_thickness", c_double),
         ("scalar", c_int32), ("scalar_lattice", c_int32), ("storage", c_int32), ("path", c_int32),
         ("f16_scale", c_double), ("check_finite", c_int32), ("pf_tau", c_double),
-        ("wall_function", c_int32), ("reserved0", c_int32), ("wall_nu", c_double),
+        ("wall_function", c_int32), ("workgroup", c_int32), ("wall_nu", c_double),
         ("les", c_int32), ("line_axis", c_int32), ("mu_L", c_double), ("mu_H", c_double),
     ]
 
@@ -145,6 +145,8 @@ def load_library(path: str = None):
     lib.ipl_batch_step.restype = c_int32
     lib.ipl_batch_set_launch.argtypes = [vp, c_int32]
     lib.ipl_batch_set_launch.restype = c_int32
+    lib.ipl_batch_set_max_lines.argtypes = [vp, c_int64]
+    lib.ipl_batch_set_max_lines.restype = c_int32
     lib.ipl_batch_counters.argtypes = [vp, POINTER(c_int64)]
     lib.ipl_batch_counters.restype = c_int32
     lib.ipl_batch_destroy.argtypes = [vp]

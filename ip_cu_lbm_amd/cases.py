@@ -117,6 +117,7 @@ class Case:
     mu_L: float = 0.0
     mu_H: float = 0.0
     line_axis: int = 0         # fused path, uniform grids: 1/2/3 = lines along x/y/z, 0 = automatic
+    workgroup: int = 0         # fused path: work-items per workgroup (multiple of 64, <= 512), 0 = automatic
     solid: Optional[np.ndarray] = None
     init: Dict[str, np.ndarray] = dataclasses.field(default_factory=dict)
     steps: int = 100
@@ -183,6 +184,7 @@ class Case:
         cfg.wall_nu = self.wall_nu
         cfg.les = self.les
         cfg.line_axis = self.line_axis
+        cfg.workgroup = self.workgroup
         cfg.mu_L = self.mu_L
         cfg.mu_H = self.mu_H
         ini = _abi.IplInit()

@@ -250,7 +250,14 @@ struct ipl_batch {
     std::vector<ipl_ctx*> members;
     std::string last_error;
     int launch = IPL_BATCH_LAUNCH_BATCHED;
+    int64_t max_lines = 0;  // members with more lines step alone (0: no limit)
     int64_t own_steps = 0;  // member-steps launched on the members' own streams
+
+    ipl::BatchMember* batched_member(size_t i) const {
+        if (launch != IPL_BATCH_LAUNCH_BATCHED) return nullptr;
+        ipl::BatchMember* m = members[i]->engine->batch_member();
+        return m && (max_lines == 0 || m->batch_lines() <= max_lines) ? m : nullptr;
+    }
     ipl::BatchRunner runner;
 };
 
@@ -292,7 +299,7 @@ int ipl_batch_step(ipl_batch* b, const int32_t* nsteps, int32_t all) {
     for (size_t i = 0; i < n; ++i) {
         const int k = all ? nsteps[0] : nsteps[i];
         if (k == 0) continue;
-        ipl::BatchMember* m = b->launch == IPL_BATCH_LAUNCH_BATCHED ? b->members[i]->engine->batch_member() : nullptr;
+        ipl::BatchMember* m = b->batched_member(i);
         if (m) {
             batched.push_back(m);
             counts.push_back(k);
@@ -304,7 +311,7 @@ int ipl_batch_step(ipl_batch* b, const int32_t* nsteps, int32_t all) {
     }
     if (!batched.empty()) {
         for (size_t i = 0; i < n; ++i)  // their own streams are idle before the batched launches
-            if (b->members[i]->engine->batch_member() && (all ? nsteps[0] : nsteps[i]) > 0) {
+            if (b->batched_member(i) && (all ? nsteps[0] : nsteps[i]) > 0) {
                 hipError_t e = hipStreamSynchronize(b->members[i]->stream);
                 if (e != hipSuccess) { b->last_error = hipGetErrorString(e); return IPL_ERR_HIP; }
             }
@@ -324,6 +331,12 @@ int ipl_batch_step(ipl_batch* b, const int32_t* nsteps, int32_t all) {
 int ipl_batch_set_launch(ipl_batch* b, int32_t launch) {
     if (!b || (launch != IPL_BATCH_LAUNCH_PER_MEMBER && launch != IPL_BATCH_LAUNCH_BATCHED)) return IPL_ERR_ARGUMENT;
     b->launch = launch;
+    return IPL_OK;
+}
+
+int ipl_batch_set_max_lines(ipl_batch* b, int64_t max_lines) {
+    if (!b || max_lines < 0) return IPL_ERR_ARGUMENT;
+    b->max_lines = max_lines;
     return IPL_OK;
 }
 

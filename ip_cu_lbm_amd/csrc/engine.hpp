@@ -104,15 +104,24 @@ struct BatchKey {
     bool operator<(const BatchKey& o) const { return tie() < o.tie(); }
 };
 
+// Workgroup of the fused step (fused.hip): `lines` line slots of `slot` work-items (whole
+// wavefronts) per workgroup of `items` work-items.
+struct WorkgroupShape {
+    int lines = 1, slot = 64, items = 64;
+};
+WorkgroupShape workgroup_shape(int ns, int requested, size_t line_lds_bytes);
+WorkgroupShape batch_workgroup_shape(int ns, size_t line_lds_bytes);
+
 // A fused-path context whose steps can run inside a batched launch.
 class BatchMember {
 public:
     virtual ~BatchMember() = default;
     virtual BatchKey batch_key() const = 0;
-    virtual int64_t batch_lines() const = 0;     // workgroups (lines) of one step
+    virtual int64_t batch_lines() const = 0;     // workgroups of one step
     virtual size_t batch_job_bytes() const = 0;  // size of its fusedk::BatchJob
-    // writes its fusedk::BatchJob for launches that start from the current buffer roles
-    virtual void batch_fill(void* job, int first_block) const = 0;
+    // writes its fusedk::BatchJob for launches that start from the current buffer roles, or from
+    // the swapped ones (after an odd number of steps not yet applied by batch_advance)
+    virtual void batch_fill(void* job, int first_block, bool swapped) const = 0;
     // the buffer roles after nsteps batched steps
     virtual void batch_advance(int nsteps) = 0;
 };

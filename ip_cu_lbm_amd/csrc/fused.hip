@@ -16,10 +16,39 @@
 
 namespace ipl {
 
+// Workgroup of the fused step: `lines` slots of `slot` work-items (a line of ns cells rounded up
+// to whole wavefronts) in `items` work-items.  A requested size (a multiple of 64, at most 512, at
+// least one slot) holds as many slots as fit it; the automatic one is one slot.  Batched launches
+// take the smallest multiple of the slot from 256 work-items up (at most 512), so jobs of lines up
+// to 128 cells, whatever their lengths, share launches of 256.  Every shape keeps the lines' LDS
+// rows within 64 KiB.
+WorkgroupShape workgroup_shape(int ns, int requested, size_t line_lds_bytes) {
+    WorkgroupShape w;
+    w.slot = ((ns + 63) / 64) * 64;
+    const int budget = (int)std::max<size_t>(1, (64 * 1024) / std::max<size_t>(line_lds_bytes, 1));
+    if (requested > 0) {
+        w.items = requested;
+        w.lines = std::max(1, std::min(requested / w.slot, budget));
+    } else {
+        w.lines = 1;
+        w.items = w.slot;
+    }
+    return w;
+}
+
+WorkgroupShape batch_workgroup_shape(int ns, size_t line_lds_bytes) {
+    WorkgroupShape w;
+    w.slot = ((ns + 63) / 64) * 64;
+    const int budget = (int)std::max<size_t>(1, (64 * 1024) / std::max<size_t>(line_lds_bytes, 1));
+    w.lines = std::max(1, std::min(w.slot >= 256 ? 1 : (256 + w.slot - 1) / w.slot, budget));
+    if (w.lines * w.slot > 512) w.lines = 512 / w.slot;
+    w.items = w.lines * w.slot;
+    return w;
+}
+
 // Stream axis of a uniform grid (any axis streams exactly): the one whose lines fill their
-// wavefronts best (cells / cells rounded up to 64), z on ties, then x; lines of at most 512
-// cells whose LDS rows fit the 64 KiB budget.  Measured: 48 x 40 x 72 D3Q19 step 7.9 -> 6.7 us
-// along x; full lines neutral.
+// workgroups best (cells of its lines / work-items, with the workgroup shape the step would
+// take), z on ties, then x; lines of at most 512 cells whose LDS rows fit the 64 KiB budget.
 int uniform_stream_axis(const Graph& g, const ipl_config& cfg) {
     if (cfg.line_axis >= 1 && cfg.line_axis <= 3) return cfg.line_axis - 1;  // the caller's (tuned) choice
     const int order[3] = {2, 0, 1};
@@ -29,8 +58,11 @@ int uniform_stream_axis(const Graph& g, const ipl_config& cfg) {
     for (int a : order) {
         const int n = g.n[a];
         if (n > 512 || (g.dim == 2 && a == 1)) continue;
-        if ((size_t)(g.Q + g.Qs + 2) * n * mt > 64 * 1024) continue;
-        const double fill = (double)n / (double)(((n + 63) / 64) * 64);
+        if (cfg.workgroup > 0 && n > cfg.workgroup) continue;
+        const size_t line_bytes = (size_t)(g.Q + g.Qs + 2) * n * mt;  // at least the step's rows
+        if (line_bytes > 64 * 1024) continue;
+        const WorkgroupShape w = workgroup_shape(n, cfg.workgroup, line_bytes);
+        const double fill = (double)(w.lines * n) / (double)w.items;  // cells per work-item
         if (fill > best_fill + 1e-12) { best_fill = fill; best = a; }
     }
     return best;
@@ -269,8 +301,24 @@ public:
         if (!(r = mem_.alloc_n(&job_dev_, 1))) return r;
         IPL_HIP_TRY(hipMemcpyAsync(vol_, vol.data(), NC * sizeof(double), hipMemcpyHostToDevice, st));
         IPL_HIP_TRY(hipStreamSynchronize(st));
-        block_ = ((G.ns + 63) / 64) * 64;
-        r = dispatch([&](auto q, auto qs) { return this->template lds_q<decltype(q)::value, decltype(qs)::value>(&lds_bytes_); });
+        size_t line_bytes = 0;
+        r = dispatch([&](auto q, auto qs) { return this->template lds_q<decltype(q)::value, decltype(qs)::value>(&line_bytes); });
+        if (!r) return r;
+        line_block_ = ((G.ns + 63) / 64) * 64;
+        const WorkgroupShape ws = workgroup_shape(G.ns, cfg_.workgroup, line_bytes);
+        G.tpl = ws.slot;
+        G.lpw = ws.lines;
+        G.groups = (int)((G.lines + ws.lines - 1) / ws.lines);
+        G.lds_line = (int)(line_bytes / sizeof(MT));
+        block_ = (unsigned)ws.items;
+        lds_bytes_ = line_bytes * (size_t)ws.lines;
+        // the shape of its batched launches
+        const WorkgroupShape bs = batch_workgroup_shape(G.ns, line_bytes);
+        batch_geo_ = G;
+        batch_geo_.lpw = bs.lines;
+        batch_geo_.groups = (int)((G.lines + bs.lines - 1) / bs.lines);
+        batch_block_ = (unsigned)bs.items;
+        batch_lds_ = line_bytes * (size_t)bs.lines;
         return r;
     }
 
@@ -317,14 +365,14 @@ public:
         k.storage = std::is_same<ST, double>::value ? IPL_F64 : (std::is_same<ST, float>::value ? IPL_F32 : IPL_F16);
         k.Q = gr_.Q; k.Qs = gr_.Qs; k.npts = gr_.npts; k.s = geo_.s; k.flux = geo_.flux;
         k.ext = geo_.wall_fn || geo_.les;
-        k.block = block_; k.lds = lds_bytes_;
+        k.block = batch_block_; k.lds = batch_lds_;
         return k;
     }
-    int64_t batch_lines() const override { return geo_.lines; }
+    int64_t batch_lines() const override { return batch_geo_.groups; }
     size_t batch_job_bytes() const override { return sizeof(fusedk::BatchJob<ST, MT>); }
-    void batch_fill(void* job, int first_block) const override {
+    void batch_fill(void* job, int first_block, bool swapped) const override {
         fusedk::BatchJob<ST, MT> j{};
-        j.G = geo_; j.tb = tb_; j.b[0] = bufs(false); j.b[1] = bufs(true); j.P = K_;
+        j.G = batch_geo_; j.tb = tb_; j.b[0] = bufs(swapped); j.b[1] = bufs(!swapped); j.P = K_;
         j.first_block = first_block;
         std::memcpy(job, &j, sizeof(j));
     }
@@ -339,12 +387,12 @@ public:
             if (geo_.need_grad) fusedk::fused_launch_scalar_fields<ST, MT, QS>(geo_, ga_, rho_n_, phi_n_, K_, stream);
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 1], stream));
             // the one-job table (sync_job) holds both parities' buffer roles
-            fusedk::fused_launch_step<ST, MT, Q, QS>(job_dev_, 1, geo_.lines, parity(), gr_.npts, geo_.s, geo_.flux,
+            fusedk::fused_launch_step<ST, MT, Q, QS>(job_dev_, 1, geo_.groups, parity(), gr_.npts, geo_.s, geo_.flux,
                                                            geo_.wall_fn || geo_.les, block_, lds_bytes_, stream);
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 2], stream));
             if (geo_.les)  // LES after update_rho, before the IB sub-step (Boegman2005run18IB.cu:338-339)
                 fusedk::fused_launch_les<ST, MT, Q, QS>(geo_, tb_, fb_, gb_, Fb_, les_vel_, les_nu_, les_tau_, les_taus_, K_,
-                                                         block_, stream);
+                                                         line_block_, stream);
             if (ib_.active() && !flow_only_) {
                 // IB sub-step on f^{n+1}, with the Force field of this step (SphereFall.cu:372-383)
                 Status r = ib_.substep(fb_, gb_, Fb_, std::is_same<ST, double>::value ? IPL_F64 : IPL_F32);
@@ -767,7 +815,11 @@ private:
 #else
     bool use_graph_ = true;
 #endif
-    unsigned block_ = 64;
+    unsigned block_ = 64;       // work-items per workgroup of the step (lpw lines)
+    unsigned line_block_ = 64;  // one line per workgroup (LES pass)
+    FGeom batch_geo_{};         // geo_ with the workgroup shape of batched launches
+    unsigned batch_block_ = 64;
+    size_t batch_lds_ = 0;
 
     int32_t* ob_idx_ = nullptr;   // open-boundary cells (internal index) and their (p, u, v, w)
     double* ob_val_ = nullptr;
@@ -827,6 +879,10 @@ bool fused_supported(const Graph& g, const ipl_config& cfg, std::string& why) {
     const size_t mt = cfg.storage == IPL_F64 ? 8 : 4;
     const size_t lds = (size_t)(g.Q + g.Qs + 2) * g.n[s] * mt;
     if (lds > 64 * 1024) { why = "line does not fit the LDS budget"; return false; }
+    if (cfg.workgroup != 0 && (cfg.workgroup % 64 != 0 || cfg.workgroup > 512 || cfg.workgroup < g.n[s])) {
+        why = "workgroup must be 0 (automatic) or a multiple of 64, at most 512, holding a whole line";
+        return false;
+    }
     for (int a = 0; a < 3; ++a)
         if (g.n[a] > 1 && !cfg.periodic[a] && g.n[a] < 2) { why = "axis too short"; return false; }
     return true;

@@ -17,6 +17,11 @@ struct FGeom {
     int s, o1, o2;   // stream axis and the two line axes
     int ns;          // cells per line
     int64_t lines, ncell;
+    int tpl;         // work-items per line slot, ns rounded up to whole wavefronts (64):
+                     // work-item i of a slot owns cell i of its line
+    int lpw;         // lines per workgroup: consecutive lines of the XCD order share a workgroup
+    int groups;      // workgroups of one step, ceil(lines / lpw)
+    int lds_line;    // LDS elements (of the arithmetic type) of one line's rows
     int wall_bc;
     int flux;        // 1 = interpolated flux along s
     int need_grad;   // the scalar-field pass runs (phi, and rho beside it, for the gradients)

@@ -680,15 +680,16 @@ __device__ inline MT grad_rho_inline(const FGeom& G, const LineNb& nb, const FTa
     return (hi - lo) * *elem(tb.invd[A], (uint32_t)x);
 }
 
-// One line of the step: workgroup work-item i owns cell i of line L.
+// One line of the step: work-item i of the line's slot owns cell i of line L.
 // EXT: the wall function and LES are compiled in (they add registers to every step otherwise)
+// line_ok: the work-item's slot holds a line of the job (else L is another line of the workgroup,
+// whose cells the work-item reads but never writes, and it only takes part in the barriers).
 template <typename ST, typename MT, int Q, int QS, int NPTS, int S, bool FLUX, bool EXT>
 __device__ __forceinline__ void fused_line(const FGeom& G, const FTables<MT>& tb, const FBufs<ST, MT>& b, const FCoef<MT>& P,
-                                           int L, MT* lds_f) {
+                                           int L, int i, bool line_ok, MT* lds_f) {
     MT* lds_g = lds_f + (sdir_count<Q, S>() + 1) * G.ns;
     constexpr int o1 = LineAxes<S>::o1, o2 = LineAxes<S>::o2;
-    const int i = threadIdx.x;
-    const bool active = i < G.ns;
+    const bool active = line_ok && i < G.ns;
     const int n2 = G.n[o2];
     const int c1 = L / n2, c2 = L - (L / n2) * n2;
     const int ii = active ? i : 0;
@@ -895,8 +896,9 @@ __device__ __forceinline__ void fused_line(const FGeom& G, const FTables<MT>& tb
 }
 
 // The fused step of one or many simulations in one launch.  The workgroups of job j are blocks
-// jobs[j].first_block .. + lines - 1 (one per line, in the job's XCD order), and its buffer roles
-// at this launch are b[step & 1] (the roles swap every step).  Jobs of one launch share the
+// jobs[j].first_block .. + groups - 1, in the job's XCD order; each holds lpw consecutive lines
+// of that order in slots of tpl work-items (whole wavefronts), and its buffer roles at this launch
+// are b[step & 1] (the roles swap every step).  Jobs of one launch share the
 // template arguments; each keeps its own geometry, tables, buffers and coefficients.  A context
 // stepped alone launches this kernel with a table of one job, so a job runs the same instruction
 // stream alone and in any batch, and its results are bit-identical either way.
@@ -935,7 +937,15 @@ k_fused_step(const BatchJob<ST, MT>* __restrict__ jobs, int njobs, int step) {
     const FTables<MT> tb = load_invariant(&cj[lo].tb);
     const FBufs<ST, MT> b = load_invariant(&cj[lo].b[step & 1]);
     const FCoef<MT> P = load_invariant(&cj[lo].P);
-    fused_line<ST, MT, Q, QS, NPTS, S, FLUX, EXT>(G, tb, b, P, xcd_line(bid - cj[lo].first_block, (int)G.lines), lds_f);
+    // tpl is a multiple of 64: a wavefront's work-items share their slot, so the line and all
+    // that follows from it (its neighbour lines, its walls) stay in scalar registers
+    const int slot = __builtin_amdgcn_readfirstlane((int)threadIdx.x / G.tpl);
+    const int i = (int)threadIdx.x - slot * G.tpl;
+    __builtin_assume(i >= 0 && i < kMaxLine);
+    const int first = xcd_line(bid - cj[lo].first_block, G.groups) * G.lpw;
+    const bool line_ok = slot < G.lpw && first + slot < (int)G.lines;
+    fused_line<ST, MT, Q, QS, NPTS, S, FLUX, EXT>(G, tb, b, P, line_ok ? first + slot : first, i, line_ok,
+                                                  lds_f + (line_ok ? slot : 0) * G.lds_line);
 }
 
 // rho (and phi) of step n for the gradients of the next fused step; V consecutive cells per

@@ -111,11 +111,14 @@ class Server:
     device memory) by a separate thread, also while the batch steps; its bytes return to the
     budget when the release has ended."""
 
-    def __init__(self, budget_bytes: float = 200e9, chunk: int = 400, library: str = None, workers: int = 4):
+    def __init__(self, budget_bytes: float = 200e9, chunk: int = 400, library: str = None, workers: int = 4,
+                 batched: bool = True, max_lines: int = None):
         self.budget = int(budget_bytes)
         self.chunk = chunk
         self.library = library
         self.workers = workers
+        self.batched = batched  # batched launches (else every job launches its own steps)
+        self.max_lines = max_lines  # jobs with more lines launch their own steps (None: the library's default)
 
     def _create(self, job: Job):
         t = time.perf_counter()
@@ -150,7 +153,7 @@ class Server:
         done: List[Job] = []
         busy = create = close = 0.0
         t0 = time.perf_counter()
-        with Batch(self.library) as batch, cf.ThreadPoolExecutor(max_workers=self.workers) as pool, \
+        with Batch(self.library, batched=self.batched, max_lines=self.max_lines) as batch, cf.ThreadPoolExecutor(max_workers=self.workers) as pool, \
                 cf.ThreadPoolExecutor(max_workers=1) as closer:
             try:
                 while pending or active or creating:
@@ -225,7 +228,7 @@ class Server:
                 "mean_s": float(lat.mean()), "makespan_s": float(makespan), "busy_s": float(busy),
                 "setup_s": float(create), "close_s": float(close),
                 "aggregate_mlups": updates / makespan / 1e6, "busy_mlups": updates / busy / 1e6,
-                "cell_updates": updates, "budget_gb": self.budget / 1e9, "chunk": self.chunk,
+                "cell_updates": updates, "budget_gb": self.budget / 1e9, "chunk": self.chunk, "batched": self.batched,
                 "setup_workers": self.workers, "max_graph_captures_per_job": int(max(captures)),
                 "plain_path_jobs": len(plain_jobs), "plain_path_reasons": sorted({r for _, r in plain_jobs}),
                 "batch_counters": counters}

@@ -86,7 +86,7 @@ def test_server_admission_and_release_accounting(monkeypatch):
                 state["closed"] += 1
 
     class FakeBatch:
-        def __init__(self, library=None):
+        def __init__(self, library=None, batched=True, max_lines=None):
             self.members = []
 
         def add(self, sim):
