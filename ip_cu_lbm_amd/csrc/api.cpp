@@ -36,6 +36,34 @@ thread_local std::string g_create_error;
 
 }  // namespace
 
+namespace {
+
+// The initial fields' checks of the set-up (make_host_state): finite p, u, v, w; positive tau, tau_s.
+ipl::Status check_initial_fields(const ipl::Graph& hdr, const ipl_config& cfg, const ipl_init* init) {
+    const int64_t N = hdr.N;
+    auto at = [&](const double* a, int64_t i, double dflt) { return a ? a[i] : dflt; };
+    for (int64_t i = 0; i < N; ++i) {
+        const double tau = at(init ? init->tau : nullptr, i, cfg.tau), taus = at(init ? init->taus : nullptr, i, cfg.taus);
+        if (!(tau > 0) || !(taus > 0)) return ipl::Status::err(IPL_ERR_INVALID_CONFIG, "tau and taus must be positive");
+    }
+    const double* v[4] = {init ? init->p : nullptr, init ? init->u : nullptr, init ? init->v : nullptr, init ? init->w : nullptr};
+    for (const double* a : v)
+        if (a)
+            for (int64_t i = 0; i < N; ++i)
+                if (!std::isfinite(a[i])) return ipl::Status::err(IPL_ERR_NONFINITE, "non-finite initial field");
+    return ipl::Status::ok();
+}
+
+bool uniform_rates(const ipl::Graph& hdr, const ipl_init* init) {
+    for (const double* a : {init ? init->tau : nullptr, init ? init->taus : nullptr})
+        if (a)
+            for (int64_t i = 1; i < hdr.N; ++i)
+                if (a[i] != a[0]) return false;
+    return true;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* ipl_error_string(int code) {
@@ -67,11 +95,31 @@ int ipl_create(const ipl_config* cfg, const ipl_init* init, ipl_ctx** out) {
     ctx->cfg = *cfg;
     ctx->cfg.solid = nullptr;
     for (int a = 0; a < 3; ++a) { ctx->cfg.pos[a] = nullptr; ctx->cfg.width[a] = nullptr; }
+    // the fused path needs only the header of the graph, its cells' coordinates and one line's
+    // stencil weights; the full tables (neighbours, ghosts, slip remap, per-cell weights) are
+    // built only for the plain path
     std::string err;
-    if (!ipl::build_graph(*cfg, ctx->graph, err)) {
+    ipl::Graph hdr;
+    if (!ipl::graph_header(*cfg, hdr, err)) {
         g_create_error = err;
         delete ctx;
         return IPL_ERR_INVALID_CONFIG;
+    }
+    std::string why;
+    const int req = cfg->path;
+    bool fused_ok = req != IPL_PATH_PLAIN && ipl::fused_supported(hdr, *cfg, why);
+    {
+        const ipl::Status v = check_initial_fields(hdr, *cfg, init);
+        if (!v) { g_create_error = v.msg; delete ctx; return v.code; }
+    }
+    if (fused_ok && !cfg->les && !uniform_rates(hdr, init)) {
+        fused_ok = false;
+        why = "the fused path needs uniform tau and taus";
+    }
+    if (req == IPL_PATH_FUSED && !fused_ok) {
+        g_create_error = "fused path: " + why;
+        delete ctx;
+        return IPL_ERR_UNSUPPORTED;
     }
     auto bail = [&](const ipl::Status& s) {
         g_create_error = s.msg;
@@ -95,20 +143,20 @@ int ipl_create(const ipl_config* cfg, const ipl_init* init, ipl_ctx** out) {
         ctx->axis_pos[a].assign(cfg->pos[a], cfg->pos[a] + na);
         ctx->axis_wid[a].assign(cfg->width[a], cfg->width[a] + na);
     }
-    ipl::HostState hs;
-    ipl::Status st = ipl::make_host_state(ctx->graph, *cfg, init, hs);
-    if (!st) return bail(st);
     // engines are built from the caller's config: its geometry pointers are valid for this call
-    std::string why;
-    const bool fused_ok = ipl::fused_supported(ctx->graph, *cfg, why);
-    const int req = cfg->path;
-    if (req == IPL_PATH_FUSED && !fused_ok) return bail(ipl::Status::err(IPL_ERR_UNSUPPORTED, "fused path: " + why));
-    if (req == IPL_PATH_FUSED || (req == IPL_PATH_AUTO && fused_ok)) {
-        ctx->engine = ipl::make_fused_engine(ctx->graph, *cfg, hs, ctx->stream, st);
+    ipl::Status st;
+    if (fused_ok) {
+        if (!ipl::build_box_graph(*cfg, ctx->graph, err)) return bail(ipl::Status::err(IPL_ERR_INVALID_CONFIG, err));
+        ctx->engine = ipl::make_fused_engine(ctx->graph, *cfg, init, ctx->stream, st);
         if (!ctx->engine && req == IPL_PATH_FUSED) return bail(st);
         if (!ctx->engine) why = st.msg;
     }
     if (!ctx->engine) {
+        ctx->graph = ipl::Graph();
+        if (!ipl::build_graph(*cfg, ctx->graph, err)) return bail(ipl::Status::err(IPL_ERR_INVALID_CONFIG, err));
+        ipl::HostState hs;
+        st = ipl::make_host_state(ctx->graph, *cfg, init, hs);
+        if (!st) return bail(st);
         ctx->path_reason = req == IPL_PATH_PLAIN ? "the plain path was requested" : "fused path: " + why;
         ctx->engine = ipl::make_plain_engine(ctx->graph, *cfg, hs, ctx->stream, st);
     }
@@ -451,9 +499,12 @@ int ipl_write_fields_csv(ipl_ctx* ctx, const char* path) {
     const ipl::Graph& g = ctx->graph;
     std::fprintf(fp, "num_calc,%d\n", (int)g.N);
     std::fprintf(fp, "x       ,y       ,z       ,delx    ,dely    ,pressure,salinity ,phi     ,rho     ,u       ,v       ,w       ,Fx      ,Fy      ,Fz      \n");
+    // cell positions and widths from the axes (a calc cell's are those of its indices)
+    auto pos = [&](int64_t i, int a) { return ctx->axis_pos[a][(size_t)g.cell_ijk[3 * i + a]]; };
+    auto wid = [&](int64_t i, int a) { return ctx->axis_wid[a][(size_t)g.cell_ijk[3 * i + a]]; };
     for (int64_t i = 0; i < g.N; ++i)
-        std::fprintf(fp, "%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f\n", g.pos[0][i], g.pos[1][i], g.pos[2][i],
-                     g.del[0][i], g.del[1][i], v[0][i], v[1][i], v[2][i], v[3][i], v[4][i], v[5][i], v[6][i], v[7][i],
+        std::fprintf(fp, "%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f\n", pos(i, 0), pos(i, 1), pos(i, 2),
+                     wid(i, 0), wid(i, 1), v[0][i], v[1][i], v[2][i], v[3][i], v[4][i], v[5][i], v[6][i], v[7][i],
                      v[8][i], v[9][i]);
     const bool ok = std::fclose(fp) == 0;
     return ok ? IPL_OK : fail(ctx, ipl::Status::err(IPL_ERR_ARGUMENT, std::string("write failed: ") + path));

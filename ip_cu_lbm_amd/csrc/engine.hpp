@@ -179,7 +179,7 @@ Params make_params(const ipl_config& cfg, const Graph& g);
 
 std::unique_ptr<Engine> make_plain_engine(const Graph& g, const ipl_config& cfg, const HostState& hs,
                                           hipStream_t st, Status& status);
-std::unique_ptr<Engine> make_fused_engine(const Graph& g, const ipl_config& cfg, const HostState& hs,
+std::unique_ptr<Engine> make_fused_engine(const Graph& g, const ipl_config& cfg, const ipl_init* init,
                                           hipStream_t st, Status& status);
 // Whether the fused path can run this configuration; fills why when it cannot.
 bool fused_supported(const Graph& g, const ipl_config& cfg, std::string& why);

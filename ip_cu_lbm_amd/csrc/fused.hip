@@ -4,6 +4,7 @@
 #include "hip_compat.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <algorithm>
 #include <cstring>
 #include <type_traits>
@@ -136,7 +137,8 @@ class FusedEngine final : public Engine, public BatchMember {
 public:
     FusedEngine(const Graph& g, const ipl_config& cfg) : gr_(g), cfg_(cfg) {}
 
-    Status init(const HostState& hs, hipStream_t st) {
+    // tau0, taus0: the relaxation times (uniform unless an LES pass sets them per cell)
+    Status init(const ipl_init* in, double tau0, double taus0, hipStream_t st) {
         stream = st;
         mem_.bind(st);
         FGeom& G = geo_;
@@ -177,16 +179,20 @@ public:
         for (int i = 0; i < 27; ++i) P_.rates[i] = cfg_.s_rates[i];
         P_.rho_L = cfg_.rho_L; P_.rho_H = cfg_.rho_H; P_.beta_s = cfg_.beta_s; P_.gravity = cfg_.gravity;
         P_.sigma = cfg_.sigma; P_.pf_thickness = cfg_.pf_thickness; P_.pf_tau = cfg_.pf_tau;
-        P_.tau = hs.tau[0]; P_.taus = hs.taus[0];
+        P_.tau = tau0; P_.taus = taus0;
         P_.boussinesq = cfg_.boussinesq; P_.collision = cfg_.collision;
         derive_coefficients();
         const int Q = gr_.Q, Qs = gr_.Qs;
         const int64_t NC = G.ncell, N = gr_.N;
-        // internal <-> canonical permutation
+        // canonical -> internal cell index (a box: canonical cell ((x * ny) + y) * nz + z)
         perm_.resize(NC);
-        for (int64_t id = 0; id < N; ++id) {
-            int c[3] = {gr_.cell_ijk[3 * id], gr_.cell_ijk[3 * id + 1], gr_.cell_ijk[3 * id + 2]};
-            perm_[id] = ((int64_t)c[G.o1] * G.n[G.o2] + c[G.o2]) * G.ns + c[G.s];
+        {
+            int64_t id = 0;
+            int c[3];
+            for (c[0] = 0; c[0] < G.n[0]; ++c[0])
+                for (c[1] = 0; c[1] < G.n[1]; ++c[1])
+                    for (c[2] = 0; c[2] < G.n[2]; ++c[2], ++id)
+                        perm_[id] = ((int64_t)c[G.o1] * G.n[G.o2] + c[G.o2]) * G.ns + c[G.s];
         }
         Status r;
         // populations
@@ -207,33 +213,31 @@ public:
         }
         if (cfg_.scalar == IPL_SCALAR_PHASE_FIELD) {
             MT** pf[4] = {&phiold_, &uold_, &vold_, &wold_};
-            const std::vector<double>* src[4] = {&hs.phi, &hs.u, &hs.v, &hs.w};
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < 4; ++q)
                 if (!(r = mem_.alloc_n(pf[q], (size_t)NC))) return r;
-                std::vector<MT> t(NC);
-                for (int64_t id = 0; id < N; ++id) t[perm_[id]] = (MT)(*src[q])[id];
-                IPL_HIP_TRY(hipMemcpyAsync(*pf[q], t.data(), NC * sizeof(MT), hipMemcpyHostToDevice, st));
-                IPL_HIP_TRY(hipStreamSynchronize(st));
-            }
         }
-        // the readout buffer (7 FP64 rows) doubles as the staging row of the population upload
+        if (G.les) {  // LES rows; tau, tau_s start from the initial per-cell values
+            if (!(r = mem_.alloc_n(&les_vel_, (size_t)3 * NC))) return r;
+            if (!(r = mem_.alloc_n(&les_nu_, (size_t)NC))) return r;
+            if (!(r = mem_.alloc_n(&les_tau_, (size_t)NC))) return r;
+            if (!(r = mem_.alloc_n(&les_taus_, (size_t)NC))) return r;
+        }
+        if (!(r = mem_.alloc_n(&vol_, (size_t)NC))) return r;
+        // the readout buffer (7 FP64 rows) doubles as the staging rows of the initial fields
         if (!(r = mem_.alloc_n(&out_, (size_t)7 * NC))) return r;
-        if (!(r = upload_pops(hs.f, Q, gr_.Ntot, fa_))) return r;
-        if (Qs && !(r = upload_pops(hs.g, Qs, gr_.Ntot, ga_))) return r;
+        if (!(r = upload_initial_state(in, tau0, taus0))) return r;
         // tables
         const int np = gr_.npts;
         std::vector<MT> W((size_t)2 * np * G.ns, MT(0)), rd(G.ns, MT(1));
         if (G.flux) {
-            // weights of line 0: cells (o1=0, o2=0, s=i) in canonical order
-            for (int i = 0; i < G.ns; ++i) {
-                int c[3] = {0, 0, 0};
-                c[G.s] = i;
-                const int64_t id = ((int64_t)c[0] * gr_.n[1] + c[1]) * gr_.n[2] + c[2];
-                for (int si = 0; si < 2; ++si)
-                    for (int j = 0; j < np; ++j)
-                        W[((size_t)si * np + j) * G.ns + i] = (MT)gr_.weights[((size_t)(G.s * 2 + si) * np + j) * N + id];
-                rd[i] = (MT)(gr_.dz / cfg_widths_[G.s][i]);
-            }
+            // the stencil weights of a line along s (the same for every line of a box)
+            ipl_config lc = cfg_;
+            for (int a = 0; a < 3; ++a) { lc.pos[a] = cfg_pos_[a].data(); lc.width[a] = cfg_widths_[a].data(); }
+            std::vector<double> lw;
+            std::string err;
+            if (!axis_line_weights(lc, G.s, lw, err)) return Status::err(IPL_ERR_INVALID_CONFIG, "stencil weights: " + err);
+            for (size_t q = 0; q < W.size(); ++q) W[q] = (MT)lw[q];
+            for (int i = 0; i < G.ns; ++i) rd[i] = (MT)(gr_.dz / cfg_widths_[G.s][i]);
         }
         MT* dW; MT* drd;
         if (!(r = mem_.alloc_n(&dW, W.size()))) return r;
@@ -276,30 +280,8 @@ public:
             IPL_HIP_TRY(hipStreamSynchronize(st));
             tb_.wid[a] = dwv;
         }
-        // per-cell volume (FP64) for the conservation sums, internal order
-        std::vector<double> vol(NC);
-        for (int64_t id = 0; id < N; ++id) {
-            const int x = gr_.cell_ijk[3 * id], y = gr_.cell_ijk[3 * id + 1], z = gr_.cell_ijk[3 * id + 2];
-            vol[perm_[id]] = cfg_widths_[0][x] * cfg_widths_[1][y] * cfg_widths_[2][z] / (gr_.dz * gr_.dz * gr_.dz);
-        }
-        if (!(r = mem_.alloc_n(&vol_, (size_t)NC))) return r;
-        if (G.les) {  // LES rows; tau, tau_s start from the initial per-cell values
-            if (!(r = mem_.alloc_n(&les_vel_, (size_t)3 * NC))) return r;
-            if (!(r = mem_.alloc_n(&les_nu_, (size_t)NC))) return r;
-            if (!(r = mem_.alloc_n(&les_tau_, (size_t)NC))) return r;
-            if (!(r = mem_.alloc_n(&les_taus_, (size_t)NC))) return r;
-            MT* dst[2] = {les_tau_, les_taus_};
-            const std::vector<double>* src[2] = {&hs.tau, &hs.taus};
-            for (int q = 0; q < 2; ++q) {
-                std::vector<MT> t(NC, MT(1));
-                for (int64_t id = 0; id < N; ++id) t[perm_[id]] = (MT)(*src[q])[id];
-                IPL_HIP_TRY(hipMemcpyAsync(dst[q], t.data(), NC * sizeof(MT), hipMemcpyHostToDevice, st));
-                IPL_HIP_TRY(hipStreamSynchronize(st));
-            }
-        }
         if (!(r = mem_.alloc_n(&partial_, (size_t)3 * 1024 + 3))) return r;
         if (!(r = mem_.alloc_n(&job_dev_, 1))) return r;
-        IPL_HIP_TRY(hipMemcpyAsync(vol_, vol.data(), NC * sizeof(double), hipMemcpyHostToDevice, st));
         IPL_HIP_TRY(hipStreamSynchronize(st));
         size_t line_bytes = 0;
         r = dispatch([&](auto q, auto qs) { return this->template lds_q<decltype(q)::value, decltype(qs)::value>(&line_bytes); });
@@ -329,16 +311,43 @@ public:
         }
     }
 
-    Status upload_pops(const std::vector<double>& src, int q, int64_t Nt, ST* dst) {
-        const int64_t NC = geo_.ncell, N = gr_.N;
-        double* tmp = out_;
-        std::vector<double> h(NC);
-        for (int k = 0; k < q; ++k) {
-            for (int64_t id = 0; id < N; ++id) h[perm_[id]] = src[(size_t)k * Nt + id];
-            IPL_HIP_TRY(hipMemcpyAsync(tmp, h.data(), NC * sizeof(double), hipMemcpyHostToDevice, stream));
-            convert_row<ST, MT>(q, geo_, nullptr, tmp, k, false, dst, stream);
-            IPL_HIP_TRY(hipStreamSynchronize(stream));
+    // The caller's initial fields (canonical order) go to the staging rows as they are, and
+    // k_init_state forms the populations, volumes and history rows on the device.  Rows the caller
+    // leaves out, and tau / tau_s unless an LES pass makes them per-cell, are not uploaded.
+    Status upload_initial_state(const ipl_init* in, double tau0, double taus0) {
+        const int64_t NC = geo_.ncell;
+        const double* src[7] = {in ? in->p : nullptr, in ? in->u : nullptr, in ? in->v : nullptr, in ? in->w : nullptr,
+                                nullptr, nullptr, nullptr};
+        if (in && cfg_.scalar == IPL_SCALAR_SALINITY) src[4] = in->S;
+        if (in && cfg_.scalar == IPL_SCALAR_PHASE_FIELD) src[4] = in->phi;
+        if (in && geo_.les) { src[5] = in->tau; src[6] = in->taus; }
+        fusedk::InitArgs A{};
+        A.rows = out_;
+        const double dflt[7] = {0, 0, 0, 0, 0, tau0, taus0};
+        for (int r = 0; r < 7; ++r) {
+            A.dflt[r] = dflt[r];
+            if (!src[r]) continue;
+            A.present |= 1u << r;
+            IPL_HIP_TRY(hipMemcpyAsync(out_ + (size_t)r * NC, src[r], NC * sizeof(double), hipMemcpyHostToDevice, stream));
         }
+        double* wid = nullptr;
+        Status r;
+        if (!(r = mem_.alloc_n(&wid, (size_t)(geo_.n[0] + geo_.n[1] + geo_.n[2])))) return r;
+        for (int a = 0, off = 0; a < 3; off += geo_.n[a], ++a) {
+            IPL_HIP_TRY(hipMemcpyAsync(wid + off, cfg_widths_[a].data(), geo_.n[a] * sizeof(double), hipMemcpyHostToDevice, stream));
+            A.wid[a] = wid + off;
+        }
+        A.dz = gr_.dz;
+        A.c = gr_.c;
+        r = dispatch([&](auto q, auto qs) {
+            fusedk::fused_launch_init<ST, MT, decltype(q)::value, decltype(qs)::value>(
+                geo_, A, fa_, ga_, vol_, phiold_, uold_, vold_, wold_, les_tau_, les_taus_, stream);
+            return Status::ok();
+        });
+        if (!r) return r;
+        IPL_HIP_TRY(hipGetLastError());
+        // the host arrays may be pageable and the staging rows are reused: wait for both
+        IPL_HIP_TRY(hipStreamSynchronize(stream));
         return Status::ok();
     }
 
@@ -888,18 +897,16 @@ bool fused_supported(const Graph& g, const ipl_config& cfg, std::string& why) {
     return true;
 }
 
-std::unique_ptr<Engine> make_fused_engine(const Graph& g, const ipl_config& cfg, const HostState& hs,
+std::unique_ptr<Engine> make_fused_engine(const Graph& g, const ipl_config& cfg, const ipl_init* init,
                                           hipStream_t st, Status& status) {
-    for (int64_t i = 1; i < g.N && !cfg.les; ++i)
-        if (hs.tau[i] != hs.tau[0] || hs.taus[i] != hs.taus[0]) {
-            status = Status::err(IPL_ERR_UNSUPPORTED, "the fused path needs uniform tau and taus");
-            return nullptr;
-        }
+    // tau, tau_s: uniform (checked by the caller) unless an LES pass sets them per cell
+    const double tau0 = init && init->tau ? init->tau[0] : cfg.tau;
+    const double taus0 = init && init->taus ? init->taus[0] : cfg.taus;
     auto make = [&](auto* tag) -> std::unique_ptr<Engine> {
         using E = std::remove_pointer_t<decltype(tag)>;
         auto e = std::make_unique<E>(g, cfg);
         e->set_axes(cfg.pos, cfg.width);
-        status = e->init(hs, st);
+        status = e->init(init, tau0, taus0, st);
         if (!status) return nullptr;
         return e;
     };

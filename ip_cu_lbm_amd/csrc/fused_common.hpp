@@ -125,6 +125,20 @@ struct BatchJob {
     int first_block;
     int pad_;
 };
+// The initial fields of a context on the device (k_init_state): rows [p, u, v, w, scalar (S or
+// phi), tau, tau_s][ncell] in canonical cell order; a row whose bit is clear in `present` holds
+// dflt[r] everywhere and is not read.
+struct InitArgs {
+    const double* rows;
+    unsigned present;
+    double dflt[7];
+    const double* wid[3];  // FP64 cell widths per axis index
+    double dz, c;
+};
+
+template <typename ST, typename MT, int Q, int QS>
+void fused_launch_init(const FGeom& G, const InitArgs& A, ST* f, ST* g, double* vol, MT* phiold, MT* uold, MT* vold,
+                       MT* wold, MT* tau, MT* taus, hipStream_t st);
 template <typename ST, typename MT, int Q, int QS>
 void fused_launch_step(const BatchJob<ST, MT>* jobs, int njobs, int64_t blocks, int step, int npts, int s, bool flux,
                              bool ext, unsigned block, size_t lds, hipStream_t st);

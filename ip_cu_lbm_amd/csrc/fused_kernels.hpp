@@ -1055,7 +1055,68 @@ __global__ void k_store_pops(FGeom G, const double* src, ST* dst, int q) {
     Store<ST, MT>::store(dst + (int64_t)q * G.ncell + n, (MT)src[n], Lattice<Q>::w(q), G.scale);
 }
 
+// Initial state of a context (the drivers' fill, LockExchange.cu:133-139): f = feq(p, u) and
+// g = geq(S or phi, u) of every cell, in FP64 with the operation order of the host fill and no
+// fused multiply-adds, so every value equals the host computation's bit for bit before it is
+// stored; the FP64 cell volumes; the phase field's history rows; the LES rate rows.  The fields
+// come in canonical cell order, as the caller holds them: internal cell n is canonical cell
+// ((x * ny) + y) * nz + z of its coordinates.
+template <typename ST, typename MT, int Q, int QS>
+__global__ void k_init_state(FGeom G, InitArgs A, ST* __restrict__ f, ST* __restrict__ g, double* __restrict__ vol,
+                             MT* __restrict__ phiold, MT* __restrict__ uold, MT* __restrict__ vold, MT* __restrict__ wold,
+                             MT* __restrict__ tau, MT* __restrict__ taus) {
+#pragma clang fp contract(off)
+    const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (n >= G.ncell) return;
+    const int s = G.s, o1 = G.o1, o2 = G.o2;
+    const int64_t L = n / G.ns;
+    int c[3];
+    c[s] = (int)(n - L * G.ns);
+    c[o1] = (int)(L / G.n[o2]);
+    c[o2] = (int)(L - (L / G.n[o2]) * G.n[o2]);
+    const int64_t id = ((int64_t)c[0] * G.n[1] + c[1]) * G.n[2] + c[2];
+    double val[7];
+#pragma unroll
+    for (int r = 0; r < 7; ++r) val[r] = (A.present >> r) & 1u ? A.rows[(int64_t)r * G.ncell + id] : A.dflt[r];
+    const double p = val[0], u = val[1], v = val[2], w = val[3], sc = val[4];
+    const double cl = A.c, c2 = cl * cl;
+    const double u2 = u * u + v * v + w * w;
+    const uint32_t NC = (uint32_t)G.ncell, cell = (uint32_t)n;
+    static_for<Q>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        constexpr Dir d = Lattice<Q>::c(k);
+        const double t = ((double)d.x * cl * u + (double)d.y * cl * v + (double)d.z * cl * w) / c2;
+        const double fk = Lattice<Q>::w(k) * (p + 3.0 * t + 4.5 * t * t - 1.5 * u2 / c2);
+        Store<ST, MT>::store(elem(f, (uint32_t)k * NC + cell), (MT)fk, Lattice<Q>::w(k), G.scale);
+    });
+    if constexpr (QS > 0) {
+        constexpr bool linear = QS == 7;
+        const double cs2inv = linear ? 4.0 : 3.0;
+        static_for<QS>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            constexpr Dir d = Lattice<QS>::c(k);
+            const double t = ((double)d.x * cl * u + (double)d.y * cl * v + (double)d.z * cl * w) / c2;
+            double e = 1.0 + cs2inv * t;
+            if constexpr (!linear) e += 4.5 * t * t - 1.5 * u2 / c2;
+            const double gk = Lattice<QS>::w(k) * sc * e;
+            Store<ST, MT>::store(elem(g, (uint32_t)k * NC + cell), (MT)gk, Lattice<QS>::w(k), G.scale);
+        });
+    }
+    *elem(vol, cell) = A.wid[0][c[0]] * A.wid[1][c[1]] * A.wid[2][c[2]] / (A.dz * A.dz * A.dz);
+    if (phiold) {
+        *elem(phiold, cell) = (MT)sc; *elem(uold, cell) = (MT)u; *elem(vold, cell) = (MT)v; *elem(wold, cell) = (MT)w;
+    }
+    if (tau) { *elem(tau, cell) = (MT)val[5]; *elem(taus, cell) = (MT)val[6]; }
+}
+
 // ---- launchers (declared in fused_common.hpp, instantiated in fused_q*.hip) ---------------
+template <typename ST, typename MT, int Q, int QS>
+void fused_launch_init(const FGeom& G, const InitArgs& A, ST* f, ST* g, double* vol, MT* phiold, MT* uold, MT* vold,
+                       MT* wold, MT* tau, MT* taus, hipStream_t st) {
+    IPL_LAUNCH((k_init_state<ST, MT, Q, QS>), (G.ncell + 255) / 256, 256, 0, st, G, A, f, g, vol, phiold, uold, vold,
+               wold, tau, taus);
+}
+
 template <typename ST, typename MT, int Q, int QS>
 void fused_launch_step(const BatchJob<ST, MT>* jobs, int njobs, int64_t blocks, int step, int npts, int s, bool flux,
                              bool ext, unsigned block, size_t lds, hipStream_t st) {
@@ -1241,7 +1302,9 @@ void fused_launch_convert(const FGeom& G, const ST* src, double* dst, int q, boo
                                                                        bool, bool, unsigned, size_t, hipStream_t);    \
     template size_t ipl::fusedk::fused_lds_bytes<ST, MT, Q, QS>(int, int);                                    \
     template void ipl::fusedk::fused_launch_readout<ST, MT, Q, QS>(const FGeom&, const ST*, const ST*, const MT*, \
-                                                                    double*, const FParams&, hipStream_t);
+                                                                    double*, const FParams&, hipStream_t);      \
+    template void ipl::fusedk::fused_launch_init<ST, MT, Q, QS>(const FGeom&, const ipl::fusedk::InitArgs&, ST*, ST*, double*, \
+                                                                 MT*, MT*, MT*, MT*, MT*, MT*, hipStream_t);
 #define IPL_FUSED_INSTANTIATE_SCALAR(ST, MT, QS)                                                           \
     template void ipl::fusedk::fused_launch_scalar_fields<ST, MT, QS>(const FGeom&, const ST*, MT*, MT*,    \
                                                                        const FCoef<MT>&, hipStream_t);

@@ -300,4 +300,47 @@ bool build_graph(const ipl_config& cfg, Graph& g, std::string& err) {
     return true;
 }
 
+bool build_box_graph(const ipl_config& cfg, Graph& g, std::string& err) {
+    if (cfg.solid) { err = "a cell mask needs the full graph"; return false; }
+    if (!graph_header(cfg, g, err)) return false;
+    const int n[3] = {cfg.nx, cfg.ny, cfg.nz};
+    g.cell_ijk.resize((size_t)3 * g.N);
+    int64_t id = 0;
+    for (int i = 0; i < n[0]; ++i)
+        for (int j = 0; j < n[1]; ++j)
+            for (int l = 0; l < n[2]; ++l, ++id) {
+                g.cell_ijk[3 * id] = i; g.cell_ijk[3 * id + 1] = j; g.cell_ijk[3 * id + 2] = l;
+            }
+    // a cell has a ghost when a link leaves the box across a wall: along an axis the lattice has
+    // components on, that is not periodic, at index 0 or n-1 (every lattice here holds the unit
+    // vectors of its axes, and every link with a component along such an axis crosses the wall)
+    int64_t interior = 1;
+    for (int a = 0; a < 3; ++a) {
+        const bool lattice_axis = !(g.dim == 2 && a == 1);
+        interior *= (!lattice_axis || cfg.periodic[a]) ? n[a] : std::max(0, n[a] - 2);
+    }
+    g.G = g.N - interior;
+    g.Ntot = g.N + g.G;
+    return true;
+}
+
+bool axis_line_weights(const ipl_config& cfg, int axis, std::vector<double>& w, std::string& err) {
+    // one line along the axis: the other axes reduced to their first cell
+    ipl_config c = cfg;
+    c.solid = nullptr;
+    int* dims[3] = {&c.nx, &c.ny, &c.nz};
+    for (int a = 0; a < 3; ++a)
+        if (a != axis) *dims[a] = 1;
+    Graph g;
+    if (!build_graph(c, g, err)) return false;
+    if (!g.axis_has_weights[axis]) { err = "no flux along the axis"; return false; }
+    const int np = g.npts, na = g.n[axis];
+    w.assign((size_t)2 * np * na, 0.0);
+    for (int si = 0; si < 2; ++si)
+        for (int j = 0; j < np; ++j)
+            for (int i = 0; i < na; ++i)  // cell i of the line has canonical index i
+                w[((size_t)si * np + j) * na + i] = g.weights[((size_t)(axis * 2 + si) * np + j) * g.N + i];
+    return true;
+}
+
 }  // namespace ipl

@@ -56,6 +56,13 @@ bool build_graph(const ipl_config& cfg, Graph& g, std::string& err);
 // The checks and the scalar fields of build_graph (lattices, sizes, flux mode, stencil, calc-cell
 // count) without its tables: enough to decide the path and size a context.
 bool graph_header(const ipl_config& cfg, Graph& g, std::string& err);
+// A box domain (no cell mask) as the fused path needs it: the header, the calc cells' coordinates
+// (canonical order) and the ghost count, with no neighbour, slip or weight tables.
+bool build_box_graph(const ipl_config& cfg, Graph& g, std::string& err);
+// Lagrange weights of the slab reconstruction along `axis` for the cells of one line along it,
+// [sign (0: +, 1: -)][j][n_axis]: those of build_graph, which on a box depend only on the index
+// along the axis (computed on a one-line copy of the configuration).
+bool axis_line_weights(const ipl_config& cfg, int axis, std::vector<double>& w, std::string& err);
 
 // Lagrange_interpolation (adv.cu:40-48) with the same product order.
 void lagrange_weights(double* coeff, const double* x, double xa, int n);

@@ -112,13 +112,15 @@ class Server:
     budget when the release has ended."""
 
     def __init__(self, budget_bytes: float = 200e9, chunk: int = 400, library: str = None, workers: int = 4,
-                 batched: bool = True, max_lines: int = None):
+                 batched: bool = True, max_lines: int = 2048):
         self.budget = int(budget_bytes)
         self.chunk = chunk
         self.library = library
         self.workers = workers
         self.batched = batched  # batched launches (else every job launches its own steps)
-        self.max_lines = max_lines  # jobs with more lines launch their own steps (None: the library's default)
+        # jobs of more than max_lines workgroups per step launch their own steps on their own streams
+        # (their launches are large enough alone); the smaller ones share batched launches (None: all)
+        self.max_lines = max_lines
 
     def _create(self, job: Job):
         t = time.perf_counter()

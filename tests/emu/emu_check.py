@@ -98,8 +98,10 @@ def main(argv):
             sim.step(int(steps))
             got = {f: sim.read(f) for f in fields}
             path = sim.path
+            counts = (sim.n_calc, sim.n_ghost)
         o = oracle.Oracle(case.replace(storage=F64))
         try:
+            same_counts = counts == (o.n_calc, o.n_ghost)
             o.step(int(steps))
             want = {f: o.read(f) for f in fields}
         finally:
@@ -109,6 +111,7 @@ def main(argv):
         errs["vel"] = rel(np.concatenate([got[f] for f in vel]), np.concatenate([want[f] for f in vel]))
         errs["finite"] = bool(all(np.all(np.isfinite(got[f])) for f in fields))
         errs["path"] = path
+        errs["counts"] = same_counts  # calc and ghost cells as the reference's builder counts them
         out[spec] = errs
         print(f"[emu] {spec}: {errs}", file=sys.stderr, flush=True)
     print(json.dumps(out))
