@@ -51,10 +51,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measure(case, steps, warmup, profile_steps=0):
+def measure(case, steps, warmup, profile_steps=0, copy_reference=False):
     """MLUPS of `steps` steps after `warmup` steps, plus per-launch profile.  The step graphs of
     both buffer parities are captured before the timed region (ipl_prepare), so it holds steps
-    only."""
+    only.  copy_reference: afterwards the flow populations are written back through ipl_write,
+    whose k_store_pops launches (an FP64 row read, an FP32 row written, per population) are the
+    copy of known size that the traffic counters are scaled by."""
     from ip_cu_lbm_amd import Simulation
     with Simulation(case) as sim:
         sim.step(warmup)
@@ -66,6 +68,8 @@ def measure(case, steps, warmup, profile_steps=0):
         n = sim.n_calc
         path = sim.path
         sums = sim.global_sums()
+        if copy_reference:
+            sim.write("f", sim.read("f"))
     return {"mlups": n * steps / (ms * 1e3), "ms_per_step": ms / steps, "wall_s": wall, "n_cells": n,
             "path": path, "profile": prof, "sums": sums.tolist()}
 
@@ -435,7 +439,7 @@ def main(argv=None):
 
     head_case = cases.lock_exchange(storage=F32, path=PATH_FUSED)
     log(f"[bench] headline: {WORKLOAD} fused FP32, warmup {a.warmup}, steps {a.steps}")
-    head = measure(head_case, a.steps, a.warmup, profile_steps=min(a.steps, 200))
+    head = measure(head_case, a.steps, a.warmup, profile_steps=min(a.steps, 200), copy_reference=a.only_headline)
     cells = head["n_cells"]
     b_alg = 2 * (19 + 7) * 4 * cells
     # The dominant kernel's mean duration over the timed region: the timed steps replay graphs of

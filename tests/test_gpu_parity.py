@@ -291,6 +291,29 @@ def test_plain_refined_xy_corner_scheme_fp64(oracle_lib):
     _compare(case, 60, ("p", "vel"), 1e-12, oracle_lib)
 
 
+@pytest.mark.parametrize("bc", [_abi.BC_SLIP, _abi.BC_BOUNCE_BACK])
+def test_plain_corner_scheme_bit_identical_runs(bc):
+    """The corner scheme's scattered contributions (set_out num3, set_fin) are added with atomics,
+    but every destination receives at most two of them in one kernel (its own a0 and the a1 of its
+    +x neighbour, or one a2, or its ghost's transfer after set_out has ended), and a sum of two
+    terms does not depend on their order: two runs are bit-identical."""
+    px, wx = cases.axis_from_widths(cases.stretched_widths(40, 1.03))
+    py, wy = cases.axis_from_widths(cases.stretched_widths(36, 1.04))
+    pz, wz = cases.axis_from_widths(np.ones(12))
+    case = cases.Case(name="corner_repeat", lattice=27, n=(40, 36, 12), pos=[px, py, pz], width=[wx, wy, wz],
+                      wall_bc=bc, flux_axes=_abi.FLUX_REFERENCE, tau=0.7, path=PATH_PLAIN, s_rates=np.full(27, 1.2))
+    X, Y, Z = case.coords()
+    case.init = {"u": 0.01 * np.sin(X / 3.0) * np.cos(Z), "v": 0.01 * np.cos(Y / 2.0),
+                 "p": cases.uniform(13, case.n_cells, -1e-3, 1e-3)}
+    runs = []
+    for _ in range(2):
+        with Simulation(case) as sim:
+            sim.step(50)
+            runs.append(sim.read("f"))
+    assert np.all(np.isfinite(runs[0]))
+    assert np.array_equal(runs[0], runs[1])
+
+
 def test_fused_longest_line_fp32(oracle_lib):
     """Lines of 512 cells, the longest the fused path takes (LDS rows of 28 x 512 x 4 B):
     FP32 within 1e-5 of the FP64 oracle; one cell more and AUTO picks the plain path."""
