@@ -287,9 +287,9 @@ int ipl_ib_read_markers(ipl_ctx* ctx, int32_t which, double* out, size_t count);
  * A batch holds contexts (jobs) of any sizes and configurations.  With batched launches (the
  * default) the members whose step is the fused kernel alone are grouped by kernel variant
  * (storage, lattices, stencil, line axis, line length, LDS) and every group advances by one
- * launch per step whose workgroups cover the lines of all its jobs (a job table on the device, the
- * launches of two steps captured in a HIP graph that is replayed and kept while the groups stay
- * the same).  Members with more in their step (an immersed body, LES, the wall function, an open
+ * launch per step whose workgroups cover the lines of all its jobs (a job table on the device;
+ * a run of at least 20 steps of one group replays a captured HIP graph of 20 launches, kept for
+ * later runs of the same shape).  Members with more in their step (an immersed body, LES, the wall function, an open
  * boundary, a 3D phase field) step alone on their own streams (graph replay), overlapping with
  * the batched launches.  Members keep their own buffers: a job's results are the same alone and
  * in any mix.  The reference runs one case per process and has no counterpart; batches are what a
