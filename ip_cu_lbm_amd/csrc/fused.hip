@@ -404,7 +404,7 @@ public:
                                                          line_block_, stream);
             if (ib_.active() && !flow_only_) {
                 // IB sub-step on f^{n+1}, with the Force field of this step (SphereFall.cu:372-383)
-                Status r = ib_.substep(fb_, gb_, Fb_, std::is_same<ST, double>::value ? IPL_F64 : IPL_F32);
+                Status r = ib_.substep(fb_, gb_, Fb_, std::is_same<ST, double>::value ? IPL_F64 : IPL_F32, parity());
                 if (!r) return r;
             }
             if (ob_n_ > 0)
@@ -413,6 +413,10 @@ public:
             std::swap(fa_, fb_);
             std::swap(ga_, gb_);
             std::swap(Fa_, Fb_);
+        }
+        if (ib_.active()) {  // the last step's IB tail, into the context's stream
+            Status r = ib_.join();
+            if (!r) return r;
         }
         IPL_HIP_TRY(hipGetLastError());
         return Status::ok();
@@ -512,6 +516,11 @@ public:
     // launches are recorded, not run).  For the other parity the buffer roles are swapped for
     // the capture and swapped back after it.
     Status capture_graph(int par) {
+        if (ib_.active()) {  // the captured first step finds its markers ranked (outside the graph)
+            Status r = ib_.join();
+            if (r) r = ib_.ensure_ranked(par);
+            if (!r) return r;
+        }
         const bool swap = par != parity();
         auto swap_roles = [&] { std::swap(fa_, fb_); std::swap(ga_, gb_); std::swap(Fa_, Fb_); };
         IPL_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -581,7 +590,7 @@ public:
             return r;
         }
         // the state of the last step: f, g and its Force field
-        return ib_.stage(which, fa_, ga_, Fa_, std::is_same<ST, double>::value ? IPL_F64 : IPL_F32);
+        return ib_.stage(which, fa_, ga_, Fa_, std::is_same<ST, double>::value ? IPL_F64 : IPL_F32, parity());
     }
 
     // FP64 factors of the step from dz, dt, c and the rates, cast once into K_

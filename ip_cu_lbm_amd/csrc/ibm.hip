@@ -640,13 +640,19 @@ __global__ void __launch_bounds__(kIbGatherLanes) k_ib_gather(IbArgs A, int it, 
         for (int a = 0; a < 3; ++a) Fbox[3 * b + a] = (it == 0 ? 0.0 : Fbox[3 * b + a]) + red[a][0];
 }
 
+// ---- k_ib_clear_runs: no marker run in any box cell, before a ranking writes its runs ---------
+__global__ void k_ib_clear_runs(IbArgs A, int2* __restrict__ range) {
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (b < A.vmax) range[b] = make_int2(0, 0);
+}
+
 // ---- k_ib_apply: update_velIBM (ibm.cu:346-365) on the box: f += w dt (c e . F) ---------------
-// It also clears the marker runs of every box cell for the next step's rank kernel.
+// It also clears the marks of the box cells for the next step's first force pass.
 template <typename ST, int Q>
 __global__ void k_ib_apply(IbArgs A, ST* __restrict__ f, const int* __restrict__ box, const double* __restrict__ Fbox,
-                           int2* __restrict__ range, int* __restrict__ mark) {
+                           int* __restrict__ mark) {
     const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (b < A.vmax) { range[b] = make_int2(0, 0); mark[b] = 0; }
+    if (b < A.vmax) mark[b] = 0;
     const int64_t vol = (int64_t)box[3] * box[4] * box[5];
     if (b >= vol || b >= A.vmax) return;
     const double F[3] = {Fbox[3 * b], Fbox[3 * b + 1], Fbox[3 * b + 2]};
@@ -812,7 +818,12 @@ void rotation_host(const double* q, double* R) {
 
 }  // namespace
 
-IbSolver::~IbSolver() = default;
+IbSolver::~IbSolver() {
+    if (side_) (void)hipStreamSynchronize(side_);
+    if (forced_) (void)hipEventDestroy(forced_);
+    if (tail_done_) (void)hipEventDestroy(tail_done_);
+    if (side_) (void)hipStreamDestroy(side_);
+}
 
 // The box that holds every marker's node block: the marker nodes span at most 2 rmax / dz + 3
 // cells per axis (widths >= dz); a periodic axis may be covered whole when the body straddles its
@@ -953,11 +964,14 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
     }
     if (!(r = mem_.alloc_n(&Fbox_, (size_t)3 * vmax_))) return r;
     if (!(r = mem_.alloc_n(&Ubox_, (size_t)3 * vmax_))) return r;
-    if (!(r = mem_.alloc_n(&range_, (size_t)vmax_))) return r;
+    for (int q = 0; q < 2; ++q)
+        if (!(r = mem_.alloc_n(&range_[q], (size_t)vmax_))) return r;
     if (!(r = mem_.alloc_n(&mark_, (size_t)vmax_))) return r;
     if (!(r = mem_.alloc_n(&body_, (size_t)ibk::B_SIZE))) return r;
-    if (!(r = mem_.alloc_n(&box_, 8))) return r;
-    if (!(r = mem_.alloc_n(&sorted_, (size_t)M_))) return r;
+    for (int q = 0; q < 2; ++q) {
+        if (!(r = mem_.alloc_n(&box_[q], 8))) return r;
+        if (!(r = mem_.alloc_n(&sorted_[q], (size_t)M_))) return r;
+    }
     IPL_HIP_TRY(hipMemcpyAsync(oposw_, cfg.marker_pos, 3 * M_ * sizeof(double), hipMemcpyHostToDevice, st));
     IPL_HIP_TRY(hipMemcpyAsync(onB_, cfg.marker_normal, 3 * M_ * sizeof(double), hipMemcpyHostToDevice, st));
     IPL_HIP_TRY(hipMemcpyAsync(posw_, posw.data(), 3 * M_ * sizeof(double), hipMemcpyHostToDevice, st));
@@ -966,10 +980,14 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
     IPL_HIP_TRY(hipMemsetAsync(Gw_, 0, 3 * M_ * sizeof(double), st));
     IPL_HIP_TRY(hipMemcpyAsync(node_, node.data(), 3 * M_ * sizeof(int), hipMemcpyHostToDevice, st));
     IPL_HIP_TRY(hipMemcpyAsync(body_, body.data(), ibk::B_SIZE * sizeof(double), hipMemcpyHostToDevice, st));
-    IPL_HIP_TRY(hipMemsetAsync(box_, 0, 8 * sizeof(int), st));
-    IPL_HIP_TRY(hipMemsetAsync(range_, 0, (size_t)vmax_ * sizeof(int2), st));  // later cleared by k_ib_apply
-    IPL_HIP_TRY(hipMemsetAsync(mark_, 0, (size_t)vmax_ * sizeof(int), st));
+    for (int q = 0; q < 2; ++q) IPL_HIP_TRY(hipMemsetAsync(box_[q], 0, 8 * sizeof(int), st));
+    IPL_HIP_TRY(hipMemsetAsync(mark_, 0, (size_t)vmax_ * sizeof(int), st));  // later cleared by k_ib_apply
     IPL_HIP_TRY(hipStreamSynchronize(st));
+    ranked_[0] = ranked_[1] = false;
+    // the stream of the step's tail (default priority, like every stream of the library)
+    IPL_HIP_TRY(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    IPL_HIP_TRY(hipEventCreateWithFlags(&forced_, hipEventDisableTiming));
+    IPL_HIP_TRY(hipEventCreateWithFlags(&tail_done_, hipEventDisableTiming));
     return Status::ok();
 }
 
@@ -1025,59 +1043,88 @@ void ubase_q(int Q, const ibk::IbArgs& A, const ST* f, const ST* g, const MT* F,
 }
 
 template <typename ST>
-void apply_q(int Q, const ibk::IbArgs& A, ST* f, const int* box, const double* Fbox, int2* range, int* mark, hipStream_t st) {
+void apply_q(int Q, const ibk::IbArgs& A, ST* f, const int* box, const double* Fbox, int* mark, hipStream_t st) {
     const unsigned blocks = (unsigned)((A.vmax + 255) / 256);
-    if (Q == 9) IPL_LAUNCH((ibk::k_ib_apply<ST, 9>), blocks, 256, 0, st, A, f, box, Fbox, range, mark);
-    else if (Q == 19) IPL_LAUNCH((ibk::k_ib_apply<ST, 19>), blocks, 256, 0, st, A, f, box, Fbox, range, mark);
-    else IPL_LAUNCH((ibk::k_ib_apply<ST, 27>), blocks, 256, 0, st, A, f, box, Fbox, range, mark);
+    if (Q == 9) IPL_LAUNCH((ibk::k_ib_apply<ST, 9>), blocks, 256, 0, st, A, f, box, Fbox, mark);
+    else if (Q == 19) IPL_LAUNCH((ibk::k_ib_apply<ST, 19>), blocks, 256, 0, st, A, f, box, Fbox, mark);
+    else IPL_LAUNCH((ibk::k_ib_apply<ST, 27>), blocks, 256, 0, st, A, f, box, Fbox, mark);
 }
 
 }  // namespace
 
-Status IbSolver::substep(void* f, const void* g, const void* F, int storage) {
+Status IbSolver::substep(void* f, const void* g, const void* F, int storage, int parity) {
     if (M_ == 0) return Status::ok();
-    Status r = rank();
-    if (r && cfg_.spm) r = spm(f, g, F, storage);  // before the marker forcing (SphereFall.cu:376-377)
-    if (r) r = forcing(f, g, F, storage);
-    if (r) r = update_vel(f, storage);
-    if (r) r = body();
-    if (r) r = points();
+    const int set = set_of(parity);
+    Status r = join();  // the previous step's tail: body, points, this step's ranking
+    if (r) r = ensure_ranked(parity);
+    if (r && cfg_.spm) r = spm(f, g, F, storage, set);  // before the marker forcing (SphereFall.cu:376-377)
+    // the forcing passes, with the tail forked after the last force pass (it ranks the next step's
+    // set when the body moves), then update_velIBM
+    if (r) r = forcing(f, g, F, storage, set, cfg_.moving ? set_of(parity + 1) : set);
+    if (r) r = update_vel(f, storage, set);
+    if (r && cfg_.moving) ranked_[set] = false;  // the nodes of set's ranking have moved
     return r;
 }
 
-// One stage of the sub-step on the current state (ipl_ib_stage): the stages that read the marker
-// runs and the box rank the markers first, as the sub-step does once.
-Status IbSolver::stage(int which, void* f, const void* g, const void* F, int storage) {
+Status IbSolver::join() {
+    if (!tail_pending_) return Status::ok();
+    IPL_HIP_TRY(hipStreamWaitEvent(st_, tail_done_, 0));
+    tail_pending_ = false;
+    return Status::ok();
+}
+
+Status IbSolver::ensure_ranked(int parity) {
+    if (M_ == 0) return Status::ok();
+    const int set = set_of(parity);
+    return ranked_[set] ? Status::ok() : rank(set, st_);
+}
+
+// One stage of the sub-step (ipl_ib_stage) on the context's stream: the stages that read the
+// marker runs and the box rank the markers first, as the sub-step has them ranked.
+Status IbSolver::stage(int which, void* f, const void* g, const void* F, int storage, int parity) {
     if (M_ == 0) return Status::err(IPL_ERR_ARGUMENT, "no body attached");
+    const int set = set_of(parity);
+    Status r = join();
+    if (!r) return r;
     switch (which) {
     case IPL_IB_STAGE_SPM:
         if (!cfg_.spm) return Status::err(IPL_ERR_ARGUMENT, "the body has no smoothed-profile forcing");
-        { Status r = rank(); return r ? spm(f, g, F, storage) : r; }
-    case IPL_IB_STAGE_FORCE: { Status r = rank(); return r ? forcing(f, g, F, storage) : r; }
-    case IPL_IB_STAGE_UPDATE_VEL: return update_vel(f, storage);
-    case IPL_IB_STAGE_BODY: return body();
-    case IPL_IB_STAGE_POINTS: return points();
+        r = rank(set, st_);
+        return r ? spm(f, g, F, storage, set) : r;
+    case IPL_IB_STAGE_FORCE:
+        r = rank(set, st_);
+        return r ? forcing(f, g, F, storage, set, -1) : r;
+    case IPL_IB_STAGE_UPDATE_VEL: return update_vel(f, storage, set);
+    case IPL_IB_STAGE_BODY: return body(st_);
+    case IPL_IB_STAGE_POINTS:
+        r = points(st_);
+        if (r && cfg_.moving) ranked_[0] = ranked_[1] = false;
+        return r;
     default: return Status::err(IPL_ERR_ARGUMENT, "IB stage id");
     }
 }
 
-// markers sorted by nearest cell, the box of their node blocks and each box cell's marker run
-Status IbSolver::rank() {
+// markers sorted by nearest cell, the box of their node blocks and each box cell's marker run,
+// into ranking set `set`
+Status IbSolver::rank(int set, hipStream_t st) {
     const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_, bpad_);
+    IPL_LAUNCH(ibk::k_ib_clear_runs, (unsigned)((vmax_ + 255) / 256), 256, 0, st, A, range_[set]);
     constexpr int kPad = 2 * ibk::kIbRankParts;
     const int mpad = (M_ + kPad - 1) / kPad * kPad;
     IPL_LAUNCH_SYNC(ibk::k_ib_rank, (unsigned)((M_ + ibk::kIbRankMarkers - 1) / ibk::kIbRankMarkers), ibk::kIbRankThreads,
-                    (size_t)mpad * sizeof(unsigned long long), st_, A, node_, sorted_, box_, range_, mpad);
+                    (size_t)mpad * sizeof(unsigned long long), st, A, node_, sorted_[set], box_[set], range_[set], mpad);
     IPL_HIP_TRY(hipGetLastError());
+    ranked_[set] = true;
     return Status::ok();
 }
 
 // SPM_ellipse3D (SphereFall3D/SphereFall.cu:13-30) on the box
-Status IbSolver::spm(void* f, const void* g, const void* F, int storage) {
+Status IbSolver::spm(void* f, const void* g, const void* F, int storage, int set) {
     const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_, bpad_);
     const int64_t items = cfg_.spm == 1 ? vmax_ : (int64_t)g_.ncell;  // the box, or every cell
     const unsigned blocks = (unsigned)((items + 255) / 256);
-#define IPL_IBS(ST, MT, QQ) IPL_LAUNCH((ibk::k_ib_spm<ST, MT, QQ>), blocks, 256, 0, st_, A, (ST*)f, (const ST*)g, (const MT*)F, box_, body_, velw_)
+    int* box = box_[set];
+#define IPL_IBS(ST, MT, QQ) IPL_LAUNCH((ibk::k_ib_spm<ST, MT, QQ>), blocks, 256, 0, st_, A, (ST*)f, (const ST*)g, (const MT*)F, box, body_, velw_)
     if (storage == IPL_F64) { if (g_.Q == 9) IPL_IBS(double, double, 9); else if (g_.Q == 19) IPL_IBS(double, double, 19); else IPL_IBS(double, double, 27); }
     else { if (g_.Q == 9) IPL_IBS(float, float, 9); else if (g_.Q == 19) IPL_IBS(float, float, 19); else IPL_IBS(float, float, 27); }
 #undef IPL_IBS
@@ -1086,53 +1133,67 @@ Status IbSolver::spm(void* f, const void* g, const void* F, int storage) {
 }
 
 // marker forces (get_IBMGw2 / IB_directForcing, repeated for multi-direct forcing) and their
-// spread onto the box (needs rank() of the current marker nodes)
-Status IbSolver::forcing(void* f, const void* g, const void* F, int storage) {
+// spread onto the box (needs ranking set `set` of the current nodes).  tail_set >= 0: once the last
+// force pass is queued, the step's tail goes to the second stream (update_IBbody reads the final
+// Gw; a moving body's update_IBpoint, search_IBlattice and ranking into tail_set follow), while
+// the last gather pass runs here.
+Status IbSolver::forcing(void* f, const void* g, const void* F, int storage, int set, int tail_set) {
     const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_, bpad_);
     const int iters = cfg_.method == IPL_IB_DIRECT ? std::max(1, cfg_.iterations) : 1;
+    int* box = box_[set];
     if (cfg_.method == IPL_IB_DIRECT) {
-        if (storage == IPL_F64) ubase_q<double, double>(g_.Q, A, (const double*)f, (const double*)g, (const double*)F, box_, Ubox_, st_);
-        else ubase_q<float, float>(g_.Q, A, (const float*)f, (const float*)g, (const float*)F, box_, Ubox_, st_);
+        if (storage == IPL_F64) ubase_q<double, double>(g_.Q, A, (const double*)f, (const double*)g, (const double*)F, box, Ubox_, st_);
+        else ubase_q<float, float>(g_.Q, A, (const float*)f, (const float*)g, (const float*)F, box, Ubox_, st_);
     }
     for (int it = 0; it < iters; ++it) {
         if (storage == IPL_F64)
-            force_q<double, double>(g_.Q, A, it, (const double*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box_, Fbox_, Ubox_,
+            force_q<double, double>(g_.Q, A, it, (const double*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box, Fbox_, Ubox_,
                                     wcache_, bcache_, mark_, st_);
         else
-            force_q<float, float>(g_.Q, A, it, (const float*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box_, Fbox_, Ubox_,
+            force_q<float, float>(g_.Q, A, it, (const float*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box, Fbox_, Ubox_,
                                   wcache_, bcache_, mark_, st_);
-        IPL_LAUNCH_SYNC(ibk::k_ib_gather, (unsigned)vmax_, ibk::kIbGatherLanes, 0, st_, A, it, sorted_, range_, contrib_, box_, Fbox_,
-                        mark_);
+        if (it == iters - 1 && tail_set >= 0) {
+            IPL_HIP_TRY(hipEventRecord(forced_, st_));
+            IPL_HIP_TRY(hipStreamWaitEvent(side_, forced_, 0));
+            Status r = body(side_);
+            if (r && cfg_.moving) r = points(side_);
+            if (r && cfg_.moving) r = rank(tail_set, side_);
+            if (!r) return r;
+            IPL_HIP_TRY(hipEventRecord(tail_done_, side_));
+            tail_pending_ = true;
+        }
+        IPL_LAUNCH_SYNC(ibk::k_ib_gather, (unsigned)vmax_, ibk::kIbGatherLanes, 0, st_, A, it, sorted_[set], range_[set], contrib_,
+                        box, Fbox_, mark_);
     }
     IPL_HIP_TRY(hipGetLastError());
     return Status::ok();
 }
 
-// update_velIBM (ibm.cu:346-365) with the spread force of forcing(); clears the marker runs
-Status IbSolver::update_vel(void* f, int storage) {
+// update_velIBM (ibm.cu:346-365) with the spread force of forcing(); clears the marks
+Status IbSolver::update_vel(void* f, int storage, int set) {
     const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_, bpad_);
-    if (storage == IPL_F64) apply_q<double>(g_.Q, A, (double*)f, box_, Fbox_, range_, mark_, st_);
-    else apply_q<float>(g_.Q, A, (float*)f, box_, Fbox_, range_, mark_, st_);
+    if (storage == IPL_F64) apply_q<double>(g_.Q, A, (double*)f, box_[set], Fbox_, mark_, st_);
+    else apply_q<float>(g_.Q, A, (float*)f, box_[set], Fbox_, mark_, st_);
     IPL_HIP_TRY(hipGetLastError());
     return Status::ok();
 }
 
 // update_IBbody (ibm.cu:367-424)
-Status IbSolver::body() {
+Status IbSolver::body(hipStream_t st) {
     const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_, bpad_);
-    IPL_LAUNCH_SYNC(ibk::k_ib_body, 1, ibk::kIbBodyThreads, 0, st_, A, Gw_, posw_, body_);
+    IPL_LAUNCH_SYNC(ibk::k_ib_body, 1, ibk::kIbBodyThreads, 0, st, A, Gw_, posw_, body_);
     IPL_HIP_TRY(hipGetLastError());
     return Status::ok();
 }
 
 // update_IBpoint + search_IBlattice (ibm.cu:426-463) of a moving body
-Status IbSolver::points() {
+Status IbSolver::points(hipStream_t st) {
     const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_, bpad_);
     if (cfg_.moving) {
         const unsigned blocks = (unsigned)((M_ + 63) / 64);
-        if (g_.Q == 9) IPL_LAUNCH((ibk::k_ib_points<9>), blocks, 64, 0, st_, A, body_, oposw_, onB_, posw_, nB_, velw_, node_);
-        else if (g_.Q == 19) IPL_LAUNCH((ibk::k_ib_points<19>), blocks, 64, 0, st_, A, body_, oposw_, onB_, posw_, nB_, velw_, node_);
-        else IPL_LAUNCH((ibk::k_ib_points<27>), blocks, 64, 0, st_, A, body_, oposw_, onB_, posw_, nB_, velw_, node_);
+        if (g_.Q == 9) IPL_LAUNCH((ibk::k_ib_points<9>), blocks, 64, 0, st, A, body_, oposw_, onB_, posw_, nB_, velw_, node_);
+        else if (g_.Q == 19) IPL_LAUNCH((ibk::k_ib_points<19>), blocks, 64, 0, st, A, body_, oposw_, onB_, posw_, nB_, velw_, node_);
+        else IPL_LAUNCH((ibk::k_ib_points<27>), blocks, 64, 0, st, A, body_, oposw_, onB_, posw_, nB_, velw_, node_);
     }
     IPL_HIP_TRY(hipGetLastError());
     return Status::ok();
@@ -1140,10 +1201,12 @@ Status IbSolver::points() {
 
 Status IbSolver::check() {
     if (M_ == 0) return Status::ok();
-    int flag = 0;
-    IPL_HIP_TRY(hipMemcpyAsync(&flag, box_ + 6, sizeof(int), hipMemcpyDeviceToHost, st_));
+    Status r = join();
+    if (!r) return r;
+    int flag[2] = {0, 0};
+    for (int q = 0; q < 2; ++q) IPL_HIP_TRY(hipMemcpyAsync(&flag[q], box_[q] + 6, sizeof(int), hipMemcpyDeviceToHost, st_));
     IPL_HIP_TRY(hipStreamSynchronize(st_));
-    if (flag) return Status::err(IPL_ERR_UNSUPPORTED, "the marker node blocks outgrew the box sized at attach");
+    if (flag[0] || flag[1]) return Status::err(IPL_ERR_UNSUPPORTED, "the marker node blocks outgrew the box sized at attach");
     return Status::ok();
 }
 

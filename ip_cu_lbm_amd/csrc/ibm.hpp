@@ -51,9 +51,17 @@ public:
     Status attach(const ipl_ib_config& cfg, const IbGrid& g, const std::vector<double>* pos, hipStream_t st);
     // The IB sub-step on the populations f of the fused path (storage F64 or F32), with F the Force
     // field of the step (same type as the arithmetic) and g the scalar populations (or null).
-    Status substep(void* f, const void* g, const void* F, int storage);
-    // One stage of the sub-step (IPL_IB_STAGE_*) on the current state.
-    Status stage(int which, void* f, const void* g, const void* F, int storage);
+    // parity: the step's buffer parity.  The step's last kernels (update_IBbody and, for a moving
+    // body, update_IBpoint, search_IBlattice and the next step's ranking) run on a second stream
+    // beside the last gather pass, update_velIBM and the next fused step; a moving body's ranking
+    // is therefore held twice, the step of parity p reading set p while set 1 - p is written.
+    Status substep(void* f, const void* g, const void* F, int storage, int parity);
+    // The second stream's work joined into the context's stream (the end of every launch of steps).
+    Status join();
+    // The ranking the step of this parity reads, before a graph of steps is captured.
+    Status ensure_ranked(int parity);
+    // One stage of the sub-step (IPL_IB_STAGE_*) on the current state (on the context's stream).
+    Status stage(int which, void* f, const void* g, const void* F, int storage, int parity);
     Status body_state(double out[19]);
     // IPL_ERR_UNSUPPORTED when a marker block left the bound the box was sized for
     Status check();
@@ -70,12 +78,15 @@ public:
     int markers() const { return M_; }
 
 private:
-    Status rank();
-    Status spm(void* f, const void* g, const void* F, int storage);
-    Status forcing(void* f, const void* g, const void* F, int storage);
-    Status update_vel(void* f, int storage);
-    Status body();
-    Status points();
+    int set_of(int parity) const { return cfg_.moving ? parity & 1 : 0; }
+    Status rank(int set, hipStream_t st);
+    Status spm(void* f, const void* g, const void* F, int storage, int set);
+    // the forcing passes; tail_set >= 0: after the last force pass the step's tail runs on the
+    // second stream, ranking into tail_set
+    Status forcing(void* f, const void* g, const void* F, int storage, int set, int tail_set);
+    Status update_vel(void* f, int storage, int set);
+    Status body(hipStream_t st);
+    Status points(hipStream_t st);
 
     IbGrid g_{};
     ipl_ib_config cfg_{};
@@ -90,10 +101,18 @@ private:
     double *contrib_ = nullptr, *Fbox_ = nullptr, *Ubox_ = nullptr, *body_ = nullptr;
     double* wcache_ = nullptr;  // direct forcing: slot weights of the first pass [M][S]
     int* bcache_ = nullptr;     // and their box indices (-2: ghost node)
-    int *node_ = nullptr, *box_ = nullptr;
-    int2* range_ = nullptr;
+    int* node_ = nullptr;
+    // the ranking (a set per step parity for a moving body, one for a fixed one): the box of the
+    // node blocks (+ an overflow flag), each box cell's run of sorted markers, the sorted keys
+    int* box_[2] = {nullptr, nullptr};
+    int2* range_[2] = {nullptr, nullptr};
+    unsigned long long* sorted_[2] = {nullptr, nullptr};
+    bool ranked_[2] = {false, false};  // the set holds the ranking of the current nodes (in stream order)
     int* mark_ = nullptr;       // box cells some marker block covers (set by the first force pass)
-    unsigned long long* sorted_ = nullptr;
+    hipStream_t side_ = nullptr;       // the step's tail
+    hipEvent_t forced_ = nullptr;      // the last force pass has been queued (Gw final)
+    hipEvent_t tail_done_ = nullptr;   // the tail has been queued
+    bool tail_pending_ = false;        // the context's stream has not yet waited for it
 };
 
 constexpr int kIbMaxMarkers = 4096;  // the rank kernel holds every marker's key in 32 KiB of LDS
