@@ -1152,8 +1152,11 @@ Status IbSolver::forcing(void* f, const void* g, const void* F, int storage, int
         else
             force_q<float, float>(g_.Q, A, it, (const float*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box, Fbox_, Ubox_,
                                   wcache_, bcache_, mark_, st_);
-        if (it == iters - 1 && tail_set >= 0) {
-            IPL_HIP_TRY(hipEventRecord(forced_, st_));
+        const bool fork = it == iters - 1 && tail_set >= 0;
+        if (fork) IPL_HIP_TRY(hipEventRecord(forced_, st_));
+        IPL_LAUNCH_SYNC(ibk::k_ib_gather, (unsigned)vmax_, ibk::kIbGatherLanes, 0, st_, A, it, sorted_[set], range_[set], contrib_,
+                        box, Fbox_, mark_);
+        if (fork) {  // queued after the last gather pass, which it does not wait for
             IPL_HIP_TRY(hipStreamWaitEvent(side_, forced_, 0));
             Status r = body(side_);
             if (r && cfg_.moving) r = points(side_);
@@ -1162,8 +1165,6 @@ Status IbSolver::forcing(void* f, const void* g, const void* F, int storage, int
             IPL_HIP_TRY(hipEventRecord(tail_done_, side_));
             tail_pending_ = true;
         }
-        IPL_LAUNCH_SYNC(ibk::k_ib_gather, (unsigned)vmax_, ibk::kIbGatherLanes, 0, st_, A, it, sorted_[set], range_[set], contrib_,
-                        box, Fbox_, mark_);
     }
     IPL_HIP_TRY(hipGetLastError());
     return Status::ok();
