@@ -435,6 +435,23 @@ def solitary_wave(nx: int = 1024, nz: int = 256, ratio: float = 1.01, width: flo
     return case
 
 
+def phase_field_3d(n=(20, 6, 16), ratio: float = 1.02, width: float = 3.0, mobility: float = 0.02,
+                   rho_ratio: float = 1.05, sigma: float = 1e-3, gravity: float = 1e-5, seed: int = SEED_BASE + 21,
+                   storage: int = _abi.F64, path: int = _abi.PATH_AUTO) -> Case:
+    """A two-layer interface in 3D: D3Q19 MRT flow + D3Q19 conservative Allen-Cahn phase field
+    (the 3D form of collision.cu:190-223, whose drivers are 2D), z stretched toward mid-depth
+    (the lock exchange's grid), periodic in x, slip walls, the interface tilted in x and y, surface
+    tension on.  phi = 1 marks the denser lower layer."""
+    c = lock_exchange(nx=n[0], ny=n[1], nz=n[2], ratio=ratio, storage=storage, path=path, scalar_lattice=0)
+    c = c.replace(name=f"phase_field_3d_{n[0]}x{n[1]}x{n[2]}", scalar=_abi.SCALAR_PHASE_FIELD, periodic=(1, 0, 0),
+                  taus=0.5 + 3 * mobility, pf_tau=0.5 + 3 * mobility, rho_L=1.0, rho_H=rho_ratio, gravity=gravity,
+                  boussinesq=0, sigma=sigma, pf_thickness=width)
+    X, Y, Z = c.coords()
+    zi = 0.5 * Z.max() + 1.5 * np.sin(2 * np.pi * X / n[0]) + 0.5 * np.cos(2 * np.pi * Y / n[1])
+    c.init = {"phi": 0.5 - 0.5 * np.tanh(2.0 * (Z - zi) / width), "p": uniform(seed, c.n_cells, -1e-4, 1e-4)}
+    return c
+
+
 @dataclasses.dataclass
 class Items:
     """The run constants a driver reads from its input/initial.csv and derives from them

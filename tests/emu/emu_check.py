@@ -35,6 +35,8 @@ def make(name, storage):
     if name == "cylinder":
         return cases.cylinder_2d(n=(48, 56), markers=140, storage=storage, path=PATH_FUSED, density_ratio=3.0,
                                  gravity=2e-4), ("p", "u", "w")
+    if name == "phase_field_3d":
+        return cases.phase_field_3d(storage=storage, path=PATH_FUSED), ("p", "u", "v", "w", "phi")
     if name in ("sphere_mdf", "sphere_stress"):  # a moving sphere: every IB kernel, markers re-ranked each step
         mdf = name == "sphere_mdf"
         n = (16, 14, 18)

@@ -27,7 +27,7 @@ import build_emu  # noqa: E402
 
 CASES = ["lock_exchange:f64:60", "lock_exchange:f32:60", "lock_exchange:f16:150", "solitary_wave:f64:60",
          "solitary_wave:f16:60", "lock_exchange_plain:f64:30", "cylinder:f64:20", "box_d3q27:f64:20",
-         "sphere_mdf:f64:12", "sphere_stress:f64:12", "batch:mix:0"]
+         "sphere_mdf:f64:12", "sphere_stress:f64:12", "phase_field_3d:f64:40", "batch:mix:0"]
 
 
 @pytest.fixture(scope="module")

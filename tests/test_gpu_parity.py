@@ -87,6 +87,19 @@ def test_solitary_wave_phase_field_fp64(oracle_lib):
     _compare(case, 150, ("p", "vel", "phi"), 1e-12, oracle_lib)
 
 
+@pytest.mark.parametrize("path", [PATH_PLAIN, _abi.PATH_FUSED])
+def test_phase_field_3d_fp64(oracle_lib, path):
+    """D3Q19 flow + D3Q19 Allen-Cahn phase field in 3D (surface tension, stretched z, tilted
+    interface): the fused path's 3D form takes the scalar-field pass for phi and rho."""
+    case = cases.phase_field_3d(n=(24, 8, 18), path=path)
+    _compare(case, 100, ("p", "vel", "phi"), 1e-12, oracle_lib)
+
+
+def test_phase_field_3d_fp32(oracle_lib):
+    case = cases.phase_field_3d(n=(24, 8, 18), storage=F32, path=_abi.PATH_FUSED)
+    _compare(case, 100, ("vel", "phi"), 1e-5, oracle_lib)
+
+
 def test_srt_collision_plain_and_fused_fp64(oracle_lib):
     """col_f_SRT (collision.cu:151-165), the collision of the reference's sphere and ellipse drivers
     (SphereFall.cu:356), on both paths: a stretched Taylor-Green channel and the lock exchange."""
