@@ -870,8 +870,8 @@ int64_t IbSolver::estimate_bytes(const ipl_ib_config& cfg, const int n[3], const
     for (int a = 0; a < 3; ++a) t += b((int64_t)(n[a] + 1) * 8);       // axis coordinates
     t += 6 * b(3 * M * 8) + b(3 * M * 4) + b(3 * M * S * 8);            // marker arrays, nodes, slot contributions
     if (cfg.method == IPL_IB_DIRECT) t += b(M * S * 8) + b(M * S * 4);  // first-pass slot weights and box indices
-    t += 2 * b(3 * vmax * 8) + b(vmax * 8) + b(vmax * 4);               // box force, velocity, runs, marks
-    t += b(ibk::B_SIZE * 8) + b(8 * 4) + b(M * 8);                      // body, box, sorted keys
+    t += 2 * b(3 * vmax * 8) + 2 * b(vmax * 8) + b(vmax * 4);           // box force, velocity, 2 x runs, marks
+    t += b(ibk::B_SIZE * 8) + 2 * (b(8 * 4) + b(M * 8));                // body, 2 x (box, sorted keys)
     return t;
 }
 
