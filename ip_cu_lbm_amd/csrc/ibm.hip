@@ -26,6 +26,8 @@ struct IbArgs {
     double sina, inv_c, inv_c2, half_itau;  // stress method: sin of the split angle, 1/c, 1/c^2, 1/(2 tau)
     int spm, bpad;                          // smoothed-profile forcing; extra box cells it needs
     double spm_a, spm_b;
+    int mds;                                // multi-direct forcing's later passes in marker space
+    double kappa;                           // their velocity change per unit overlap force: c^2 dt spread / 3
     double rho_f, density, mass, inertia[3], gravity[3];
     int64_t vmax;
     const double* pos[3];
@@ -370,6 +372,7 @@ __global__ void k_ib_spm(IbArgs A, ST* __restrict__ f, const ST* __restrict__ g,
 // (plus c^2 dt F_ib / 3 of earlier passes) interpolated, (U_w - u)/dt spread.  Slot sums are reduced in
 // a fixed tree, so every run adds in the same order.
 constexpr int kIbLanes = 64;
+constexpr int kIbAxisWeights = 15;  // per marker: 3 axes x 5 nodes (3-node blocks leave the last two 0)
 
 template <typename ST, typename MT, int Q>
 __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const ST* __restrict__ f, const int* __restrict__ node,
@@ -378,7 +381,9 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
                                                       double* __restrict__ contrib, const int* __restrict__ box,
                                                       const double* __restrict__ Fbox, const double* __restrict__ Ubox,
                                                       double* __restrict__ wcache, int* __restrict__ bcache,
-                                                      int* __restrict__ mark) {
+                                                      int* __restrict__ mark, double* __restrict__ wax,
+                                                      double* __restrict__ um, double* __restrict__ gnext,
+                                                      double* __restrict__ gtot) {
     constexpr int kRows = Q > 20 ? Q : 20;  // the slot sums, then the 20 moment sums
     __shared__ double red[kIbLanes * kRows];
     __shared__ double share[3];
@@ -409,6 +414,12 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
         const double xw[3] = {posw[3 * m], posw[3 * m + 1], posw[3 * m + 2]};
         double W[3][5];
         axis_weights(A, base, xw, W);
+        if (A.mds && t < kIbAxisWeights) {  // the axis weights for the later passes; a ghost node's is 0
+            const int a = t / 5, i = t % 5;
+            const int x = base[a] + A.lo + i;
+            const bool in = A.periodic[a] || (x >= 0 && x < A.n[a]);
+            wax[(size_t)m * kIbAxisWeights + t] = i <= A.hi - A.lo && in ? W[a][i] : 0.0;
+        }
         for (int j = 0; j < 2; ++j) {
             const int s = t + j * kIbLanes;
             if (s >= A.S) break;
@@ -553,8 +564,10 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
             const double gwv = (U[a] - red[a * kIbLanes]) / A.dt;
             share[a] = gwv;
             Gw[3 * m + a] = (it == 0 ? 0.0 : Gw[3 * m + a]) + -gwv * A.wscale;
+            if (A.mds) { um[3 * m + a] = red[a * kIbLanes]; gnext[3 * m + a] = gwv; gtot[3 * m + a] = gwv; }
         }
     }
+    if (A.mds) return;  // the later passes run in marker space (k_ib_correct), the last writes the slots
     __syncthreads();
     // 3. contribution of every slot to the spread force
     const double g0 = share[0], g1 = share[1], g2 = share[2];
@@ -563,6 +576,142 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
         if (s >= A.S) break;
         const bool ok = oksl[j];
         const double wgt = wsl[j];
+        out[3 * s + 0] = ok ? g0 * A.spread * wgt : 0.0;
+        out[3 * s + 1] = ok ? g1 * A.spread * wgt : 0.0;
+        out[3 * s + 2] = ok ? g2 * A.spread * wgt : 0.0;
+    }
+}
+
+// ---- k_ib_correct: a later multi-direct forcing pass, in marker space ---------------------------
+// The spread of the previous pass's marker forces g changes the node velocities by (c^2 dt / 3) dF
+// (update_velIBM), so the velocity interpolated at marker m changes by
+//   kappa sum_m' O_mm' g_m',  O_mm' = sum over the cells both node blocks reach of w_m w_m'.
+// Node weights are products of axis weights (a ghost node's zero on its own axis), so O_mm' is the
+// product over the axes of the overlaps of the two markers' axis weights.  One 64-lane workgroup
+// per marker: the markers near it (listed with the ranking, k_ib_near) are taken one per lane in
+// list order and the lane sums are added in a fixed tree, so every run adds in the same order.  The last pass writes every slot's share
+// of the summed force of all passes (the spread of a sum is the sum of the spreads), which one
+// gather then adds up per cell.
+// Overlap of two markers' axis weights on axis a: node i of m (cell bm + lo + i) meets node j of
+// k (cell bk + lo + j) where j = i + bm - bk, taking the nearest periodic image (a periodic axis
+// of marker-space passes holds at least 9 cells, so no other image can meet).
+__device__ inline double axis_overlap(const IbArgs& A, int a, const double* wm, const double* __restrict__ wk, int bm, int bk) {
+    int d0 = bm - bk;
+    if (A.periodic[a]) {
+        if (2 * d0 > A.n[a]) d0 -= A.n[a];
+        else if (2 * d0 < -A.n[a]) d0 += A.n[a];
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const int j = i + d0;
+        if (j >= 0 && j < 5) s += wm[i] * wk[j];
+    }
+    return s;
+}
+
+// ---- k_ib_near: the markers whose node blocks can meet marker m's ------------------------------
+// Nearest cells within the block span on every axis (nearest periodic image), in index order: one
+// 64-lane workgroup per marker votes over 64 markers at a time and appends the hits in lane order.
+// The first kIbNearMax go to m's row; the count is the full one (a longer list is not kept, and
+// the marker-space pass then scans all markers for m itself).
+constexpr int kIbNearMax = 256;
+
+__device__ inline bool node_near(const IbArgs& A, const int bm[3], const int* __restrict__ node, int k) {
+    const int span = A.hi - A.lo;
+    bool near = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int d = node[3 * k + a] - bm[a];
+        if (A.periodic[a]) {
+            if (2 * d > A.n[a]) d -= A.n[a];
+            else if (2 * d < -A.n[a]) d += A.n[a];
+        }
+        near = near && d <= span && d >= -span;
+    }
+    return near;
+}
+
+__global__ void __launch_bounds__(kIbLanes) k_ib_near(IbArgs A, const int* __restrict__ node, int* __restrict__ near,
+                                                     int* __restrict__ near_count) {
+    const int m = blockIdx.x, t = threadIdx.x;
+    const int bm[3] = {node[3 * m], node[3 * m + 1], node[3 * m + 2]};
+    int* row = near + (size_t)m * kIbNearMax;
+    int cnt = 0;  // the same on every lane
+    for (int base = 0; base < A.M; base += kIbLanes) {
+        const int k = base + t;
+        const bool hit = k < A.M && node_near(A, bm, node, k);
+        const unsigned long long mask = __ballot(hit);
+        const int pos = cnt + (int)__popcll(mask & __lanemask_lt());
+        if (hit && pos < kIbNearMax) row[pos] = k;
+        cnt += (int)__popcll(mask);
+    }
+    if (t == 0) near_count[m] = cnt;
+}
+
+__global__ void __launch_bounds__(kIbLanes) k_ib_correct(IbArgs A, int last, const int* __restrict__ node,
+                                                        const int* __restrict__ near, const int* __restrict__ near_count,
+                                                        const double* __restrict__ velw, const double* __restrict__ wax,
+                                                        double* __restrict__ um, const double* __restrict__ gprev,
+                                                        double* __restrict__ gnext, double* __restrict__ gtot,
+                                                        double* __restrict__ Gw, const double* __restrict__ wcache,
+                                                        const int* __restrict__ bcache, double* __restrict__ contrib) {
+    __shared__ double red[3][kIbLanes];
+    __shared__ double share[3];
+    const int m = blockIdx.x, t = threadIdx.x;
+    const int bm[3] = {node[3 * m], node[3 * m + 1], node[3 * m + 2]};
+    double wm[3][5];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) wm[a][i] = wax[(size_t)m * kIbAxisWeights + 5 * a + i];
+    double acc[3] = {0.0, 0.0, 0.0};
+    // the overlap with marker k times k's force, added to the lane's share of the sum
+    auto add = [&](int k) {
+        const double* wk = wax + (size_t)k * kIbAxisWeights;
+        double o = axis_overlap(A, 0, wm[0], wk, bm[0], node[3 * k]);
+        o *= axis_overlap(A, 1, wm[1], wk + 5, bm[1], node[3 * k + 1]);
+        o *= axis_overlap(A, 2, wm[2], wk + 10, bm[2], node[3 * k + 2]);
+        acc[0] += o * gprev[3 * k]; acc[1] += o * gprev[3 * k + 1]; acc[2] += o * gprev[3 * k + 2];
+    };
+    // lane l takes entries l, l + 64, ... of m's list of near markers (index order), or, where the
+    // list was too long to keep, tests every marker l, l + 64, ... itself
+    const int cnt = near_count[m];
+    if (cnt <= kIbNearMax) {
+        const int* row = near + (size_t)m * kIbNearMax;
+        for (int e = t; e < cnt; e += kIbLanes) add(row[e]);
+    } else {
+        for (int k = t; k < A.M; k += kIbLanes)
+            if (node_near(A, bm, node, k)) add(k);
+    }
+    for (int a = 0; a < 3; ++a) red[a][t] = acc[a];
+    __syncthreads();
+    for (int w = kIbLanes / 2; w > 0; w >>= 1) {
+        if (t < w)
+            for (int a = 0; a < 3; ++a) red[a][t] += red[a][t + w];
+        __syncthreads();
+    }
+    if (t == 0) {
+        for (int a = 0; a < 3; ++a) {
+            const double u = um[3 * m + a] + A.kappa * red[a][0];
+            const double gwv = (velw[3 * m + a] - u) / A.dt;
+            um[3 * m + a] = u;
+            gnext[3 * m + a] = gwv;
+            const double gt = gtot[3 * m + a] + gwv;
+            gtot[3 * m + a] = gt;
+            Gw[3 * m + a] += -gwv * A.wscale;
+            share[a] = gt;
+        }
+    }
+    if (!last) return;  // uniform
+    __syncthreads();
+    const double g0 = share[0], g1 = share[1], g2 = share[2];
+    double* out = contrib + (size_t)m * A.S * 3;
+    for (int j = 0; j < 2; ++j) {
+        const int s = t + j * kIbLanes;
+        if (s >= A.S) break;
+        const bool ok = bcache[(size_t)m * A.S + s] != -2;
+        const double wgt = wcache[(size_t)m * A.S + s];
         out[3 * s + 0] = ok ? g0 * A.spread * wgt : 0.0;
         out[3 * s + 1] = ok ? g1 * A.spread * wgt : 0.0;
         out[3 * s + 2] = ok ? g2 * A.spread * wgt : 0.0;
@@ -851,6 +1000,18 @@ int64_t IbSolver::box_extent(const ipl_ib_config& cfg, const int n[3], const int
     return vmax;
 }
 
+// Multi-direct forcing runs its later passes in marker space where node weights are products of
+// axis weights: every 3D block, and the 2D 5 x 5 block (the reference's 2D 3 x 3 block moves its
+// (-1, -1) node to (+1, +1), which no product of axis weights gives); and where a node block meets
+// another marker's through one periodic image only (periodic lattice axes of 9 cells or more).
+bool IbSolver::marker_space_passes(const ipl_ib_config& cfg, int dim, const int n[3], const int periodic[3]) {
+    if (cfg.method != IPL_IB_DIRECT || cfg.iterations <= 1) return false;
+    if (dim == 2 && cfg.kernel != IPL_IB_KERNEL_PESKIN4) return false;
+    for (int a = 0; a < 3; ++a)
+        if (!(dim == 2 && a == 1) && periodic[a] && n[a] < 9) return false;
+    return true;
+}
+
 // The device bytes attach() allocates, in the order it allocates them (each buffer at least 256 B,
 // as the arena rounds empty requests).
 int64_t IbSolver::estimate_bytes(const ipl_ib_config& cfg, const int n[3], const int periodic[3], int dim, double dz) {
@@ -870,6 +1031,8 @@ int64_t IbSolver::estimate_bytes(const ipl_ib_config& cfg, const int n[3], const
     for (int a = 0; a < 3; ++a) t += b((int64_t)(n[a] + 1) * 8);       // axis coordinates
     t += 6 * b(3 * M * 8) + b(3 * M * 4) + b(3 * M * S * 8);            // marker arrays, nodes, slot contributions
     if (cfg.method == IPL_IB_DIRECT) t += b(M * S * 8) + b(M * S * 4);  // first-pass slot weights and box indices
+    if (marker_space_passes(cfg, dim, n, periodic))  // axis weights, 4 marker vectors, 2 x near lists
+        t += b(M * ibk::kIbAxisWeights * 8) + 4 * b(3 * M * 8) + 2 * (b(M * ibk::kIbNearMax * 4) + b(M * 4));
     t += 2 * b(3 * vmax * 8) + 2 * b(vmax * 8) + b(vmax * 4);           // box force, velocity, 2 x runs, marks
     t += b(ibk::B_SIZE * 8) + 2 * (b(8 * 4) + b(M * 8));                // body, 2 x (box, sorted keys)
     return t;
@@ -962,6 +1125,17 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
         if (!(r = mem_.alloc_n(&wcache_, (size_t)M_ * S_))) return r;
         if (!(r = mem_.alloc_n(&bcache_, (size_t)M_ * S_))) return r;
     }
+    if (marker_space_passes(cfg, g.dim, g.n, g.periodic)) {  // axis weights, velocity, pass forces, summed force
+        if (!(r = mem_.alloc_n(&wax_, (size_t)M_ * ibk::kIbAxisWeights))) return r;
+        if (!(r = mem_.alloc_n(&um_, (size_t)3 * M_))) return r;
+        for (int q = 0; q < 2; ++q)
+            if (!(r = mem_.alloc_n(&gm_[q], (size_t)3 * M_))) return r;
+        if (!(r = mem_.alloc_n(&gtot_, (size_t)3 * M_))) return r;
+        for (int q = 0; q < 2; ++q) {
+            if (!(r = mem_.alloc_n(&near_[q], (size_t)M_ * ibk::kIbNearMax))) return r;
+            if (!(r = mem_.alloc_n(&near_count_[q], (size_t)M_))) return r;
+        }
+    }
     if (!(r = mem_.alloc_n(&Fbox_, (size_t)3 * vmax_))) return r;
     if (!(r = mem_.alloc_n(&Ubox_, (size_t)3 * vmax_))) return r;
     for (int q = 0; q < 2; ++q)
@@ -1022,14 +1196,16 @@ ibk::IbArgs make_args(const IbGrid& g, const ipl_ib_config& cfg, int M, int S, i
     for (int a = 0; a < 3; ++a) { A.inertia[a] = cfg.inertia[a]; A.gravity[a] = cfg.gravity[a]; }
     A.vmax = vmax;
     A.spm = cfg.spm; A.bpad = bpad; A.spm_a = cfg.spm_a; A.spm_b = cfg.spm_b;
+    A.mds = IbSolver::marker_space_passes(cfg, g.dim, g.n, g.periodic);
+    A.kappa = c2 * g.dt * A.spread / 3.0;
     return A;
 }
 
 template <typename ST, typename MT>
 void force_q(int Q, const ibk::IbArgs& A, int it, const ST* f, const int* node, const double* posw, const double* nB,
              const double* velw, double* Gw, double* contrib, const int* box, const double* Fbox, const double* Ubox,
-             double* wcache, int* bcache, int* mark, hipStream_t st) {
-#define IPL_IBF(QQ) IPL_LAUNCH_SYNC((ibk::k_ib_force<ST, MT, QQ>), A.M, ibk::kIbLanes, 0, st, A, it, f, node, posw, nB, velw, Gw, contrib, box, Fbox, Ubox, wcache, bcache, mark)
+             double* wcache, int* bcache, int* mark, double* wax, double* um, double* gnext, double* gtot, hipStream_t st) {
+#define IPL_IBF(QQ) IPL_LAUNCH_SYNC((ibk::k_ib_force<ST, MT, QQ>), A.M, ibk::kIbLanes, 0, st, A, it, f, node, posw, nB, velw, Gw, contrib, box, Fbox, Ubox, wcache, bcache, mark, wax, um, gnext, gtot)
     if (Q == 9) IPL_IBF(9); else if (Q == 19) IPL_IBF(19); else IPL_IBF(27);
 #undef IPL_IBF
 }
@@ -1113,6 +1289,8 @@ Status IbSolver::rank(int set, hipStream_t st) {
     const int mpad = (M_ + kPad - 1) / kPad * kPad;
     IPL_LAUNCH_SYNC(ibk::k_ib_rank, (unsigned)((M_ + ibk::kIbRankMarkers - 1) / ibk::kIbRankMarkers), ibk::kIbRankThreads,
                     (size_t)mpad * sizeof(unsigned long long), st, A, node_, sorted_[set], box_[set], range_[set], mpad);
+    if (A.mds)  // the near-marker lists of the marker-space passes
+        IPL_LAUNCH_SYNC(ibk::k_ib_near, (unsigned)M_, ibk::kIbLanes, 0, st, A, node_, near_[set], near_count_[set]);
     IPL_HIP_TRY(hipGetLastError());
     ranked_[set] = true;
     return Status::ok();
@@ -1145,14 +1323,21 @@ Status IbSolver::forcing(void* f, const void* g, const void* F, int storage, int
         if (storage == IPL_F64) ubase_q<double, double>(g_.Q, A, (const double*)f, (const double*)g, (const double*)F, box, Ubox_, st_);
         else ubase_q<float, float>(g_.Q, A, (const float*)f, (const float*)g, (const float*)F, box, Ubox_, st_);
     }
-    for (int it = 0; it < iters; ++it) {
+    // marker-space passes: the first force pass, the later passes in marker space, one gather of
+    // the summed force
+    const int spreads = A.mds ? 1 : iters;
+    for (int it = 0; it < spreads; ++it) {
         if (storage == IPL_F64)
             force_q<double, double>(g_.Q, A, it, (const double*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box, Fbox_, Ubox_,
-                                    wcache_, bcache_, mark_, st_);
+                                    wcache_, bcache_, mark_, wax_, um_, gm_[0], gtot_, st_);
         else
             force_q<float, float>(g_.Q, A, it, (const float*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box, Fbox_, Ubox_,
-                                  wcache_, bcache_, mark_, st_);
-        const bool fork = it == iters - 1 && tail_set >= 0;
+                                  wcache_, bcache_, mark_, wax_, um_, gm_[0], gtot_, st_);
+        for (int p = 1; A.mds && p < iters; ++p)
+            IPL_LAUNCH_SYNC(ibk::k_ib_correct, (unsigned)M_, ibk::kIbLanes, 0, st_, A, p == iters - 1 ? 1 : 0, node_,
+                            near_[set], near_count_[set], velw_,
+                            wax_, um_, gm_[(p - 1) & 1], gm_[p & 1], gtot_, Gw_, wcache_, bcache_, contrib_);
+        const bool fork = it == spreads - 1 && tail_set >= 0;
         if (fork) IPL_HIP_TRY(hipEventRecord(forced_, st_));
         IPL_LAUNCH_SYNC(ibk::k_ib_gather, (unsigned)vmax_, ibk::kIbGatherLanes, 0, st_, A, it, sorted_[set], range_[set], contrib_,
                         box, Fbox_, mark_);

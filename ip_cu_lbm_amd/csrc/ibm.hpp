@@ -14,7 +14,10 @@
 //   k_ib_gather  one wavefront per cell of the box: the spread force of the cell, summed over
 //                markers in index order per slot and over slots in a fixed tree (no atomics: the
 //                sum has one order whatever the scheduling)
-//   (force + gather repeated for multi-direct forcing)
+//   (multi-direct forcing: the later passes in marker space, k_ib_correct: one 64-lane workgroup
+//    per marker sums the overlaps of its node block with every other marker's, weighted by their
+//    last forces; the spread of the summed force is gathered once.  Where node weights are not
+//    products of axis weights, force + gather are repeated per pass)
 //   k_ib_apply   update_velIBM on the cells of the box: f += w dt (e . F); clears the marker runs
 //   k_ib_body    one workgroup: FP64 force / torque sums in a fixed tree, buoyancy, Adams-Bashforth
 //                update of the body (update_IBbody)
@@ -70,6 +73,8 @@ public:
     int64_t device_bytes() const { return (int64_t)mem_.total(); }
     // device bytes attach() allocates for this body on a grid of n cells per axis
     static int64_t estimate_bytes(const ipl_ib_config& cfg, const int n[3], const int periodic[3], int dim, double dz);
+    // multi-direct forcing's passes after the first run in marker space (k_ib_correct)
+    static bool marker_space_passes(const ipl_ib_config& cfg, int dim, const int n[3], const int periodic[3]);
     static int64_t box_extent(const ipl_ib_config& cfg, const int n[3], const int periodic[3], int dim, double dz,
                               double rmax, int ext[3], int* pad);
 
@@ -101,6 +106,11 @@ private:
     double *contrib_ = nullptr, *Fbox_ = nullptr, *Ubox_ = nullptr, *body_ = nullptr;
     double* wcache_ = nullptr;  // direct forcing: slot weights of the first pass [M][S]
     int* bcache_ = nullptr;     // and their box indices (-2: ghost node)
+    // marker-space passes: axis weights [M][3][5], interpolated velocity, the forces of the last two
+    // passes and their sum [M][3]
+    double *wax_ = nullptr, *um_ = nullptr, *gm_[2] = {nullptr, nullptr}, *gtot_ = nullptr;
+    // and, per ranking set, each marker's list of markers whose node blocks can meet its own
+    int *near_[2] = {nullptr, nullptr}, *near_count_[2] = {nullptr, nullptr};
     int* node_ = nullptr;
     // the ranking (a set per step parity for a moving body, one for a fixed one): the box of the
     // node blocks (+ an overflow flag), each box cell's run of sorted markers, the sorted keys

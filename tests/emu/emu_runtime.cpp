@@ -24,6 +24,7 @@
 thread_local ipl_emu_idx3 threadIdx, blockIdx, blockDim, gridDim;
 thread_local std::barrier<>* ipl_emu_barrier = nullptr;
 thread_local unsigned char* ipl_emu_lds = nullptr;
+namespace ipl_emu { thread_local unsigned long long ballot_word = 0; }
 
 namespace ipl_emu {
 

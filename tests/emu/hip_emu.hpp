@@ -52,6 +52,22 @@ inline void __syncthreads() {
     if (ipl_emu::in_fiber()) ipl_emu::fiber_barrier();
 }
 
+// Wavefront vote of a 64-work-item workgroup (the library's kernels that vote are one wavefront
+// each and vote from uniform control flow): every work-item posts its bit in one barrier phase and
+// reads the word in the next; the word is cleared in the phase before, after every work-item has
+// read the previous vote.
+namespace ipl_emu { extern thread_local unsigned long long ballot_word; }
+inline unsigned long long __ballot(int p) {
+    __syncthreads();
+    if (threadIdx.x == 0) ipl_emu::ballot_word = 0;
+    __syncthreads();
+    if (p) ipl_emu::ballot_word |= 1ull << threadIdx.x;
+    __syncthreads();
+    return ipl_emu::ballot_word;
+}
+inline unsigned long long __lanemask_lt() { return (1ull << (threadIdx.x & 63)) - 1; }
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+
 template <typename T> inline T atomicAdd(T* p, T v) {
     std::atomic_ref<T> r(*p);
     T old = r.load();

@@ -127,14 +127,17 @@ def test_ib_smoothed_profile_fp64(oracle_lib, q):
         assert np.abs(s.body_state()["force"] - gb["force"]).max() > 1e-8  # the interior forcing acts
 
 
-def test_ib_most_markers_fp64(oracle_lib):
+@pytest.mark.parametrize("method", ["stress", "mdf"])
+def test_ib_most_markers_fp64(oracle_lib, method):
     """4096 markers, the most one body takes (the rank kernel's LDS), against the oracle; 4097 is
-    refused with an error."""
+    refused with an error.  With multi-direct forcing every marker has more near markers than its
+    list keeps (256), so the marker-space passes scan all markers themselves."""
     n = (28, 28, 30)
-    c = cases.falling_sphere(n=n, diameter=14, markers=4096, storage=F64, path=PATH_FUSED, tau=0.7, gravity=1e-4,
-                             method=_abi.IB_STRESS, kernel=_abi.IB_KERNEL_REFERENCE)
+    kw = dict(method=_abi.IB_STRESS, kernel=_abi.IB_KERNEL_REFERENCE) if method == "stress" else \
+        dict(method=_abi.IB_DIRECT, kernel=_abi.IB_KERNEL_PESKIN4, iterations=3, density_ratio=8.0)
+    c = cases.falling_sphere(n=n, diameter=14, markers=4096, storage=F64, path=PATH_FUSED, tau=0.7, gravity=1e-4, **kw)
     c.body.pos = (14.2, 13.9, 15.3)
-    _compare(c, 12, oracle_lib, 1e-12)
+    _compare(c, 12, oracle_lib, 1e-12, gw_tol=1e-10)
     over = cases.falling_sphere(n=n, diameter=14, markers=4097, storage=F64, path=PATH_FUSED,
                                 method=_abi.IB_STRESS, kernel=_abi.IB_KERNEL_REFERENCE)
     over.body.pos = (14.2, 13.9, 15.3)
