@@ -365,6 +365,73 @@ __global__ void k_ib_spm(IbArgs A, ST* __restrict__ f, const ST* __restrict__ g,
     }
 }
 
+// ---- k_ib_weights: direct forcing's node weights, computed with the ranking ---------------------
+// One 64-lane workgroup per marker: the slot weights and box indices of its node block
+// (IB_directForcing, ibm.cu:292-313, or the Peskin support; -2: a ghost node), the axis weights of
+// the marker-space passes (a ghost node's zero on its own axis), and the marks of the box cells
+// the block covers (the gather visits marked cells only).  The markers do not move between the
+// ranking and the forcing passes, so every pass reads these.
+constexpr int kIbAxisWeights = 15;  // per marker: 3 axes x 5 nodes (3-node blocks leave the last two 0)
+
+__global__ void __launch_bounds__(64) k_ib_weights(IbArgs A, const int* __restrict__ node, const double* __restrict__ posw,
+                                                   const int* __restrict__ box, double* __restrict__ wcache,
+                                                   int* __restrict__ bcache, double* __restrict__ wax, int* __restrict__ mark) {
+    __shared__ double xs[3][5], Wl[3][5];
+    __shared__ int okl[3][5];
+    const int m = blockIdx.x, t = threadIdx.x;
+    const int base[3] = {node[3 * m], node[3 * m + 1], node[3 * m + 2]};
+    const int np = A.hi - A.lo + 1;
+    // lane 5a + i: node i of the block along axis a (its coordinate, and whether it is a ghost)
+    const int a = t / 5, i = t % 5;
+    const bool axis_lane = t < kIbAxisWeights && i < np && !(A.dim == 2 && a == 1);
+    if (axis_lane) {
+        int d[3] = {0, 0, 0}, c[3];
+        double pp[3];
+        d[a] = A.lo + i;
+        okl[a][i] = node_at(A, base, d, c, pp);
+        xs[a][i] = pp[a];
+    }
+    __syncthreads();
+    // its weight (axis_weights' rules for direct forcing: the Peskin function, or 3-point Lagrange)
+    if (t < kIbAxisWeights) {
+        double w = 0.0;
+        if (A.dim == 2 && a == 1) {
+            w = i == -A.lo ? 1.0 : 0.0;
+        } else if (i < np) {
+            const double xw = posw[3 * m + a];
+            if (A.kernel == IPL_IB_KERNEL_PESKIN4) {
+                w = peskin4(fabs(xw - xs[a][i]) / A.dz) * (okl[a][i] ? 1.0 : 0.0);
+            } else {
+                w = 1.0;
+                for (int j = 0; j < 3; ++j)
+                    if (j != i) w *= (xw - xs[a][j]) / (xs[a][i] - xs[a][j]);
+            }
+        }
+        Wl[a][i] = w;
+        if (A.mds) {  // a ghost node's axis weight is zero
+            const int x = base[a] + A.lo + i;
+            const bool in = A.periodic[a] || (x >= 0 && x < A.n[a]);
+            wax[(size_t)m * kIbAxisWeights + t] = i < np && in ? w : 0.0;
+        }
+    }
+    __syncthreads();
+    for (int j = 0; j < 2; ++j) {
+        const int s = t + j * 64;
+        if (s >= A.S) break;
+        int d[3], cc[3];
+        double pp[3];
+        slot_offset(A, s, d);
+        const bool ok = node_at(A, base, d, cc, pp);
+        int64_t bx = -1;
+        if (ok) {
+            bx = box_index(A, box, cc);
+            if (bx >= 0 && bx < A.vmax) mark[bx] = 1;
+        }
+        wcache[(size_t)m * A.S + s] = slot_weight(A, Wl, s);
+        bcache[(size_t)m * A.S + s] = ok ? (int)bx : -2;
+    }
+}
+
 // ---- k_ib_force: one 64-lane workgroup per marker, lanes over the slots of its node block ----
 // Stress method (get_IBMGw2, ibm.cu:124-274): f interpolated at the marker, split into outer (fw1)
 // and inner (fw2) populations by the normal; minus the traction of (fw1 - fw2) spread, marker force
@@ -372,7 +439,6 @@ __global__ void k_ib_spm(IbArgs A, ST* __restrict__ f, const ST* __restrict__ g,
 // (plus c^2 dt F_ib / 3 of earlier passes) interpolated, (U_w - u)/dt spread.  Slot sums are reduced in
 // a fixed tree, so every run adds in the same order.
 constexpr int kIbLanes = 64;
-constexpr int kIbAxisWeights = 15;  // per marker: 3 axes x 5 nodes (3-node blocks leave the last two 0)
 
 template <typename ST, typename MT, int Q>
 __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const ST* __restrict__ f, const int* __restrict__ node,
@@ -380,10 +446,9 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
                                                       const double* __restrict__ velw, double* __restrict__ Gw,
                                                       double* __restrict__ contrib, const int* __restrict__ box,
                                                       const double* __restrict__ Fbox, const double* __restrict__ Ubox,
-                                                      double* __restrict__ wcache, int* __restrict__ bcache,
-                                                      int* __restrict__ mark, double* __restrict__ wax,
-                                                      double* __restrict__ um, double* __restrict__ gnext,
-                                                      double* __restrict__ gtot) {
+                                                      const double* __restrict__ wcache, const int* __restrict__ bcache,
+                                                      int* __restrict__ mark, double* __restrict__ um,
+                                                      double* __restrict__ gnext, double* __restrict__ gtot) {
     constexpr int kRows = Q > 20 ? Q : 20;  // the slot sums, then the 20 moment sums
     __shared__ double red[kIbLanes * kRows];
     __shared__ double share[3];
@@ -393,14 +458,14 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
     double* out = contrib + (size_t)m * A.S * 3;
     const int nr = A.method == IPL_IB_STRESS ? Q : 3;
     // the lane's slots (at most two: S <= 2 x 64): node weight, node valid (not a ghost) and its
-    // cell (stress) or box index (direct forcing).  Direct forcing's later passes read them back
-    // from the first pass's cache: the markers do not move between passes.
+    // cell (stress) or box index (direct forcing).  Direct forcing reads them from the ranking's
+    // weights (k_ib_weights).
     double wsl[2] = {0.0, 0.0};
     bool oksl[2] = {false, false};
     int64_t bsl[2] = {-1, -1};
     uint32_t isl[2] = {0, 0};
     const bool direct = A.method == IPL_IB_DIRECT;
-    if (direct && it > 0) {
+    if (direct) {
         for (int j = 0; j < 2; ++j) {
             const int s = t + j * kIbLanes;
             if (s >= A.S) break;
@@ -414,12 +479,6 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
         const double xw[3] = {posw[3 * m], posw[3 * m + 1], posw[3 * m + 2]};
         double W[3][5];
         axis_weights(A, base, xw, W);
-        if (A.mds && t < kIbAxisWeights) {  // the axis weights for the later passes; a ghost node's is 0
-            const int a = t / 5, i = t % 5;
-            const int x = base[a] + A.lo + i;
-            const bool in = A.periodic[a] || (x >= 0 && x < A.n[a]);
-            wax[(size_t)m * kIbAxisWeights + t] = i <= A.hi - A.lo && in ? W[a][i] : 0.0;
-        }
         for (int j = 0; j < 2; ++j) {
             const int s = t + j * kIbLanes;
             if (s >= A.S) break;
@@ -430,13 +489,8 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
             wsl[j] = slot_weight(A, W, s);
             if (oksl[j]) {
                 const int64_t bx = box_index(A, box, cc);
-                if (direct) bsl[j] = bx;
-                else isl[j] = cell_of(A, cc);
+                isl[j] = cell_of(A, cc);
                 if (bx >= 0 && bx < A.vmax) mark[bx] = 1;  // the gather visits marked cells only
-            }
-            if (direct) {
-                wcache[(size_t)m * A.S + s] = wsl[j];
-                bcache[(size_t)m * A.S + s] = oksl[j] ? (int)bsl[j] : -2;
             }
         }
     }
@@ -674,12 +728,26 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_correct(IbArgs A, int last, con
         o *= axis_overlap(A, 2, wm[2], wk + 10, bm[2], node[3 * k + 2]);
         acc[0] += o * gprev[3 * k]; acc[1] += o * gprev[3 * k + 1]; acc[2] += o * gprev[3 * k + 2];
     };
+    // What does not depend on the list is read first, so that its latency overlaps the list's:
+    // lane a < 3 holds component a of m's velocity, forces and Gw; the last pass's slot weights.
+    const int cnt = near_count[m];
+    const int* row = near + (size_t)m * kIbNearMax;
+    const int first = row[t];  // t < 64 <= kIbNearMax: inside the row, used only when t < cnt
+    const int a3 = t < 3 ? t : 0;
+    const double velw_a = velw[3 * m + a3], um_a = um[3 * m + a3], gtot_a = gtot[3 * m + a3], Gw_a = Gw[3 * m + a3];
+    double wsl[2] = {0.0, 0.0};
+    bool oksl[2] = {false, false};
+    if (last)
+        for (int j = 0; j < 2; ++j) {
+            const int s = t + j * kIbLanes;
+            if (s >= A.S) break;
+            oksl[j] = bcache[(size_t)m * A.S + s] != -2;
+            wsl[j] = wcache[(size_t)m * A.S + s];
+        }
     // lane l takes entries l, l + 64, ... of m's list of near markers (index order), or, where the
     // list was too long to keep, tests every marker l, l + 64, ... itself
-    const int cnt = near_count[m];
     if (cnt <= kIbNearMax) {
-        const int* row = near + (size_t)m * kIbNearMax;
-        for (int e = t; e < cnt; e += kIbLanes) add(row[e]);
+        for (int e = t; e < cnt; e += kIbLanes) add(e == t ? first : row[e]);
     } else {
         for (int k = t; k < A.M; k += kIbLanes)
             if (node_near(A, bm, node, k)) add(k);
@@ -691,17 +759,15 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_correct(IbArgs A, int last, con
             for (int a = 0; a < 3; ++a) red[a][t] += red[a][t + w];
         __syncthreads();
     }
-    if (t == 0) {
-        for (int a = 0; a < 3; ++a) {
-            const double u = um[3 * m + a] + A.kappa * red[a][0];
-            const double gwv = (velw[3 * m + a] - u) / A.dt;
-            um[3 * m + a] = u;
-            gnext[3 * m + a] = gwv;
-            const double gt = gtot[3 * m + a] + gwv;
-            gtot[3 * m + a] = gt;
-            Gw[3 * m + a] += -gwv * A.wscale;
-            share[a] = gt;
-        }
+    if (t < 3) {  // component t
+        const double u = um_a + A.kappa * red[t][0];
+        const double gwv = (velw_a - u) / A.dt;
+        um[3 * m + t] = u;
+        gnext[3 * m + t] = gwv;
+        const double gt = gtot_a + gwv;
+        gtot[3 * m + t] = gt;
+        Gw[3 * m + t] = Gw_a + -gwv * A.wscale;
+        share[t] = gt;
     }
     if (!last) return;  // uniform
     __syncthreads();
@@ -710,11 +776,9 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_correct(IbArgs A, int last, con
     for (int j = 0; j < 2; ++j) {
         const int s = t + j * kIbLanes;
         if (s >= A.S) break;
-        const bool ok = bcache[(size_t)m * A.S + s] != -2;
-        const double wgt = wcache[(size_t)m * A.S + s];
-        out[3 * s + 0] = ok ? g0 * A.spread * wgt : 0.0;
-        out[3 * s + 1] = ok ? g1 * A.spread * wgt : 0.0;
-        out[3 * s + 2] = ok ? g2 * A.spread * wgt : 0.0;
+        out[3 * s + 0] = oksl[j] ? g0 * A.spread * wsl[j] : 0.0;
+        out[3 * s + 1] = oksl[j] ? g1 * A.spread * wsl[j] : 0.0;
+        out[3 * s + 2] = oksl[j] ? g2 * A.spread * wsl[j] : 0.0;
     }
 }
 
@@ -796,12 +860,12 @@ __global__ void k_ib_clear_runs(IbArgs A, int2* __restrict__ range) {
 }
 
 // ---- k_ib_apply: update_velIBM (ibm.cu:346-365) on the box: f += w dt (c e . F) ---------------
-// It also clears the marks of the box cells for the next step's first force pass.
+// A moving body's marks are cleared for the next ranking of this set (a fixed body keeps its own).
 template <typename ST, int Q>
 __global__ void k_ib_apply(IbArgs A, ST* __restrict__ f, const int* __restrict__ box, const double* __restrict__ Fbox,
                            int* __restrict__ mark) {
     const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (b < A.vmax) mark[b] = 0;
+    if (A.moving && b < A.vmax) mark[b] = 0;
     const int64_t vol = (int64_t)box[3] * box[4] * box[5];
     if (b >= vol || b >= A.vmax) return;
     const double F[3] = {Fbox[3 * b], Fbox[3 * b + 1], Fbox[3 * b + 2]};
@@ -1033,7 +1097,7 @@ int64_t IbSolver::estimate_bytes(const ipl_ib_config& cfg, const int n[3], const
     if (cfg.method == IPL_IB_DIRECT) t += b(M * S * 8) + b(M * S * 4);  // first-pass slot weights and box indices
     if (marker_space_passes(cfg, dim, n, periodic))  // axis weights, 4 marker vectors, 2 x near lists
         t += b(M * ibk::kIbAxisWeights * 8) + 4 * b(3 * M * 8) + 2 * (b(M * ibk::kIbNearMax * 4) + b(M * 4));
-    t += 2 * b(3 * vmax * 8) + 2 * b(vmax * 8) + b(vmax * 4);           // box force, velocity, 2 x runs, marks
+    t += 2 * b(3 * vmax * 8) + 2 * b(vmax * 8) + 2 * b(vmax * 4);       // box force, velocity, 2 x (runs, marks)
     t += b(ibk::B_SIZE * 8) + 2 * (b(8 * 4) + b(M * 8));                // body, 2 x (box, sorted keys)
     return t;
 }
@@ -1140,7 +1204,8 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
     if (!(r = mem_.alloc_n(&Ubox_, (size_t)3 * vmax_))) return r;
     for (int q = 0; q < 2; ++q)
         if (!(r = mem_.alloc_n(&range_[q], (size_t)vmax_))) return r;
-    if (!(r = mem_.alloc_n(&mark_, (size_t)vmax_))) return r;
+    for (int q = 0; q < 2; ++q)
+        if (!(r = mem_.alloc_n(&mark_[q], (size_t)vmax_))) return r;
     if (!(r = mem_.alloc_n(&body_, (size_t)ibk::B_SIZE))) return r;
     for (int q = 0; q < 2; ++q) {
         if (!(r = mem_.alloc_n(&box_[q], 8))) return r;
@@ -1155,9 +1220,11 @@ Status IbSolver::attach(const ipl_ib_config& cfg, const IbGrid& g, const std::ve
     IPL_HIP_TRY(hipMemcpyAsync(node_, node.data(), 3 * M_ * sizeof(int), hipMemcpyHostToDevice, st));
     IPL_HIP_TRY(hipMemcpyAsync(body_, body.data(), ibk::B_SIZE * sizeof(double), hipMemcpyHostToDevice, st));
     for (int q = 0; q < 2; ++q) IPL_HIP_TRY(hipMemsetAsync(box_[q], 0, 8 * sizeof(int), st));
-    IPL_HIP_TRY(hipMemsetAsync(mark_, 0, (size_t)vmax_ * sizeof(int), st));  // later cleared by k_ib_apply
+    for (int q = 0; q < 2; ++q)  // later cleared by k_ib_apply (moving body)
+        IPL_HIP_TRY(hipMemsetAsync(mark_[q], 0, (size_t)vmax_ * sizeof(int), st));
     IPL_HIP_TRY(hipStreamSynchronize(st));
     ranked_[0] = ranked_[1] = false;
+    weights_set_ = -1;
     // the stream of the step's tail (default priority, like every stream of the library)
     IPL_HIP_TRY(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
     IPL_HIP_TRY(hipEventCreateWithFlags(&forced_, hipEventDisableTiming));
@@ -1204,8 +1271,8 @@ ibk::IbArgs make_args(const IbGrid& g, const ipl_ib_config& cfg, int M, int S, i
 template <typename ST, typename MT>
 void force_q(int Q, const ibk::IbArgs& A, int it, const ST* f, const int* node, const double* posw, const double* nB,
              const double* velw, double* Gw, double* contrib, const int* box, const double* Fbox, const double* Ubox,
-             double* wcache, int* bcache, int* mark, double* wax, double* um, double* gnext, double* gtot, hipStream_t st) {
-#define IPL_IBF(QQ) IPL_LAUNCH_SYNC((ibk::k_ib_force<ST, MT, QQ>), A.M, ibk::kIbLanes, 0, st, A, it, f, node, posw, nB, velw, Gw, contrib, box, Fbox, Ubox, wcache, bcache, mark, wax, um, gnext, gtot)
+             const double* wcache, const int* bcache, int* mark, double* um, double* gnext, double* gtot, hipStream_t st) {
+#define IPL_IBF(QQ) IPL_LAUNCH_SYNC((ibk::k_ib_force<ST, MT, QQ>), A.M, ibk::kIbLanes, 0, st, A, it, f, node, posw, nB, velw, Gw, contrib, box, Fbox, Ubox, wcache, bcache, mark, um, gnext, gtot)
     if (Q == 9) IPL_IBF(9); else if (Q == 19) IPL_IBF(19); else IPL_IBF(27);
 #undef IPL_IBF
 }
@@ -1252,7 +1319,8 @@ Status IbSolver::join() {
 Status IbSolver::ensure_ranked(int parity) {
     if (M_ == 0) return Status::ok();
     const int set = set_of(parity);
-    return ranked_[set] ? Status::ok() : rank(set, st_);
+    const bool weighted = cfg_.method != IPL_IB_DIRECT || weights_set_ == set;  // the caches hold set's weights
+    return ranked_[set] && weighted ? Status::ok() : rank(set, st_);
 }
 
 // One stage of the sub-step (ipl_ib_stage) on the context's stream: the stages that read the
@@ -1289,8 +1357,11 @@ Status IbSolver::rank(int set, hipStream_t st) {
     const int mpad = (M_ + kPad - 1) / kPad * kPad;
     IPL_LAUNCH_SYNC(ibk::k_ib_rank, (unsigned)((M_ + ibk::kIbRankMarkers - 1) / ibk::kIbRankMarkers), ibk::kIbRankThreads,
                     (size_t)mpad * sizeof(unsigned long long), st, A, node_, sorted_[set], box_[set], range_[set], mpad);
+    if (cfg_.method == IPL_IB_DIRECT)  // the node weights of the forcing passes
+        IPL_LAUNCH_SYNC(ibk::k_ib_weights, (unsigned)M_, 64, 0, st, A, node_, posw_, box_[set], wcache_, bcache_, wax_, mark_[set]);
     if (A.mds)  // the near-marker lists of the marker-space passes
         IPL_LAUNCH_SYNC(ibk::k_ib_near, (unsigned)M_, ibk::kIbLanes, 0, st, A, node_, near_[set], near_count_[set]);
+    weights_set_ = set;
     IPL_HIP_TRY(hipGetLastError());
     ranked_[set] = true;
     return Status::ok();
@@ -1329,10 +1400,10 @@ Status IbSolver::forcing(void* f, const void* g, const void* F, int storage, int
     for (int it = 0; it < spreads; ++it) {
         if (storage == IPL_F64)
             force_q<double, double>(g_.Q, A, it, (const double*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box, Fbox_, Ubox_,
-                                    wcache_, bcache_, mark_, wax_, um_, gm_[0], gtot_, st_);
+                                    wcache_, bcache_, mark_[set], um_, gm_[0], gtot_, st_);
         else
             force_q<float, float>(g_.Q, A, it, (const float*)f, node_, posw_, nB_, velw_, Gw_, contrib_, box, Fbox_, Ubox_,
-                                  wcache_, bcache_, mark_, wax_, um_, gm_[0], gtot_, st_);
+                                  wcache_, bcache_, mark_[set], um_, gm_[0], gtot_, st_);
         for (int p = 1; A.mds && p < iters; ++p)
             IPL_LAUNCH_SYNC(ibk::k_ib_correct, (unsigned)M_, ibk::kIbLanes, 0, st_, A, p == iters - 1 ? 1 : 0, node_,
                             near_[set], near_count_[set], velw_,
@@ -1340,7 +1411,7 @@ Status IbSolver::forcing(void* f, const void* g, const void* F, int storage, int
         const bool fork = it == spreads - 1 && tail_set >= 0;
         if (fork) IPL_HIP_TRY(hipEventRecord(forced_, st_));
         IPL_LAUNCH_SYNC(ibk::k_ib_gather, (unsigned)vmax_, ibk::kIbGatherLanes, 0, st_, A, it, sorted_[set], range_[set], contrib_,
-                        box, Fbox_, mark_);
+                        box, Fbox_, mark_[set]);
         if (fork) {  // queued after the last gather pass, which it does not wait for
             IPL_HIP_TRY(hipStreamWaitEvent(side_, forced_, 0));
             Status r = body(side_);
@@ -1358,8 +1429,8 @@ Status IbSolver::forcing(void* f, const void* g, const void* F, int storage, int
 // update_velIBM (ibm.cu:346-365) with the spread force of forcing(); clears the marks
 Status IbSolver::update_vel(void* f, int storage, int set) {
     const ibk::IbArgs A = make_args(g_, cfg_, M_, S_, reach_lo_, reach_hi_, vmax_, pos_, bpad_);
-    if (storage == IPL_F64) apply_q<double>(g_.Q, A, (double*)f, box_[set], Fbox_, mark_, st_);
-    else apply_q<float>(g_.Q, A, (float*)f, box_[set], Fbox_, mark_, st_);
+    if (storage == IPL_F64) apply_q<double>(g_.Q, A, (double*)f, box_[set], Fbox_, mark_[set], st_);
+    else apply_q<float>(g_.Q, A, (float*)f, box_[set], Fbox_, mark_[set], st_);
     IPL_HIP_TRY(hipGetLastError());
     return Status::ok();
 }

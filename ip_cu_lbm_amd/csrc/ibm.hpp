@@ -104,7 +104,7 @@ private:
     double *pos_[3] = {nullptr, nullptr, nullptr};
     double *oposw_ = nullptr, *onB_ = nullptr, *posw_ = nullptr, *nB_ = nullptr, *velw_ = nullptr, *Gw_ = nullptr;
     double *contrib_ = nullptr, *Fbox_ = nullptr, *Ubox_ = nullptr, *body_ = nullptr;
-    double* wcache_ = nullptr;  // direct forcing: slot weights of the first pass [M][S]
+    double* wcache_ = nullptr;  // direct forcing: slot weights [M][S] (computed with the ranking)
     int* bcache_ = nullptr;     // and their box indices (-2: ghost node)
     // marker-space passes: axis weights [M][3][5], interpolated velocity, the forces of the last two
     // passes and their sum [M][3]
@@ -118,7 +118,10 @@ private:
     int2* range_[2] = {nullptr, nullptr};
     unsigned long long* sorted_[2] = {nullptr, nullptr};
     bool ranked_[2] = {false, false};  // the set holds the ranking of the current nodes (in stream order)
-    int* mark_ = nullptr;       // box cells some marker block covers (set by the first force pass)
+    // per ranking set: box cells some marker block covers (set by the ranking's weights for direct
+    // forcing, by the force pass for the stress method; cleared by k_ib_apply for a moving body)
+    int* mark_[2] = {nullptr, nullptr};
+    int weights_set_ = -1;      // the ranking set whose node weights wcache_, bcache_, wax_ hold
     hipStream_t side_ = nullptr;       // the step's tail
     hipEvent_t forced_ = nullptr;      // the last force pass has been queued (Gw final)
     hipEvent_t tail_done_ = nullptr;   // the tail has been queued
