@@ -5,20 +5,26 @@
 //   k_ib_rank    markers sorted by (nearest cell, index) by counting ranks, every workgroup
 //                holding all markers' cells in LDS; the bounding box of their node blocks and the
 //                run of sorted markers nearest to each box cell
+//   k_ib_weights (direct forcing, with the ranking) every marker's node weights: slot weights and
+//                box indices, axis weights, and the marks of the box cells its block covers
+//   k_ib_near    (multi-direct forcing, with the ranking) each marker's list of the markers whose
+//                node blocks can meet its own
 //   k_ib_ubase   (direct forcing) update_rho's velocity at every cell of the box
-//   k_ib_force   one 64-lane workgroup per marker, lanes over the block slots: node weights,
-//                interpolation of f (stress method, get_IBMGw2) or of the node velocity (direct
-//                forcing, IB_directForcing) reduced in a fixed tree, marker force (stress method:
-//                one lane per direction, moment sums in a fixed tree), and the contribution of
-//                every block slot to the spread force
+//   k_ib_force   one 64-lane workgroup per marker, lanes over the block slots: interpolation of f
+//                (stress method, get_IBMGw2, with its own node weights) or of the node velocity
+//                (direct forcing, IB_directForcing) reduced in a fixed tree, marker force (stress
+//                method: one lane per direction, moment sums in a fixed tree), and the
+//                contribution of every block slot to the spread force
+//   k_ib_correct (multi-direct forcing) a later pass in marker space: the velocity change at the
+//                marker is the overlap of its node block with every near marker's, weighted by
+//                their last forces (the spread is linear); the last pass writes the slots' shares
+//                of the force summed over the passes.  Where node weights are not products of
+//                axis weights, force + gather are repeated per pass instead.
 //   k_ib_gather  one wavefront per cell of the box: the spread force of the cell, summed over
 //                markers in index order per slot and over slots in a fixed tree (no atomics: the
 //                sum has one order whatever the scheduling)
-//   (multi-direct forcing: the later passes in marker space, k_ib_correct: one 64-lane workgroup
-//    per marker sums the overlaps of its node block with every other marker's, weighted by their
-//    last forces; the spread of the summed force is gathered once.  Where node weights are not
-//    products of axis weights, force + gather are repeated per pass)
-//   k_ib_apply   update_velIBM on the cells of the box: f += w dt (e . F); clears the marker runs
+//   k_ib_apply   update_velIBM on the cells of the box: f += w dt (e . F); clears a moving body's
+//                marks
 //   k_ib_body    one workgroup: FP64 force / torque sums in a fixed tree, buoyancy, Adams-Bashforth
 //                update of the body (update_IBbody)
 //   k_ib_points  one work-item per marker: update_IBpoint and search_IBlattice
