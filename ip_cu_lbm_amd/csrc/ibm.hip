@@ -180,13 +180,12 @@ __global__ void __launch_bounds__(kIbRankThreads) k_ib_rank(IbArgs A, const int*
             int l = work[a][0] + A.lo - A.bpad, h = work[3 + a][0] + A.hi + A.bpad;
             if (A.periodic[a]) {
                 if (h - l + 1 >= A.n[a]) { l = 0; h = A.n[a] - 1; }
-                else if (l < 0) l += A.n[a];
             } else {
                 l = max(l, 0);
                 h = min(h, A.n[a] - 1);
             }
-            bx[a] = l;
             bx[3 + a] = h - l + 1;
+            bx[a] = l < 0 ? l + A.n[a] : l;  // a periodic box across the low seam starts at its image
             vol *= bx[3 + a];
         }
         if (blockIdx.x == 0) {

@@ -90,6 +90,30 @@ def test_ib_periodic_axis_fp64(oracle_lib):
     _compare(_sphere(periodic=(1, 0, 0)), 40, oracle_lib, 1e-12)
 
 
+@pytest.mark.parametrize("method,x", [("stress", 0.6), ("mdf", 0.6), ("mdf", 4.6), ("mdf", 18.9)])
+def test_ib_periodic_seam_fp64(oracle_lib, method, x):
+    """The body's node blocks across the seam of a periodic axis (below it, x = 0.6 and 4.6, and
+    above it, 18.9): the box of the blocks starts at its periodic image, and node blocks, their
+    overlaps and the near-marker lists take the nearest periodic image."""
+    if method == "stress":
+        c = _sphere(periodic=(1, 0, 0))
+    else:
+        c = _sphere(method=_abi.IB_DIRECT, kernel=_abi.IB_KERNEL_PESKIN4, iters=3, density_ratio=8.0, periodic=(1, 0, 0))
+    c.body.pos = (x, c.body.pos[1], c.body.pos[2])
+    _compare(c, 30, oracle_lib, 1e-12, gw_tol=1e-10)
+
+
+def test_ib_short_periodic_axis_mdf_fp64(oracle_lib):
+    """A periodic axis of 8 cells, shorter than two node blocks: one block can meet another through
+    two periodic images, so the passes stay force + gather pairs."""
+    n = (8, 18, 24)
+    c = cases.falling_sphere(n=n, diameter=5.0, marker_set="2", storage=F64, path=PATH_FUSED, tau=0.7, gravity=2e-4,
+                             method=_abi.IB_DIRECT, kernel=_abi.IB_KERNEL_PESKIN4, iterations=3, density_ratio=8.0)
+    c = c.replace(periodic=(1, 0, 0))
+    c.body.pos = (4.2, 9.1, 12.3)
+    _compare(c, 30, oracle_lib, 1e-12, gw_tol=1e-10)
+
+
 def test_ib_fixed_body_fp64(oracle_lib):
     gb, ob = _compare(_sphere(moving=0), 40, oracle_lib, 1e-12)
     assert np.allclose(gb["pos"], ob["pos"]) and np.allclose(gb["vel"], 0.0)
