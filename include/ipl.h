@@ -297,6 +297,7 @@ int ipl_ib_read_markers(ipl_ctx* ctx, int32_t which, double* out, size_t count);
 enum { IPL_BATCH_LAUNCH_PER_MEMBER = 0, IPL_BATCH_LAUNCH_BATCHED = 1 };
 typedef struct ipl_batch ipl_batch;
 int ipl_batch_create(ipl_batch** out);
+/* A member shares the device with the others (ipl_set_shared(ctx, 1) until it is removed). */
 int ipl_batch_add(ipl_batch* batch, ipl_ctx* ctx);
 int ipl_batch_remove(ipl_batch* batch, ipl_ctx* ctx);
 /* Advances member i by nsteps[i] steps (members in the order they were added), or every member by
@@ -331,6 +332,14 @@ int ipl_set_graph(ipl_ctx* ctx, int32_t on);
 int ipl_prepare(ipl_ctx* ctx);
 /* Number of graph captures the context has made so far (diagnostics). */
 int ipl_graph_captures(ipl_ctx* ctx, int32_t* count);
+/* on = 1: the context shares the device with other contexts stepping at the same time (a job
+ * server's jobs), so an immersed body's step tail (body update, marker points, next ranking) runs
+ * on the context's own stream after the forcing instead of on a second stream beside the next
+ * step, whose fork and join cost more than they hide beside other contexts' work; on = 0 (the
+ * default): the second stream.  ipl_batch_add sets it, ipl_batch_remove clears it.  Set it before
+ * ipl_prepare: a change drops the captured step graphs.  The reference steps one case per process
+ * and has no counterpart. */
+int ipl_set_shared(ipl_ctx* ctx, int32_t on);
 
 /* Open (inflow / outflow) boundary, replacing set_velocity1 + set_wall_boundary1
  * (Karman_vortex_2D/Karman_vortexxz.cu:38-57, called at :391-392 and :425-430): at the end of

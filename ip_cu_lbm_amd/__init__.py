@@ -91,6 +91,11 @@ class Simulation:
         """Captures the step graphs now (recorded, not run), so later steps pay no capture."""
         self._check(self._lib.ipl_prepare(self._h))
 
+    def set_shared(self, on: bool):
+        """The context shares the device with other contexts stepping at the same time: an
+        immersed body's step tail stays on the context's stream (set before prepare())."""
+        self._check(self._lib.ipl_set_shared(self._h, 1 if on else 0))
+
     def graph_captures(self) -> int:
         """Number of graph captures this context has made."""
         n = ctypes.c_int32()

@@ -161,6 +161,8 @@ def load_library(path: str = None):
     lib.ipl_set_graph.restype = c_int32
     lib.ipl_prepare.argtypes = [vp]
     lib.ipl_prepare.restype = c_int32
+    lib.ipl_set_shared.argtypes = [vp, c_int32]
+    lib.ipl_set_shared.restype = c_int32
     lib.ipl_graph_captures.argtypes = [vp, POINTER(c_int32)]
     lib.ipl_graph_captures.restype = c_int32
     lib.ipl_set_open_boundary.argtypes = [vp, POINTER(c_int32), c_int64, _DP, _DP, _DP, _DP]

@@ -321,6 +321,8 @@ int ipl_batch_add(ipl_batch* b, ipl_ctx* ctx) {
     if (!b || !ctx) return IPL_ERR_ARGUMENT;
     for (ipl_ctx* m : b->members)
         if (m == ctx) return IPL_ERR_ARGUMENT;
+    ipl::Status s = ctx->engine->set_shared(true);
+    if (!s) return fail(ctx, s);
     b->members.push_back(ctx);
     return IPL_OK;
 }
@@ -330,7 +332,8 @@ int ipl_batch_remove(ipl_batch* b, ipl_ctx* ctx) {
     for (size_t i = 0; i < b->members.size(); ++i)
         if (b->members[i] == ctx) {
             b->members.erase(b->members.begin() + (long)i);
-            return IPL_OK;
+            ipl::Status s = ctx->engine->set_shared(false);
+            return s ? IPL_OK : fail(ctx, s);
         }
     return IPL_ERR_ARGUMENT;
 }
@@ -413,6 +416,12 @@ int ipl_set_graph(ipl_ctx* ctx, int32_t on) {
 int ipl_prepare(ipl_ctx* ctx) {
     if (!ctx) return IPL_ERR_ARGUMENT;
     ipl::Status s = ctx->engine->prepare();
+    return s ? IPL_OK : fail(ctx, s);
+}
+
+int ipl_set_shared(ipl_ctx* ctx, int32_t on) {
+    if (!ctx) return IPL_ERR_ARGUMENT;
+    ipl::Status s = ctx->engine->set_shared(on != 0);
     return s ? IPL_OK : fail(ctx, s);
 }
 

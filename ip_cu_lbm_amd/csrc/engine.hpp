@@ -141,6 +141,9 @@ public:
     virtual void set_graph(bool) {}
     // capture the step graphs ahead of the first step (no device work)
     virtual Status prepare() { return Status::ok(); }
+    // the context shares the device with others that step at the same time (a batch member):
+    // work that a context alone overlaps on a second stream stays on its own stream
+    virtual Status set_shared(bool) { return Status::ok(); }
     // number of graph captures so far
     virtual int graph_captures() const { return 0; }
     virtual Status stage(int s) = 0;

@@ -557,6 +557,20 @@ public:
         use_graph_ = on;
     }
 
+    // The immersed boundary's tail (body, points, next ranking) overlaps the next step on a second
+    // stream when the context has the device to itself; beside other contexts' steps the fork and
+    // join cost more than they hide, so a batch member keeps the tail on its own stream.  The
+    // step graphs hold the fork: a change drops them (recaptured at the next launch).
+    Status set_shared(bool shared) override {
+        shared_ = shared;
+        if (!ib_.active() || ib_.inline_tail() == shared) return Status::ok();
+        Status r = ib_.join();
+        if (!r) return r;
+        ib_.set_inline_tail(shared);
+        drop_graph();
+        return Status::ok();
+    }
+
     ~FusedEngine() override { drop_graph(); }
 
     Status ib_attach(const ipl_ib_config& cfg) override {
@@ -572,6 +586,7 @@ public:
         Status r = ib_.attach(cfg, g, cfg_pos_, stream);
         if (r) {
             geo_.half_force = 0;  // u = sum c e f after update_velIBM, no force half step
+            ib_.set_inline_tail(shared_);
             drop_graph();  // the captured step has no IB sub-step
         }
         return r;
@@ -833,6 +848,7 @@ private:
 #else
     bool use_graph_ = true;
 #endif
+    bool shared_ = false;  // a batch member (set_shared)
     unsigned block_ = 64;       // work-items per workgroup of the step (lpw lines)
     unsigned line_block_ = 64;  // one line per workgroup (LES pass)
     FGeom batch_geo_{};         // geo_ with the workgroup shape of batched launches

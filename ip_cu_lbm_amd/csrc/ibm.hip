@@ -1407,7 +1407,7 @@ Status IbSolver::forcing(void* f, const void* g, const void* F, int storage, int
             IPL_LAUNCH_SYNC(ibk::k_ib_correct, (unsigned)M_, ibk::kIbLanes, 0, st_, A, p == iters - 1 ? 1 : 0, node_,
                             near_[set], near_count_[set], velw_,
                             wax_, um_, gm_[(p - 1) & 1], gm_[p & 1], gtot_, Gw_, wcache_, bcache_, contrib_);
-        const bool fork = it == spreads - 1 && tail_set >= 0;
+        const bool fork = it == spreads - 1 && tail_set >= 0 && !inline_tail_;
         if (fork) IPL_HIP_TRY(hipEventRecord(forced_, st_));
         IPL_LAUNCH_SYNC(ibk::k_ib_gather, (unsigned)vmax_, ibk::kIbGatherLanes, 0, st_, A, it, sorted_[set], range_[set], contrib_,
                         box, Fbox_, mark_[set]);
@@ -1420,6 +1420,12 @@ Status IbSolver::forcing(void* f, const void* g, const void* F, int storage, int
             IPL_HIP_TRY(hipEventRecord(tail_done_, side_));
             tail_pending_ = true;
         }
+    }
+    if (tail_set >= 0 && inline_tail_) {  // the tail after the last gather pass, on this stream
+        Status r = body(st_);
+        if (r && cfg_.moving) r = points(st_);
+        if (r && cfg_.moving) r = rank(tail_set, st_);
+        if (!r) return r;
     }
     IPL_HIP_TRY(hipGetLastError());
     return Status::ok();

@@ -67,6 +67,9 @@ public:
     Status substep(void* f, const void* g, const void* F, int storage, int parity);
     // The second stream's work joined into the context's stream (the end of every launch of steps).
     Status join();
+    // the tail on the context's stream after the last gather pass (no second stream)
+    void set_inline_tail(bool on) { inline_tail_ = on; }
+    bool inline_tail() const { return inline_tail_; }
     // The ranking the step of this parity reads, before a graph of steps is captured.
     Status ensure_ranked(int parity);
     // One stage of the sub-step (IPL_IB_STAGE_*) on the current state (on the context's stream).
@@ -132,6 +135,7 @@ private:
     hipEvent_t forced_ = nullptr;      // the last force pass has been queued (Gw final)
     hipEvent_t tail_done_ = nullptr;   // the tail has been queued
     bool tail_pending_ = false;        // the context's stream has not yet waited for it
+    bool inline_tail_ = false;         // the tail runs on the context's stream instead
 };
 
 constexpr int kIbMaxMarkers = 4096;  // the rank kernel holds every marker's key in 32 KiB of LDS

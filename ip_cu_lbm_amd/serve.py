@@ -126,6 +126,7 @@ class Server:
         t = time.perf_counter()
         sim = Simulation(job.case, self.library)
         try:
+            sim.set_shared(True)  # it steps beside the other jobs (as a batch member)
             sim.prepare()  # the job's step graphs, captured here rather than in the stepping loop
         except Exception:
             sim.close()

@@ -119,7 +119,9 @@ def test_batch_uniform_count():
 def _launch_mix():
     """Members of several batched-launch groups: D2Q9 FP32 at three sizes whose lines round to one
     workgroup size and one that does not, D3Q19+D3Q7 FP32 at two sizes, D2Q9 FP64, a phase-field
-    D2Q9+D2Q9 wave, and an immersed-boundary cylinder (stepped on its own stream)."""
+    D2Q9+D2Q9 wave, an immersed-boundary cylinder and a falling sphere with multi-direct forcing
+    (stepped on their own streams, the body's step tail on the member's stream: alone it runs on a
+    second stream)."""
     f2 = ("p", "u", "w")
     return [(cases.lattice_taylor_green(nx=96, nz=48, storage=F32, path=PATH_FUSED), f2, True),
             (cases.lattice_taylor_green(nx=80, nz=72, storage=F32, path=PATH_FUSED), f2, True),
@@ -129,7 +131,9 @@ def _launch_mix():
             (cases.lock_exchange(nx=36, ny=10, nz=20, storage=F32, path=PATH_FUSED), ("p", "u", "v", "w", "S"), True),
             (cases.lattice_taylor_green(nx=64, nz=40, storage=F64, path=PATH_FUSED), f2, True),
             (cases.solitary_wave(nx=64, nz=32, x0=20, sigma=1e-4, path=PATH_FUSED), ("p", "u", "w", "phi"), True),
-            (cases.cylinder_2d(n=(72, 80), markers=160, storage=F32, path=PATH_FUSED), f2, False)]
+            (cases.cylinder_2d(n=(72, 80), markers=160, storage=F32, path=PATH_FUSED), f2, False),
+            (cases.falling_sphere(n=(20, 18, 24), diameter=8.0, marker_set="3", storage=F32, path=PATH_FUSED, tau=0.7,
+                                  gravity=2e-4, density_ratio=8.0), ("p", "u", "v", "w"), False)]
 
 
 def test_batched_launches_match_alone():
@@ -138,7 +142,8 @@ def test_batched_launches_match_alone():
     as phases of decreasing membership and leave the buffer roles swapped); the counters show
     which member-steps the batched launches advanced.  Per-member launches give the same."""
     mix = _launch_mix()
-    rounds = [[30, 25, 40, 7, 19, 1, 13, 22, 9], [0, 31, 2, 40, 40, 40, 5, 0, 11], [17, 17, 17, 17, 3, 0, 28, 30, 6]]
+    rounds = [[30, 25, 40, 7, 19, 1, 13, 22, 9, 21], [0, 31, 2, 40, 40, 40, 5, 0, 11, 20],
+              [17, 17, 17, 17, 3, 0, 28, 30, 6, 3]]
     alone = []
     for i, (case, fields, _b) in enumerate(mix):
         with Simulation(case) as s:

@@ -74,6 +74,9 @@ def test_server_admission_and_release_accounting(monkeypatch):
             assert self.open
             self.prepared += 1
 
+        def set_shared(self, on):
+            assert self.open and on and not self.prepared  # before the graphs are captured
+
         def graph_captures(self):
             return 2 * self.prepared
 
