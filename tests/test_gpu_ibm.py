@@ -103,6 +103,24 @@ def test_ib_periodic_seam_fp64(oracle_lib, method, x):
     _compare(c, 30, oracle_lib, 1e-12, gw_tol=1e-10)
 
 
+@pytest.mark.parametrize("method", ["stress", "mdf"])
+def test_ib_body_at_a_wall_fp64(oracle_lib, method):
+    """Node blocks that reach ghost nodes beyond a wall, which take no weight (and, for multi-
+    direct forcing, no axis weight): multi-direct forcing with the body 0.6 cells from the x wall
+    and 0.7 from the bottom wall; the stress method at the x wall only, where its weights are the
+    masked Peskin function.  (Its y and z weights are 3-point Lagrange over node positions, and a
+    ghost node sits on its owner, get_IBMGw2 ibm.cu:146-160: beyond a wall on those axes two of
+    the three positions coincide and the weights are not finite, in the oracle as in the
+    reference; the drivers keep the body clear of the walls.)"""
+    if method == "stress":
+        c = _sphere()
+        c.body.pos = (4.6, c.body.pos[1], c.body.pos[2])  # radius 4: the markers reach x = 0.6
+    else:
+        c = _sphere(method=_abi.IB_DIRECT, kernel=_abi.IB_KERNEL_PESKIN4, iters=3, density_ratio=8.0)
+        c.body.pos = (4.6, c.body.pos[1], 4.7)
+    _compare(c, 20, oracle_lib, 1e-12, gw_tol=1e-10)
+
+
 def test_ib_short_periodic_axis_mdf_fp64(oracle_lib):
     """A periodic axis of 8 cells, shorter than two node blocks: one block can meet another through
     two periodic images, so the passes stay force + gather pairs."""
