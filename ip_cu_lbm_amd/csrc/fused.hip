@@ -71,6 +71,9 @@ int uniform_stream_axis(const Graph& g, const ipl_config& cfg) {
 
 namespace {
 
+// MI355X's last-level (Infinity) cache, shared by the XCDs
+constexpr double kLastLevelCacheBytes = 256.0 * 1024 * 1024;
+
 using fusedk::FBufs;
 using fusedk::FGeom;
 using fusedk::FParams;
@@ -294,6 +297,10 @@ public:
         G.lds_line = (int)(line_bytes / sizeof(MT));
         block_ = (unsigned)ws.items;
         lds_bytes_ = line_bytes * (size_t)ws.lines;
+        // population rows one step reads and writes; beyond the last-level cache the next step
+        // cannot find them there, and non-temporal accesses stream them instead
+        const double pop_bytes = 2.0 * (gr_.Q + gr_.Qs) * sizeof(ST) * (double)G.ncell;
+        streaming_ = pop_bytes > kLastLevelCacheBytes;
         // the shape of its batched launches
         const WorkgroupShape bs = batch_workgroup_shape(G.ns, line_bytes);
         batch_geo_ = G;
@@ -397,7 +404,7 @@ public:
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 1], stream));
             // the one-job table (sync_job) holds both parities' buffer roles
             fusedk::fused_launch_step<ST, MT, Q, QS>(job_dev_, 1, geo_.groups, parity(), gr_.npts, geo_.s, geo_.flux,
-                                                           geo_.wall_fn || geo_.les, block_, lds_bytes_, stream);
+                                                           geo_.wall_fn || geo_.les, streaming_, block_, lds_bytes_, stream);
             if (ev) IPL_HIP_TRY(hipEventRecord((*ev)[4 * s + 2], stream));
             if (geo_.les)  // LES after update_rho, before the IB sub-step (Boegman2005run18IB.cu:338-339)
                 fusedk::fused_launch_les<ST, MT, Q, QS>(geo_, tb_, fb_, gb_, Fb_, les_vel_, les_nu_, les_tau_, les_taus_, K_,
@@ -866,6 +873,7 @@ private:
         }
     }
     size_t lds_bytes_ = 0;
+    bool streaming_ = false;  // non-temporal population accesses (arrays beyond the last-level cache)
 };
 
 }  // namespace
@@ -874,7 +882,7 @@ Status fused_batch_launch(const BatchKey& key, const void* jobs, int njobs, int6
 #define IPL_BATCH(STT, MTT, QQ, QSS)                                                                                  \
     if (key.Q == QQ && key.Qs == QSS) {                                                                               \
         fusedk::fused_launch_step<STT, MTT, QQ, QSS>((const fusedk::BatchJob<STT, MTT>*)jobs, njobs, blocks, step, key.npts, \
-                                                           key.s, key.flux != 0, key.ext != 0, key.block, key.lds, st);   \
+                                                           key.s, key.flux != 0, key.ext != 0, false, key.block, key.lds, st); \
         IPL_HIP_TRY(hipGetLastError());                                                                                \
         return Status::ok();                                                                                           \
     }

@@ -141,7 +141,7 @@ void fused_launch_init(const FGeom& G, const InitArgs& A, ST* f, ST* g, double* 
                        MT* wold, MT* tau, MT* taus, hipStream_t st);
 template <typename ST, typename MT, int Q, int QS>
 void fused_launch_step(const BatchJob<ST, MT>* jobs, int njobs, int64_t blocks, int step, int npts, int s, bool flux,
-                             bool ext, unsigned block, size_t lds, hipStream_t st);
+                             bool ext, bool nt, unsigned block, size_t lds, hipStream_t st);
 template <typename ST, typename MT, int Q, int QS>
 size_t fused_lds_bytes(int s, int ns);
 template <typename ST, typename MT, int QS>

@@ -81,6 +81,23 @@ template <typename MT> struct Store<__half, MT> {
     }
 };
 
+// The population rows a step streams: each cell's own rows read once, every row written once.
+// With NT the accesses carry the non-temporal hint, for steps whose population arrays exceed the
+// last-level cache (the next step would not find them there); otherwise plain accesses, which keep
+// them cached for the next step.
+template <bool NT, typename ST, typename MT, typename P> __device__ inline MT load_pop(P p, double w, float s) {
+#ifndef IPL_CPU_EMULATION
+    if constexpr (NT && !std::is_same<ST, __half>::value) return (MT)__builtin_nontemporal_load(p);
+#endif
+    return Store<ST, MT>::load(p, w, s);
+}
+template <bool NT, typename ST, typename MT, typename P> __device__ inline void store_pop(P p, MT v, double w, float s) {
+#ifndef IPL_CPU_EMULATION
+    if constexpr (NT && !std::is_same<ST, __half>::value) { __builtin_nontemporal_store((ST)v, p); return; }
+#endif
+    Store<ST, MT>::store(p, v, w, s);
+}
+
 // compile-time loop: f(std::integral_constant<int, I>) for I = 0 .. N-1
 template <int N, int I = 0, typename F>
 __device__ inline void static_for(F&& f) {
@@ -192,7 +209,7 @@ __device__ inline LineNb line_neighbours(const FGeom& G, int c1, int c2) {
 
 // Writes the transported value of direction K whose source is the cell at S-index si of the
 // workgroup's line; the destination is that cell + c_K.
-template <typename ST, typename MT, int Q, int K, int S>
+template <typename ST, typename MT, int Q, int K, int S, bool NT>
 __device__ inline void push(const FGeom& G, const LineNb& nb, ST* __restrict__ dst, int si, MT value) {
     constexpr int o1 = LineAxes<S>::o1, o2 = LineAxes<S>::o2;
     constexpr int ka = comp<Q>(K, o1), kb = comp<Q>(K, o2), ks = comp<Q>(K, S);
@@ -213,18 +230,18 @@ __device__ inline void push(const FGeom& G, const LineNb& nb, ST* __restrict__ d
     if (!out1 && !out2) {
         const uint32_t lbase = nb.template base<ka, kb>();
         if (!sout) {
-            Store<ST, MT>::store(elem(dst, K * NC + lbase + (uint32_t)xs), value, wk, G.scale);
+            store_pop<NT, ST, MT>(elem(dst, K * NC + lbase + (uint32_t)xs), value, wk, G.scale);
         } else if (G.wall_bc == IPL_BC_BOUNCE_BACK) {
-            Store<ST, MT>::store(elem(dst, kopp * NC + own), value, wk, G.scale);
+            store_pop<NT, ST, MT>(elem(dst, kopp * NC + own), value, wk, G.scale);
         } else {
             constexpr int kr = reflect_dir<Q>(K, 1 << S);
-            Store<ST, MT>::store(elem(dst, kr * NC + lbase + (uint32_t)si), value, wk, G.scale);
+            store_pop<NT, ST, MT>(elem(dst, kr * NC + lbase + (uint32_t)si), value, wk, G.scale);
         }
         return;
     }
     // boundary lines only
     if (G.wall_bc == IPL_BC_BOUNCE_BACK) {
-        Store<ST, MT>::store(elem(dst, kopp * NC + own), value, wk, G.scale);
+        store_pop<NT, ST, MT>(elem(dst, kopp * NC + own), value, wk, G.scale);
         return;
     }
     // slip: reflect the destination and the direction across every wall crossed; the crossed
@@ -242,14 +259,14 @@ __device__ inline void push(const FGeom& G, const LineNb& nb, ST* __restrict__ d
     constexpr int r4 = reflect_dir<Q>(K, 4), r5 = reflect_dir<Q>(K, 5), r6 = reflect_dir<Q>(K, 6);
     constexpr int r7 = reflect_dir<Q>(K, 7);
     const int kr = out == 1 ? r1 : out == 2 ? r2 : out == 3 ? r3 : out == 4 ? r4 : out == 5 ? r5 : out == 6 ? r6 : r7;
-    Store<ST, MT>::store(elem(dst, (uint32_t)kr * NC + lbase + (uint32_t)ys), value, wk, G.scale);
+    store_pop<NT, ST, MT>(elem(dst, (uint32_t)kr * NC + lbase + (uint32_t)ys), value, wk, G.scale);
 }
 
 // Slab flux, flux update and push of one population set (flow or scalar) of a line.
 // P: collided populations of this work-item's cell.  lds: [ns][SL] of the workgroup, one row of
 // the S-directions' values per cell (SL = their count + 1, an odd stride for the LDS banks), so a
 // direction's slot is an immediate offset from the cell's row.
-template <typename ST, typename MT, int Q, int NPTS, int S, bool FLUX>
+template <typename ST, typename MT, int Q, int NPTS, int S, bool FLUX, bool NT>
 __device__ inline void transport(const FGeom& G, const LineNb& nb, const MT (&Wp)[NPTS], const MT (&Wm)[NPTS], MT rn,
                                  ST* __restrict__ dst, const MT (&P)[Q], MT* lds, int i, bool active) {
     const int ns = G.ns;
@@ -279,9 +296,9 @@ __device__ inline void transport(const FGeom& G, const LineNb& nb, const MT (&Wp
             constexpr double wk = Lattice<Q>::w(k);
             if constexpr (sg == 0) {
                 if (!nb.template out1<ka>() && !nb.template out2<kb>())
-                    Store<ST, MT>::store(elem(dst, k * NC + nb.template base<ka, kb>() + (uint32_t)i), P[k], wk, G.scale);
+                    store_pop<NT, ST, MT>(elem(dst, k * NC + nb.template base<ka, kb>() + (uint32_t)i), P[k], wk, G.scale);
                 else
-                    push<ST, MT, Q, k, S>(G, nb, dst, i, P[k]);
+                    push<ST, MT, Q, k, S, NT>(G, nb, dst, i, P[k]);
             } else if constexpr (sg > 0) {
                 constexpr int kb_ = opposite<Q>(k);
                 constexpr int sk = sdir_slot<Q, S>(k), skb = sdir_slot<Q, S>(kb_);
@@ -303,11 +320,11 @@ __device__ inline void transport(const FGeom& G, const LineNb& nb, const MT (&Wp
                         a1 = first ? rkb * NC + bwd : k * NC + fwd;
                         a2 = last ? rk * NC + fwd : kb_ * NC + bwd;
                     }
-                    Store<ST, MT>::store(elem(dst, a1), first ? P[kb_] : Uk, wk, G.scale);
-                    Store<ST, MT>::store(elem(dst, a2), last ? P[k] : Ukb, wk, G.scale);
+                    store_pop<NT, ST, MT>(elem(dst, a1), first ? P[kb_] : Uk, wk, G.scale);
+                    store_pop<NT, ST, MT>(elem(dst, a2), last ? P[k] : Ukb, wk, G.scale);
                 } else {
-                    push<ST, MT, Q, k, S>(G, nb, dst, i, P[k]);
-                    push<ST, MT, Q, kb_, S>(G, nb, dst, i, P[kb_]);
+                    push<ST, MT, Q, k, S, NT>(G, nb, dst, i, P[k]);
+                    push<ST, MT, Q, kb_, S, NT>(G, nb, dst, i, P[kb_]);
                 }
             }
         });
@@ -360,9 +377,9 @@ __device__ inline void transport(const FGeom& G, const LineNb& nb, const MT (&Wp
             // no component along S: set_tmp returns P exactly (adv.cu:64-65,241); the destination
             // keeps the S-index
             if (!nb.template out1<ka>() && !nb.template out2<kb>())
-                Store<ST, MT>::store(elem(dst, k * NC + nb.template base<ka, kb>() + (uint32_t)i), P[k], wk, G.scale);
+                store_pop<NT, ST, MT>(elem(dst, k * NC + nb.template base<ka, kb>() + (uint32_t)i), P[k], wk, G.scale);
             else
-                push<ST, MT, Q, k, S>(G, nb, dst, i, P[k]);
+                push<ST, MT, Q, k, S, NT>(G, nb, dst, i, P[k]);
         } else if constexpr (sg > 0) {
             // the pair (k, kb_) of opposite S-directions.  U of this cell is the transported value
             // of m = n - sg S, which lands back on this cell's S-index: Uk comes from m = i - 1,
@@ -395,14 +412,14 @@ __device__ inline void transport(const FGeom& G, const LineNb& nb, const MT (&Wp
                     a1 = first ? rkb * NC + bwd : k * NC + fwd;
                     a2 = last ? rk * NC + fwd : kb_ * NC + bwd;
                 }
-                Store<ST, MT>::store(elem(dst, a1), first ? Tkb : Uk, wk, G.scale);
-                Store<ST, MT>::store(elem(dst, a2), last ? Tk : Ukb, wk, G.scale);
+                store_pop<NT, ST, MT>(elem(dst, a1), first ? Tkb : Uk, wk, G.scale);
+                store_pop<NT, ST, MT>(elem(dst, a2), last ? Tk : Ukb, wk, G.scale);
             } else {
                 // boundary lines: the general redirect of push()
-                if (!first) push<ST, MT, Q, k, S>(G, nb, dst, im, Uk);
-                if (!last) push<ST, MT, Q, kb_, S>(G, nb, dst, ip, Ukb);
-                if (last) push<ST, MT, Q, k, S>(G, nb, dst, i, P[mk] + (Fv[k] * rn - Fv[mk] * rn));
-                if (first) push<ST, MT, Q, kb_, S>(G, nb, dst, i, P[mkb] + (Fv[kb_] * rn - Fv[mkb] * rn));
+                if (!first) push<ST, MT, Q, k, S, NT>(G, nb, dst, im, Uk);
+                if (!last) push<ST, MT, Q, kb_, S, NT>(G, nb, dst, ip, Ukb);
+                if (last) push<ST, MT, Q, k, S, NT>(G, nb, dst, i, P[mk] + (Fv[k] * rn - Fv[mk] * rn));
+                if (first) push<ST, MT, Q, kb_, S, NT>(G, nb, dst, i, P[mkb] + (Fv[kb_] * rn - Fv[mkb] * rn));
             }
         }
     });
@@ -684,7 +701,7 @@ __device__ inline MT grad_rho_inline(const FGeom& G, const LineNb& nb, const FTa
 // EXT: the wall function and LES are compiled in (they add registers to every step otherwise)
 // line_ok: the work-item's slot holds a line of the job (else L is another line of the workgroup,
 // whose cells the work-item reads but never writes, and it only takes part in the barriers).
-template <typename ST, typename MT, int Q, int QS, int NPTS, int S, bool FLUX, bool EXT>
+template <typename ST, typename MT, int Q, int QS, int NPTS, int S, bool FLUX, bool EXT, bool NT>
 __device__ __forceinline__ void fused_line(const FGeom& G, const FTables<MT>& tb, const FBufs<ST, MT>& b, const FCoef<MT>& P,
                                            int L, int i, bool line_ok, MT* lds_f) {
     MT* lds_g = lds_f + (sdir_count<Q, S>() + 1) * G.ns;
@@ -707,7 +724,7 @@ __device__ __forceinline__ void fused_line(const FGeom& G, const FTables<MT>& tb
     if constexpr (QS > 0) {
         if (active) {
 #pragma unroll
-            for (int k = 0; k < QS; ++k) g[k] = Store<ST, MT>::load(elem(gsrc, k * NC + n), Lattice<QS>::w(k), G.inv_scale);
+            for (int k = 0; k < QS; ++k) g[k] = load_pop<NT, ST, MT>(elem(gsrc, k * NC + n), Lattice<QS>::w(k), G.inv_scale);
         }
         if (G.inline_rho || G.inline_phi) {  // uniform over the launch: every work-item reaches the barrier
             if (active) {
@@ -721,7 +738,7 @@ __device__ __forceinline__ void fused_line(const FGeom& G, const FTables<MT>& tb
     }
     if (active) {
 #pragma unroll
-        for (int k = 0; k < Q; ++k) Pf[k] = Store<ST, MT>::load(elem(fsrc, k * NC + n), Lattice<Q>::w(k), G.inv_scale);
+        for (int k = 0; k < Q; ++k) Pf[k] = load_pop<NT, ST, MT>(elem(fsrc, k * NC + n), Lattice<Q>::w(k), G.inv_scale);
         // ---- macroscopic fields of step n (update_rho, collision.cu:244-254) ----
         MT p = Pf[0], ex = 0, ey = 0, ez = 0;
         static_for<Q - 1>([&](auto kc) {
@@ -891,8 +908,8 @@ __device__ __forceinline__ void fused_line(const FGeom& G, const FTables<MT>& tb
         }
         rn = *elem(tb.rdel, (uint32_t)ii);
     }
-    transport<ST, MT, Q, NPTS, S, FLUX>(G, nb, Wp, Wm, rn, b.f_dst, Pf, lds_f, ii, active);
-    if constexpr (QS > 0) transport<ST, MT, QS, NPTS, S, FLUX>(G, nb, Wp, Wm, rn, b.g_dst, Pg, lds_g, ii, active);
+    transport<ST, MT, Q, NPTS, S, FLUX, NT>(G, nb, Wp, Wm, rn, b.f_dst, Pf, lds_f, ii, active);
+    if constexpr (QS > 0) transport<ST, MT, QS, NPTS, S, FLUX, NT>(G, nb, Wp, Wm, rn, b.g_dst, Pg, lds_g, ii, active);
 }
 
 // The fused step of one or many simulations in one launch.  The workgroups of job j are blocks
@@ -921,7 +938,7 @@ template <typename T> __device__ inline T load_invariant(const IPL_CONST_AS T* p
     return v;
 }
 
-template <typename ST, typename MT, int Q, int QS, int NPTS, int S, bool FLUX, bool EXT>
+template <typename ST, typename MT, int Q, int QS, int NPTS, int S, bool FLUX, bool EXT, bool NT>
 __global__ void __launch_bounds__(kMaxLine) IPL_FUSED_OCCUPANCY
 k_fused_step(const BatchJob<ST, MT>* __restrict__ jobs, int njobs, int step) {
     IPL_DYNAMIC_LDS(MT, lds_f);
@@ -944,7 +961,7 @@ k_fused_step(const BatchJob<ST, MT>* __restrict__ jobs, int njobs, int step) {
     __builtin_assume(i >= 0 && i < kMaxLine);
     const int first = xcd_line(bid - cj[lo].first_block, G.groups) * G.lpw;
     const bool line_ok = slot < G.lpw && first + slot < (int)G.lines;
-    fused_line<ST, MT, Q, QS, NPTS, S, FLUX, EXT>(G, tb, b, P, line_ok ? first + slot : first, i, line_ok,
+    fused_line<ST, MT, Q, QS, NPTS, S, FLUX, EXT, NT>(G, tb, b, P, line_ok ? first + slot : first, i, line_ok,
                                                   lds_f + (line_ok ? slot : 0) * G.lds_line);
 }
 
@@ -1119,13 +1136,15 @@ void fused_launch_init(const FGeom& G, const InitArgs& A, ST* f, ST* g, double* 
 
 template <typename ST, typename MT, int Q, int QS>
 void fused_launch_step(const BatchJob<ST, MT>* jobs, int njobs, int64_t blocks, int step, int npts, int s, bool flux,
-                             bool ext, unsigned block, size_t lds, hipStream_t st) {
+                             bool ext, bool nt, unsigned block, size_t lds, hipStream_t st) {
     const unsigned grid = (unsigned)blocks;
-    // ext: the wall function or LES is compiled in (it adds registers to every step otherwise)
+    // ext: the wall function or LES is compiled in (it adds registers to every step otherwise);
+    // nt: non-temporal population accesses (taken only without ext)
 #define IPL_FSTEPB(NP, SS, FL)                                                                                        \
     do {                                                                                                              \
-        if (ext) IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS, FL, true>), grid, block, lds, st, jobs, njobs, step); \
-        else IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS, FL, false>), grid, block, lds, st, jobs, njobs, step); \
+        if (ext) IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS, FL, true, false>), grid, block, lds, st, jobs, njobs, step); \
+        else if (nt) IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS, FL, false, true>), grid, block, lds, st, jobs, njobs, step); \
+        else IPL_LAUNCH_SYNC((k_fused_step<ST, MT, Q, QS, NP, SS, FL, false, false>), grid, block, lds, st, jobs, njobs, step); \
     } while (0)
     if (!flux) {
         if constexpr (Lattice<Q>::dim == 2) {
@@ -1299,7 +1318,7 @@ void fused_launch_convert(const FGeom& G, const ST* src, double* dst, int q, boo
                                                               const MT*, MT*, MT*, MT*, MT*, const FCoef<MT>&,        \
                                                               unsigned, hipStream_t);                                 \
     template void ipl::fusedk::fused_launch_step<ST, MT, Q, QS>(const BatchJob<ST, MT>*, int, int64_t, int, int, int, \
-                                                                       bool, bool, unsigned, size_t, hipStream_t);    \
+                                                                       bool, bool, bool, unsigned, size_t, hipStream_t); \
     template size_t ipl::fusedk::fused_lds_bytes<ST, MT, Q, QS>(int, int);                                    \
     template void ipl::fusedk::fused_launch_readout<ST, MT, Q, QS>(const FGeom&, const ST*, const ST*, const MT*, \
                                                                     double*, const FParams&, hipStream_t);      \
