@@ -847,8 +847,8 @@ __device__ __forceinline__ void fused_line(const FGeom& G, const FTables<MT>& tb
                 if constexpr (QS > 0) om_c = MT(1) - MT(1) / *elem(b.taus, n);
             }
         }
-        collide_pairs<MT, Q>(Pf, p, u, v, w, u2c, Fx, Fy, Fz, fkc, ic, P, les, itau_c);
-        // ---- scalar collision ----
+        // ---- scalar collision (before the flow's: g and its collision are then no longer live
+        // while the flow collides) ----
         if constexpr (QS > 0) {
             if (G.scalar == IPL_SCALAR_SALINITY) {  // col_g_reg (collision.cu:167-188)
                 constexpr bool linear = QS == 7;
@@ -897,6 +897,7 @@ __device__ __forceinline__ void fused_line(const FGeom& G, const FTables<MT>& tb
                 });
             }
         }
+        collide_pairs<MT, Q>(Pf, p, u, v, w, u2c, Fx, Fy, Fz, fkc, ic, P, les, itau_c);
     }
     // ---- interpolated propagation of f and g (one set of slab weights for both) ----
     MT Wp[NPTS], Wm[NPTS], rn = MT(0);
@@ -908,8 +909,9 @@ __device__ __forceinline__ void fused_line(const FGeom& G, const FTables<MT>& tb
         }
         rn = *elem(tb.rdel, (uint32_t)ii);
     }
-    transport<ST, MT, Q, NPTS, S, FLUX, NT>(G, nb, Wp, Wm, rn, b.f_dst, Pf, lds_f, ii, active);
+    // the scalar's populations first, whose collided values were formed first, then the flow's
     if constexpr (QS > 0) transport<ST, MT, QS, NPTS, S, FLUX, NT>(G, nb, Wp, Wm, rn, b.g_dst, Pg, lds_g, ii, active);
+    transport<ST, MT, Q, NPTS, S, FLUX, NT>(G, nb, Wp, Wm, rn, b.f_dst, Pf, lds_f, ii, active);
 }
 
 // The fused step of one or many simulations in one launch.  The workgroups of job j are blocks
