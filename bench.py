@@ -45,6 +45,8 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBPS = 8000.0
 WORKLOAD = "lock_exchange_512x32x64_d3q19_d3q7"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", f"traffic_{WORKLOAD}_fp32.json")
+SPHERE_WORKLOAD = "falling_sphere_192x192x384_d3q19"
+SPHERE_TRAFFIC_FILE = os.path.join(ROOT, "profiles", f"traffic_{SPHERE_WORKLOAD}_fp32.json")
 
 
 def log(*a):
@@ -74,10 +76,12 @@ def measure(case, steps, warmup, profile_steps=0, copy_reference=False):
             "path": path, "profile": prof, "sums": sums.tolist()}
 
 
-def measure_sphere(steps, warmup):
+def measure_sphere(steps, warmup, copy_reference=False):
     """The falling-sphere workload (192 x 192 x 384 D3Q19 FP32, 1800 Fibonacci markers, multi-
     direct forcing with 3 passes and the 4-point Peskin kernel, FP64 body): MLUPS of the whole
-    step, the fused kernel's time and bandwidth (2 x 19 x 4 B per cell), the IB sub-step's share."""
+    step, the fused kernel's time and bandwidth (2 x 19 x 4 B per cell), the IB sub-step's share.
+    copy_reference: as in measure(), the populations are written back afterwards (the copy the
+    traffic counters are scaled by)."""
     from ip_cu_lbm_amd import Simulation, cases
     case = cases.falling_sphere()
     with Simulation(case) as sim:
@@ -87,6 +91,8 @@ def measure_sphere(steps, warmup):
         prof = sim.profile_steps(min(steps, 50))
         body = sim.body_state()
         n = sim.n_calc
+        if copy_reference:
+            sim.write("f", sim.read("f"))
     step_us = ms / steps * 1e3
     kern_us = prof[0] * 1e3
     return {"workload": "falling_sphere_192x192x384_d3q19_ibm_mdf3_peskin4", "cells": n,
@@ -340,9 +346,10 @@ def read_traffic():
     return d.get("hbm_bytes_per_launch")
 
 
-def pmc_summary(fetch_dir, write_dir, cells):
+def pmc_summary(fetch_dir, write_dir, cells, sphere=False):
     """Turns two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) into the traffic
-    file, scaling each counter by its reading on the library's copy of a known-size buffer."""
+    file, scaling each counter by its reading on the library's copy of a known-size buffer; of
+    the headline, or with `sphere` of the falling sphere's fused kernel (--only-sphere)."""
     import csv
     import glob
 
@@ -365,18 +372,19 @@ def pmc_summary(fetch_dir, write_dir, cells):
     fs = (cells * 8 / 1024.0) / f_copy if f_copy else 1.0
     ws = (cells * 4 / 1024.0) / w_copy if w_copy else 1.0
     total = (f_main * fs + w_main * ws) * 1024.0
-    d = {"workload": WORKLOAD, "fetch_kib": f_main, "write_kib": w_main, "fetch_scale": fs, "write_scale": ws,
-         "hbm_bytes_per_launch": total, "algorithmic_bytes_per_launch": 2 * 26 * 4 * cells,
+    run = "--only-sphere --steps 10 --warmup 2" if sphere else "--only-headline --steps 20 --warmup 2"
+    d = {"workload": SPHERE_WORKLOAD if sphere else WORKLOAD, "fetch_kib": f_main, "write_kib": w_main,
+         "fetch_scale": fs, "write_scale": ws, "hbm_bytes_per_launch": total,
+         "algorithmic_bytes_per_launch": 2 * (19 if sphere else 26) * 4 * cells,
          "commands": [
-             "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o run -- "
-             "python bench.py --only-headline --steps 20 --warmup 2",
-             "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o run -- "
-             "python bench.py --only-headline --steps 20 --warmup 2",
-             "python bench.py --pmc-summary gpurun_out/pmc_fetch gpurun_out/pmc_write"],
+             f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o run -- python bench.py {run}",
+             f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o run -- python bench.py {run}",
+             "python bench.py --pmc-summary gpurun_out/pmc_fetch gpurun_out/pmc_write" + (" --pmc-sphere" if sphere else "")],
          "copy_kernel": f"k_store_pops<float,float,*>: reads an FP64 row and writes an FP32 row of {cells:,} values "
                         f"({cells * 8 // 1024} KiB read, {cells * 4 // 1024} KiB written)"}
-    os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
-    with open(TRAFFIC_FILE, "w") as fh:
+    out_file = SPHERE_TRAFFIC_FILE if sphere else TRAFFIC_FILE
+    os.makedirs(os.path.dirname(out_file), exist_ok=True)
+    with open(out_file, "w") as fh:
         json.dump(d, fh, indent=1)
     print(json.dumps(d))
 
@@ -402,6 +410,8 @@ def main(argv=None):
                     help="run the falling sphere STEPS steps in FP32 and FP64 and print their settling velocity and drag")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the traffic file from two rocprofv3 --pmc output directories and exit")
+    ap.add_argument("--pmc-sphere", action="store_true",
+                    help="with --pmc-summary: the runs were of --only-sphere (the falling sphere's traffic file)")
     a = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -414,7 +424,10 @@ def main(argv=None):
     from ip_cu_lbm_amd import F32, F64, PATH_FUSED, PATH_PLAIN, cases
 
     if a.pmc_summary:
-        pmc_summary(a.pmc_summary[0], a.pmc_summary[1], 512 * 32 * 64)
+        if a.pmc_sphere:
+            pmc_summary(a.pmc_summary[0], a.pmc_summary[1], 192 * 192 * 384, sphere=True)
+        else:
+            pmc_summary(a.pmc_summary[0], a.pmc_summary[1], 512 * 32 * 64)
         return 0
 
     if a.sphere_precision:
@@ -422,7 +435,7 @@ def main(argv=None):
         return 0
 
     if a.only_sphere:
-        print(json.dumps(measure_sphere(a.steps, a.warmup)), flush=True)
+        print(json.dumps(measure_sphere(a.steps, a.warmup, copy_reference=True)), flush=True)
         return 0
 
     if a.only_small:
