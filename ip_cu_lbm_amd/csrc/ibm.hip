@@ -648,7 +648,9 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_force(IbArgs A, int it, const S
 // Overlap of two markers' axis weights on axis a: node i of m (cell bm + lo + i) meets node j of
 // k (cell bk + lo + j) where j = i + bm - bk, taking the nearest periodic image (a periodic axis
 // of marker-space passes holds at least 9 cells, so no other image can meet).
-__device__ inline double axis_overlap(const IbArgs& A, int a, const double* wm, const double* __restrict__ wk, int bm, int bk) {
+// wk holds k's five axis weights in registers (loaded whole, so that the loads need not wait for
+// the node offset), and the node pairs are selected with compile-time indices.
+__device__ inline double axis_overlap(const IbArgs& A, int a, const double* wm, const double* wk, int bm, int bk) {
     int d0 = bm - bk;
     if (A.periodic[a]) {
         if (2 * d0 > A.n[a]) d0 -= A.n[a];
@@ -656,10 +658,10 @@ __device__ inline double axis_overlap(const IbArgs& A, int a, const double* wm, 
     }
     double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        const int j = i + d0;
-        if (j >= 0 && j < 5) s += wm[i] * wk[j];
-    }
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            if (j - i == d0) s += wm[i] * wk[j];
     return s;
 }
 
@@ -721,7 +723,9 @@ __global__ void __launch_bounds__(kIbLanes) k_ib_correct(IbArgs A, int last, con
     double acc[3] = {0.0, 0.0, 0.0};
     // the overlap with marker k times k's force, added to the lane's share of the sum
     auto add = [&](int k) {
-        const double* wk = wax + (size_t)k * kIbAxisWeights;
+        double wk[kIbAxisWeights];
+#pragma unroll
+        for (int q = 0; q < kIbAxisWeights; ++q) wk[q] = wax[(size_t)k * kIbAxisWeights + q];
         double o = axis_overlap(A, 0, wm[0], wk, bm[0], node[3 * k]);
         o *= axis_overlap(A, 1, wm[1], wk + 5, bm[1], node[3 * k + 1]);
         o *= axis_overlap(A, 2, wm[2], wk + 10, bm[2], node[3 * k + 2]);
