@@ -98,14 +98,6 @@ template <bool NT, typename ST, typename MT, typename P> __device__ inline void 
     Store<ST, MT>::store(p, v, w, s);
 }
 
-// compile-time loop: f(std::integral_constant<int, I>) for I = 0 .. N-1
-template <int N, int I = 0, typename F>
-__device__ inline void static_for(F&& f) {
-    if constexpr (I < N) {
-        f(std::integral_constant<int, I>{});
-        static_for<N, I + 1>(f);
-    }
-}
 
 // number of directions with a component along axis S before direction k (the LDS slot)
 template <int Q, int S> constexpr int sdir_slot(int k) {

@@ -7,6 +7,8 @@
 // lets the compiler fold the integer moment matrix into adds and subtracts.
 #pragma once
 
+#include <type_traits>
+
 #if defined(__HIPCC__)
 #define IPL_HD __host__ __device__
 #else
@@ -88,6 +90,17 @@ template <int Q> constexpr IPL_HD int mirror(int k, int axis) {
 // unit direction along axis with sign s
 template <int Q> constexpr IPL_HD int axis_dir(int axis, int s) {
     return axis == 0 ? find_dir<Q>(s, 0, 0) : (axis == 1 ? find_dir<Q>(0, s, 0) : find_dir<Q>(0, 0, s));
+}
+
+// compile-time loop: f(std::integral_constant<int, I>) for I = 0 .. N-1.  Table lookups at the
+// indices it passes (moment<Q>(r, k) and the like) fold to constants: a plain loop that the
+// compiler does not unroll fully (27 x 27 terms) would evaluate them at run time.
+template <int N, int I = 0, typename F>
+IPL_HD inline void static_for(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<N, I + 1>(f);
+    }
 }
 
 // MRT moment basis, row i evaluated at direction k.  D2Q9 and D3Q27 reproduce the rows of

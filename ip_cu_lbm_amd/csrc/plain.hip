@@ -40,7 +40,7 @@ __device__ inline T dpos(const PlainView<T>& s, const Params& P, int axis, int a
 
 // ---- equ_f (collision.cu:3-14) -----------------------------------------------------------
 template <typename T, int Q>
-__global__ void k_equ_f(PlainView<T> s, Params P) {
+__global__ void __launch_bounds__(kBlock) k_equ_f(PlainView<T> s, Params P) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const T c = P.c, c2 = P.c * P.c;
@@ -56,7 +56,7 @@ __global__ void k_equ_f(PlainView<T> s, Params P) {
 
 // ---- Force (collision.cu:57-112) ---------------------------------------------------------
 template <typename T, int Q>
-__global__ void k_force(PlainView<T> s, Params P) {
+__global__ void __launch_bounds__(kBlock) k_force(PlainView<T> s, Params P) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     constexpr bool d3 = Lattice<Q>::dim == 3;
@@ -98,46 +98,43 @@ __global__ void k_force(PlainView<T> s, Params P) {
 // MRT in moment space: m = M d, m_j *= s_j (stress rows 1/tau of the cell), f -= M^-1 m.
 // M and M^-1 are compile-time constants, so the transforms reduce to adds.
 template <typename T, int Q>
-__global__ void k_collide(PlainView<T> s, Params P) {
+__global__ void __launch_bounds__(kBlock) k_collide(PlainView<T> s, Params P) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const int64_t Nt = s.Ntot;
-    T d[Q], fk[Q];
+    T d[Q];  // (the forcing term is read again where it is added: fewer live registers)
 #pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        fk[k] = s.Fk[k * Nt + i];
-        d[k] = s.f[k * Nt + i] - s.feq[k * Nt + i] + fk[k] / T(2.0);
-    }
+    for (int k = 0; k < Q; ++k) d[k] = s.f[k * Nt + i] - s.feq[k * Nt + i] + s.Fk[k * Nt + i] / T(2.0);
     const T tau = s.tau[i];
     if (P.collision == IPL_COLL_SRT) {
 #pragma unroll
         for (int k = 0; k < Q; ++k) {
-            const T fn = s.f[k * Nt + i] - d[k] / tau + fk[k];
+            const T fn = s.f[k * Nt + i] - d[k] / tau + s.Fk[k * Nt + i];
             s.ftmp[k * Nt + i] = fn;
         }
     } else {
         const T itau = T(1) / tau;
         T m[Q];
-#pragma unroll
-        for (int r = 0; r < Q; ++r) {
+        static_for<Q>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
             T acc = 0;
-#pragma unroll
-            for (int k = 0; k < Q; ++k) {
-                const double mk = moment<Q>(r, k);
-                if (mk != 0.0) acc += T(mk) * d[k];
-            }
+            static_for<Q>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                constexpr double mk = moment<Q>(r, k);
+                if constexpr (mk != 0.0) acc += T(mk) * d[k];
+            });
             m[r] = acc * (is_stress<Q>(r) ? itau : T(P.rates[r]));
-        }
-#pragma unroll
-        for (int k = 0; k < Q; ++k) {
+        });
+        static_for<Q>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
             T acc = 0;
-#pragma unroll
-            for (int r = 0; r < Q; ++r) {
-                const double mi = moment_inv<Q>(k, r);
-                if (mi != 0.0) acc += T(mi) * m[r];
-            }
-            s.ftmp[k * Nt + i] = s.f[k * Nt + i] - acc + fk[k];
-        }
+            static_for<Q>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                constexpr double mi = moment_inv<Q>(k, r);
+                if constexpr (mi != 0.0) acc += T(mi) * m[r];
+            });
+            s.ftmp[k * Nt + i] = s.f[k * Nt + i] - acc + s.Fk[k * Nt + i];
+        });
     }
 #pragma unroll
     for (int k = 0; k < Q; ++k) { s.f[k * Nt + i] = 0; s.feq[k * Nt + i] = 0; }
@@ -146,7 +143,7 @@ __global__ void k_collide(PlainView<T> s, Params P) {
 // ---- equ_g (collision.cu:16-27): the scalar's equilibrium, written into its populations ----
 // (linear with 1/cs^2 = 4 on D3Q7, as in col_g_reg below)
 template <typename T, int QS>
-__global__ void k_equ_g(PlainView<T> s, Params P, const T* __restrict__ sc) {
+__global__ void __launch_bounds__(kBlock) k_equ_g(PlainView<T> s, Params P, const T* __restrict__ sc) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const T c = P.c, c2 = P.c * P.c;
@@ -166,7 +163,7 @@ __global__ void k_equ_g(PlainView<T> s, Params P, const T* __restrict__ sc) {
 
 // ---- col_g_reg (collision.cu:167-188) ----------------------------------------------------
 template <typename T, int QS>
-__global__ void k_col_g_reg(PlainView<T> s, Params P) {
+__global__ void __launch_bounds__(kBlock) k_col_g_reg(PlainView<T> s, Params P) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const int64_t Nt = s.Ntot;
@@ -200,7 +197,7 @@ __global__ void k_col_g_reg(PlainView<T> s, Params P) {
 
 // ---- col_PF (collision.cu:190-223) -------------------------------------------------------
 template <typename T, int Q>
-__global__ void k_col_pf(PlainView<T> s, Params P) {
+__global__ void __launch_bounds__(kBlock) k_col_pf(PlainView<T> s, Params P) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     constexpr bool d3 = Lattice<Q>::dim == 3;
@@ -240,7 +237,7 @@ __global__ void k_col_pf(PlainView<T> s, Params P) {
 
 // ---- update_scalar (collision.cu:225-236) ------------------------------------------------
 template <typename T, int QS>
-__global__ void k_update_scalar(PlainView<T> s, T* out) {
+__global__ void __launch_bounds__(kBlock) k_update_scalar(PlainView<T> s, T* out) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     T a = 0;
@@ -251,7 +248,7 @@ __global__ void k_update_scalar(PlainView<T> s, T* out) {
 
 // ---- update_rho (collision.cu:238-257) ---------------------------------------------------
 template <typename T, int Q>
-__global__ void k_update_rho(PlainView<T> s, Params P) {
+__global__ void __launch_bounds__(kBlock) k_update_rho(PlainView<T> s, Params P) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const T c = P.c;
@@ -272,14 +269,14 @@ __global__ void k_update_rho(PlainView<T> s, Params P) {
 
 // ---- set_wall_rho (basic.cu:254-261) / resetF (collision.cu:258-264) ---------------------
 template <typename T>
-__global__ void k_set_wall_field(PlainView<T> s, T* a) {
+__global__ void __launch_bounds__(kBlock) k_set_wall_field(PlainView<T> s, T* a) {
     const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + s.N;
     if (g >= s.Ntot) return;
     a[g] = a[s.neib[g]];
 }
 
 template <typename T>
-__global__ void k_reset_force(PlainView<T> s) {
+__global__ void __launch_bounds__(kBlock) k_reset_force(PlainView<T> s) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     s.Fx[i] = 0; s.Fy[i] = 0; s.Fz[i] = 0;
@@ -291,7 +288,7 @@ template <typename T> struct Pops { T *f, *ftmp, *fout, *fin; const int32_t *nex
 
 // set_wall_f (adv.cu:3-13)
 template <typename T, int QF>
-__global__ void k_set_wall_f(PlainView<T> s, Pops<T> b) {
+__global__ void __launch_bounds__(kBlock) k_set_wall_f(PlainView<T> s, Pops<T> b) {
     const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + s.N;
     if (g >= s.Ntot) return;
     const int o = s.neib[g];
@@ -368,7 +365,7 @@ __device__ inline void flux_corner(const PlainView<T>& s, const Pops<T>& b, cons
 
 // set_out (adv.cu:50-147)
 template <typename T, int QN, int QF>
-__global__ void k_set_out(PlainView<T> s, Pops<T> b, Params P) {
+__global__ void __launch_bounds__(kBlock) k_set_out(PlainView<T> s, Pops<T> b, Params P) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const int64_t Nt = s.Ntot;
@@ -395,7 +392,7 @@ __global__ void k_set_out(PlainView<T> s, Pops<T> b, Params P) {
 
 // set_fin (adv.cu:148-175), corner scheme only; atomics where the reference races
 template <typename T, int QF>
-__global__ void k_set_fin(PlainView<T> s, Pops<T> b) {
+__global__ void __launch_bounds__(kBlock) k_set_fin(PlainView<T> s, Pops<T> b) {
     const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + s.N;
     if (g >= s.Ntot) return;
     const int64_t Nt = s.Ntot;
@@ -426,7 +423,7 @@ __global__ void k_set_fin(PlainView<T> s, Pops<T> b) {
 
 // reset_f (adv.cu:176-187)
 template <typename T, int QF>
-__global__ void k_reset_f(PlainView<T> s, Pops<T> b) {
+__global__ void __launch_bounds__(kBlock) k_reset_f(PlainView<T> s, Pops<T> b) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
 #pragma unroll
@@ -440,7 +437,7 @@ __global__ void k_reset_f(PlainView<T> s, Pops<T> b) {
 // set_wall_f_inv (adv.cu:188-232): ghost <- owner, mirrored along the flux axis; the 3D
 // corner scheme mirrors x and y and takes the xy / corner directions through nextB/nextK
 template <typename T, int QN, int QF>
-__global__ void k_set_wall_f_inv(PlainView<T> s, Pops<T> b, T* a, Params P) {
+__global__ void __launch_bounds__(kBlock) k_set_wall_f_inv(PlainView<T> s, Pops<T> b, T* a, Params P) {
     const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + s.N;
     if (g >= s.Ntot) return;
     const int64_t Nt = s.Ntot;
@@ -475,7 +472,7 @@ __global__ void k_set_wall_f_inv(PlainView<T> s, Pops<T> b, T* a, Params P) {
 
 // set_tmp (adv.cu:234-244)
 template <typename T, int QN, int QF>
-__global__ void k_set_tmp(PlainView<T> s, Pops<T> b, Params P) {
+__global__ void __launch_bounds__(kBlock) k_set_tmp(PlainView<T> s, Pops<T> b, Params P) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const int64_t Nt = s.Ntot;
@@ -495,7 +492,7 @@ __global__ void k_set_tmp(PlainView<T> s, Pops<T> b, Params P) {
 
 // set_wall_propagation (set_bound.cu:149-168)
 template <typename T, int QF>
-__global__ void k_set_wall_propagation(PlainView<T> s, Pops<T> b, Params P) {
+__global__ void __launch_bounds__(kBlock) k_set_wall_propagation(PlainView<T> s, Pops<T> b, Params P) {
     const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + s.N;
     if (g >= s.Ntot) return;
     const int64_t Nt = s.Ntot, G = s.Ntot - s.N, wid = g - s.N;
@@ -513,7 +510,7 @@ __global__ void k_set_wall_propagation(PlainView<T> s, Pops<T> b, Params P) {
 
 // propagation (propagation.cu:2-12)
 template <typename T, int QF>
-__global__ void k_propagation(PlainView<T> s, Pops<T> b) {
+__global__ void __launch_bounds__(kBlock) k_propagation(PlainView<T> s, Pops<T> b) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const int64_t Nt = s.Ntot;
@@ -523,7 +520,7 @@ __global__ void k_propagation(PlainView<T> s, Pops<T> b) {
 
 // reset_wall_fin (basic.cu:274-284)
 template <typename T, int QF>
-__global__ void k_reset_wall_fin(PlainView<T> s, Pops<T> b) {
+__global__ void __launch_bounds__(kBlock) k_reset_wall_fin(PlainView<T> s, Pops<T> b) {
     const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + s.N;
     if (g >= s.Ntot) return;
 #pragma unroll
