@@ -86,6 +86,33 @@ def test_ib_falling_sphere_d16_fp64(oracle_lib):
     assert gb["vel"][2] < 0  # falling along -z
 
 
+def test_ib_falling_sphere_full_size_fp64_vs_oracle(oracle_lib):
+    """The falling-sphere workload at its full size (192 x 192 x 384 D3Q19, 1800 Fibonacci markers,
+    diameter 24, multi-direct forcing with 3 passes and the 4-point Peskin kernel), FP64 on the
+    fused path against the oracle after 4 steps (about 30 s on the host's cores): fluid fields,
+    marker forces, body force and velocity."""
+    case = cases.falling_sphere(storage=F64, path=PATH_FUSED)
+    with Simulation(case) as sim:
+        sim.step(4)
+        gv = np.stack([sim.read(x) for x in "uvw"])
+        gp = sim.read("p")
+        gf, gb = sim.markers("force"), sim.body_state()
+    print("falling sphere full size: GPU done, oracle ...", flush=True)
+    o = oracle_lib.Oracle(case)
+    try:
+        o.step(4)
+        ov = np.stack([o.read(x) for x in "uvw"])
+        op = o.read("p")
+        of, ob = o.markers("force"), o.body_state()
+    finally:
+        o.close()
+    errs = {"vel": rel_l2(gv, ov), "p": rel_l2(gp, op), "Gw": rel_l2(gf, of.reshape(gf.shape)),
+            "force": rel_l2(gb["force"], ob["force"]), "vel_body": rel_l2(gb["vel"], ob["vel"])}
+    print("falling sphere full size vs oracle, 4 steps:", errs, flush=True)
+    for k, e in errs.items():
+        assert e <= (1e-10 if k == "Gw" else 1e-12), (k, e)
+
+
 def test_ib_periodic_axis_fp64(oracle_lib):
     _compare(_sphere(periodic=(1, 0, 0)), 40, oracle_lib, 1e-12)
 
