@@ -257,6 +257,36 @@ def test_cylinder_2d_stress_fp64(oracle_lib):
         assert errs[1] <= (1e-10 if iters > 1 else 1e-12)
 
 
+@pytest.mark.parametrize("x", [4.9, 43.1])
+@pytest.mark.parametrize("method,kernel,iters", [(_abi.IB_STRESS, _abi.IB_KERNEL_REFERENCE, 1),
+                                                 (_abi.IB_DIRECT, _abi.IB_KERNEL_PESKIN4, 3)])
+def test_cylinder_2d_across_a_periodic_seam_fp64(oracle_lib, method, kernel, iters, x):
+    """A 2D body whose node blocks cross the seam of a periodic x axis (radius 4.8: markers from
+    x = 0.1, or up to 47.9 of 48): the blocks wrap, the box of node blocks starts at the seam's
+    image, and the marker-space passes take the nearest periodic image.  (Markers themselves stay
+    inside the axis: the nearest-cell search, as search_IBlattice through the neighbour table,
+    does not move a marker's node across the seam, so a marker beyond it would sit far from its
+    node, in the oracle as here; the 2D stress method's renormalised weights are then 0/0.)"""
+    from conftest import rel_l2
+    c = cases.cylinder_2d(n=(48, 56), markers=140, storage=F64, path=PATH_FUSED, density_ratio=3.0, gravity=2e-4)
+    c = c.replace(periodic=(1, 0, 0))
+    c.body.pos = (x, 0.5, 27.6)
+    c.body.method, c.body.kernel, c.body.iterations = method, kernel, iters
+    X, _, Z = c.coords()
+    c.init = {"u": 0.01 * np.sin(2 * np.pi * Z / 56), "p": cases.uniform(9, c.n_cells, -1e-4, 1e-4)}
+    with Simulation(c) as sim:
+        sim.step(40)
+        gb, gf, gu = sim.body_state(), sim.markers("force"), np.stack([sim.read("u"), sim.read("w")])
+    o = oracle_lib.Oracle(c)
+    o.step(40)
+    ob, of, ou = o.body_state(), o.markers("force"), np.stack([o.read("u"), o.read("w")])
+    o.close()
+    errs = (rel_l2(gu, ou), rel_l2(gf, of), rel_l2(gb["force"], ob["force"]), rel_l2(gb["vel"], ob["vel"]))
+    print("cylinder across the seam", method, kernel, iters, errs, flush=True)
+    assert errs[0] <= 1e-12 and errs[2] <= 1e-12 and errs[3] <= 1e-12
+    assert errs[1] <= (1e-10 if iters > 1 else 1e-12)
+
+
 def test_path_reason_names_the_fallback():
     """Under the automatic path a configuration the fused path refuses runs on the plain path, and
     the context says why (ipl_path_reason); a fused context has no reason."""

@@ -121,7 +121,10 @@ def test_ib_periodic_axis_fp64(oracle_lib):
 def test_ib_periodic_seam_fp64(oracle_lib, method, x):
     """The body's node blocks across the seam of a periodic axis (below it, x = 0.6 and 4.6, and
     above it, 18.9): the box of the blocks starts at its periodic image, and node blocks, their
-    overlaps and the near-marker lists take the nearest periodic image."""
+    overlaps and the near-marker lists take the nearest periodic image.  At 0.6 and 18.9 some
+    markers lie beyond the axis's ends; the nearest-cell search (search_IBlattice, through the
+    neighbour table) does not move their node across the seam, so they keep a far node and no
+    weight, in the oracle as here."""
     if method == "stress":
         c = _sphere(periodic=(1, 0, 0))
     else:
