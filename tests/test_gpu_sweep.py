@@ -95,3 +95,81 @@ def test_sweep_against_oracle_fp64(oracle_lib, seed):
             assert e <= 1e-12, (path, k, e)
         ran.append(path)
     assert PATH_PLAIN in ran
+
+
+IB_SEEDS = list(range(32))
+
+
+def _ib_config(seed):
+    """A body in a small box: a sphere (D3Q19 or D3Q27) or a 2D cylinder, the stress method or
+    direct forcing (3-point Lagrange or 4-point Peskin, 1-3 passes), moving or fixed, periodic
+    flags, and a position anywhere its markers stay inside the box.  Axes whose node weights are
+    3-point Lagrange (y and z of the stress method, every axis of direct forcing's reference
+    kernel) keep the body a cell from their walls: a ghost node sits on its owner, so beyond a wall
+    two of the three positions coincide and the weights are not finite, as in the reference."""
+    rng = np.random.default_rng(cases.SEED_BASE + 900 + seed)
+    two_d = rng.random() < 0.3
+    stress = rng.random() < 0.4
+    if stress:
+        method, kernel, iters = _abi.IB_STRESS, _abi.IB_KERNEL_REFERENCE, 1
+    else:
+        method = _abi.IB_DIRECT
+        kernel = int(rng.choice([_abi.IB_KERNEL_REFERENCE, _abi.IB_KERNEL_PESKIN4]))
+        iters = int(rng.integers(1, 4))
+    if two_d:
+        n = (int(rng.integers(24, 48)), int(rng.integers(24, 48)))
+        c = cases.cylinder_2d(n=n, diameter=float(rng.uniform(6.0, 9.0)), markers=int(rng.integers(60, 160)),
+                              storage=F64, path=PATH_FUSED, density_ratio=float(rng.uniform(3.0, 8.0)),
+                              gravity=2e-4, method=method)
+        dims = (n[0], 1, n[1])
+    else:
+        dims = tuple(int(v) for v in rng.integers(16, 26, size=3))
+        c = cases.falling_sphere(n=dims, diameter=float(rng.uniform(5.0, 8.0)), marker_set=str(int(rng.choice([2, 3]))),
+                                 storage=F64, path=PATH_FUSED, tau=float(rng.uniform(0.6, 0.8)), gravity=2e-4,
+                                 method=method, kernel=kernel, iterations=iters, density_ratio=8.0)
+        c = c.replace(lattice=int(rng.choice([19, 27])))
+    c.body.kernel, c.body.iterations = kernel, iters
+    c.body.moving = int(rng.random() < 0.8)
+    axes = [0, 2] if two_d else [0, 1, 2]
+    periodic = tuple(int(rng.random() < 0.35) if a in axes else 0 for a in range(3))
+    c = c.replace(periodic=periodic, name=f"ib_sweep_{seed}")
+    R = float(np.abs(c.body.markers).max())
+    pos = []
+    for a in range(3):
+        if a == 1 and two_d:
+            pos.append(0.5)
+            continue
+        lagrange = (stress and a != 0) or (not stress and kernel == _abi.IB_KERNEL_REFERENCE)
+        margin = R + (1.2 if (lagrange and not periodic[a]) else 0.1)
+        pos.append(float(rng.uniform(margin, dims[a] - margin)))
+    c.body.pos = tuple(pos)
+    X, Y, Z = c.coords()
+    c.init = {"u": 0.01 * np.sin(2 * np.pi * Z / dims[2]), "p": cases.uniform(11 + seed, c.n_cells, -1e-4, 1e-4)}
+    return c
+
+
+@pytest.mark.parametrize("seed", IB_SEEDS)
+def test_ib_sweep_against_oracle_fp64(oracle_lib, seed):
+    """Fluid, marker forces, nearest cells and the body state after 15 steps, <= 1e-12 (marker
+    forces of direct forcing with several passes <= 1e-10: a difference of nearly equal
+    velocities)."""
+    c = _ib_config(seed)
+    with Simulation(c) as sim:
+        sim.step(15)
+        gu = np.stack([sim.read(x) for x in ("u", "w")])
+        gf, gn, gb = sim.markers("force"), sim.markers("node"), sim.body_state()
+    o = oracle_lib.Oracle(c)
+    o.step(15)
+    ou = np.stack([o.read(x) for x in ("u", "w")])
+    of, on, ob = o.markers("force"), o.markers("node"), o.body_state()
+    o.close()
+    b = c.body
+    errs = {"vel": rel_l2(gu, ou), "Gw": rel_l2(gf, of.reshape(gf.shape)), "force": rel_l2(gb["force"], ob["force"])}
+    if b.moving:
+        errs["vel_body"] = rel_l2(gb["vel"], ob["vel"])
+    print(c.name, c.lattice, c.n, "periodic", c.periodic, "method", b.method, "kernel", b.kernel, "iters", b.iterations,
+          "moving", b.moving, "pos", tuple(round(x, 2) for x in b.pos), errs, flush=True)
+    assert np.all(np.isfinite(gu)) and np.all(np.isfinite(gf))
+    assert np.array_equal(gn, on)
+    for k, e in errs.items():
+        assert e <= (1e-10 if (k == "Gw" and b.iterations > 1) else 1e-12), (k, e)
