@@ -10,7 +10,7 @@ does (wall rules on periodic-free axes, refined axes next to walls, line axes ac
 import numpy as np
 import pytest
 
-from ip_cu_lbm_amd import F64, PATH_FUSED, PATH_PLAIN, Simulation, _abi, cases
+from ip_cu_lbm_amd import F32, F64, PATH_FUSED, PATH_PLAIN, Simulation, _abi, cases
 from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -173,3 +173,31 @@ def test_ib_sweep_against_oracle_fp64(oracle_lib, seed):
     assert np.array_equal(gn, on)
     for k, e in errs.items():
         assert e <= (1e-10 if (k == "Gw" and b.iterations > 1) else 1e-12), (k, e)
+
+
+@pytest.mark.parametrize("seed", SEEDS[:16])
+def test_sweep_fp32_storage_against_oracle(oracle_lib, seed):
+    """The same configurations with FP32 populations (FP32 arithmetic) on both paths: velocity and
+    S within 1e-5 of the FP64 oracle after 20 steps."""
+    base = _config(seed).replace(storage=F32)
+    fields = ["u", "w"] + (["v"] if base.lattice != 9 else []) + (["S"] if base.scalar else [])
+    o = oracle_lib.Oracle(base.replace(storage=F64))
+    o.step(20)
+    want = {f: o.read(f) for f in fields}
+    o.close()
+    for path in (PATH_PLAIN, PATH_FUSED):
+        try:
+            sim = Simulation(base.replace(path=path))
+        except _abi.IplError:
+            assert path == PATH_FUSED
+            continue
+        with sim:
+            sim.step(20)
+            got = {f: sim.read(f) for f in fields}
+        vel = [f for f in ("u", "v", "w") if f in fields]
+        errs = {"vel": rel_l2(np.concatenate([got[f] for f in vel]), np.concatenate([want[f] for f in vel]))}
+        if "S" in fields:
+            errs["S"] = rel_l2(got["S"], want["S"])
+        print(base.name, "fp32 path", path, errs, flush=True)
+        for k, e in errs.items():
+            assert e <= 1e-5, (path, k, e)
