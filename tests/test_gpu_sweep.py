@@ -201,3 +201,39 @@ def test_sweep_fp32_storage_against_oracle(oracle_lib, seed):
         print(base.name, "fp32 path", path, errs, flush=True)
         for k, e in errs.items():
             assert e <= 1e-5, (path, k, e)
+
+
+@pytest.mark.parametrize("seed", SEEDS[:24])
+def test_sweep_les_and_wall_function_fp64(oracle_lib, seed):
+    """The configurations with MRT collision and the LES model (Boegman2005run18IB's coherent-
+    structure model, per-cell tau and tau_s) or the wall function's Spalding shear on the cells
+    next to walls, drawn per seed (the fused path's EXT step), against the oracle after 20 steps."""
+    rng = np.random.default_rng(cases.SEED_BASE + 700 + seed)
+    base = _config(seed).replace(collision=_abi.COLL_MRT, path=PATH_FUSED)
+    if rng.random() < 0.5:
+        mu = float(rng.uniform(0.01, 0.03))
+        base = base.replace(les=1, mu_L=mu, mu_H=mu)
+        what = "les"
+    else:
+        base = base.replace(wall_function=1, wall_nu=float(rng.uniform(0.002, 0.01)))
+        what = "wall function"
+    fields = ["p", "u", "w"] + (["v"] if base.lattice != 9 else []) + (["S"] if base.scalar else [])
+    try:
+        sim = Simulation(base)
+    except _abi.IplError:
+        pytest.skip("a configuration the fused path does not take")
+    with sim:
+        sim.step(20)
+        got = {f: sim.read(f) for f in fields}
+    o = oracle_lib.Oracle(base)
+    o.step(20)
+    want = {f: o.read(f) for f in fields}
+    o.close()
+    vel = [f for f in ("u", "v", "w") if f in fields]
+    errs = {"p": rel_l2(got["p"], want["p"]),
+            "vel": rel_l2(np.concatenate([got[f] for f in vel]), np.concatenate([want[f] for f in vel]))}
+    if "S" in fields:
+        errs["S"] = rel_l2(got["S"], want["S"])
+    print(base.name, what, errs, flush=True)
+    for k, e in errs.items():
+        assert e <= 1e-12, (what, k, e)
