@@ -165,6 +165,9 @@ bool graph_header(const ipl_config& cfg, Graph& g, std::string& err) {
     } else { err = "unsupported flux axes"; return false; }
     g.npts = cfg.stencil == 3 ? 3 : (cfg.stencil == 5 ? 5 : 0);
     if (g.npts == 0) { err = "stencil must be 3 or 5"; return false; }
+    // the corner directions' tensor stencil is 5 x 5 (adv.cu:100-145), and the one-axis
+    // directions of the same grid share its per-axis weights
+    if (g.flux_mode == FLUX_MODE_CORNER && g.npts != 5) { err = "the two-axis corner scheme needs the 5-point stencil"; return false; }
     bool flux_axis_on[3] = {false, false, false};
     if (g.flux_mode == FLUX_MODE_ONE_AXIS) flux_axis_on[g.flux_axis] = true;
     if (g.flux_mode == FLUX_MODE_CORNER) flux_axis_on[0] = flux_axis_on[1] = true;

@@ -385,6 +385,8 @@ bool build(Oracle& o, const ipl_config& cfg, const ipl_init* init) {
         o.rt = ref_tables_3d();
     } else { o.err = "unsupported flux axes"; return false; }
     o.npts = cfg.stencil == 3 ? 3 : 5;
+    // the reference's corner scheme is written for the 5 x 5 stencil only (adv.cu:100-145)
+    if (o.flux_mode == FLUX_CORNER && o.npts != 5) { o.err = "the two-axis corner scheme needs the 5-point stencil"; return false; }
     // width >= dz on flux axes (the slab point must lie inside the cell, adv.cu:77,85-86)
     for (int a = 0; a < 3; ++a) {
         const bool isflux = (o.flux_mode == FLUX_ONE_AXIS && o.flux_axis == a) ||

@@ -66,6 +66,19 @@ def test_builder_rejects_narrow_cells():
     assert lib.ipl_build_tables(ctypes.byref(cfg), 0, None, ctypes.byref(n)) == _abi.IPL_ERR_INVALID_CONFIG
 
 
+def test_corner_scheme_rejects_the_3_point_stencil(oracle_lib):
+    """Flux along x and y in 3D takes the corner scheme, whose tensor stencil is 5 x 5: a
+    3-point stencil is refused by the library and by the oracle alike."""
+    case = _box3d(19, nx=8, ny=8, nz=6).replace(stencil=3)
+    lib = _abi.load_library()
+    cfg, ini, keep = case.to_c()
+    n = ctypes.c_int64()
+    assert lib.ipl_build_tables(ctypes.byref(cfg), 0, None, ctypes.byref(n)) == _abi.IPL_ERR_INVALID_CONFIG
+    assert lib.ipl_build_tables(ctypes.byref(case.replace(stencil=5).to_c()[0]), 0, None, ctypes.byref(n)) == _abi.IPL_OK
+    with pytest.raises(ValueError, match="5-point"):
+        oracle_lib.Oracle(case)
+
+
 def test_stencil_weights_partition_of_unity():
     case = cases.lattice_taylor_green(nx=32, nz=8)
     w = _tables(case, 3, np.float64).reshape(3, 2, 5, -1)
