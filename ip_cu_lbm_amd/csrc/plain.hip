@@ -18,15 +18,16 @@ constexpr int kBlock = 256;
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // Reference direction tables of the 3D two-axis corner scheme (set_cpm / set_cxypm,
-// basic.cu:122-153); unset entries are 0 (= the cell itself), as in the reference.
-__constant__ int c_cp[27] = {0, 1, 2, 3, 4, 0, 0, 7, 8, 9, 10, 1, 1, 3, 3, 2, 2, 4, 4, 7, 7, 8, 8, 9, 9, 10, 10};
-__constant__ int c_cm[27] = {0, 3, 4, 1, 2, 0, 0, 10, 9, 8, 7, 3, 3, 1, 1, 4, 4, 2, 2, 10, 10, 9, 9, 8, 8, 7, 7};
-__constant__ int c_cip[27] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 8, 8, 7, 7, 10, 10, 9, 9};
-__constant__ int c_cim[27] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 9, 9, 10, 10, 7, 7, 8, 8};
-__constant__ int c_cxp[27] = {0, 1, 0, 3, 0, 0, 0, 1, 1, 3, 3, 1, 1, 3, 3, 0, 0, 0, 0, 1, 1, 1, 1, 3, 3, 3, 3};
-__constant__ int c_cxm[27] = {0, 3, 0, 1, 0, 0, 0, 3, 3, 1, 1, 3, 3, 1, 1, 0, 0, 0, 0, 3, 3, 3, 3, 1, 1, 1, 1};
-__constant__ int c_cyp[27] = {0, 0, 2, 0, 4, 0, 0, 2, 4, 2, 4, 0, 0, 0, 0, 2, 2, 4, 4, 2, 2, 4, 4, 2, 2, 4, 4};
-__constant__ int c_cym[27] = {0, 0, 4, 0, 2, 0, 0, 4, 2, 4, 2, 0, 0, 0, 0, 4, 4, 2, 2, 4, 4, 2, 2, 4, 4, 2, 2};
+// basic.cu:122-153); unset entries are 0 (= the cell itself), as in the reference.  The kernels
+// index them only at compile-time directions.
+constexpr int kCp[27] = {0, 1, 2, 3, 4, 0, 0, 7, 8, 9, 10, 1, 1, 3, 3, 2, 2, 4, 4, 7, 7, 8, 8, 9, 9, 10, 10};
+constexpr int kCm[27] = {0, 3, 4, 1, 2, 0, 0, 10, 9, 8, 7, 3, 3, 1, 1, 4, 4, 2, 2, 10, 10, 9, 9, 8, 8, 7, 7};
+constexpr int kCip[27] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 8, 8, 7, 7, 10, 10, 9, 9};
+constexpr int kCim[27] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 9, 9, 10, 10, 7, 7, 8, 8};
+constexpr int kCxp[27] = {0, 1, 0, 3, 0, 0, 0, 1, 1, 3, 3, 1, 1, 3, 3, 0, 0, 0, 0, 1, 1, 1, 1, 3, 3, 3, 3};
+constexpr int kCxm[27] = {0, 3, 0, 1, 0, 0, 0, 3, 3, 1, 1, 3, 3, 1, 1, 0, 0, 0, 0, 3, 3, 3, 3, 1, 1, 1, 1};
+constexpr int kCyp[27] = {0, 0, 2, 0, 4, 0, 0, 2, 4, 2, 4, 0, 0, 0, 0, 2, 2, 4, 4, 2, 2, 4, 4, 2, 2, 4, 4};
+constexpr int kCym[27] = {0, 0, 4, 0, 2, 0, 0, 4, 2, 4, 2, 0, 0, 0, 0, 4, 4, 2, 2, 4, 4, 2, 2, 4, 4, 2, 2};
 
 template <typename T> __device__ inline void atomic_add(T* a, T v) { atomicAdd(a, v); }
 
@@ -319,32 +320,43 @@ __device__ inline void flux_1d(const PlainView<T>& s, const Pops<T>& b, const Pa
     b.fout[k * Nt + i] = F * s.rdel[axis][i];
 }
 
-// corner directions of the 3D x-y scheme (adv.cu:100-145); the 5x5 tensor stencil and
-// three scattered contributions, with FP atomics as in the reference
-template <typename T, int QF>
-__device__ inline void flux_corner(const PlainView<T>& s, const Pops<T>& b, const Params& P, int64_t i, int kk) {
+// set_out of the 3D x-y scheme (adv.cu:50-147), grouped so that what directions share is read
+// once per cell.  The one-axis directions of one axis and sign (x, xz, or y, yz) share their
+// neighbours, 5-point weights and widths.  The corner directions of one x-y orientation (sx, sy)
+// share their 5 x 5 stencil in the x-y plane: the xy direction and the two corner directions
+// z = +-1 take the same direction tables except cip / cim, which are 0 for the xy direction, so
+// that its anti-diagonal points are the cell itself (the rest neighbour of a cell is the cell).
+// Each direction's arithmetic is the reference's num3 loop; its three scattered contributions
+// are added with FP atomics as in the reference (at most two land on one destination, so their
+// order cannot change a bit).  A scheduling fence after each direction group keeps the compiler
+// from hoisting the next group's gathers over the current one: hoisted, the FP64 D3Q27 kernel
+// held 256 VGPRs (one wave per SIMD) and the FP32 one 157; fenced, the falling-sphere grid's
+// plain step runs 7 % (FP32) and 10 % (FP64) faster than with one stencil per direction.
+template <typename T, int QN, int QF, int AX, int SG>
+__device__ inline void corner_scheme_axis(const PlainView<T>& s, const Pops<T>& b, int64_t i) {
+    constexpr int dp = axis_dir<QN>(AX, SG), dm = axis_dir<QN>(AX, -SG);
     const int64_t Nt = s.Ntot;
-    auto nb = [&](int cell, int d) { return s.neib[d * Nt + cell]; };
-    int idf[25];
-    idf[12] = (int)i;
-    idf[13] = nb(i, c_cxp[kk]); idf[14] = nb(idf[13], c_cxp[kk]);
-    idf[11] = nb(i, c_cxm[kk]); idf[10] = nb(idf[11], c_cxm[kk]);
-    idf[17] = nb(i, c_cyp[kk]); idf[22] = nb(idf[17], c_cyp[kk]);
-    idf[7] = nb(i, c_cym[kk]); idf[2] = nb(idf[7], c_cym[kk]);
-    idf[18] = nb(i, c_cp[kk]); idf[24] = nb(idf[18], c_cp[kk]);
-    idf[6] = nb(i, c_cm[kk]); idf[0] = nb(idf[6], c_cm[kk]);
-    idf[8] = nb(i, c_cip[kk]); idf[4] = nb(idf[8], c_cip[kk]);
-    idf[16] = nb(i, c_cim[kk]); idf[20] = nb(idf[16], c_cim[kk]);
-    idf[19] = nb(idf[14], c_cyp[kk]); idf[9] = nb(idf[14], c_cym[kk]);
-    idf[15] = nb(idf[10], c_cyp[kk]); idf[5] = nb(idf[10], c_cym[kk]);
-    idf[23] = nb(idf[22], c_cxp[kk]); idf[21] = nb(idf[22], c_cxm[kk]);
-    idf[3] = nb(idf[2], c_cxp[kk]); idf[1] = nb(idf[2], c_cxm[kk]);
-    const int sx = comp<QF>(kk, 0), sy = comp<QF>(kk, 1);
-    const T* Wx = s.W + (size_t)((0 * 2 + (sx > 0 ? 0 : 1)) * 5) * s.N;
-    const T* Wy = s.W + (size_t)((1 * 2 + (sy > 0 ? 0 : 1)) * 5) * s.N;
-    T cxw[5], cyw[5];
-#pragma unroll
-    for (int j = 0; j < 5; ++j) { cxw[j] = Wx[j * s.N + i]; cyw[j] = Wy[j * s.N + i]; }
+    const int cp = s.neib[dp * Nt + i], cm = s.neib[dm * Nt + i];
+    const int cpp = s.neib[dp * Nt + cp], cmm = s.neib[dm * Nt + cm];
+    const T* W = s.W + (size_t)((AX * 2 + (SG > 0 ? 0 : 1)) * 5) * s.N;
+    const T w0 = W[0 * s.N + i], w1 = W[1 * s.N + i], w2 = W[2 * s.N + i], w3 = W[3 * s.N + i], w4 = W[4 * s.N + i];
+    const T rp = s.rdel[AX][cp], ri = s.rdel[AX][i];
+    static_for<QF>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if constexpr (comp<QF>(k, AX) == SG && comp<QF>(k, 1 - AX) == 0) {
+            const T F = w0 * b.ftmp[k * Nt + cmm] + w1 * b.ftmp[k * Nt + cm] + w2 * b.ftmp[k * Nt + i] +
+                        w3 * b.ftmp[k * Nt + cp] + w4 * b.ftmp[k * Nt + cpp];
+            b.fin[k * Nt + i] = F * rp;
+            b.fout[k * Nt + i] = F * ri;
+        }
+    });
+}
+
+template <typename T>
+__device__ inline void corner_dir(const PlainView<T>& s, const Pops<T>& b, const Params& P, int64_t i, int kk,
+                                  const int (&idf)[25], const T (&cxw)[5], const T (&cyw)[5], T dXi, T dYi,
+                                  T dX13, T dY13, T dX17, T dY17) {
+    const int64_t Nt = s.Ntot;
     const T* ft = b.ftmp + (size_t)kk * Nt;
     T fx = 0, fy = 0, fxy = 0;
 #pragma unroll
@@ -356,11 +368,77 @@ __device__ inline void flux_corner(const PlainView<T>& s, const Pops<T>& b, cons
 #pragma unroll
         for (int c = 0; c < 5; ++c) fxy += cxw[a] * cyw[c] * ft[idf[a + c * 5]];
     const T dz = P.dz;
+    b.fout[kk * Nt + i] = fx * dz / dXi + fy * dz / dYi - fxy * (dz * dz) / (dXi * dYi);
+    atomic_add(&b.fin[kk * Nt + i], fxy * (dz * dz) / (dX13 * dY17));
+    atomic_add(&b.fin[kk * Nt + idf[11]], (fy - fxy * dz / dX17) * dz / dY17);   // the -x neighbour
+    atomic_add(&b.f[kk * Nt + idf[7]], (fx - fxy * dz / dY13) * dz / dX13);      // the -y neighbour
+}
+
+template <typename T, int QF, int SX, int SY>
+__device__ inline void corner_scheme_orientation(const PlainView<T>& s, const Pops<T>& b, const Params& P, int64_t i) {
+    constexpr int kxy = find_dir<QF>(SX, SY, 0);
+    constexpr int kz1 = QF == 27 ? find_dir<QF>(SX, SY, 1) : kxy, kz2 = QF == 27 ? find_dir<QF>(SX, SY, -1) : kxy;
+    static_assert(kxy > 0 && kz1 > 0 && kz2 > 0, "an x-y orientation without its directions");
+    static_assert(kCxp[kz1] == kCxp[kxy] && kCxm[kz1] == kCxm[kxy] && kCyp[kz1] == kCyp[kxy] &&
+                  kCym[kz1] == kCym[kxy] && kCp[kz1] == kCp[kxy] && kCm[kz1] == kCm[kxy] &&
+                  kCxp[kz2] == kCxp[kxy] && kCxm[kz2] == kCxm[kxy] && kCyp[kz2] == kCyp[kxy] &&
+                  kCym[kz2] == kCym[kxy] && kCp[kz2] == kCp[kxy] && kCm[kz2] == kCm[kxy] &&
+                  kCip[kz2] == kCip[kz1] && kCim[kz2] == kCim[kz1] && kCip[kxy] == 0 && kCim[kxy] == 0,
+                  "directions of one orientation share their stencil");
+    constexpr int xp = kCxp[kxy], xm = kCxm[kxy], yp = kCyp[kxy], ym = kCym[kxy], dpp = kCp[kxy], dmm = kCm[kxy];
+    const int64_t Nt = s.Ntot;
+    auto nb = [&](int cell, int d) { return s.neib[d * Nt + cell]; };
+    int idf[25];
+    idf[12] = (int)i;
+    idf[13] = nb(i, xp); idf[14] = nb(idf[13], xp);
+    idf[11] = nb(i, xm); idf[10] = nb(idf[11], xm);
+    idf[17] = nb(i, yp); idf[22] = nb(idf[17], yp);
+    idf[7] = nb(i, ym); idf[2] = nb(idf[7], ym);
+    idf[18] = nb(i, dpp); idf[24] = nb(idf[18], dpp);
+    idf[6] = nb(i, dmm); idf[0] = nb(idf[6], dmm);
+    idf[19] = nb(idf[14], yp); idf[9] = nb(idf[14], ym);
+    idf[15] = nb(idf[10], yp); idf[5] = nb(idf[10], ym);
+    idf[23] = nb(idf[22], xp); idf[21] = nb(idf[22], xm);
+    idf[3] = nb(idf[2], xp); idf[1] = nb(idf[2], xm);
+    idf[8] = idf[4] = idf[16] = idf[20] = (int)i;
+    const T* Wx = s.W + (size_t)((0 * 2 + (SX > 0 ? 0 : 1)) * 5) * s.N;
+    const T* Wy = s.W + (size_t)((1 * 2 + (SY > 0 ? 0 : 1)) * 5) * s.N;
+    T cxw[5], cyw[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) { cxw[j] = Wx[j * s.N + i]; cyw[j] = Wy[j * s.N + i]; }
     const T* dX = s.del[0]; const T* dY = s.del[1];
-    b.fout[kk * Nt + i] = fx * dz / dX[i] + fy * dz / dY[i] - fxy * (dz * dz) / (dX[i] * dY[i]);
-    atomic_add(&b.fin[kk * Nt + i], fxy * (dz * dz) / (dX[idf[13]] * dY[idf[17]]));
-    atomic_add(&b.fin[kk * Nt + nb(i, c_cxm[kk])], (fy - fxy * dz / dX[idf[17]]) * dz / dY[idf[17]]);
-    atomic_add(&b.f[kk * Nt + nb(i, c_cym[kk])], (fx - fxy * dz / dY[idf[13]]) * dz / dX[idf[13]]);
+    const T dXi = dX[i], dYi = dY[i], dX13 = dX[idf[13]], dY13 = dY[idf[13]], dX17 = dX[idf[17]], dY17 = dY[idf[17]];
+    corner_dir<T>(s, b, P, i, kxy, idf, cxw, cyw, dXi, dYi, dX13, dY13, dX17, dY17);
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (QF == 27) {
+        constexpr int ip = kCip[kz1], im = kCim[kz1];
+        idf[8] = nb(i, ip); idf[4] = nb(idf[8], ip);
+        idf[16] = nb(i, im); idf[20] = nb(idf[16], im);
+        corner_dir<T>(s, b, P, i, kz1, idf, cxw, cyw, dXi, dYi, dX13, dY13, dX17, dY17);
+        __builtin_amdgcn_sched_barrier(0);
+        corner_dir<T>(s, b, P, i, kz2, idf, cxw, cyw, dXi, dYi, dX13, dY13, dX17, dY17);
+    }
+}
+
+template <typename T, int QN, int QF>
+__device__ inline void set_out_corner_scheme(const PlainView<T>& s, const Pops<T>& b, const Params& P, int64_t i) {
+    const int64_t Nt = s.Ntot;
+    static_for<QF>([&](auto kc) {  // no x or y component: no flux
+        constexpr int k = decltype(kc)::value;
+        if constexpr (comp<QF>(k, 0) == 0 && comp<QF>(k, 1) == 0) {
+            const T v = b.ftmp[k * Nt + i];
+            b.fout[k * Nt + i] = v;
+            b.fin[k * Nt + i] = v;
+        }
+    });
+    corner_scheme_axis<T, QN, QF, 0, 1>(s, b, i); __builtin_amdgcn_sched_barrier(0);
+    corner_scheme_axis<T, QN, QF, 0, -1>(s, b, i); __builtin_amdgcn_sched_barrier(0);
+    corner_scheme_axis<T, QN, QF, 1, 1>(s, b, i); __builtin_amdgcn_sched_barrier(0);
+    corner_scheme_axis<T, QN, QF, 1, -1>(s, b, i); __builtin_amdgcn_sched_barrier(0);
+    corner_scheme_orientation<T, QF, 1, 1>(s, b, P, i); __builtin_amdgcn_sched_barrier(0);
+    corner_scheme_orientation<T, QF, 1, -1>(s, b, P, i); __builtin_amdgcn_sched_barrier(0);
+    corner_scheme_orientation<T, QF, -1, 1>(s, b, P, i); __builtin_amdgcn_sched_barrier(0);
+    corner_scheme_orientation<T, QF, -1, -1>(s, b, P, i);
 }
 
 // set_out (adv.cu:50-147)
@@ -369,23 +447,21 @@ __global__ void __launch_bounds__(kBlock) k_set_out(PlainView<T> s, Pops<T> b, P
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const int64_t Nt = s.Ntot;
+    if constexpr (QF == 19 || QF == 27) {  // the corner scheme is 3D and takes no D3Q7 scalar
+        if (P.flux_mode == FLUX_MODE_CORNER) {
+            set_out_corner_scheme<T, QN, QF>(s, b, P, i);
+            return;
+        }
+    }
 #pragma unroll
     for (int k = 0; k < QF; ++k) {
-        int nflux = 0, ax = 0;
-        if (P.flux_mode == FLUX_MODE_ONE_AXIS) {
-            if (comp<QF>(k, P.flux_axis) != 0) { nflux = 1; ax = P.flux_axis; }
-        } else if (P.flux_mode == FLUX_MODE_CORNER) {
-            const bool hx = comp<QF>(k, 0) != 0, hy = comp<QF>(k, 1) != 0;
-            nflux = (int)hx + (int)hy; ax = hx ? 0 : 1;
-        }
-        if (nflux == 0) {
+        const bool flux = P.flux_mode == FLUX_MODE_ONE_AXIS && comp<QF>(k, P.flux_axis) != 0;
+        if (!flux) {
             const T v = b.ftmp[k * Nt + i];
             b.fout[k * Nt + i] = v;
             b.fin[k * Nt + i] = v;
-        } else if (nflux == 1) {
-            flux_1d<T, QN, QF>(s, b, P, i, k, ax);
         } else {
-            flux_corner<T, QF>(s, b, P, i, k);
+            flux_1d<T, QN, QF>(s, b, P, i, k, P.flux_axis);
         }
     }
 }
@@ -476,18 +552,19 @@ __global__ void __launch_bounds__(kBlock) k_set_tmp(PlainView<T> s, Pops<T> b, P
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= s.N) return;
     const int64_t Nt = s.Ntot;
-#pragma unroll
-    for (int k = 0; k < QF; ++k) {
+    static_for<QF>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        constexpr int dcorner = QF == 19 || QF == 27 ? kCp[k] : 0;
         int d = 0;
         if (P.flux_mode == FLUX_MODE_ONE_AXIS) {
             const int sg = comp<QF>(k, P.flux_axis);
             if (sg) d = axis_dir<QN>(P.flux_axis, sg);
         } else if (P.flux_mode == FLUX_MODE_CORNER) {
-            d = c_cp[k];
+            d = dcorner;
         }
         const int n = d ? s.neib[d * Nt + i] : (int)i;
         b.ftmp[k * Nt + i] = b.f[k * Nt + n] + (b.fin[k * Nt + i] - b.fout[k * Nt + n]);
-    }
+    });
 }
 
 // set_wall_propagation (set_bound.cu:149-168)

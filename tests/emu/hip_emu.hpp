@@ -24,6 +24,7 @@
 #define __forceinline__ inline
 #define __builtin_amdgcn_readfirstlane(x) (x)  // the values it is given are uniform over a wavefront
 #define __builtin_assume(x) ((void)0)
+#define __builtin_amdgcn_sched_barrier(x) ((void)0)  // an instruction-scheduling fence only
 #define __constant__
 #define __shared__ static
 #define __launch_bounds__(...)
