@@ -24,8 +24,9 @@ bounded number of steps of the same workload, on the host's cores.
 
 Beside the headline the line carries: the fused FP64 and plain FP64/FP32 paths on the same
 workload, the falling sphere with its immersed boundary, the phase-field solitary wave and the
-stretched Taylor-Green vortex, the line-axis tuner's timings and the serving mix (p50/p99 latency,
-aggregate MLUPS).  --precision runs the lock exchange and the solitary wave to their full step
+stretched Taylor-Green vortex, the falling-sphere driver's grid refined in x and y (two-axis
+corner scheme, plain path, FP32 and FP64), the line-axis tuner's timings and the serving mix
+(p50/p99 latency, aggregate MLUPS).  --precision runs the lock exchange and the solitary wave to their full step
 counts in FP32 and FP64 (and, with --precision-oracle, against the oracle), --sphere-precision
 the falling sphere.
 
@@ -289,6 +290,27 @@ def measure_small_configs(steps):
     return out
 
 
+def measure_corner_grid(steps=40):
+    """The falling-sphere driver's own grid (SphereFall.cu:80-128): 100 coarse columns in x and y,
+    refined 2x and 4x around the sphere's path (cases.sphere_fall_divs), 100 uniform cells in z,
+    D3Q27 MRT, slip walls.  With x and y both refined the step takes the two-axis corner scheme
+    (adv.cu:100-175), which runs on the plain path; FP32 and FP64 populations."""
+    import numpy as np
+    from ip_cu_lbm_amd import F32, F64, PATH_PLAIN, _abi, cases
+    px, wx = cases.subdivided_axis(cases.sphere_fall_divs(100, 0.005), 4.0)
+    pz, wz = cases.axis_from_widths(np.ones(100))
+    out = {"workload": f"sphere_fall_grid_{len(px)}x{len(px)}x100_d3q27_corner_scheme", "path": "plain"}
+    for st, name in ((F32, "fp32"), (F64, "fp64")):
+        case = cases.Case(name="sphere_fall_grid", lattice=27, n=(len(px), len(px), 100), pos=[px, px.copy(), pz],
+                          width=[wx, wx.copy(), wz], wall_bc=_abi.BC_SLIP, flux_axes=_abi.FLUX_REFERENCE, tau=0.7,
+                          path=PATH_PLAIN, storage=st, s_rates=np.full(27, 1.2))
+        r = measure(case, steps, 5)
+        out["cells"] = r["n_cells"]
+        out[f"{name}_mlups"] = round(r["mlups"], 1)
+        out[f"{name}_ms_per_step"] = round(r["ms_per_step"], 3)
+    return out
+
+
 def measure_tuner(steps=100):
     """The line-axis autotuner (ip_cu_lbm_amd.tune) on uniform grids: every admissible axis and
     the library's own choice timed over `steps` steps (graph replay, HIP events):
@@ -486,6 +508,9 @@ def main(argv=None):
         sw = measure_small_configs(a.steps)
         log(f"[bench] solitary wave: fused FP32 {sw['solitary_wave']['fused_fp32_mlups']:.0f} MLUPS, "
             f"Taylor-Green: fused FP64 {sw['taylor_green']['fused_fp64_mlups']:.0f} MLUPS")
+        corner = measure_corner_grid()
+        log(f"[bench] sphere-driver grid (corner scheme, plain): FP32 {corner['fp32_mlups']:.0f} MLUPS, "
+            f"FP64 {corner['fp64_mlups']:.0f} MLUPS")
         tuned = measure_tuner()
         log("[bench] line-axis tuner: " + "; ".join(f"{t['workload']} {t['tuned_axis']} x{t['tuned_vs_auto']}" for t in tuned))
         srv = measure_serving()
@@ -503,6 +528,7 @@ def main(argv=None):
             "falling_sphere": sph,
             "solitary_wave": sw["solitary_wave"],
             "taylor_green": sw["taylor_green"],
+            "sphere_fall_grid": corner,
             "line_axis_tuner": tuned,
             "serving": srv,
         }
